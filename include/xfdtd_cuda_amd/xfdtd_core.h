@@ -1,0 +1,497 @@
+// xfdtd_core.h -- host-side scene model and set-up of the FDTD engine.
+//
+// XFDTD_CUDA only time-steps state that its host library XFDTD_CORE builds
+// (SimulationHD::init calls host()->init(n), src/simulation/simulation_hd.cu:
+// 259-267).  XFDTD_CORE is a separate project that this repository does not
+// have, so this header restates the subset of it that SimulationHD consumes
+// (the accessors listed at simulation_hd.cu:275-503 and in the HD mirrors):
+// Simulation, GridSpace, EMF, FDTD coefficients, PML, TFSF2D/3D, Waveform,
+// NFFFTFrequencyDomain and the Drude / Debye / M-Lorentz ADE storages.
+// Class and method names follow XFDTD_CORE's as used by the reference
+// drivers (test/dielectric_sphere_scatter.cu, test/benchmark.cu,
+// test/cylinder_scatter_2d.cu, test/dispersive_sphere_scatter/*).
+//
+// The numerical set-up (voxelisation, CPML profiles, incident-field table,
+// NF2FF phasors, ADE coefficients) is this project's own choice wherever
+// XFDTD_CORE's is not visible in the reference; each choice is documented
+// at its definition.
+#ifndef XFDTD_CUDA_AMD_XFDTD_CORE_H_
+#define XFDTD_CUDA_AMD_XFDTD_CORE_H_
+
+#include <array>
+#include <complex>
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../xfdtd_hip.h"
+
+namespace xfdtd {
+
+using Real = double;
+using Index = std::size_t;
+
+namespace constant {
+inline constexpr Real PI = 3.14159265358979323846;
+inline constexpr Real EPSILON_0 = 8.854187817e-12;
+inline constexpr Real MU_0 = 4.0 * PI * 1e-7;
+inline const Real C_0 = 1.0 / __builtin_sqrt(EPSILON_0 * MU_0);
+inline const Real ETA_0 = __builtin_sqrt(MU_0 / EPSILON_0);
+inline constexpr Real INF = std::numeric_limits<Real>::infinity();
+}  // namespace constant
+
+struct XFDTDException : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// Row-major N-d array of doubles (last index fastest) that owns its memory.
+class Array {
+ public:
+  Array() = default;
+  explicit Array(std::vector<int64_t> shape, Real fill = 0.0);
+  Real* data() { return _data.data(); }
+  const Real* data() const { return _data.data(); }
+  std::size_t size() const { return _data.size(); }
+  const std::vector<int64_t>& shape() const { return _shape; }
+  Real& operator[](std::size_t i) { return _data[i]; }
+  Real operator[](std::size_t i) const { return _data[i]; }
+  Real& at(int64_t i, int64_t j, int64_t k) {
+    return _data[(i * _shape[1] + j) * _shape[2] + k];
+  }
+  Real at(int64_t i, int64_t j, int64_t k) const {
+    return _data[(i * _shape[1] + j) * _shape[2] + k];
+  }
+  xf_array view();
+  void fill(Real v);
+  bool empty() const { return _data.empty(); }
+
+ private:
+  std::vector<Real> _data;
+  std::vector<int64_t> _shape;
+};
+
+struct Vector {
+  Real x{}, y{}, z{};
+};
+
+namespace Axis {
+enum class XYZ { X = 0, Y = 1, Z = 2 };
+enum class Direction { XN = 0, XP = 1, YN = 2, YP = 3, ZN = 4, ZP = 5 };
+}  // namespace Axis
+
+// ------------------------------------------------------------------ shapes
+class Shape {
+ public:
+  virtual ~Shape() = default;
+  virtual bool isInside(const Vector& p) const = 0;
+  virtual Vector lower() const = 0;  // bounding box
+  virtual Vector upper() const = 0;
+};
+
+class Cube : public Shape {
+ public:
+  Cube(Vector origin, Vector size) : _origin(origin), _size(size) {}
+  bool isInside(const Vector& p) const override;
+  Vector lower() const override { return _origin; }
+  Vector upper() const override;
+
+ private:
+  Vector _origin, _size;
+};
+
+class Sphere : public Shape {
+ public:
+  Sphere(Vector center, Real radius) : _center(center), _radius(radius) {}
+  bool isInside(const Vector& p) const override;
+  Vector lower() const override;
+  Vector upper() const override;
+  Vector center() const { return _center; }
+  Real radius() const { return _radius; }
+
+ private:
+  Vector _center;
+  Real _radius;
+};
+
+// Circular cylinder along z (the 2-D scatterer of test/cylinder_scatter_2d.cu).
+class Cylinder : public Shape {
+ public:
+  Cylinder(Vector center, Real radius, Real height)
+      : _center(center), _radius(radius), _height(height) {}
+  bool isInside(const Vector& p) const override;
+  Vector lower() const override;
+  Vector upper() const override;
+
+ private:
+  Vector _center;
+  Real _radius, _height;
+};
+
+// --------------------------------------------------------------- materials
+struct ElectroMagneticProperty {
+  Real epsilon_r{1}, mu_r{1}, sigma_e{0}, sigma_m{0};
+};
+
+// One second-order ADE pole in the M-Lorentz form
+//   chi_p(w) = (a0 + a1 jw) / (b0 + b1 jw + b2 (jw)^2).
+// Lorentz: a0 = de*w0^2, b0 = w0^2, b1 = 2*delta, b2 = 1.
+// Drude:   a0 = wp^2, b0 = 0, b1 = gamma, b2 = 1.
+struct MLorentzPole {
+  Real a0{}, a1{}, b0{}, b1{}, b2{};
+};
+
+class Material {
+ public:
+  enum class Dispersion { None = 0, Drude = 1, Debye = 2, MLorentz = 3 };
+  Material(std::string name, ElectroMagneticProperty emp)
+      : _name(std::move(name)), _emp(emp) {}
+  static std::unique_ptr<Material> createAir();
+  static std::unique_ptr<Material> createPec();
+  const std::string& name() const { return _name; }
+  const ElectroMagneticProperty& emp() const { return _emp; }
+  Dispersion dispersion() const { return _dispersion; }
+  // Drude: omega_p, gamma per pole.  Debye: delta_eps, tau per pole.
+  const std::vector<Real>& p0() const { return _p0; }
+  const std::vector<Real>& p1() const { return _p1; }
+  const std::vector<MLorentzPole>& mlorPoles() const { return _mlor; }
+
+  // Drude medium: eps(w) = eps_inf - sum wp^2 / (w^2 - j w gamma).
+  static std::unique_ptr<Material> makeDrudeMedium(
+      std::string name, Real eps_inf, std::vector<Real> omega_p,
+      std::vector<Real> gamma);
+  // Debye medium: eps(w) = eps_inf + sum de / (1 + j w tau).
+  static std::unique_ptr<Material> makeDebyeMedium(
+      std::string name, Real eps_inf, std::vector<Real> delta_eps,
+      std::vector<Real> tau);
+  // Lorentz medium: eps = eps_inf + sum de w0^2 / (w0^2 + 2 j w delta - w^2).
+  static std::unique_ptr<Material> makeLorentzMedium(
+      std::string name, Real eps_inf, std::vector<Real> delta_eps,
+      std::vector<Real> omega_0, std::vector<Real> delta);
+  // General M-Lorentz medium (XFDTD_CORE MLorentzMaterial::makeMLorentz).
+  static std::unique_ptr<Material> makeMLorentz(std::string name,
+                                                Real eps_inf,
+                                                std::vector<MLorentzPole> p);
+  // Relative permittivity at angular frequency w (analytic).
+  std::complex<Real> relativePermittivity(Real w) const;
+  // The same medium as M-Lorentz poles (Drude and Lorentz map exactly).
+  std::vector<MLorentzPole> asMLorentz() const;
+
+ private:
+  std::string _name;
+  ElectroMagneticProperty _emp;
+  Dispersion _dispersion{Dispersion::None};
+  std::vector<Real> _p0, _p1;
+  std::vector<MLorentzPole> _mlor;
+};
+
+class Object {
+ public:
+  Object(std::string name, std::unique_ptr<Shape> shape,
+         std::unique_ptr<Material> material)
+      : _name(std::move(name)),
+        _shape(std::move(shape)),
+        _material(std::move(material)) {}
+  const std::string& name() const { return _name; }
+  const Shape& shape() const { return *_shape; }
+  const Material& material() const { return *_material; }
+
+ private:
+  std::string _name;
+  std::unique_ptr<Shape> _shape;
+  std::unique_ptr<Material> _material;
+};
+
+// --------------------------------------------------------------- waveforms
+class Waveform {
+ public:
+  enum class Kind { Gaussian = 0, CosineModulatedGaussian = 1, Sine = 2 };
+  // exp(-((t - t0)/tau)^2)
+  static std::unique_ptr<Waveform> gaussian(Real tau, Real t0);
+  // exp(-((t - t0)/tau)^2) * cos(2 pi f (t - t0))
+  static std::unique_ptr<Waveform> cosineModulatedGaussian(Real tau, Real t0,
+                                                           Real f);
+  static std::unique_ptr<Waveform> sine(Real f);
+  Real operator()(Real t) const;
+  // Sampled at t = n dt, n = 0..nt-1 (filled by init).
+  void init(Real dt, Index nt);
+  const std::vector<Real>& time() const { return _time; }
+  const std::vector<Real>& value() const { return _value; }
+  Kind kind() const { return _kind; }
+
+ private:
+  Kind _kind{Kind::Gaussian};
+  Real _tau{}, _t0{}, _f{};
+  std::vector<Real> _time, _value;
+};
+
+class Simulation;
+
+// ------------------------------------------------------------------- grid
+class GridSpace {
+ public:
+  enum class Dimension { TWO = 2, THREE = 3 };
+  Dimension dimension() const { return _dim; }
+  Index sizeX() const { return _nx; }
+  Index sizeY() const { return _ny; }
+  Index sizeZ() const { return _nz; }
+  Real basedDx() const { return _dx; }
+  Real basedDy() const { return _dy; }
+  Real basedDz() const { return _dz; }
+  Vector origin() const { return _origin; }
+  // node coordinate
+  Real nodeX(Index i) const { return _origin.x + _dx * static_cast<Real>(i); }
+  Real nodeY(Index j) const { return _origin.y + _dy * static_cast<Real>(j); }
+  Real nodeZ(Index k) const { return _origin.z + _dz * static_cast<Real>(k); }
+
+ private:
+  friend class Simulation;
+  Dimension _dim{Dimension::THREE};
+  Index _nx{}, _ny{}, _nz{};
+  Real _dx{}, _dy{}, _dz{};
+  Vector _origin{};
+};
+
+class TimeParam {
+ public:
+  Real dt() const { return _dt; }
+  Index startTimeStep() const { return _start; }
+  Index size() const { return _size; }
+  Index currentTimeStep() const { return _current; }
+  Index endTimeStep() const { return _start + _size; }
+  void nextStep() { ++_current; }
+
+ private:
+  friend class Simulation;
+  Real _dt{};
+  Index _start{}, _size{}, _current{};
+};
+
+// --------------------------------------------------------------- boundary
+class Boundary {
+ public:
+  virtual ~Boundary() = default;
+};
+
+// Convolutional PML (Roden & Gedney 2000, CFS-CPML) on one face.
+// Profiles (this project's choice; XFDTD_CORE's defaults are not visible):
+//   sigma(r) = sigma_max (r/L)^m, sigma_max = sigma_ratio * 0.8 (m+1)/(eta0 dl)
+//   kappa(r) = 1 + (kappa_max - 1)(r/L)^m
+//   alpha(r) = alpha_min + (alpha_max - alpha_min)(1 - r/L)
+//   b = exp(-(sigma/kappa + alpha) dt/eps0), a = sigma/(sigma kappa +
+//   kappa^2 alpha) (b - 1), with r the depth from the inner interface.
+class PML : public Boundary {
+ public:
+  PML(Index thickness, Axis::Direction direction, Index order = 3,
+      Real sigma_ratio = 1.0, Real alpha_min = 0.0, Real alpha_max = 0.05,
+      Real kappa_max = 1.0);
+  Index thickness() const { return _thickness; }
+  Axis::Direction direction() const { return _direction; }
+  Axis::XYZ mainAxis() const;
+  xf_pml& desc() { return _desc; }
+
+ private:
+  friend class Simulation;
+  Index _thickness;
+  Axis::Direction _direction;
+  Index _order;
+  Real _sigma_ratio, _alpha_min, _alpha_max, _kappa_max;
+  Array _ca_e, _cb_e, _ca_h, _cb_h;
+  Array _c_ea, _c_eb, _c_ha, _c_hb;
+  Array _psi_ea, _psi_eb, _psi_ha, _psi_hb;
+  std::vector<Real> _kappa_e, _kappa_h;
+  xf_pml _desc{};
+};
+
+// ------------------------------------------------------------------- TFSF
+class WaveformSource {
+ public:
+  virtual ~WaveformSource() = default;
+};
+
+// Total-field / scattered-field plane wave (3-D, or 2-D TMz with
+// dimension 2).  The incident field comes from a 1-D auxiliary FDTD grid
+// with the dispersion-matching cell ratio of test/2d.cu:793-843, an
+// incident point two aux cells before the first TF corner, and a
+// first-order Mur end condition (test/2d.cu:850-890); the aux E is driven
+// by the waveform sampled at t = n dt.
+class TFSF : public WaveformSource {
+ public:
+  TFSF(Index start_x, Index start_y, Index start_z, Real theta, Real phi,
+       Real psi, std::unique_ptr<Waveform> waveform, int dimension);
+  Waveform* waveform() { return _waveform.get(); }
+  xf_tfsf& desc() { return _desc; }
+  int dimension() const { return _dimension; }
+  Real theta() const { return _theta; }
+  Real phi() const { return _phi; }
+  Real psi() const { return _psi; }
+  const std::array<Real, 3>& kVector() const { return _k; }
+
+ private:
+  friend class Simulation;
+  Index _sx, _sy, _sz;
+  Real _theta, _phi, _psi;
+  std::unique_ptr<Waveform> _waveform;
+  int _dimension;
+  std::array<Real, 3> _k{};
+  Array _pxi, _pyi, _pzi, _pxh, _pyh, _pzh, _e_inc, _h_inc;
+  xf_tfsf _desc{};
+};
+
+class TFSF3D : public TFSF {
+ public:
+  TFSF3D(Index start_x, Index start_y, Index start_z, Real theta, Real phi,
+         Real psi, std::unique_ptr<Waveform> waveform)
+      : TFSF(start_x, start_y, start_z, theta, phi, psi, std::move(waveform),
+             3) {}
+};
+
+class TFSF2D : public TFSF {
+ public:
+  // TMz plane wave in the xy plane at azimuth phi.
+  TFSF2D(Index start_x, Index start_y, Real phi,
+         std::unique_ptr<Waveform> waveform)
+      : TFSF(start_x, start_y, 0, constant::PI / 2, phi, 0,
+             std::move(waveform), 2) {}
+};
+
+// ---------------------------------------------------------------- NF2FF
+class NF2FF {
+ public:
+  virtual ~NF2FF() = default;
+};
+
+// Frequency-domain Huygens box at an inset of (sx, sy, sz) cells from the
+// outer boundary.  Phasors (this project's choice, matching the reference's
+// time axes eTime = (n+1) dt, hTime = (n+1/2) dt, time_param.cu:13-31):
+//   transform_e[n] = dt exp(-j w (n+1) dt),  transform_h[n] = dt exp(-j w (n+1/2) dt).
+class NFFFTFrequencyDomain : public NF2FF {
+ public:
+  NFFFTFrequencyDomain(Index sx, Index sy, Index sz,
+                       std::vector<Real> frequencies);
+  Index freqCount() const { return _freqs.size(); }
+  const std::vector<Real>& frequencies() const { return _freqs; }
+  xf_nf2ff& desc() { return _desc; }
+  // Far field along directions (theta_i, phi): complex E_theta, E_phi
+  // (per unit incident spectrum when normalise is true) and bistatic RCS.
+  struct FarField {
+    std::vector<std::complex<Real>> e_theta, e_phi;
+    std::vector<Real> rcs;
+  };
+  FarField processFarField(Index freq_index, const std::vector<Real>& theta,
+                           Real phi, std::complex<Real> incident) const;
+
+ private:
+  friend class Simulation;
+  Index _sx, _sy, _sz;
+  std::vector<Real> _freqs;
+  const GridSpace* _grid{};
+  std::vector<Array> _cur;           // [freq][face][4]
+  std::vector<Array> _te, _th;       // [freq]
+  std::vector<xf_nf2ff_freq> _fdesc;
+  xf_nf2ff _desc{};
+};
+
+// --------------------------------------------------------------- ADE
+// ADE storage in the reference's layout (ade_method.cuh:14-183).
+// Coefficients (this project's derivation, second-order accurate):
+//  Drude  dJ/dt + g J = e0 wp^2 E   at n+1/2:
+//    cJJ = (1 - g dt/2)/(1 + g dt/2), cJE = e0 wp^2 dt/2 / (1 + g dt/2).
+//  Debye  tau dJ/dt + J = e0 de dE/dt at n+1/2:
+//    cJJ = (1 - dt/2tau)/(1 + dt/2tau), cJE = (e0 de / tau)/(1 + dt/2tau).
+//  M-Lorentz  b0 J + b1 J' + b2 J'' = e0 (a0 E' + a1 E'') central at n:
+//    K = b1/2dt + b2/dt^2, cJJ = (2 b2/dt^2 - b0)/K,
+//    cJJp = (b1/2dt - b2/dt^2)/K, cJEn = e0 (a0/2dt + a1/dt^2)/K,
+//    cJE = -2 e0 a1/(dt^2 K), cJEp = e0 (a1/dt^2 - a0/2dt)/K.
+//  Ampere at n+1/2 with J^{n+1/2} = (J^{n+1} + J^n)/2 gives the Yee
+//  coefficients and cEJsum = -1/(2 D), cJsumJ = 1 + cJJ, cJsumJp = cJJp,
+//  cEEp = -sum cJEp/(2 D), D = e0 eps_inf/dt + sigma/2 + sum(cJE or cJEn)/2.
+class ADEMethodStorage {
+ public:
+  int kind() const { return _desc.kind; }
+  Index numPole() const { return _np; }
+  xf_ade& desc() { return _desc; }
+
+ private:
+  friend class Simulation;
+  Index _np{};
+  Array _cjj, _cjjp, _cjen, _cje, _cjep, _cjsj, _cjsjp, _cejs, _ceep;
+  Array _exp, _eyp, _ezp, _jx, _jy, _jz, _jxp, _jyp, _jzp;
+  xf_ade _desc{};
+};
+
+// -------------------------------------------------------------- Simulation
+// Scene + set-up.  Simulation(dx, dy, dz, cfl): the grid covers the first
+// object ("domain"); each PML adds its thickness outside it.  A domain
+// infinite along z gives a 2-D grid (nz = 1).
+class Simulation {
+ public:
+  Simulation(Real dx, Real dy, Real dz, Real cfl);
+  void addObject(std::shared_ptr<Object> o) { _objects.push_back(std::move(o)); }
+  void addBoundary(std::shared_ptr<Boundary> b) { _boundaries.push_back(std::move(b)); }
+  void addWaveformSource(std::shared_ptr<WaveformSource> w) { _sources.push_back(std::move(w)); }
+  void addNF2FF(std::shared_ptr<NF2FF> n) { _nf2ffs.push_back(std::move(n)); }
+
+  // Coefficient layout handed to the engine: dense per-field arrays (the
+  // reference's layout) or a per-cell material index + material table.
+  enum class CoefficientLayout { Dense = 0, Compact = 1 };
+  void setCoefficientLayout(CoefficientLayout l) { _layout = l; }
+
+  // Builds grid, materials, coefficients, PML, TFSF tables, NF2FF phasors,
+  // ADE storage and zeroed fields for time_step steps.
+  void init(Index time_step);
+  bool initialized() const { return _initialized; }
+
+  const std::shared_ptr<GridSpace>& gridSpace() const { return _grid; }
+  TimeParam& timeParam() { return _time; }
+  const std::vector<std::shared_ptr<Object>>& objects() const { return _objects; }
+  const std::vector<std::shared_ptr<Boundary>>& boundaries() const { return _boundaries; }
+  const std::vector<std::shared_ptr<WaveformSource>>& waveformSources() const { return _sources; }
+  const std::vector<std::shared_ptr<NF2FF>>& nf2ffs() const { return _nf2ffs; }
+  ADEMethodStorage* aDEMethodStorage() { return _ade.get(); }
+  Real cfl() const { return _cfl; }
+
+  // The mirrored state as POD (pointers into this object's arrays).
+  xf_scene* scene() { return &_scene; }
+  Array& field(int attr_e0_h1, int axis) { return _fields[attr_e0_h1 * 3 + axis]; }
+  Array& coefficient(int n) { return _coeff[n]; }
+  const std::vector<uint16_t>& cellMaterial() const { return _cell_mat; }
+  Index materialCount() const { return _mat_table.size(); }
+  const Material& material(Index m) const { return *_mat_table[m]; }
+
+ private:
+  Real _dx, _dy, _dz, _cfl;
+  CoefficientLayout _layout{CoefficientLayout::Dense};
+  std::vector<std::shared_ptr<Object>> _objects;
+  std::vector<std::shared_ptr<Boundary>> _boundaries;
+  std::vector<std::shared_ptr<WaveformSource>> _sources;
+  std::vector<std::shared_ptr<NF2FF>> _nf2ffs;
+  std::shared_ptr<GridSpace> _grid;
+  TimeParam _time;
+  std::array<Array, 6> _fields;
+  std::array<Array, 18> _coeff;
+  std::vector<uint16_t> _cell_mat;
+  std::vector<const Material*> _mat_table;
+  std::vector<Real> _mat_coeff;  // (n_mat, 18)
+  std::unique_ptr<ADEMethodStorage> _ade;
+  std::vector<xf_pml> _pml_desc;
+  std::vector<xf_nf2ff> _nf_desc;
+  xf_scene _scene{};
+  bool _initialized{false};
+
+  void buildGrid();
+  void voxelize();
+  void buildCoefficients();
+  void buildPML(PML& p);
+  void buildTFSF(TFSF& t);
+  void buildNF2FF(NFFFTFrequencyDomain& n);
+  void buildADE();
+  void fillScene();
+};
+
+}  // namespace xfdtd
+
+#endif  // XFDTD_CUDA_AMD_XFDTD_CORE_H_

@@ -1,0 +1,240 @@
+/*
+ * xfdtd_hip.h -- C ABI of the MI355X (gfx950) FDTD time-stepping engine.
+ *
+ * The engine replaces the per-timestep device path of XFDTD_CUDA's
+ * xfdtd::cuda::SimulationHD::run(n) (src/simulation/simulation_hd.cu:106-257):
+ * the Yee E/H updators, the CPML correctors, the TFSF corrector, the
+ * NF2FF frequency-domain accumulation and the ADE (Drude / Debye / M-Lorentz)
+ * updators.  Everything crosses this boundary as plain pointers and sizes.
+ *
+ * Data model.  An xf_scene describes the state SimulationHD mirrors from the
+ * host simulation: the six field arrays, the 18 Yee coefficient arrays, the
+ * PML faces, the TFSF source, the NF2FF surfaces and the ADE storage.  Arrays
+ * are row-major with the last index fastest (z fastest for 3-D arrays), exactly
+ * the reference's xfdtd::cuda::Tensor layout (include/xfdtd_cuda/tensor.cuh:
+ * 203-212).  Host arrays are borrowed: xf_upload reads them, xf_download
+ * writes the state arrays (fields, psi, ADE J/E_prev, NF2FF currents) back in
+ * place, like TensorHD::copyDeviceToHost (include/xfdtd_cuda/tensor_hd.cuh:
+ * 126-142).
+ *
+ * Field shapes (nx, ny, nz = cell counts, GridSpace::sizeX/Y/Z):
+ *   ex (nx, ny+1, nz+1)  ey (nx+1, ny, nz+1)  ez (nx+1, ny+1, nz)
+ *   hx (nx+1, ny, nz)    hy (nx, ny+1, nz)    hz (nx, ny, nz+1)
+ * Coefficient arrays have the shape of the field they update.
+ *
+ * Every function returns XF_OK (0) or an XF_ERR_* code; xf_last_error()
+ * gives the message of the last failure on the calling thread.
+ */
+#ifndef XFDTD_HIP_H_
+#define XFDTD_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XF_ABI_VERSION 1
+
+enum xf_status {
+  XF_OK = 0,
+  XF_ERR_INVALID = 1,     /* bad argument, shape or index range          */
+  XF_ERR_HIP = 2,         /* HIP runtime error                           */
+  XF_ERR_STATE = 3,       /* call out of order (e.g. run before upload)  */
+  XF_ERR_UNSUPPORTED = 4, /* configuration the engine does not implement */
+  XF_ERR_NOMEM = 5        /* device or host allocation failed            */
+};
+
+enum xf_axis { XF_AXIS_X = 0, XF_AXIS_Y = 1, XF_AXIS_Z = 2 };
+
+/* NF2FF face order used throughout (reference Axis::Direction XN..ZP). */
+enum xf_face { XF_XN = 0, XF_XP = 1, XF_YN = 2, XF_YP = 3, XF_ZN = 4, XF_ZP = 5 };
+
+enum xf_ade_kind {
+  XF_ADE_NONE = 0,
+  XF_ADE_DRUDE = 1,   /* drude_ade_updator.cu:35-60      */
+  XF_ADE_DEBYE = 2,   /* debye_ade_updator.cu:27-52      */
+  XF_ADE_MLORENTZ = 3 /* m_lor_ade_updator.cu:27-88      */
+};
+
+/* Arithmetic of the time-stepping kernels (a run-time choice; the reference
+ * fixes Real at compile time, tensor_texture_ref.cuh:38-52). */
+enum xf_precision {
+  XF_FP64 = 0,      /* fields, coefficients and arithmetic in double       */
+  XF_FP32 = 1,      /* fields and Yee/CPML/TFSF arithmetic in float        */
+  XF_BF16_STORE = 2 /* fields stored bf16 (scaled), arithmetic in float    */
+};
+
+/* Half-open index range, reference IndexRange (index_task.cuh:16-40). */
+typedef struct xf_range {
+  int64_t start, end;
+} xf_range;
+
+/* Index box, reference IndexTask (index_task.cuh:42-70). */
+typedef struct xf_task {
+  xf_range x, y, z;
+} xf_task;
+
+/* A borrowed host array of doubles.  Complex arrays carry a trailing
+ * dimension of 2 (interleaved re/im, the layout of std::complex<double>). */
+typedef struct xf_array {
+  double* data;
+  int64_t shape[5];
+  int32_t ndim;
+} xf_array;
+
+/* One CPML face: reference PMLCorrectorHD<xyz> (src/boundary/
+ * pml_corrector_hd.cu:7-34) and PMLCorrector<xyz> (pml_corrector.cuh:17-35). */
+typedef struct xf_pml {
+  int32_t main_axis; /* enum xf_axis */
+  xf_task task;      /* PML::task(): cell box of the slab */
+  int64_t global_e_start, global_h_start; /* globalE/HNodeStartIndexMainAxis */
+  int64_t node_e_start, node_h_start;     /* nodeE/HNodeStartIndexMainAxis   */
+  int64_t offset_c;                       /* offsetC()                       */
+  xf_array coeff_a_e, coeff_b_e, coeff_a_h, coeff_b_h;           /* 1-D   */
+  xf_array c_ea_psi_hb, c_eb_psi_ha, c_ha_psi_eb, c_hb_psi_ea;   /* 3-D   */
+  xf_array ea_psi_hb, eb_psi_ha, ha_psi_eb, hb_psi_ea; /* 3-D state, r/w   */
+} xf_pml;
+
+/* TFSF plane-wave source: reference TFSFCorrectorHD (src/waveform_source/
+ * tfsf/tfsf_corrector_hd.cu:9-74) and TFSFCorrector (tfsf_corrector.cuh). */
+typedef struct xf_tfsf {
+  int32_t dimension; /* 2 (TFSF2D agency) or 3 (TFSF3D agency) */
+  xf_task total_task;
+  xf_array proj_x_int, proj_y_int, proj_z_int;    /* 1-D */
+  xf_array proj_x_half, proj_y_half, proj_z_half; /* 1-D */
+  xf_array e_inc, h_inc;                          /* (nt, n_aux) */
+  double cax, cbx, cay, cby, caz, cbz;
+  double transform_e[3], transform_h[3];
+} xf_tfsf;
+
+/* One frequency of an NF2FF frequency-domain surface: reference
+ * SurfaceCurrentSetFD (nf2ff_frequency_domain_hd.cu:140-247). */
+typedef struct xf_nf2ff_freq {
+  xf_array transform_e, transform_h; /* complex, (nt, 2)                 */
+  xf_array cur[6][4]; /* per face: ja, jb, ma, mb, complex (sx, sy, sz, 2) */
+} xf_nf2ff_freq;
+
+/* NF2FF frequency-domain box: reference NF2FFFrequencyDomainHD. */
+typedef struct xf_nf2ff {
+  xf_task face_task[6]; /* globalTaskSurfaceXN..ZP; empty ranges skip a face */
+  int32_t n_freq;
+  xf_nf2ff_freq* freq;
+} xf_nf2ff;
+
+/* ADE storage: reference ADEMethodStorage (src/include/material/ade_method/
+ * ade_method.cuh:14-183).  4-D arrays are (nx+1, ny+1, nz+1, num_pole),
+ * 3-D arrays are (nx+1, ny+1, nz+1). */
+typedef struct xf_ade {
+  int32_t kind; /* enum xf_ade_kind */
+  int64_t num_pole;
+  xf_array coeff_j_j, coeff_j_j_p, coeff_j_e_n, coeff_j_e, coeff_j_e_p;
+  xf_array coeff_j_sum_j, coeff_j_sum_j_p;
+  xf_array coeff_e_j_sum, coeff_e_e_p;
+  xf_array ex_prev, ey_prev, ez_prev;               /* state */
+  xf_array jx, jy, jz, jx_prev, jy_prev, jz_prev;   /* state */
+} xf_ade;
+
+/* The whole mirrored simulation (GridSpace + CalculationParam + EMF + the
+ * correctors/monitors SimulationHD::run assembles, simulation_hd.cu:106-226). */
+typedef struct xf_scene {
+  int32_t abi_version; /* XF_ABI_VERSION */
+  int32_t dimension;   /* 2 (TE updator, Ez/Hx/Hy) or 3 */
+  int64_t nx, ny, nz;  /* cells */
+  double dt;
+  int64_t start_step;   /* TimeParam::startTimeStep()   */
+  int64_t n_steps;      /* TimeParam::size()            */
+  int64_t current_step; /* TimeParam::currentTimeStep() */
+  xf_array ex, ey, ez, hx, hy, hz; /* EMF, state */
+  /* FDTDCoefficient (fdtd_coefficient.cuh:149-167), in this order:
+   * cexe cexhy cexhz ceye ceyhz ceyhx ceze cezhx cezhy
+   * chxh chxey chxez chyh chyez chyex chzh chzex chzey */
+  xf_array coeff[18];
+  /* Optional compact form of coeff[] (no reference counterpart; lets a host
+   * that knows its materials skip the 18 dense arrays).  Used when
+   * coeff[0].data is NULL: the coefficient n (0..17, order above) of any
+   * field at index (i, j, k) is material_coeff[m * 18 + n] with
+   * m = cell_material[(ci * ny + cj) * nz + ck], c* = min(index, n* - 1). */
+  const uint16_t* cell_material;
+  int32_t n_material;
+  const double* material_coeff;
+  int32_t n_pml;
+  xf_pml* pml; /* order of registration: X faces, then Y, then Z */
+  xf_tfsf* tfsf;   /* NULL: no TFSF source */
+  int32_t n_nf2ff;
+  xf_nf2ff* nf2ff;
+  xf_ade* ade; /* NULL: non-dispersive */
+} xf_scene;
+
+/* Launch plan.  grid_dim/block_dim are the reference's setGridDim/
+ * setBlockDim values (simulation_hd.cu:269-273); the engine picks its own
+ * tiling and records them only as hints. */
+typedef struct xf_plan {
+  int32_t precision;    /* enum xf_precision */
+  int32_t grid_dim[3];  /* hint */
+  int32_t block_dim[3]; /* hint */
+  int32_t use_graph;    /* capture each step in a hipGraph and replay */
+  int32_t timing;       /* record per-kernel HIP events in xf_run */
+  int32_t tile_y;       /* 0 = default; rows of the Yee tile (1..16) */
+  int32_t device;       /* HIP device ordinal */
+} xf_plan;
+
+/* Per-kernel device time accumulated by xf_run while plan.timing != 0. */
+typedef struct xf_timing {
+  double step_ms;        /* wall-clock (events) of the whole run / steps */
+  double yee_h_ms;       /* sum over launches of the H-pass kernel        */
+  double yee_e_ms;       /* sum over launches of the E-pass kernel        */
+  double tfsf_ms, nf2ff_ms, ade_ms;
+  int64_t yee_h_launches, yee_e_launches, steps;
+  double bytes_per_cell_step; /* algorithmic HBM bytes of the H+E passes   */
+  double run_ms;              /* event time of the last xf_run             */
+} xf_timing;
+
+typedef struct xf_handle xf_handle;
+
+/* Replaces SimulationHD's constructor + init + makeDomainHD/agency assembly
+ * (simulation_hd.cu:44,106-226,259-267,407-503).  Validates every shape and
+ * index range against the kernels' assumptions; nothing is allocated on the
+ * device yet. */
+int xf_create(xf_scene* scene, const xf_plan* plan, xf_handle** out);
+
+/* Replaces SimulationHD::copyHostToDevice (simulation_hd.cu:48-60) and the
+ * per-agency copyHostToDevice calls: allocates device memory and uploads. */
+int xf_upload(xf_handle* h);
+
+/* Replaces DomainHD::run (src/domain/domain_hd.cu:35-47): advances n steps
+ * (H, correct H, E, correct E, record, next).  Synchronous. */
+int xf_run(xf_handle* h, int64_t n_steps);
+
+/* Replaces SimulationHD::copyDeviceToHost (simulation_hd.cu:62-86) plus the
+ * NF2FF and PML psi copy-back: writes all state arrays and current_step
+ * into the scene. */
+int xf_download(xf_handle* h);
+
+/* Replaces SimulationHD::releaseDevice / destructor (simulation_hd.cu:88-104). */
+int xf_destroy(xf_handle* h);
+
+/* Current time step of the engine (TimeParam::currentTimeStep). */
+int xf_current_step(const xf_handle* h, int64_t* step);
+
+/* Per-kernel timing of the runs since the last reset; xf_set_timing turns
+ * the per-kernel events of later xf_run calls on or off. */
+int xf_get_timing(const xf_handle* h, xf_timing* out);
+int xf_reset_timing(xf_handle* h);
+int xf_set_timing(xf_handle* h, int32_t on);
+
+/* Description of the chosen layout (coefficient compression, tile). */
+int xf_describe(const xf_handle* h, char* buf, size_t len);
+
+/* Last error message of the calling thread ("" if none). */
+const char* xf_last_error(void);
+
+/* Number of visible HIP devices (0 without a GPU); never faults. */
+int xf_device_count(int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XFDTD_HIP_H_ */

@@ -1,0 +1,56 @@
+"""CPU oracle of the FDTD time step (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+this module; the product (xfdtd_cuda_amd) never does.  It wraps
+oracle/libxf_oracle.so (built from xf_oracle.c by oracle/Makefile), which
+advances an xf_scene in place on the host with the reference's algorithm.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "libxf_oracle.so")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise ImportError(f"{LIB} is not built (make -C oracle)")
+        l = ctypes.CDLL(LIB)
+        l.xfo_run.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        l.xfo_run.restype = ctypes.c_int
+        l.xfo_last_error.restype = ctypes.c_char_p
+        l.xfo_num_threads.restype = ctypes.c_int
+        l.xfo_set_num_threads.argtypes = [ctypes.c_int]
+        l.xfo_yee_component_triple_loop.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        l.xfo_yee_component_triple_loop.restype = ctypes.c_int
+        _lib = l
+    return _lib
+
+
+def run(scene_ptr: int, n_steps: int) -> None:
+    """Advance the scene n_steps on the CPU (in place)."""
+    st = lib().xfo_run(ctypes.c_void_p(scene_ptr), n_steps)
+    if st != 0:
+        raise RuntimeError("oracle: " + lib().xfo_last_error().decode())
+
+
+def set_threads(n: int) -> None:
+    lib().xfo_set_num_threads(n)
+
+
+def threads() -> int:
+    return lib().xfo_num_threads()
+
+
+def yee_component_triple_loop(scene_ptr: int, is_e: bool, axis: int) -> None:
+    lib().xfo_yee_component_triple_loop(ctypes.c_void_p(scene_ptr), int(is_e), axis)

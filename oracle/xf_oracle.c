@@ -1,0 +1,632 @@
+/*
+ * xf_oracle.c -- CPU restatement of XFDTD_CUDA's per-timestep FDTD path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP engine in
+ * xfdtd_cuda_amd/csrc: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product never calls it.
+ *
+ * It advances an xf_scene (include/xfdtd_hip.h) in place, following the
+ * reference kernels operation by operation, in the reference's step order
+ * (src/domain/domain_hd.cu:35-87):
+ *   updateH -> correctH (TFSF, then PML X, Y, Z) -> updateE ->
+ *   correctE (TFSF, then PML) -> record (NF2FF-FD) -> nextStep.
+ * Floating-point expressions keep the reference's evaluation order and are
+ * compiled without contraction (-ffp-contract=off), so a GPU build with the
+ * same order and no FMA reproduces the result bit for bit.
+ *
+ * Pins (see tests/): Yee coefficients and dispersion against closed forms,
+ * PEC-cavity energy, CPML reflection, TFSF leakage, the DFT of a known
+ * sinusoid for NF2FF, the analytic permittivity of the ADE media, and
+ * a brute-force triple-loop form of the flat Yee update.
+ */
+#include "xf_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------- helpers */
+
+static inline double* at3(const xf_array* a, int64_t i, int64_t j, int64_t k) {
+  return a->data + ((i * a->shape[1] + j) * a->shape[2] + k);
+}
+
+static inline double* at4(const xf_array* a, int64_t i, int64_t j, int64_t k,
+                          int64_t p) {
+  return a->data + (((i * a->shape[1] + j) * a->shape[2] + k) * a->shape[3] + p);
+}
+
+/* complex arrays: trailing dimension 2 */
+static inline double* atc3(const xf_array* a, int64_t i, int64_t j, int64_t k) {
+  return a->data + (((i * a->shape[1] + j) * a->shape[2] + k) * 2);
+}
+
+static inline double at2(const xf_array* a, int64_t r, int64_t c) {
+  return a->data[r * a->shape[1] + c];
+}
+
+/* transform::xYZToABC / aBCToXYZ (XFDTD_CORE util/transform/abc_xyz.h, used
+ * at update_scheme.cuh:172-178): X -> (a,b,c) = (y,z,x), Y -> (z,x,y),
+ * Z -> (x,y,z). */
+static inline void xyz_to_abc(int axis, int64_t i, int64_t j, int64_t k,
+                              int64_t* a, int64_t* b, int64_t* c) {
+  if (axis == 0) { *a = j; *b = k; *c = i; }
+  else if (axis == 1) { *a = k; *b = i; *c = j; }
+  else { *a = i; *b = j; *c = k; }
+}
+
+static inline void abc_to_xyz(int axis, int64_t a, int64_t b, int64_t c,
+                              int64_t* i, int64_t* j, int64_t* k) {
+  if (axis == 0) { *i = c; *j = a; *k = b; }
+  else if (axis == 1) { *i = b; *j = c; *k = a; }
+  else { *i = a; *j = b; *k = c; }
+}
+
+static inline int64_t range_get_start(const xf_task* t, int axis) {
+  return axis == 0 ? t->x.start : axis == 1 ? t->y.start : t->z.start;
+}
+static inline int64_t range_get_end(const xf_task* t, int axis) {
+  return axis == 0 ? t->x.end : axis == 1 ? t->y.end : t->z.end;
+}
+
+/* Coefficient n (order of xf_scene.coeff) at (i, j, k): the dense array, or
+ * the compact material table with clamped cell indices. */
+static inline double coef(const xf_scene* s, int n, int64_t i, int64_t j,
+                          int64_t k) {
+  if (s->coeff[n].data) return *at3(&s->coeff[n], i, j, k);
+  const int64_t ci = i < s->nx ? i : s->nx - 1;
+  const int64_t cj = j < s->ny ? j : s->ny - 1;
+  const int64_t ck = k < s->nz ? k : s->nz - 1;
+  return s->material_coeff[(int64_t)s->cell_material[(ci * s->ny + cj) * s->nz + ck] * 18 + n];
+}
+
+static const xf_array* efield(const xf_scene* s, int axis) {
+  return axis == 0 ? &s->ex : axis == 1 ? &s->ey : &s->ez;
+}
+static const xf_array* hfield(const xf_scene* s, int axis) {
+  return axis == 0 ? &s->hx : axis == 1 ? &s->hy : &s->hz;
+}
+
+/* ---------------------------------------------------------- Yee updates */
+
+/* eNext / hNext: update_scheme.cuh:27-32, 58-63. */
+static inline double e_next(double cece, double ec, double cecha, double ha_p,
+                            double ha_q, double cechb, double hb_p,
+                            double hb_q) {
+  return cece * ec + cecha * (ha_p - ha_q) + cechb * (hb_p - hb_q);
+}
+static inline double h_next(double chch, double hc, double chcea, double ea_p,
+                            double ea_q, double chceb, double eb_p,
+                            double eb_q) {
+  return chch * hc + chcea * (ea_q - ea_p) + chceb * (eb_q - eb_p);
+}
+
+/* update<attribute, xyz>(emf, coeff, start, end, nx, ny, nz):
+ * update_scheme.cuh:141-197 over the whole node range [0, nx*ny*nz). */
+static void yee_update(const xf_scene* s, int is_e, int xyz) {
+  const int a_axis = (xyz + 1) % 3, b_axis = (xyz + 2) % 3;
+  const xf_array* f = is_e ? efield(s, xyz) : hfield(s, xyz);
+  const xf_array* fa = is_e ? hfield(s, a_axis) : efield(s, a_axis);
+  const xf_array* fb = is_e ? hfield(s, b_axis) : efield(s, b_axis);
+  const int base = (is_e ? 0 : 9) + 3 * xyz;
+  const int64_t nx = s->nx, ny = s->ny, nz = s->nz;
+  const int64_t off = is_e ? -1 : 1;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nx; ++i) {
+    for (int64_t j = 0; j < ny; ++j) {
+      for (int64_t k = 0; k < nz; ++k) {
+        int64_t a, b, c, ia, ja, ka, ib, jb, kb;
+        xyz_to_abc(xyz, i, j, k, &a, &b, &c);
+        abc_to_xyz(xyz, a, b + off, c, &ia, &ja, &ka);
+        abc_to_xyz(xyz, a + off, b, c, &ib, &jb, &kb);
+        if (is_e) {
+          if (a == 0 || b == 0) continue;
+          *at3(f, i, j, k) = e_next(coef(s, base, i, j, k), *at3(f, i, j, k),
+                                    coef(s, base + 1, i, j, k), *at3(fa, i, j, k),
+                                    *at3(fa, ia, ja, ka), coef(s, base + 2, i, j, k),
+                                    *at3(fb, i, j, k), *at3(fb, ib, jb, kb));
+        } else {
+          *at3(f, i, j, k) = h_next(coef(s, base, i, j, k), *at3(f, i, j, k),
+                                    coef(s, base + 1, i, j, k), *at3(fa, i, j, k),
+                                    *at3(fa, ia, ja, ka), coef(s, base + 2, i, j, k),
+                                    *at3(fb, i, j, k), *at3(fb, ib, jb, kb));
+        }
+      }
+    }
+  }
+}
+
+/* ADE E update: TemplateADEUpdateScheme::updateE
+ * (template_ade_update_scheme.cuh:15-86) with the model's calculateJSum /
+ * updateJ / ePrevious / coeffEPrev / recordEPrevious (drude_ade_updator.cu:
+ * 35-60, debye_ade_updator.cu:27-52, m_lor_ade_updator.cu:27-88, and the
+ * zero-returning ePrevious/coeffEPrev of drude/debye_ade_updator.cuh). */
+static void ade_update_e(const xf_scene* s, int xyz) {
+  const xf_ade* ad = s->ade;
+  const int a_axis = (xyz + 1) % 3, b_axis = (xyz + 2) % 3;
+  const xf_array* f = efield(s, xyz);
+  const xf_array* fa = hfield(s, a_axis);
+  const xf_array* fb = hfield(s, b_axis);
+  const int base = 3 * xyz;
+  const xf_array* jarr = xyz == 0 ? &ad->jx : xyz == 1 ? &ad->jy : &ad->jz;
+  const xf_array* jparr =
+      xyz == 0 ? &ad->jx_prev : xyz == 1 ? &ad->jy_prev : &ad->jz_prev;
+  const xf_array* eprev =
+      xyz == 0 ? &ad->ex_prev : xyz == 1 ? &ad->ey_prev : &ad->ez_prev;
+  const int64_t np = ad->num_pole;
+  const int kind = ad->kind;
+  const int64_t nx = s->nx, ny = s->ny, nz = s->nz;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nx; ++i) {
+    for (int64_t j = 0; j < ny; ++j) {
+      for (int64_t k = 0; k < nz; ++k) {
+        int64_t a, b, c, ia, ja, ka, ib, jb, kb;
+        xyz_to_abc(xyz, i, j, k, &a, &b, &c);
+        abc_to_xyz(xyz, a, b - 1, c, &ia, &ja, &ka);
+        abc_to_xyz(xyz, a - 1, b, c, &ib, &jb, &kb);
+        if (a == 0 || b == 0) continue;
+        /* calculateJSum */
+        double j_sum = 0;
+        for (int64_t p = 0; p < np; ++p) {
+          if (kind == XF_ADE_MLORENTZ) {
+            j_sum += *at4(&ad->coeff_j_sum_j, i, j, k, p) * *at4(jarr, i, j, k, p) +
+                     *at4(&ad->coeff_j_sum_j_p, i, j, k, p) * *at4(jparr, i, j, k, p);
+          } else {
+            j_sum += *at4(&ad->coeff_j_sum_j, i, j, k, p) * *at4(jarr, i, j, k, p);
+          }
+        }
+        double e_prev = 0.0, coeff_e_e_p = 0.0;
+        if (kind == XF_ADE_MLORENTZ) {
+          e_prev = *at3(eprev, i, j, k);
+          coeff_e_e_p = *at3(&ad->coeff_e_e_p, i, j, k);
+        }
+        const double e_cur = *at3(f, i, j, k);
+        const double e_new =
+            coeff_e_e_p * e_prev +
+            e_next(coef(s, base, i, j, k), *at3(f, i, j, k), coef(s, base + 1, i, j, k),
+                   *at3(fa, i, j, k), *at3(fa, ia, ja, ka), coef(s, base + 2, i, j, k),
+                   *at3(fb, i, j, k), *at3(fb, ib, jb, kb)) +
+            *at3(&ad->coeff_e_j_sum, i, j, k) * j_sum;
+        *at3(f, i, j, k) = e_new;
+        /* updateJ */
+        for (int64_t p = 0; p < np; ++p) {
+          double* jv = at4(jarr, i, j, k, p);
+          if (kind == XF_ADE_DRUDE) {
+            *jv = *at4(&ad->coeff_j_j, i, j, k, p) * *jv +
+                  *at4(&ad->coeff_j_e, i, j, k, p) * (e_new + e_cur);
+          } else if (kind == XF_ADE_DEBYE) {
+            *jv = *at4(&ad->coeff_j_j, i, j, k, p) * *jv +
+                  *at4(&ad->coeff_j_e, i, j, k, p) * (e_new - e_cur);
+          } else {
+            double* jpv = at4(jparr, i, j, k, p);
+            const double j_next =
+                *at4(&ad->coeff_j_e_n, i, j, k, p) * e_new +
+                *at4(&ad->coeff_j_e, i, j, k, p) * e_cur +
+                *at4(&ad->coeff_j_e_p, i, j, k, p) * e_prev +
+                *at4(&ad->coeff_j_j, i, j, k, p) * *jv +
+                *at4(&ad->coeff_j_j_p, i, j, k, p) * *jpv;
+            *jpv = *jv;
+            *jv = j_next;
+          }
+        }
+        /* recordEPrevious */
+        if (kind == XF_ADE_MLORENTZ) *at3(eprev, i, j, k) = e_cur;
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ CPML */
+
+/* correctPML<attribute, xyz> (pml_corrector.cu:9-48) over the box
+ * a in [a0,a1), b in [b0,b1), c in [c0,c1). */
+static void correct_pml_box(int is_e, int xyz, const xf_array* field,
+                            const xf_array* psi, const xf_array* coeff_a,
+                            const xf_array* coeff_b, const xf_array* dual,
+                            const xf_array* c_psi, int64_t a0, int64_t a1,
+                            int64_t b0, int64_t b1, int64_t c0, int64_t c1,
+                            int64_t pml_global_start, int64_t pml_node_start,
+                            int64_t offset_c) {
+  const int64_t off = is_e ? -1 : 1;
+#pragma omp parallel for schedule(static) collapse(2)
+  for (int64_t c = c0; c < c1; ++c) {
+    for (int64_t a = a0; a < a1; ++a) {
+      for (int64_t b = b0; b < b1; ++b) {
+        int64_t i, j, k, il, jl, kl, id, jd, kd;
+        abc_to_xyz(xyz, a, b, c, &i, &j, &k);
+        const int64_t global_c = c + offset_c;
+        const double ca = coeff_a->data[global_c - pml_global_start];
+        const double cb = coeff_b->data[global_c - pml_global_start];
+        const int64_t layer = c - pml_node_start;
+        abc_to_xyz(xyz, a, b, layer, &il, &jl, &kl);
+        const double cpsi = *at3(c_psi, il, jl, kl);
+        double* pv = at3(psi, il, jl, kl);
+        double* fv = at3(field, i, j, k);
+        const double dp = *at3(dual, i, j, k);
+        abc_to_xyz(xyz, a, b, c + off, &id, &jd, &kd);
+        const double dq = *at3(dual, id, jd, kd);
+        if (is_e) {
+          *pv = cb * *pv + ca * (dp - dq);
+        } else {
+          *pv = cb * *pv + ca * (dq - dp);
+        }
+        *fv += cpsi * *pv;
+      }
+    }
+  }
+}
+
+/* PMLCorrector<xyz>::correctE / correctH (pml_corrector.cu:50-270): the union
+ * of all thread sub-tasks; the "last block" lateral +1 extension becomes an
+ * extension of the whole range. */
+static void pml_correct(const xf_scene* s, const xf_pml* p, int is_e) {
+  const int xyz = p->main_axis;
+  const int a_axis = (xyz + 1) % 3, b_axis = (xyz + 2) % 3;
+  int64_t as, bs, cs, ae, be, ce;
+  xyz_to_abc(xyz, p->task.x.start, p->task.y.start, p->task.z.start, &as, &bs,
+             &cs);
+  xyz_to_abc(xyz, p->task.x.end, p->task.y.end, p->task.z.end, &ae, &be, &ce);
+  if (!(p->task.x.start < p->task.x.end && p->task.y.start < p->task.y.end &&
+        p->task.z.start < p->task.z.end))
+    return;
+  if (is_e) {
+    const int64_t off = (cs == 0) ? 1 : 0;
+    /* EA: [a_s, a_e), [b_s, b_e + 1) */
+    correct_pml_box(1, xyz, efield(s, a_axis), &p->ea_psi_hb, &p->coeff_a_e,
+                    &p->coeff_b_e, hfield(s, b_axis), &p->c_ea_psi_hb, as, ae,
+                    bs, be + 1, cs + off, ce + off, p->global_e_start,
+                    p->node_e_start, p->offset_c);
+    /* EB: [a_s, a_e + 1), [b_s, b_e) */
+    correct_pml_box(1, xyz, efield(s, b_axis), &p->eb_psi_ha, &p->coeff_a_e,
+                    &p->coeff_b_e, hfield(s, a_axis), &p->c_eb_psi_ha, as,
+                    ae + 1, bs, be, cs + off, ce + off, p->global_e_start,
+                    p->node_e_start, p->offset_c);
+  } else {
+    /* HA: [a_s, a_e + 1), [b_s, b_e) */
+    correct_pml_box(0, xyz, hfield(s, a_axis), &p->ha_psi_eb, &p->coeff_a_h,
+                    &p->coeff_b_h, efield(s, b_axis), &p->c_ha_psi_eb, as,
+                    ae + 1, bs, be, cs, ce, p->global_h_start, p->node_h_start,
+                    p->offset_c);
+    /* HB: [a_s, a_e), [b_s, b_e + 1) */
+    correct_pml_box(0, xyz, hfield(s, b_axis), &p->hb_psi_ea, &p->coeff_a_h,
+                    &p->coeff_b_h, efield(s, a_axis), &p->c_hb_psi_ea, as, ae,
+                    bs, be + 1, cs, ce, p->global_h_start, p->node_h_start,
+                    p->offset_c);
+  }
+}
+
+/* ------------------------------------------------------------------ TFSF */
+
+/* exInc..hzInc (tfsf_corrector.cu:18-130). */
+static double tfsf_inc(const xf_tfsf* tf, int is_e, int comp, int64_t t,
+                       int64_t i, int64_t j, int64_t k) {
+  const int64_t gi = tf->total_task.x.start, gj = tf->total_task.y.start,
+                gk = tf->total_task.z.start;
+  double projection;
+  if (is_e) {
+    if (comp == 0) {
+      projection = tf->proj_x_half.data[i - gi + 1] +
+                   tf->proj_y_int.data[j - gj] + tf->proj_z_int.data[k - gk];
+    } else if (comp == 1) {
+      projection = tf->proj_x_int.data[i - gi] +
+                   tf->proj_y_half.data[j - gj + 1] +
+                   tf->proj_z_int.data[k - gk];
+    } else {
+      projection = tf->proj_x_int.data[i - gi] + tf->proj_y_int.data[j - gj] +
+                   tf->proj_z_half.data[k - gk + 1];
+    }
+  } else {
+    if (comp == 0) {
+      projection = tf->proj_x_int.data[i - gi] +
+                   tf->proj_y_half.data[j - gj + 1] +
+                   tf->proj_z_half.data[k - gk + 1] - 0.5;
+    } else if (comp == 1) {
+      projection = tf->proj_x_half.data[i - gi + 1] +
+                   tf->proj_y_int.data[j - gj] +
+                   tf->proj_z_half.data[k - gk + 1] - 0.5;
+    } else {
+      projection = tf->proj_x_half.data[i - gi + 1] +
+                   tf->proj_y_half.data[j - gj + 1] +
+                   tf->proj_z_int.data[k - gk] - 0.5;
+    }
+  }
+  const size_t index = (size_t)projection;
+  const double weight = projection - (double)index;
+  const xf_array* tab = is_e ? &tf->e_inc : &tf->h_inc;
+  const double tr = is_e ? tf->transform_e[comp] : tf->transform_h[comp];
+  return tr * ((1 - weight) * at2(tab, t, (int64_t)index) +
+               weight * at2(tab, t, (int64_t)index + 1));
+}
+
+/* correctTFSF<direction, attribute, field_xyz> (tfsf_corrector.cuh:20-170)
+ * over the union of the thread tasks of one face. face: 0..5 = XN..ZP. */
+static void tfsf_correct_face(const xf_scene* s, int face, int is_e,
+                              int field_xyz, int64_t t) {
+  const xf_tfsf* tf = s->tfsf;
+  const int xyz = face / 2;
+  const int negative = (face % 2) == 0;
+  const int a_axis = (xyz + 1) % 3, b_axis = (xyz + 2) % 3;
+  const int dual_field_xyz = (field_xyz == a_axis) ? b_axis : a_axis;
+  const xf_task* tt = &tf->total_task;
+  /* buildTask<direction> */
+  xf_task task = *tt;
+  const int64_t cpos = negative ? range_get_start(tt, xyz) : range_get_end(tt, xyz);
+  if (xyz == 0) { task.x.start = cpos; task.x.end = cpos + 1; }
+  else if (xyz == 1) { task.y.start = cpos; task.y.end = cpos + 1; }
+  else { task.z.start = cpos; task.z.end = cpos + 1; }
+  if (!(task.x.start < task.x.end && task.y.start < task.y.end &&
+        task.z.start < task.z.end))
+    return;
+  int64_t as, bs, cs, ae, be, ce;
+  xyz_to_abc(xyz, task.x.start, task.y.start, task.z.start, &as, &bs, &cs);
+  xyz_to_abc(xyz, task.x.end, task.y.end, task.z.end, &ae, &be, &ce);
+  const int offset_a = ((is_e && field_xyz == b_axis) || (!is_e && field_xyz == a_axis)) ? 1 : 0;
+  const int offset_b = offset_a == 1 ? 0 : 1;
+  if (offset_a == 1 && ae == range_get_end(tt, a_axis)) ae += offset_a;
+  if (offset_b == 1 && be == range_get_end(tt, b_axis)) be += offset_b;
+  const int equation_flag = is_e ? 1 : -1;
+  const int different_flag = (field_xyz == a_axis) ? -1 : 1;
+  const int direction_flag = negative ? -1 : 1;
+  const int coefficient_flag = equation_flag * different_flag * direction_flag;
+  const double cax[3] = {tf->cax, tf->cay, tf->caz};
+  const double cbx[3] = {tf->cbx, tf->cby, tf->cbz};
+  const double coefficient = (is_e ? cax[xyz] : cbx[xyz]) * coefficient_flag;
+  const xf_array* field = is_e ? efield(s, field_xyz) : hfield(s, field_xyz);
+  int64_t is, js, ks, ie, je, ke;
+  abc_to_xyz(xyz, as, bs, cs, &is, &js, &ks);
+  abc_to_xyz(xyz, ae, be, ce, &ie, &je, &ke);
+  for (int64_t i = is; i < ie; ++i) {
+    for (int64_t j = js; j < je; ++j) {
+      for (int64_t k = ks; k < ke; ++k) {
+        int64_t a, b, c;
+        xyz_to_abc(xyz, i, j, k, &a, &b, &c);
+        if (negative) {
+          if (is_e) {
+            int64_t id, jd, kd;
+            abc_to_xyz(xyz, a, b, c - 1, &id, &jd, &kd);
+            const double v = tfsf_inc(tf, 0, dual_field_xyz, t, id, jd, kd);
+            *at3(field, i, j, k) += v * coefficient;
+          } else {
+            const double v = tfsf_inc(tf, 1, dual_field_xyz, t, i, j, k);
+            int64_t ii, jj, kk;
+            abc_to_xyz(xyz, a, b, c - 1, &ii, &jj, &kk);
+            *at3(field, ii, jj, kk) += v * coefficient;
+          }
+          continue;
+        }
+        const double v = tfsf_inc(tf, !is_e, dual_field_xyz, t, i, j, k);
+        *at3(field, i, j, k) += v * coefficient;
+      }
+    }
+  }
+}
+
+/* TFSFCorrector3DAgency / TFSFCorrector2DAgency::correctE/H launch order
+ * (tfsf_corrector.cu:155-271). */
+static void tfsf_correct(const xf_scene* s, int is_e, int64_t t) {
+  if (s->tfsf->dimension == 2) {
+    if (is_e) {
+      tfsf_correct_face(s, XF_XN, 1, 2, t);
+      tfsf_correct_face(s, XF_XP, 1, 2, t);
+      tfsf_correct_face(s, XF_YN, 1, 2, t);
+      tfsf_correct_face(s, XF_YP, 1, 2, t);
+    } else {
+      tfsf_correct_face(s, XF_XN, 0, 1, t);
+      tfsf_correct_face(s, XF_XP, 0, 1, t);
+      tfsf_correct_face(s, XF_YN, 0, 0, t);
+      tfsf_correct_face(s, XF_YP, 0, 0, t);
+    }
+    return;
+  }
+  const int e = is_e;
+  tfsf_correct_face(s, XF_XN, e, 2, t);
+  tfsf_correct_face(s, XF_XP, e, 2, t);
+  tfsf_correct_face(s, XF_YN, e, 2, t);
+  tfsf_correct_face(s, XF_YP, e, 2, t);
+  tfsf_correct_face(s, XF_ZN, e, 0, t);
+  tfsf_correct_face(s, XF_ZP, e, 0, t);
+  tfsf_correct_face(s, XF_YN, e, 0, t);
+  tfsf_correct_face(s, XF_YP, e, 0, t);
+  tfsf_correct_face(s, XF_ZN, e, 1, t);
+  tfsf_correct_face(s, XF_ZP, e, 1, t);
+  tfsf_correct_face(s, XF_XN, e, 1, t);
+  tfsf_correct_face(s, XF_XP, e, 1, t);
+}
+
+/* ----------------------------------------------------------------- NF2FF */
+
+/* interpolateSurfaceCenter<xyz, field> (interpolate_scheme.cuh:11-134). */
+static double interp(const xf_scene* s, int face_axis, int is_e, int comp,
+                     int64_t i, int64_t j, int64_t k) {
+  const xf_array* f = is_e ? efield(s, comp) : hfield(s, comp);
+#define F(ii, jj, kk) (*at3(f, (ii), (jj), (kk)))
+  if (face_axis == 0) {
+    if (!is_e && comp == 2)
+      return 0.25 * (F(i, j, k + 1) + F(i, j, k) + F(i - 1, j, k + 1) + F(i - 1, j, k));
+    if (!is_e && comp == 1)
+      return 0.25 * (F(i, j + 1, k) + F(i, j, k) + F(i - 1, j + 1, k) + F(i - 1, j, k));
+    if (is_e && comp == 2) return 0.5 * (F(i, j + 1, k) + F(i, j, k));
+    if (is_e && comp == 1) return 0.5 * (F(i, j, k + 1) + F(i, j, k));
+  } else if (face_axis == 1) {
+    if (!is_e && comp == 2)
+      return 0.25 * (F(i, j, k + 1) + F(i, j, k) + F(i, j - 1, k + 1) + +F(i, j - 1, k));
+    if (!is_e && comp == 0)
+      return 0.25 * (F(i + 1, j, k) + F(i, j, k) + F(i + 1, j - 1, k) + F(i, j - 1, k));
+    if (is_e && comp == 2) return 0.5 * (F(i + 1, j, k) + F(i, j, k));
+    if (is_e && comp == 0) return 0.5 * (F(i, j, k + 1) + F(i, j, k));
+  } else {
+    if (!is_e && comp == 0)
+      return 0.25 * (F(i + 1, j, k) + F(i, j, k) + F(i, j, k - 1) + F(i + 1, j, k - 1));
+    if (!is_e && comp == 1)
+      return 0.25 * (F(i, j, k) + F(i, j + 1, k) + F(i, j, k - 1) + F(i, j + 1, k - 1));
+    if (is_e && comp == 0) return 0.5 * (F(i, j + 1, k) + F(i, j, k));
+    if (is_e && comp == 1) return 0.5 * (F(i + 1, j, k) + F(i, j, k));
+  }
+#undef F
+  return 0.0;
+}
+
+/* NF2FFFrequencyDomainData<D>::calculateJ / calculateM
+ * (nf2ff_frequency_domain.data.cu:80-194). */
+static void nf2ff_face(const xf_scene* s, const xf_nf2ff* nf,
+                       const xf_nf2ff_freq* fr, int face, int64_t t) {
+  const xf_task* task = &nf->face_task[face];
+  if (!(task->x.start < task->x.end && task->y.start < task->y.end &&
+        task->z.start < task->z.end))
+    return;
+  const int xyz = face / 2;
+  const int positive = face % 2;
+  const int a_axis = (xyz + 1) % 3, b_axis = (xyz + 2) % 3;
+  const double coff_ja = positive ? -1.0 : 1.0;
+  const double coff_jb = positive ? 1.0 : -1.0;
+  const double coff_ma = positive ? 1.0 : -1.0;
+  const double coff_mb = positive ? -1.0 : 1.0;
+  const double* th = fr->transform_h.data + 2 * t;
+  const double* te = fr->transform_e.data + 2 * t;
+  const xf_array* ja = &fr->cur[face][0];
+  const xf_array* jb = &fr->cur[face][1];
+  const xf_array* ma = &fr->cur[face][2];
+  const xf_array* mb = &fr->cur[face][3];
+  const int64_t oi = task->x.start, oj = task->y.start, ok = task->z.start;
+  for (int64_t i = task->x.start; i < task->x.end; ++i) {
+    for (int64_t j = task->y.start; j < task->y.end; ++j) {
+      for (int64_t k = task->z.start; k < task->z.end; ++k) {
+        double r;
+        double* p;
+        r = coff_ja * interp(s, xyz, 0, b_axis, i, j, k);
+        p = atc3(ja, i - oi, j - oj, k - ok);
+        p[0] += r * th[0];
+        p[1] += r * th[1];
+        r = coff_jb * interp(s, xyz, 0, a_axis, i, j, k);
+        p = atc3(jb, i - oi, j - oj, k - ok);
+        p[0] += r * th[0];
+        p[1] += r * th[1];
+      }
+    }
+  }
+  for (int64_t i = task->x.start; i < task->x.end; ++i) {
+    for (int64_t j = task->y.start; j < task->y.end; ++j) {
+      for (int64_t k = task->z.start; k < task->z.end; ++k) {
+        double r;
+        double* p;
+        r = coff_ma * interp(s, xyz, 1, b_axis, i, j, k);
+        p = atc3(ma, i - oi, j - oj, k - ok);
+        p[0] += r * te[0];
+        p[1] += r * te[1];
+        r = coff_mb * interp(s, xyz, 1, a_axis, i, j, k);
+        p = atc3(mb, i - oi, j - oj, k - ok);
+        p[0] += r * te[0];
+        p[1] += r * te[1];
+      }
+    }
+  }
+}
+
+/* --------------------------------------------------------------- the loop */
+
+static char g_err[512];
+
+const char* xfo_last_error(void) { return g_err; }
+
+int xfo_run(xf_scene* s, int64_t n_steps) {
+  g_err[0] = 0;
+  if (!s || s->abi_version != XF_ABI_VERSION) {
+    snprintf(g_err, sizeof g_err, "bad scene");
+    return 1;
+  }
+  for (int64_t n = 0; n < n_steps; ++n) {
+    const int64_t t = s->current_step;
+    /* updateH: BasicUpdator::updateH (basic_updator.cuh:16-32) */
+    yee_update(s, 0, 0);
+    yee_update(s, 0, 1);
+    yee_update(s, 0, 2);
+    /* correctH (domain_hd.cu:51-55) */
+    if (s->tfsf) tfsf_correct(s, 0, t);
+    for (int32_t p = 0; p < s->n_pml; ++p) pml_correct(s, &s->pml[p], 0);
+    /* updateE */
+    if (s->ade && s->ade->kind != XF_ADE_NONE) {
+      ade_update_e(s, 0);
+      ade_update_e(s, 1);
+      ade_update_e(s, 2);
+    } else if (s->dimension == 2) {
+      yee_update(s, 1, 2); /* BasicUpdatorTE::updateE */
+    } else {
+      yee_update(s, 1, 0);
+      yee_update(s, 1, 1);
+      yee_update(s, 1, 2);
+    }
+    /* correctE */
+    if (s->tfsf) tfsf_correct(s, 1, t);
+    for (int32_t p = 0; p < s->n_pml; ++p) pml_correct(s, &s->pml[p], 1);
+    /* record (domain_hd.cu:65-77) */
+    for (int32_t q = 0; q < s->n_nf2ff; ++q) {
+      const xf_nf2ff* nf = &s->nf2ff[q];
+      for (int32_t f = 0; f < nf->n_freq; ++f)
+        for (int face = 0; face < 6; ++face)
+          nf2ff_face(s, nf, &nf->freq[f], face, t);
+    }
+    /* nextStep */
+    s->current_step += 1;
+  }
+  return 0;
+}
+
+int xfo_num_threads(void) {
+#ifdef _OPENMP
+  extern int omp_get_max_threads(void);
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void xfo_set_num_threads(int n) {
+#ifdef _OPENMP
+  extern void omp_set_num_threads(int);
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+/* Brute-force form of one Yee component update, written as the reference's
+ * triple-loop update (update_scheme.cuh:65-130) instead of the flat
+ * linear-index loop; tests check that both forms agree. */
+int xfo_yee_component_triple_loop(xf_scene* s, int is_e, int xyz) {
+  const int a_axis = (xyz + 1) % 3, b_axis = (xyz + 2) % 3;
+  const xf_array* f = is_e ? efield(s, xyz) : hfield(s, xyz);
+  const xf_array* fa = is_e ? hfield(s, a_axis) : efield(s, a_axis);
+  const xf_array* fb = is_e ? hfield(s, b_axis) : efield(s, b_axis);
+  const int base = (is_e ? 0 : 9) + 3 * xyz;
+  int64_t is = 0, js = 0, ks = 0;
+  /* The triple-loop form has no a==0||b==0 guard; start E loops at 1 on the
+   * two tangential axes instead. */
+  if (is_e) {
+    if (a_axis == 0 || b_axis == 0) is = 1;
+    if (a_axis == 1 || b_axis == 1) js = 1;
+    if (a_axis == 2 || b_axis == 2) ks = 1;
+  }
+  for (int64_t i = is; i < s->nx; ++i)
+    for (int64_t j = js; j < s->ny; ++j)
+      for (int64_t k = ks; k < s->nz; ++k) {
+        int64_t di[3] = {0, 0, 0};
+        const int64_t off = is_e ? -1 : 1;
+        /* neighbour of field_a along axis b, of field_b along axis a */
+        di[b_axis] = off;
+        const double fa_q = *at3(fa, i + di[0], j + di[1], k + di[2]);
+        di[b_axis] = 0;
+        di[a_axis] = off;
+        const double fb_q = *at3(fb, i + di[0], j + di[1], k + di[2]);
+        const double cf = coef(s, base, i, j, k);
+        const double ca = coef(s, base + 1, i, j, k);
+        const double cb = coef(s, base + 2, i, j, k);
+        const double fv = *at3(f, i, j, k);
+        if (is_e)
+          *at3(f, i, j, k) = e_next(cf, fv, ca, *at3(fa, i, j, k), fa_q, cb,
+                                    *at3(fb, i, j, k), fb_q);
+        else
+          *at3(f, i, j, k) = h_next(cf, fv, ca, *at3(fa, i, j, k), fa_q, cb,
+                                    *at3(fb, i, j, k), fb_q);
+      }
+  return 0;
+}

@@ -1,0 +1,240 @@
+"""GPU <-> CPU-oracle parity of the HIP time-stepping path (FP64).
+
+Each case builds the same scene twice with the host set-up, advances one
+copy with the HIP engine through the C ABI (or the SimulationHD drop-in) and
+the other with the oracle (oracle/xf_oracle.c, the reference kernels
+restated), then compares every state array the reference writes back:
+fields, CPML psi, NF2FF currents, ADE J / E_prev.  The FP64 bar is
+rel-L2 <= 1e-12 (the engine and oracle share evaluation order and neither
+contracts to FMA, so in practice the arrays are bit-identical; the test also
+reports max |diff|).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import xfdtd_cuda_amd as xf
+from xfdtd_cuda_amd import scenes
+from conftest import FIELDS, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _state_names(sim):
+    names = list(FIELDS)
+    for p in range(6):
+        try:
+            sim.array(f"pml.{p}.ea_psi_hb")
+        except xf.XFError:
+            break
+        names += [f"pml.{p}.{m}" for m in ("ea_psi_hb", "eb_psi_ha", "ha_psi_eb", "hb_psi_ea")]
+    for q in range(4):
+        for f in range(4):
+            try:
+                sim.array(f"nf2ff.{q}.{f}.transform_e")
+            except xf.XFError:
+                break
+            for face in xf.FACES:
+                for c in ("ja", "jb", "ma", "mb"):
+                    try:
+                        sim.array(f"nf2ff.{q}.{f}.{face}.{c}")
+                    except xf.XFError:
+                        continue
+                    names.append(f"nf2ff.{q}.{f}.{face}.{c}")
+    return names
+
+
+def _compare(gpu_sim, cpu_sim, names=None, tol=TOL, label=""):
+    names = names or _state_names(cpu_sim)
+    worst = 0.0
+    for n in names:
+        a, b = gpu_sim.array(n), cpu_sim.array(n)
+        if a.size == 0:
+            continue
+        err = rel_l2(a, b)
+        worst = max(worst, err)
+        assert err <= tol, f"{label} {n}: rel-L2 {err:.3e} > {tol:.1e} (max|diff| {np.abs(a - b).max():.3e})"
+    return worst
+
+
+def _run_engine(sim, steps, **kw):
+    eng = xf.Engine(sim, **kw).upload()
+    eng.run(steps).download()
+    eng.close()
+
+
+def _pair(build, steps, run_gpu=_run_engine, **kw):
+    g, c = build(), build()
+    run_gpu(g, steps, **kw)
+    oracle.run(c.scene_ptr(), steps)
+    return g, c
+
+
+def test_sphere_pml_tfsf_nf2ff(gpu_available):
+    build = lambda: scenes.dielectric_sphere(n=24, pml=6, steps=140, tfsf_inset=9, nf2ff_inset=7)
+    g, c = _pair(build, 140)
+    _compare(g, c, label="sphere")
+    # the run must have produced non-trivial state everywhere we compare
+    assert np.abs(c.array("ex")).max() > 0.1
+    assert np.abs(c.array("pml.0.ea_psi_hb")).max() > 0
+    assert np.abs(c.nf2ff_current(0, 0, "ZP", "ja")).max() > 0
+    for n in FIELDS:
+        assert np.array_equal(g.array(n), c.array(n)), f"{n} not bit-identical"
+
+
+def test_sphere_uneven_sizes_and_steps_in_chunks(gpu_available):
+    """Non-cubic grid with odd extents, run as 3 separate xf_run calls."""
+    def build():
+        s = xf.Simulation(3e-3, 3e-3, 3e-3, 0.95)
+        s.add_cube("domain", (-0.05, -0.04, -0.062), (0.1, 0.08, 0.124), xf.Material.air())
+        s.add_sphere("s", (0.004, -0.003, 0.01), 0.021, xf.Material.dielectric(4.0, 1.5, 0.01))
+        s.add_pml(5, "XN").add_pml(7, "XP").add_pml(4, "YN").add_pml(6, "YP")
+        s.add_pml(5, "ZN").add_pml(5, "ZP")
+        tau = 20 * 3e-3 / 6e8
+        s.add_tfsf3d((9, 10, 11), 0.7, 0.4, 0.3, xf.Waveform.gaussian(tau, 4.5 * tau))
+        s.add_nf2ff_fd((7, 7, 8), [1e9, 2.5e9])
+        s.init(150)
+        return s
+
+    def run3(sim, steps):
+        eng = xf.Engine(sim).upload()
+        eng.run(40).run(60).run(steps - 100).download()
+        eng.close()
+
+    g, c = _pair(build, 150, run_gpu=run3)
+    _compare(g, c, label="uneven")
+
+
+def test_oblique_incidence_all_quadrants(gpu_available):
+    """TFSF with incident directions in every octant sign pattern."""
+    for theta, phi in ((2.3, 4.0), (0.4, 2.2), (1.9, 5.5)):
+        def build(theta=theta, phi=phi):
+            s = xf.Simulation(3e-3, 3e-3, 3e-3, 0.9)
+            s.add_cube("domain", (-0.045,) * 3, (0.09,) * 3, xf.Material.air())
+            s.add_pml_all(5)
+            tau = 20 * 3e-3 / 6e8
+            s.add_tfsf3d((8, 8, 8), theta, phi, 0.5, xf.Waveform.gaussian(tau, 4.5 * tau))
+            s.init(100)
+            return s
+        g, c = _pair(build, 100)
+        _compare(g, c, label=f"oblique {theta},{phi}")
+
+
+def test_compact_layout_matches_dense(gpu_available):
+    dense = scenes.dielectric_sphere(n=20, pml=5, steps=100, tfsf_inset=8, nf2ff_inset=6)
+    comp = scenes.dielectric_sphere(n=20, pml=5, steps=100, tfsf_inset=8, nf2ff_inset=6,
+                                    layout="compact")
+    cpu = scenes.dielectric_sphere(n=20, pml=5, steps=100, tfsf_inset=8, nf2ff_inset=6,
+                                   layout="compact")
+    _run_engine(dense, 100)
+    _run_engine(comp, 100)
+    oracle.run(cpu.scene_ptr(), 100)
+    for n in FIELDS:
+        assert np.array_equal(dense.array(n), comp.array(n)), n
+    _compare(comp, cpu, label="compact")
+
+
+def test_graph_replay_matches_eager(gpu_available):
+    build = lambda: scenes.dielectric_sphere(n=20, pml=5, steps=90, tfsf_inset=8, nf2ff_inset=6)
+    a, b = build(), build()
+    ea = xf.Engine(a, use_graph=True).upload()
+    # several graph replays with eager work and syncs between them
+    ea.run(30).run(30).run(30).download()
+    ea.close()
+    _run_engine(b, 90)
+    for n in _state_names(b):
+        assert np.array_equal(a.array(n), b.array(n)), n
+
+
+def test_simulation_hd_dropin(gpu_available):
+    """The reference's calling sequence: SimulationHD(&s); setGridDim;
+    setBlockDim; run(n) -- through the C++ class."""
+    build = lambda: scenes.dielectric_sphere(n=22, pml=6, steps=110, tfsf_inset=9, nf2ff_inset=7,
+                                             init=False)
+    g, c = build(), build()
+    hd = xf.SimulationHD(g)
+    hd.setGridDim((4, 4, 4))
+    hd.setBlockDim((8, 8, 8))
+    hd.run(110)
+    c.init(110)
+    oracle.run(c.scene_ptr(), 110)
+    _compare(g, c, label="SimulationHD")
+
+
+def test_cylinder_2d(gpu_available):
+    build = lambda: scenes.cylinder_2d(n=80, pml=8, steps=200, tfsf_inset=14, nf2ff_inset=11)
+    g, c = _pair(build, 200)
+    _compare(g, c, label="2d")
+    assert np.abs(c.array("ez")).max() > 0.1
+    # Ex, Ey, Hz stay zero in the TE updator
+    for n in ("ex", "ey", "hz"):
+        assert np.abs(g.array(n)).max() == 0
+
+
+def _dispersive(kind):
+    def build():
+        dl = 3e-3
+        s = xf.Simulation(dl, dl, dl, 0.9)
+        s.add_cube("domain", (-0.042,) * 3, (0.084,) * 3, xf.Material.air())
+        tp = 2 * np.pi
+        if kind == "drude":
+            m = xf.Material.drude(1.5, [tp * 2e9], [tp * 0.2e9])
+        elif kind == "debye":
+            m = xf.Material.debye(2.0, [3.0, 1.0], [1e-10, 3e-11])
+        else:
+            m = xf.Material.lorentz(1.2, [2.0], [tp * 1.5e9], [tp * 0.1e9])
+        s.add_sphere("shell", (0, 0, 0), 0.02, m)
+        if kind == "mix":
+            s.add_sphere("core", (0, 0, 0), 0.01, xf.Material.drude(1.0, [tp * 2e9], [tp * 0.2e9]))
+        s.add_pml_all(5)
+        tau = 20 * dl / 6e8
+        s.add_tfsf3d((8, 8, 8), 0.0, 0.0, 0.0, xf.Waveform.gaussian(tau, 4.5 * tau))
+        s.add_nf2ff_fd((6, 6, 6), [1e9])
+        s.init(160)
+        return s
+    return build
+
+
+@pytest.mark.parametrize("kind", ["drude", "debye", "lorentz", "mix"])
+def test_ade(gpu_available, kind):
+    build = _dispersive(kind)
+    g, c = _pair(build, 160)
+    _compare(g, c, label=kind)
+    names = ["ade.jx", "ade.jy", "ade.jz"]
+    if kind in ("lorentz", "mix"):
+        names += ["ade.jx_prev", "ade.jz_prev"]
+    for n in names:
+        # E_prev and J of non-dispersive cells are not tracked on the device
+        # (multiplied by zero coefficients); J stays exactly zero there.
+        assert rel_l2(g.array(n), c.array(n)) <= TOL, n
+    assert np.abs(c.array("ade.jx")).max() > 0
+
+
+def test_fp32_close_to_fp64(gpu_available):
+    build = lambda: scenes.dielectric_sphere(n=24, pml=6, steps=140, tfsf_inset=9, nf2ff_inset=7)
+    g, c = _pair(build, 140, precision="fp32")
+    for n in ("ex", "ey", "ez", "hy"):
+        assert rel_l2(g.array(n), c.array(n)) < 1e-4, n
+    assert rel_l2(g.array("nf2ff.0.0.XP.jb"), c.array("nf2ff.0.0.XP.jb")) < 1e-4
+
+
+def test_bf16_storage_runs(gpu_available):
+    build = lambda: scenes.dielectric_sphere(n=24, pml=6, steps=140, tfsf_inset=9, nf2ff_inset=7)
+    g, c = _pair(build, 140, precision="bf16")
+    err = rel_l2(g.array("ex"), c.array("ex"))
+    assert np.isfinite(err) and err < 0.2
+
+
+def test_config0_geometry(gpu_available):
+    """The 156^3 dielectric-sphere driver scene (140^3 + 8-cell CPML), first
+    150 of its 2400 steps (the oracle finishes these in seconds)."""
+    build = lambda: scenes.dielectric_sphere(steps=2400)
+    g, c = build(), build()
+    eng = xf.Engine(g).upload()
+    eng.run(150).download()
+    eng.close()
+    oracle.run(c.scene_ptr(), 150)
+    _compare(g, c, names=list(FIELDS) + ["pml.0.ea_psi_hb", "pml.5.hb_psi_ea",
+                                         "nf2ff.0.0.ZN.ja", "nf2ff.0.0.XP.mb"], label="cfg0")

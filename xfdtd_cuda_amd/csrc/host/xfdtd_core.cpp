@@ -1,0 +1,1092 @@
+// xfdtd_core.cpp -- host-side set-up of the FDTD scene (the XFDTD_CORE subset
+// that xfdtd::cuda::SimulationHD consumes).  See include/xfdtd_cuda_amd/
+// xfdtd_core.h for the documented numerical choices.
+#include "../../../include/xfdtd_cuda_amd/xfdtd_core.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+namespace xfdtd {
+
+using constant::C_0;
+using constant::EPSILON_0;
+using constant::ETA_0;
+using constant::MU_0;
+using constant::PI;
+
+// ------------------------------------------------------------------ Array
+Array::Array(std::vector<int64_t> shape, Real fill) : _shape(std::move(shape)) {
+  std::size_t n = 1;
+  for (auto s : _shape) {
+    if (s < 0) throw XFDTDException("Array: negative extent");
+    n *= static_cast<std::size_t>(s);
+  }
+  _data.assign(n, fill);
+}
+
+xf_array Array::view() {
+  xf_array a{};
+  a.data = _data.empty() ? nullptr : _data.data();
+  a.ndim = static_cast<int32_t>(_shape.size());
+  for (std::size_t d = 0; d < _shape.size() && d < 5; ++d) a.shape[d] = _shape[d];
+  return a;
+}
+
+void Array::fill(Real v) { std::fill(_data.begin(), _data.end(), v); }
+
+// ----------------------------------------------------------------- shapes
+bool Cube::isInside(const Vector& p) const {
+  auto in = [](Real v, Real o, Real s) {
+    if (std::isinf(s)) return true;
+    return v >= o && v <= o + s;
+  };
+  return in(p.x, _origin.x, _size.x) && in(p.y, _origin.y, _size.y) &&
+         in(p.z, _origin.z, _size.z);
+}
+
+Vector Cube::upper() const {
+  return {_origin.x + _size.x, _origin.y + _size.y, _origin.z + _size.z};
+}
+
+bool Sphere::isInside(const Vector& p) const {
+  const Real dx = p.x - _center.x, dy = p.y - _center.y, dz = p.z - _center.z;
+  return dx * dx + dy * dy + dz * dz <= _radius * _radius;
+}
+Vector Sphere::lower() const {
+  return {_center.x - _radius, _center.y - _radius, _center.z - _radius};
+}
+Vector Sphere::upper() const {
+  return {_center.x + _radius, _center.y + _radius, _center.z + _radius};
+}
+
+bool Cylinder::isInside(const Vector& p) const {
+  const Real dx = p.x - _center.x, dy = p.y - _center.y;
+  if (dx * dx + dy * dy > _radius * _radius) return false;
+  if (std::isinf(_height)) return true;
+  return p.z >= _center.z && p.z <= _center.z + _height;
+}
+Vector Cylinder::lower() const {
+  return {_center.x - _radius, _center.y - _radius, _center.z};
+}
+Vector Cylinder::upper() const {
+  return {_center.x + _radius, _center.y + _radius, _center.z + _height};
+}
+
+// --------------------------------------------------------------- material
+std::unique_ptr<Material> Material::createAir() {
+  return std::make_unique<Material>("air", ElectroMagneticProperty{1, 1, 0, 0});
+}
+
+// A perfect conductor as a very large electric conductivity, the form used
+// by the reference's standalone 2-D solver (test/2d.cu:627-646).
+std::unique_ptr<Material> Material::createPec() {
+  return std::make_unique<Material>("pec",
+                                    ElectroMagneticProperty{1, 1, 1e10, 0});
+}
+
+std::unique_ptr<Material> Material::makeDrudeMedium(std::string name,
+                                                    Real eps_inf,
+                                                    std::vector<Real> omega_p,
+                                                    std::vector<Real> gamma) {
+  if (omega_p.size() != gamma.size() || omega_p.empty())
+    throw XFDTDException("makeDrudeMedium: pole arrays differ");
+  auto m = std::make_unique<Material>(std::move(name),
+                                      ElectroMagneticProperty{eps_inf, 1, 0, 0});
+  m->_dispersion = Dispersion::Drude;
+  m->_p0 = std::move(omega_p);
+  m->_p1 = std::move(gamma);
+  return m;
+}
+
+std::unique_ptr<Material> Material::makeDebyeMedium(std::string name,
+                                                    Real eps_inf,
+                                                    std::vector<Real> de,
+                                                    std::vector<Real> tau) {
+  if (de.size() != tau.size() || de.empty())
+    throw XFDTDException("makeDebyeMedium: pole arrays differ");
+  auto m = std::make_unique<Material>(std::move(name),
+                                      ElectroMagneticProperty{eps_inf, 1, 0, 0});
+  m->_dispersion = Dispersion::Debye;
+  m->_p0 = std::move(de);
+  m->_p1 = std::move(tau);
+  return m;
+}
+
+std::unique_ptr<Material> Material::makeLorentzMedium(
+    std::string name, Real eps_inf, std::vector<Real> de,
+    std::vector<Real> omega_0, std::vector<Real> delta) {
+  if (de.size() != omega_0.size() || de.size() != delta.size() || de.empty())
+    throw XFDTDException("makeLorentzMedium: pole arrays differ");
+  std::vector<MLorentzPole> p;
+  for (std::size_t i = 0; i < de.size(); ++i) {
+    const Real w2 = omega_0[i] * omega_0[i];
+    p.push_back({de[i] * w2, 0.0, w2, 2.0 * delta[i], 1.0});
+  }
+  return makeMLorentz(std::move(name), eps_inf, std::move(p));
+}
+
+std::unique_ptr<Material> Material::makeMLorentz(std::string name,
+                                                 Real eps_inf,
+                                                 std::vector<MLorentzPole> p) {
+  if (p.empty()) throw XFDTDException("makeMLorentz: no pole");
+  auto m = std::make_unique<Material>(std::move(name),
+                                      ElectroMagneticProperty{eps_inf, 1, 0, 0});
+  m->_dispersion = Dispersion::MLorentz;
+  m->_mlor = std::move(p);
+  return m;
+}
+
+std::vector<MLorentzPole> Material::asMLorentz() const {
+  std::vector<MLorentzPole> out;
+  switch (_dispersion) {
+    case Dispersion::Drude:
+      for (std::size_t i = 0; i < _p0.size(); ++i)
+        out.push_back({_p0[i] * _p0[i], 0.0, 0.0, _p1[i], 1.0});
+      break;
+    case Dispersion::Debye:
+      for (std::size_t i = 0; i < _p0.size(); ++i)
+        out.push_back({_p0[i], 0.0, 1.0, _p1[i], 0.0});
+      break;
+    case Dispersion::MLorentz:
+      out = _mlor;
+      break;
+    default:
+      break;
+  }
+  return out;
+}
+
+std::complex<Real> Material::relativePermittivity(Real w) const {
+  using C = std::complex<Real>;
+  const C jw(0.0, w);
+  C eps(_emp.epsilon_r, 0.0);
+  eps += _emp.sigma_e / (jw * EPSILON_0);
+  switch (_dispersion) {
+    case Dispersion::Drude:
+      for (std::size_t i = 0; i < _p0.size(); ++i)
+        eps -= _p0[i] * _p0[i] / (w * w - jw * _p1[i]);
+      break;
+    case Dispersion::Debye:
+      for (std::size_t i = 0; i < _p0.size(); ++i)
+        eps += _p0[i] / (1.0 + jw * _p1[i]);
+      break;
+    case Dispersion::MLorentz:
+      for (const auto& p : _mlor)
+        eps += (p.a0 + p.a1 * jw) / (p.b0 + p.b1 * jw + p.b2 * jw * jw);
+      break;
+    default:
+      break;
+  }
+  return eps;
+}
+
+// --------------------------------------------------------------- waveform
+std::unique_ptr<Waveform> Waveform::gaussian(Real tau, Real t0) {
+  auto w = std::make_unique<Waveform>();
+  w->_kind = Kind::Gaussian;
+  w->_tau = tau;
+  w->_t0 = t0;
+  return w;
+}
+
+std::unique_ptr<Waveform> Waveform::cosineModulatedGaussian(Real tau, Real t0,
+                                                            Real f) {
+  auto w = std::make_unique<Waveform>();
+  w->_kind = Kind::CosineModulatedGaussian;
+  w->_tau = tau;
+  w->_t0 = t0;
+  w->_f = f;
+  return w;
+}
+
+std::unique_ptr<Waveform> Waveform::sine(Real f) {
+  auto w = std::make_unique<Waveform>();
+  w->_kind = Kind::Sine;
+  w->_f = f;
+  return w;
+}
+
+Real Waveform::operator()(Real t) const {
+  switch (_kind) {
+    case Kind::Gaussian: {
+      const Real u = (t - _t0) / _tau;
+      return std::exp(-u * u);
+    }
+    case Kind::CosineModulatedGaussian: {
+      const Real u = (t - _t0) / _tau;
+      return std::exp(-u * u) * std::cos(2 * PI * _f * (t - _t0));
+    }
+    case Kind::Sine:
+      return std::sin(2 * PI * _f * t);
+  }
+  return 0;
+}
+
+void Waveform::init(Real dt, Index nt) {
+  _time.resize(nt);
+  _value.resize(nt);
+  for (Index n = 0; n < nt; ++n) {
+    _time[n] = dt * static_cast<Real>(n);
+    _value[n] = (*this)(_time[n]);
+  }
+}
+
+// -------------------------------------------------------------------- PML
+PML::PML(Index thickness, Axis::Direction direction, Index order,
+         Real sigma_ratio, Real alpha_min, Real alpha_max, Real kappa_max)
+    : _thickness(thickness),
+      _direction(direction),
+      _order(order),
+      _sigma_ratio(sigma_ratio),
+      _alpha_min(alpha_min),
+      _alpha_max(alpha_max),
+      _kappa_max(kappa_max) {
+  if (thickness == 0) throw XFDTDException("PML: zero thickness");
+  if (kappa_max < 1.0) throw XFDTDException("PML: kappa_max < 1");
+}
+
+Axis::XYZ PML::mainAxis() const {
+  return static_cast<Axis::XYZ>(static_cast<int>(_direction) / 2);
+}
+
+// ------------------------------------------------------------------- TFSF
+TFSF::TFSF(Index start_x, Index start_y, Index start_z, Real theta, Real phi,
+           Real psi, std::unique_ptr<Waveform> waveform, int dimension)
+    : _sx(start_x),
+      _sy(start_y),
+      _sz(start_z),
+      _theta(theta),
+      _phi(phi),
+      _psi(psi),
+      _waveform(std::move(waveform)),
+      _dimension(dimension) {
+  if (!_waveform) throw XFDTDException("TFSF: no waveform");
+}
+
+// ------------------------------------------------------------------ NF2FF
+NFFFTFrequencyDomain::NFFFTFrequencyDomain(Index sx, Index sy, Index sz,
+                                           std::vector<Real> frequencies)
+    : _sx(sx), _sy(sy), _sz(sz), _freqs(std::move(frequencies)) {
+  if (_freqs.empty()) throw XFDTDException("NFFFTFrequencyDomain: no frequency");
+}
+
+NFFFTFrequencyDomain::FarField NFFFTFrequencyDomain::processFarField(
+    Index freq_index, const std::vector<Real>& theta, Real phi,
+    std::complex<Real> incident) const {
+  using C = std::complex<Real>;
+  if (!_grid) throw XFDTDException("processFarField: simulation not initialised");
+  if (freq_index >= _freqs.size()) throw XFDTDException("processFarField: bad frequency index");
+  const Real f = _freqs[freq_index];
+  const Real k0 = 2 * PI * f / C_0;
+  const GridSpace& g = *_grid;
+  FarField out;
+  out.e_theta.resize(theta.size());
+  out.e_phi.resize(theta.size());
+  out.rcs.resize(theta.size());
+  const Real dx = g.basedDx(), dy = g.basedDy(), dz = g.basedDz();
+  for (std::size_t n = 0; n < theta.size(); ++n) {
+    const Real th = theta[n];
+    const Real rx = std::sin(th) * std::cos(phi), ry = std::sin(th) * std::sin(phi),
+               rz = std::cos(th);
+    C nv[3] = {0, 0, 0}, lv[3] = {0, 0, 0};
+    for (int face = 0; face < 6; ++face) {
+      const xf_task& t = _desc.face_task[face];
+      if (!(t.x.start < t.x.end && t.y.start < t.y.end && t.z.start < t.z.end)) continue;
+      const int axis = face / 2;
+      const int a = (axis + 1) % 3, b = (axis + 2) % 3;
+      const Real ds = axis == 0 ? dy * dz : axis == 1 ? dz * dx : dx * dy;
+      const Array* cur = &_cur[(freq_index * 6 + face) * 4];
+      const int64_t sy = t.y.end - t.y.start, sz = t.z.end - t.z.start;
+      for (int64_t i = t.x.start; i < t.x.end; ++i)
+        for (int64_t j = t.y.start; j < t.y.end; ++j)
+          for (int64_t k = t.z.start; k < t.z.end; ++k) {
+            // face centre: on the face plane, half a cell along a and b
+            Real p[3] = {g.nodeX(i), g.nodeY(j), g.nodeZ(k)};
+            const Real d[3] = {dx, dy, dz};
+            p[a] += 0.5 * d[a];
+            p[b] += 0.5 * d[b];
+            if (g.dimension() == GridSpace::Dimension::TWO) p[2] = 0.0;
+            const C phase = std::exp(C(0.0, k0 * (rx * p[0] + ry * p[1] + rz * p[2]))) * ds;
+            const int64_t lin = (((i - t.x.start) * sy + (j - t.y.start)) * sz + (k - t.z.start)) * 2;
+            const C ja(cur[0][lin], cur[0][lin + 1]);
+            const C jb(cur[1][lin], cur[1][lin + 1]);
+            const C ma(cur[2][lin], cur[2][lin + 1]);
+            const C mb(cur[3][lin], cur[3][lin + 1]);
+            nv[a] += ja * phase;
+            nv[b] += jb * phase;
+            lv[a] += ma * phase;
+            lv[b] += mb * phase;
+          }
+    }
+    const Real ct = std::cos(th), st = std::sin(th), cp = std::cos(phi), sp = std::sin(phi);
+    const C n_th = nv[0] * ct * cp + nv[1] * ct * sp - nv[2] * st;
+    const C n_ph = -nv[0] * sp + nv[1] * cp;
+    const C l_th = lv[0] * ct * cp + lv[1] * ct * sp - lv[2] * st;
+    const C l_ph = -lv[0] * sp + lv[1] * cp;
+    const C pre = C(0.0, -k0 / (4 * PI));
+    C e_th = pre * (l_ph + ETA_0 * n_th);
+    C e_ph = -pre * (l_th - ETA_0 * n_ph);
+    if (std::abs(incident) > 0) {
+      e_th /= incident;
+      e_ph /= incident;
+    }
+    out.e_theta[n] = e_th;
+    out.e_phi[n] = e_ph;
+    out.rcs[n] = 4 * PI * (std::norm(e_th) + std::norm(e_ph));
+  }
+  return out;
+}
+
+// -------------------------------------------------------------- Simulation
+Simulation::Simulation(Real dx, Real dy, Real dz, Real cfl)
+    : _dx(dx), _dy(dy), _dz(dz), _cfl(cfl) {
+  if (!(dx > 0 && dy > 0 && dz > 0)) throw XFDTDException("Simulation: cell size <= 0");
+  if (!(cfl > 0 && cfl <= 1)) throw XFDTDException("Simulation: cfl not in (0, 1]");
+}
+
+namespace {
+Index pmlThickness(const std::vector<std::shared_ptr<Boundary>>& bs,
+                   Axis::Direction d) {
+  Index t = 0;
+  for (auto& b : bs) {
+    auto p = dynamic_cast<PML*>(b.get());
+    if (p && p->direction() == d) t = std::max(t, p->thickness());
+  }
+  return t;
+}
+}  // namespace
+
+void Simulation::buildGrid() {
+  if (_objects.empty()) throw XFDTDException("Simulation::init: no domain object");
+  const Shape& dom = _objects.front()->shape();
+  const Vector lo = dom.lower(), hi = dom.upper();
+  auto cells = [](Real lo, Real hi, Real d) -> Index {
+    const Real n = std::round((hi - lo) / d);
+    if (!(n >= 1)) throw XFDTDException("Simulation::init: domain smaller than a cell");
+    return static_cast<Index>(n);
+  };
+  _grid = std::make_shared<GridSpace>();
+  GridSpace& g = *_grid;
+  const bool two_d = std::isinf(lo.z) || std::isinf(hi.z);
+  g._dim = two_d ? GridSpace::Dimension::TWO : GridSpace::Dimension::THREE;
+  g._dx = _dx;
+  g._dy = _dy;
+  g._dz = _dz;
+  const Index lxn = pmlThickness(_boundaries, Axis::Direction::XN);
+  const Index lxp = pmlThickness(_boundaries, Axis::Direction::XP);
+  const Index lyn = pmlThickness(_boundaries, Axis::Direction::YN);
+  const Index lyp = pmlThickness(_boundaries, Axis::Direction::YP);
+  const Index lzn = pmlThickness(_boundaries, Axis::Direction::ZN);
+  const Index lzp = pmlThickness(_boundaries, Axis::Direction::ZP);
+  g._nx = cells(lo.x, hi.x, _dx) + lxn + lxp;
+  g._ny = cells(lo.y, hi.y, _dy) + lyn + lyp;
+  g._origin.x = lo.x - _dx * static_cast<Real>(lxn);
+  g._origin.y = lo.y - _dy * static_cast<Real>(lyn);
+  if (two_d) {
+    if (lzn || lzp) throw XFDTDException("Simulation::init: z PML on a 2-D grid");
+    g._nz = 1;
+    g._origin.z = -0.5 * _dz;
+  } else {
+    g._nz = cells(lo.z, hi.z, _dz) + lzn + lzp;
+    g._origin.z = lo.z - _dz * static_cast<Real>(lzn);
+  }
+  const Real inv2 = two_d ? 1 / (_dx * _dx) + 1 / (_dy * _dy)
+                          : 1 / (_dx * _dx) + 1 / (_dy * _dy) + 1 / (_dz * _dz);
+  _time._dt = _cfl / (C_0 * std::sqrt(inv2));
+}
+
+// Voxelisation (this project's choice): every field at index (i, j, k) takes
+// the material of cell (i, j, k), sampled at the cell centre; the last object
+// containing the centre wins; cells outside every object (the PML margin)
+// take the domain's material.  Indices beyond the last cell clamp to it.
+void Simulation::voxelize() {
+  const GridSpace& g = *_grid;
+  _mat_table.clear();
+  std::map<const Material*, uint16_t> idx;
+  std::vector<uint16_t> obj_mat;
+  for (auto& o : _objects) {
+    const Material* m = &o->material();
+    auto it = idx.find(m);
+    if (it == idx.end()) {
+      if (_mat_table.size() >= 65535) throw XFDTDException("too many materials");
+      it = idx.emplace(m, static_cast<uint16_t>(_mat_table.size())).first;
+      _mat_table.push_back(m);
+    }
+    obj_mat.push_back(it->second);
+  }
+  const Index nx = g.sizeX(), ny = g.sizeY(), nz = g.sizeZ();
+  _cell_mat.assign(nx * ny * nz, obj_mat[0]);
+  for (std::size_t o = 1; o < _objects.size(); ++o) {
+    const Shape& s = _objects[o]->shape();
+    const Vector lo = s.lower(), hi = s.upper();
+    auto range = [](Real lo, Real hi, Real org, Real d, Index n, Index& a, Index& b) {
+      Real fa = std::isinf(lo) ? 0 : std::floor((lo - org) / d - 1);
+      Real fb = std::isinf(hi) ? static_cast<Real>(n) : std::ceil((hi - org) / d + 1);
+      fa = std::max<Real>(fa, 0);
+      fb = std::min<Real>(fb, static_cast<Real>(n));
+      a = static_cast<Index>(fa);
+      b = static_cast<Index>(std::max(fa, fb));
+    };
+    Index i0, i1, j0, j1, k0, k1;
+    range(lo.x, hi.x, g.origin().x, _dx, nx, i0, i1);
+    range(lo.y, hi.y, g.origin().y, _dy, ny, j0, j1);
+    if (g.dimension() == GridSpace::Dimension::TWO) {
+      k0 = 0;
+      k1 = 1;
+    } else {
+      range(lo.z, hi.z, g.origin().z, _dz, nz, k0, k1);
+    }
+    const uint16_t m = obj_mat[o];
+#pragma omp parallel for schedule(static)
+    for (int64_t i = static_cast<int64_t>(i0); i < static_cast<int64_t>(i1); ++i)
+      for (Index j = j0; j < j1; ++j)
+        for (Index k = k0; k < k1; ++k) {
+          Vector p{g.nodeX(i) + 0.5 * _dx, g.nodeY(j) + 0.5 * _dy,
+                   g.dimension() == GridSpace::Dimension::TWO ? 0.0
+                                                              : g.nodeZ(k) + 0.5 * _dz};
+          if (s.isInside(p)) _cell_mat[(i * ny + j) * nz + k] = m;
+        }
+  }
+}
+
+namespace {
+
+// Per-material ADE data derived from the poles (see ADEMethodStorage).
+struct AdeCoeff {
+  std::vector<Real> cjj, cjjp, cjen, cje, cjep, cjsj, cjsjp;
+  Real cejs{0}, ceep{0};
+  Real ce{1}, ccurl{0};  // E self and 1/D for the curl terms
+};
+
+AdeCoeff adeCoefficients(const Material& m, int kind, Index np, Real dt) {
+  AdeCoeff c;
+  c.cjj.assign(np, 0);
+  c.cjjp.assign(np, 0);
+  c.cjen.assign(np, 0);
+  c.cje.assign(np, 0);
+  c.cjep.assign(np, 0);
+  c.cjsj.assign(np, 0);
+  c.cjsjp.assign(np, 0);
+  const Real eps = EPSILON_0 * m.emp().epsilon_r;
+  const Real sig = m.emp().sigma_e;
+  if (kind == XF_ADE_DRUDE) {
+    Real sum_b = 0;
+    for (Index p = 0; p < m.p0().size(); ++p) {
+      const Real wp = m.p0()[p], g = m.p1()[p];
+      c.cjj[p] = (1 - g * dt / 2) / (1 + g * dt / 2);
+      c.cje[p] = EPSILON_0 * wp * wp * dt / 2 / (1 + g * dt / 2);
+      c.cjsj[p] = 1 + c.cjj[p];
+      sum_b += c.cje[p];
+    }
+    const Real d = eps / dt + sig / 2 + sum_b / 2;
+    c.ce = (eps / dt - sig / 2 - sum_b / 2) / d;
+    c.ccurl = 1 / d;
+    c.cejs = -1 / (2 * d);
+  } else if (kind == XF_ADE_DEBYE) {
+    Real sum_b = 0;
+    for (Index p = 0; p < m.p0().size(); ++p) {
+      const Real de = m.p0()[p], tau = m.p1()[p];
+      c.cjj[p] = (1 - dt / (2 * tau)) / (1 + dt / (2 * tau));
+      c.cje[p] = (EPSILON_0 * de / tau) / (1 + dt / (2 * tau));
+      c.cjsj[p] = 1 + c.cjj[p];
+      sum_b += c.cje[p];
+    }
+    const Real d = eps / dt + sig / 2 + sum_b / 2;
+    c.ce = (eps / dt - sig / 2 + sum_b / 2) / d;
+    c.ccurl = 1 / d;
+    c.cejs = -1 / (2 * d);
+  } else {
+    const auto poles = m.asMLorentz();
+    Real sum_en = 0, sum_e = 0, sum_ep = 0;
+    for (Index p = 0; p < poles.size(); ++p) {
+      const auto& q = poles[p];
+      const Real k = q.b1 / (2 * dt) + q.b2 / (dt * dt);
+      if (k == 0) throw XFDTDException("M-Lorentz pole with b1 = b2 = 0");
+      c.cjj[p] = (2 * q.b2 / (dt * dt) - q.b0) / k;
+      c.cjjp[p] = (q.b1 / (2 * dt) - q.b2 / (dt * dt)) / k;
+      c.cjen[p] = EPSILON_0 * (q.a0 / (2 * dt) + q.a1 / (dt * dt)) / k;
+      c.cje[p] = -2 * EPSILON_0 * q.a1 / (dt * dt * k);
+      c.cjep[p] = EPSILON_0 * (q.a1 / (dt * dt) - q.a0 / (2 * dt)) / k;
+      c.cjsj[p] = 1 + c.cjj[p];
+      c.cjsjp[p] = c.cjjp[p];
+      sum_en += c.cjen[p];
+      sum_e += c.cje[p];
+      sum_ep += c.cjep[p];
+    }
+    const Real d = eps / dt + sig / 2 + sum_en / 2;
+    c.ce = (eps / dt - sig / 2 - sum_e / 2) / d;
+    c.ccurl = 1 / d;
+    c.cejs = -1 / (2 * d);
+    c.ceep = -sum_ep / (2 * d);
+  }
+  return c;
+}
+
+int adeKindOf(const std::vector<const Material*>& mats) {
+  bool any = false, all_drude = true, all_debye = true;
+  for (auto m : mats) {
+    if (m->dispersion() == Material::Dispersion::None) continue;
+    any = true;
+    if (m->dispersion() != Material::Dispersion::Drude) all_drude = false;
+    if (m->dispersion() != Material::Dispersion::Debye) all_debye = false;
+  }
+  if (!any) return XF_ADE_NONE;
+  if (all_drude) return XF_ADE_DRUDE;
+  if (all_debye) return XF_ADE_DEBYE;
+  return XF_ADE_MLORENTZ;
+}
+
+Index polesOf(const Material& m, int kind) {
+  if (m.dispersion() == Material::Dispersion::None) return 0;
+  if (kind == XF_ADE_MLORENTZ) return m.asMLorentz().size();
+  return m.p0().size();
+}
+
+}  // namespace
+
+// Yee coefficients per material, in the form of test/2d.cu:694-735:
+//   ce = (2 eps - sigma dt)/(2 eps + sigma dt), c = +-(2 dt/dl)/(2 eps + sigma dt)
+// with the sign and cell size of each curl term fixed by the reference's
+// cyclic (a, b, c) convention (update_scheme.cuh:27-63).  Dispersive media
+// use the ADE form (see ADEMethodStorage).
+void Simulation::buildCoefficients() {
+  const Real dt = _time._dt;
+  const Real d[3] = {_dx, _dy, _dz};
+  const int kind = adeKindOf(_mat_table);
+  Index np = 0;
+  for (auto m : _mat_table) np = std::max(np, polesOf(*m, kind));
+  const std::size_t nm = _mat_table.size();
+  _mat_coeff.assign(nm * 18, 0.0);
+  for (std::size_t m = 0; m < nm; ++m) {
+    const Material& mat = *_mat_table[m];
+    const Real eps = EPSILON_0 * mat.emp().epsilon_r;
+    const Real mu = MU_0 * mat.emp().mu_r;
+    const Real se = mat.emp().sigma_e, sm = mat.emp().sigma_m;
+    Real* row = &_mat_coeff[m * 18];
+    for (int c = 0; c < 3; ++c) {
+      const int a = (c + 1) % 3, b = (c + 2) % 3;
+      if (mat.dispersion() != Material::Dispersion::None) {
+        const AdeCoeff ac = adeCoefficients(mat, kind, np, dt);
+        row[3 * c] = ac.ce;
+        row[3 * c + 1] = -ac.ccurl / d[b];
+        row[3 * c + 2] = ac.ccurl / d[a];
+      } else {
+        const Real den = 2 * eps + se * dt;
+        row[3 * c] = (2 * eps - se * dt) / den;
+        row[3 * c + 1] = -(2 * dt / d[b]) / den;
+        row[3 * c + 2] = (2 * dt / d[a]) / den;
+      }
+      const Real denh = 2 * mu + sm * dt;
+      row[9 + 3 * c] = (2 * mu - sm * dt) / denh;
+      row[9 + 3 * c + 1] = (2 * dt / d[b]) / denh;
+      row[9 + 3 * c + 2] = -(2 * dt / d[a]) / denh;
+    }
+  }
+  const GridSpace& g = *_grid;
+  const int64_t nx = g.sizeX(), ny = g.sizeY(), nz = g.sizeZ();
+  if (_layout == CoefficientLayout::Compact) {
+    for (auto& c : _coeff) c = Array();
+    return;
+  }
+  for (int n = 0; n < 18; ++n) {
+    const int attr = n / 9, comp = (n % 9) / 3;
+    std::vector<int64_t> shape = {nx + 1, ny + 1, nz + 1};
+    // E_c spans n along c and n+1 across; H_c spans n+1 along c and n across.
+    for (int ax = 0; ax < 3; ++ax) {
+      const int64_t nn = ax == 0 ? nx : ax == 1 ? ny : nz;
+      shape[ax] = (attr == 0) ? (ax == comp ? nn : nn + 1) : (ax == comp ? nn + 1 : nn);
+    }
+    _coeff[n] = Array(shape);
+    Array& arr = _coeff[n];
+    const int64_t s1 = shape[1], s2 = shape[2];
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < shape[0]; ++i) {
+      const int64_t ci = std::min(i, nx - 1);
+      for (int64_t j = 0; j < s1; ++j) {
+        const int64_t cj = std::min(j, ny - 1);
+        for (int64_t k = 0; k < s2; ++k) {
+          const int64_t ck = std::min(k, nz - 1);
+          const uint16_t m = _cell_mat[(ci * ny + cj) * nz + ck];
+          arr[(i * s1 + j) * s2 + k] = _mat_coeff[m * 18 + n];
+        }
+      }
+    }
+  }
+}
+
+// PML set-up, with the index conventions PMLCorrector reads
+// (pml_corrector.cu:9-48, 50-270): on the negative face the corrected E
+// nodes are c = 1..L (the kernel shifts by one when the slab starts at 0),
+// on the positive face c = n-L..n-1; H cells are c = start..start+L-1.
+void Simulation::buildPML(PML& p) {
+  const GridSpace& g = *_grid;
+  const int axis = static_cast<int>(p.mainAxis());
+  const int a = (axis + 1) % 3, b = (axis + 2) % 3;
+  const int64_t n[3] = {static_cast<int64_t>(g.sizeX()), static_cast<int64_t>(g.sizeY()),
+                        static_cast<int64_t>(g.sizeZ())};
+  const int64_t L = static_cast<int64_t>(p.thickness());
+  if (L >= n[axis]) throw XFDTDException("PML thicker than the grid");
+  const bool negative = static_cast<int>(p.direction()) % 2 == 0;
+  const int64_t c0 = negative ? 0 : n[axis] - L;
+  xf_pml& dsc = p._desc;
+  dsc = xf_pml{};
+  dsc.main_axis = axis;
+  xf_range r[3] = {{0, n[0]}, {0, n[1]}, {0, n[2]}};
+  r[axis] = {c0, c0 + L};
+  dsc.task = {r[0], r[1], r[2]};
+  dsc.global_e_start = negative ? 1 : c0;
+  dsc.node_e_start = dsc.global_e_start;
+  dsc.global_h_start = c0;
+  dsc.node_h_start = c0;
+  dsc.offset_c = 0;
+  const Real dl = axis == 0 ? _dx : axis == 1 ? _dy : _dz;
+  const Real dt = _time._dt;
+  const Real m = static_cast<Real>(p._order);
+  const Real sigma_max = p._sigma_ratio * 0.8 * (m + 1) / (ETA_0 * dl);
+  auto profile = [&](Real rho, Real& ca, Real& cb, Real& kappa) {
+    const Real x = rho / static_cast<Real>(L);
+    const Real sigma = sigma_max * std::pow(x, m);
+    kappa = 1 + (p._kappa_max - 1) * std::pow(x, m);
+    const Real alpha = p._alpha_min + (p._alpha_max - p._alpha_min) * (1 - x);
+    cb = std::exp(-(sigma / kappa + alpha) * dt / EPSILON_0);
+    ca = sigma == 0 ? 0.0 : sigma / (sigma * kappa + kappa * kappa * alpha) * (cb - 1);
+  };
+  p._ca_e = Array({L});
+  p._cb_e = Array({L});
+  p._ca_h = Array({L});
+  p._cb_h = Array({L});
+  p._kappa_e.assign(L, 1);
+  p._kappa_h.assign(L, 1);
+  for (int64_t l = 0; l < L; ++l) {
+    const int64_t ce = dsc.global_e_start + l;  // E node index along axis
+    const Real rho_e = negative ? static_cast<Real>(L - ce) : static_cast<Real>(ce - c0);
+    profile(rho_e, p._ca_e[l], p._cb_e[l], p._kappa_e[l]);
+    const int64_t ch = dsc.global_h_start + l;  // H at ch + 1/2
+    const Real rho_h = negative ? static_cast<Real>(L) - (static_cast<Real>(ch) + 0.5)
+                                : static_cast<Real>(ch) + 0.5 - static_cast<Real>(c0);
+    profile(rho_h, p._ca_h[l], p._cb_h[l], p._kappa_h[l]);
+  }
+  // psi / c_psi arrays: index aBCToXYZ(a, b, layer) over the corrected
+  // lateral ranges (E_A: a<n_a, b<=n_b; E_B: a<=n_a, b<n_b;
+  // H_A: a<=n_a, b<n_b; H_B: a<n_a, b<=n_b).
+  auto shape_of = [&](int64_t na, int64_t nb) {
+    std::vector<int64_t> s(3);
+    s[a] = na;
+    s[b] = nb;
+    s[axis] = L;
+    return s;
+  };
+  p._psi_ea = Array(shape_of(n[a], n[b] + 1));
+  p._psi_eb = Array(shape_of(n[a] + 1, n[b]));
+  p._psi_ha = Array(shape_of(n[a] + 1, n[b]));
+  p._psi_hb = Array(shape_of(n[a], n[b] + 1));
+  p._c_ea = Array(p._psi_ea.shape());
+  p._c_eb = Array(p._psi_eb.shape());
+  p._c_ha = Array(p._psi_ha.shape());
+  p._c_hb = Array(p._psi_hb.shape());
+  // c_psi is the unscaled coefficient of the corrected field's curl term
+  // along the PML axis: E_A term c_a (index 1), E_B term c_b (index 2),
+  // H_A term c_a, H_B term c_b.
+  auto fill_cpsi = [&](Array& cp, int attr, int comp, int which, int64_t node_start) {
+    const auto& s = cp.shape();
+    const int64_t nxx = n[0], nyy = n[1], nzz = n[2];
+    const int coeff_n = attr * 9 + comp * 3 + which;
+    for (int64_t i = 0; i < s[0]; ++i)
+      for (int64_t j = 0; j < s[1]; ++j)
+        for (int64_t k = 0; k < s[2]; ++k) {
+          int64_t pos[3] = {i, j, k};
+          pos[axis] += node_start;
+          const int64_t ci = std::min(pos[0], nxx - 1), cj = std::min(pos[1], nyy - 1),
+                        ck = std::min(pos[2], nzz - 1);
+          const uint16_t mm = _cell_mat[(ci * nyy + cj) * nzz + ck];
+          cp.at(i, j, k) = _mat_coeff[mm * 18 + coeff_n];
+        }
+  };
+  fill_cpsi(p._c_ea, 0, a, 1, dsc.node_e_start);
+  fill_cpsi(p._c_eb, 0, b, 2, dsc.node_e_start);
+  fill_cpsi(p._c_ha, 1, a, 1, dsc.node_h_start);
+  fill_cpsi(p._c_hb, 1, b, 2, dsc.node_h_start);
+  // kappa stretching of the main curl coefficients along the PML axis.
+  if (p._kappa_max != 1.0) {
+    if (_layout == CoefficientLayout::Compact)
+      throw XFDTDException("PML kappa_max != 1 needs the dense coefficient layout");
+    auto scale = [&](int coeff_n, int64_t start, const std::vector<Real>& kap) {
+      Array& arr = _coeff[coeff_n];
+      const auto& s = arr.shape();
+      for (int64_t i = 0; i < s[0]; ++i)
+        for (int64_t j = 0; j < s[1]; ++j)
+          for (int64_t k = 0; k < s[2]; ++k) {
+            const int64_t pos[3] = {i, j, k};
+            const int64_t l = pos[axis] - start;
+            if (l >= 0 && l < L) arr.at(i, j, k) /= kap[l];
+          }
+    };
+    scale(0 * 9 + a * 3 + 1, dsc.node_e_start, p._kappa_e);
+    scale(0 * 9 + b * 3 + 2, dsc.node_e_start, p._kappa_e);
+    scale(1 * 9 + a * 3 + 1, dsc.node_h_start, p._kappa_h);
+    scale(1 * 9 + b * 3 + 2, dsc.node_h_start, p._kappa_h);
+  }
+  dsc.coeff_a_e = p._ca_e.view();
+  dsc.coeff_b_e = p._cb_e.view();
+  dsc.coeff_a_h = p._ca_h.view();
+  dsc.coeff_b_h = p._cb_h.view();
+  dsc.c_ea_psi_hb = p._c_ea.view();
+  dsc.c_eb_psi_ha = p._c_eb.view();
+  dsc.c_ha_psi_eb = p._c_ha.view();
+  dsc.c_hb_psi_ea = p._c_hb.view();
+  dsc.ea_psi_hb = p._psi_ea.view();
+  dsc.eb_psi_ha = p._psi_eb.view();
+  dsc.ha_psi_eb = p._psi_ha.view();
+  dsc.hb_psi_ea = p._psi_hb.view();
+}
+
+void Simulation::buildTFSF(TFSF& tf) {
+  const GridSpace& g = *_grid;
+  const int64_t n[3] = {static_cast<int64_t>(g.sizeX()), static_cast<int64_t>(g.sizeY()),
+                        static_cast<int64_t>(g.sizeZ())};
+  const bool two_d = g.dimension() == GridSpace::Dimension::TWO;
+  if (two_d != (tf._dimension == 2))
+    throw XFDTDException("TFSF dimension does not match the grid");
+  const int64_t st[3] = {static_cast<int64_t>(tf._sx), static_cast<int64_t>(tf._sy),
+                         two_d ? 0 : static_cast<int64_t>(tf._sz)};
+  xf_range r[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    if (two_d && ax == 2) {
+      r[ax] = {0, 1};
+      continue;
+    }
+    if (2 * st[ax] + 1 > n[ax] || st[ax] < 1)
+      throw XFDTDException("TFSF box does not fit in the grid");
+    r[ax] = {st[ax], n[ax] - st[ax]};
+  }
+  xf_tfsf& dsc = tf._desc;
+  dsc = xf_tfsf{};
+  dsc.dimension = tf._dimension;
+  dsc.total_task = {r[0], r[1], r[2]};
+  const Real st_t = std::sin(tf._theta), ct = std::cos(tf._theta);
+  const Real sp = std::sin(tf._phi), cp = std::cos(tf._phi);
+  const Real ss = std::sin(tf._psi), cs = std::cos(tf._psi);
+  Real k[3] = {st_t * cp, st_t * sp, ct};
+  for (auto& v : k)
+    if (std::abs(v) < 1e-6) v = 0;
+  tf._k = {k[0], k[1], k[2]};
+  // dispersion-matching ratio (test/2d.cu:793-797)
+  const Real ratio =
+      1.0 / std::sqrt(std::pow(st_t, 4) * (std::pow(cp, 4) + std::pow(sp, 4)) + std::pow(ct, 4));
+  Real incident_point[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    const Real corner = k[ax] >= 0 ? static_cast<Real>(r[ax].start) : static_cast<Real>(r[ax].end);
+    incident_point[ax] = corner - 2 * k[ax] / ratio;
+  }
+  Array* ints[3] = {&tf._pxi, &tf._pyi, &tf._pzi};
+  Array* halves[3] = {&tf._pxh, &tf._pyh, &tf._pzh};
+  Real bound = 0;
+  for (int ax = 0; ax < 3; ++ax) {
+    const int64_t size = r[ax].end - r[ax].start;
+    *ints[ax] = Array({size + 2});
+    *halves[ax] = Array({size + 2});
+    Real mx = 0;
+    for (int64_t i = 0; i < size + 2; ++i) {
+      const Real pos = static_cast<Real>(i + r[ax].start) - incident_point[ax];
+      (*ints[ax])[i] = pos * k[ax] * ratio;
+      (*halves[ax])[i] = (pos - 0.5) * k[ax] * ratio;
+      mx = std::max({mx, std::abs((*ints[ax])[i]), std::abs((*halves[ax])[i])});
+    }
+    bound += mx;
+  }
+  const Index nt = _time._size;
+  const int64_t n_aux = static_cast<int64_t>(std::ceil(bound)) + 4;
+  const int64_t nl = n_aux - 1;  // H points
+  tf._e_inc = Array({static_cast<int64_t>(nt), n_aux});
+  tf._h_inc = Array({static_cast<int64_t>(nt), nl});
+  // 1-D auxiliary grid, leapfrog: e(t) from h(t-1), source at l = 0, Mur end;
+  // h(t) from e(t).
+  const Real dl = _dx / ratio;
+  const Real ceih = -_time._dt / (EPSILON_0 * dl);
+  const Real chie = -_time._dt / (MU_0 * dl);
+  const Real c = C_0;
+  const Real abc0 = (c * _time._dt - dl) / (c * _time._dt + dl);
+  const Real abc1 = 2 * dl / (c * _time._dt + dl);
+  tf._waveform->init(_time._dt, nt);
+  std::vector<Real> e(n_aux, 0), h(nl, 0), e_prev(n_aux, 0), e_prev2(n_aux, 0);
+  for (Index t = 0; t < nt; ++t) {
+    e_prev2 = e_prev;
+    e_prev = e;
+    for (int64_t l = 1; l < nl; ++l) e[l] = e_prev[l] + ceih * (h[l] - h[l - 1]);
+    e[0] = tf._waveform->value()[t];
+    e[nl] = -e_prev2[nl - 1] + abc0 * (e[nl - 1] + e_prev2[nl]) +
+            abc1 * (e_prev[nl] + e_prev[nl - 1]);
+    for (int64_t l = 0; l < nl; ++l) h[l] = h[l] + chie * (e[l + 1] - e[l]);
+    std::copy(e.begin(), e.end(), tf._e_inc.data() + t * n_aux);
+    std::copy(h.begin(), h.end(), tf._h_inc.data() + t * nl);
+  }
+  // polarisation (test/2d.cu:911-927)
+  const Real kex = ss, key = cs;
+  const Real te[3] = {kex * (-sp) + key * (cp * ct), kex * cp + key * (sp * ct), key * (-st_t)};
+  const Real th[3] = {k[1] * te[2] - k[2] * te[1], k[2] * te[0] - k[0] * te[2],
+                      k[0] * te[1] - k[1] * te[0]};
+  for (int ax = 0; ax < 3; ++ax) {
+    dsc.transform_e[ax] = te[ax];
+    dsc.transform_h[ax] = th[ax];
+  }
+  dsc.cax = _time._dt / (EPSILON_0 * _dx);
+  dsc.cbx = _time._dt / (MU_0 * _dx);
+  dsc.cay = _time._dt / (EPSILON_0 * _dy);
+  dsc.cby = _time._dt / (MU_0 * _dy);
+  dsc.caz = _time._dt / (EPSILON_0 * _dz);
+  dsc.cbz = _time._dt / (MU_0 * _dz);
+  dsc.proj_x_int = tf._pxi.view();
+  dsc.proj_y_int = tf._pyi.view();
+  dsc.proj_z_int = tf._pzi.view();
+  dsc.proj_x_half = tf._pxh.view();
+  dsc.proj_y_half = tf._pyh.view();
+  dsc.proj_z_half = tf._pzh.view();
+  dsc.e_inc = tf._e_inc.view();
+  dsc.h_inc = tf._h_inc.view();
+}
+
+void Simulation::buildNF2FF(NFFFTFrequencyDomain& nf) {
+  const GridSpace& g = *_grid;
+  nf._grid = _grid.get();
+  const int64_t n[3] = {static_cast<int64_t>(g.sizeX()), static_cast<int64_t>(g.sizeY()),
+                        static_cast<int64_t>(g.sizeZ())};
+  const bool two_d = g.dimension() == GridSpace::Dimension::TWO;
+  const int64_t s[3] = {static_cast<int64_t>(nf._sx), static_cast<int64_t>(nf._sy),
+                        static_cast<int64_t>(nf._sz)};
+  xf_nf2ff& dsc = nf._desc;
+  dsc = xf_nf2ff{};
+  for (int face = 0; face < 6; ++face) {
+    const int axis = face / 2;
+    xf_range r[3];
+    bool empty = false;
+    for (int ax = 0; ax < 3; ++ax) {
+      if (two_d && ax == 2) {
+        r[ax] = {0, 1};
+        if (axis == 2) empty = true;
+        continue;
+      }
+      if (s[ax] < 1 || 2 * s[ax] + 1 > n[ax])
+        throw XFDTDException("NF2FF box does not fit in the grid");
+      r[ax] = {s[ax], n[ax] - s[ax]};
+    }
+    const int64_t c = face % 2 == 0 ? r[axis].start : r[axis].end;
+    r[axis] = {c, c + 1};
+    if (empty) r[0] = r[1] = r[2] = {0, 0};
+    dsc.face_task[face] = {r[0], r[1], r[2]};
+  }
+  const Index nt = _time._size;
+  const Index nfreq = nf._freqs.size();
+  nf._cur.clear();
+  nf._te.clear();
+  nf._th.clear();
+  nf._cur.reserve(nfreq * 24);
+  for (Index f = 0; f < nfreq; ++f) {
+    const Real w = 2 * PI * nf._freqs[f];
+    Array te({static_cast<int64_t>(nt), 2}), th({static_cast<int64_t>(nt), 2});
+    for (Index t = 0; t < nt; ++t) {
+      const Real tt_e = _time._dt * (static_cast<Real>(t) + 1);
+      const Real tt_h = _time._dt * (static_cast<Real>(t) + 0.5);
+      te[2 * t] = _time._dt * std::cos(-w * tt_e);
+      te[2 * t + 1] = _time._dt * std::sin(-w * tt_e);
+      th[2 * t] = _time._dt * std::cos(-w * tt_h);
+      th[2 * t + 1] = _time._dt * std::sin(-w * tt_h);
+    }
+    nf._te.push_back(std::move(te));
+    nf._th.push_back(std::move(th));
+    for (int face = 0; face < 6; ++face) {
+      const xf_task& t = dsc.face_task[face];
+      for (int q = 0; q < 4; ++q)
+        nf._cur.emplace_back(std::vector<int64_t>{t.x.end - t.x.start, t.y.end - t.y.start,
+                                                  t.z.end - t.z.start, 2});
+    }
+  }
+  nf._fdesc.assign(nfreq, xf_nf2ff_freq{});
+  for (Index f = 0; f < nfreq; ++f) {
+    nf._fdesc[f].transform_e = nf._te[f].view();
+    nf._fdesc[f].transform_h = nf._th[f].view();
+    for (int face = 0; face < 6; ++face)
+      for (int q = 0; q < 4; ++q) nf._fdesc[f].cur[face][q] = nf._cur[(f * 6 + face) * 4 + q].view();
+  }
+  dsc.n_freq = static_cast<int32_t>(nfreq);
+  dsc.freq = nf._fdesc.data();
+}
+
+void Simulation::buildADE() {
+  const int kind = adeKindOf(_mat_table);
+  if (kind == XF_ADE_NONE) {
+    _ade.reset();
+    return;
+  }
+  Index np = 0;
+  for (auto m : _mat_table) np = std::max(np, polesOf(*m, kind));
+  const GridSpace& g = *_grid;
+  const int64_t nx = g.sizeX(), ny = g.sizeY(), nz = g.sizeZ();
+  _ade = std::make_unique<ADEMethodStorage>();
+  ADEMethodStorage& s = *_ade;
+  s._np = np;
+  const std::vector<int64_t> s4 = {nx + 1, ny + 1, nz + 1, static_cast<int64_t>(np)};
+  const std::vector<int64_t> s3 = {nx + 1, ny + 1, nz + 1};
+  const bool mlor = kind == XF_ADE_MLORENTZ;
+  s._cjj = Array(s4);
+  s._cje = Array(s4);
+  s._cjsj = Array(s4);
+  s._cejs = Array(s3);
+  s._jx = Array(s4);
+  s._jy = Array(s4);
+  s._jz = Array(s4);
+  if (mlor) {
+    s._cjjp = Array(s4);
+    s._cjen = Array(s4);
+    s._cjep = Array(s4);
+    s._cjsjp = Array(s4);
+    s._ceep = Array(s3);
+    s._exp = Array(s3);
+    s._eyp = Array(s3);
+    s._ezp = Array(s3);
+    s._jxp = Array(s4);
+    s._jyp = Array(s4);
+    s._jzp = Array(s4);
+  }
+  std::vector<AdeCoeff> per_mat(_mat_table.size());
+  std::vector<char> disp(_mat_table.size(), 0);
+  for (std::size_t m = 0; m < _mat_table.size(); ++m) {
+    if (_mat_table[m]->dispersion() == Material::Dispersion::None) continue;
+    disp[m] = 1;
+    per_mat[m] = adeCoefficients(*_mat_table[m], kind, np, _time._dt);
+  }
+  for (int64_t i = 0; i <= nx; ++i)
+    for (int64_t j = 0; j <= ny; ++j)
+      for (int64_t k = 0; k <= nz; ++k) {
+        const int64_t ci = std::min(i, nx - 1), cj = std::min(j, ny - 1), ck = std::min(k, nz - 1);
+        const uint16_t m = _cell_mat[(ci * ny + cj) * nz + ck];
+        if (!disp[m]) continue;
+        const AdeCoeff& c = per_mat[m];
+        const int64_t l3 = (i * (ny + 1) + j) * (nz + 1) + k;
+        s._cejs[l3] = c.cejs;
+        if (mlor) s._ceep[l3] = c.ceep;
+        for (Index p = 0; p < np; ++p) {
+          const int64_t l4 = l3 * static_cast<int64_t>(np) + static_cast<int64_t>(p);
+          s._cjj[l4] = c.cjj[p];
+          s._cje[l4] = c.cje[p];
+          s._cjsj[l4] = c.cjsj[p];
+          if (mlor) {
+            s._cjjp[l4] = c.cjjp[p];
+            s._cjen[l4] = c.cjen[p];
+            s._cjep[l4] = c.cjep[p];
+            s._cjsjp[l4] = c.cjsjp[p];
+          }
+        }
+      }
+  xf_ade& d = s._desc;
+  d = xf_ade{};
+  d.kind = kind;
+  d.num_pole = static_cast<int64_t>(np);
+  d.coeff_j_j = s._cjj.view();
+  d.coeff_j_j_p = s._cjjp.view();
+  d.coeff_j_e_n = s._cjen.view();
+  d.coeff_j_e = s._cje.view();
+  d.coeff_j_e_p = s._cjep.view();
+  d.coeff_j_sum_j = s._cjsj.view();
+  d.coeff_j_sum_j_p = s._cjsjp.view();
+  d.coeff_e_j_sum = s._cejs.view();
+  d.coeff_e_e_p = s._ceep.view();
+  d.ex_prev = s._exp.view();
+  d.ey_prev = s._eyp.view();
+  d.ez_prev = s._ezp.view();
+  d.jx = s._jx.view();
+  d.jy = s._jy.view();
+  d.jz = s._jz.view();
+  d.jx_prev = s._jxp.view();
+  d.jy_prev = s._jyp.view();
+  d.jz_prev = s._jzp.view();
+}
+
+void Simulation::fillScene() {
+  const GridSpace& g = *_grid;
+  _scene = xf_scene{};
+  _scene.abi_version = XF_ABI_VERSION;
+  _scene.dimension = static_cast<int32_t>(g.dimension());
+  _scene.nx = g.sizeX();
+  _scene.ny = g.sizeY();
+  _scene.nz = g.sizeZ();
+  _scene.dt = _time._dt;
+  _scene.start_step = _time._start;
+  _scene.n_steps = _time._size;
+  _scene.current_step = _time._current;
+  _scene.ex = _fields[0].view();
+  _scene.ey = _fields[1].view();
+  _scene.ez = _fields[2].view();
+  _scene.hx = _fields[3].view();
+  _scene.hy = _fields[4].view();
+  _scene.hz = _fields[5].view();
+  for (int n = 0; n < 18; ++n) _scene.coeff[n] = _coeff[n].view();
+  if (_layout == CoefficientLayout::Compact) {
+    _scene.cell_material = _cell_mat.data();
+    _scene.n_material = static_cast<int32_t>(_mat_table.size());
+    _scene.material_coeff = _mat_coeff.data();
+  }
+  _pml_desc.clear();
+  for (int ax = 0; ax < 3; ++ax)
+    for (auto& b : _boundaries) {
+      auto p = dynamic_cast<PML*>(b.get());
+      if (p && static_cast<int>(p->mainAxis()) == ax) _pml_desc.push_back(p->desc());
+    }
+  _scene.n_pml = static_cast<int32_t>(_pml_desc.size());
+  _scene.pml = _pml_desc.empty() ? nullptr : _pml_desc.data();
+  _scene.tfsf = nullptr;
+  for (auto& w : _sources) {
+    auto t = dynamic_cast<TFSF*>(w.get());
+    if (t) {
+      _scene.tfsf = &t->desc();
+      break;
+    }
+  }
+  _nf_desc.clear();
+  for (auto& n : _nf2ffs) {
+    auto f = dynamic_cast<NFFFTFrequencyDomain*>(n.get());
+    if (f) _nf_desc.push_back(f->desc());
+  }
+  _scene.n_nf2ff = static_cast<int32_t>(_nf_desc.size());
+  _scene.nf2ff = _nf_desc.empty() ? nullptr : _nf_desc.data();
+  _scene.ade = _ade ? &_ade->desc() : nullptr;
+}
+
+void Simulation::init(Index time_step) {
+  _time._start = 0;
+  _time._size = time_step;
+  _time._current = 0;
+  buildGrid();
+  voxelize();
+  buildCoefficients();
+  const GridSpace& g = *_grid;
+  const int64_t nx = g.sizeX(), ny = g.sizeY(), nz = g.sizeZ();
+  _fields[0] = Array({nx, ny + 1, nz + 1});
+  _fields[1] = Array({nx + 1, ny, nz + 1});
+  _fields[2] = Array({nx + 1, ny + 1, nz});
+  _fields[3] = Array({nx + 1, ny, nz});
+  _fields[4] = Array({nx, ny + 1, nz});
+  _fields[5] = Array({nx, ny, nz + 1});
+  for (int ax = 0; ax < 3; ++ax)
+    for (auto& b : _boundaries) {
+      auto p = dynamic_cast<PML*>(b.get());
+      if (p && static_cast<int>(p->mainAxis()) == ax) buildPML(*p);
+    }
+  int n_tfsf = 0;
+  for (auto& w : _sources) {
+    auto t = dynamic_cast<TFSF*>(w.get());
+    if (t) {
+      if (++n_tfsf > 1) throw XFDTDException("only one TFSF source is supported");
+      buildTFSF(*t);
+    }
+  }
+  for (auto& n : _nf2ffs) {
+    auto f = dynamic_cast<NFFFTFrequencyDomain*>(n.get());
+    if (f) buildNF2FF(*f);
+  }
+  buildADE();
+  fillScene();
+  _initialized = true;
+}
+
+}  // namespace xfdtd

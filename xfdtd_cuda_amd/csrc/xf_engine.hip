@@ -1,0 +1,1480 @@
+// xf_engine.hip -- host side of the gfx950 FDTD engine: the C ABI of
+// include/xfdtd_hip.h.  Validates the scene, lays the state out in HBM,
+// compresses the Yee coefficients, and runs the DomainHD step loop
+// (src/domain/domain_hd.cu:35-87) as 2-6 launches per step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/xfdtd_hip.h"
+#include "xf_kernels.inl"
+
+namespace xf {
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+
+#define XF_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(XF_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+#define XF_TRY(call)              \
+  do {                            \
+    int s_ = (call);              \
+    if (s_ != XF_OK) return s_;   \
+  } while (0)
+
+bool empty_task(const xf_task& t) {
+  return !(t.x.start < t.x.end && t.y.start < t.y.end && t.z.start < t.z.end);
+}
+
+int64_t task_lo(const xf_task& t, int ax) { return ax == 0 ? t.x.start : ax == 1 ? t.y.start : t.z.start; }
+int64_t task_hi(const xf_task& t, int ax) { return ax == 0 ? t.x.end : ax == 1 ? t.y.end : t.z.end; }
+
+void xyz_to_abc(int axis, const int64_t x[3], int64_t abc[3]) {
+  abc[0] = x[(axis + 1) % 3];
+  abc[1] = x[(axis + 2) % 3];
+  abc[2] = x[axis];
+}
+
+int check_array(const xf_array& a, std::vector<int64_t> shape, const std::string& name) {
+  if (a.data == nullptr) return fail(XF_ERR_INVALID, name + ": null data");
+  if (a.ndim != static_cast<int32_t>(shape.size()))
+    return fail(XF_ERR_INVALID, name + ": expected " + std::to_string(shape.size()) + " dims, got " +
+                                    std::to_string(a.ndim));
+  for (size_t d = 0; d < shape.size(); ++d)
+    if (a.shape[d] != shape[d])
+      return fail(XF_ERR_INVALID, name + ": extent " + std::to_string(d) + " is " +
+                                      std::to_string(a.shape[d]) + ", expected " +
+                                      std::to_string(shape[d]));
+  return XF_OK;
+}
+
+int64_t numel(const xf_array& a) {
+  int64_t n = 1;
+  for (int d = 0; d < a.ndim; ++d) n *= a.shape[d];
+  return n;
+}
+
+// Field shape of component comp (0..2) of E (attr 0) or H (attr 1).
+std::vector<int64_t> field_shape(int attr, int comp, int64_t nx, int64_t ny, int64_t nz) {
+  const int64_t n[3] = {nx, ny, nz};
+  std::vector<int64_t> s(3);
+  for (int ax = 0; ax < 3; ++ax) s[ax] = (attr == 0) ? (ax == comp ? n[ax] : n[ax] + 1) : (ax == comp ? n[ax] + 1 : n[ax]);
+  return s;
+}
+
+struct TfsfJobHost {
+  int comp, face;
+  int64_t o[3], n[3];
+  double coef;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- engine
+struct Engine {
+  xf_scene* sc{};
+  xf_plan plan{};
+  int dim{3};
+  int64_t nx{}, ny{}, nz{};
+  int64_t sx{}, sy{}, total{};  // layout: 3-D idx = i*sx + j*sy + k; 2-D idx = i*sx + j
+  size_t esize{8};              // bytes per stored value
+  bool uploaded{false};
+  int64_t cur_step{0};
+  int64_t nt{0};
+  int device{0};
+  hipStream_t stream{};
+  std::vector<void*> allocs;
+  // fields
+  void* fld[6]{};
+  // coefficient ids and tables (compute type)
+  void* eid{};
+  void* hid{};
+  int id_bytes{1};
+  int ntab_e{0}, ntab_h{0};
+  void* etab{};
+  void* htab{};
+  unsigned char* eflag{};
+  std::vector<double> etab_h, htab_h;  // host copies (double)
+  std::vector<unsigned char> eflag_h;
+  // PML
+  std::vector<int> pml_order;  // scene index sorted by axis
+  struct PmlDev {
+    void *ca_e, *cb_e, *ca_h, *cb_h;
+    void* cpsi[4];
+    void* psi[4];
+  };
+  std::vector<PmlDev> pml;
+  // TFSF
+  std::vector<TfsfJobHost> tfsf_jobs[2];  // [0] H pass, [1] E pass
+  void* proj_int[3]{};
+  void* proj_half[3]{};
+  void* e_inc{};
+  void* h_inc{};
+  // NF2FF
+  struct NfDev {
+    double2* cur;
+    double2* te;
+    double2* th;
+    NfPass pass;
+  };
+  std::vector<NfDev> nf;
+  // ADE
+  AdeDev ade{};
+  bool has_ade{false};
+  std::vector<int64_t> ade_cells;  // host (i,j,k) linear index in (nx+1)(ny+1)(nz+1)
+  // time counter for graph replay
+  int* t_dev{};
+  hipGraph_t graph{};
+  hipGraphExec_t graph_exec{};
+  bool graph_ready{false};
+  // timing
+  xf_timing timing{};
+  int tile_y{4};
+  int chunk{0};
+  std::string layout_desc;
+
+  ~Engine() { release(); }
+
+  void release() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    graph_ready = false;
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+    uploaded = false;
+  }
+
+  int alloc(void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(XF_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    allocs.push_back(*p);
+    return XF_OK;
+  }
+
+  template <typename T>
+  int upload_vec(void** p, const std::vector<T>& v) {
+    XF_TRY(alloc(p, v.size() * sizeof(T)));
+    if (!v.empty()) XF_HIP(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return XF_OK;
+  }
+
+  // compute-type copy of a host double array
+  int upload_c(void** p, const double* src, int64_t n) {
+    if (plan.precision == XF_FP64) {
+      std::vector<double> v(src, src + n);
+      return upload_vec(p, v);
+    }
+    std::vector<float> v(n);
+    for (int64_t i = 0; i < n; ++i) v[i] = static_cast<float>(src[i]);
+    return upload_vec(p, v);
+  }
+
+  int download_c(double* dst, const void* p, int64_t n) {
+    if (plan.precision == XF_FP64) {
+      XF_HIP(hipMemcpy(dst, p, n * sizeof(double), hipMemcpyDeviceToHost));
+      return XF_OK;
+    }
+    std::vector<float> v(n);
+    XF_HIP(hipMemcpy(v.data(), p, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) dst[i] = v[i];
+    return XF_OK;
+  }
+
+  int validate();
+  int build_tfsf_jobs();
+  int check_overlaps();
+  int upload();
+  int pack_fields();
+  int unpack_fields();
+  int build_ids();
+  int upload_pml();
+  int upload_tfsf();
+  int upload_nf2ff();
+  int upload_ade();
+  int download();
+  int run(int64_t n);
+  template <typename S>
+  int launch_step(int t, bool use_tdev, bool timing_events, hipEvent_t* ev);
+  template <typename S>
+  int launch_h(int t);
+  template <typename S>
+  int launch_e(int t);
+  template <typename S>
+  int launch_post_h(int t, const int* tp);
+  template <typename S>
+  int launch_post_e(int t, const int* tp);
+
+  double bytes_per_cell_step() const;
+};
+
+// ------------------------------------------------------------ validation
+int Engine::validate() {
+  if (sc->abi_version != XF_ABI_VERSION) return fail(XF_ERR_INVALID, "scene ABI version mismatch");
+  dim = sc->dimension;
+  nx = sc->nx;
+  ny = sc->ny;
+  nz = sc->nz;
+  if (dim != 2 && dim != 3) return fail(XF_ERR_INVALID, "dimension must be 2 or 3");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(XF_ERR_INVALID, "empty grid");
+  if (dim == 2 && nz != 1) return fail(XF_ERR_INVALID, "a 2-D scene has nz = 1");
+  if (nx > (1 << 20) || ny > (1 << 20) || nz > (1 << 20)) return fail(XF_ERR_INVALID, "grid too large");
+  if (!(sc->dt > 0)) return fail(XF_ERR_INVALID, "dt must be positive");
+  nt = sc->start_step + sc->n_steps;
+  cur_step = sc->current_step;
+  const xf_array* E[3] = {&sc->ex, &sc->ey, &sc->ez};
+  const xf_array* H[3] = {&sc->hx, &sc->hy, &sc->hz};
+  const char* en[3] = {"ex", "ey", "ez"};
+  const char* hn[3] = {"hx", "hy", "hz"};
+  for (int c = 0; c < 3; ++c) {
+    XF_TRY(check_array(*E[c], field_shape(0, c, nx, ny, nz), en[c]));
+    XF_TRY(check_array(*H[c], field_shape(1, c, nx, ny, nz), hn[c]));
+  }
+  if (sc->coeff[0].data) {
+    for (int n = 0; n < 18; ++n)
+      XF_TRY(check_array(sc->coeff[n], field_shape(n / 9, (n % 9) / 3, nx, ny, nz),
+                         "coeff." + std::to_string(n)));
+  } else {
+    if (!sc->cell_material || !sc->material_coeff || sc->n_material < 1)
+      return fail(XF_ERR_INVALID, "no coefficients: dense arrays or a material table are required");
+  }
+  if (sc->n_pml < 0 || sc->n_pml > kMaxPml) return fail(XF_ERR_UNSUPPORTED, "at most 6 PML faces are supported");
+  const int64_t n3[3] = {nx, ny, nz};
+  for (int p = 0; p < sc->n_pml; ++p) {
+    const xf_pml& P = sc->pml[p];
+    const std::string nm = "pml." + std::to_string(p);
+    if (P.main_axis < 0 || P.main_axis > 2) return fail(XF_ERR_INVALID, nm + ": bad axis");
+    if (dim == 2 && P.main_axis == 2) return fail(XF_ERR_UNSUPPORTED, nm + ": z PML on a 2-D grid");
+    if (empty_task(P.task)) continue;
+    const int ax = P.main_axis, a = (ax + 1) % 3, b = (ax + 2) % 3;
+    for (int d = 0; d < 3; ++d)
+      if (task_lo(P.task, d) < 0 || task_hi(P.task, d) > n3[d])
+        return fail(XF_ERR_INVALID, nm + ": task outside the grid");
+    const int64_t cs = task_lo(P.task, ax), ce = task_hi(P.task, ax);
+    const int64_t off = cs == 0 ? 1 : 0;
+    const int64_t ce0 = cs + off, ce1 = ce + off;  // E nodes
+    // E nodes along c must have an H neighbour at c-1 and lie inside the E
+    // tangential extent (n+1); H at c+1 needs E extent n+1.
+    if (ce0 < 1 || ce1 > n3[ax] + 1) return fail(XF_ERR_INVALID, nm + ": E range outside the grid");
+    const int64_t L = ce - cs;
+    const xf_array* c1[4] = {&P.coeff_a_e, &P.coeff_b_e, &P.coeff_a_h, &P.coeff_b_h};
+    for (int q = 0; q < 4; ++q) {
+      if (!c1[q]->data || c1[q]->ndim != 1) return fail(XF_ERR_INVALID, nm + ": 1-D coefficient missing");
+    }
+    // 1-D index c + off_c - gstart over the corrected ranges
+    auto in1 = [&](const xf_array& arr, int64_t c0, int64_t c1_, int64_t gs) {
+      return c0 + P.offset_c - gs >= 0 && c1_ - 1 + P.offset_c - gs < arr.shape[0];
+    };
+    if (!in1(P.coeff_a_e, ce0, ce1, P.global_e_start) || !in1(P.coeff_b_e, ce0, ce1, P.global_e_start) ||
+        !in1(P.coeff_a_h, cs, ce, P.global_h_start) || !in1(P.coeff_b_h, cs, ce, P.global_h_start))
+      return fail(XF_ERR_INVALID, nm + ": 1-D coefficients do not cover the slab");
+    // psi arrays: index aBCToXYZ(a, b, layer) over the corrected ranges
+    const int64_t a0 = task_lo(P.task, a), a1 = task_hi(P.task, a);
+    const int64_t b0 = task_lo(P.task, b), b1 = task_hi(P.task, b);
+    struct R {
+      const xf_array* psi;
+      const xf_array* cpsi;
+      int64_t amax, bmax, l0, l1;
+    } rs[4] = {
+        {&P.ea_psi_hb, &P.c_ea_psi_hb, a1 - 1, b1, ce0 - P.node_e_start, ce1 - 1 - P.node_e_start},
+        {&P.eb_psi_ha, &P.c_eb_psi_ha, a1, b1 - 1, ce0 - P.node_e_start, ce1 - 1 - P.node_e_start},
+        {&P.ha_psi_eb, &P.c_ha_psi_eb, a1, b1 - 1, cs - P.node_h_start, ce - 1 - P.node_h_start},
+        {&P.hb_psi_ea, &P.c_hb_psi_ea, a1 - 1, b1, cs - P.node_h_start, ce - 1 - P.node_h_start}};
+    for (int q = 0; q < 4; ++q) {
+      if (!rs[q].psi->data || !rs[q].cpsi->data || rs[q].psi->ndim != 3 || rs[q].cpsi->ndim != 3)
+        return fail(XF_ERR_INVALID, nm + ": psi arrays missing");
+      for (int d = 0; d < 3; ++d)
+        if (rs[q].psi->shape[d] != rs[q].cpsi->shape[d]) return fail(XF_ERR_INVALID, nm + ": psi/c_psi shapes differ");
+      if (rs[q].l0 < 0) return fail(XF_ERR_INVALID, nm + ": negative psi layer");
+      int64_t mx[3];
+      mx[a] = rs[q].amax;
+      mx[b] = rs[q].bmax;
+      mx[ax] = rs[q].l1;
+      int64_t mn[3];
+      mn[a] = a0;
+      mn[b] = b0;
+      mn[ax] = rs[q].l0;
+      for (int d = 0; d < 3; ++d)
+        if (mn[d] < 0 || mx[d] >= rs[q].psi->shape[d]) return fail(XF_ERR_INVALID, nm + ": psi array too small");
+    }
+    (void)L;
+  }
+  if (sc->tfsf) XF_TRY(build_tfsf_jobs());
+  for (int q = 0; q < sc->n_nf2ff; ++q) {
+    const xf_nf2ff& N = sc->nf2ff[q];
+    if (N.n_freq < 1 || !N.freq) return fail(XF_ERR_INVALID, "nf2ff: no frequency");
+    for (int face = 0; face < 6; ++face) {
+      const xf_task& t = N.face_task[face];
+      if (empty_task(t)) continue;
+      const int ax = face / 2;
+      if (dim == 2 && ax == 2) return fail(XF_ERR_UNSUPPORTED, "nf2ff: z face on a 2-D grid");
+      if (task_hi(t, ax) - task_lo(t, ax) != 1) return fail(XF_ERR_INVALID, "nf2ff: face task is not one cell thick");
+      // interpolation reads index-1 along the face normal's tangential axes
+      // and index+1; keep them inside [0, n]
+      for (int d = 0; d < 3; ++d) {
+        if (dim == 2 && d == 2) {
+          if (t.z.start != 0 || t.z.end != 1) return fail(XF_ERR_INVALID, "nf2ff: 2-D face z range must be [0,1)");
+          continue;
+        }
+        if (task_lo(t, d) < 1 || task_hi(t, d) > n3[d] - 1)
+          return fail(XF_ERR_INVALID, "nf2ff: face task must stay one cell inside the grid");
+      }
+      for (int f = 0; f < N.n_freq; ++f)
+        for (int c = 0; c < 4; ++c)
+          XF_TRY(check_array(N.freq[f].cur[face][c],
+                             {t.x.end - t.x.start, t.y.end - t.y.start, t.z.end - t.z.start, 2},
+                             "nf2ff current"));
+    }
+    for (int f = 0; f < N.n_freq; ++f) {
+      XF_TRY(check_array(N.freq[f].transform_e, {nt, 2}, "nf2ff transform_e"));
+      XF_TRY(check_array(N.freq[f].transform_h, {nt, 2}, "nf2ff transform_h"));
+    }
+  }
+  if (sc->ade && sc->ade->kind != XF_ADE_NONE) {
+    const xf_ade& A = *sc->ade;
+    if (dim != 3) return fail(XF_ERR_UNSUPPORTED, "ADE media need a 3-D grid");
+    if (A.kind < 1 || A.kind > 3) return fail(XF_ERR_INVALID, "ade: unknown kind");
+    if (A.num_pole < 1 || A.num_pole > 64) return fail(XF_ERR_INVALID, "ade: bad pole count");
+    const std::vector<int64_t> s4 = {nx + 1, ny + 1, nz + 1, A.num_pole}, s3 = {nx + 1, ny + 1, nz + 1};
+    XF_TRY(check_array(A.coeff_j_j, s4, "ade.coeff_j_j"));
+    XF_TRY(check_array(A.coeff_j_e, s4, "ade.coeff_j_e"));
+    XF_TRY(check_array(A.coeff_j_sum_j, s4, "ade.coeff_j_sum_j"));
+    XF_TRY(check_array(A.coeff_e_j_sum, s3, "ade.coeff_e_j_sum"));
+    XF_TRY(check_array(A.jx, s4, "ade.jx"));
+    XF_TRY(check_array(A.jy, s4, "ade.jy"));
+    XF_TRY(check_array(A.jz, s4, "ade.jz"));
+    if (A.kind == XF_ADE_MLORENTZ) {
+      XF_TRY(check_array(A.coeff_j_j_p, s4, "ade.coeff_j_j_p"));
+      XF_TRY(check_array(A.coeff_j_e_n, s4, "ade.coeff_j_e_n"));
+      XF_TRY(check_array(A.coeff_j_e_p, s4, "ade.coeff_j_e_p"));
+      XF_TRY(check_array(A.coeff_j_sum_j_p, s4, "ade.coeff_j_sum_j_p"));
+      XF_TRY(check_array(A.coeff_e_e_p, s3, "ade.coeff_e_e_p"));
+      XF_TRY(check_array(A.ex_prev, s3, "ade.ex_prev"));
+      XF_TRY(check_array(A.ey_prev, s3, "ade.ey_prev"));
+      XF_TRY(check_array(A.ez_prev, s3, "ade.ez_prev"));
+      XF_TRY(check_array(A.jx_prev, s4, "ade.jx_prev"));
+      XF_TRY(check_array(A.jy_prev, s4, "ade.jy_prev"));
+      XF_TRY(check_array(A.jz_prev, s4, "ade.jz_prev"));
+    }
+    has_ade = true;
+  }
+  return XF_OK;
+}
+
+// TFSF jobs in the reference's launch order, with written boxes; checks every
+// incident-table index the kernel will form (tfsf_corrector.cuh:20-170).
+int Engine::build_tfsf_jobs() {
+  const xf_tfsf& T = *sc->tfsf;
+  if (T.dimension != dim) return fail(XF_ERR_INVALID, "tfsf: dimension does not match the grid");
+  const xf_task& tt = T.total_task;
+  if (empty_task(tt)) return fail(XF_ERR_INVALID, "tfsf: empty total task");
+  const int64_t n3[3] = {nx, ny, nz};
+  for (int d = 0; d < 3; ++d) {
+    if (dim == 2 && d == 2) {
+      if (tt.z.start != 0 || tt.z.end != 1) return fail(XF_ERR_INVALID, "tfsf: 2-D task z range must be [0,1)");
+      continue;
+    }
+    if (task_lo(tt, d) < 1 || task_hi(tt, d) > n3[d] - 1) return fail(XF_ERR_INVALID, "tfsf: box must stay inside the grid");
+  }
+  const xf_array* pi[3] = {&T.proj_x_int, &T.proj_y_int, &T.proj_z_int};
+  const xf_array* ph[3] = {&T.proj_x_half, &T.proj_y_half, &T.proj_z_half};
+  for (int d = 0; d < 3; ++d)
+    if (!pi[d]->data || !ph[d]->data || pi[d]->ndim != 1 || ph[d]->ndim != 1)
+      return fail(XF_ERR_INVALID, "tfsf: projection arrays missing");
+  if (!T.e_inc.data || !T.h_inc.data || T.e_inc.ndim != 2 || T.h_inc.ndim != 2)
+    return fail(XF_ERR_INVALID, "tfsf: incident tables missing");
+  if (T.e_inc.shape[0] < nt || T.h_inc.shape[0] < nt) return fail(XF_ERR_INVALID, "tfsf: incident tables shorter than the run");
+  // Job order (tfsf_corrector.cu:155-271): (face, component)
+  struct FC {
+    int face, comp;
+  };
+  std::vector<FC> order_e, order_h;
+  if (dim == 3) {
+    const FC o[12] = {{XF_XN, 2}, {XF_XP, 2}, {XF_YN, 2}, {XF_YP, 2}, {XF_ZN, 0}, {XF_ZP, 0},
+                      {XF_YN, 0}, {XF_YP, 0}, {XF_ZN, 1}, {XF_ZP, 1}, {XF_XN, 1}, {XF_XP, 1}};
+    order_e.assign(o, o + 12);
+    order_h.assign(o, o + 12);
+  } else {
+    order_e = {{XF_XN, 2}, {XF_XP, 2}, {XF_YN, 2}, {XF_YP, 2}};
+    order_h = {{XF_XN, 1}, {XF_XP, 1}, {XF_YN, 0}, {XF_YP, 0}};
+  }
+  const double ca[3] = {T.cax, T.cay, T.caz}, cb[3] = {T.cbx, T.cby, T.cbz};
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool is_e = pass == 1;
+    tfsf_jobs[pass].clear();
+    for (const FC& fc : (is_e ? order_e : order_h)) {
+      const int ax = fc.face / 2, a = (ax + 1) % 3, b = (ax + 2) % 3;
+      const bool negative = fc.face % 2 == 0;
+      const int64_t cpos = negative ? task_lo(tt, ax) : task_hi(tt, ax);
+      int64_t lo[3] = {tt.x.start, tt.y.start, tt.z.start}, hi[3] = {tt.x.end, tt.y.end, tt.z.end};
+      lo[ax] = cpos;
+      hi[ax] = cpos + 1;
+      const int offset_a = ((is_e && fc.comp == b) || (!is_e && fc.comp == a)) ? 1 : 0;
+      if (offset_a)
+        hi[a] += 1;
+      else
+        hi[b] += 1;
+      TfsfJobHost J;
+      J.comp = fc.comp;
+      J.face = fc.face;
+      for (int d = 0; d < 3; ++d) {
+        J.o[d] = lo[d];
+        J.n[d] = hi[d] - lo[d];
+      }
+      if (negative && !is_e) J.o[ax] -= 1;  // H written at c-1
+      const int equation_flag = is_e ? 1 : -1;
+      const int different_flag = (fc.comp == a) ? -1 : 1;
+      const int direction_flag = negative ? -1 : 1;
+      J.coef = (is_e ? ca[ax] : cb[ax]) * (equation_flag * different_flag * direction_flag);
+      // written cells inside the field's own shape
+      const auto fs = field_shape(is_e ? 0 : 1, fc.comp, nx, ny, nz);
+      for (int d = 0; d < 3; ++d)
+        if (J.o[d] < 0 || J.o[d] + J.n[d] > fs[d]) return fail(XF_ERR_INVALID, "tfsf: correction outside the field");
+      // incident-table indices at every cell
+      const int dual = (fc.comp == a) ? b : a;
+      const bool inc_e = !is_e;
+      const xf_array& tab = inc_e ? T.e_inc : T.h_inc;
+      const int64_t g[3] = {tt.x.start, tt.y.start, tt.z.start};
+      for (int64_t i = J.o[0]; i < J.o[0] + J.n[0]; ++i)
+        for (int64_t j = J.o[1]; j < J.o[1] + J.n[1]; ++j)
+          for (int64_t k = J.o[2]; k < J.o[2] + J.n[2]; ++k) {
+            int64_t p[3] = {i, j, k};
+            if (negative) p[ax] += is_e ? -1 : 1;
+            const int64_t l[3] = {p[0] - g[0], p[1] - g[1], p[2] - g[2]};
+            // which axes are half-integer for this incident component
+            bool half[3];
+            for (int d = 0; d < 3; ++d) half[d] = inc_e ? (d == dual) : (d != dual);
+            double proj = 0;
+            for (int d = 0; d < 3; ++d) {
+              const int64_t ix = half[d] ? l[d] + 1 : l[d];
+              const xf_array& arr = half[d] ? *ph[d] : *pi[d];
+              if (ix < 0 || ix >= arr.shape[0]) return fail(XF_ERR_INVALID, "tfsf: projection index out of range");
+              proj += arr.data[ix];
+            }
+            if (!inc_e) proj -= 0.5;
+            if (!(proj >= 0)) return fail(XF_ERR_INVALID, "tfsf: negative projection");
+            const int64_t index = static_cast<int64_t>(proj);
+            if (index + 1 >= tab.shape[1]) return fail(XF_ERR_INVALID, "tfsf: incident table too short");
+          }
+      tfsf_jobs[pass].push_back(J);
+    }
+  }
+  return XF_OK;
+}
+
+// Fusing the PML corrections into the Yee kernels applies them before the
+// TFSF corrections; the reference applies TFSF first (simulation_hd.cu:
+// 124-160).  Both only add to a field, so the result is the same unless one
+// cell receives both, which is rejected here.
+int Engine::check_overlaps() {
+  if (!sc->tfsf) return XF_OK;
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool is_e = pass == 1;
+    for (const auto& J : tfsf_jobs[pass]) {
+      for (int p = 0; p < sc->n_pml; ++p) {
+        const xf_pml& P = sc->pml[p];
+        if (empty_task(P.task)) continue;
+        const int ax = P.main_axis, a = (ax + 1) % 3, b = (ax + 2) % 3;
+        const int64_t cs = task_lo(P.task, ax), ce = task_hi(P.task, ax);
+        const int64_t off = (is_e && cs == 0) ? 1 : 0;
+        int64_t lo[3], hi[3];
+        lo[ax] = cs + off;
+        hi[ax] = ce + off;
+        lo[a] = task_lo(P.task, a);
+        hi[a] = task_hi(P.task, a) + 1;
+        lo[b] = task_lo(P.task, b);
+        hi[b] = task_hi(P.task, b) + 1;
+        // only the two tangential components are corrected
+        if (J.comp == ax) continue;
+        bool overlap = true;
+        for (int d = 0; d < 3; ++d)
+          if (J.o[d] >= hi[d] || J.o[d] + J.n[d] <= lo[d]) overlap = false;
+        if (overlap) return fail(XF_ERR_UNSUPPORTED, "tfsf box overlaps a PML slab; move the TF/SF boundary inside the PML");
+      }
+    }
+  }
+  return XF_OK;
+}
+
+double Engine::bytes_per_cell_step() const {
+  // H pass: read H3 + E3, write H3; E pass: read E3 + H3, write E3, plus one
+  // coefficient index per pass.
+  const double fields = 2.0 * 9.0 * static_cast<double>(esize);
+  return fields + 2.0 * id_bytes;
+}
+
+// ------------------------------------------------------------- upload
+int Engine::pack_fields() {
+  const xf_array* src[6] = {&sc->ex, &sc->ey, &sc->ez, &sc->hx, &sc->hy, &sc->hz};
+  const int prec = plan.precision;
+  const double se = 1.0, sh = (prec == XF_BF16_STORE) ? 376.730313668 : 1.0;
+  std::vector<unsigned char> buf(total * esize);
+  for (int c = 0; c < 6; ++c) {
+    const xf_array& a = *src[c];
+    const bool live = dim == 3 || c == 2 || c == 3 || c == 4;
+    if (!live) {
+      // 2-D: Ex, Ey, Hz are identically zero in the TE updator
+      for (int64_t n = 0; n < numel(a); ++n)
+        if (a.data[n] != 0.0) return fail(XF_ERR_UNSUPPORTED, "2-D grid with nonzero Ex/Ey/Hz");
+      fld[c] = nullptr;
+      continue;
+    }
+    std::fill(buf.begin(), buf.end(), 0);
+    const double scale = c < 3 ? se : sh;
+    for (int64_t i = 0; i < a.shape[0]; ++i)
+      for (int64_t j = 0; j < a.shape[1]; ++j)
+        for (int64_t k = 0; k < a.shape[2]; ++k) {
+          const double v = a.data[(i * a.shape[1] + j) * a.shape[2] + k];
+          const int64_t idx = dim == 3 ? i * sx + j * sy + k : i * sx + j;
+          if (prec == XF_FP64)
+            reinterpret_cast<double*>(buf.data())[idx] = v;
+          else if (prec == XF_FP32)
+            reinterpret_cast<float*>(buf.data())[idx] = static_cast<float>(v);
+          else
+            reinterpret_cast<__hip_bfloat16*>(buf.data())[idx] = __float2bfloat16(static_cast<float>(v * scale));
+        }
+    XF_TRY(alloc(&fld[c], total * esize));
+    XF_HIP(hipMemcpy(fld[c], buf.data(), total * esize, hipMemcpyHostToDevice));
+  }
+  return XF_OK;
+}
+
+int Engine::unpack_fields() {
+  xf_array* dst[6] = {&sc->ex, &sc->ey, &sc->ez, &sc->hx, &sc->hy, &sc->hz};
+  const int prec = plan.precision;
+  const double sh = (prec == XF_BF16_STORE) ? 376.730313668 : 1.0;
+  std::vector<unsigned char> buf(total * esize);
+  for (int c = 0; c < 6; ++c) {
+    if (!fld[c]) continue;
+    XF_HIP(hipMemcpy(buf.data(), fld[c], total * esize, hipMemcpyDeviceToHost));
+    xf_array& a = *dst[c];
+    const double inv = c < 3 ? 1.0 : 1.0 / sh;
+    for (int64_t i = 0; i < a.shape[0]; ++i)
+      for (int64_t j = 0; j < a.shape[1]; ++j)
+        for (int64_t k = 0; k < a.shape[2]; ++k) {
+          const int64_t idx = dim == 3 ? i * sx + j * sy + k : i * sx + j;
+          double v;
+          if (prec == XF_FP64)
+            v = reinterpret_cast<const double*>(buf.data())[idx];
+          else if (prec == XF_FP32)
+            v = reinterpret_cast<const float*>(buf.data())[idx];
+          else
+            v = static_cast<double>(__bfloat162float(reinterpret_cast<const __hip_bfloat16*>(buf.data())[idx])) * inv;
+          a.data[(i * a.shape[1] + j) * a.shape[2] + k] = v;
+        }
+  }
+  return XF_OK;
+}
+
+namespace {
+struct KeyHash {
+  size_t operator()(const std::vector<uint64_t>& k) const {
+    uint64_t h = 1469598103934665603ull;
+    for (uint64_t v : k) {
+      h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+    }
+    return static_cast<size_t>(h);
+  }
+};
+uint64_t bits(double v) {
+  uint64_t u;
+  std::memcpy(&u, &v, 8);
+  return u;
+}
+}  // namespace
+
+// Per-cell coefficient indices.  Dense input: the 9 coefficients of each
+// pass at (i, j, k) (each from its own field-shaped array) are deduplicated
+// bit-exactly into a table.  Compact input: the material index is the id.
+int Engine::build_ids() {
+  std::vector<uint32_t> ide(nx * ny * nz), idh(nx * ny * nz);
+  etab_h.clear();
+  htab_h.clear();
+  eflag_h.clear();
+  // dispersive cells
+  std::vector<unsigned char> disp;
+  if (has_ade) {
+    const xf_ade& A = *sc->ade;
+    disp.assign(nx * ny * nz, 0);
+    const int64_t np = A.num_pole;
+    for (int64_t i = 0; i < nx; ++i)
+      for (int64_t j = 0; j < ny; ++j)
+        for (int64_t k = 0; k < nz; ++k) {
+          const int64_t l3 = (i * (ny + 1) + j) * (nz + 1) + k;
+          bool d = A.coeff_e_j_sum.data[l3] != 0.0;
+          if (A.kind == XF_ADE_MLORENTZ && A.coeff_e_e_p.data[l3] != 0.0) d = true;
+          for (int64_t p = 0; p < np && !d; ++p) {
+            const int64_t l4 = l3 * np + p;
+            if (A.coeff_j_j.data[l4] != 0.0 || A.coeff_j_e.data[l4] != 0.0 || A.coeff_j_sum_j.data[l4] != 0.0) d = true;
+            if (A.kind == XF_ADE_MLORENTZ &&
+                (A.coeff_j_j_p.data[l4] != 0.0 || A.coeff_j_e_n.data[l4] != 0.0 ||
+                 A.coeff_j_e_p.data[l4] != 0.0 || A.coeff_j_sum_j_p.data[l4] != 0.0))
+              d = true;
+          }
+          disp[(i * ny + j) * nz + k] = d ? 1 : 0;
+        }
+  }
+  if (sc->coeff[0].data == nullptr) {
+    const int nm = sc->n_material;
+    if (has_ade) return fail(XF_ERR_UNSUPPORTED, "ADE media need the dense coefficient layout");
+    for (int m = 0; m < nm; ++m) {
+      for (int n = 0; n < 9; ++n) {
+        etab_h.push_back(sc->material_coeff[m * 18 + n]);
+        htab_h.push_back(sc->material_coeff[m * 18 + 9 + n]);
+      }
+      eflag_h.push_back(0);
+    }
+    for (int64_t c = 0; c < nx * ny * nz; ++c) {
+      const uint16_t m = sc->cell_material[c];
+      if (m >= nm) return fail(XF_ERR_INVALID, "cell material index out of range");
+      ide[c] = idh[c] = m;
+    }
+    ntab_e = ntab_h = nm;
+  } else {
+    std::unordered_map<std::vector<uint64_t>, uint32_t, KeyHash> me, mh;
+    std::vector<uint64_t> ke(10), kh(9), last_e, last_h;
+    uint32_t last_ide = 0, last_idh = 0;
+    for (int64_t i = 0; i < nx; ++i)
+      for (int64_t j = 0; j < ny; ++j)
+        for (int64_t k = 0; k < nz; ++k) {
+          const int64_t c = (i * ny + j) * nz + k;
+          for (int n = 0; n < 18; ++n) {
+            const xf_array& a = sc->coeff[n];
+            const double v = a.data[(i * a.shape[1] + j) * a.shape[2] + k];
+            if (n < 9)
+              ke[n] = bits(v);
+            else
+              kh[n - 9] = bits(v);
+          }
+          ke[9] = has_ade ? disp[c] : 0;
+          if (ke == last_e) {
+            ide[c] = last_ide;
+          } else {
+            auto it = me.find(ke);
+            if (it == me.end()) {
+              it = me.emplace(ke, static_cast<uint32_t>(me.size())).first;
+              for (int n = 0; n < 9; ++n) {
+                double v;
+                std::memcpy(&v, &ke[n], 8);
+                etab_h.push_back(v);
+              }
+              eflag_h.push_back(static_cast<unsigned char>(ke[9]));
+            }
+            ide[c] = it->second;
+            last_e = ke;
+            last_ide = it->second;
+          }
+          if (kh == last_h) {
+            idh[c] = last_idh;
+          } else {
+            auto it = mh.find(kh);
+            if (it == mh.end()) {
+              it = mh.emplace(kh, static_cast<uint32_t>(mh.size())).first;
+              for (int n = 0; n < 9; ++n) {
+                double v;
+                std::memcpy(&v, &kh[n], 8);
+                htab_h.push_back(v);
+              }
+            }
+            idh[c] = it->second;
+            last_h = kh;
+            last_idh = it->second;
+          }
+        }
+    ntab_e = static_cast<int>(me.size());
+    ntab_h = static_cast<int>(mh.size());
+  }
+  if (ntab_e > 65536 || ntab_h > 65536)
+    return fail(XF_ERR_UNSUPPORTED, "more than 65536 distinct coefficient rows; the engine needs a smaller coefficient variety");
+  id_bytes = (ntab_e <= 256 && ntab_h <= 256) ? 1 : 2;
+  // scatter ids into the padded layout
+  auto put = [&](void** dptr, const std::vector<uint32_t>& id) -> int {
+    if (id_bytes == 1) {
+      std::vector<uint8_t> v(total, 0);
+      for (int64_t i = 0; i < nx; ++i)
+        for (int64_t j = 0; j < ny; ++j)
+          for (int64_t k = 0; k < nz; ++k)
+            v[dim == 3 ? i * sx + j * sy + k : i * sx + j] = static_cast<uint8_t>(id[(i * ny + j) * nz + k]);
+      return upload_vec(dptr, v);
+    }
+    std::vector<uint16_t> v(total, 0);
+    for (int64_t i = 0; i < nx; ++i)
+      for (int64_t j = 0; j < ny; ++j)
+        for (int64_t k = 0; k < nz; ++k)
+          v[dim == 3 ? i * sx + j * sy + k : i * sx + j] = static_cast<uint16_t>(id[(i * ny + j) * nz + k]);
+    return upload_vec(dptr, v);
+  };
+  XF_TRY(put(&eid, ide));
+  XF_TRY(put(&hid, idh));
+  XF_TRY(upload_c(&etab, etab_h.data(), static_cast<int64_t>(etab_h.size())));
+  XF_TRY(upload_c(&htab, htab_h.data(), static_cast<int64_t>(htab_h.size())));
+  void* fl = nullptr;
+  XF_TRY(upload_vec(&fl, eflag_h));
+  eflag = static_cast<unsigned char*>(fl);
+  if (has_ade) {
+    ade_cells.clear();
+    for (int64_t i = 0; i < nx; ++i)
+      for (int64_t j = 0; j < ny; ++j)
+        for (int64_t k = 0; k < nz; ++k)
+          if (disp[(i * ny + j) * nz + k]) ade_cells.push_back((i * (ny + 1) + j) * (nz + 1) + k);
+  }
+  return XF_OK;
+}
+
+int Engine::upload_pml() {
+  pml_order.clear();
+  for (int ax = 0; ax < 3; ++ax)
+    for (int p = 0; p < sc->n_pml; ++p)
+      if (sc->pml[p].main_axis == ax && !empty_task(sc->pml[p].task)) pml_order.push_back(p);
+  pml.assign(pml_order.size(), PmlDev{});
+  for (size_t q = 0; q < pml_order.size(); ++q) {
+    const xf_pml& P = sc->pml[pml_order[q]];
+    PmlDev& D = pml[q];
+    XF_TRY(upload_c(&D.ca_e, P.coeff_a_e.data, numel(P.coeff_a_e)));
+    XF_TRY(upload_c(&D.cb_e, P.coeff_b_e.data, numel(P.coeff_b_e)));
+    XF_TRY(upload_c(&D.ca_h, P.coeff_a_h.data, numel(P.coeff_a_h)));
+    XF_TRY(upload_c(&D.cb_h, P.coeff_b_h.data, numel(P.coeff_b_h)));
+    const xf_array* cp[4] = {&P.c_ea_psi_hb, &P.c_eb_psi_ha, &P.c_ha_psi_eb, &P.c_hb_psi_ea};
+    const xf_array* ps[4] = {&P.ea_psi_hb, &P.eb_psi_ha, &P.ha_psi_eb, &P.hb_psi_ea};
+    for (int c = 0; c < 4; ++c) {
+      XF_TRY(upload_c(&D.cpsi[c], cp[c]->data, numel(*cp[c])));
+      XF_TRY(upload_c(&D.psi[c], ps[c]->data, numel(*ps[c])));
+    }
+  }
+  return XF_OK;
+}
+
+int Engine::upload_tfsf() {
+  if (!sc->tfsf) return XF_OK;
+  const xf_tfsf& T = *sc->tfsf;
+  const xf_array* pi[3] = {&T.proj_x_int, &T.proj_y_int, &T.proj_z_int};
+  const xf_array* ph[3] = {&T.proj_x_half, &T.proj_y_half, &T.proj_z_half};
+  for (int d = 0; d < 3; ++d) {
+    XF_TRY(upload_c(&proj_int[d], pi[d]->data, numel(*pi[d])));
+    XF_TRY(upload_c(&proj_half[d], ph[d]->data, numel(*ph[d])));
+  }
+  XF_TRY(upload_c(&e_inc, T.e_inc.data, numel(T.e_inc)));
+  XF_TRY(upload_c(&h_inc, T.h_inc.data, numel(T.h_inc)));
+  return XF_OK;
+}
+
+int Engine::upload_nf2ff() {
+  nf.clear();
+  for (int q = 0; q < sc->n_nf2ff; ++q) {
+    const xf_nf2ff& N = sc->nf2ff[q];
+    NfDev D{};
+    NfPass& P = D.pass;
+    int64_t start = 0, off = 0;
+    for (int face = 0; face < 6; ++face) {
+      const xf_task& t = N.face_task[face];
+      NfFace& F = P.face[face];
+      F.start = start;
+      F.off = off;
+      if (empty_task(t)) {
+        F.o[0] = F.o[1] = F.o[2] = 0;
+        F.n[0] = F.n[1] = F.n[2] = 0;
+        continue;
+      }
+      F.o[0] = static_cast<int>(t.x.start);
+      F.o[1] = static_cast<int>(t.y.start);
+      F.o[2] = static_cast<int>(t.z.start);
+      F.n[0] = static_cast<int>(t.x.end - t.x.start);
+      F.n[1] = static_cast<int>(t.y.end - t.y.start);
+      F.n[2] = static_cast<int>(t.z.end - t.z.start);
+      const int64_t cells = static_cast<int64_t>(F.n[0]) * F.n[1] * F.n[2];
+      start += cells;
+      off += cells;
+    }
+    P.total = start;
+    P.block = off;
+    P.nfreq = N.n_freq;
+    P.nt = nt;
+    // current buffer [freq][face][comp][cell]
+    std::vector<double2> cur(static_cast<size_t>(N.n_freq) * off * 4);
+    std::vector<double2> te(static_cast<size_t>(N.n_freq) * nt), th(te.size());
+    for (int f = 0; f < N.n_freq; ++f) {
+      for (int face = 0; face < 6; ++face) {
+        const NfFace& F = P.face[face];
+        const int64_t cells = static_cast<int64_t>(F.n[0]) * F.n[1] * F.n[2];
+        for (int c = 0; c < 4; ++c) {
+          const xf_array& a = N.freq[f].cur[face][c];
+          for (int64_t n = 0; n < cells; ++n)
+            cur[f * off * 4 + F.off * 4 + c * cells + n] = make_double2(a.data[2 * n], a.data[2 * n + 1]);
+        }
+      }
+      for (int64_t t = 0; t < nt; ++t) {
+        te[f * nt + t] = make_double2(N.freq[f].transform_e.data[2 * t], N.freq[f].transform_e.data[2 * t + 1]);
+        th[f * nt + t] = make_double2(N.freq[f].transform_h.data[2 * t], N.freq[f].transform_h.data[2 * t + 1]);
+      }
+    }
+    void *pc, *pe, *ph;
+    XF_TRY(upload_vec(&pc, cur));
+    XF_TRY(upload_vec(&pe, te));
+    XF_TRY(upload_vec(&ph, th));
+    D.cur = static_cast<double2*>(pc);
+    D.te = static_cast<double2*>(pe);
+    D.th = static_cast<double2*>(ph);
+    P.cur = D.cur;
+    P.te = D.te;
+    P.th = D.th;
+    nf.push_back(D);
+  }
+  return XF_OK;
+}
+
+int Engine::upload_ade() {
+  if (!has_ade) return XF_OK;
+  const xf_ade& A = *sc->ade;
+  const int64_t n = static_cast<int64_t>(ade_cells.size());
+  const int64_t np = A.num_pole;
+  ade = AdeDev{};
+  ade.kind = A.kind;
+  ade.np = static_cast<int>(np);
+  ade.n = n;
+  std::vector<int64_t> idx(n);
+  std::vector<int> ijk(3 * n);
+  std::vector<double> cejs(n), ceep(n), cjj(np * n), cjjp(np * n), cjen(np * n), cje(np * n),
+      cjep(np * n), cjsj(np * n), cjsjp(np * n), eprev(3 * n), jc(3 * np * n), jp(3 * np * n);
+  const xf_array* J[3] = {&A.jx, &A.jy, &A.jz};
+  const xf_array* JP[3] = {&A.jx_prev, &A.jy_prev, &A.jz_prev};
+  const xf_array* EP[3] = {&A.ex_prev, &A.ey_prev, &A.ez_prev};
+  const bool ml = A.kind == XF_ADE_MLORENTZ;
+  for (int64_t c = 0; c < n; ++c) {
+    const int64_t l3 = ade_cells[c];
+    const int64_t k = l3 % (nz + 1), j = (l3 / (nz + 1)) % (ny + 1), i = l3 / ((nz + 1) * (ny + 1));
+    idx[c] = i * sx + j * sy + k;
+    ijk[3 * c] = static_cast<int>(i);
+    ijk[3 * c + 1] = static_cast<int>(j);
+    ijk[3 * c + 2] = static_cast<int>(k);
+    cejs[c] = A.coeff_e_j_sum.data[l3];
+    ceep[c] = ml ? A.coeff_e_e_p.data[l3] : 0.0;
+    for (int64_t p = 0; p < np; ++p) {
+      const int64_t l4 = l3 * np + p, pc = p * n + c;
+      cjj[pc] = A.coeff_j_j.data[l4];
+      cje[pc] = A.coeff_j_e.data[l4];
+      cjsj[pc] = A.coeff_j_sum_j.data[l4];
+      if (ml) {
+        cjjp[pc] = A.coeff_j_j_p.data[l4];
+        cjen[pc] = A.coeff_j_e_n.data[l4];
+        cjep[pc] = A.coeff_j_e_p.data[l4];
+        cjsjp[pc] = A.coeff_j_sum_j_p.data[l4];
+      }
+      for (int comp = 0; comp < 3; ++comp) {
+        jc[(comp * np + p) * n + c] = J[comp]->data[l4];
+        if (ml) jp[(comp * np + p) * n + c] = JP[comp]->data[l4];
+      }
+    }
+    if (ml)
+      for (int comp = 0; comp < 3; ++comp) eprev[comp * n + c] = EP[comp]->data[l3];
+  }
+  void* p;
+  XF_TRY(upload_vec(&p, idx));
+  ade.idx = static_cast<const int64_t*>(p);
+  XF_TRY(upload_vec(&p, ijk));
+  ade.ijk = static_cast<const int*>(p);
+  XF_TRY(upload_vec(&p, cejs));
+  ade.cejs = static_cast<const double*>(p);
+  XF_TRY(upload_vec(&p, ceep));
+  ade.ceep = static_cast<const double*>(p);
+  XF_TRY(upload_vec(&p, cjj));
+  ade.cjj = static_cast<const double*>(p);
+  XF_TRY(upload_vec(&p, cjjp));
+  ade.cjjp = static_cast<const double*>(p);
+  XF_TRY(upload_vec(&p, cjen));
+  ade.cjen = static_cast<const double*>(p);
+  XF_TRY(upload_vec(&p, cje));
+  ade.cje = static_cast<const double*>(p);
+  XF_TRY(upload_vec(&p, cjep));
+  ade.cjep = static_cast<const double*>(p);
+  XF_TRY(upload_vec(&p, cjsj));
+  ade.cjsj = static_cast<const double*>(p);
+  XF_TRY(upload_vec(&p, cjsjp));
+  ade.cjsjp = static_cast<const double*>(p);
+  XF_TRY(upload_vec(&p, eprev));
+  ade.eprev = static_cast<double*>(p);
+  XF_TRY(upload_vec(&p, jc));
+  ade.jc = static_cast<double*>(p);
+  XF_TRY(upload_vec(&p, jp));
+  ade.jp = static_cast<double*>(p);
+  return XF_OK;
+}
+
+int Engine::upload() {
+  release();
+  XF_HIP(hipSetDevice(device));
+  XF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  esize = plan.precision == XF_FP64 ? 8 : plan.precision == XF_FP32 ? 4 : 2;
+  if (dim == 3) {
+    const int64_t align = 128 / static_cast<int64_t>(esize);
+    const int64_t pz = ((nz + 1 + align - 1) / align) * align;
+    sy = pz;
+    sx = (ny + 1) * pz;
+    total = (nx + 1) * sx;
+  } else {
+    const int64_t align = 128 / static_cast<int64_t>(esize);
+    const int64_t py = ((ny + 1 + align - 1) / align) * align;
+    sy = 1;
+    sx = py;
+    total = (nx + 1) * sx;
+  }
+  XF_TRY(build_ids());
+  XF_TRY(pack_fields());
+  XF_TRY(upload_pml());
+  XF_TRY(upload_tfsf());
+  XF_TRY(upload_nf2ff());
+  XF_TRY(upload_ade());
+  void* tp;
+  XF_TRY(alloc(&tp, sizeof(int)));
+  t_dev = static_cast<int*>(tp);
+  tile_y = plan.tile_y > 0 ? std::min(plan.tile_y, 16) : 4;
+  // x chunk of the marching kernels: enough blocks to cover 256 CUs several
+  // times over, with long marches for register reuse.
+  if (dim == 3) {
+    const int64_t cols = ((nz + 1 + 63) / 64) * ((ny + 1 + tile_y - 1) / tile_y);
+    int64_t nchunk = std::max<int64_t>(1, (8 * 256 + cols - 1) / cols);
+    nchunk = std::min<int64_t>(nchunk, nx + 1);
+    chunk = static_cast<int>((nx + 1 + nchunk - 1) / nchunk);
+  }
+  XF_HIP(hipDeviceSynchronize());
+  uploaded = true;
+  layout_desc = std::string("dim=") + std::to_string(dim) + " pad_stride=" + std::to_string(dim == 3 ? sy : sx) +
+                " id_bytes=" + std::to_string(id_bytes) + " etab=" + std::to_string(ntab_e) +
+                " htab=" + std::to_string(ntab_h) + " chunk=" + std::to_string(chunk) +
+                " tile_y=" + std::to_string(tile_y) + " pml_faces=" + std::to_string(pml.size()) +
+                " ade_cells=" + std::to_string(ade_cells.size());
+  return XF_OK;
+}
+
+// ------------------------------------------------------------- download
+int Engine::download() {
+  if (!uploaded) return fail(XF_ERR_STATE, "download before upload");
+  XF_HIP(hipStreamSynchronize(stream));
+  XF_TRY(unpack_fields());
+  for (size_t q = 0; q < pml_order.size(); ++q) {
+    xf_pml& P = sc->pml[pml_order[q]];
+    xf_array* ps[4] = {&P.ea_psi_hb, &P.eb_psi_ha, &P.ha_psi_eb, &P.hb_psi_ea};
+    for (int c = 0; c < 4; ++c) XF_TRY(download_c(ps[c]->data, pml[q].psi[c], numel(*ps[c])));
+  }
+  for (int q = 0; q < static_cast<int>(nf.size()); ++q) {
+    const xf_nf2ff& N = sc->nf2ff[q];
+    const NfPass& P = nf[q].pass;
+    std::vector<double2> cur(static_cast<size_t>(N.n_freq) * P.block * 4);
+    XF_HIP(hipMemcpy(cur.data(), nf[q].cur, cur.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    for (int f = 0; f < N.n_freq; ++f)
+      for (int face = 0; face < 6; ++face) {
+        const NfFace& F = P.face[face];
+        const int64_t cells = static_cast<int64_t>(F.n[0]) * F.n[1] * F.n[2];
+        for (int c = 0; c < 4; ++c) {
+          xf_array& a = N.freq[f].cur[face][c];
+          for (int64_t n = 0; n < cells; ++n) {
+            const double2 v = cur[f * P.block * 4 + F.off * 4 + c * cells + n];
+            a.data[2 * n] = v.x;
+            a.data[2 * n + 1] = v.y;
+          }
+        }
+      }
+  }
+  if (has_ade) {
+    const xf_ade& A = *sc->ade;
+    const int64_t n = ade.n, np = ade.np;
+    std::vector<double> eprev(3 * n), jc(3 * np * n), jp(3 * np * n);
+    XF_HIP(hipMemcpy(jc.data(), ade.jc, jc.size() * 8, hipMemcpyDeviceToHost));
+    XF_HIP(hipMemcpy(jp.data(), ade.jp, jp.size() * 8, hipMemcpyDeviceToHost));
+    XF_HIP(hipMemcpy(eprev.data(), ade.eprev, eprev.size() * 8, hipMemcpyDeviceToHost));
+    const xf_array* J[3] = {&A.jx, &A.jy, &A.jz};
+    const xf_array* JP[3] = {&A.jx_prev, &A.jy_prev, &A.jz_prev};
+    const xf_array* EP[3] = {&A.ex_prev, &A.ey_prev, &A.ez_prev};
+    const bool ml = A.kind == XF_ADE_MLORENTZ;
+    for (int64_t c = 0; c < n; ++c) {
+      const int64_t l3 = ade_cells[c];
+      for (int comp = 0; comp < 3; ++comp) {
+        for (int64_t p = 0; p < np; ++p) {
+          J[comp]->data[l3 * np + p] = jc[(comp * np + p) * n + c];
+          if (ml) JP[comp]->data[l3 * np + p] = jp[(comp * np + p) * n + c];
+        }
+        if (ml) EP[comp]->data[l3] = eprev[comp * n + c];
+      }
+    }
+  }
+  sc->current_step = cur_step;
+  return XF_OK;
+}
+
+// ------------------------------------------------------------- launches
+namespace {
+template <typename C>
+void fill_pml_pass(const Engine& E, PmlPass<C>& P, bool is_e) {
+  P.n = static_cast<int>(E.pml_order.size());
+  for (int q = 0; q < P.n; ++q) {
+    const xf_pml& S = E.sc->pml[E.pml_order[q]];
+    const auto& D = E.pml[q];
+    PmlFace<C>& F = P.f[q];
+    const int ax = S.main_axis, a = (ax + 1) % 3, b = (ax + 2) % 3;
+    F.axis = ax;
+    const int64_t cs = task_lo(S.task, ax), ce = task_hi(S.task, ax);
+    const int64_t off = (is_e && cs == 0) ? 1 : 0;
+    F.c0 = static_cast<int>(cs + off);
+    F.c1 = static_cast<int>(ce + off);
+    F.a0 = static_cast<int>(task_lo(S.task, a));
+    F.a1 = static_cast<int>(task_hi(S.task, a));
+    F.b0 = static_cast<int>(task_lo(S.task, b));
+    F.b1 = static_cast<int>(task_hi(S.task, b));
+    F.gstart = static_cast<int>(is_e ? S.global_e_start : S.global_h_start);
+    F.nstart = static_cast<int>(is_e ? S.node_e_start : S.node_h_start);
+    F.off_c = static_cast<int>(S.offset_c);
+    F.ca = static_cast<const C*>(is_e ? D.ca_e : D.ca_h);
+    F.cb = static_cast<const C*>(is_e ? D.cb_e : D.cb_h);
+    const int ia = is_e ? 0 : 2, ib = is_e ? 1 : 3;
+    const xf_array* ps[4] = {&S.ea_psi_hb, &S.eb_psi_ha, &S.ha_psi_eb, &S.hb_psi_ea};
+    F.psi_a = static_cast<C*>(D.psi[ia]);
+    F.cpsi_a = static_cast<const C*>(D.cpsi[ia]);
+    F.psi_b = static_cast<C*>(D.psi[ib]);
+    F.cpsi_b = static_cast<const C*>(D.cpsi[ib]);
+    F.sa1 = ps[ia]->shape[1];
+    F.sa2 = ps[ia]->shape[2];
+    F.sb1 = ps[ib]->shape[1];
+    F.sb2 = ps[ib]->shape[2];
+  }
+}
+
+template <typename S>
+Fields<S> make_fields(const Engine& E) {
+  Fields<S> f;
+  for (int c = 0; c < 3; ++c) {
+    f.e[c] = static_cast<S*>(E.fld[c]);
+    f.h[c] = static_cast<S*>(E.fld[3 + c]);
+  }
+  return f;
+}
+
+template <typename C>
+Scales<C> make_scales(const Engine& E) {
+  Scales<C> s;
+  const double sh = E.plan.precision == XF_BF16_STORE ? 376.730313668 : 1.0;
+  s.inv_e = 1;
+  s.s_e = 1;
+  s.inv_h = static_cast<C>(1.0 / sh);
+  s.s_h = static_cast<C>(sh);
+  return s;
+}
+
+Dims make_dims(const Engine& E) {
+  Dims d;
+  d.nx = static_cast<int>(E.nx);
+  d.ny = static_cast<int>(E.ny);
+  d.nz = static_cast<int>(E.nz);
+  d.sx = E.sx;
+  d.sy = E.sy;
+  return d;
+}
+}  // namespace
+
+template <typename S>
+int Engine::launch_h(int t) {
+  (void)t;
+  using C = typename Store<S>::C;
+  PmlPass<C> P{};
+  fill_pml_pass<C>(*this, P, false);
+  const auto f = make_fields<S>(*this);
+  const auto scl = make_scales<C>(*this);
+  const Dims d = make_dims(*this);
+  const size_t tab_bytes = static_cast<size_t>(ntab_h) * 9 * sizeof(C);
+  const bool lds = tab_bytes <= 32768;
+  const size_t shm = lds ? tab_bytes : 0;
+  const bool has_pml = P.n > 0;
+  const C* tab = static_cast<const C*>(htab);
+  if (dim == 3) {
+    dim3 block(64, tile_y);
+    dim3 grid((unsigned)((nz + 1 + 63) / 64), (unsigned)((ny + 1 + tile_y - 1) / tile_y),
+              (unsigned)((nx + 1 + chunk - 1) / chunk));
+#define XF_LH3(I, L, PM)                                                                              \
+  hipLaunchKernelGGL((k_yee_h3<S, I, L, PM>), grid, block, shm, stream, f, static_cast<const I*>(hid), \
+                     tab, ntab_h, P, d, chunk, scl)
+    if (id_bytes == 1) {
+      if (lds) { if (has_pml) XF_LH3(uint8_t, true, true); else XF_LH3(uint8_t, true, false); }
+      else { if (has_pml) XF_LH3(uint8_t, false, true); else XF_LH3(uint8_t, false, false); }
+    } else {
+      if (lds) { if (has_pml) XF_LH3(uint16_t, true, true); else XF_LH3(uint16_t, true, false); }
+      else { if (has_pml) XF_LH3(uint16_t, false, true); else XF_LH3(uint16_t, false, false); }
+    }
+#undef XF_LH3
+  } else {
+    dim3 block(64, 4);
+    dim3 grid((unsigned)((ny + 1 + 63) / 64), (unsigned)((nx + 1 + 3) / 4));
+#define XF_LH2(I, L, PM)                                                                              \
+  hipLaunchKernelGGL((k_yee_h2<S, I, L, PM>), grid, block, shm, stream, f, static_cast<const I*>(hid), \
+                     tab, ntab_h, P, d, scl)
+    if (id_bytes == 1) {
+      if (lds) { if (has_pml) XF_LH2(uint8_t, true, true); else XF_LH2(uint8_t, true, false); }
+      else { if (has_pml) XF_LH2(uint8_t, false, true); else XF_LH2(uint8_t, false, false); }
+    } else {
+      if (lds) { if (has_pml) XF_LH2(uint16_t, true, true); else XF_LH2(uint16_t, true, false); }
+      else { if (has_pml) XF_LH2(uint16_t, false, true); else XF_LH2(uint16_t, false, false); }
+    }
+#undef XF_LH2
+  }
+  XF_HIP(hipGetLastError());
+  return XF_OK;
+}
+
+template <typename S>
+int Engine::launch_e(int t) {
+  (void)t;
+  using C = typename Store<S>::C;
+  PmlPass<C> P{};
+  fill_pml_pass<C>(*this, P, true);
+  const auto f = make_fields<S>(*this);
+  const auto scl = make_scales<C>(*this);
+  const Dims d = make_dims(*this);
+  const size_t tab_bytes = static_cast<size_t>(ntab_e) * 9 * sizeof(C);
+  const bool lds = tab_bytes <= 32768;
+  const size_t shm = lds ? tab_bytes : 0;
+  const bool has_pml = P.n > 0;
+  const C* tab = static_cast<const C*>(etab);
+  if (dim == 3) {
+    dim3 block(64, tile_y);
+    dim3 grid((unsigned)((nz + 1 + 63) / 64), (unsigned)((ny + 1 + tile_y - 1) / tile_y),
+              (unsigned)((nx + 1 + chunk - 1) / chunk));
+#define XF_LE3(I, L, PM, AD)                                                                          \
+  hipLaunchKernelGGL((k_yee_e3<S, I, L, PM, AD>), grid, block, shm, stream, f,                         \
+                     static_cast<const I*>(eid), tab, ntab_e, eflag, P, d, chunk, scl)
+#define XF_LE3A(I, L, PM) \
+  do { if (has_ade) XF_LE3(I, L, PM, true); else XF_LE3(I, L, PM, false); } while (0)
+    if (id_bytes == 1) {
+      if (lds) { if (has_pml) XF_LE3A(uint8_t, true, true); else XF_LE3A(uint8_t, true, false); }
+      else { if (has_pml) XF_LE3A(uint8_t, false, true); else XF_LE3A(uint8_t, false, false); }
+    } else {
+      if (lds) { if (has_pml) XF_LE3A(uint16_t, true, true); else XF_LE3A(uint16_t, true, false); }
+      else { if (has_pml) XF_LE3A(uint16_t, false, true); else XF_LE3A(uint16_t, false, false); }
+    }
+#undef XF_LE3A
+#undef XF_LE3
+    XF_HIP(hipGetLastError());
+    if (has_ade && ade.n > 0) {
+      dim3 b1(256), g1((unsigned)((ade.n + 255) / 256));
+      if (id_bytes == 1)
+        hipLaunchKernelGGL((k_ade_e3<S, uint8_t>), g1, b1, 0, stream, f, static_cast<const uint8_t*>(eid), tab, ade, d, scl);
+      else
+        hipLaunchKernelGGL((k_ade_e3<S, uint16_t>), g1, b1, 0, stream, f, static_cast<const uint16_t*>(eid), tab, ade, d, scl);
+    }
+  } else {
+    dim3 block(64, 4);
+    dim3 grid((unsigned)((ny + 1 + 63) / 64), (unsigned)((nx + 1 + 3) / 4));
+#define XF_LE2(I, L, PM)                                                                              \
+  hipLaunchKernelGGL((k_yee_e2<S, I, L, PM>), grid, block, shm, stream, f, static_cast<const I*>(eid), \
+                     tab, ntab_e, P, d, scl)
+    if (id_bytes == 1) {
+      if (lds) { if (has_pml) XF_LE2(uint8_t, true, true); else XF_LE2(uint8_t, true, false); }
+      else { if (has_pml) XF_LE2(uint8_t, false, true); else XF_LE2(uint8_t, false, false); }
+    } else {
+      if (lds) { if (has_pml) XF_LE2(uint16_t, true, true); else XF_LE2(uint16_t, true, false); }
+      else { if (has_pml) XF_LE2(uint16_t, false, true); else XF_LE2(uint16_t, false, false); }
+    }
+#undef XF_LE2
+  }
+  XF_HIP(hipGetLastError());
+  return XF_OK;
+}
+
+namespace {
+template <typename C>
+void fill_tfsf_pass(const Engine& E, TfsfPass<C>& P, int pass) {
+  const auto& jobs = E.tfsf_jobs[pass];
+  P.njob = static_cast<int>(jobs.size());
+  P.is_e = pass == 1;
+  int64_t start = 0;
+  for (int q = 0; q < P.njob; ++q) {
+    const TfsfJobHost& J = jobs[q];
+    TfsfJob<C>& D = P.job[q];
+    D.comp = J.comp;
+    D.face = J.face;
+    for (int d = 0; d < 3; ++d) {
+      D.o[d] = static_cast<int>(J.o[d]);
+      D.n[d] = static_cast<int>(J.n[d]);
+    }
+    D.start = start;
+    D.coef = static_cast<C>(J.coef);
+    start += J.n[0] * J.n[1] * J.n[2];
+  }
+  P.total = start;
+  for (int d = 0; d < 3; ++d) {
+    P.proj_int[d] = static_cast<const C*>(E.proj_int[d]);
+    P.proj_half[d] = static_cast<const C*>(E.proj_half[d]);
+  }
+  const xf_tfsf& T = *E.sc->tfsf;
+  P.inc = static_cast<const C*>(pass == 1 ? E.h_inc : E.e_inc);
+  P.n_aux = pass == 1 ? T.h_inc.shape[1] : T.e_inc.shape[1];
+  P.g[0] = static_cast<int>(T.total_task.x.start);
+  P.g[1] = static_cast<int>(T.total_task.y.start);
+  P.g[2] = static_cast<int>(T.total_task.z.start);
+  for (int d = 0; d < 3; ++d) P.tr[d] = static_cast<C>(pass == 1 ? T.transform_h[d] : T.transform_e[d]);
+}
+}  // namespace
+
+template <typename S>
+int Engine::launch_post_h(int t, const int* tp) {
+  using C = typename Store<S>::C;
+  if (sc->tfsf) {
+    TfsfPass<C> P{};
+    fill_tfsf_pass<C>(*this, P, 0);
+    if (P.total > 0)
+      hipLaunchKernelGGL((k_tfsf<S>), dim3((unsigned)((P.total + 255) / 256)), dim3(256), 0, stream,
+                         make_fields<S>(*this), P, t, tp, make_scales<C>(*this), make_dims(*this), dim == 2 ? 1 : 0);
+  }
+  XF_HIP(hipGetLastError());
+  return XF_OK;
+}
+
+template <typename S>
+int Engine::launch_post_e(int t, const int* tp) {
+  using C = typename Store<S>::C;
+  if (sc->tfsf) {
+    TfsfPass<C> P{};
+    fill_tfsf_pass<C>(*this, P, 1);
+    if (P.total > 0)
+      hipLaunchKernelGGL((k_tfsf<S>), dim3((unsigned)((P.total + 255) / 256)), dim3(256), 0, stream,
+                         make_fields<S>(*this), P, t, tp, make_scales<C>(*this), make_dims(*this), dim == 2 ? 1 : 0);
+  }
+  for (auto& N : nf) {
+    if (N.pass.total > 0)
+      hipLaunchKernelGGL((k_nf2ff<S>), dim3((unsigned)((N.pass.total + 255) / 256)), dim3(256), 0, stream,
+                         make_fields<S>(*this), N.pass, t, tp, make_dims(*this), dim == 2 ? 1 : 0,
+                         make_scales<C>(*this));
+  }
+  XF_HIP(hipGetLastError());
+  return XF_OK;
+}
+
+template <typename S>
+int Engine::launch_step(int t, bool use_tdev, bool timing_events, hipEvent_t* ev) {
+  const int* tp = use_tdev ? t_dev : nullptr;
+  if (timing_events) XF_HIP(hipEventRecord(ev[0], stream));
+  XF_TRY(launch_h<S>(t));
+  if (timing_events) XF_HIP(hipEventRecord(ev[1], stream));
+  XF_TRY(launch_post_h<S>(t, tp));
+  if (timing_events) XF_HIP(hipEventRecord(ev[2], stream));
+  XF_TRY(launch_e<S>(t));
+  if (timing_events) XF_HIP(hipEventRecord(ev[3], stream));
+  XF_TRY(launch_post_e<S>(t, tp));
+  if (timing_events) XF_HIP(hipEventRecord(ev[4], stream));
+  if (use_tdev) {
+    hipLaunchKernelGGL(k_next_step, dim3(1), dim3(1), 0, stream, t_dev);
+    XF_HIP(hipGetLastError());
+  }
+  return XF_OK;
+}
+
+int Engine::run(int64_t n) {
+  if (!uploaded) return fail(XF_ERR_STATE, "run before upload");
+  if (n < 0) return fail(XF_ERR_INVALID, "negative step count");
+  if (n == 0) return XF_OK;
+  if ((sc->tfsf || !nf.empty()) && cur_step + n > nt)
+    return fail(XF_ERR_INVALID, "run past the end of the time axis the TFSF/NF2FF tables were built for");
+  XF_HIP(hipSetDevice(device));
+  const bool timing_on = plan.timing != 0;
+  const bool graph_on = plan.use_graph != 0;
+  std::vector<hipEvent_t> ev;
+  hipEvent_t run0, run1;
+  XF_HIP(hipEventCreate(&run0));
+  XF_HIP(hipEventCreate(&run1));
+  if (timing_on) {
+    ev.resize(5 * n);
+    for (auto& e : ev) XF_HIP(hipEventCreate(&e));
+  }
+  XF_HIP(hipEventRecord(run0, stream));
+  int status = XF_OK;
+  auto step = [&](int64_t s) -> int {
+    const int t = static_cast<int>(cur_step + s);
+    hipEvent_t* e = timing_on ? &ev[5 * s] : nullptr;
+    switch (plan.precision) {
+      case XF_FP64:
+        return launch_step<double>(t, false, timing_on, e);
+      case XF_FP32:
+        return launch_step<float>(t, false, timing_on, e);
+      default:
+        return launch_step<__hip_bfloat16>(t, false, timing_on, e);
+    }
+  };
+  if (graph_on && !timing_on) {
+    if (!graph_ready) {
+      XF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      int st;
+      switch (plan.precision) {
+        case XF_FP64:
+          st = launch_step<double>(0, true, false, nullptr);
+          break;
+        case XF_FP32:
+          st = launch_step<float>(0, true, false, nullptr);
+          break;
+        default:
+          st = launch_step<__hip_bfloat16>(0, true, false, nullptr);
+      }
+      hipGraph_t g;
+      hipError_t ce = hipStreamEndCapture(stream, &g);
+      if (st != XF_OK) return st;
+      if (ce != hipSuccess) return fail(XF_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      graph = g;
+      XF_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+      graph_ready = true;
+    }
+    const int t0 = static_cast<int>(cur_step);
+    XF_HIP(hipMemcpyAsync(t_dev, &t0, sizeof(int), hipMemcpyHostToDevice, stream));
+    XF_HIP(hipStreamSynchronize(stream));
+    XF_HIP(hipEventRecord(run0, stream));
+    for (int64_t s = 0; s < n; ++s) XF_HIP(hipGraphLaunch(graph_exec, stream));
+  } else {
+    for (int64_t s = 0; s < n && status == XF_OK; ++s) status = step(s);
+  }
+  XF_HIP(hipEventRecord(run1, stream));
+  XF_HIP(hipEventSynchronize(run1));
+  if (status != XF_OK) return status;
+  XF_HIP(hipGetLastError());
+  float ms = 0;
+  XF_HIP(hipEventElapsedTime(&ms, run0, run1));
+  timing.run_ms = ms;
+  timing.steps += n;
+  timing.step_ms = ms / static_cast<double>(n);
+  if (timing_on) {
+    for (int64_t s = 0; s < n; ++s) {
+      float a, b, c, e;
+      XF_HIP(hipEventElapsedTime(&a, ev[5 * s], ev[5 * s + 1]));
+      XF_HIP(hipEventElapsedTime(&b, ev[5 * s + 1], ev[5 * s + 2]));
+      XF_HIP(hipEventElapsedTime(&c, ev[5 * s + 2], ev[5 * s + 3]));
+      XF_HIP(hipEventElapsedTime(&e, ev[5 * s + 3], ev[5 * s + 4]));
+      timing.yee_h_ms += a;
+      timing.tfsf_ms += b;
+      timing.yee_e_ms += c;
+      timing.nf2ff_ms += e;
+    }
+    timing.yee_h_launches += n;
+    timing.yee_e_launches += n;
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+  (void)hipEventDestroy(run0);
+  (void)hipEventDestroy(run1);
+  timing.bytes_per_cell_step = bytes_per_cell_step();
+  cur_step += n;
+  return XF_OK;
+}
+
+}  // namespace xf
+
+struct xf_handle {
+  xf::Engine eng;
+};
+
+extern "C" {
+
+const char* xf_last_error(void) { return xf::g_err.c_str(); }
+
+int xf_device_count(int* count) {
+  if (!count) return xf::fail(XF_ERR_INVALID, "null count");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return XF_OK;
+}
+
+int xf_create(xf_scene* scene, const xf_plan* plan, xf_handle** out) {
+  xf::g_err.clear();
+  if (!scene || !out) return xf::fail(XF_ERR_INVALID, "null scene or out");
+  *out = nullptr;
+  auto h = new (std::nothrow) xf_handle;
+  if (!h) return xf::fail(XF_ERR_NOMEM, "out of host memory");
+  h->eng.sc = scene;
+  if (plan) h->eng.plan = *plan;
+  if (h->eng.plan.precision < XF_FP64 || h->eng.plan.precision > XF_BF16_STORE) {
+    delete h;
+    return xf::fail(XF_ERR_INVALID, "unknown precision");
+  }
+  h->eng.device = h->eng.plan.device;
+  int st = h->eng.validate();
+  if (st == XF_OK) st = h->eng.check_overlaps();
+  if (st != XF_OK) {
+    delete h;
+    return st;
+  }
+  *out = h;
+  return XF_OK;
+}
+
+int xf_upload(xf_handle* h) {
+  xf::g_err.clear();
+  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return xf::fail(XF_ERR_HIP, "no HIP device");
+  if (h->eng.device < 0 || h->eng.device >= n) return xf::fail(XF_ERR_INVALID, "bad device ordinal");
+  const int st = h->eng.upload();
+  if (st != XF_OK) h->eng.release();
+  return st;
+}
+
+int xf_run(xf_handle* h, int64_t n_steps) {
+  xf::g_err.clear();
+  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+  return h->eng.run(n_steps);
+}
+
+int xf_download(xf_handle* h) {
+  xf::g_err.clear();
+  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+  return h->eng.download();
+}
+
+int xf_destroy(xf_handle* h) {
+  delete h;
+  return XF_OK;
+}
+
+int xf_current_step(const xf_handle* h, int64_t* step) {
+  if (!h || !step) return xf::fail(XF_ERR_INVALID, "null argument");
+  *step = h->eng.cur_step;
+  return XF_OK;
+}
+
+int xf_get_timing(const xf_handle* h, xf_timing* out) {
+  if (!h || !out) return xf::fail(XF_ERR_INVALID, "null argument");
+  *out = h->eng.timing;
+  out->bytes_per_cell_step = h->eng.bytes_per_cell_step();
+  return XF_OK;
+}
+
+int xf_reset_timing(xf_handle* h) {
+  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+  h->eng.timing = xf_timing{};
+  return XF_OK;
+}
+
+int xf_set_timing(xf_handle* h, int32_t on) {
+  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+  h->eng.plan.timing = on ? 1 : 0;
+  return XF_OK;
+}
+
+int xf_describe(const xf_handle* h, char* buf, size_t len) {
+  if (!h || !buf || len == 0) return xf::fail(XF_ERR_INVALID, "null argument");
+  std::snprintf(buf, len, "%s", h->eng.layout_desc.c_str());
+  return XF_OK;
+}
+
+}  // extern "C"

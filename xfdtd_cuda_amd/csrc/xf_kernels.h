@@ -1,0 +1,133 @@
+// xf_kernels.h -- device-side types of the gfx950 FDTD engine.
+//
+// Layout in HBM.  All field components share one padded node grid of
+// (nx+1) x (ny+1) x pz values (3-D; z fastest, pz = nz+1 rounded up to a
+// 128-byte multiple) or (nx+1) x py values (2-D TMz; y fastest), so one
+// linear index addresses Ex..Hz of the same (i, j, k) and every wave64 load
+// is a contiguous, aligned row segment.  Yee coefficients are compressed to
+// a per-cell uint8/uint16 index into a per-pass table of 9 values (the three
+// (self, curl_a, curl_b) triples of the pass), which replaces the 18 dense
+// per-cell arrays of the reference (fdtd_coefficient.cuh:149-167).
+#ifndef XF_KERNELS_H_
+#define XF_KERNELS_H_
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xf {
+
+constexpr int kMaxPml = 6;
+constexpr int kMaxTfsfJobs = 12;
+
+// Storage <-> compute conversion.  FP64 and FP32 store what they compute;
+// bf16 stores value * scale (scale per E/H) and computes in float.
+template <typename S>
+struct Store;
+template <>
+struct Store<double> {
+  using C = double;
+  static __device__ __forceinline__ C ld(const double* p, int64_t i, C) { return p[i]; }
+  static __device__ __forceinline__ void st(double* p, int64_t i, C v, C) { p[i] = v; }
+};
+template <>
+struct Store<float> {
+  using C = float;
+  static __device__ __forceinline__ C ld(const float* p, int64_t i, C) { return p[i]; }
+  static __device__ __forceinline__ void st(float* p, int64_t i, C v, C) { p[i] = v; }
+};
+template <>
+struct Store<__hip_bfloat16> {
+  using C = float;
+  static __device__ __forceinline__ C ld(const __hip_bfloat16* p, int64_t i, C inv) {
+    return __bfloat162float(p[i]) * inv;
+  }
+  static __device__ __forceinline__ void st(__hip_bfloat16* p, int64_t i, C v, C scale) {
+    p[i] = __float2bfloat16(v * scale);
+  }
+};
+
+template <typename S>
+struct Fields {
+  S* e[3];
+  S* h[3];
+};
+
+template <typename C>
+struct Scales {
+  C inv_e, inv_h;  // stored -> physical
+  C s_e, s_h;      // physical -> stored
+};
+
+struct Dims {
+  int nx, ny, nz;  // cells
+  int64_t sx, sy;  // strides of i and j (k stride is 1 in 3-D; unused in 2-D)
+};
+
+// One PML face for one pass (E or H).  Component A is E_A/H_A of the face's
+// (a, b, c) frame, B is E_B/H_B (pml_corrector.cu:50-270).
+template <typename C>
+struct PmlFace {
+  int axis;
+  int c0, c1;          // corrected nodes along the main axis
+  int a0, a1, b0, b1;  // lateral task (extended by one per component)
+  int gstart, nstart, off_c;
+  const C* ca;
+  const C* cb;
+  C* psi_a;
+  const C* cpsi_a;
+  C* psi_b;
+  const C* cpsi_b;
+  int64_t sa1, sa2, sb1, sb2;  // shape[1], shape[2] of the two psi arrays
+};
+
+template <typename C>
+struct PmlPass {
+  int n;
+  PmlFace<C> f[kMaxPml];
+};
+
+// A TFSF face correction of one component (one reference launch of
+// __tFSFcorrect, tfsf_corrector.cu:148-153) as a written-cell box.
+template <typename C>
+struct TfsfJob {
+  int comp, face;
+  int o[3], n[3];  // written box origin and extent
+  int64_t start;   // first flat thread index of the job
+  C coef;          // c{a|b}{axis} * coefficient_flag
+};
+
+template <typename C>
+struct TfsfPass {
+  int njob;
+  int is_e;
+  TfsfJob<C> job[kMaxTfsfJobs];
+  int64_t total;
+  const C* proj_int[3];
+  const C* proj_half[3];
+  const C* inc;  // e_inc for the H pass, h_inc for the E pass
+  int64_t n_aux;
+  int g[3];  // total_task start
+  C tr[3];   // transform_e (H pass) or transform_h (E pass)
+};
+
+struct NfFace {
+  int o[3], n[3];
+  int64_t start;  // first flat thread index
+  int64_t off;    // offset of the face in a frequency block (cells)
+};
+
+struct NfPass {
+  NfFace face[6];
+  int64_t total;     // surface cells over the 6 faces
+  int64_t block;     // cells of one frequency block (sum of face sizes)
+  int nfreq;
+  int64_t nt;
+  double2* cur;      // [freq][face cells][4] -> laid out [freq][face][comp][cell]
+  const double2* te; // [freq][nt]
+  const double2* th;
+};
+
+}  // namespace xf
+
+#endif  // XF_KERNELS_H_

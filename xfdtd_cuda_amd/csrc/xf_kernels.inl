@@ -1,0 +1,555 @@
+// xf_kernels.inl -- gfx950 kernels of the FDTD time step (included by
+// xf_engine.hip, one translation unit).
+//
+// All floating-point expressions keep the evaluation order of the reference
+// kernels and the file is built with -ffp-contract=off, so the FP64 path
+// rounds exactly like the CPU oracle built from the same order.
+#include "xf_kernels.h"
+
+namespace xf {
+
+// ------------------------------------------------------------------ PML
+// One face of correctPML (pml_corrector.cu:9-48) applied to the cell
+// (i, j, k) of the current pass; v/have/dirty hold the three components of
+// the pass being written (loaded lazily when not main-updated).
+template <int AX, bool ISE, bool TWO_D, typename S, typename C>
+__device__ __forceinline__ void pml_face(const PmlFace<C>& F, int i, int j, int k,
+                                         int64_t idx, const Fields<S>& f, const Dims& d,
+                                         const Scales<C>& sc, C (&v)[3], bool (&have)[3],
+                                         bool (&dirty)[3]) {
+  constexpr int A = (AX + 1) % 3, B = (AX + 2) % 3;
+  const int a = AX == 0 ? j : AX == 1 ? k : i;
+  const int b = AX == 0 ? k : AX == 1 ? i : j;
+  const int c = AX == 0 ? i : AX == 1 ? j : k;
+  if (c < F.c0 || c >= F.c1) return;
+  const bool in_a = ISE ? (a >= F.a0 && a < F.a1 && b >= F.b0 && b <= F.b1)
+                        : (a >= F.a0 && a <= F.a1 && b >= F.b0 && b < F.b1);
+  const bool in_b = ISE ? (a >= F.a0 && a <= F.a1 && b >= F.b0 && b < F.b1)
+                        : (a >= F.a0 && a < F.a1 && b >= F.b0 && b <= F.b1);
+  if (!in_a && !in_b) return;
+  const int64_t stc = AX == 0 ? d.sx : AX == 1 ? d.sy : 1;
+  const int ci = c + F.off_c - F.gstart;
+  const int layer = c - F.nstart;
+  const C ca = F.ca[ci], cb = F.cb[ci];
+  // psi index aBCToXYZ(a, b, layer)
+  const int px = AX == 0 ? layer : AX == 1 ? b : a;
+  const int py = AX == 0 ? a : AX == 1 ? layer : b;
+  const int pz = AX == 0 ? b : AX == 1 ? a : layer;
+  const C inv_self = ISE ? sc.inv_e : sc.inv_h;
+  const C inv_dual = ISE ? sc.inv_h : sc.inv_e;
+  S* const* self = ISE ? f.e : f.h;
+  S* const* dual = ISE ? f.h : f.e;
+  if (in_a && !(TWO_D && (self[A] == nullptr || dual[B] == nullptr))) {
+    const C dp = Store<S>::ld(dual[B], idx, inv_dual);
+    const C dq = Store<S>::ld(dual[B], ISE ? idx - stc : idx + stc, inv_dual);
+    const int64_t p = ((int64_t)px * F.sa1 + py) * F.sa2 + pz;
+    C psi = F.psi_a[p];
+    if (ISE)
+      psi = cb * psi + ca * (dp - dq);
+    else
+      psi = cb * psi + ca * (dq - dp);
+    F.psi_a[p] = psi;
+    if (!have[A]) {
+      v[A] = Store<S>::ld(self[A], idx, inv_self);
+      have[A] = true;
+    }
+    v[A] += F.cpsi_a[p] * psi;
+    dirty[A] = true;
+  }
+  if (in_b && !(TWO_D && (self[B] == nullptr || dual[A] == nullptr))) {
+    const C dp = Store<S>::ld(dual[A], idx, inv_dual);
+    const C dq = Store<S>::ld(dual[A], ISE ? idx - stc : idx + stc, inv_dual);
+    const int64_t p = ((int64_t)px * F.sb1 + py) * F.sb2 + pz;
+    C psi = F.psi_b[p];
+    if (ISE)
+      psi = cb * psi + ca * (dp - dq);
+    else
+      psi = cb * psi + ca * (dq - dp);
+    F.psi_b[p] = psi;
+    if (!have[B]) {
+      v[B] = Store<S>::ld(self[B], idx, inv_self);
+      have[B] = true;
+    }
+    v[B] += F.cpsi_b[p] * psi;
+    dirty[B] = true;
+  }
+}
+
+template <bool ISE, bool TWO_D, typename S, typename C>
+__device__ __forceinline__ void pml_all(const PmlPass<C>& P, int i, int j, int k, int64_t idx,
+                                        const Fields<S>& f, const Dims& d, const Scales<C>& sc,
+                                        C (&v)[3], bool (&have)[3], bool (&dirty)[3]) {
+  for (int q = 0; q < P.n; ++q) {
+    const PmlFace<C>& F = P.f[q];
+    if (F.axis == 0)
+      pml_face<0, ISE, TWO_D>(F, i, j, k, idx, f, d, sc, v, have, dirty);
+    else if (F.axis == 1)
+      pml_face<1, ISE, TWO_D>(F, i, j, k, idx, f, d, sc, v, have, dirty);
+    else
+      pml_face<2, ISE, TWO_D>(F, i, j, k, idx, f, d, sc, v, have, dirty);
+  }
+}
+
+// Loads the pass's coefficient table into LDS (or points at global memory).
+template <typename C, bool LDS>
+__device__ __forceinline__ const C* load_table(const C* __restrict__ g, int nval, C* smem) {
+  if (!LDS) return g;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthr = blockDim.x * blockDim.y;
+  for (int n = tid; n < nval; n += nthr) smem[n] = g[n];
+  __syncthreads();
+  return smem;
+}
+
+// ---------------------------------------------------------- 3-D Yee, H pass
+// BasicUpdator::updateH (basic_updator.cuh:16-32) with hNext
+// (update_scheme.cuh:58-63) for Hx, Hy, Hz fused, plus the H corrections of
+// every PML face.  A thread owns one (j, k) column and marches i through a
+// chunk, carrying Ey(i+1), Ez(i+1) as the next cell's Ey(i), Ez(i).
+// Block: 64 lanes along k x blockDim.y rows along j.
+template <typename S, typename I, bool LDS, bool PML>
+__global__ void __launch_bounds__(256)
+    k_yee_h3(Fields<S> f, const I* __restrict__ hid,
+             const typename Store<S>::C* __restrict__ htab, int ntab,
+             PmlPass<typename Store<S>::C> pml, Dims d, int chunk,
+             Scales<typename Store<S>::C> sc) {
+  using C = typename Store<S>::C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* tab = load_table<C, LDS>(htab, ntab * 9, reinterpret_cast<C*>(smem_raw));
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (k > d.nz || j > d.ny) return;
+  const int i0 = blockIdx.z * chunk;
+  const int i1 = min(i0 + chunk, d.nx + 1);
+  const bool jk_main = j < d.ny && k < d.nz;
+  int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
+  const C ie = sc.inv_e, ih = sc.inv_h;
+  C ey_i = 0, ez_i = 0;
+  if (jk_main && i0 < d.nx) {
+    ey_i = Store<S>::ld(f.e[1], idx, ie);
+    ez_i = Store<S>::ld(f.e[2], idx, ie);
+  }
+  for (int i = i0; i < i1; ++i, idx += d.sx) {
+    C v[3] = {0, 0, 0};
+    bool have[3] = {false, false, false};
+    bool dirty[3] = {false, false, false};
+    if (jk_main && i < d.nx) {
+      const C* t = tab + 9 * (int)hid[idx];
+      const C ex_i = Store<S>::ld(f.e[0], idx, ie);
+      const C ex_k = Store<S>::ld(f.e[0], idx + 1, ie);
+      const C ex_j = Store<S>::ld(f.e[0], idx + d.sy, ie);
+      const C ey_k = Store<S>::ld(f.e[1], idx + 1, ie);
+      const C ey_n = Store<S>::ld(f.e[1], idx + d.sx, ie);
+      const C ez_j = Store<S>::ld(f.e[2], idx + d.sy, ie);
+      const C ez_n = Store<S>::ld(f.e[2], idx + d.sx, ie);
+      const C hx = Store<S>::ld(f.h[0], idx, ih);
+      const C hy = Store<S>::ld(f.h[1], idx, ih);
+      const C hz = Store<S>::ld(f.h[2], idx, ih);
+      v[0] = t[0] * hx + t[1] * (ey_k - ey_i) + t[2] * (ez_j - ez_i);
+      v[1] = t[3] * hy + t[4] * (ez_n - ez_i) + t[5] * (ex_k - ex_i);
+      v[2] = t[6] * hz + t[7] * (ex_j - ex_i) + t[8] * (ey_n - ey_i);
+      have[0] = have[1] = have[2] = true;
+      dirty[0] = dirty[1] = dirty[2] = true;
+      ey_i = ey_n;
+      ez_i = ez_n;
+    }
+    if (PML) pml_all<false, false>(pml, i, j, k, idx, f, d, sc, v, have, dirty);
+    if (dirty[0]) Store<S>::st(f.h[0], idx, v[0], sc.s_h);
+    if (dirty[1]) Store<S>::st(f.h[1], idx, v[1], sc.s_h);
+    if (dirty[2]) Store<S>::st(f.h[2], idx, v[2], sc.s_h);
+  }
+}
+
+// ---------------------------------------------------------- 3-D Yee, E pass
+// BasicUpdator3D::updateE (basic_updator_3d.cu:10-34) with eNext
+// (update_scheme.cuh:27-32, skipping a == 0 || b == 0) fused over Ex, Ey,
+// Ez, plus the E corrections of every PML face.  Carries Hy(i), Hz(i) as the
+// next cell's Hy(i-1), Hz(i-1).  Cells whose table row is flagged dispersive
+// are left to the ADE kernel.
+template <typename S, typename I, bool LDS, bool PML, bool ADE>
+__global__ void __launch_bounds__(256)
+    k_yee_e3(Fields<S> f, const I* __restrict__ eid,
+             const typename Store<S>::C* __restrict__ etab, int ntab,
+             const unsigned char* __restrict__ eflag, PmlPass<typename Store<S>::C> pml,
+             Dims d, int chunk, Scales<typename Store<S>::C> sc) {
+  using C = typename Store<S>::C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* tab = load_table<C, LDS>(etab, ntab * 9, reinterpret_cast<C*>(smem_raw));
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (k > d.nz || j > d.ny) return;
+  const int i0 = blockIdx.z * chunk;
+  const int i1 = min(i0 + chunk, d.nx + 1);
+  const bool jk_main = j < d.ny && k < d.nz;
+  int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
+  const C ie = sc.inv_e, ih = sc.inv_h;
+  C hy_m = 0, hz_m = 0;  // H at i-1
+  if (jk_main && i0 > 0 && i0 < d.nx) {
+    hy_m = Store<S>::ld(f.h[1], idx - d.sx, ih);
+    hz_m = Store<S>::ld(f.h[2], idx - d.sx, ih);
+  }
+  for (int i = i0; i < i1; ++i, idx += d.sx) {
+    C v[3] = {0, 0, 0};
+    bool have[3] = {false, false, false};
+    bool dirty[3] = {false, false, false};
+    if (jk_main && i < d.nx) {
+      const int id = (int)eid[idx];
+      const C hy = Store<S>::ld(f.h[1], idx, ih);
+      const C hz = Store<S>::ld(f.h[2], idx, ih);
+      if (!(ADE && eflag[id])) {
+        const C* t = tab + 9 * id;
+        const C hx = Store<S>::ld(f.h[0], idx, ih);
+        if (j > 0 && k > 0) {
+          const C hy_k = Store<S>::ld(f.h[1], idx - 1, ih);
+          const C hz_j = Store<S>::ld(f.h[2], idx - d.sy, ih);
+          v[0] = t[0] * Store<S>::ld(f.e[0], idx, ie) + t[1] * (hy - hy_k) + t[2] * (hz - hz_j);
+          have[0] = dirty[0] = true;
+        }
+        if (k > 0 && i > 0) {
+          const C hx_k = Store<S>::ld(f.h[0], idx - 1, ih);
+          v[1] = t[3] * Store<S>::ld(f.e[1], idx, ie) + t[4] * (hz - hz_m) + t[5] * (hx - hx_k);
+          have[1] = dirty[1] = true;
+        }
+        if (i > 0 && j > 0) {
+          const C hx_j = Store<S>::ld(f.h[0], idx - d.sy, ih);
+          v[2] = t[6] * Store<S>::ld(f.e[2], idx, ie) + t[7] * (hx - hx_j) + t[8] * (hy - hy_m);
+          have[2] = dirty[2] = true;
+        }
+      }
+      hy_m = hy;
+      hz_m = hz;
+    }
+    if (PML) pml_all<true, false>(pml, i, j, k, idx, f, d, sc, v, have, dirty);
+    if (dirty[0]) Store<S>::st(f.e[0], idx, v[0], sc.s_e);
+    if (dirty[1]) Store<S>::st(f.e[1], idx, v[1], sc.s_e);
+    if (dirty[2]) Store<S>::st(f.e[2], idx, v[2], sc.s_e);
+  }
+}
+
+// ------------------------------------------------------- 2-D TMz, H and E
+// The TE updator (basic_updator_te.cu:9-20, basic_updator.cuh:16-32) on a
+// 2-D grid (nz = 1): Ex, Ey and Hz stay exactly zero there, so only Hx, Hy
+// and Ez are stored and updated; the dropped terms are products with zero
+// differences.  One thread per (i, j) cell, 64 lanes along j.
+template <typename S, typename I, bool LDS, bool PML>
+__global__ void __launch_bounds__(256)
+    k_yee_h2(Fields<S> f, const I* __restrict__ hid,
+             const typename Store<S>::C* __restrict__ htab, int ntab,
+             PmlPass<typename Store<S>::C> pml, Dims d, Scales<typename Store<S>::C> sc) {
+  using C = typename Store<S>::C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* tab = load_table<C, LDS>(htab, ntab * 9, reinterpret_cast<C*>(smem_raw));
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (j > d.ny || i > d.nx) return;
+  const int64_t idx = (int64_t)i * d.sx + j;
+  C v[3] = {0, 0, 0};
+  bool have[3] = {false, false, false};
+  bool dirty[3] = {false, false, false};
+  if (i < d.nx && j < d.ny) {
+    const C* t = tab + 9 * (int)hid[idx];
+    const C ez = Store<S>::ld(f.e[2], idx, sc.inv_e);
+    const C ez_j = Store<S>::ld(f.e[2], idx + 1, sc.inv_e);
+    const C ez_n = Store<S>::ld(f.e[2], idx + d.sx, sc.inv_e);
+    const C zero = 0;
+    // Hx: chxh*Hx + chxey*(Ey(k+1)-Ey(k)) + chxez*(Ez(j+1)-Ez(j)), Ey = 0
+    v[0] = t[0] * Store<S>::ld(f.h[0], idx, sc.inv_h) + t[1] * zero + t[2] * (ez_j - ez);
+    // Hy: chyh*Hy + chyez*(Ez(i+1)-Ez(i)) + chyex*(Ex(k+1)-Ex(k)), Ex = 0
+    v[1] = t[3] * Store<S>::ld(f.h[1], idx, sc.inv_h) + t[4] * (ez_n - ez) + t[5] * zero;
+    have[0] = have[1] = dirty[0] = dirty[1] = true;
+  }
+  if (PML) pml_all<false, true>(pml, i, j, 0, idx, f, d, sc, v, have, dirty);
+  if (dirty[0]) Store<S>::st(f.h[0], idx, v[0], sc.s_h);
+  if (dirty[1]) Store<S>::st(f.h[1], idx, v[1], sc.s_h);
+}
+
+template <typename S, typename I, bool LDS, bool PML>
+__global__ void __launch_bounds__(256)
+    k_yee_e2(Fields<S> f, const I* __restrict__ eid,
+             const typename Store<S>::C* __restrict__ etab, int ntab,
+             PmlPass<typename Store<S>::C> pml, Dims d, Scales<typename Store<S>::C> sc) {
+  using C = typename Store<S>::C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* tab = load_table<C, LDS>(etab, ntab * 9, reinterpret_cast<C*>(smem_raw));
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (j > d.ny || i > d.nx) return;
+  const int64_t idx = (int64_t)i * d.sx + j;
+  C v[3] = {0, 0, 0};
+  bool have[3] = {false, false, false};
+  bool dirty[3] = {false, false, false};
+  if (i < d.nx && j < d.ny && i > 0 && j > 0) {
+    const C* t = tab + 9 * (int)eid[idx];
+    const C hx = Store<S>::ld(f.h[0], idx, sc.inv_h);
+    const C hx_j = Store<S>::ld(f.h[0], idx - 1, sc.inv_h);
+    const C hy = Store<S>::ld(f.h[1], idx, sc.inv_h);
+    const C hy_i = Store<S>::ld(f.h[1], idx - d.sx, sc.inv_h);
+    v[2] = t[6] * Store<S>::ld(f.e[2], idx, sc.inv_e) + t[7] * (hx - hx_j) + t[8] * (hy - hy_i);
+    have[2] = dirty[2] = true;
+  }
+  if (PML) pml_all<true, true>(pml, i, j, 0, idx, f, d, sc, v, have, dirty);
+  if (dirty[2]) Store<S>::st(f.e[2], idx, v[2], sc.s_e);
+}
+
+// ------------------------------------------------------------------ TFSF
+// exInc..hzInc (tfsf_corrector.cu:18-130): interpolated 1-D incident value.
+template <typename C>
+__device__ __forceinline__ C tfsf_inc(const TfsfPass<C>& P, bool inc_is_e, int comp, int t,
+                                      int i, int j, int k) {
+  const int li = i - P.g[0], lj = j - P.g[1], lk = k - P.g[2];
+  C proj;
+  if (inc_is_e) {
+    if (comp == 0)
+      proj = P.proj_half[0][li + 1] + P.proj_int[1][lj] + P.proj_int[2][lk];
+    else if (comp == 1)
+      proj = P.proj_int[0][li] + P.proj_half[1][lj + 1] + P.proj_int[2][lk];
+    else
+      proj = P.proj_int[0][li] + P.proj_int[1][lj] + P.proj_half[2][lk + 1];
+  } else {
+    if (comp == 0)
+      proj = P.proj_int[0][li] + P.proj_half[1][lj + 1] + P.proj_half[2][lk + 1] - (C)0.5;
+    else if (comp == 1)
+      proj = P.proj_half[0][li + 1] + P.proj_int[1][lj] + P.proj_half[2][lk + 1] - (C)0.5;
+    else
+      proj = P.proj_half[0][li + 1] + P.proj_half[1][lj + 1] + P.proj_int[2][lk] - (C)0.5;
+  }
+  const uint64_t index = (uint64_t)proj;
+  const C weight = proj - (C)index;
+  const C* row = P.inc + (int64_t)t * P.n_aux;
+  return P.tr[comp] * ((1 - weight) * row[index] + weight * row[index + 1]);
+}
+
+// All TFSF face corrections of one pass in one launch.  Each thread owns a
+// written cell of one job; cells shared by several faces of the same
+// component are applied by the first of them, in the reference's launch
+// order (TFSFCorrector3DAgency/2DAgency, tfsf_corrector.cu:155-271).
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_tfsf(Fields<S> f, TfsfPass<typename Store<S>::C> P, int t_arg, const int* t_dev,
+           Scales<typename Store<S>::C> sc, Dims d, int two_d) {
+  using C = typename Store<S>::C;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= P.total) return;
+  const int t = t_dev ? *t_dev : t_arg;
+  int q = 0;
+  while (q + 1 < P.njob && tid >= P.job[q + 1].start) ++q;
+  const TfsfJob<C>& J = P.job[q];
+  int64_t r = tid - J.start;
+  const int lk = (int)(r % J.n[2]);
+  r /= J.n[2];
+  const int lj = (int)(r % J.n[1]);
+  const int li = (int)(r / J.n[1]);
+  const int cell[3] = {J.o[0] + li, J.o[1] + lj, J.o[2] + lk};
+  auto inside = [&](const TfsfJob<C>& X) {
+    return cell[0] >= X.o[0] && cell[0] < X.o[0] + X.n[0] && cell[1] >= X.o[1] &&
+           cell[1] < X.o[1] + X.n[1] && cell[2] >= X.o[2] && cell[2] < X.o[2] + X.n[2];
+  };
+  for (int p = 0; p < q; ++p)
+    if (P.job[p].comp == J.comp && inside(P.job[p])) return;
+  const int64_t idx = two_d ? (int64_t)cell[0] * d.sx + cell[1]
+                            : (int64_t)cell[0] * d.sx + (int64_t)cell[1] * d.sy + cell[2];
+  S* field = P.is_e ? f.e[J.comp] : f.h[J.comp];
+  const C inv = P.is_e ? sc.inv_e : sc.inv_h;
+  const C scl = P.is_e ? sc.s_e : sc.s_h;
+  C v = Store<S>::ld(field, idx, inv);
+  for (int p = q; p < P.njob; ++p) {
+    const TfsfJob<C>& X = P.job[p];
+    if (X.comp != J.comp || !inside(X)) continue;
+    const int axis = X.face / 2;
+    const bool negative = (X.face % 2) == 0;
+    const int a_axis = (axis + 1) % 3, b_axis = (axis + 2) % 3;
+    const int dual = (X.comp == a_axis) ? b_axis : a_axis;
+    int pos[3] = {cell[0], cell[1], cell[2]};
+    // E pass, negative face: incident H read at c-1; H pass, negative face:
+    // H written at c-1 from incident E at c.
+    if (negative) pos[axis] += P.is_e ? -1 : 1;
+    const C inc = tfsf_inc(P, !P.is_e, dual, t, pos[0], pos[1], pos[2]);
+    v += inc * X.coef;
+  }
+  Store<S>::st(field, idx, v, scl);
+}
+
+// ----------------------------------------------------------------- NF2FF
+template <typename S>
+__device__ __forceinline__ typename Store<S>::C fld(const S* p, int64_t idx,
+                                                    typename Store<S>::C inv) {
+  return p ? Store<S>::ld(p, idx, inv) : (typename Store<S>::C)0;
+}
+
+// interpolateSurfaceCenter<xyz, field> (interpolate_scheme.cuh:11-134).
+template <typename S>
+__device__ __forceinline__ typename Store<S>::C nf_interp(const Fields<S>& f, const Dims& d,
+                                                          int two_d, int face_axis, bool is_e,
+                                                          int comp, int i, int j, int k,
+                                                          const Scales<typename Store<S>::C>& sc) {
+  using C = typename Store<S>::C;
+  const S* p = is_e ? f.e[comp] : f.h[comp];
+  const C inv = is_e ? sc.inv_e : sc.inv_h;
+  const int64_t sx = d.sx, sy = two_d ? 1 : d.sy, sz = two_d ? 0 : 1;
+  const int64_t id = (int64_t)i * sx + (int64_t)j * sy + (int64_t)k * sz;
+  if (p == nullptr) return 0;
+  auto F = [&](int di, int dj, int dk) { return fld(p, id + di * sx + dj * sy + dk * sz, inv); };
+  if (face_axis == 0) {
+    if (!is_e && comp == 2) return (C)0.25 * (F(0, 0, 1) + F(0, 0, 0) + F(-1, 0, 1) + F(-1, 0, 0));
+    if (!is_e && comp == 1) return (C)0.25 * (F(0, 1, 0) + F(0, 0, 0) + F(-1, 1, 0) + F(-1, 0, 0));
+    if (is_e && comp == 2) return (C)0.5 * (F(0, 1, 0) + F(0, 0, 0));
+    return (C)0.5 * (F(0, 0, 1) + F(0, 0, 0));
+  } else if (face_axis == 1) {
+    if (!is_e && comp == 2) return (C)0.25 * (F(0, 0, 1) + F(0, 0, 0) + F(0, -1, 1) + F(0, -1, 0));
+    if (!is_e && comp == 0) return (C)0.25 * (F(1, 0, 0) + F(0, 0, 0) + F(1, -1, 0) + F(0, -1, 0));
+    if (is_e && comp == 2) return (C)0.5 * (F(1, 0, 0) + F(0, 0, 0));
+    return (C)0.5 * (F(0, 0, 1) + F(0, 0, 0));
+  }
+  if (!is_e && comp == 0) return (C)0.25 * (F(1, 0, 0) + F(0, 0, 0) + F(0, 0, -1) + F(1, 0, -1));
+  if (!is_e && comp == 1) return (C)0.25 * (F(0, 0, 0) + F(0, 1, 0) + F(0, 0, -1) + F(0, 1, -1));
+  if (is_e && comp == 0) return (C)0.5 * (F(0, 1, 0) + F(0, 0, 0));
+  return (C)0.5 * (F(1, 0, 0) + F(0, 0, 0));
+}
+
+// calculateJ / calculateM of all six faces and all frequencies in one launch
+// (nf2ff_frequency_domain.data.cu:80-232).  The four interpolated tangential
+// fields are formed once per surface cell and accumulated into every
+// frequency's complex FP64 sums.
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_nf2ff(Fields<S> f, NfPass P, int t_arg, const int* t_dev, Dims d, int two_d,
+            Scales<typename Store<S>::C> sc) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= P.total) return;
+  const int t = t_dev ? *t_dev : t_arg;
+  int face = 0;
+  while (face < 5 && tid >= P.face[face + 1].start) ++face;
+  const NfFace& F = P.face[face];
+  int64_t r = tid - F.start;
+  const int64_t cell = r;
+  const int lk = (int)(r % F.n[2]);
+  r /= F.n[2];
+  const int lj = (int)(r % F.n[1]);
+  const int li = (int)(r / F.n[1]);
+  const int i = F.o[0] + li, j = F.o[1] + lj, k = F.o[2] + lk;
+  const int axis = face / 2;
+  const bool positive = face % 2;
+  const int a_axis = (axis + 1) % 3, b_axis = (axis + 2) % 3;
+  const double hb = (double)nf_interp(f, d, two_d, axis, false, b_axis, i, j, k, sc);
+  const double ha = (double)nf_interp(f, d, two_d, axis, false, a_axis, i, j, k, sc);
+  const double eb = (double)nf_interp(f, d, two_d, axis, true, b_axis, i, j, k, sc);
+  const double ea = (double)nf_interp(f, d, two_d, axis, true, a_axis, i, j, k, sc);
+  const double r_ja = (positive ? -1.0 : 1.0) * hb;
+  const double r_jb = (positive ? 1.0 : -1.0) * ha;
+  const double r_ma = (positive ? 1.0 : -1.0) * eb;
+  const double r_mb = (positive ? -1.0 : 1.0) * ea;
+  const int64_t nface = (int64_t)F.n[0] * F.n[1] * F.n[2];
+  for (int q = 0; q < P.nfreq; ++q) {
+    const double2 th = P.th[(int64_t)q * P.nt + t];
+    const double2 te = P.te[(int64_t)q * P.nt + t];
+    double2* base = P.cur + (int64_t)q * P.block * 4 + F.off * 4 + cell;
+    double2 v;
+    v = base[0];
+    v.x += r_ja * th.x;
+    v.y += r_ja * th.y;
+    base[0] = v;
+    v = base[nface];
+    v.x += r_jb * th.x;
+    v.y += r_jb * th.y;
+    base[nface] = v;
+    v = base[2 * nface];
+    v.x += r_ma * te.x;
+    v.y += r_ma * te.y;
+    base[2 * nface] = v;
+    v = base[3 * nface];
+    v.x += r_mb * te.x;
+    v.y += r_mb * te.y;
+    base[3 * nface] = v;
+  }
+}
+
+// ------------------------------------------------------------------ ADE
+// Device state of the dispersive cells, compacted (the reference keeps
+// full-grid 4-D arrays, ade_method.cuh:176-182).  Per-cell and per-pole
+// coefficients and the J recursion stay in FP64.
+struct AdeDev {
+  int kind;
+  int np;
+  int64_t n;                // dispersive cells
+  const int64_t* idx;       // padded-grid index of each cell
+  const int* ijk;           // (i, j, k) of each cell
+  const double* cejs;       // [n]
+  const double* ceep;       // [n]
+  const double* cjj;        // [np][n]
+  const double* cjjp;
+  const double* cjen;
+  const double* cje;
+  const double* cjep;
+  const double* cjsj;
+  const double* cjsjp;
+  double* eprev;            // [3][n]
+  double* jc;               // [3][np][n]
+  double* jp;               // [3][np][n]
+};
+
+// TemplateADEUpdateScheme::updateE (template_ade_update_scheme.cuh:15-86)
+// for the three components of each dispersive cell, with the model's
+// calculateJSum / updateJ / recordEPrevious.
+template <typename S, typename I>
+__global__ void __launch_bounds__(256)
+    k_ade_e3(Fields<S> f, const I* __restrict__ eid,
+             const typename Store<S>::C* __restrict__ etab, AdeDev A, Dims d,
+             Scales<typename Store<S>::C> sc) {
+  using C = typename Store<S>::C;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= A.n) return;
+  const int64_t idx = A.idx[c];
+  const int i = A.ijk[3 * c], j = A.ijk[3 * c + 1], k = A.ijk[3 * c + 2];
+  const C* t = etab + 9 * (int)eid[idx];
+  const int cell[3] = {i, j, k};
+  const int64_t st[3] = {d.sx, d.sy, 1};
+  for (int comp = 0; comp < 3; ++comp) {
+    const int a_axis = (comp + 1) % 3, b_axis = (comp + 2) % 3;
+    const int a = cell[a_axis], b = cell[b_axis];
+    if (a == 0 || b == 0) continue;
+    // eNext with ha = H_a (difference along b), hb = H_b (along a)
+    const double ha_p = (double)Store<S>::ld(f.h[a_axis], idx, sc.inv_h);
+    const double ha_q = (double)Store<S>::ld(f.h[a_axis], idx - st[b_axis], sc.inv_h);
+    const double hb_p = (double)Store<S>::ld(f.h[b_axis], idx, sc.inv_h);
+    const double hb_q = (double)Store<S>::ld(f.h[b_axis], idx - st[a_axis], sc.inv_h);
+    const double e_cur = (double)Store<S>::ld(f.e[comp], idx, sc.inv_e);
+    double j_sum = 0;
+    for (int p = 0; p < A.np; ++p) {
+      const int64_t pj = ((int64_t)comp * A.np + p) * A.n + c;
+      const int64_t pc = (int64_t)p * A.n + c;
+      if (A.kind == 3)
+        j_sum += A.cjsj[pc] * A.jc[pj] + A.cjsjp[pc] * A.jp[pj];
+      else
+        j_sum += A.cjsj[pc] * A.jc[pj];
+    }
+    double e_prev = 0.0, ceep = 0.0;
+    if (A.kind == 3) {
+      e_prev = A.eprev[(int64_t)comp * A.n + c];
+      ceep = A.ceep[c];
+    }
+    const double tc = (double)t[3 * comp], ta = (double)t[3 * comp + 1],
+                 tb = (double)t[3 * comp + 2];
+    const double e_new = ceep * e_prev + (tc * e_cur + ta * (ha_p - ha_q) + tb * (hb_p - hb_q)) +
+                         A.cejs[c] * j_sum;
+    Store<S>::st(f.e[comp], idx, (C)e_new, sc.s_e);
+    for (int p = 0; p < A.np; ++p) {
+      const int64_t pj = ((int64_t)comp * A.np + p) * A.n + c;
+      const int64_t pc = (int64_t)p * A.n + c;
+      if (A.kind == 1) {
+        A.jc[pj] = A.cjj[pc] * A.jc[pj] + A.cje[pc] * (e_new + e_cur);
+      } else if (A.kind == 2) {
+        A.jc[pj] = A.cjj[pc] * A.jc[pj] + A.cje[pc] * (e_new - e_cur);
+      } else {
+        const double jn = A.cjen[pc] * e_new + A.cje[pc] * e_cur + A.cjep[pc] * e_prev +
+                          A.cjj[pc] * A.jc[pj] + A.cjjp[pc] * A.jp[pj];
+        A.jp[pj] = A.jc[pj];
+        A.jc[pj] = jn;
+      }
+    }
+    if (A.kind == 3) A.eprev[(int64_t)comp * A.n + c] = e_cur;
+  }
+}
+
+__global__ void k_next_step(int* t) { *t += 1; }
+
+}  // namespace xf

@@ -1,0 +1,125 @@
+"""The reference drivers' scenes as builders of Simulation objects.
+
+dielectric_sphere   test/dielectric_sphere_scatter.cu:20-107 at dl = 2.5e-3:
+                    a 0.35 m vacuum cube (140^3 cells) + 8-cell CPML on six
+                    faces, a r = 0.1 m sphere (eps_r 3, mu_r 2), a Gaussian
+                    TFSF plane wave (tau = 20 dl / 6e8, t0 = 4.5 tau) at a
+                    15-cell inset, a 1 GHz NF2FF box at an 11-cell inset.
+vacuum_benchmark    test/benchmark.cu:105-163 (type 0..4).
+cylinder_2d         test/cylinder_scatter_2d.cu:28-183 in the 1000 x 1000,
+                    dl = 1e-3 form: PEC cylinder r = 0.1 m, 10-cell CPML.
+large_sphere        the dielectric sphere scaled to n^3 cells (+ pml), sphere
+                    radius 0.35 * edge, eps_r ~ U[2, 6] from a seed.
+dispersive_sphere   Drude core + Lorentz shell, n^3 cells + 10-cell CPML.
+
+Insets the reference does not fix (2-D NF2FF/TFSF, scaled boxes) are chosen
+to keep the CPML < NF2FF < TFSF order of the 3-D driver.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .simulation import FACES, Material, Simulation, Waveform
+
+
+def dielectric_sphere(n: int = 140, pml: int = 8, steps: int = 2400, dl: float = 2.5e-3,
+                      tfsf_inset: int = 15, nf2ff_inset: int = 11, freqs=(1e9,),
+                      radius: float | None = None, eps_r: float = 3.0, mu_r: float = 2.0,
+                      layout: str = "dense", init: bool = True) -> Simulation:
+    """n interior cells per edge (140 reproduces the driver's 0.35 m cube)."""
+    edge = n * dl
+    half = edge / 2
+    radius = (0.1 / 0.35) * edge if radius is None else radius
+    s = Simulation(dl, dl, dl, 0.9)
+    s.add_cube("domain", (-half, -half, -half), (edge, edge, edge), Material.air())
+    s.add_sphere("dielectric_sphere", (0, 0, 0), radius, Material.dielectric(eps_r, mu_r))
+    tau = (dl * 20) / 6e8
+    s.add_tfsf3d((tfsf_inset,) * 3, 0.0, 0.0, 0.0, Waveform.gaussian(tau, 4.5 * tau))
+    if pml > 0:
+        s.add_pml_all(pml)
+    if nf2ff_inset > 0 and freqs:
+        s.add_nf2ff_fd((nf2ff_inset,) * 3, list(freqs))
+    s.set_coefficient_layout(layout)
+    if init:
+        s.init(steps)
+    return s
+
+
+def vacuum_benchmark(kind: int = 4, n: int = 128, dl: float = 2.5e-3, steps: int = 2400,
+                     layout: str = "dense", init: bool = True) -> Simulation:
+    """benchmark --type kind: 0 updator only, 1 +TFSF, 2 +PML, 3 +TFSF+PML,
+    4 +TFSF+PML+NF2FF, on an n^3 vacuum cube (+8-cell PML when used)."""
+    half = n * dl / 2
+    s = Simulation(dl, dl, dl, 0.9)
+    s.add_cube("domain", (-half, -half, -half), (n * dl,) * 3, Material.air())
+    if kind >= 4:
+        s.add_nf2ff_fd((11, 11, 11), [1e9])
+    if kind in (2, 3, 4):
+        s.add_pml_all(8)
+    if kind in (1, 3, 4):
+        tau = dl * 20 / 6e8
+        s.add_tfsf3d((15, 15, 15), 0.0, 0.0, 0.0, Waveform.gaussian(tau, 4.5 * tau))
+    s.set_coefficient_layout(layout)
+    if init:
+        s.init(steps)
+    return s
+
+
+def cylinder_2d(n: int = 1000, pml: int = 10, steps: int = 4000, dl: float = 1e-3,
+                radius: float | None = None, tfsf_inset: int = 15, nf2ff_inset: int = 12,
+                freqs=(1e9,), phi: float = 0.0, init: bool = True) -> Simulation:
+    """2-D TMz PEC cylinder (n x n cells + pml on four faces)."""
+    edge = n * dl
+    half = edge / 2
+    radius = 0.1 * edge if radius is None else radius
+    s = Simulation(dl, dl, 1.0, 0.9)
+    s.add_cube("domain", (-half, -half, -np.inf), (edge, edge, np.inf), Material.air())
+    s.add_cylinder("cylinder", (0.0, 0.0, -np.inf), radius, np.inf, Material.pec())
+    tau = 20 * dl / 6e8
+    s.add_tfsf2d((tfsf_inset, tfsf_inset), phi, Waveform.gaussian(tau, 4.5 * tau))
+    if pml > 0:
+        s.add_pml_all(pml, faces=("XN", "XP", "YN", "YP"))
+    if nf2ff_inset > 0 and freqs:
+        s.add_nf2ff_fd((nf2ff_inset, nf2ff_inset, 0), list(freqs))
+    if init:
+        s.init(steps)
+    return s
+
+
+def large_sphere(n: int = 512, pml: int = 12, steps: int = 4000, seed: int = 2024,
+                 dl: float = 2.5e-3, layout: str = "compact", freqs=(1e9,),
+                 init: bool = True) -> Simulation:
+    """The dielectric-sphere scene scaled to n^3 (+2*pml) cells; sphere radius
+    0.35 * domain edge, eps_r drawn from U[2, 6] with numpy's default_rng(seed).
+    TF/SF and NF2FF insets keep the 8 < 11 < 15 gaps of the 140^3 driver."""
+    eps_r = float(np.random.default_rng(seed).uniform(2.0, 6.0))
+    edge = n * dl
+    return dielectric_sphere(n=n, pml=pml, steps=steps, dl=dl, tfsf_inset=pml + 7,
+                             nf2ff_inset=pml + 3, freqs=freqs, radius=0.35 * edge,
+                             eps_r=eps_r, mu_r=1.0, layout=layout, init=init)
+
+
+def dispersive_sphere(n: int = 256, pml: int = 10, steps: int = 6000, dl: float = 2.5e-3,
+                      freqs=(0.5e9, 1e9, 2e9), init: bool = True) -> Simulation:
+    """Drude core (omega_p = 2 pi 2 GHz, gamma = 2 pi 0.2 GHz) inside a
+    one-pole Lorentz shell (delta_eps = 2, omega_0 = 2 pi 1.5 GHz,
+    delta = 2 pi 0.1 GHz).  Radii: shell 0.3 * edge, core 0.2 * edge."""
+    edge = n * dl
+    half = edge / 2
+    s = Simulation(dl, dl, dl, 0.9)
+    s.add_cube("domain", (-half, -half, -half), (edge,) * 3, Material.air())
+    tp = 2 * np.pi
+    s.add_sphere("shell", (0, 0, 0), 0.3 * edge,
+                 Material.lorentz(1.0, [2.0], [tp * 1.5e9], [tp * 0.1e9]))
+    s.add_sphere("core", (0, 0, 0), 0.2 * edge, Material.drude(1.0, [tp * 2e9], [tp * 0.2e9]))
+    tau = 20 * dl / 6e8
+    s.add_tfsf3d((pml + 7,) * 3, 0.0, 0.0, 0.0, Waveform.gaussian(tau, 4.5 * tau))
+    s.add_pml_all(pml)
+    s.add_nf2ff_fd((pml + 3,) * 3, list(freqs))
+    if init:
+        s.init(steps)
+    return s
+
+
+__all__ = ["FACES", "cylinder_2d", "dielectric_sphere", "dispersive_sphere", "large_sphere",
+           "vacuum_benchmark"]
