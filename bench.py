@@ -1,0 +1,170 @@
+"""Benchmark of the FDTD time step on one MI355X.
+
+Workload: the large dielectric-sphere scene -- 512^3 cells + a 12-cell CPML
+on all six faces (536^3 = 153,990,656 cells), sphere of radius 0.35 x edge
+with eps_r drawn from U[2, 6] (seed 2024), Gaussian TFSF plane wave, 1 GHz
+NF2FF box -- stepped in FP64 by the HIP engine.  A step is one full time
+step (H pass + CPML, TFSF, E pass + CPML, TFSF, NF2FF accumulation) over the
+whole grid; the metric is Yee cell-updates per second (cells x steps / s,
+cells including the CPML, as the reference's benchmark counts them,
+test/benchmark.cu:209-216).
+
+The timed region is exactly --steps steps after --warmup untimed ones, with
+fields, coefficients and tables already resident in HBM.  Per-kernel HIP
+events recorded on the engine's stream inside that region give the average
+duration of the dominant kernel, from which the roofline fraction is formed
+with the algorithmic bytes of one launch.  The CPU baseline is the oracle
+(a C restatement of the reference kernels, OpenMP over x) timed on a few
+steps of the same scene on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
+
+
+def pml_bytes(sim, pass_name: str, cbytes: int) -> int:
+    """Algorithmic CPML bytes of one pass: for every psi element updated,
+    read psi, write psi, read c_psi (the four psi arrays of a face hold
+    exactly the corrected ranges)."""
+    import xfdtd_cuda_amd as xf
+    names = ("ea_psi_hb", "eb_psi_ha") if pass_name == "e" else ("ha_psi_eb", "hb_psi_ea")
+    total = 0
+    for p in range(6):
+        try:
+            for n in names:
+                total += sim.array(f"pml.{p}.{n}").size
+        except xf.XFError:
+            break
+    return total * 3 * cbytes
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"])
+    ap.add_argument("--n", type=int, default=512, help="interior cells per edge")
+    ap.add_argument("--pml", type=int, default=12)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-stream", action="store_true")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this "
+              "build (see DESIGN.md for how the path would shard)", file=sys.stderr)
+        return 2
+
+    import xfdtd_cuda_amd as xf
+    from xfdtd_cuda_amd import _native, scenes
+
+    if xf.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+
+    K, W = args.steps, args.warmup
+    total_steps = W + K + (0 if args.no_cpu else args.cpu_steps) + 1
+    t_setup = time.perf_counter()
+    sim = scenes.large_sphere(n=args.n, pml=args.pml, steps=total_steps, layout="compact")
+    (nx, ny, nz), dt, _ = sim.info()
+    cells = nx * ny * nz
+    eng = xf.Engine(sim, precision=args.precision, timing=False).upload()
+    setup_s = time.perf_counter() - t_setup
+    print(f"# setup {setup_s:.1f} s, grid {nx}x{ny}x{nz}, {eng.describe()}", file=sys.stderr,
+          flush=True)
+
+    copy_gbps = None
+    if not args.no_stream:
+        # one 16-byte element per lane, non-temporal: the fastest copy shape
+        # measured on MI355X (grid-stride copy loops reach ~4.8 TB/s only)
+        copy_gbps = _native.stream_bench(2 << 30, 10, 0, 6)
+        print(f"# stream copy {copy_gbps:.0f} GB/s", file=sys.stderr, flush=True)
+
+    eng.run(W)
+    eng.set_timing(True)
+    eng.reset_timing()
+    t0 = time.perf_counter()
+    eng.run(K)  # synchronous: returns after the last event of the region
+    wall = time.perf_counter() - t0
+    tm = eng.timing()
+    eng.set_timing(False)
+
+    ms_per_step = tm["run_ms"] / K
+    value = cells * K / (tm["run_ms"] * 1e-3) / 1e9  # Gcells/s
+    sbytes = {"fp64": 8, "fp32": 4, "bf16": 2}[args.precision]
+    cbytes = 8 if args.precision == "fp64" else 4
+    id_bytes = 1  # compact layout: uint8 material index (<= 256 materials)
+    per_cell_pass = 9 * sbytes + id_bytes
+    bytes_h = cells * per_cell_pass + pml_bytes(sim, "h", cbytes)
+    bytes_e = cells * per_cell_pass + pml_bytes(sim, "e", cbytes)
+    h_ms = tm["yee_h_ms"] / tm["yee_h_launches"]
+    e_ms = tm["yee_e_ms"] / tm["yee_e_launches"]
+    if e_ms >= h_ms:
+        kernel, avg_ms, alg_bytes = "k_yee_e3", e_ms, bytes_e
+    else:
+        kernel, avg_ms, alg_bytes = "k_yee_h3", h_ms, bytes_h
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{args.precision}_{nx}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        eng.download()
+        threads = oracle.threads()
+        tc = time.perf_counter()
+        oracle.run(sim.scene_ptr(), args.cpu_steps)
+        cpu_s = time.perf_counter() - tc
+        cpu = {"value": cells * args.cpu_steps / cpu_s / 1e9, "unit": "Gcells/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} FP64 steps of the same {nx}x{ny}x{nz} scene "
+                         f"({cpu_s:.1f} s), oracle/xf_oracle.c with {threads} OpenMP threads"}
+    eng.close()
+
+    out = {
+        "metric": "Yee cell-updates/s (Gcells/s) & % HBM roofline",
+        "value": round(value, 4),
+        "unit": "Gcells/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": f"dielectric sphere {nx - 2 * args.pml}^3 + {args.pml}-cell CPML "
+                               f"({nx}x{ny}x{nz} cells), TFSF + NF2FF 1 GHz, {args.precision}",
+                   "cells": cells, "layout": "compact (uint8 material index)",
+                   "wall_s": round(wall, 4)},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
+                     "avg_launch_ms": round(avg_ms, 4),
+                     "h_ms": round(h_ms, 4), "e_ms": round(e_ms, 4),
+                     "measured_copy_GBps": round(copy_gbps, 1) if copy_gbps else None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

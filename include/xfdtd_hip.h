@@ -176,9 +176,19 @@ typedef struct xf_plan {
   int32_t block_dim[3]; /* hint */
   int32_t use_graph;    /* capture each step in a hipGraph and replay */
   int32_t timing;       /* record per-kernel HIP events in xf_run */
-  int32_t tile_y;       /* 0 = default; rows of the Yee tile (1..16) */
+  int32_t tile_y;       /* 0 = default; rows of the Yee tile (1..4) */
   int32_t device;       /* HIP device ordinal */
+  int32_t chunks_x;     /* 0 = auto; x-chunks of the 3-D marching kernels */
+  int32_t tiling_flags; /* 0 = auto, else XF_TILE_SET | XF_TILE_* bits    */
 } xf_plan;
+
+/* Tiling flags of the 3-D Yee kernels. */
+enum xf_tile_flags {
+  XF_TILE_XCD_REMAP = 1, /* XCD-aware block order                        */
+  XF_TILE_PIPELINE = 2,  /* software-pipelined march (2 planes in flight) */
+  XF_TILE_NT = 4,        /* non-temporal loads/stores of the updated field */
+  XF_TILE_SET = 256      /* the other bits are explicit, not auto        */
+};
 
 /* Per-kernel device time accumulated by xf_run while plan.timing != 0. */
 typedef struct xf_timing {
@@ -224,6 +234,11 @@ int xf_get_timing(const xf_handle* h, xf_timing* out);
 int xf_reset_timing(xf_handle* h);
 int xf_set_timing(xf_handle* h, int32_t on);
 
+/* Changes the tiling of the 3-D Yee kernels between runs (the autotuner's
+ * knob): rows per tile (1..4, 0 = auto), x-chunks (0 = auto), flags as in
+ * xf_plan.tiling_flags. */
+int xf_set_tiling(xf_handle* h, int32_t tile_y, int32_t chunks_x, int32_t flags);
+
 /* Description of the chosen layout (coefficient compression, tile). */
 int xf_describe(const xf_handle* h, char* buf, size_t len);
 
@@ -232,6 +247,17 @@ const char* xf_last_error(void);
 
 /* Number of visible HIP devices (0 without a GPU); never faults. */
 int xf_device_count(int* count);
+
+/* Measurement helper (no reference counterpart): copies `bytes` between two
+ * device buffers `reps` times with a plain wave64 copy kernel
+ * (xf_stream_copy_kernel) and returns the copy bandwidth (read + write
+ * bytes per second / 1e9).  Used as the measured HBM peak and as the
+ * known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters.
+ * variant: 0 one load in flight per lane, 1 four, 2 four non-temporal,
+ * 3 eight, 4 eight non-temporal, 5 one element per lane without a loop,
+ * 6 the same non-temporal; blocks: grid size of 0..4 (0 = 8192). */
+int xf_stream_bench(int64_t bytes, int32_t reps, int32_t device, int32_t variant, int32_t blocks,
+                    double* gbps);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,8 @@ def lib() -> ctypes.CDLL:
         l.xfo_set_num_threads.argtypes = [ctypes.c_int]
         l.xfo_yee_component_triple_loop.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         l.xfo_yee_component_triple_loop.restype = ctypes.c_int
+        l.xfo_yee_component.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        l.xfo_yee_component.restype = ctypes.c_int
         _lib = l
     return _lib
 
@@ -54,3 +56,7 @@ def threads() -> int:
 
 def yee_component_triple_loop(scene_ptr: int, is_e: bool, axis: int) -> None:
     lib().xfo_yee_component_triple_loop(ctypes.c_void_p(scene_ptr), int(is_e), axis)
+
+
+def yee_component(scene_ptr: int, is_e: bool, axis: int) -> None:
+    lib().xfo_yee_component(ctypes.c_void_p(scene_ptr), int(is_e), axis)

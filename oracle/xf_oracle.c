@@ -589,6 +589,13 @@ void xfo_set_num_threads(int n) {
 #endif
 }
 
+/* One Yee component update in the flat linear-index form of the engine's
+ * reference path (update_scheme.cuh:141-197). */
+int xfo_yee_component(xf_scene* s, int is_e, int xyz) {
+  yee_update(s, is_e, xyz);
+  return 0;
+}
+
 /* Brute-force form of one Yee component update, written as the reference's
  * triple-loop update (update_scheme.cuh:65-130) instead of the flat
  * linear-index loop; tests check that both forms agree. */

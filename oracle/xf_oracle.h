@@ -16,7 +16,9 @@ int xfo_run(xf_scene* s, int64_t n_steps);
 const char* xfo_last_error(void);
 int xfo_num_threads(void);
 void xfo_set_num_threads(int n);
-/* One Yee component update written as the reference's triple loop. */
+/* One Yee component update: flat form, and written as the reference's
+ * triple loop. */
+int xfo_yee_component(xf_scene* s, int is_e, int xyz);
 int xfo_yee_component_triple_loop(xf_scene* s, int is_e, int xyz);
 
 #ifdef __cplusplus

@@ -41,7 +41,15 @@ class XfPlan(ctypes.Structure):
         ("timing", c_i32),
         ("tile_y", c_i32),
         ("device", c_i32),
+        ("chunks_x", c_i32),
+        ("tiling_flags", c_i32),
     ]
+
+
+TILE_XCD_REMAP = 1
+TILE_PIPELINE = 2
+TILE_NT = 4
+TILE_SET = 256
 
 
 class XfTiming(ctypes.Structure):
@@ -101,6 +109,10 @@ def hip() -> ctypes.CDLL:
         lib.xf_set_timing.argtypes = [c_vp, c_i32]
         lib.xf_describe.argtypes = [c_vp, ctypes.c_char_p, ctypes.c_size_t]
         lib.xf_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        lib.xf_stream_bench.argtypes = [c_i64, c_i32, c_i32, c_i32, c_i32, ctypes.POINTER(c_f64)]
+        lib.xf_stream_bench.restype = ctypes.c_int
+        lib.xf_set_tiling.argtypes = [c_vp, c_i32, c_i32, c_i32]
+        lib.xf_set_tiling.restype = ctypes.c_int
         for fn in ("xf_create", "xf_upload", "xf_run", "xf_download", "xf_destroy",
                    "xf_current_step", "xf_get_timing", "xf_reset_timing", "xf_set_timing",
                    "xf_describe",
@@ -167,3 +179,12 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     hip().xf_device_count(ctypes.byref(n))
     return int(n.value)
+
+
+def stream_bench(nbytes: int, reps: int = 10, device: int = 0, variant: int = 0,
+                 blocks: int = 0) -> float:
+    """Measured device copy bandwidth in GB/s (read + write bytes)."""
+    g = ctypes.c_double(0.0)
+    check_hip(hip().xf_stream_bench(nbytes, reps, device, variant, blocks, ctypes.byref(g)),
+              "xf_stream_bench")
+    return g.value

@@ -145,7 +145,13 @@ struct Engine {
   // timing
   xf_timing timing{};
   int tile_y{4};
-  int chunk{0};
+  int chunk{1};
+  int nkb{1}, njb{1}, nch_eff{1};
+  int xcd{0};
+  int pipeline{1};
+  int nt_streams{0};
+  void set_tiling(int ty, int chunks_x, int flags);
+  Tiling tiling() const;
   std::string layout_desc;
 
   ~Engine() { release(); }
@@ -946,23 +952,61 @@ int Engine::upload() {
   void* tp;
   XF_TRY(alloc(&tp, sizeof(int)));
   t_dev = static_cast<int*>(tp);
-  tile_y = plan.tile_y > 0 ? std::min(plan.tile_y, 16) : 4;
-  // x chunk of the marching kernels: enough blocks to cover 256 CUs several
-  // times over, with long marches for register reuse.
-  if (dim == 3) {
-    const int64_t cols = ((nz + 1 + 63) / 64) * ((ny + 1 + tile_y - 1) / tile_y);
-    int64_t nchunk = std::max<int64_t>(1, (8 * 256 + cols - 1) / cols);
-    nchunk = std::min<int64_t>(nchunk, nx + 1);
-    chunk = static_cast<int>((nx + 1 + nchunk - 1) / nchunk);
-  }
+  set_tiling(plan.tile_y, plan.chunks_x, plan.tiling_flags);
   XF_HIP(hipDeviceSynchronize());
   uploaded = true;
+  return XF_OK;
+}
+
+// Tiling of the 3-D marching kernels.  chunks_x = 0 gives marches of about
+// four planes: measured at 536^3 on MI355X, many short marches beat long
+// ones (a long march leaves a partly empty tail round of blocks and keeps
+// far-apart planes open at once), while the carried plane still saves a
+// neighbour re-read every step of the march.
+void Engine::set_tiling(int ty, int chunks_x, int flags) {
+  // 64 x tile_y threads must fit the kernels' __launch_bounds__(256)
+  tile_y = ty > 0 ? std::min(ty, 4) : 2;
+  if (flags & XF_TILE_SET) {
+    xcd = (flags & XF_TILE_XCD_REMAP) ? 1 : 0;
+    pipeline = (flags & XF_TILE_PIPELINE) ? 1 : 0;
+    nt_streams = (flags & XF_TILE_NT) ? 1 : 0;
+  } else {
+    xcd = 0;
+    pipeline = 1;
+    nt_streams = 0;
+  }
+  if (dim == 3) {
+    nkb = static_cast<int>((nz + 1 + 63) / 64);
+    njb = static_cast<int>((ny + 1 + tile_y - 1) / tile_y);
+    int64_t nch = chunks_x;
+    if (nch <= 0) nch = std::max<int64_t>(1, (nx + 1 + 3) / 4);
+    nch = std::min<int64_t>(nch, nx + 1);
+    chunk = static_cast<int>((nx + 1 + nch - 1) / nch);
+    nch_eff = static_cast<int>((nx + 1 + chunk - 1) / chunk);
+  }
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  graph_exec = nullptr;
+  graph = nullptr;
+  graph_ready = false;
   layout_desc = std::string("dim=") + std::to_string(dim) + " pad_stride=" + std::to_string(dim == 3 ? sy : sx) +
                 " id_bytes=" + std::to_string(id_bytes) + " etab=" + std::to_string(ntab_e) +
                 " htab=" + std::to_string(ntab_h) + " chunk=" + std::to_string(chunk) +
-                " tile_y=" + std::to_string(tile_y) + " pml_faces=" + std::to_string(pml.size()) +
+                " nchunk=" + std::to_string(nch_eff) + " tile_y=" + std::to_string(tile_y) +
+                " xcd=" + std::to_string(xcd) + " pipeline=" + std::to_string(pipeline) +
+                " nt=" + std::to_string(nt_streams) +
+                " pml_faces=" + std::to_string(pml.size()) +
                 " ade_cells=" + std::to_string(ade_cells.size());
-  return XF_OK;
+}
+
+Tiling Engine::tiling() const {
+  Tiling t;
+  t.nkb = nkb;
+  t.njb = njb;
+  t.nch = nch_eff;
+  t.chunk = chunk;
+  t.xcd = xcd;
+  return t;
 }
 
 // ------------------------------------------------------------- download
@@ -1104,12 +1148,21 @@ int Engine::launch_h(int t) {
   const bool has_pml = P.n > 0;
   const C* tab = static_cast<const C*>(htab);
   if (dim == 3) {
+    const Tiling T = tiling();
     dim3 block(64, tile_y);
-    dim3 grid((unsigned)((nz + 1 + 63) / 64), (unsigned)((ny + 1 + tile_y - 1) / tile_y),
-              (unsigned)((nx + 1 + chunk - 1) / chunk));
-#define XF_LH3(I, L, PM)                                                                              \
-  hipLaunchKernelGGL((k_yee_h3<S, I, L, PM>), grid, block, shm, stream, f, static_cast<const I*>(hid), \
-                     tab, ntab_h, P, d, chunk, scl)
+    dim3 grid((unsigned)(T.nkb * T.njb * T.nch));
+#define XF_LH3(I, L, PM)                                                                               \
+  do {                                                                                                 \
+    if (pipeline && nt_streams)                                                                        \
+      hipLaunchKernelGGL((k_yee_h3p<S, I, L, PM, true>), grid, block, shm, stream, f,                   \
+                         static_cast<const I*>(hid), tab, ntab_h, P, d, T, scl);                        \
+    else if (pipeline)                                                                                 \
+      hipLaunchKernelGGL((k_yee_h3p<S, I, L, PM, false>), grid, block, shm, stream, f,                  \
+                         static_cast<const I*>(hid), tab, ntab_h, P, d, T, scl);                        \
+    else                                                                                               \
+      hipLaunchKernelGGL((k_yee_h3<S, I, L, PM>), grid, block, shm, stream, f,                          \
+                         static_cast<const I*>(hid), tab, ntab_h, P, d, T, scl);                        \
+  } while (0)
     if (id_bytes == 1) {
       if (lds) { if (has_pml) XF_LH3(uint8_t, true, true); else XF_LH3(uint8_t, true, false); }
       else { if (has_pml) XF_LH3(uint8_t, false, true); else XF_LH3(uint8_t, false, false); }
@@ -1153,11 +1206,20 @@ int Engine::launch_e(int t) {
   const C* tab = static_cast<const C*>(etab);
   if (dim == 3) {
     dim3 block(64, tile_y);
-    dim3 grid((unsigned)((nz + 1 + 63) / 64), (unsigned)((ny + 1 + tile_y - 1) / tile_y),
-              (unsigned)((nx + 1 + chunk - 1) / chunk));
-#define XF_LE3(I, L, PM, AD)                                                                          \
-  hipLaunchKernelGGL((k_yee_e3<S, I, L, PM, AD>), grid, block, shm, stream, f,                         \
-                     static_cast<const I*>(eid), tab, ntab_e, eflag, P, d, chunk, scl)
+    const Tiling T = tiling();
+    dim3 grid((unsigned)(T.nkb * T.njb * T.nch));
+#define XF_LE3(I, L, PM, AD)                                                                           \
+  do {                                                                                                 \
+    if (pipeline && nt_streams)                                                                        \
+      hipLaunchKernelGGL((k_yee_e3p<S, I, L, PM, AD, true>), grid, block, shm, stream, f,               \
+                         static_cast<const I*>(eid), tab, ntab_e, eflag, P, d, T, scl);                 \
+    else if (pipeline)                                                                                 \
+      hipLaunchKernelGGL((k_yee_e3p<S, I, L, PM, AD, false>), grid, block, shm, stream, f,              \
+                         static_cast<const I*>(eid), tab, ntab_e, eflag, P, d, T, scl);                 \
+    else                                                                                               \
+      hipLaunchKernelGGL((k_yee_e3<S, I, L, PM, AD>), grid, block, shm, stream, f,                      \
+                         static_cast<const I*>(eid), tab, ntab_e, eflag, P, d, T, scl);                 \
+  } while (0)
 #define XF_LE3A(I, L, PM) \
   do { if (has_ade) XF_LE3(I, L, PM, true); else XF_LE3(I, L, PM, false); } while (0)
     if (id_bytes == 1) {
@@ -1462,6 +1524,59 @@ int xf_get_timing(const xf_handle* h, xf_timing* out) {
 int xf_reset_timing(xf_handle* h) {
   if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
   h->eng.timing = xf_timing{};
+  return XF_OK;
+}
+
+int xf_stream_bench(int64_t bytes, int32_t reps, int32_t device, int32_t variant, int32_t blocks,
+                    double* gbps) {
+  xf::g_err.clear();
+  if (!gbps || bytes < 16 || reps < 1) return xf::fail(XF_ERR_INVALID, "bad stream bench arguments");
+  if (variant < 0 || variant > 6) return xf::fail(XF_ERR_INVALID, "bad stream bench variant");
+  if (hipSetDevice(device) != hipSuccess) return xf::fail(XF_ERR_HIP, "hipSetDevice failed");
+  const int64_t n = bytes / 16;
+  xf::xf_d2 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n * 16) != hipSuccess) return xf::fail(XF_ERR_NOMEM, "stream bench allocation failed");
+  if (hipMalloc(&b, n * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return xf::fail(XF_ERR_NOMEM, "stream bench allocation failed");
+  }
+  hipEvent_t e0, e1;
+  (void)hipMemset(a, 0, n * 16);
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  const dim3 block(256), grid(blocks > 0 ? blocks : 256 * 8 * 4);
+  auto launch = [&]() {
+    switch (variant) {
+      case 0: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<1, false>), grid, block, 0, 0, a, b, n); break;
+      case 1: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<4, false>), grid, block, 0, 0, a, b, n); break;
+      case 2: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<4, true>), grid, block, 0, 0, a, b, n); break;
+      case 3: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<8, false>), grid, block, 0, 0, a, b, n); break;
+      case 4: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<8, true>), grid, block, 0, 0, a, b, n); break;
+      case 5: hipLaunchKernelGGL((xf::xf_stream_copy1_kernel<false>), dim3((unsigned)((n + 255) / 256)), block, 0, 0, a, b, n); break;
+      default: hipLaunchKernelGGL((xf::xf_stream_copy1_kernel<true>), dim3((unsigned)((n + 255) / 256)), block, 0, 0, a, b, n); break;
+    }
+  };
+  launch();  // warm-up
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; ++r) launch();
+  (void)hipEventRecord(e1, 0);
+  hipError_t err = hipEventSynchronize(e1);
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (err != hipSuccess) return xf::fail(XF_ERR_HIP, hipGetErrorString(err));
+  *gbps = 2.0 * static_cast<double>(n) * 16.0 * reps / (ms * 1e-3) / 1e9;
+  return XF_OK;
+}
+
+int xf_set_tiling(xf_handle* h, int32_t tile_y, int32_t chunks_x, int32_t flags) {
+  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+  // the Yee kernels are compiled for at most 256 threads per block (64 x 4)
+  if (tile_y < 0 || tile_y > 4 || chunks_x < 0) return xf::fail(XF_ERR_INVALID, "bad tiling (tile_y 0..4)");
+  h->eng.set_tiling(tile_y, chunks_x, flags);
   return XF_OK;
 }
 

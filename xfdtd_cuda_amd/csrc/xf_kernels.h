@@ -24,17 +24,32 @@ constexpr int kMaxTfsfJobs = 12;
 // bf16 stores value * scale (scale per E/H) and computes in float.
 template <typename S>
 struct Store;
+// ld_nt / st_nt are the non-temporal forms, used for the streams a pass
+// reads once and rewrites (the updated component itself), so they do not
+// displace the neighbour rows other waves still need from L2.
 template <>
 struct Store<double> {
   using C = double;
   static __device__ __forceinline__ C ld(const double* p, int64_t i, C) { return p[i]; }
   static __device__ __forceinline__ void st(double* p, int64_t i, C v, C) { p[i] = v; }
+  static __device__ __forceinline__ C ld_nt(const double* p, int64_t i, C) {
+    return __builtin_nontemporal_load(p + i);
+  }
+  static __device__ __forceinline__ void st_nt(double* p, int64_t i, C v, C) {
+    __builtin_nontemporal_store(v, p + i);
+  }
 };
 template <>
 struct Store<float> {
   using C = float;
   static __device__ __forceinline__ C ld(const float* p, int64_t i, C) { return p[i]; }
   static __device__ __forceinline__ void st(float* p, int64_t i, C v, C) { p[i] = v; }
+  static __device__ __forceinline__ C ld_nt(const float* p, int64_t i, C) {
+    return __builtin_nontemporal_load(p + i);
+  }
+  static __device__ __forceinline__ void st_nt(float* p, int64_t i, C v, C) {
+    __builtin_nontemporal_store(v, p + i);
+  }
 };
 template <>
 struct Store<__hip_bfloat16> {
@@ -44,6 +59,12 @@ struct Store<__hip_bfloat16> {
   }
   static __device__ __forceinline__ void st(__hip_bfloat16* p, int64_t i, C v, C scale) {
     p[i] = __float2bfloat16(v * scale);
+  }
+  static __device__ __forceinline__ C ld_nt(const __hip_bfloat16* p, int64_t i, C inv) {
+    return ld(p, i, inv);
+  }
+  static __device__ __forceinline__ void st_nt(__hip_bfloat16* p, int64_t i, C v, C scale) {
+    st(p, i, v, scale);
   }
 };
 
@@ -63,6 +84,33 @@ struct Dims {
   int nx, ny, nz;  // cells
   int64_t sx, sy;  // strides of i and j (k stride is 1 in 3-D; unused in 2-D)
 };
+
+// Work decomposition of the 3-D marching kernels: a 1-D grid of nkb * njb *
+// nch blocks; block (kb, jb, ch) owns k in [64 kb, 64 kb + 64), j in
+// [ty jb, ty jb + ty) and marches i through [chunk ch, chunk (ch + 1)).
+// With xcd != 0 the hardware block id is remapped so that each of the 8 XCDs
+// (which receive hardware ids round-robin) owns a contiguous run of
+// logical tiles (k fastest, then j), keeping the j/k halo rows of
+// neighbouring tiles in the same XCD's L2.
+struct Tiling {
+  int nkb, njb, nch;
+  int chunk;
+  int xcd;
+};
+
+__device__ __forceinline__ void tile_coords(const Tiling& T, int& kb, int& jb, int& ch) {
+  const int nblk = T.nkb * T.njb * T.nch;
+  int id = blockIdx.x;
+  if (T.xcd) {
+    const int q = nblk / 8, r = nblk % 8;
+    const int x = id % 8, n = id / 8;
+    id = x * q + min(x, r) + n;
+  }
+  kb = id % T.nkb;
+  id /= T.nkb;
+  jb = id % T.njb;
+  ch = id / T.njb;
+}
 
 // One PML face for one pass (E or H).  Component A is E_A/H_A of the face's
 // (a, b, c) frame, B is E_B/H_B (pml_corrector.cu:50-270).

@@ -111,16 +111,18 @@ template <typename S, typename I, bool LDS, bool PML>
 __global__ void __launch_bounds__(256)
     k_yee_h3(Fields<S> f, const I* __restrict__ hid,
              const typename Store<S>::C* __restrict__ htab, int ntab,
-             PmlPass<typename Store<S>::C> pml, Dims d, int chunk,
+             PmlPass<typename Store<S>::C> pml, Dims d, Tiling T,
              Scales<typename Store<S>::C> sc) {
   using C = typename Store<S>::C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const C* tab = load_table<C, LDS>(htab, ntab * 9, reinterpret_cast<C*>(smem_raw));
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int kb, jb, ch;
+  tile_coords(T, kb, jb, ch);
+  const int k = kb * 64 + threadIdx.x;
+  const int j = jb * blockDim.y + threadIdx.y;
   if (k > d.nz || j > d.ny) return;
-  const int i0 = blockIdx.z * chunk;
-  const int i1 = min(i0 + chunk, d.nx + 1);
+  const int i0 = ch * T.chunk;
+  const int i1 = min(i0 + T.chunk, d.nx + 1);
   const bool jk_main = j < d.ny && k < d.nz;
   int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
   const C ie = sc.inv_e, ih = sc.inv_h;
@@ -171,15 +173,17 @@ __global__ void __launch_bounds__(256)
     k_yee_e3(Fields<S> f, const I* __restrict__ eid,
              const typename Store<S>::C* __restrict__ etab, int ntab,
              const unsigned char* __restrict__ eflag, PmlPass<typename Store<S>::C> pml,
-             Dims d, int chunk, Scales<typename Store<S>::C> sc) {
+             Dims d, Tiling T, Scales<typename Store<S>::C> sc) {
   using C = typename Store<S>::C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const C* tab = load_table<C, LDS>(etab, ntab * 9, reinterpret_cast<C*>(smem_raw));
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  int kb, jb, ch;
+  tile_coords(T, kb, jb, ch);
+  const int k = kb * 64 + threadIdx.x;
+  const int j = jb * blockDim.y + threadIdx.y;
   if (k > d.nz || j > d.ny) return;
-  const int i0 = blockIdx.z * chunk;
-  const int i1 = min(i0 + chunk, d.nx + 1);
+  const int i0 = ch * T.chunk;
+  const int i1 = min(i0 + T.chunk, d.nx + 1);
   const bool jk_main = j < d.ny && k < d.nz;
   int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
   const C ie = sc.inv_e, ih = sc.inv_h;
@@ -223,6 +227,180 @@ __global__ void __launch_bounds__(256)
     if (dirty[0]) Store<S>::st(f.e[0], idx, v[0], sc.s_e);
     if (dirty[1]) Store<S>::st(f.e[1], idx, v[1], sc.s_e);
     if (dirty[2]) Store<S>::st(f.e[2], idx, v[2], sc.s_e);
+  }
+}
+
+// -------------------------------------------- 3-D Yee, software-pipelined
+// The same H and E passes with the loads of plane i+1 issued before plane i
+// is computed and stored, so each wave keeps two planes of loads in
+// flight.  Arithmetic and its order are those of k_yee_h3 / k_yee_e3.
+template <typename S>
+struct HLoads {
+  typename Store<S>::C ex_i, ex_k, ex_j, ey_k, ey_n, ez_j, ez_n, hx, hy, hz;
+  int id;
+};
+
+template <typename S, typename I, bool NT>
+__device__ __forceinline__ void h_load(HLoads<S>& L, const Fields<S>& f, const I* __restrict__ hid,
+                                       int64_t idx, const Dims& d, typename Store<S>::C ie,
+                                       typename Store<S>::C ih) {
+  L.id = (int)hid[idx];
+  L.ex_i = Store<S>::ld(f.e[0], idx, ie);
+  L.ex_k = Store<S>::ld(f.e[0], idx + 1, ie);
+  L.ex_j = Store<S>::ld(f.e[0], idx + d.sy, ie);
+  L.ey_k = Store<S>::ld(f.e[1], idx + 1, ie);
+  L.ey_n = Store<S>::ld(f.e[1], idx + d.sx, ie);
+  L.ez_j = Store<S>::ld(f.e[2], idx + d.sy, ie);
+  L.ez_n = Store<S>::ld(f.e[2], idx + d.sx, ie);
+  L.hx = NT ? Store<S>::ld_nt(f.h[0], idx, ih) : Store<S>::ld(f.h[0], idx, ih);
+  L.hy = NT ? Store<S>::ld_nt(f.h[1], idx, ih) : Store<S>::ld(f.h[1], idx, ih);
+  L.hz = NT ? Store<S>::ld_nt(f.h[2], idx, ih) : Store<S>::ld(f.h[2], idx, ih);
+}
+
+template <typename S, typename I, bool LDS, bool PML, bool NT>
+__global__ void __launch_bounds__(256)
+    k_yee_h3p(Fields<S> f, const I* __restrict__ hid,
+              const typename Store<S>::C* __restrict__ htab, int ntab,
+              PmlPass<typename Store<S>::C> pml, Dims d, Tiling T,
+              Scales<typename Store<S>::C> sc) {
+  using C = typename Store<S>::C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* tab = load_table<C, LDS>(htab, ntab * 9, reinterpret_cast<C*>(smem_raw));
+  int kb, jb, ch;
+  tile_coords(T, kb, jb, ch);
+  const int k = kb * 64 + threadIdx.x;
+  const int j = jb * blockDim.y + threadIdx.y;
+  if (k > d.nz || j > d.ny) return;
+  const int i0 = ch * T.chunk;
+  const int i1 = min(i0 + T.chunk, d.nx + 1);
+  const bool jk_main = j < d.ny && k < d.nz;
+  int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
+  const C ie = sc.inv_e, ih = sc.inv_h;
+  C ey_i = 0, ez_i = 0;
+  HLoads<S> cur{}, nxt{};
+  if (jk_main && i0 < d.nx) {
+    ey_i = Store<S>::ld(f.e[1], idx, ie);
+    ez_i = Store<S>::ld(f.e[2], idx, ie);
+    h_load<S, I, NT>(cur, f, hid, idx, d, ie, ih);
+  }
+  for (int i = i0; i < i1; ++i, idx += d.sx) {
+    const bool main_i = jk_main && i < d.nx;
+    if (jk_main && i + 1 < d.nx && i + 1 < i1) h_load<S, I, NT>(nxt, f, hid, idx + d.sx, d, ie, ih);
+    C v[3] = {0, 0, 0};
+    bool have[3] = {false, false, false};
+    bool dirty[3] = {false, false, false};
+    if (main_i) {
+      const C* t = tab + 9 * cur.id;
+      v[0] = t[0] * cur.hx + t[1] * (cur.ey_k - ey_i) + t[2] * (cur.ez_j - ez_i);
+      v[1] = t[3] * cur.hy + t[4] * (cur.ez_n - ez_i) + t[5] * (cur.ex_k - cur.ex_i);
+      v[2] = t[6] * cur.hz + t[7] * (cur.ex_j - cur.ex_i) + t[8] * (cur.ey_n - ey_i);
+      have[0] = have[1] = have[2] = true;
+      dirty[0] = dirty[1] = dirty[2] = true;
+      ey_i = cur.ey_n;
+      ez_i = cur.ez_n;
+    }
+    if (PML) pml_all<false, false>(pml, i, j, k, idx, f, d, sc, v, have, dirty);
+    if (NT) {
+      if (dirty[0]) Store<S>::st_nt(f.h[0], idx, v[0], sc.s_h);
+      if (dirty[1]) Store<S>::st_nt(f.h[1], idx, v[1], sc.s_h);
+      if (dirty[2]) Store<S>::st_nt(f.h[2], idx, v[2], sc.s_h);
+    } else {
+      if (dirty[0]) Store<S>::st(f.h[0], idx, v[0], sc.s_h);
+      if (dirty[1]) Store<S>::st(f.h[1], idx, v[1], sc.s_h);
+      if (dirty[2]) Store<S>::st(f.h[2], idx, v[2], sc.s_h);
+    }
+    cur = nxt;
+  }
+}
+
+template <typename S>
+struct ELoads {
+  typename Store<S>::C ex, ey, ez, hx, hx_k, hx_j, hy, hy_k, hz, hz_j;
+  int id;
+};
+
+template <typename S, typename I, bool NT>
+__device__ __forceinline__ void e_load(ELoads<S>& L, const Fields<S>& f, const I* __restrict__ eid,
+                                       int64_t idx, const Dims& d, bool j_in, bool k_in,
+                                       typename Store<S>::C ie, typename Store<S>::C ih) {
+  L.id = (int)eid[idx];
+  L.ex = NT ? Store<S>::ld_nt(f.e[0], idx, ie) : Store<S>::ld(f.e[0], idx, ie);
+  L.ey = NT ? Store<S>::ld_nt(f.e[1], idx, ie) : Store<S>::ld(f.e[1], idx, ie);
+  L.ez = NT ? Store<S>::ld_nt(f.e[2], idx, ie) : Store<S>::ld(f.e[2], idx, ie);
+  L.hx = Store<S>::ld(f.h[0], idx, ih);
+  L.hy = Store<S>::ld(f.h[1], idx, ih);
+  L.hz = Store<S>::ld(f.h[2], idx, ih);
+  L.hx_k = k_in ? Store<S>::ld(f.h[0], idx - 1, ih) : (typename Store<S>::C)0;
+  L.hy_k = k_in ? Store<S>::ld(f.h[1], idx - 1, ih) : (typename Store<S>::C)0;
+  L.hx_j = j_in ? Store<S>::ld(f.h[0], idx - d.sy, ih) : (typename Store<S>::C)0;
+  L.hz_j = j_in ? Store<S>::ld(f.h[2], idx - d.sy, ih) : (typename Store<S>::C)0;
+}
+
+template <typename S, typename I, bool LDS, bool PML, bool ADE, bool NT>
+__global__ void __launch_bounds__(256)
+    k_yee_e3p(Fields<S> f, const I* __restrict__ eid,
+              const typename Store<S>::C* __restrict__ etab, int ntab,
+              const unsigned char* __restrict__ eflag, PmlPass<typename Store<S>::C> pml,
+              Dims d, Tiling T, Scales<typename Store<S>::C> sc) {
+  using C = typename Store<S>::C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* tab = load_table<C, LDS>(etab, ntab * 9, reinterpret_cast<C*>(smem_raw));
+  int kb, jb, ch;
+  tile_coords(T, kb, jb, ch);
+  const int k = kb * 64 + threadIdx.x;
+  const int j = jb * blockDim.y + threadIdx.y;
+  if (k > d.nz || j > d.ny) return;
+  const int i0 = ch * T.chunk;
+  const int i1 = min(i0 + T.chunk, d.nx + 1);
+  const bool jk_main = j < d.ny && k < d.nz;
+  const bool j_in = j > 0, k_in = k > 0;
+  int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
+  const C ie = sc.inv_e, ih = sc.inv_h;
+  C hy_m = 0, hz_m = 0;
+  ELoads<S> cur{}, nxt{};
+  if (jk_main && i0 < d.nx) {
+    if (i0 > 0) {
+      hy_m = Store<S>::ld(f.h[1], idx - d.sx, ih);
+      hz_m = Store<S>::ld(f.h[2], idx - d.sx, ih);
+    }
+    e_load<S, I, NT>(cur, f, eid, idx, d, j_in, k_in, ie, ih);
+  }
+  for (int i = i0; i < i1; ++i, idx += d.sx) {
+    if (jk_main && i + 1 < d.nx && i + 1 < i1)
+      e_load<S, I, NT>(nxt, f, eid, idx + d.sx, d, j_in, k_in, ie, ih);
+    C v[3] = {0, 0, 0};
+    bool have[3] = {false, false, false};
+    bool dirty[3] = {false, false, false};
+    if (jk_main && i < d.nx) {
+      if (!(ADE && eflag[cur.id])) {
+        const C* t = tab + 9 * cur.id;
+        if (j_in && k_in) {
+          v[0] = t[0] * cur.ex + t[1] * (cur.hy - cur.hy_k) + t[2] * (cur.hz - cur.hz_j);
+          have[0] = dirty[0] = true;
+        }
+        if (k_in && i > 0) {
+          v[1] = t[3] * cur.ey + t[4] * (cur.hz - hz_m) + t[5] * (cur.hx - cur.hx_k);
+          have[1] = dirty[1] = true;
+        }
+        if (i > 0 && j_in) {
+          v[2] = t[6] * cur.ez + t[7] * (cur.hx - cur.hx_j) + t[8] * (cur.hy - hy_m);
+          have[2] = dirty[2] = true;
+        }
+      }
+      hy_m = cur.hy;
+      hz_m = cur.hz;
+    }
+    if (PML) pml_all<true, false>(pml, i, j, k, idx, f, d, sc, v, have, dirty);
+    if (NT) {
+      if (dirty[0]) Store<S>::st_nt(f.e[0], idx, v[0], sc.s_e);
+      if (dirty[1]) Store<S>::st_nt(f.e[1], idx, v[1], sc.s_e);
+      if (dirty[2]) Store<S>::st_nt(f.e[2], idx, v[2], sc.s_e);
+    } else {
+      if (dirty[0]) Store<S>::st(f.e[0], idx, v[0], sc.s_e);
+      if (dirty[1]) Store<S>::st(f.e[1], idx, v[1], sc.s_e);
+      if (dirty[2]) Store<S>::st(f.e[2], idx, v[2], sc.s_e);
+    }
+    cur = nxt;
   }
 }
 
@@ -551,5 +729,41 @@ __global__ void __launch_bounds__(256)
 }
 
 __global__ void k_next_step(int* t) { *t += 1; }
+
+// Plain streaming copy, 16 bytes per lane, grid-stride, U independent loads
+// in flight per lane, optionally non-temporal.
+typedef double xf_d2 __attribute__((ext_vector_type(2)));
+template <int U, bool NT>
+__global__ void __launch_bounds__(256)
+    xf_stream_copy_kernel(const xf_d2* __restrict__ src, xf_d2* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + (U - 1) * stride < n; i += U * stride) {
+    xf_d2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      v[u] = NT ? __builtin_nontemporal_load(&src[i + u * stride]) : src[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (NT)
+        __builtin_nontemporal_store(v[u], &dst[i + u * stride]);
+      else
+        dst[i + u * stride] = v[u];
+    }
+  }
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// One 16-byte element per lane, one pass (the BabelStream copy shape).
+template <bool NT>
+__global__ void __launch_bounds__(256)
+    xf_stream_copy1_kernel(const xf_d2* __restrict__ src, xf_d2* __restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (NT)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
+  else
+    dst[i] = src[i];
+}
 
 }  // namespace xf

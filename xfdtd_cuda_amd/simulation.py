@@ -291,7 +291,7 @@ def scene_array(scene_ptr: int, name: str) -> np.ndarray:
         f"array {name}")
     shp = tuple(int(shape[d]) for d in range(ndim.value))
     if not data:
-        return np.zeros(shp)
+        return np.zeros((0,))  # array absent from this scene (e.g. compact layout)
     return np.ctypeslib.as_array(data, shape=shp)
 
 
@@ -372,6 +372,19 @@ class Engine:
         buf = ctypes.create_string_buffer(512)
         _native.check_hip(self._lib.xf_describe(self._h, buf, 512), "xf_describe")
         return buf.value.decode()
+
+    def set_tiling(self, tile_y: int = 0, chunks_x: int = 0, xcd_remap: bool | None = None,
+                   pipeline: bool | None = None, nontemporal: bool | None = None):
+        """Tiling of the 3-D Yee kernels; None everywhere leaves the flags to
+        the engine's defaults."""
+        flags = 0
+        if xcd_remap is not None or pipeline is not None or nontemporal is not None:
+            flags = _native.TILE_SET
+            flags |= _native.TILE_XCD_REMAP if xcd_remap else 0
+            flags |= _native.TILE_PIPELINE if (pipeline is None or pipeline) else 0
+            flags |= _native.TILE_NT if nontemporal else 0
+        _native.check_hip(self._lib.xf_set_tiling(self._h, tile_y, chunks_x, flags),
+                          "xf_set_tiling")
 
     def set_timing(self, on: bool):
         _native.check_hip(self._lib.xf_set_timing(self._h, int(on)), "xf_set_timing")
