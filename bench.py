@@ -18,6 +18,7 @@ with the algorithmic bytes of one launch.  The CPU baseline is the oracle
 steps of the same scene on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
+       python bench.py --workload batch   (64 uneven grids, FP32, batched kernels)
 """
 from __future__ import annotations
 
@@ -26,6 +27,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
@@ -55,18 +58,25 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"])
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16"],
+                    help="default: fp64 for the sphere, fp32 for the batch")
     ap.add_argument("--n", type=int, default=512, help="interior cells per edge")
     ap.add_argument("--pml", type=int, default=12)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "batch"],
+                    help="sphere: one 512^3 + CPML grid (default); batch: 64 uneven grids")
+    ap.add_argument("--count", type=int, default=64, help="grids of the batch workload")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this "
-              "build (see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; runs over several GPUs are not "
+              "implemented (only --gpus 1 is accepted)", file=sys.stderr)
         return 2
+    if args.workload == "batch":
+        return bench_batch(args)
+    args.precision = args.precision or "fp64"
 
     import xfdtd_cuda_amd as xf
     from xfdtd_cuda_amd import _native, scenes
@@ -160,6 +170,111 @@ def main(argv=None) -> int:
                      "avg_launch_ms": round(avg_ms, 4),
                      "h_ms": round(h_ms, 4), "e_ms": round(e_ms, 4),
                      "measured_copy_GBps": round(copy_gbps, 1) if copy_gbps else None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_batch(args) -> int:
+    """64 sphere grids of uneven sizes (edges U[48, 160] + 8-cell CPML,
+    radius U[0.2, 0.4] x edge, eps_r U[1.5, 8], seed 7) stepped together in
+    one persistent launch per Yee pass under a hipGraph, tile shape from the
+    autotuner.  Besides the batch rate the line carries the same grids run
+    one after another through the single-grid engine, and the batch in FP64,
+    for the speed-ups batching and FP32 give."""
+    import xfdtd_cuda_amd as xf
+    from xfdtd_cuda_amd import scenes
+
+    prec = args.precision or "fp32"
+    K, W, KT = args.steps, args.warmup, 5
+    steps_needed = W + K + KT + 8
+    t0 = time.perf_counter()
+    sims = scenes.batch_spheres(count=args.count, steps=max(steps_needed, 2 * K + 8))
+    cells = sum(int(np.prod(s.info()[0])) for s in sims)
+    print(f"# batch set-up {time.perf_counter() - t0:.1f} s, {len(sims)} grids, {cells} cells",
+          file=sys.stderr, flush=True)
+
+    def timed_batch(precision):
+        b = xf.Batch(sims, precision=precision, use_graph=True).upload()
+        best = b.autotune(trial_steps=3)
+        b.run(W)
+        b.reset_timing()
+        b.run(K)
+        run_ms = b.timing()["run_ms"]
+        b.set_timing(True)
+        b.reset_timing()
+        b.run(KT)
+        tm = b.timing()
+        desc = b.describe()
+        b.close()
+        return run_ms, tm, best, desc
+
+    run_ms, tm, best, desc = timed_batch(prec)
+    print(f"# batch {prec}: {run_ms / K:.3f} ms/step, {desc}", file=sys.stderr, flush=True)
+    value = cells * K / (run_ms * 1e-3) / 1e9
+
+    # the same grids one after another through the single-grid engine
+    seq_ms = 0.0
+    for s in sims:
+        e = xf.Engine(s, precision=prec, use_graph=True).upload()
+        e.run(2)
+        e.run(K)
+        seq_ms += e.timing()["run_ms"]
+        e.close()
+    seq_value = cells * K / (seq_ms * 1e-3) / 1e9
+    print(f"# sequential {prec}: {seq_ms / K:.3f} ms/step", file=sys.stderr, flush=True)
+
+    run64, _, best64, _ = timed_batch("fp64")
+    value64 = cells * K / (run64 * 1e-3) / 1e9
+    print(f"# batch fp64: {run64 / K:.3f} ms/step", file=sys.stderr, flush=True)
+
+    sbytes = {"fp64": 8, "fp32": 4, "bf16": 2}[prec]
+    cbytes = 8 if prec == "fp64" else 4
+    bytes_h = sum(int(np.prod(s.info()[0])) * (9 * sbytes + 1) + pml_bytes(s, "h", cbytes)
+                  for s in sims)
+    bytes_e = sum(int(np.prod(s.info()[0])) * (9 * sbytes + 1) + pml_bytes(s, "e", cbytes)
+                  for s in sims)
+    h_ms = tm["yee_h_ms"] / tm["yee_h_launches"]
+    e_ms = tm["yee_e_ms"] / tm["yee_e_launches"]
+    if e_ms >= h_ms:
+        kernel, avg_ms, alg_bytes = "k_yee_e3_batch", e_ms, bytes_e
+    else:
+        kernel, avg_ms, alg_bytes = "k_yee_h3_batch", h_ms, bytes_h
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        threads = oracle.threads()
+        tc = time.perf_counter()
+        for s in sims:
+            oracle.run(s.scene_ptr(), 1)
+        cpu_s = time.perf_counter() - tc
+        cpu = {"value": cells / cpu_s / 1e9, "unit": "Gcells/s", "cores": threads, "kind": "port",
+               "sample": f"1 FP64 step of each of the {len(sims)} grids ({cpu_s:.1f} s), "
+                         f"oracle/xf_oracle.c with {threads} OpenMP threads"}
+
+    out = {
+        "metric": "Yee cell-updates/s (Gcells/s) & % HBM roofline",
+        "value": round(value, 4), "unit": "Gcells/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(run_ms / K, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32"}[prec],
+        "data": "synthetic",
+        "config": {"workload": f"batch of {len(sims)} sphere grids, edges U[48,160] + 8-cell CPML "
+                               f"({cells} cells), TFSF + NF2FF, {prec}, one persistent launch "
+                               "per pass, hipGraph step",
+                   "cells": cells, "tuning": best, "describe": desc,
+                   "sequential_single_grid_gcells": round(seq_value, 4),
+                   "batch_speedup_vs_sequential": round(value / seq_value, 3),
+                   "fp64_batch_gcells": round(value64, 4),
+                   "speedup_vs_fp64": round(value / value64, 3), "fp64_tuning": best64},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_ms, 4),
+                     "h_ms": round(h_ms, 4), "e_ms": round(e_ms, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -185,7 +185,6 @@ typedef struct xf_plan {
 /* Tiling flags of the 3-D Yee kernels. */
 enum xf_tile_flags {
   XF_TILE_XCD_REMAP = 1, /* XCD-aware block order                        */
-  XF_TILE_PIPELINE = 2,  /* software-pipelined march (2 planes in flight) */
   XF_TILE_NT = 4,        /* non-temporal loads/stores of the updated field */
   XF_TILE_SET = 256      /* the other bits are explicit, not auto        */
 };
@@ -258,6 +257,47 @@ int xf_device_count(int* count);
  * 6 the same non-temporal; blocks: grid size of 0..4 (0 = 8192). */
 int xf_stream_bench(int64_t bytes, int32_t reps, int32_t device, int32_t variant, int32_t blocks,
                     double* gbps);
+
+/* Tile shape of the 3-D Yee kernels, as set by hand or picked by an
+ * autotuner (no reference counterpart: the reference takes the user's
+ * grid/block dims, SimulationHD::setGridDim/setBlockDim,
+ * simulation_hd.cuh:51-55, and decomposes flat index ranges). */
+typedef struct xf_tuning {
+  int32_t tile_y;      /* rows of a tile, 1..4 (block = 64 x tile_y)        */
+  int32_t planes;      /* x planes one block marches through (>= 1)        */
+  int32_t blocks;      /* batch only: persistent grid size, 0 = resident   */
+  int32_t nontemporal; /* stream the updated component non-temporally      */
+  int32_t xcd_remap;   /* single grid only: XCD-aware block order          */
+  double step_ms;      /* autotune: measured time of one step              */
+} xf_tuning;
+
+/* Measures candidate tilings on the engine's own scene, trial_steps steps
+ * each after one warm-up step, restoring the state after every trial, so
+ * the simulation is unchanged; applies and returns the fastest. */
+int xf_autotune(xf_handle* h, int32_t trial_steps, xf_tuning* best);
+
+/* ---- Batches: many independent 3-D scenes of uneven sizes stepped
+ * together (no reference counterpart; the reference runs one
+ * SimulationHD per scene).  Every step is one persistent launch of the H
+ * kernel over the tiles of all grids, one batched TFSF launch, one
+ * persistent E launch, one batched TFSF + NF2FF launch; with
+ * plan.use_graph a step is captured once as a hipGraph and replayed.
+ * All scenes share plan.precision, the current step and the time axis;
+ * ADE media and 2-D grids are not accepted (XF_ERR_UNSUPPORTED). */
+typedef struct xf_batch xf_batch;
+
+int xf_batch_create(xf_scene* const* scenes, int32_t n, const xf_plan* plan, xf_batch** out);
+int xf_batch_upload(xf_batch* b);
+int xf_batch_run(xf_batch* b, int64_t n_steps);
+int xf_batch_download(xf_batch* b);
+int xf_batch_destroy(xf_batch* b);
+int xf_batch_set_tuning(xf_batch* b, const xf_tuning* t);
+int xf_batch_get_tuning(const xf_batch* b, xf_tuning* t);
+int xf_batch_autotune(xf_batch* b, int32_t trial_steps, xf_tuning* best);
+int xf_batch_get_timing(const xf_batch* b, xf_timing* out);
+int xf_batch_reset_timing(xf_batch* b);
+int xf_batch_set_timing(xf_batch* b, int32_t on);
+int xf_batch_describe(const xf_batch* b, char* buf, size_t len);
 
 #ifdef __cplusplus
 }

@@ -123,3 +123,50 @@ def test_run_past_tables_rejected():
     p = s.array("tfsf.h_inc")
     assert p.shape[0] == 20
     assert np.isfinite(e).all()
+
+
+def _batch_create(sims, **plan):
+    p = _native.XfPlan()
+    for k, v in plan.items():
+        setattr(p, k, v)
+    arr = (ctypes.c_void_p * max(1, len(sims)))(*[s.scene_ptr() for s in sims])
+    h = ctypes.c_void_p()
+    st = _native.hip().xf_batch_create(arr, len(sims), ctypes.byref(p), ctypes.byref(h))
+    return st, h
+
+
+def test_batch_create_validates_without_a_device():
+    sims = [scenes.dielectric_sphere(n=n, pml=3, steps=10, tfsf_inset=6, nf2ff_inset=4)
+            for n in (12, 15)]
+    st, h = _batch_create(sims)
+    assert st == 0, _msg()
+    if xf.device_count() == 0:
+        assert _native.hip().xf_batch_upload(h) == 2 and "device" in _msg()
+        assert _native.hip().xf_batch_run(h, 1) == 3
+    _native.hip().xf_batch_destroy(h)
+
+
+def test_batch_rejects_2d_ade_and_empty():
+    s3 = scenes.dielectric_sphere(n=12, pml=3, steps=10, tfsf_inset=6, nf2ff_inset=4)
+    s2 = scenes.cylinder_2d(n=40, pml=4, steps=10, tfsf_inset=8, nf2ff_inset=6)
+    st, _ = _batch_create([s3, s2])
+    assert st == 4 and "3-D" in _msg()
+    sd = xf.Simulation(3e-3, 3e-3, 3e-3, 0.9)
+    sd.add_cube("domain", (0, 0, 0), (0.03,) * 3, xf.Material.air())
+    sd.add_sphere("d", (0.015,) * 3, 0.006, xf.Material.drude(1.0, [1e10], [1e9]))
+    sd.init(10)
+    st, _ = _batch_create([s3, sd])
+    assert st == 4 and "ADE" in _msg()
+    st, _ = _batch_create([])
+    assert st == 1
+    st, _ = _batch_create([s3], precision=9)
+    assert st == 1 and "precision" in _msg()
+
+
+def test_batch_and_tuning_null_arguments():
+    lib = _native.hip()
+    t = _native.XfTuning()
+    assert lib.xf_batch_run(None, 1) == 1
+    assert lib.xf_batch_set_tuning(None, ctypes.byref(t)) == 1
+    assert lib.xf_batch_autotune(None, 1, ctypes.byref(t)) == 1
+    assert lib.xf_autotune(None, 1, ctypes.byref(t)) == 1

@@ -1,0 +1,187 @@
+"""Batched stepping of many independent grids (xf_batch_*) and the tile
+autotuner, against the oracle run grid by grid.
+
+The batched kernels run the same tile bodies as the single-grid kernels,
+only scheduled from one persistent launch over the tiles of all grids, so
+every grid of a FP64 batch must match its own oracle run bit for bit; FP32
+batches must match single-grid FP32 engine runs bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import xfdtd_cuda_amd as xf
+from xfdtd_cuda_amd import scenes
+from conftest import FIELDS
+
+pytestmark = pytest.mark.gpu
+
+
+def _uneven(steps, count=5, seed=3, layout="compact"):
+    """Small grids of uneven sizes (odd extents, different CPML widths,
+    one without TFSF, one without NF2FF)."""
+    rng = np.random.default_rng(seed)
+    sims = []
+    for g in range(count):
+        n = int(rng.integers(9, 24))
+        pml = int(rng.integers(3, 6))
+        t_in = 4 if n >= 12 else 3
+        radius = float(rng.uniform(0.2, 0.35)) * n * 3e-3
+        eps_r = float(rng.uniform(1.5, 8))
+        if g != 1:
+            s = scenes.dielectric_sphere(
+                n=n, pml=pml, steps=steps, dl=3e-3, tfsf_inset=pml + t_in,
+                nf2ff_inset=(pml + 1) if g != 2 else 0, radius=radius, eps_r=eps_r, mu_r=1.0,
+                layout=layout, init=False)
+        else:  # no TFSF: seed the field instead
+            s = xf.Simulation(3e-3, 3e-3, 3e-3, 0.9)
+            h = n * 3e-3 / 2
+            s.add_cube("domain", (-h, -h, -h), (n * 3e-3,) * 3, xf.Material.air())
+            s.add_sphere("s", (0, 0, 0), 0.3 * n * 3e-3, xf.Material.dielectric(2.5, 1.0))
+            s.add_pml_all(pml)
+            s.add_nf2ff_fd((pml + 1,) * 3, [1e9])
+            s.set_coefficient_layout(layout)
+        s.init(steps)
+        if g == 1:
+            ez = s.array("ez")
+            c = ez.shape[0] // 2
+            ez[c - 1:c + 2, c - 1:c + 2, :] = 1.0
+        sims.append(s)
+    return sims
+
+
+def _oracle_copies(build, steps):
+    sims = build()
+    for s in sims:
+        oracle.run(s.scene_ptr(), steps)
+    return sims
+
+
+def _assert_identical(gpu_sims, ref_sims, label):
+    for g, (a, b) in enumerate(zip(gpu_sims, ref_sims)):
+        for n in FIELDS:
+            assert np.array_equal(a.array(n), b.array(n)), f"{label}: grid {g} {n} differs"
+        for p in range(6):
+            try:
+                pa = a.array(f"pml.{p}.ea_psi_hb")
+            except xf.XFError:
+                break
+            assert np.array_equal(pa, b.array(f"pml.{p}.ea_psi_hb")), f"{label}: grid {g} psi {p}"
+        try:
+            ja = a.nf2ff_current(0, 0, "ZP", "ja")
+        except xf.XFError:
+            continue
+        assert np.array_equal(ja, b.nf2ff_current(0, 0, "ZP", "ja")), f"{label}: grid {g} nf2ff"
+
+
+def test_batch_matches_oracle_grid_by_grid(gpu_available):
+    steps = 50
+    build = lambda: _uneven(steps)
+    ref = _oracle_copies(build, steps)
+    sims = build()
+    b = xf.Batch(sims).upload()
+    b.run(20).run(steps - 20).download()
+    desc = b.describe()
+    b.close()
+    assert "grids=5" in desc
+    _assert_identical(sims, ref, "batch")
+    assert np.abs(ref[0].array("ex")).max() > 1e-3
+
+
+@pytest.mark.parametrize("tile_y,planes,blocks,nt", [
+    (1, 1, 0, True), (2, 3, 7, False), (4, 100, 1, True), (3, 2, 0, False), (1, 4, 13, True)])
+def test_batch_tunings_bit_identical(gpu_available, tile_y, planes, blocks, nt):
+    steps = 30
+    build = lambda: _uneven(steps, count=4, seed=11)
+    ref = _oracle_copies(build, steps)
+    sims = build()
+    b = xf.Batch(sims).upload()
+    b.set_tuning(tile_y=tile_y, planes=planes, blocks=blocks, nontemporal=nt)
+    b.run(steps).download()
+    b.close()
+    _assert_identical(sims, ref, f"tuning ty={tile_y} planes={planes} blocks={blocks}")
+
+
+def test_batch_graph_replay_matches_eager(gpu_available):
+    steps = 40
+    build = lambda: _uneven(steps, count=4, seed=5)
+    ref = _oracle_copies(build, steps)
+    sims = build()
+    b = xf.Batch(sims, use_graph=True).upload()
+    # several graph runs with host work and synchronisation between them
+    b.run(10)
+    b.download()
+    b.run(15).run(15).download()
+    b.close()
+    _assert_identical(sims, ref, "graph")
+
+
+def test_batch_autotune_leaves_state_unchanged(gpu_available):
+    steps = 40
+    build = lambda: _uneven(steps + 20, count=3, seed=9)
+    ref = build()
+    for s in ref:
+        oracle.run(s.scene_ptr(), steps)
+    sims = build()
+    b = xf.Batch(sims).upload()
+    b.run(5)
+    best = b.autotune(trial_steps=2)
+    assert 1 <= best["tile_y"] <= 4 and best["planes"] >= 1 and best["step_ms"] > 0
+    assert b.tuning()["tile_y"] == best["tile_y"]
+    b.run(steps - 5).download()
+    b.close()
+    _assert_identical(sims, ref, "autotune")
+
+
+def test_engine_autotune_leaves_state_unchanged(gpu_available):
+    steps = 40
+    build = lambda: scenes.dielectric_sphere(n=20, pml=5, steps=steps + 30, tfsf_inset=9,
+                                             nf2ff_inset=7)
+    c = build()
+    oracle.run(c.scene_ptr(), steps)
+    g = build()
+    eng = xf.Engine(g).upload()
+    eng.run(7)
+    best = eng.autotune(trial_steps=2)
+    assert best["step_ms"] > 0
+    eng.run(steps - 7).download()
+    eng.close()
+    for n in FIELDS:
+        assert np.array_equal(g.array(n), c.array(n)), n
+
+
+def test_batch_fp32_matches_single_grid_fp32(gpu_available):
+    steps = 40
+    build = lambda: _uneven(steps, count=4, seed=21)
+    singles = build()
+    for s in singles:
+        e = xf.Engine(s, precision="fp32").upload()
+        e.run(steps).download()
+        e.close()
+    sims = build()
+    b = xf.Batch(sims, precision="fp32").upload()
+    b.run(steps).download()
+    b.close()
+    _assert_identical(sims, singles, "fp32")
+
+
+def test_batch_mixed_id_widths(gpu_available):
+    """One grid needs 16-bit coefficient ids (> 256 distinct rows): the
+    whole batch switches to 16-bit ids and stays exact."""
+    steps = 25
+
+    def build():
+        sims = _uneven(steps, count=3, seed=2, layout="dense")
+        s = sims[0]
+        ce = s.array("coeff.0")  # cexe
+        rng = np.random.default_rng(0)
+        ce[...] = ce * (1.0 - 1e-4 * rng.integers(0, 300, ce.shape))
+        return sims
+
+    ref = _oracle_copies(build, steps)
+    sims = build()
+    b = xf.Batch(sims).upload()
+    assert "id_bytes=2" in b.describe()
+    b.run(steps).download()
+    b.close()
+    _assert_identical(sims, ref, "mixed ids")

@@ -107,6 +107,24 @@ def test_sphere_uneven_sizes_and_steps_in_chunks(gpu_available):
     _compare(g, c, label="uneven")
 
 
+@pytest.mark.parametrize("tile_y,chunks,xcd,nt", [
+    (1, 0, False, True), (2, 0, False, False), (3, 7, True, True), (4, 1, False, True),
+    (1, 1000, True, False), (2, 3, False, True)])
+def test_tiling_variants_bit_identical(gpu_available, tile_y, chunks, xcd, nt):
+    """Every tile shape / march length / block order / streaming flag gives
+    the same bits: the tiling only reorders independent cell updates."""
+    build = lambda: scenes.dielectric_sphere(n=21, pml=5, steps=60, tfsf_inset=8, nf2ff_inset=6)
+    g, c = build(), build()
+    eng = xf.Engine(g).upload()
+    eng.set_tiling(tile_y=tile_y, chunks_x=chunks, xcd_remap=xcd, nontemporal=nt)
+    eng.run(60).download()
+    eng.close()
+    oracle.run(c.scene_ptr(), 60)
+    for n in FIELDS:
+        assert np.array_equal(g.array(n), c.array(n)), f"{n} differs (ty={tile_y} ch={chunks})"
+    _compare(g, c, label=f"tiling ty={tile_y} ch={chunks}")
+
+
 def test_oblique_incidence_all_quadrants(gpu_available):
     """TFSF with incident directions in every octant sign pattern."""
     for theta, phi in ((2.3, 4.0), (0.4, 2.2), (1.9, 5.5)):
@@ -238,3 +256,29 @@ def test_config0_geometry(gpu_available):
     oracle.run(c.scene_ptr(), 150)
     _compare(g, c, names=list(FIELDS) + ["pml.0.ea_psi_hb", "pml.5.hb_psi_ea",
                                          "nf2ff.0.0.ZN.ja", "nf2ff.0.0.XP.mb"], label="cfg0")
+
+
+def test_cpp_dropin_driver(gpu_available, tmp_path):
+    """tests/cpp/simulation_hd_driver.cpp: the reference driver's C++
+    calling sequence against xfdtd::cuda::SimulationHD; its Ex and far field
+    match the oracle run of the same scene."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "simulation_hd_driver")
+    out = tmp_path / "drv.bin"
+    r = subprocess.run([exe, "20", "90", str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.int64, count=4)
+    nx, ny, nz, steps = (int(v) for v in raw)
+    data = np.fromfile(out, dtype=np.float64)[4:]
+    ex = data[: nx * (ny + 1) * (nz + 1)].reshape(nx, ny + 1, nz + 1)
+    rcs = data[nx * (ny + 1) * (nz + 1):]
+    edge = 20 * 5e-3
+    c = scenes.dielectric_sphere(n=20, pml=6, steps=90, dl=5e-3, tfsf_inset=9, nf2ff_inset=7,
+                                 radius=edge * 0.1 / 0.35)
+    oracle.run(c.scene_ptr(), 90)
+    assert steps == 90
+    assert rel_l2(ex, c.array("ex")) <= TOL
+    theta = -np.pi + 2 * np.pi * np.arange(360) / 359.0
+    _, _, rcs_c = c.far_field(0, 0, theta, 0.0, c.incident_spectrum(1e9))
+    assert rel_l2(rcs, rcs_c) <= 1e-10

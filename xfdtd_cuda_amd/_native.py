@@ -47,7 +47,6 @@ class XfPlan(ctypes.Structure):
 
 
 TILE_XCD_REMAP = 1
-TILE_PIPELINE = 2
 TILE_NT = 4
 TILE_SET = 256
 
@@ -67,6 +66,19 @@ class XfTiming(ctypes.Structure):
         ("steps", c_i64),
         ("bytes_per_cell_step", c_f64),
         ("run_ms", c_f64),
+    ]
+
+
+class XfTuning(ctypes.Structure):
+    """xf_tuning of include/xfdtd_hip.h."""
+
+    _fields_ = [
+        ("tile_y", c_i32),
+        ("planes", c_i32),
+        ("blocks", c_i32),
+        ("nontemporal", c_i32),
+        ("xcd_remap", c_i32),
+        ("step_ms", c_f64),
     ]
 
 
@@ -113,6 +125,26 @@ def hip() -> ctypes.CDLL:
         lib.xf_stream_bench.restype = ctypes.c_int
         lib.xf_set_tiling.argtypes = [c_vp, c_i32, c_i32, c_i32]
         lib.xf_set_tiling.restype = ctypes.c_int
+        pt = ctypes.POINTER(XfTuning)
+        batch_sigs = {
+            "xf_autotune": [c_vp, c_i32, pt],
+            "xf_batch_create": [ctypes.POINTER(c_vp), c_i32, ctypes.POINTER(XfPlan),
+                                ctypes.POINTER(c_vp)],
+            "xf_batch_upload": [c_vp],
+            "xf_batch_run": [c_vp, c_i64],
+            "xf_batch_download": [c_vp],
+            "xf_batch_destroy": [c_vp],
+            "xf_batch_set_tuning": [c_vp, pt],
+            "xf_batch_get_tuning": [c_vp, pt],
+            "xf_batch_autotune": [c_vp, c_i32, pt],
+            "xf_batch_get_timing": [c_vp, ctypes.POINTER(XfTiming)],
+            "xf_batch_reset_timing": [c_vp],
+            "xf_batch_set_timing": [c_vp, c_i32],
+            "xf_batch_describe": [c_vp, ctypes.c_char_p, ctypes.c_size_t],
+        }
+        for name, args in batch_sigs.items():
+            getattr(lib, name).argtypes = args
+            getattr(lib, name).restype = ctypes.c_int
         for fn in ("xf_create", "xf_upload", "xf_run", "xf_download", "xf_destroy",
                    "xf_current_step", "xf_get_timing", "xf_reset_timing", "xf_set_timing",
                    "xf_describe",

@@ -62,7 +62,7 @@ def build_driver():
         return
     out = os.path.join(ROOT, "tests", "cpp", "simulation_hd_driver")
     _run([CXX, *HOST_FLAGS, "-o", out, src, "-L" + HERE, "-lxfdtd_host", "-lxfdtd_hip",
-          "-Wl,-rpath," + HERE], ROOT)
+          "-Wl,-rpath,$ORIGIN/../../xfdtd_cuda_amd"], ROOT)
 
 
 def build_all():

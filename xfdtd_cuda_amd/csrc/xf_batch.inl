@@ -1,0 +1,646 @@
+// xf_batch.inl -- many independent 3-D grids stepped together, and the
+// tile-shape autotuner (single grid and batch).  Included at the end of
+// xf_engine.hip: each grid of a batch is a full Engine (its own fields,
+// tables, CPML, TFSF and NF2FF state, uploaded and downloaded as usual); the
+// batch only replaces the per-grid launches of a step by one launch per
+// kernel over all grids.
+//
+// Work list.  The H (and E) pass of grid g consists of nkb * njb * nch tiles
+// (64 k x tile_y j x `planes` i).  The tiles of all grids are concatenated
+// (tile_start[g] = first tile of grid g) and a persistent grid of blocks,
+// sized to what the device holds resident, walks the list with a stride of
+// gridDim.x.  Consecutive tiles are neighbours in k, then j, so the blocks in
+// flight at any moment cover a compact slab of one or two grids.
+#include <climits>
+#include <memory>
+
+namespace xf {
+
+struct Batch {
+  std::vector<std::unique_ptr<xf_handle>> inst;
+  xf_plan plan{};
+  int device{0};
+  hipStream_t stream{};
+  bool uploaded{false};
+  int64_t cur_step{0};
+  int64_t nt{INT64_MAX};  // end of the shortest time table
+  int id_bytes{1};
+  bool any_pml{false};
+  int64_t cells{0};
+  xf_tuning tune{1, 4, 0, 1, 0, 0.0};
+  // device descriptors (sizes fixed by the scenes, rewritten by configure)
+  std::vector<void*> allocs;
+  void* pass_h{};
+  void* pass_e{};
+  int* tile_start{};
+  int total_tiles{0};
+  int grid_blocks{0};
+  int resident{0};
+  void* tf_items[2]{};
+  int64_t* tf_start[2]{};
+  int n_tf[2]{0, 0};
+  int64_t tf_total[2]{0, 0};
+  void* nf_items{};
+  int64_t* nf_start{};
+  int n_nf{0};
+  int64_t nf_total{0};
+  int* t_dev{};
+  hipGraph_t graph{};
+  hipGraphExec_t graph_exec{};
+  bool graph_ready{false};
+  xf_timing timing{};
+  std::string desc;
+
+  ~Batch() { release(); }
+
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    graph_ready = false;
+  }
+
+  void release() {
+    drop_graph();
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+    pass_h = pass_e = nullptr;
+    tile_start = nullptr;
+    tf_items[0] = tf_items[1] = nf_items = nullptr;
+    tf_start[0] = tf_start[1] = nf_start = nullptr;
+    t_dev = nullptr;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+    uploaded = false;
+  }
+
+  int alloc(void** p, size_t bytes) {
+    *p = nullptr;
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) return fail(XF_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(*p);
+    return XF_OK;
+  }
+
+  // copies host bytes into *p, allocating it on first use
+  int put(void** p, const void* src, size_t bytes) {
+    if (!*p) XF_TRY(alloc(p, bytes));
+    if (bytes) XF_HIP(hipMemcpy(*p, src, bytes, hipMemcpyHostToDevice));
+    return XF_OK;
+  }
+
+  Engine& eng(size_t g) { return inst[g]->eng; }
+
+  int create(xf_scene* const* scenes, int n, const xf_plan* pl);
+  int upload();
+  int configure();
+  template <typename S>
+  int configure_t();
+  template <typename S>
+  int launch_step(int t, bool use_tdev, bool ev_on, hipEvent_t* ev);
+  int run(int64_t n);
+  int download();
+  int snapshot();
+  int restore();
+  int autotune(int trial_steps, xf_tuning* best);
+};
+
+int Batch::create(xf_scene* const* scenes, int n, const xf_plan* pl) {
+  if (n < 1) return fail(XF_ERR_INVALID, "a batch needs at least one scene");
+  if (pl) plan = *pl;
+  if (plan.precision < XF_FP64 || plan.precision > XF_BF16_STORE) return fail(XF_ERR_INVALID, "unknown precision");
+  device = plan.device;
+  xf_plan p = plan;
+  p.use_graph = 0;
+  p.timing = 0;
+  for (int g = 0; g < n; ++g) {
+    if (!scenes[g]) return fail(XF_ERR_INVALID, "null scene " + std::to_string(g));
+    xf_handle* h = nullptr;
+    const int st = xf_create(scenes[g], &p, &h);
+    if (st != XF_OK) return fail(st, "scene " + std::to_string(g) + ": " + g_err);
+    inst.emplace_back(h);
+    Engine& E = h->eng;
+    if (E.dim != 3) return fail(XF_ERR_UNSUPPORTED, "scene " + std::to_string(g) + ": batches take 3-D grids only");
+    if (E.has_ade) return fail(XF_ERR_UNSUPPORTED, "scene " + std::to_string(g) + ": ADE media are not batched");
+    if (g == 0) cur_step = E.cur_step;
+    if (E.cur_step != cur_step) return fail(XF_ERR_INVALID, "scenes of a batch must share the current step");
+    if (E.sc->tfsf || E.sc->n_nf2ff > 0) nt = std::min(nt, E.nt);
+    cells += E.nx * E.ny * E.nz;
+  }
+  return XF_OK;
+}
+
+int Batch::upload() {
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) return fail(XF_ERR_HIP, "no HIP device");
+  if (device < 0 || device >= nd) return fail(XF_ERR_INVALID, "bad device ordinal");
+  release();
+  for (auto& h : inst) XF_TRY(h->eng.upload());
+  // one id width for the whole batch (the kernels are compiled per width)
+  bool wide = false;
+  for (auto& h : inst) wide = wide || h->eng.id_bytes == 2;
+  if (wide) {
+    for (auto& h : inst) {
+      if (h->eng.id_bytes == 2) continue;
+      h->eng.wide_ids = true;
+      XF_TRY(h->eng.upload());
+    }
+  }
+  id_bytes = wide ? 2 : 1;
+  XF_HIP(hipSetDevice(device));
+  XF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  void* tp;
+  XF_TRY(alloc(&tp, sizeof(int)));
+  t_dev = static_cast<int*>(tp);
+  XF_TRY(configure());
+  uploaded = true;
+  return XF_OK;
+}
+
+namespace {
+template <typename S, typename I, bool PM, bool NT>
+int batch_occupancy(int threads) {
+  int a = 0, b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_yee_h3_batch<S, I, PM, NT>, threads, 0) != hipSuccess) a = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_yee_e3_batch<S, I, PM, NT>, threads, 0) != hipSuccess) b = 1;
+  return std::max(1, std::min(a, b));
+}
+
+template <typename S>
+int batch_occupancy_any(int id_bytes, bool pml, bool nt, int threads) {
+#define XF_OCC(I, PM, NT) return batch_occupancy<S, I, PM, NT>(threads)
+  if (id_bytes == 1) {
+    if (pml) { if (nt) XF_OCC(uint8_t, true, true); else XF_OCC(uint8_t, true, false); }
+    else { if (nt) XF_OCC(uint8_t, false, true); else XF_OCC(uint8_t, false, false); }
+  } else {
+    if (pml) { if (nt) XF_OCC(uint16_t, true, true); else XF_OCC(uint16_t, true, false); }
+    else { if (nt) XF_OCC(uint16_t, false, true); else XF_OCC(uint16_t, false, false); }
+  }
+#undef XF_OCC
+}
+}  // namespace
+
+template <typename S>
+int Batch::configure_t() {
+  using C = typename Store<S>::C;
+  const int n = static_cast<int>(inst.size());
+  std::vector<YeePass<S>> ph(n), pe(n);
+  std::vector<int> ts(n + 1, 0);
+  int64_t tiles = 0;
+  any_pml = false;
+  const int flags = XF_TILE_SET | (tune.nontemporal ? XF_TILE_NT : 0);
+  for (int g = 0; g < n; ++g) {
+    Engine& E = eng(g);
+    const int64_t chunks = (E.nx + 1 + tune.planes - 1) / tune.planes;
+    E.set_tiling(tune.tile_y, static_cast<int>(chunks), flags);
+    ph[g] = E.yee_pass<S>(false);
+    pe[g] = E.yee_pass<S>(true);
+    any_pml = any_pml || ph[g].pml.n > 0;
+    ts[g] = static_cast<int>(tiles);
+    tiles += static_cast<int64_t>(ph[g].nkb) * ph[g].njb * ph[g].nch;
+    if (tiles > INT_MAX / 2) return fail(XF_ERR_UNSUPPORTED, "batch too large for 32-bit tile ids");
+  }
+  ts[n] = static_cast<int>(tiles);
+  total_tiles = static_cast<int>(tiles);
+  XF_TRY(put(&pass_h, ph.data(), ph.size() * sizeof(YeePass<S>)));
+  XF_TRY(put(&pass_e, pe.data(), pe.size() * sizeof(YeePass<S>)));
+  void* p = tile_start;
+  XF_TRY(put(&p, ts.data(), ts.size() * sizeof(int)));
+  tile_start = static_cast<int*>(p);
+  // TFSF passes and NF2FF boxes
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<TfsfItem<S>> items;
+    std::vector<int64_t> st(1, 0);
+    for (int g = 0; g < n; ++g) {
+      Engine& E = eng(g);
+      if (!E.sc->tfsf) continue;
+      TfsfItem<S> it{};
+      it.f = make_fields<S>(E);
+      it.sc = make_scales<C>(E);
+      it.d = make_dims(E);
+      fill_tfsf_pass<C>(E, it.P, pass);
+      if (it.P.total == 0) continue;
+      items.push_back(it);
+      st.push_back(st.back() + it.P.total);
+    }
+    n_tf[pass] = static_cast<int>(items.size());
+    tf_total[pass] = st.back();
+    XF_TRY(put(&tf_items[pass], items.data(), items.size() * sizeof(TfsfItem<S>)));
+    void* q = tf_start[pass];
+    XF_TRY(put(&q, st.data(), st.size() * sizeof(int64_t)));
+    tf_start[pass] = static_cast<int64_t*>(q);
+  }
+  {
+    std::vector<NfItem<S>> items;
+    std::vector<int64_t> st(1, 0);
+    for (int g = 0; g < n; ++g) {
+      Engine& E = eng(g);
+      for (auto& N : E.nf) {
+        if (N.pass.total == 0) continue;
+        NfItem<S> it{};
+        it.f = make_fields<S>(E);
+        it.sc = make_scales<C>(E);
+        it.d = make_dims(E);
+        it.P = N.pass;
+        items.push_back(it);
+        st.push_back(st.back() + N.pass.total);
+      }
+    }
+    n_nf = static_cast<int>(items.size());
+    nf_total = st.back();
+    XF_TRY(put(&nf_items, items.data(), items.size() * sizeof(NfItem<S>)));
+    void* q = nf_start;
+    XF_TRY(put(&q, st.data(), st.size() * sizeof(int64_t)));
+    nf_start = static_cast<int64_t*>(q);
+  }
+  // persistent grid: what the device holds resident, unless set
+  hipDeviceProp_t prop;
+  XF_HIP(hipGetDeviceProperties(&prop, device));
+  resident = batch_occupancy_any<S>(id_bytes, any_pml, tune.nontemporal != 0, 64 * tune.tile_y) *
+             prop.multiProcessorCount;
+  grid_blocks = tune.blocks > 0 ? tune.blocks : resident;
+  grid_blocks = std::max(1, std::min(grid_blocks, total_tiles));
+  return XF_OK;
+}
+
+int Batch::configure() {
+  if (tune.tile_y < 1 || tune.tile_y > 4) return fail(XF_ERR_INVALID, "tile_y must be 1..4");
+  if (tune.planes < 1) return fail(XF_ERR_INVALID, "planes must be >= 1");
+  if (tune.blocks < 0) return fail(XF_ERR_INVALID, "blocks must be >= 0");
+  XF_HIP(hipSetDevice(device));
+  drop_graph();
+  int st;
+  switch (plan.precision) {
+    case XF_FP64: st = configure_t<double>(); break;
+    case XF_FP32: st = configure_t<float>(); break;
+    default: st = configure_t<__hip_bfloat16>(); break;
+  }
+  if (st != XF_OK) return st;
+  desc = "grids=" + std::to_string(inst.size()) + " cells=" + std::to_string(cells) +
+         " tiles=" + std::to_string(total_tiles) + " blocks=" + std::to_string(grid_blocks) +
+         " resident=" + std::to_string(resident) + " tile_y=" + std::to_string(tune.tile_y) +
+         " planes=" + std::to_string(tune.planes) + " nt=" + std::to_string(tune.nontemporal) +
+         " id_bytes=" + std::to_string(id_bytes) + " tfsf=" + std::to_string(n_tf[0]) +
+         " nf2ff=" + std::to_string(n_nf);
+  return XF_OK;
+}
+
+template <typename S>
+int Batch::launch_step(int t, bool use_tdev, bool ev_on, hipEvent_t* ev) {
+  const int* tp = use_tdev ? t_dev : nullptr;
+  const int n = static_cast<int>(inst.size());
+  const dim3 block(64, tune.tile_y), grid(grid_blocks);
+  const bool nt_on = tune.nontemporal != 0;
+#define XF_BY(KER, I, PM, NT, PASS) \
+  hipLaunchKernelGGL((KER<S, I, PM, NT>), grid, block, 0, stream, static_cast<const YeePass<S>*>(PASS), tile_start, n)
+#define XF_BYALL(KER, PASS)                                                                    \
+  do {                                                                                         \
+    if (id_bytes == 1) {                                                                       \
+      if (any_pml) { if (nt_on) XF_BY(KER, uint8_t, true, true, PASS); else XF_BY(KER, uint8_t, true, false, PASS); } \
+      else { if (nt_on) XF_BY(KER, uint8_t, false, true, PASS); else XF_BY(KER, uint8_t, false, false, PASS); }       \
+    } else {                                                                                   \
+      if (any_pml) { if (nt_on) XF_BY(KER, uint16_t, true, true, PASS); else XF_BY(KER, uint16_t, true, false, PASS); } \
+      else { if (nt_on) XF_BY(KER, uint16_t, false, true, PASS); else XF_BY(KER, uint16_t, false, false, PASS); }       \
+    }                                                                                          \
+  } while (0)
+  if (ev_on) XF_HIP(hipEventRecord(ev[0], stream));
+  XF_BYALL(k_yee_h3_batch, pass_h);
+  if (ev_on) XF_HIP(hipEventRecord(ev[1], stream));
+  if (tf_total[0] > 0)
+    hipLaunchKernelGGL((k_tfsf_batch<S>), dim3((unsigned)((tf_total[0] + 255) / 256)), dim3(256), 0, stream,
+                       static_cast<const TfsfItem<S>*>(tf_items[0]), tf_start[0], n_tf[0], t, tp);
+  if (ev_on) XF_HIP(hipEventRecord(ev[2], stream));
+  XF_BYALL(k_yee_e3_batch, pass_e);
+  if (ev_on) XF_HIP(hipEventRecord(ev[3], stream));
+  if (tf_total[1] > 0)
+    hipLaunchKernelGGL((k_tfsf_batch<S>), dim3((unsigned)((tf_total[1] + 255) / 256)), dim3(256), 0, stream,
+                       static_cast<const TfsfItem<S>*>(tf_items[1]), tf_start[1], n_tf[1], t, tp);
+  if (nf_total > 0)
+    hipLaunchKernelGGL((k_nf2ff_batch<S>), dim3((unsigned)((nf_total + 255) / 256)), dim3(256), 0, stream,
+                       static_cast<const NfItem<S>*>(nf_items), nf_start, n_nf, t, tp);
+  if (ev_on) XF_HIP(hipEventRecord(ev[4], stream));
+  if (use_tdev) hipLaunchKernelGGL(k_next_step, dim3(1), dim3(1), 0, stream, t_dev);
+#undef XF_BYALL
+#undef XF_BY
+  XF_HIP(hipGetLastError());
+  return XF_OK;
+}
+
+int Batch::run(int64_t n) {
+  if (!uploaded) return fail(XF_ERR_STATE, "run before upload");
+  if (n < 0) return fail(XF_ERR_INVALID, "negative step count");
+  if (n == 0) return XF_OK;
+  if (nt != INT64_MAX && cur_step + n > nt)
+    return fail(XF_ERR_INVALID, "run past the end of the time axis the TFSF/NF2FF tables were built for");
+  XF_HIP(hipSetDevice(device));
+  const bool timing_on = plan.timing != 0;
+  const bool graph_on = plan.use_graph != 0 && !timing_on;
+  auto step = [&](int t, bool tdev, bool ev_on, hipEvent_t* e) -> int {
+    switch (plan.precision) {
+      case XF_FP64: return launch_step<double>(t, tdev, ev_on, e);
+      case XF_FP32: return launch_step<float>(t, tdev, ev_on, e);
+      default: return launch_step<__hip_bfloat16>(t, tdev, ev_on, e);
+    }
+  };
+  std::vector<hipEvent_t> ev;
+  if (timing_on) {
+    ev.resize(5 * n);
+    for (auto& e : ev) XF_HIP(hipEventCreate(&e));
+  }
+  hipEvent_t run0, run1;
+  XF_HIP(hipEventCreate(&run0));
+  XF_HIP(hipEventCreate(&run1));
+  int status = XF_OK;
+  if (graph_on) {
+    if (!graph_ready) {
+      XF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      const int st = step(0, true, false, nullptr);
+      hipGraph_t g;
+      const hipError_t ce = hipStreamEndCapture(stream, &g);
+      if (st != XF_OK) return st;
+      if (ce != hipSuccess) return fail(XF_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      graph = g;
+      XF_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+      graph_ready = true;
+    }
+    const int t0 = static_cast<int>(cur_step);
+    XF_HIP(hipMemcpyAsync(t_dev, &t0, sizeof(int), hipMemcpyHostToDevice, stream));
+    XF_HIP(hipStreamSynchronize(stream));
+    XF_HIP(hipEventRecord(run0, stream));
+    for (int64_t s = 0; s < n; ++s) XF_HIP(hipGraphLaunch(graph_exec, stream));
+  } else {
+    XF_HIP(hipEventRecord(run0, stream));
+    for (int64_t s = 0; s < n && status == XF_OK; ++s)
+      status = step(static_cast<int>(cur_step + s), false, timing_on, timing_on ? &ev[5 * s] : nullptr);
+  }
+  XF_HIP(hipEventRecord(run1, stream));
+  XF_HIP(hipEventSynchronize(run1));
+  if (status != XF_OK) return status;
+  XF_HIP(hipGetLastError());
+  float ms = 0;
+  XF_HIP(hipEventElapsedTime(&ms, run0, run1));
+  timing.run_ms = ms;
+  timing.steps += n;
+  timing.step_ms = ms / static_cast<double>(n);
+  if (timing_on) {
+    for (int64_t s = 0; s < n; ++s) {
+      float a, b, c, e;
+      XF_HIP(hipEventElapsedTime(&a, ev[5 * s], ev[5 * s + 1]));
+      XF_HIP(hipEventElapsedTime(&b, ev[5 * s + 1], ev[5 * s + 2]));
+      XF_HIP(hipEventElapsedTime(&c, ev[5 * s + 2], ev[5 * s + 3]));
+      XF_HIP(hipEventElapsedTime(&e, ev[5 * s + 3], ev[5 * s + 4]));
+      timing.yee_h_ms += a;
+      timing.tfsf_ms += b;
+      timing.yee_e_ms += c;
+      timing.nf2ff_ms += e;
+    }
+    timing.yee_h_launches += n;
+    timing.yee_e_launches += n;
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+  (void)hipEventDestroy(run0);
+  (void)hipEventDestroy(run1);
+  // algorithmic bytes per cell-step, cell-weighted over the grids
+  double bytes = 0;
+  for (auto& h : inst) bytes += h->eng.bytes_per_cell_step() * static_cast<double>(h->eng.nx * h->eng.ny * h->eng.nz);
+  timing.bytes_per_cell_step = bytes / static_cast<double>(cells);
+  cur_step += n;
+  for (auto& h : inst) h->eng.cur_step = cur_step;
+  return XF_OK;
+}
+
+int Batch::download() {
+  if (!uploaded) return fail(XF_ERR_STATE, "download before upload");
+  XF_HIP(hipStreamSynchronize(stream));
+  for (auto& h : inst) XF_TRY(h->eng.download());
+  return XF_OK;
+}
+
+int Batch::snapshot() {
+  for (auto& h : inst) XF_TRY(h->eng.snapshot());
+  return XF_OK;
+}
+
+int Batch::restore() {
+  XF_HIP(hipStreamSynchronize(stream));
+  for (auto& h : inst) XF_TRY(h->eng.restore());
+  return XF_OK;
+}
+
+// Candidate tilings: rows per tile x planes per march x persistent grid
+// (resident, half of it) x non-temporal streams.  Each runs one warm-up
+// step and trial_steps timed steps from the same state.
+int Batch::autotune(int trial_steps, xf_tuning* best) {
+  if (!uploaded) return fail(XF_ERR_STATE, "autotune before upload");
+  if (trial_steps < 1) return fail(XF_ERR_INVALID, "trial_steps must be >= 1");
+  if (nt != INT64_MAX && cur_step + trial_steps + 1 > nt)
+    return fail(XF_ERR_INVALID, "not enough time-table steps left to tune");
+  XF_TRY(snapshot());
+  const int64_t step0 = cur_step;
+  const xf_timing keep = timing;
+  const int keep_timing = plan.timing;
+  plan.timing = 0;
+  xf_tuning win{};
+  win.step_ms = 1e30;
+  int status = XF_OK;
+  for (int ty : {1, 2, 4}) {
+    for (int planes : {2, 4, 8}) {
+      for (int nt_on : {1, 0}) {
+        for (int half : {0, 1}) {
+          tune = xf_tuning{ty, planes, 0, nt_on, 0, 0.0};
+          status = configure();
+          if (status == XF_OK && half) {
+            tune.blocks = std::max(1, resident / 2);
+            status = configure();
+          }
+          if (status == XF_OK) status = run(1);
+          if (status == XF_OK) status = run(trial_steps);
+          if (status != XF_OK) break;
+          const double ms = timing.run_ms / trial_steps;
+          if (ms < win.step_ms) {
+            win = tune;
+            win.step_ms = ms;
+          }
+          cur_step = step0;
+          for (auto& h : inst) h->eng.cur_step = step0;
+          status = restore();
+          if (status != XF_OK) break;
+        }
+        if (status != XF_OK) break;
+      }
+      if (status != XF_OK) break;
+    }
+    if (status != XF_OK) break;
+  }
+  plan.timing = keep_timing;
+  timing = keep;
+  cur_step = step0;
+  for (auto& h : inst) h->eng.cur_step = step0;
+  if (status != XF_OK) {
+    (void)restore();
+    return status;
+  }
+  tune = win;
+  XF_TRY(configure());
+  if (best) *best = win;
+  return XF_OK;
+}
+
+// Single grid: rows per tile x planes per march x non-temporal streams.
+int Engine::autotune(int trial_steps, xf_tuning* best) {
+  if (!uploaded) return fail(XF_ERR_STATE, "autotune before upload");
+  if (trial_steps < 1) return fail(XF_ERR_INVALID, "trial_steps must be >= 1");
+  if ((sc->tfsf || !nf.empty()) && cur_step + trial_steps + 1 > nt)
+    return fail(XF_ERR_INVALID, "not enough time-table steps left to tune");
+  XF_TRY(snapshot());
+  const int64_t step0 = cur_step;
+  const xf_timing keep = timing;
+  const int keep_timing = plan.timing;
+  plan.timing = 0;
+  xf_tuning win{};
+  win.step_ms = 1e30;
+  int status = XF_OK;
+  const std::vector<int> tys = dim == 3 ? std::vector<int>{1, 2, 4} : std::vector<int>{1};
+  const std::vector<int> pls = dim == 3 ? std::vector<int>{2, 4, 8, 16} : std::vector<int>{1};
+  for (int ty : tys) {
+    for (int planes : pls) {
+      for (int nt_on : {1, 0}) {
+        const int chunks = static_cast<int>((nx + 1 + planes - 1) / planes);
+        set_tiling(ty, chunks, XF_TILE_SET | (nt_on ? XF_TILE_NT : 0));
+        status = run(1);
+        if (status == XF_OK) status = run(trial_steps);
+        if (status != XF_OK) break;
+        const double ms = timing.run_ms / trial_steps;
+        if (ms < win.step_ms) win = xf_tuning{ty, planes, 0, nt_on, 0, ms};
+        cur_step = step0;
+        status = restore();
+        if (status != XF_OK) break;
+      }
+      if (status != XF_OK) break;
+    }
+    if (status != XF_OK) break;
+  }
+  plan.timing = keep_timing;
+  timing = keep;
+  cur_step = step0;
+  if (status != XF_OK) {
+    (void)restore();
+    return status;
+  }
+  set_tiling(win.tile_y, static_cast<int>((nx + 1 + win.planes - 1) / win.planes),
+             XF_TILE_SET | (win.nontemporal ? XF_TILE_NT : 0));
+  if (best) *best = win;
+  return XF_OK;
+}
+
+}  // namespace xf
+
+struct xf_batch {
+  xf::Batch b;
+};
+
+extern "C" {
+
+int xf_autotune(xf_handle* h, int32_t trial_steps, xf_tuning* best) {
+  xf::g_err.clear();
+  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+  return h->eng.autotune(trial_steps, best);
+}
+
+int xf_batch_create(xf_scene* const* scenes, int32_t n, const xf_plan* plan, xf_batch** out) {
+  xf::g_err.clear();
+  if (!scenes || !out) return xf::fail(XF_ERR_INVALID, "null scenes or out");
+  *out = nullptr;
+  auto b = new (std::nothrow) xf_batch;
+  if (!b) return xf::fail(XF_ERR_NOMEM, "out of host memory");
+  const int st = b->b.create(scenes, n, plan);
+  if (st != XF_OK) {
+    delete b;
+    return st;
+  }
+  *out = b;
+  return XF_OK;
+}
+
+int xf_batch_upload(xf_batch* b) {
+  xf::g_err.clear();
+  if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
+  const int st = b->b.upload();
+  if (st != XF_OK) b->b.release();
+  return st;
+}
+
+int xf_batch_run(xf_batch* b, int64_t n_steps) {
+  xf::g_err.clear();
+  if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
+  return b->b.run(n_steps);
+}
+
+int xf_batch_download(xf_batch* b) {
+  xf::g_err.clear();
+  if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
+  return b->b.download();
+}
+
+int xf_batch_destroy(xf_batch* b) {
+  delete b;
+  return XF_OK;
+}
+
+int xf_batch_set_tuning(xf_batch* b, const xf_tuning* t) {
+  xf::g_err.clear();
+  if (!b || !t) return xf::fail(XF_ERR_INVALID, "null argument");
+  const xf_tuning keep = b->b.tune;
+  b->b.tune = *t;
+  b->b.tune.step_ms = 0;
+  if (t->tile_y < 1 || t->tile_y > 4 || t->planes < 1 || t->blocks < 0) {
+    b->b.tune = keep;
+    return xf::fail(XF_ERR_INVALID, "bad tuning (tile_y 1..4, planes >= 1, blocks >= 0)");
+  }
+  if (!b->b.uploaded) return XF_OK;
+  const int st = b->b.configure();
+  if (st != XF_OK) {
+    b->b.tune = keep;
+    (void)b->b.configure();
+  }
+  return st;
+}
+
+int xf_batch_get_tuning(const xf_batch* b, xf_tuning* t) {
+  if (!b || !t) return xf::fail(XF_ERR_INVALID, "null argument");
+  *t = b->b.tune;
+  return XF_OK;
+}
+
+int xf_batch_autotune(xf_batch* b, int32_t trial_steps, xf_tuning* best) {
+  xf::g_err.clear();
+  if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
+  return b->b.autotune(trial_steps, best);
+}
+
+int xf_batch_get_timing(const xf_batch* b, xf_timing* out) {
+  if (!b || !out) return xf::fail(XF_ERR_INVALID, "null argument");
+  *out = b->b.timing;
+  return XF_OK;
+}
+
+int xf_batch_reset_timing(xf_batch* b) {
+  if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
+  b->b.timing = xf_timing{};
+  return XF_OK;
+}
+
+int xf_batch_set_timing(xf_batch* b, int32_t on) {
+  if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
+  b->b.plan.timing = on ? 1 : 0;
+  return XF_OK;
+}
+
+int xf_batch_describe(const xf_batch* b, char* buf, size_t len) {
+  if (!b || !buf || len == 0) return xf::fail(XF_ERR_INVALID, "null argument");
+  std::snprintf(buf, len, "%s", b->b.desc.c_str());
+  return XF_OK;
+}
+
+}  // extern "C"

@@ -44,12 +44,6 @@ bool empty_task(const xf_task& t) {
 int64_t task_lo(const xf_task& t, int ax) { return ax == 0 ? t.x.start : ax == 1 ? t.y.start : t.z.start; }
 int64_t task_hi(const xf_task& t, int ax) { return ax == 0 ? t.x.end : ax == 1 ? t.y.end : t.z.end; }
 
-void xyz_to_abc(int axis, const int64_t x[3], int64_t abc[3]) {
-  abc[0] = x[(axis + 1) % 3];
-  abc[1] = x[(axis + 2) % 3];
-  abc[2] = x[axis];
-}
-
 int check_array(const xf_array& a, std::vector<int64_t> shape, const std::string& name) {
   if (a.data == nullptr) return fail(XF_ERR_INVALID, name + ": null data");
   if (a.ndim != static_cast<int32_t>(shape.size()))
@@ -99,12 +93,17 @@ struct Engine {
   int device{0};
   hipStream_t stream{};
   std::vector<void*> allocs;
+  // device buffers a time step rewrites (fields, psi, NF2FF sums, ADE
+  // state) and their backup for snapshot()/restore()
+  std::vector<std::pair<void*, size_t>> state;
+  std::vector<void*> backup;
   // fields
   void* fld[6]{};
   // coefficient ids and tables (compute type)
   void* eid{};
   void* hid{};
   int id_bytes{1};
+  bool wide_ids{false};  // force 16-bit ids (a batch needs one id width)
   int ntab_e{0}, ntab_h{0};
   void* etab{};
   void* htab{};
@@ -148,10 +147,14 @@ struct Engine {
   int chunk{1};
   int nkb{1}, njb{1}, nch_eff{1};
   int xcd{0};
-  int pipeline{1};
-  int nt_streams{0};
+  int nt_streams{1};
   void set_tiling(int ty, int chunks_x, int flags);
   Tiling tiling() const;
+  int snapshot();  // copy every state buffer aside (device to device)
+  int restore();   // put the copies back; the step counter is the caller's
+  int autotune(int trial_steps, xf_tuning* best);
+  template <typename S>
+  YeePass<S> yee_pass(bool is_e) const;
   std::string layout_desc;
 
   ~Engine() { release(); }
@@ -164,6 +167,8 @@ struct Engine {
     graph_ready = false;
     for (void* p : allocs) (void)hipFree(p);
     allocs.clear();
+    state.clear();
+    backup.clear();
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
     uploaded = false;
@@ -561,6 +566,7 @@ int Engine::pack_fields() {
         }
     XF_TRY(alloc(&fld[c], total * esize));
     XF_HIP(hipMemcpy(fld[c], buf.data(), total * esize, hipMemcpyHostToDevice));
+    state.emplace_back(fld[c], total * esize);
   }
   return XF_OK;
 }
@@ -712,7 +718,7 @@ int Engine::build_ids() {
   }
   if (ntab_e > 65536 || ntab_h > 65536)
     return fail(XF_ERR_UNSUPPORTED, "more than 65536 distinct coefficient rows; the engine needs a smaller coefficient variety");
-  id_bytes = (ntab_e <= 256 && ntab_h <= 256) ? 1 : 2;
+  id_bytes = (ntab_e <= 256 && ntab_h <= 256 && !wide_ids) ? 1 : 2;
   // scatter ids into the padded layout
   auto put = [&](void** dptr, const std::vector<uint32_t>& id) -> int {
     if (id_bytes == 1) {
@@ -765,6 +771,7 @@ int Engine::upload_pml() {
     for (int c = 0; c < 4; ++c) {
       XF_TRY(upload_c(&D.cpsi[c], cp[c]->data, numel(*cp[c])));
       XF_TRY(upload_c(&D.psi[c], ps[c]->data, numel(*ps[c])));
+      state.emplace_back(D.psi[c], numel(*ps[c]) * (plan.precision == XF_FP64 ? 8 : 4));
     }
   }
   return XF_OK;
@@ -838,6 +845,7 @@ int Engine::upload_nf2ff() {
     XF_TRY(upload_vec(&pe, te));
     XF_TRY(upload_vec(&ph, th));
     D.cur = static_cast<double2*>(pc);
+    state.emplace_back(pc, cur.size() * sizeof(double2));
     D.te = static_cast<double2*>(pe);
     D.th = static_cast<double2*>(ph);
     P.cur = D.cur;
@@ -918,10 +926,13 @@ int Engine::upload_ade() {
   ade.cjsjp = static_cast<const double*>(p);
   XF_TRY(upload_vec(&p, eprev));
   ade.eprev = static_cast<double*>(p);
+  state.emplace_back(p, eprev.size() * sizeof(double));
   XF_TRY(upload_vec(&p, jc));
   ade.jc = static_cast<double*>(p);
+  state.emplace_back(p, jc.size() * sizeof(double));
   XF_TRY(upload_vec(&p, jp));
   ade.jp = static_cast<double*>(p);
+  state.emplace_back(p, jp.size() * sizeof(double));
   return XF_OK;
 }
 
@@ -965,15 +976,13 @@ int Engine::upload() {
 // neighbour re-read every step of the march.
 void Engine::set_tiling(int ty, int chunks_x, int flags) {
   // 64 x tile_y threads must fit the kernels' __launch_bounds__(256)
-  tile_y = ty > 0 ? std::min(ty, 4) : 2;
+  tile_y = ty > 0 ? std::min(ty, 4) : 1;
   if (flags & XF_TILE_SET) {
     xcd = (flags & XF_TILE_XCD_REMAP) ? 1 : 0;
-    pipeline = (flags & XF_TILE_PIPELINE) ? 1 : 0;
     nt_streams = (flags & XF_TILE_NT) ? 1 : 0;
   } else {
     xcd = 0;
-    pipeline = 1;
-    nt_streams = 0;
+    nt_streams = 1;
   }
   if (dim == 3) {
     nkb = static_cast<int>((nz + 1 + 63) / 64);
@@ -993,10 +1002,31 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
                 " id_bytes=" + std::to_string(id_bytes) + " etab=" + std::to_string(ntab_e) +
                 " htab=" + std::to_string(ntab_h) + " chunk=" + std::to_string(chunk) +
                 " nchunk=" + std::to_string(nch_eff) + " tile_y=" + std::to_string(tile_y) +
-                " xcd=" + std::to_string(xcd) + " pipeline=" + std::to_string(pipeline) +
-                " nt=" + std::to_string(nt_streams) +
+                " xcd=" + std::to_string(xcd) + " nt=" + std::to_string(nt_streams) +
                 " pml_faces=" + std::to_string(pml.size()) +
                 " ade_cells=" + std::to_string(ade_cells.size());
+}
+
+int Engine::snapshot() {
+  if (!uploaded) return fail(XF_ERR_STATE, "snapshot before upload");
+  XF_HIP(hipSetDevice(device));
+  if (backup.size() != state.size()) {
+    backup.assign(state.size(), nullptr);
+    for (size_t q = 0; q < state.size(); ++q) XF_TRY(alloc(&backup[q], state[q].second));
+  }
+  for (size_t q = 0; q < state.size(); ++q)
+    XF_HIP(hipMemcpyAsync(backup[q], state[q].first, state[q].second, hipMemcpyDeviceToDevice, stream));
+  XF_HIP(hipStreamSynchronize(stream));
+  return XF_OK;
+}
+
+int Engine::restore() {
+  if (backup.size() != state.size()) return fail(XF_ERR_STATE, "restore without snapshot");
+  XF_HIP(hipSetDevice(device));
+  for (size_t q = 0; q < state.size(); ++q)
+    XF_HIP(hipMemcpyAsync(state[q].first, backup[q], state[q].second, hipMemcpyDeviceToDevice, stream));
+  XF_HIP(hipStreamSynchronize(stream));
+  return XF_OK;
 }
 
 Tiling Engine::tiling() const {
@@ -1134,6 +1164,25 @@ Dims make_dims(const Engine& E) {
 }  // namespace
 
 template <typename S>
+YeePass<S> Engine::yee_pass(bool is_e) const {
+  using C = typename Store<S>::C;
+  YeePass<S> D{};
+  D.f = make_fields<S>(*this);
+  D.ids = is_e ? eid : hid;
+  D.tab = static_cast<const C*>(is_e ? etab : htab);
+  D.ntab = is_e ? ntab_e : ntab_h;
+  D.eflag = is_e ? eflag : nullptr;
+  D.d = make_dims(*this);
+  D.sc = make_scales<C>(*this);
+  D.nkb = nkb;
+  D.njb = njb;
+  D.chunk = chunk;
+  D.nch = nch_eff;
+  fill_pml_pass<C>(*this, D.pml, is_e);
+  return D;
+}
+
+template <typename S>
 int Engine::launch_h(int t) {
   (void)t;
   using C = typename Store<S>::C;
@@ -1148,20 +1197,15 @@ int Engine::launch_h(int t) {
   const bool has_pml = P.n > 0;
   const C* tab = static_cast<const C*>(htab);
   if (dim == 3) {
-    const Tiling T = tiling();
+    const YeePass<S> D = yee_pass<S>(false);
     dim3 block(64, tile_y);
-    dim3 grid((unsigned)(T.nkb * T.njb * T.nch));
-#define XF_LH3(I, L, PM)                                                                               \
-  do {                                                                                                 \
-    if (pipeline && nt_streams)                                                                        \
-      hipLaunchKernelGGL((k_yee_h3p<S, I, L, PM, true>), grid, block, shm, stream, f,                   \
-                         static_cast<const I*>(hid), tab, ntab_h, P, d, T, scl);                        \
-    else if (pipeline)                                                                                 \
-      hipLaunchKernelGGL((k_yee_h3p<S, I, L, PM, false>), grid, block, shm, stream, f,                  \
-                         static_cast<const I*>(hid), tab, ntab_h, P, d, T, scl);                        \
-    else                                                                                               \
-      hipLaunchKernelGGL((k_yee_h3<S, I, L, PM>), grid, block, shm, stream, f,                          \
-                         static_cast<const I*>(hid), tab, ntab_h, P, d, T, scl);                        \
+    dim3 grid((unsigned)(D.nkb * D.njb * D.nch));
+#define XF_LH3(I, L, PM)                                                                     \
+  do {                                                                                       \
+    if (nt_streams)                                                                          \
+      hipLaunchKernelGGL((k_yee_h3<S, I, L, PM, true>), grid, block, shm, stream, D, xcd);   \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_yee_h3<S, I, L, PM, false>), grid, block, shm, stream, D, xcd);  \
   } while (0)
     if (id_bytes == 1) {
       if (lds) { if (has_pml) XF_LH3(uint8_t, true, true); else XF_LH3(uint8_t, true, false); }
@@ -1205,20 +1249,15 @@ int Engine::launch_e(int t) {
   const bool has_pml = P.n > 0;
   const C* tab = static_cast<const C*>(etab);
   if (dim == 3) {
+    const YeePass<S> D = yee_pass<S>(true);
     dim3 block(64, tile_y);
-    const Tiling T = tiling();
-    dim3 grid((unsigned)(T.nkb * T.njb * T.nch));
-#define XF_LE3(I, L, PM, AD)                                                                           \
-  do {                                                                                                 \
-    if (pipeline && nt_streams)                                                                        \
-      hipLaunchKernelGGL((k_yee_e3p<S, I, L, PM, AD, true>), grid, block, shm, stream, f,               \
-                         static_cast<const I*>(eid), tab, ntab_e, eflag, P, d, T, scl);                 \
-    else if (pipeline)                                                                                 \
-      hipLaunchKernelGGL((k_yee_e3p<S, I, L, PM, AD, false>), grid, block, shm, stream, f,              \
-                         static_cast<const I*>(eid), tab, ntab_e, eflag, P, d, T, scl);                 \
-    else                                                                                               \
-      hipLaunchKernelGGL((k_yee_e3<S, I, L, PM, AD>), grid, block, shm, stream, f,                      \
-                         static_cast<const I*>(eid), tab, ntab_e, eflag, P, d, T, scl);                 \
+    dim3 grid((unsigned)(D.nkb * D.njb * D.nch));
+#define XF_LE3(I, L, PM, AD)                                                                     \
+  do {                                                                                           \
+    if (nt_streams)                                                                              \
+      hipLaunchKernelGGL((k_yee_e3<S, I, L, PM, AD, true>), grid, block, shm, stream, D, xcd);   \
+    else                                                                                         \
+      hipLaunchKernelGGL((k_yee_e3<S, I, L, PM, AD, false>), grid, block, shm, stream, D, xcd);  \
   } while (0)
 #define XF_LE3A(I, L, PM) \
   do { if (has_ade) XF_LE3(I, L, PM, true); else XF_LE3(I, L, PM, false); } while (0)
@@ -1593,3 +1632,5 @@ int xf_describe(const xf_handle* h, char* buf, size_t len) {
 }
 
 }  // extern "C"
+
+#include "xf_batch.inl"

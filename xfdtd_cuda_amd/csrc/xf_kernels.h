@@ -135,6 +135,23 @@ struct PmlPass {
   PmlFace<C> f[kMaxPml];
 };
 
+// Everything one 3-D Yee pass (H or E) of one grid needs: the kernels take
+// it as a kernel argument (one grid) or from an array in device memory
+// (a batch of grids, one launch).
+template <typename S>
+struct YeePass {
+  using C = typename Store<S>::C;
+  Fields<S> f;
+  const void* ids;              // uint8_t or uint16_t per cell (padded layout)
+  const C* tab;                 // 9 coefficients per id
+  int ntab;
+  const unsigned char* eflag;   // E pass: table rows left to the ADE kernel
+  Dims d;
+  Scales<C> sc;
+  int nkb, njb, chunk, nch;     // tiling of this grid
+  PmlPass<C> pml;
+};
+
 // A TFSF face correction of one component (one reference launch of
 // __tFSFcorrect, tfsf_corrector.cu:148-153) as a written-cell box.
 template <typename C>
@@ -174,6 +191,26 @@ struct NfPass {
   double2* cur;      // [freq][face cells][4] -> laid out [freq][face][comp][cell]
   const double2* te; // [freq][nt]
   const double2* th;
+};
+
+// One grid's TFSF pass or NF2FF box inside a batch of grids: the batched
+// kernels read these from device memory, one per grid (or box).
+template <typename S>
+struct TfsfItem {
+  using C = typename Store<S>::C;
+  Fields<S> f;
+  Scales<C> sc;
+  Dims d;
+  TfsfPass<C> P;
+};
+
+template <typename S>
+struct NfItem {
+  using C = typename Store<S>::C;
+  Fields<S> f;
+  Scales<C> sc;
+  Dims d;
+  NfPass P;
 };
 
 }  // namespace xf

@@ -101,308 +101,7 @@ __device__ __forceinline__ const C* load_table(const C* __restrict__ g, int nval
   return smem;
 }
 
-// ---------------------------------------------------------- 3-D Yee, H pass
-// BasicUpdator::updateH (basic_updator.cuh:16-32) with hNext
-// (update_scheme.cuh:58-63) for Hx, Hy, Hz fused, plus the H corrections of
-// every PML face.  A thread owns one (j, k) column and marches i through a
-// chunk, carrying Ey(i+1), Ez(i+1) as the next cell's Ey(i), Ez(i).
-// Block: 64 lanes along k x blockDim.y rows along j.
-template <typename S, typename I, bool LDS, bool PML>
-__global__ void __launch_bounds__(256)
-    k_yee_h3(Fields<S> f, const I* __restrict__ hid,
-             const typename Store<S>::C* __restrict__ htab, int ntab,
-             PmlPass<typename Store<S>::C> pml, Dims d, Tiling T,
-             Scales<typename Store<S>::C> sc) {
-  using C = typename Store<S>::C;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const C* tab = load_table<C, LDS>(htab, ntab * 9, reinterpret_cast<C*>(smem_raw));
-  int kb, jb, ch;
-  tile_coords(T, kb, jb, ch);
-  const int k = kb * 64 + threadIdx.x;
-  const int j = jb * blockDim.y + threadIdx.y;
-  if (k > d.nz || j > d.ny) return;
-  const int i0 = ch * T.chunk;
-  const int i1 = min(i0 + T.chunk, d.nx + 1);
-  const bool jk_main = j < d.ny && k < d.nz;
-  int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
-  const C ie = sc.inv_e, ih = sc.inv_h;
-  C ey_i = 0, ez_i = 0;
-  if (jk_main && i0 < d.nx) {
-    ey_i = Store<S>::ld(f.e[1], idx, ie);
-    ez_i = Store<S>::ld(f.e[2], idx, ie);
-  }
-  for (int i = i0; i < i1; ++i, idx += d.sx) {
-    C v[3] = {0, 0, 0};
-    bool have[3] = {false, false, false};
-    bool dirty[3] = {false, false, false};
-    if (jk_main && i < d.nx) {
-      const C* t = tab + 9 * (int)hid[idx];
-      const C ex_i = Store<S>::ld(f.e[0], idx, ie);
-      const C ex_k = Store<S>::ld(f.e[0], idx + 1, ie);
-      const C ex_j = Store<S>::ld(f.e[0], idx + d.sy, ie);
-      const C ey_k = Store<S>::ld(f.e[1], idx + 1, ie);
-      const C ey_n = Store<S>::ld(f.e[1], idx + d.sx, ie);
-      const C ez_j = Store<S>::ld(f.e[2], idx + d.sy, ie);
-      const C ez_n = Store<S>::ld(f.e[2], idx + d.sx, ie);
-      const C hx = Store<S>::ld(f.h[0], idx, ih);
-      const C hy = Store<S>::ld(f.h[1], idx, ih);
-      const C hz = Store<S>::ld(f.h[2], idx, ih);
-      v[0] = t[0] * hx + t[1] * (ey_k - ey_i) + t[2] * (ez_j - ez_i);
-      v[1] = t[3] * hy + t[4] * (ez_n - ez_i) + t[5] * (ex_k - ex_i);
-      v[2] = t[6] * hz + t[7] * (ex_j - ex_i) + t[8] * (ey_n - ey_i);
-      have[0] = have[1] = have[2] = true;
-      dirty[0] = dirty[1] = dirty[2] = true;
-      ey_i = ey_n;
-      ez_i = ez_n;
-    }
-    if (PML) pml_all<false, false>(pml, i, j, k, idx, f, d, sc, v, have, dirty);
-    if (dirty[0]) Store<S>::st(f.h[0], idx, v[0], sc.s_h);
-    if (dirty[1]) Store<S>::st(f.h[1], idx, v[1], sc.s_h);
-    if (dirty[2]) Store<S>::st(f.h[2], idx, v[2], sc.s_h);
-  }
-}
-
-// ---------------------------------------------------------- 3-D Yee, E pass
-// BasicUpdator3D::updateE (basic_updator_3d.cu:10-34) with eNext
-// (update_scheme.cuh:27-32, skipping a == 0 || b == 0) fused over Ex, Ey,
-// Ez, plus the E corrections of every PML face.  Carries Hy(i), Hz(i) as the
-// next cell's Hy(i-1), Hz(i-1).  Cells whose table row is flagged dispersive
-// are left to the ADE kernel.
-template <typename S, typename I, bool LDS, bool PML, bool ADE>
-__global__ void __launch_bounds__(256)
-    k_yee_e3(Fields<S> f, const I* __restrict__ eid,
-             const typename Store<S>::C* __restrict__ etab, int ntab,
-             const unsigned char* __restrict__ eflag, PmlPass<typename Store<S>::C> pml,
-             Dims d, Tiling T, Scales<typename Store<S>::C> sc) {
-  using C = typename Store<S>::C;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const C* tab = load_table<C, LDS>(etab, ntab * 9, reinterpret_cast<C*>(smem_raw));
-  int kb, jb, ch;
-  tile_coords(T, kb, jb, ch);
-  const int k = kb * 64 + threadIdx.x;
-  const int j = jb * blockDim.y + threadIdx.y;
-  if (k > d.nz || j > d.ny) return;
-  const int i0 = ch * T.chunk;
-  const int i1 = min(i0 + T.chunk, d.nx + 1);
-  const bool jk_main = j < d.ny && k < d.nz;
-  int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
-  const C ie = sc.inv_e, ih = sc.inv_h;
-  C hy_m = 0, hz_m = 0;  // H at i-1
-  if (jk_main && i0 > 0 && i0 < d.nx) {
-    hy_m = Store<S>::ld(f.h[1], idx - d.sx, ih);
-    hz_m = Store<S>::ld(f.h[2], idx - d.sx, ih);
-  }
-  for (int i = i0; i < i1; ++i, idx += d.sx) {
-    C v[3] = {0, 0, 0};
-    bool have[3] = {false, false, false};
-    bool dirty[3] = {false, false, false};
-    if (jk_main && i < d.nx) {
-      const int id = (int)eid[idx];
-      const C hy = Store<S>::ld(f.h[1], idx, ih);
-      const C hz = Store<S>::ld(f.h[2], idx, ih);
-      if (!(ADE && eflag[id])) {
-        const C* t = tab + 9 * id;
-        const C hx = Store<S>::ld(f.h[0], idx, ih);
-        if (j > 0 && k > 0) {
-          const C hy_k = Store<S>::ld(f.h[1], idx - 1, ih);
-          const C hz_j = Store<S>::ld(f.h[2], idx - d.sy, ih);
-          v[0] = t[0] * Store<S>::ld(f.e[0], idx, ie) + t[1] * (hy - hy_k) + t[2] * (hz - hz_j);
-          have[0] = dirty[0] = true;
-        }
-        if (k > 0 && i > 0) {
-          const C hx_k = Store<S>::ld(f.h[0], idx - 1, ih);
-          v[1] = t[3] * Store<S>::ld(f.e[1], idx, ie) + t[4] * (hz - hz_m) + t[5] * (hx - hx_k);
-          have[1] = dirty[1] = true;
-        }
-        if (i > 0 && j > 0) {
-          const C hx_j = Store<S>::ld(f.h[0], idx - d.sy, ih);
-          v[2] = t[6] * Store<S>::ld(f.e[2], idx, ie) + t[7] * (hx - hx_j) + t[8] * (hy - hy_m);
-          have[2] = dirty[2] = true;
-        }
-      }
-      hy_m = hy;
-      hz_m = hz;
-    }
-    if (PML) pml_all<true, false>(pml, i, j, k, idx, f, d, sc, v, have, dirty);
-    if (dirty[0]) Store<S>::st(f.e[0], idx, v[0], sc.s_e);
-    if (dirty[1]) Store<S>::st(f.e[1], idx, v[1], sc.s_e);
-    if (dirty[2]) Store<S>::st(f.e[2], idx, v[2], sc.s_e);
-  }
-}
-
-// -------------------------------------------- 3-D Yee, software-pipelined
-// The same H and E passes with the loads of plane i+1 issued before plane i
-// is computed and stored, so each wave keeps two planes of loads in
-// flight.  Arithmetic and its order are those of k_yee_h3 / k_yee_e3.
-template <typename S>
-struct HLoads {
-  typename Store<S>::C ex_i, ex_k, ex_j, ey_k, ey_n, ez_j, ez_n, hx, hy, hz;
-  int id;
-};
-
-template <typename S, typename I, bool NT>
-__device__ __forceinline__ void h_load(HLoads<S>& L, const Fields<S>& f, const I* __restrict__ hid,
-                                       int64_t idx, const Dims& d, typename Store<S>::C ie,
-                                       typename Store<S>::C ih) {
-  L.id = (int)hid[idx];
-  L.ex_i = Store<S>::ld(f.e[0], idx, ie);
-  L.ex_k = Store<S>::ld(f.e[0], idx + 1, ie);
-  L.ex_j = Store<S>::ld(f.e[0], idx + d.sy, ie);
-  L.ey_k = Store<S>::ld(f.e[1], idx + 1, ie);
-  L.ey_n = Store<S>::ld(f.e[1], idx + d.sx, ie);
-  L.ez_j = Store<S>::ld(f.e[2], idx + d.sy, ie);
-  L.ez_n = Store<S>::ld(f.e[2], idx + d.sx, ie);
-  L.hx = NT ? Store<S>::ld_nt(f.h[0], idx, ih) : Store<S>::ld(f.h[0], idx, ih);
-  L.hy = NT ? Store<S>::ld_nt(f.h[1], idx, ih) : Store<S>::ld(f.h[1], idx, ih);
-  L.hz = NT ? Store<S>::ld_nt(f.h[2], idx, ih) : Store<S>::ld(f.h[2], idx, ih);
-}
-
-template <typename S, typename I, bool LDS, bool PML, bool NT>
-__global__ void __launch_bounds__(256)
-    k_yee_h3p(Fields<S> f, const I* __restrict__ hid,
-              const typename Store<S>::C* __restrict__ htab, int ntab,
-              PmlPass<typename Store<S>::C> pml, Dims d, Tiling T,
-              Scales<typename Store<S>::C> sc) {
-  using C = typename Store<S>::C;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const C* tab = load_table<C, LDS>(htab, ntab * 9, reinterpret_cast<C*>(smem_raw));
-  int kb, jb, ch;
-  tile_coords(T, kb, jb, ch);
-  const int k = kb * 64 + threadIdx.x;
-  const int j = jb * blockDim.y + threadIdx.y;
-  if (k > d.nz || j > d.ny) return;
-  const int i0 = ch * T.chunk;
-  const int i1 = min(i0 + T.chunk, d.nx + 1);
-  const bool jk_main = j < d.ny && k < d.nz;
-  int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
-  const C ie = sc.inv_e, ih = sc.inv_h;
-  C ey_i = 0, ez_i = 0;
-  HLoads<S> cur{}, nxt{};
-  if (jk_main && i0 < d.nx) {
-    ey_i = Store<S>::ld(f.e[1], idx, ie);
-    ez_i = Store<S>::ld(f.e[2], idx, ie);
-    h_load<S, I, NT>(cur, f, hid, idx, d, ie, ih);
-  }
-  for (int i = i0; i < i1; ++i, idx += d.sx) {
-    const bool main_i = jk_main && i < d.nx;
-    if (jk_main && i + 1 < d.nx && i + 1 < i1) h_load<S, I, NT>(nxt, f, hid, idx + d.sx, d, ie, ih);
-    C v[3] = {0, 0, 0};
-    bool have[3] = {false, false, false};
-    bool dirty[3] = {false, false, false};
-    if (main_i) {
-      const C* t = tab + 9 * cur.id;
-      v[0] = t[0] * cur.hx + t[1] * (cur.ey_k - ey_i) + t[2] * (cur.ez_j - ez_i);
-      v[1] = t[3] * cur.hy + t[4] * (cur.ez_n - ez_i) + t[5] * (cur.ex_k - cur.ex_i);
-      v[2] = t[6] * cur.hz + t[7] * (cur.ex_j - cur.ex_i) + t[8] * (cur.ey_n - ey_i);
-      have[0] = have[1] = have[2] = true;
-      dirty[0] = dirty[1] = dirty[2] = true;
-      ey_i = cur.ey_n;
-      ez_i = cur.ez_n;
-    }
-    if (PML) pml_all<false, false>(pml, i, j, k, idx, f, d, sc, v, have, dirty);
-    if (NT) {
-      if (dirty[0]) Store<S>::st_nt(f.h[0], idx, v[0], sc.s_h);
-      if (dirty[1]) Store<S>::st_nt(f.h[1], idx, v[1], sc.s_h);
-      if (dirty[2]) Store<S>::st_nt(f.h[2], idx, v[2], sc.s_h);
-    } else {
-      if (dirty[0]) Store<S>::st(f.h[0], idx, v[0], sc.s_h);
-      if (dirty[1]) Store<S>::st(f.h[1], idx, v[1], sc.s_h);
-      if (dirty[2]) Store<S>::st(f.h[2], idx, v[2], sc.s_h);
-    }
-    cur = nxt;
-  }
-}
-
-template <typename S>
-struct ELoads {
-  typename Store<S>::C ex, ey, ez, hx, hx_k, hx_j, hy, hy_k, hz, hz_j;
-  int id;
-};
-
-template <typename S, typename I, bool NT>
-__device__ __forceinline__ void e_load(ELoads<S>& L, const Fields<S>& f, const I* __restrict__ eid,
-                                       int64_t idx, const Dims& d, bool j_in, bool k_in,
-                                       typename Store<S>::C ie, typename Store<S>::C ih) {
-  L.id = (int)eid[idx];
-  L.ex = NT ? Store<S>::ld_nt(f.e[0], idx, ie) : Store<S>::ld(f.e[0], idx, ie);
-  L.ey = NT ? Store<S>::ld_nt(f.e[1], idx, ie) : Store<S>::ld(f.e[1], idx, ie);
-  L.ez = NT ? Store<S>::ld_nt(f.e[2], idx, ie) : Store<S>::ld(f.e[2], idx, ie);
-  L.hx = Store<S>::ld(f.h[0], idx, ih);
-  L.hy = Store<S>::ld(f.h[1], idx, ih);
-  L.hz = Store<S>::ld(f.h[2], idx, ih);
-  L.hx_k = k_in ? Store<S>::ld(f.h[0], idx - 1, ih) : (typename Store<S>::C)0;
-  L.hy_k = k_in ? Store<S>::ld(f.h[1], idx - 1, ih) : (typename Store<S>::C)0;
-  L.hx_j = j_in ? Store<S>::ld(f.h[0], idx - d.sy, ih) : (typename Store<S>::C)0;
-  L.hz_j = j_in ? Store<S>::ld(f.h[2], idx - d.sy, ih) : (typename Store<S>::C)0;
-}
-
-template <typename S, typename I, bool LDS, bool PML, bool ADE, bool NT>
-__global__ void __launch_bounds__(256)
-    k_yee_e3p(Fields<S> f, const I* __restrict__ eid,
-              const typename Store<S>::C* __restrict__ etab, int ntab,
-              const unsigned char* __restrict__ eflag, PmlPass<typename Store<S>::C> pml,
-              Dims d, Tiling T, Scales<typename Store<S>::C> sc) {
-  using C = typename Store<S>::C;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const C* tab = load_table<C, LDS>(etab, ntab * 9, reinterpret_cast<C*>(smem_raw));
-  int kb, jb, ch;
-  tile_coords(T, kb, jb, ch);
-  const int k = kb * 64 + threadIdx.x;
-  const int j = jb * blockDim.y + threadIdx.y;
-  if (k > d.nz || j > d.ny) return;
-  const int i0 = ch * T.chunk;
-  const int i1 = min(i0 + T.chunk, d.nx + 1);
-  const bool jk_main = j < d.ny && k < d.nz;
-  const bool j_in = j > 0, k_in = k > 0;
-  int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
-  const C ie = sc.inv_e, ih = sc.inv_h;
-  C hy_m = 0, hz_m = 0;
-  ELoads<S> cur{}, nxt{};
-  if (jk_main && i0 < d.nx) {
-    if (i0 > 0) {
-      hy_m = Store<S>::ld(f.h[1], idx - d.sx, ih);
-      hz_m = Store<S>::ld(f.h[2], idx - d.sx, ih);
-    }
-    e_load<S, I, NT>(cur, f, eid, idx, d, j_in, k_in, ie, ih);
-  }
-  for (int i = i0; i < i1; ++i, idx += d.sx) {
-    if (jk_main && i + 1 < d.nx && i + 1 < i1)
-      e_load<S, I, NT>(nxt, f, eid, idx + d.sx, d, j_in, k_in, ie, ih);
-    C v[3] = {0, 0, 0};
-    bool have[3] = {false, false, false};
-    bool dirty[3] = {false, false, false};
-    if (jk_main && i < d.nx) {
-      if (!(ADE && eflag[cur.id])) {
-        const C* t = tab + 9 * cur.id;
-        if (j_in && k_in) {
-          v[0] = t[0] * cur.ex + t[1] * (cur.hy - cur.hy_k) + t[2] * (cur.hz - cur.hz_j);
-          have[0] = dirty[0] = true;
-        }
-        if (k_in && i > 0) {
-          v[1] = t[3] * cur.ey + t[4] * (cur.hz - hz_m) + t[5] * (cur.hx - cur.hx_k);
-          have[1] = dirty[1] = true;
-        }
-        if (i > 0 && j_in) {
-          v[2] = t[6] * cur.ez + t[7] * (cur.hx - cur.hx_j) + t[8] * (cur.hy - hy_m);
-          have[2] = dirty[2] = true;
-        }
-      }
-      hy_m = cur.hy;
-      hz_m = cur.hz;
-    }
-    if (PML) pml_all<true, false>(pml, i, j, k, idx, f, d, sc, v, have, dirty);
-    if (NT) {
-      if (dirty[0]) Store<S>::st_nt(f.e[0], idx, v[0], sc.s_e);
-      if (dirty[1]) Store<S>::st_nt(f.e[1], idx, v[1], sc.s_e);
-      if (dirty[2]) Store<S>::st_nt(f.e[2], idx, v[2], sc.s_e);
-    } else {
-      if (dirty[0]) Store<S>::st(f.e[0], idx, v[0], sc.s_e);
-      if (dirty[1]) Store<S>::st(f.e[1], idx, v[1], sc.s_e);
-      if (dirty[2]) Store<S>::st(f.e[2], idx, v[2], sc.s_e);
-    }
-    cur = nxt;
-  }
-}
+#include "xf_yee3d.inl"
 
 // ------------------------------------------------------- 2-D TMz, H and E
 // The TE updator (basic_updator_te.cu:9-20, basic_updator.cuh:16-32) on a
@@ -502,13 +201,10 @@ __device__ __forceinline__ C tfsf_inc(const TfsfPass<C>& P, bool inc_is_e, int c
 // component are applied by the first of them, in the reference's launch
 // order (TFSFCorrector3DAgency/2DAgency, tfsf_corrector.cu:155-271).
 template <typename S>
-__global__ void __launch_bounds__(256)
-    k_tfsf(Fields<S> f, TfsfPass<typename Store<S>::C> P, int t_arg, const int* t_dev,
-           Scales<typename Store<S>::C> sc, Dims d, int two_d) {
+__device__ __forceinline__ void tfsf_cell(const Fields<S>& f, const TfsfPass<typename Store<S>::C>& P,
+                                          int t, const Scales<typename Store<S>::C>& sc, const Dims& d,
+                                          int two_d, int64_t tid) {
   using C = typename Store<S>::C;
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= P.total) return;
-  const int t = t_dev ? *t_dev : t_arg;
   int q = 0;
   while (q + 1 < P.njob && tid >= P.job[q + 1].start) ++q;
   const TfsfJob<C>& J = P.job[q];
@@ -545,6 +241,15 @@ __global__ void __launch_bounds__(256)
     v += inc * X.coef;
   }
   Store<S>::st(field, idx, v, scl);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_tfsf(Fields<S> f, TfsfPass<typename Store<S>::C> P, int t_arg, const int* t_dev,
+           Scales<typename Store<S>::C> sc, Dims d, int two_d) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= P.total) return;
+  tfsf_cell<S>(f, P, t_dev ? *t_dev : t_arg, sc, d, two_d, tid);
 }
 
 // ----------------------------------------------------------------- NF2FF
@@ -589,12 +294,9 @@ __device__ __forceinline__ typename Store<S>::C nf_interp(const Fields<S>& f, co
 // fields are formed once per surface cell and accumulated into every
 // frequency's complex FP64 sums.
 template <typename S>
-__global__ void __launch_bounds__(256)
-    k_nf2ff(Fields<S> f, NfPass P, int t_arg, const int* t_dev, Dims d, int two_d,
-            Scales<typename Store<S>::C> sc) {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= P.total) return;
-  const int t = t_dev ? *t_dev : t_arg;
+__device__ __forceinline__ void nf2ff_cell(const Fields<S>& f, const NfPass& P, int t, const Dims& d,
+                                           int two_d, const Scales<typename Store<S>::C>& sc,
+                                           int64_t tid) {
   int face = 0;
   while (face < 5 && tid >= P.face[face + 1].start) ++face;
   const NfFace& F = P.face[face];
@@ -639,6 +341,53 @@ __global__ void __launch_bounds__(256)
     v.y += r_mb * te.y;
     base[3 * nface] = v;
   }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_nf2ff(Fields<S> f, NfPass P, int t_arg, const int* t_dev, Dims d, int two_d,
+            Scales<typename Store<S>::C> sc) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= P.total) return;
+  nf2ff_cell<S>(f, P, t_dev ? *t_dev : t_arg, d, two_d, sc, tid);
+}
+
+// ----------------------------------------------------------- batched forms
+// Item g owns flat indices [start[g], start[g + 1]); start[n] is the total.
+__device__ __forceinline__ int owner_of(const int64_t* __restrict__ start, int n, int64_t x) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// The TFSF pass of every grid of a batch in one launch.
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_tfsf_batch(const TfsfItem<S>* __restrict__ items, const int64_t* __restrict__ start, int n,
+                 int t_arg, const int* t_dev) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= start[n]) return;
+  const int g = owner_of(start, n, tid);
+  const TfsfItem<S>& I = items[g];
+  tfsf_cell<S>(I.f, I.P, t_dev ? *t_dev : t_arg, I.sc, I.d, 0, tid - start[g]);
+}
+
+// The NF2FF accumulation of every box of a batch in one launch.
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_nf2ff_batch(const NfItem<S>* __restrict__ items, const int64_t* __restrict__ start, int n,
+                  int t_arg, const int* t_dev) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= start[n]) return;
+  const int g = owner_of(start, n, tid);
+  const NfItem<S>& I = items[g];
+  nf2ff_cell<S>(I.f, I.P, t_dev ? *t_dev : t_arg, I.d, 0, I.sc, tid - start[g]);
 }
 
 // ------------------------------------------------------------------ ADE

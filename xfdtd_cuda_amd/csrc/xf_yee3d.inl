@@ -1,0 +1,251 @@
+// xf_yee3d.inl -- the 3-D Yee passes (included by xf_kernels.inl).
+//
+// H pass: BasicUpdator::updateH (basic_updator.cuh:16-32) with hNext
+// (update_scheme.cuh:58-63) for Hx, Hy, Hz fused, plus the H corrections of
+// every PML face (pml_corrector.cu:180-270).
+// E pass: BasicUpdator3D::updateE (basic_updator_3d.cu:10-34) with eNext
+// (update_scheme.cuh:27-32, skipping a == 0 || b == 0) for Ex, Ey, Ez
+// fused, plus the E corrections of every PML face (pml_corrector.cu:50-178).
+//
+// Work decomposition: a tile is 64 lanes along k x blockDim.y rows along j;
+// each thread marches i through the tile's x-chunk, carrying the plane it
+// has loaded as the next cell's i-1 (E pass: Hy, Hz) or i (H pass: Ey, Ez)
+// neighbour, and issuing the loads of plane i+1 before computing plane i
+// (two planes of loads in flight per wave).  Arithmetic and its order are
+// the reference's; the file is built without FMA contraction.
+
+template <typename S>
+struct HLoads {
+  typename Store<S>::C ex_i, ex_k, ex_j, ey_k, ey_n, ez_j, ez_n, hx, hy, hz;
+  int id;
+};
+
+template <typename S, typename I, bool NT>
+__device__ __forceinline__ void h_load(HLoads<S>& L, const Fields<S>& f, const I* __restrict__ hid,
+                                       int64_t idx, const Dims& d, typename Store<S>::C ie,
+                                       typename Store<S>::C ih) {
+  L.id = (int)hid[idx];
+  L.ex_i = Store<S>::ld(f.e[0], idx, ie);
+  L.ex_k = Store<S>::ld(f.e[0], idx + 1, ie);
+  L.ex_j = Store<S>::ld(f.e[0], idx + d.sy, ie);
+  L.ey_k = Store<S>::ld(f.e[1], idx + 1, ie);
+  L.ey_n = Store<S>::ld(f.e[1], idx + d.sx, ie);
+  L.ez_j = Store<S>::ld(f.e[2], idx + d.sy, ie);
+  L.ez_n = Store<S>::ld(f.e[2], idx + d.sx, ie);
+  L.hx = NT ? Store<S>::ld_nt(f.h[0], idx, ih) : Store<S>::ld(f.h[0], idx, ih);
+  L.hy = NT ? Store<S>::ld_nt(f.h[1], idx, ih) : Store<S>::ld(f.h[1], idx, ih);
+  L.hz = NT ? Store<S>::ld_nt(f.h[2], idx, ih) : Store<S>::ld(f.h[2], idx, ih);
+}
+
+template <typename S>
+struct ELoads {
+  typename Store<S>::C ex, ey, ez, hx, hx_k, hx_j, hy, hy_k, hz, hz_j;
+  int id;
+};
+
+template <typename S, typename I, bool NT>
+__device__ __forceinline__ void e_load(ELoads<S>& L, const Fields<S>& f, const I* __restrict__ eid,
+                                       int64_t idx, const Dims& d, bool j_in, bool k_in,
+                                       typename Store<S>::C ie, typename Store<S>::C ih) {
+  using C = typename Store<S>::C;
+  L.id = (int)eid[idx];
+  L.ex = NT ? Store<S>::ld_nt(f.e[0], idx, ie) : Store<S>::ld(f.e[0], idx, ie);
+  L.ey = NT ? Store<S>::ld_nt(f.e[1], idx, ie) : Store<S>::ld(f.e[1], idx, ie);
+  L.ez = NT ? Store<S>::ld_nt(f.e[2], idx, ie) : Store<S>::ld(f.e[2], idx, ie);
+  L.hx = Store<S>::ld(f.h[0], idx, ih);
+  L.hy = Store<S>::ld(f.h[1], idx, ih);
+  L.hz = Store<S>::ld(f.h[2], idx, ih);
+  L.hx_k = k_in ? Store<S>::ld(f.h[0], idx - 1, ih) : (C)0;
+  L.hy_k = k_in ? Store<S>::ld(f.h[1], idx - 1, ih) : (C)0;
+  L.hx_j = j_in ? Store<S>::ld(f.h[0], idx - d.sy, ih) : (C)0;
+  L.hz_j = j_in ? Store<S>::ld(f.h[2], idx - d.sy, ih) : (C)0;
+}
+
+template <typename S, bool NT>
+__device__ __forceinline__ void st3(S* const* dst, int64_t idx, const typename Store<S>::C (&v)[3],
+                                    const bool (&dirty)[3], typename Store<S>::C scale) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (!dirty[c]) continue;
+    if (NT)
+      Store<S>::st_nt(dst[c], idx, v[c], scale);
+    else
+      Store<S>::st(dst[c], idx, v[c], scale);
+  }
+}
+
+// One tile of the H pass.  Lanes beyond the grid return at once; the tile
+// bodies hold no barrier, so the batched kernels may loop over tiles with
+// some lanes idle.
+template <typename S, typename I, bool PML, bool NT>
+__device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename Store<S>::C* tab,
+                                            int kb, int jb, int ch) {
+  using C = typename Store<S>::C;
+  const Dims& d = D.d;
+  const int k = kb * 64 + threadIdx.x;
+  const int j = jb * blockDim.y + threadIdx.y;
+  if (k > d.nz || j > d.ny) return;
+  const I* __restrict__ hid = static_cast<const I*>(D.ids);
+  const Fields<S>& f = D.f;
+  const Scales<C>& sc = D.sc;
+  const int i0 = ch * D.chunk;
+  const int i1 = min(i0 + D.chunk, d.nx + 1);
+  const bool jk_main = j < d.ny && k < d.nz;
+  int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
+  const C ie = sc.inv_e, ih = sc.inv_h;
+  C ey_i = 0, ez_i = 0;
+  HLoads<S> cur{}, nxt{};
+  if (jk_main && i0 < d.nx) {
+    ey_i = Store<S>::ld(f.e[1], idx, ie);
+    ez_i = Store<S>::ld(f.e[2], idx, ie);
+    h_load<S, I, NT>(cur, f, hid, idx, d, ie, ih);
+  }
+  for (int i = i0; i < i1; ++i, idx += d.sx) {
+    if (jk_main && i + 1 < d.nx && i + 1 < i1) h_load<S, I, NT>(nxt, f, hid, idx + d.sx, d, ie, ih);
+    C v[3] = {0, 0, 0};
+    bool have[3] = {false, false, false};
+    bool dirty[3] = {false, false, false};
+    if (jk_main && i < d.nx) {
+      const C* t = tab + 9 * cur.id;
+      v[0] = t[0] * cur.hx + t[1] * (cur.ey_k - ey_i) + t[2] * (cur.ez_j - ez_i);
+      v[1] = t[3] * cur.hy + t[4] * (cur.ez_n - ez_i) + t[5] * (cur.ex_k - cur.ex_i);
+      v[2] = t[6] * cur.hz + t[7] * (cur.ex_j - cur.ex_i) + t[8] * (cur.ey_n - ey_i);
+      have[0] = have[1] = have[2] = true;
+      dirty[0] = dirty[1] = dirty[2] = true;
+      ey_i = cur.ey_n;
+      ez_i = cur.ez_n;
+    }
+    if (PML) pml_all<false, false>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty);
+    st3<S, NT>(f.h, idx, v, dirty, sc.s_h);
+    cur = nxt;
+  }
+}
+
+// One tile of the E pass; table rows flagged dispersive are left to the
+// ADE kernel.
+template <typename S, typename I, bool PML, bool ADE, bool NT>
+__device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename Store<S>::C* tab,
+                                            int kb, int jb, int ch) {
+  using C = typename Store<S>::C;
+  const Dims& d = D.d;
+  const int k = kb * 64 + threadIdx.x;
+  const int j = jb * blockDim.y + threadIdx.y;
+  if (k > d.nz || j > d.ny) return;
+  const I* __restrict__ eid = static_cast<const I*>(D.ids);
+  const Fields<S>& f = D.f;
+  const Scales<C>& sc = D.sc;
+  const int i0 = ch * D.chunk;
+  const int i1 = min(i0 + D.chunk, d.nx + 1);
+  const bool jk_main = j < d.ny && k < d.nz;
+  const bool j_in = j > 0, k_in = k > 0;
+  int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
+  const C ie = sc.inv_e, ih = sc.inv_h;
+  C hy_m = 0, hz_m = 0;  // H at i-1
+  ELoads<S> cur{}, nxt{};
+  if (jk_main && i0 < d.nx) {
+    if (i0 > 0) {
+      hy_m = Store<S>::ld(f.h[1], idx - d.sx, ih);
+      hz_m = Store<S>::ld(f.h[2], idx - d.sx, ih);
+    }
+    e_load<S, I, NT>(cur, f, eid, idx, d, j_in, k_in, ie, ih);
+  }
+  for (int i = i0; i < i1; ++i, idx += d.sx) {
+    if (jk_main && i + 1 < d.nx && i + 1 < i1)
+      e_load<S, I, NT>(nxt, f, eid, idx + d.sx, d, j_in, k_in, ie, ih);
+    C v[3] = {0, 0, 0};
+    bool have[3] = {false, false, false};
+    bool dirty[3] = {false, false, false};
+    if (jk_main && i < d.nx) {
+      if (!(ADE && D.eflag[cur.id])) {
+        const C* t = tab + 9 * cur.id;
+        if (j_in && k_in) {
+          v[0] = t[0] * cur.ex + t[1] * (cur.hy - cur.hy_k) + t[2] * (cur.hz - cur.hz_j);
+          have[0] = dirty[0] = true;
+        }
+        if (k_in && i > 0) {
+          v[1] = t[3] * cur.ey + t[4] * (cur.hz - hz_m) + t[5] * (cur.hx - cur.hx_k);
+          have[1] = dirty[1] = true;
+        }
+        if (i > 0 && j_in) {
+          v[2] = t[6] * cur.ez + t[7] * (cur.hx - cur.hx_j) + t[8] * (cur.hy - hy_m);
+          have[2] = dirty[2] = true;
+        }
+      }
+      hy_m = cur.hy;
+      hz_m = cur.hz;
+    }
+    if (PML) pml_all<true, false>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty);
+    st3<S, NT>(f.e, idx, v, dirty, sc.s_e);
+    cur = nxt;
+  }
+}
+
+// One grid: the descriptor is a kernel argument, block -> tile by
+// tile_coords (optionally XCD-aware).
+template <typename S, typename I, bool LDS, bool PML, bool NT>
+__global__ void __launch_bounds__(256) k_yee_h3(YeePass<S> D, int xcd) {
+  using C = typename Store<S>::C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* tab = load_table<C, LDS>(D.tab, D.ntab * 9, reinterpret_cast<C*>(smem_raw));
+  Tiling T{D.nkb, D.njb, D.nch, D.chunk, xcd};
+  int kb, jb, ch;
+  tile_coords(T, kb, jb, ch);
+  yee_h3_tile<S, I, PML, NT>(D, tab, kb, jb, ch);
+}
+
+template <typename S, typename I, bool LDS, bool PML, bool ADE, bool NT>
+__global__ void __launch_bounds__(256) k_yee_e3(YeePass<S> D, int xcd) {
+  using C = typename Store<S>::C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* tab = load_table<C, LDS>(D.tab, D.ntab * 9, reinterpret_cast<C*>(smem_raw));
+  Tiling T{D.nkb, D.njb, D.nch, D.chunk, xcd};
+  int kb, jb, ch;
+  tile_coords(T, kb, jb, ch);
+  yee_e3_tile<S, I, PML, ADE, NT>(D, tab, kb, jb, ch);
+}
+
+// A batch of grids of different sizes in one persistent launch: the grid is
+// sized to the chip's resident capacity and each block walks the
+// concatenated tile list of all grids (tile t, t + gridDim.x, ...).
+// tile_start[g] is the first tile of grid g (tile_start[n] = total).  All
+// blocks run the same number of loop trips up to one, so the launch drains
+// when the list is exhausted.
+__device__ __forceinline__ int batch_instance(const int* __restrict__ tile_start, int n, int t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tile_start[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename S, typename I, bool PML, bool NT>
+__global__ void __launch_bounds__(256)
+    k_yee_h3_batch(const YeePass<S>* __restrict__ passes, const int* __restrict__ tile_start, int n) {
+  const int total = tile_start[n];
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    const int g = batch_instance(tile_start, n, t);
+    const YeePass<S>& D = passes[g];
+    const int local = t - tile_start[g];
+    const int kb = local % D.nkb;
+    const int rest = local / D.nkb;
+    yee_h3_tile<S, I, PML, NT>(D, D.tab, kb, rest % D.njb, rest / D.njb);
+  }
+}
+
+template <typename S, typename I, bool PML, bool NT>
+__global__ void __launch_bounds__(256)
+    k_yee_e3_batch(const YeePass<S>* __restrict__ passes, const int* __restrict__ tile_start, int n) {
+  const int total = tile_start[n];
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    const int g = batch_instance(tile_start, n, t);
+    const YeePass<S>& D = passes[g];
+    const int local = t - tile_start[g];
+    const int kb = local % D.nkb;
+    const int rest = local / D.nkb;
+    yee_e3_tile<S, I, PML, false, NT>(D, D.tab, kb, rest % D.njb, rest / D.njb);
+  }
+}

@@ -10,6 +10,7 @@ cylinder_2d         test/cylinder_scatter_2d.cu:28-183 in the 1000 x 1000,
                     dl = 1e-3 form: PEC cylinder r = 0.1 m, 10-cell CPML.
 large_sphere        the dielectric sphere scaled to n^3 cells (+ pml), sphere
                     radius 0.35 * edge, eps_r ~ U[2, 6] from a seed.
+batch_spheres       64 sphere scatterers of uneven sizes for a batched run.
 dispersive_sphere   Drude core + Lorentz shell, n^3 cells + 10-cell CPML.
 
 Insets the reference does not fix (2-D NF2FF/TFSF, scaled boxes) are chosen
@@ -99,6 +100,32 @@ def large_sphere(n: int = 512, pml: int = 12, steps: int = 4000, seed: int = 202
                              eps_r=eps_r, mu_r=1.0, layout=layout, init=init)
 
 
+def batch_spheres(count: int = 64, seed: int = 7, edge_range=(48, 160), pml: int = 8,
+                  steps: int = 2000, dl: float = 2.5e-3, freqs=(1e9,), layout: str = "compact",
+                  init: bool = True) -> list[Simulation]:
+    """A batch of independent sphere scatterers of uneven sizes: cube edge
+    (interior cells) ~ U{edge_range}, sphere radius ~ U[0.2, 0.4] x edge,
+    eps_r ~ U[1.5, 8], all from numpy's default_rng(seed), each with a
+    pml-cell CPML, a Gaussian TFSF plane wave and an NF2FF box.  The TF box
+    inset is min(7, floor(edge/2 - r) - 1) cells inside the CPML so the box
+    always encloses the sphere (a 0.4 x edge sphere in a 48-cell cube leaves
+    4 cells); the NF2FF surface sits max(1, inset - 4) cells inside the CPML.
+    """
+    rng = np.random.default_rng(seed)
+    sims = []
+    for _ in range(count):
+        n = int(rng.integers(edge_range[0], edge_range[1] + 1))
+        frac = float(rng.uniform(0.2, 0.4))
+        eps_r = float(rng.uniform(1.5, 8.0))
+        r_cells = frac * n
+        t_in = max(2, min(7, int(np.floor(n / 2 - r_cells)) - 1))
+        n_in = max(1, t_in - 4)
+        sims.append(dielectric_sphere(n=n, pml=pml, steps=steps, dl=dl, tfsf_inset=pml + t_in,
+                                      nf2ff_inset=pml + n_in, freqs=freqs, radius=r_cells * dl,
+                                      eps_r=eps_r, mu_r=1.0, layout=layout, init=init))
+    return sims
+
+
 def dispersive_sphere(n: int = 256, pml: int = 10, steps: int = 6000, dl: float = 2.5e-3,
                       freqs=(0.5e9, 1e9, 2e9), init: bool = True) -> Simulation:
     """Drude core (omega_p = 2 pi 2 GHz, gamma = 2 pi 0.2 GHz) inside a
@@ -121,5 +148,5 @@ def dispersive_sphere(n: int = 256, pml: int = 10, steps: int = 6000, dl: float 
     return s
 
 
-__all__ = ["FACES", "cylinder_2d", "dielectric_sphere", "dispersive_sphere", "large_sphere",
-           "vacuum_benchmark"]
+__all__ = ["FACES", "batch_spheres", "cylinder_2d", "dielectric_sphere", "dispersive_sphere",
+           "large_sphere", "vacuum_benchmark"]

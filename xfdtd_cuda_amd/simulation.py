@@ -16,7 +16,7 @@ from typing import Iterable, Sequence
 import numpy as np
 
 from . import _native
-from ._native import XfhMaterial, XfPlan, XfTiming, c_dp, c_i32, c_i64, c_vp
+from ._native import XfhMaterial, XfPlan, XfTiming, XfTuning, c_dp, c_i32, c_i64, c_vp
 
 DIRECTIONS = {"XN": 0, "XP": 1, "YN": 2, "YP": 3, "ZN": 4, "ZP": 5}
 FACES = ("XN", "XP", "YN", "YP", "ZN", "ZP")
@@ -374,24 +374,114 @@ class Engine:
         return buf.value.decode()
 
     def set_tiling(self, tile_y: int = 0, chunks_x: int = 0, xcd_remap: bool | None = None,
-                   pipeline: bool | None = None, nontemporal: bool | None = None):
-        """Tiling of the 3-D Yee kernels; None everywhere leaves the flags to
-        the engine's defaults."""
+                   nontemporal: bool | None = None):
+        """Tiling of the 3-D Yee kernels; None leaves a flag to the engine's
+        default (no XCD remap, non-temporal streams on)."""
         flags = 0
-        if xcd_remap is not None or pipeline is not None or nontemporal is not None:
+        if xcd_remap is not None or nontemporal is not None:
             flags = _native.TILE_SET
             flags |= _native.TILE_XCD_REMAP if xcd_remap else 0
-            flags |= _native.TILE_PIPELINE if (pipeline is None or pipeline) else 0
-            flags |= _native.TILE_NT if nontemporal else 0
+            flags |= _native.TILE_NT if (nontemporal is None or nontemporal) else 0
         _native.check_hip(self._lib.xf_set_tiling(self._h, tile_y, chunks_x, flags),
                           "xf_set_tiling")
 
     def set_timing(self, on: bool):
         _native.check_hip(self._lib.xf_set_timing(self._h, int(on)), "xf_set_timing")
 
+    def autotune(self, trial_steps: int = 3) -> dict:
+        """Times candidate tilings on this scene (state restored after each)
+        and keeps the fastest; returns it."""
+        t = XfTuning()
+        _native.check_hip(self._lib.xf_autotune(self._h, trial_steps, ctypes.byref(t)),
+                          "xf_autotune")
+        return _tuning_dict(t)
+
     def close(self):
         if getattr(self, "_h", None):
             self._lib.xf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _tuning_dict(t: XfTuning) -> dict:
+    return {k: getattr(t, k) for k, _ in XfTuning._fields_}
+
+
+class Batch:
+    """Many independent 3-D scenes stepped together by the engine's batched
+    kernels (xf_batch_* of include/xfdtd_hip.h): one persistent launch per
+    Yee pass over the tiles of every grid, one launch for all TFSF faces and
+    one for all NF2FF boxes per step, optionally replayed as a hipGraph."""
+
+    def __init__(self, sims, precision: str = "fp64", use_graph: bool = False,
+                 timing: bool = False, device: int = 0):
+        self._lib = _native.hip()
+        self._sims = list(sims)
+        plan = XfPlan()
+        plan.precision = PRECISIONS[precision]
+        plan.use_graph = int(use_graph)
+        plan.timing = int(timing)
+        plan.device = device
+        ptrs = (c_vp * len(self._sims))(*[s.scene_ptr() for s in self._sims])
+        h = c_vp()
+        self._h = None
+        _native.check_hip(self._lib.xf_batch_create(ptrs, len(self._sims), ctypes.byref(plan),
+                                                    ctypes.byref(h)), "xf_batch_create")
+        self._h = h
+
+    def upload(self):
+        _native.check_hip(self._lib.xf_batch_upload(self._h), "xf_batch_upload")
+        return self
+
+    def run(self, n: int):
+        _native.check_hip(self._lib.xf_batch_run(self._h, n), "xf_batch_run")
+        return self
+
+    def download(self):
+        _native.check_hip(self._lib.xf_batch_download(self._h), "xf_batch_download")
+        return self
+
+    def set_tuning(self, tile_y: int = 1, planes: int = 4, blocks: int = 0,
+                   nontemporal: bool = True):
+        t = XfTuning(tile_y, planes, blocks, int(nontemporal), 0, 0.0)
+        _native.check_hip(self._lib.xf_batch_set_tuning(self._h, ctypes.byref(t)),
+                          "xf_batch_set_tuning")
+        return self
+
+    def tuning(self) -> dict:
+        t = XfTuning()
+        _native.check_hip(self._lib.xf_batch_get_tuning(self._h, ctypes.byref(t)),
+                          "xf_batch_get_tuning")
+        return _tuning_dict(t)
+
+    def autotune(self, trial_steps: int = 3) -> dict:
+        t = XfTuning()
+        _native.check_hip(self._lib.xf_batch_autotune(self._h, trial_steps, ctypes.byref(t)),
+                          "xf_batch_autotune")
+        return _tuning_dict(t)
+
+    def timing(self) -> dict:
+        t = XfTiming()
+        _native.check_hip(self._lib.xf_batch_get_timing(self._h, ctypes.byref(t)),
+                          "xf_batch_get_timing")
+        return {k: getattr(t, k) for k, _ in XfTiming._fields_}
+
+    def reset_timing(self):
+        _native.check_hip(self._lib.xf_batch_reset_timing(self._h), "xf_batch_reset_timing")
+
+    def set_timing(self, on: bool):
+        _native.check_hip(self._lib.xf_batch_set_timing(self._h, int(on)), "xf_batch_set_timing")
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(512)
+        _native.check_hip(self._lib.xf_batch_describe(self._h, buf, 512), "xf_batch_describe")
+        return buf.value.decode()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.xf_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):
