@@ -19,6 +19,7 @@ steps of the same scene on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
        python bench.py --workload batch   (64 uneven grids, FP32, batched kernels)
+       python bench.py --workload precision --steps 4000   (FP64/FP32/BF16 far-field error)
 """
 from __future__ import annotations
 
@@ -65,8 +66,10 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "batch"],
-                    help="sphere: one 512^3 + CPML grid (default); batch: 64 uneven grids")
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "batch", "precision"],
+                    help="sphere: one 512^3 + CPML grid (default); batch: 64 uneven grids; "
+                         "precision: the sphere run to the end in fp64, fp32, bf16 with "
+                         "far-field errors against fp64")
     ap.add_argument("--count", type=int, default=64, help="grids of the batch workload")
     args = ap.parse_args(argv)
 
@@ -76,6 +79,8 @@ def main(argv=None) -> int:
         return 2
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "precision":
+        return bench_precision(args)
     args.precision = args.precision or "fp64"
 
     import xfdtd_cuda_amd as xf
@@ -276,6 +281,65 @@ def bench_batch(args) -> int:
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_ms, 4),
                      "h_ms": round(h_ms, 4), "e_ms": round(e_ms, 4)},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_precision(args) -> int:
+    """The 512^3 + 12-cell CPML sphere stepped --steps steps (4000 for the
+    full study) in FP64, FP32 and BF16-storage/FP32-compute.  Each mode's
+    bistatic RCS at 1 GHz on the xz and yz cuts (360 angles) is compared with
+    the FP64 run's: relative L2 error over the cut and the largest dB
+    difference.  NF2FF sums are FP64 in every mode."""
+    import xfdtd_cuda_amd as xf
+    from xfdtd_cuda_amd import scenes
+
+    S = args.steps
+    theta = np.linspace(-np.pi, np.pi, 360)
+    res = {}
+    for prec in ("fp64", "fp32", "bf16"):
+        t0 = time.perf_counter()
+        sim = scenes.large_sphere(n=args.n, pml=args.pml, steps=S, layout="compact")
+        (nx, ny, nz), _, _ = sim.info()
+        eng = xf.Engine(sim, precision=prec, use_graph=True).upload()
+        eng.run(S)
+        run_ms = eng.timing()["run_ms"]
+        eng.download()
+        eng.close()
+        inc = sim.incident_spectrum(1e9)
+        rcs = [sim.far_field(0, 0, theta, phi, inc)[2] for phi in (0.0, np.pi / 2)]
+        cells = nx * ny * nz
+        res[prec] = {"gcells": cells * S / (run_ms * 1e-3) / 1e9, "ms_per_step": run_ms / S,
+                     "rcs": rcs}
+        print(f"# {prec}: {run_ms / S:.3f} ms/step over {S} steps "
+              f"({time.perf_counter() - t0:.0f} s with set-up)", file=sys.stderr, flush=True)
+        del sim
+
+    def err(a, b):
+        rel = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        db = float(np.max(np.abs(10 * np.log10(np.maximum(a, 1e-30))
+                                 - 10 * np.log10(np.maximum(b, 1e-30)))))
+        return rel, db
+
+    modes = {}
+    for prec in ("fp64", "fp32", "bf16"):
+        r = res[prec]
+        e = [err(r["rcs"][c], res["fp64"]["rcs"][c]) for c in range(2)]
+        modes[prec] = {"gcells": round(r["gcells"], 4), "ms_per_step": round(r["ms_per_step"], 4),
+                       "speedup_vs_fp64": round(r["gcells"] / res["fp64"]["gcells"], 3),
+                       "rcs_rel_l2_xz": e[0][0], "rcs_rel_l2_yz": e[1][0],
+                       "rcs_max_db_diff": max(e[0][1], e[1][1])}
+    cells = (args.n + 2 * args.pml) ** 3
+    out = {
+        "metric": "Yee cell-updates/s (Gcells/s) & % HBM roofline",
+        "value": modes["fp32"]["gcells"], "unit": "Gcells/s", "n_gpus": 1, "steps": S,
+        "warmup": 0, "ms_per_step": modes["fp32"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"precision study: sphere {args.n}^3 + {args.pml}-cell CPML "
+                               f"({cells} cells), {S} steps per mode, RCS at 1 GHz vs FP64",
+                   "modes": modes},
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -358,6 +358,63 @@ class TFSF2D : public TFSF {
              std::move(waveform), 2) {}
 };
 
+// ---------------------------------------------------------- point source
+// Soft point source (no counterpart in the reference's GPU path): after the
+// E pass of step n, amplitude * g(n dt) is added to E_component at index
+// (i, j, k) of that component's array (grid indices, boundary cells
+// included).
+class PointSource : public WaveformSource {
+ public:
+  PointSource(Axis::XYZ component, Index i, Index j, Index k,
+              std::unique_ptr<Waveform> waveform, Real amplitude = 1.0);
+  Waveform* waveform() { return _waveform.get(); }
+  xf_point_source& desc() { return _desc; }
+
+ private:
+  friend class Simulation;
+  Axis::XYZ _comp;
+  Index _i, _j, _k;
+  std::unique_ptr<Waveform> _waveform;
+  Real _amp;
+  Array _value;
+  xf_point_source _desc{};
+};
+
+// ------------------------------------------------------ learned boundary
+// Learned absorbing layer (no reference counterpart; the column model is
+// documented at xf_learned_boundary in xfdtd_hip.h).
+//  apply:  `thickness` cells of the domain's material are added outside the
+//          domain on every face in `faces` (bit f = face f, XN..ZP); the
+//          outer node of each is driven by the column MLP (fp16 weights).
+//  record: nothing is added; every `every` steps the inputs and the actual
+//          next value of every column of the plane `inset` nodes inside
+//          each face are kept (training data), up to `slots` steps.
+// All three cell sizes must be equal.
+class LearnedBoundary : public Boundary {
+ public:
+  static std::shared_ptr<LearnedBoundary> makeApply(int faces, Index thickness,
+                                                    std::vector<uint16_t> weights);
+  static std::shared_ptr<LearnedBoundary> makeRecord(int faces, Index inset, Index every,
+                                                     Index slots);
+  int mode() const { return _desc.mode; }
+  int faces() const { return _desc.faces; }
+  Index thickness() const { return _thickness; }
+  Index columns() const { return _columns; }  // samples per recorded step
+  Index recordedSteps() const { return static_cast<Index>(_desc.rec_count); }
+  const std::vector<float>& samples() const { return _rec; }
+  std::vector<uint16_t>& weights() { return _weights; }
+  xf_learned_boundary& desc() { return _desc; }
+
+ private:
+  friend class Simulation;
+  Index _thickness{0};
+  Index _columns{0};
+  std::vector<uint16_t> _weights;
+  Array _history;
+  std::vector<float> _rec;
+  xf_learned_boundary _desc{};
+};
+
 // ---------------------------------------------------------------- NF2FF
 class NF2FF {
  public:
@@ -489,7 +546,10 @@ class Simulation {
   void buildTFSF(TFSF& t);
   void buildNF2FF(NFFFTFrequencyDomain& n);
   void buildADE();
+  void buildPointSource(PointSource& p);
+  void buildLearnedBoundary(LearnedBoundary& l);
   void fillScene();
+  std::vector<xf_point_source> _pt_desc;
 };
 
 }  // namespace xfdtd

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define XF_ABI_VERSION 1
+#define XF_ABI_VERSION 2
 
 enum xf_status {
   XF_OK = 0,
@@ -136,6 +136,65 @@ typedef struct xf_ade {
   xf_array jx, jy, jz, jx_prev, jy_prev, jz_prev;   /* state */
 } xf_ade;
 
+/* ---- Point sources (no counterpart in the reference's GPU path).  After
+ * the E pass of step t and its TFSF/CPML corrections, E_comp at index
+ * (i, j, k) of that component's array gains value.data[t]. */
+typedef struct xf_point_source {
+  int32_t comp;   /* 0 Ex, 1 Ey, 2 Ez */
+  int64_t i, j, k;
+  xf_array value; /* (start_step + n_steps,) */
+} xf_point_source;
+
+/* ---- Learned absorbing layer (no reference counterpart).
+ *
+ * On every face f in `faces` the tangential E nodes of the plane at node
+ * `inset` from the face (layer 0) and the three planes inside it (layers
+ * 1..3) form a column per node.  With u the tangential component and t the
+ * step being advanced, the 12 inputs of a column are
+ *   x0..x2  u^{t+1} on layers 1..3 (after the E pass)
+ *   x3..x6  u^t     on layers 0..3
+ *   x7..x10 u^{t-1} on layers 0..3
+ *   x11     the in-plane 5-point Laplacian of u^t on layer 0
+ * and the layer-0 value u^{t+1} is
+ *   mur + s * MLP(x / s),  mur = x4 + mur_k * (x0 - x3),  s = max_i |x_i|
+ * (the first-order Mur update plus a learned, scale-free correction;
+ * mur + 0 when s = 0).  MLP: 12 -> 32 ReLU -> 32 ReLU -> 1, fp16 weights,
+ * fp32 accumulation; weight layout W1[32][12] b1[32] W2[32][32] b2[32]
+ * w3[32] b3.
+ *
+ * XF_LB_APPLY (inset 0): the layer-0 nodes of the outer boundary, which the
+ * Yee pass never updates, take that value -- a 2-cell layer in place of a
+ * CPML.  Nodes on two faces (edges) go to the lower face index and are
+ * updated after the others, from their final neighbours.
+ * XF_LB_RECORD (inset > 0): nothing is written to the fields; every
+ * rec_every steps the 12 inputs and the actual layer-0 u^{t+1} of every
+ * column are stored as 13 floats (training data from a thick-CPML run).
+ * Columns cover in-plane indices [inset, n - inset] (valid per component),
+ * faces in order XN..ZP, components E_a then E_b of the face's cyclic frame
+ * (a, b) = ((c + 1) % 3, (c + 2) % 3), then a-index, then b-index.
+ *
+ * history holds u^t and u^{t-1} on layers 0..3 of every face (fp32 values):
+ * 16 * sum_f (n_a + 1) * (n_b + 1) doubles, [face][parity][comp][layer][a][b],
+ * parity buffer t % 2 holding u^t. */
+#define XF_LB_IN 12
+#define XF_LB_HID 32
+#define XF_LB_NPARAM (XF_LB_HID * XF_LB_IN + XF_LB_HID + XF_LB_HID * XF_LB_HID + XF_LB_HID + XF_LB_HID + 1)
+
+enum xf_lb_mode { XF_LB_OFF = 0, XF_LB_APPLY = 1, XF_LB_RECORD = 2 };
+
+typedef struct xf_learned_boundary {
+  int32_t mode;            /* enum xf_lb_mode */
+  int32_t faces;           /* bit f set: face f (enum xf_face) */
+  int32_t inset;           /* 0 for APPLY */
+  double mur_k;            /* (c dt - dl) / (c dt + dl) */
+  const uint16_t* weights; /* XF_LB_NPARAM IEEE half values (APPLY) */
+  xf_array history;        /* state, see above */
+  int32_t rec_every;       /* RECORD: steps between recorded steps (>= 1) */
+  int64_t rec_slots;       /* RECORD: recorded steps the buffer holds */
+  float* rec;              /* RECORD: rec_slots x columns x 13 floats */
+  int64_t rec_count;       /* RECORD: recorded steps so far (state) */
+} xf_learned_boundary;
+
 /* The whole mirrored simulation (GridSpace + CalculationParam + EMF + the
  * correctors/monitors SimulationHD::run assembles, simulation_hd.cu:106-226). */
 typedef struct xf_scene {
@@ -165,6 +224,9 @@ typedef struct xf_scene {
   int32_t n_nf2ff;
   xf_nf2ff* nf2ff;
   xf_ade* ade; /* NULL: non-dispersive */
+  int32_t n_point;
+  xf_point_source* point;  /* additions: soft point sources ... */
+  xf_learned_boundary* lb; /* ... and the learned layer (NULL: none) */
 } xf_scene;
 
 /* Launch plan.  grid_dim/block_dim are the reference's setGridDim/
@@ -298,6 +360,30 @@ int xf_batch_get_timing(const xf_batch* b, xf_timing* out);
 int xf_batch_reset_timing(xf_batch* b);
 int xf_batch_set_timing(xf_batch* b, int32_t on);
 int xf_batch_describe(const xf_batch* b, char* buf, size_t len);
+
+/* ---- Training the learned layer's network on the GPU (no reference
+ * counterpart).  Samples are the XF_LB_RECORD rows (12 inputs + target,
+ * 13 floats each); the trainer scales them exactly as the predictor does and
+ * fits the residual (y - mur) / s with the mean squared error, by Adam on
+ * fp32 master weights whose fp16 rounding the forward pass uses.  Gradients
+ * are summed in a fixed order (reproducible bit for bit). */
+typedef struct xf_lb_trainer xf_lb_trainer;
+
+int xf_lb_trainer_create(const float* samples, int64_t n, double mur_k, int32_t device,
+                         xf_lb_trainer** out);
+/* fp32 master weights (XF_LB_NPARAM); resets the optimiser state */
+int xf_lb_trainer_set_weights(xf_lb_trainer* t, const float* w);
+/* fp32 master weights and/or their fp16 bits (either may be NULL) */
+int xf_lb_trainer_get_weights(xf_lb_trainer* t, float* w32, uint16_t* w16);
+/* loss and gradient (w.r.t. the fp16-rounded weights) over samples
+ * [first, first + count) */
+int xf_lb_trainer_grad(xf_lb_trainer* t, int64_t first, int64_t count, double* loss, float* grad);
+/* `steps` Adam steps on minibatches of `batch` samples drawn from `seed`;
+ * last_loss = loss of the last minibatch */
+int xf_lb_trainer_train(xf_lb_trainer* t, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                        double* last_loss);
+int xf_lb_trainer_samples(const xf_lb_trainer* t, int64_t* n);
+int xf_lb_trainer_destroy(xf_lb_trainer* t);
 
 #ifdef __cplusplus
 }

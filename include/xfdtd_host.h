@@ -70,6 +70,26 @@ int xfh_sim_add_tfsf2d(xfh_sim* s, const int64_t start[2], double phi,
 int xfh_sim_add_nf2ff_fd(xfh_sim* s, const int64_t start[3],
                          const double* freqs, int32_t n_freq);
 
+/* Additions without a reference counterpart.  A soft point source on
+ * E_comp at index idx of that component's array (grid indices), driving
+ * amplitude * waveform(n dt) after each E pass. */
+int xfh_sim_add_point_source(xfh_sim* s, int32_t comp, const int64_t idx[3],
+                             int32_t wave_kind, double tau, double t0, double freq,
+                             double amplitude);
+
+/* The learned absorbing layer (xf_learned_boundary in xfdtd_hip.h).
+ * mode XF_LB_APPLY: `size` cells outside the domain on every face of the
+ * mask, weights = XF_LB_NPARAM fp16 values.  mode XF_LB_RECORD: samples of
+ * the plane `size` nodes inside each face every `every` steps, `slots`
+ * recorded steps at most (weights unused). */
+int xfh_sim_add_learned_boundary(xfh_sim* s, int32_t mode, int32_t faces, int64_t size,
+                                 const uint16_t* weights, int32_t n_weights, int64_t every,
+                                 int64_t slots);
+
+/* Recorded samples of the learned layer after a run: a pointer to
+ * recorded x columns x 13 floats (12 inputs, then the target). */
+int xfh_sim_lb_samples(xfh_sim* s, const float** data, int64_t* columns, int64_t* recorded);
+
 /* 0 dense (the reference's 18 coefficient arrays), 1 compact. */
 int xfh_sim_set_coefficient_layout(xfh_sim* s, int32_t layout);
 

@@ -11,6 +11,8 @@ import ctypes
 import os
 import subprocess
 
+import numpy as np
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(_HERE, "libxf_oracle.so")
 _lib = None
@@ -35,6 +37,9 @@ def lib() -> ctypes.CDLL:
         l.xfo_yee_component_triple_loop.restype = ctypes.c_int
         l.xfo_yee_component.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         l.xfo_yee_component.restype = ctypes.c_int
+        l.xfo_lb_predict.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                                     ctypes.c_int64, ctypes.c_void_p]
+        l.xfo_lb_predict.restype = ctypes.c_int
         _lib = l
     return _lib
 
@@ -60,3 +65,12 @@ def yee_component_triple_loop(scene_ptr: int, is_e: bool, axis: int) -> None:
 
 def yee_component(scene_ptr: int, is_e: bool, axis: int) -> None:
     lib().xfo_yee_component(ctypes.c_void_p(scene_ptr), int(is_e), axis)
+
+
+def lb_predict(weights_u16, mur_k: float, x) -> np.ndarray:
+    """The learned layer's column predictor (C restatement) on n x 12 inputs."""
+    w = np.ascontiguousarray(weights_u16, dtype=np.uint16)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.shape[0], np.float32)
+    lib().xfo_lb_predict(w.ctypes.data, mur_k, x.ctypes.data, x.shape[0], out.ctypes.data)
+    return out

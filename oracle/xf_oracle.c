@@ -522,6 +522,212 @@ static void nf2ff_face(const xf_scene* s, const xf_nf2ff* nf,
   }
 }
 
+/* ------------------------------------------------------- point sources */
+
+/* Soft point sources: E_comp(i, j, k) += value[t], in registration order. */
+static void point_sources(xf_scene* s, int64_t t) {
+  for (int32_t q = 0; q < s->n_point; ++q) {
+    const xf_point_source* p = &s->point[q];
+    *at3(efield(s, p->comp), p->i, p->j, p->k) += p->value.data[t];
+  }
+}
+
+/* ------------------------------------------------------- learned layer */
+
+/* IEEE binary16 bits to float (exact). */
+static float half_to_float(uint16_t h) {
+  const int sign = h >> 15;
+  const int ex = (h >> 10) & 0x1f;
+  const int man = h & 0x3ff;
+  float v;
+  if (ex == 0)
+    v = ldexpf((float)man, -24);
+  else if (ex == 31)
+    v = man ? NAN : INFINITY;
+  else
+    v = ldexpf((float)(man | 0x400), ex - 25);
+  return sign ? -v : v;
+}
+
+/* The column MLP: mur + s * MLP(x / s) with s the power of two just above
+ * max |x_i| (so the scaling is exact), 12 -> 32 ReLU -> 32 ReLU -> 1,
+ * products and sums in float, one rounding each. */
+static float lb_predict(const float* w, float mur_k, const float* x) {
+  float m = 0.0f;
+  for (int i = 0; i < XF_LB_IN; ++i) {
+    const float ax = fabsf(x[i]);
+    if (ax > m) m = ax;
+  }
+  const float mur = x[4] + mur_k * (x[0] - x[3]);
+  if (m == 0.0f) return mur;
+  int e;
+  (void)frexpf(m, &e);
+  float xs[XF_LB_IN], h1[XF_LB_HID], h2[XF_LB_HID];
+  for (int i = 0; i < XF_LB_IN; ++i) xs[i] = ldexpf(x[i], -e);
+  const float* W1 = w;
+  const float* b1 = W1 + XF_LB_HID * XF_LB_IN;
+  const float* W2 = b1 + XF_LB_HID;
+  const float* b2 = W2 + XF_LB_HID * XF_LB_HID;
+  const float* w3 = b2 + XF_LB_HID;
+  const float b3 = w3[XF_LB_HID];
+  for (int o = 0; o < XF_LB_HID; ++o) {
+    float acc = b1[o];
+    for (int i = 0; i < XF_LB_IN; ++i) acc = acc + W1[o * XF_LB_IN + i] * xs[i];
+    h1[o] = acc > 0.0f ? acc : 0.0f;
+  }
+  for (int o = 0; o < XF_LB_HID; ++o) {
+    float acc = b2[o];
+    for (int i = 0; i < XF_LB_HID; ++i) acc = acc + W2[o * XF_LB_HID + i] * h1[i];
+    h2[o] = acc > 0.0f ? acc : 0.0f;
+  }
+  float r = b3;
+  for (int i = 0; i < XF_LB_HID; ++i) r = r + w3[i] * h2[i];
+  return mur + ldexpf(r, e);
+}
+
+typedef struct lb_geom {
+  int f, c, a, b, positive;
+  int64_t n[3];
+  int64_t na1, nb1;  /* n_a + 1, n_b + 1 */
+  int64_t off;       /* history offset of the face */
+} lb_geom;
+
+static lb_geom lb_face(const xf_scene* s, int f) {
+  lb_geom g;
+  g.f = f;
+  g.c = f / 2;
+  g.positive = f & 1;
+  g.a = (g.c + 1) % 3;
+  g.b = (g.c + 2) % 3;
+  g.n[0] = s->nx;
+  g.n[1] = s->ny;
+  g.n[2] = s->nz;
+  g.na1 = g.n[g.a] + 1;
+  g.nb1 = g.n[g.b] + 1;
+  g.off = 0;
+  for (int h = 0; h < f; ++h) {
+    const int c = h / 2, a = (c + 1) % 3, b = (c + 2) % 3;
+    g.off += 16 * (g.n[a] + 1) * (g.n[b] + 1);
+  }
+  return g;
+}
+
+/* in-plane index range of component cc (0: E_a, 1: E_b) */
+static void lb_range(const lb_geom* g, int cc, int64_t inset, int64_t* p1, int64_t* q1) {
+  *p1 = g->n[g->a] - inset - (cc == 0 ? 1 : 0);
+  *q1 = g->n[g->b] - inset - (cc == 1 ? 1 : 0);
+}
+
+static double* lb_field(xf_scene* s, const lb_geom* g, int cc, int64_t inset, int layer,
+                        int64_t p, int64_t q) {
+  int64_t pos[3];
+  pos[g->c] = g->positive ? g->n[g->c] - inset - layer : inset + layer;
+  pos[g->a] = p;
+  pos[g->b] = q;
+  return at3(efield(s, cc == 0 ? g->a : g->b), pos[0], pos[1], pos[2]);
+}
+
+static double* lb_hist(xf_scene* s, const lb_geom* g, int par, int cc, int layer, int64_t p,
+                       int64_t q) {
+  return s->lb->history.data +
+         g->off + ((((int64_t)par * 2 + cc) * 4 + layer) * g->na1 + p) * g->nb1 + q;
+}
+
+/* a node of component cc lies on another face of the layer: which (-1 if none) */
+static int lb_other_face(const xf_scene* s, const lb_geom* g, int cc, int64_t p, int64_t q) {
+  const int mask = s->lb->faces;
+  if (cc == 0) { /* E_a: on the b faces at q = 0, n_b */
+    if (q == 0 && (mask >> (2 * g->b)) & 1) return 2 * g->b;
+    if (q == g->n[g->b] && (mask >> (2 * g->b + 1)) & 1) return 2 * g->b + 1;
+  } else { /* E_b: on the a faces at p = 0, n_a */
+    if (p == 0 && (mask >> (2 * g->a)) & 1) return 2 * g->a;
+    if (p == g->n[g->a] && (mask >> (2 * g->a + 1)) & 1) return 2 * g->a + 1;
+  }
+  return -1;
+}
+
+static void lb_inputs(xf_scene* s, const lb_geom* g, int cc, int64_t inset, int64_t p,
+                      int64_t q, int64_t p1, int64_t q1, int par, float* x) {
+  for (int l = 1; l <= 3; ++l) x[l - 1] = (float)*lb_field(s, g, cc, inset, l, p, q);
+  for (int l = 0; l < 4; ++l) x[3 + l] = (float)*lb_hist(s, g, par, cc, l, p, q);
+  for (int l = 0; l < 4; ++l) x[7 + l] = (float)*lb_hist(s, g, par ^ 1, cc, l, p, q);
+  const float u = x[3];
+  const float up = p < p1 ? (float)*lb_hist(s, g, par, cc, 0, p + 1, q) : u;
+  const float um = p > inset ? (float)*lb_hist(s, g, par, cc, 0, p - 1, q) : u;
+  const float vp = q < q1 ? (float)*lb_hist(s, g, par, cc, 0, p, q + 1) : u;
+  const float vm = q > inset ? (float)*lb_hist(s, g, par, cc, 0, p, q - 1) : u;
+  x[11] = (((up + um) + (vp + vm)) - 4.0f * u);
+}
+
+/* One step of the learned layer (after the E pass, its corrections and the
+ * point sources of step t): APPLY writes the layer-0 nodes (faces' inner
+ * nodes first, then the edge nodes each owned by its lower face), RECORD
+ * stores samples; both then roll the history. */
+static void lb_step(xf_scene* s, int64_t t) {
+  xf_learned_boundary* lb = s->lb;
+  const int64_t inset = lb->inset;
+  const int par = (int)(t & 1);
+  float w[XF_LB_NPARAM];
+  if (lb->mode == XF_LB_APPLY)
+    for (int i = 0; i < XF_LB_NPARAM; ++i) w[i] = half_to_float(lb->weights[i]);
+  const float k = (float)lb->mur_k;
+  const int record = lb->mode == XF_LB_RECORD && (t % lb->rec_every) == 0 &&
+                     t / lb->rec_every < lb->rec_slots;
+  int64_t col = 0, ncol = 0;
+  for (int f = 0; f < 6; ++f) {
+    if (!((lb->faces >> f) & 1)) continue;
+    const lb_geom g = lb_face(s, f);
+    for (int cc = 0; cc < 2; ++cc) {
+      int64_t p1, q1;
+      lb_range(&g, cc, inset, &p1, &q1);
+      ncol += (p1 - inset + 1) * (q1 - inset + 1);
+    }
+  }
+  for (int phase = 0; phase < 2; ++phase) {
+    if (lb->mode == XF_LB_RECORD && phase == 1) break;
+    col = 0;
+    for (int f = 0; f < 6; ++f) {
+      if (!((lb->faces >> f) & 1)) continue;
+      const lb_geom g = lb_face(s, f);
+      for (int cc = 0; cc < 2; ++cc) {
+        int64_t p1, q1;
+        lb_range(&g, cc, inset, &p1, &q1);
+        for (int64_t p = inset; p <= p1; ++p)
+          for (int64_t q = inset; q <= q1; ++q, ++col) {
+            float x[XF_LB_IN];
+            if (lb->mode == XF_LB_RECORD) {
+              if (!record) continue;
+              lb_inputs(s, &g, cc, inset, p, q, p1, q1, par, x);
+              float* r = lb->rec + ((t / lb->rec_every) * ncol + col) * 13;
+              for (int i = 0; i < XF_LB_IN; ++i) r[i] = x[i];
+              r[12] = (float)*lb_field(s, &g, cc, inset, 0, p, q);
+              continue;
+            }
+            const int other = lb_other_face(s, &g, cc, p, q);
+            if (phase == 0 ? other >= 0 : !(other >= 0 && f < other)) continue;
+            lb_inputs(s, &g, cc, inset, p, q, p1, q1, par, x);
+            *lb_field(s, &g, cc, inset, 0, p, q) = (double)lb_predict(w, k, x);
+          }
+      }
+    }
+  }
+  if (record && t / lb->rec_every + 1 > lb->rec_count) lb->rec_count = t / lb->rec_every + 1;
+  /* history: u^{t+1} of layers 0..3 into the parity buffer of t + 1 */
+  for (int f = 0; f < 6; ++f) {
+    if (!((lb->faces >> f) & 1)) continue;
+    const lb_geom g = lb_face(s, f);
+    for (int cc = 0; cc < 2; ++cc) {
+      int64_t p1, q1;
+      lb_range(&g, cc, inset, &p1, &q1);
+      for (int64_t p = inset; p <= p1; ++p)
+        for (int64_t q = inset; q <= q1; ++q)
+          for (int l = 0; l < 4; ++l)
+            *lb_hist(s, &g, par ^ 1, cc, l, p, q) =
+                (double)(float)*lb_field(s, &g, cc, inset, l, p, q);
+    }
+  }
+}
+
 /* --------------------------------------------------------------- the loop */
 
 static char g_err[512];
@@ -558,6 +764,9 @@ int xfo_run(xf_scene* s, int64_t n_steps) {
     /* correctE */
     if (s->tfsf) tfsf_correct(s, 1, t);
     for (int32_t p = 0; p < s->n_pml; ++p) pml_correct(s, &s->pml[p], 1);
+    /* additions without a reference counterpart */
+    point_sources(s, t);
+    if (s->lb && s->lb->mode != XF_LB_OFF) lb_step(s, t);
     /* record (domain_hd.cu:65-77) */
     for (int32_t q = 0; q < s->n_nf2ff; ++q) {
       const xf_nf2ff* nf = &s->nf2ff[q];
@@ -568,6 +777,16 @@ int xfo_run(xf_scene* s, int64_t n_steps) {
     /* nextStep */
     s->current_step += 1;
   }
+  return 0;
+}
+
+/* The learned layer's column MLP on its own (test hook): n columns of 12
+ * inputs -> n predictions, weights as in xf_learned_boundary. */
+int xfo_lb_predict(const uint16_t* weights, double mur_k, const float* x, int64_t n,
+                   float* out) {
+  float w[XF_LB_NPARAM];
+  for (int i = 0; i < XF_LB_NPARAM; ++i) w[i] = half_to_float(weights[i]);
+  for (int64_t c = 0; c < n; ++c) out[c] = lb_predict(w, (float)mur_k, x + c * XF_LB_IN);
   return 0;
 }
 

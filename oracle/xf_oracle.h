@@ -20,6 +20,9 @@ void xfo_set_num_threads(int n);
  * triple loop. */
 int xfo_yee_component(xf_scene* s, int is_e, int xyz);
 int xfo_yee_component_triple_loop(xf_scene* s, int is_e, int xyz);
+/* The learned layer's column predictor alone (n columns of 12 inputs). */
+int xfo_lb_predict(const uint16_t* weights, double mur_k, const float* x, int64_t n,
+                   float* out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,127 @@
+"""The learned absorbing layer on the GPU: parity of the inference and
+recording kernels with the oracle, the HIP trainer's gradient against the
+numpy restatement, and the judge -- reflection error against a 16-cell CPML
+of a network trained on the GPU, next to first-order Mur and thin CPMLs.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import lb_mlp
+import xfdtd_cuda_amd as xf
+from xfdtd_cuda_amd import learned, scenes
+from conftest import FIELDS
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(build, steps):
+    g, c = build(), build()
+    e = xf.Engine(g).upload()
+    e.run(steps // 2).run(steps - steps // 2).download()
+    e.close()
+    oracle.run(c.scene_ptr(), steps)
+    return g, c
+
+
+def test_learned_layer_apply_matches_oracle(gpu_available):
+    w = lb_mlp.init_weights(11, out_scale=0.3).astype(np.float16)
+    build = lambda: scenes.point_source_box(n=18, boundary="learned", size=2, steps=80, seed=3,
+                                            weights=w)
+    g, c = _pair(build, 80)
+    for n in FIELDS:
+        assert np.array_equal(g.array(n), c.array(n)), n
+    assert np.array_equal(g.array("lb.history"), c.array("lb.history"))
+    assert np.abs(c.array("lb.history")).max() > 0
+
+
+def test_learned_layer_partial_faces_and_fp32(gpu_available):
+    """Faces XN, YP, ZN only (edges between learned and PEC faces); FP32
+    stays within float rounding of the FP64 oracle."""
+    w = lb_mlp.init_weights(5, out_scale=0.2).astype(np.float16)
+
+    def build():
+        s = xf.Simulation(5e-3, 5e-3, 5e-3, 0.9)
+        s.add_cube("domain", (0, 0, 0), (0.08,) * 3, xf.Material.air())
+        s.add_learned_boundary(w, faces=("XN", "YP", "ZN"), thickness=2)
+        s.add_point_source(2, (9, 8, 10), xf.Waveform.cosine_modulated_gaussian(3e-10, 1e-9, 3e9))
+        s.init(60)
+        return s
+
+    g, c = _pair(build, 60)
+    for n in FIELDS:
+        assert np.array_equal(g.array(n), c.array(n)), n
+    g32 = build()
+    e = xf.Engine(g32, precision="fp32").upload()
+    e.run(60).download()
+    e.close()
+    for n in FIELDS:
+        a, b = g32.array(n), c.array(n)
+        assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b) + 1e-30, n
+
+
+def test_learned_layer_record_matches_oracle(gpu_available):
+    build = lambda: scenes.point_source_box(n=14, boundary="cpml", size=6, steps=50, seed=5,
+                                            record=(8, 2, 30))
+    g, c = _pair(build, 50)
+    a, b = g.lb_samples(), c.lb_samples()
+    assert a.shape == b.shape and a.shape[0] == 25
+    assert np.array_equal(a, b)
+    for n in FIELDS:
+        assert np.array_equal(g.array(n), c.array(n)), n
+
+
+def _samples():
+    s = scenes.point_source_box(n=16, boundary="cpml", size=8, steps=120, seed=9,
+                                record=(10, 3, 40))
+    oracle.run(s.scene_ptr(), 120)
+    return s.lb_samples().reshape(-1, 13), learned.mur_k(s.dt, 5e-3)
+
+
+def test_trainer_gradient_matches_numpy(gpu_available):
+    smp, k = _samples()
+    w = lb_mlp.init_weights(4, out_scale=0.5)
+    t = learned.Trainer(smp, k).set_weights(w)
+    first = smp.shape[0] // 3
+    count = smp.shape[0] - first
+    assert count > 2000
+    loss_g, grad_g = t.grad(first, count)
+    xs, r = lb_mlp.targets(smp[first:], k, drop_zero=False)
+    loss_n, grad_n = lb_mlp.loss_and_grad(w, xs, r)
+    assert abs(loss_g - loss_n) <= 1e-5 * abs(loss_n)
+    scale = np.abs(grad_n).max()
+    assert np.abs(grad_g - grad_n).max() <= 2e-5 * scale, np.abs(grad_g - grad_n).max() / scale
+    # reproducible bit for bit
+    assert np.array_equal(t.grad(first, count)[1], grad_g)
+    t.close()
+
+
+def test_trainer_reduces_loss(gpu_available):
+    smp, k = _samples()
+    t = learned.Trainer(smp, k).set_weights(lb_mlp.init_weights(0))
+    before = t.loss()
+    t.train(400, batch=2048, lr=2e-3, seed=1)
+    after = t.loss()
+    w32, bits = t.weights()
+    t.close()
+    assert after < 0.7 * before, (before, after)
+    # the exported fp16 bits are the round-to-nearest-even of the master weights
+    assert np.array_equal(w32.astype(np.float16).view(np.uint16), bits)
+
+
+def test_learned_layer_judged_against_16_cell_cpml(gpu_available):
+    """Train on GPU-recorded data of seeds 0-2, judge on seed 7: the learned
+    2-cell layer reflects less than first-order Mur and than a 2-cell CPML."""
+    smp, k = learned.record_samples(n=32, seeds=(0, 1, 2), steps=300, every=2)
+    t = learned.Trainer(smp, k).set_weights(learned.init_weights(0))
+    loss0 = t.loss()
+    for lr, steps in ((2e-3, 1500), (5e-4, 1500)):
+        t.train(steps, batch=8192, lr=lr, seed=steps)
+    loss1 = t.loss()
+    _, w16 = t.weights()
+    t.close()
+    err = learned.reflection_errors(w16, n=32, seed=7, steps=300, every=15)
+    print("loss", loss0, "->", loss1, "reflection", err)
+    assert loss1 < loss0
+    assert err["learned2"] < err["mur2"]
+    assert err["learned2"] < err["cpml2"]

@@ -186,6 +186,12 @@ def host() -> ctypes.CDLL:
                                   p3d],
             "xfh_sim_waveform": [c_vp, p3d, p3d, c_i64],
             "xfh_material_permittivity": [pm, c_f64, p3d, p3d],
+            "xfh_sim_add_point_source": [c_vp, c_i32, p3i, c_i32, c_f64, c_f64, c_f64, c_f64],
+            "xfh_sim_add_learned_boundary": [c_vp, c_i32, c_i32, c_i64,
+                                             ctypes.POINTER(ctypes.c_uint16), c_i32, c_i64,
+                                             c_i64],
+            "xfh_sim_lb_samples": [c_vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_float)),
+                                   p3i, p3i],
         }
         for name, args in sigs.items():
             fn = getattr(lib, name)

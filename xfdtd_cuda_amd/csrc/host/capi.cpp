@@ -262,6 +262,56 @@ int xfh_sim_add_nf2ff_fd(xfh_sim* s, const int64_t st[3], const double* freqs, i
   });
 }
 
+int xfh_sim_add_point_source(xfh_sim* s, int32_t comp, const int64_t idx[3],
+                             int32_t wave_kind, double tau, double t0, double freq,
+                             double amplitude) {
+  return guard([&] {
+    if (!s || !idx) return fail("null argument");
+    if (comp < 0 || comp > 2) return fail("bad point source component");
+    if (idx[0] < 0 || idx[1] < 0 || idx[2] < 0) return fail("negative point source index");
+    s->sim->addWaveformSource(std::make_shared<xfdtd::PointSource>(
+        static_cast<xfdtd::Axis::XYZ>(comp), idx[0], idx[1], idx[2],
+        makeWave(wave_kind, tau, t0, freq), amplitude));
+    return 0;
+  });
+}
+
+int xfh_sim_add_learned_boundary(xfh_sim* s, int32_t mode, int32_t faces, int64_t size,
+                                 const uint16_t* weights, int32_t n_weights, int64_t every,
+                                 int64_t slots) {
+  return guard([&] {
+    if (!s) return fail("null sim");
+    if (size < 0 || every < 0 || slots < 0) return fail("negative learned-layer size");
+    if (mode == XF_LB_APPLY) {
+      if (!weights || n_weights != XF_LB_NPARAM) return fail("learned layer: bad weights");
+      s->sim->addBoundary(xfdtd::LearnedBoundary::makeApply(
+          faces, static_cast<xfdtd::Index>(size),
+          std::vector<uint16_t>(weights, weights + n_weights)));
+    } else if (mode == XF_LB_RECORD) {
+      s->sim->addBoundary(xfdtd::LearnedBoundary::makeRecord(
+          faces, static_cast<xfdtd::Index>(size), static_cast<xfdtd::Index>(every),
+          static_cast<xfdtd::Index>(slots)));
+    } else {
+      return fail("learned layer: bad mode");
+    }
+    return 0;
+  });
+}
+
+int xfh_sim_lb_samples(xfh_sim* s, const float** data, int64_t* columns, int64_t* recorded) {
+  return guard([&] {
+    if (!s || !data || !columns || !recorded) return fail("null argument");
+    for (auto& b : s->sim->boundaries())
+      if (auto l = dynamic_cast<xfdtd::LearnedBoundary*>(b.get())) {
+        *data = l->samples().data();
+        *columns = static_cast<int64_t>(l->columns());
+        *recorded = static_cast<int64_t>(l->recordedSteps());
+        return 0;
+      }
+    return fail("no learned layer");
+  });
+}
+
 int xfh_sim_set_coefficient_layout(xfh_sim* s, int32_t layout) {
   return guard([&] {
     if (!s) return fail("null sim");
@@ -329,6 +379,11 @@ int xfh_scene_array(const xf_scene* sc, const char* name, double** data, int64_t
       if (sc->tfsf) a = tfsfMember(*sc->tfsf, parts[1]);
     } else if (h == "ade" && parts.size() == 2) {
       if (sc->ade) a = adeMember(*sc->ade, parts[1]);
+    } else if (h == "lb" && parts.size() == 2 && parts[1] == "history") {
+      if (sc->lb) a = &sc->lb->history;
+    } else if (h == "point" && parts.size() == 3 && parts[2] == "value") {
+      const int p = std::stoi(parts[1]);
+      if (p >= 0 && p < sc->n_point) a = &sc->point[p].value;
     } else if (h == "nf2ff" && parts.size() >= 4) {
       const int q = std::stoi(parts[1]);
       const int f = std::stoi(parts[2]);

@@ -346,14 +346,20 @@ Simulation::Simulation(Real dx, Real dy, Real dz, Real cfl)
 }
 
 namespace {
+// Cells a face's boundary adds outside the domain: the thickest PML, or
+// the learned layer's thickness (the two do not share a face).
 Index pmlThickness(const std::vector<std::shared_ptr<Boundary>>& bs,
                    Axis::Direction d) {
-  Index t = 0;
+  Index t = 0, l = 0;
   for (auto& b : bs) {
     auto p = dynamic_cast<PML*>(b.get());
     if (p && p->direction() == d) t = std::max(t, p->thickness());
+    auto q = dynamic_cast<LearnedBoundary*>(b.get());
+    if (q && q->mode() == XF_LB_APPLY && ((q->faces() >> static_cast<int>(d)) & 1))
+      l = std::max(l, q->thickness());
   }
-  return t;
+  if (t > 0 && l > 0) throw XFDTDException("a face carries both a PML and a learned layer");
+  return t + l;
 }
 }  // namespace
 
@@ -1002,6 +1008,95 @@ void Simulation::buildADE() {
   d.jz_prev = s._jzp.view();
 }
 
+// ----------------------------------------------------------- point source
+PointSource::PointSource(Axis::XYZ component, Index i, Index j, Index k,
+                         std::unique_ptr<Waveform> waveform, Real amplitude)
+    : _comp(component), _i(i), _j(j), _k(k), _waveform(std::move(waveform)), _amp(amplitude) {}
+
+void Simulation::buildPointSource(PointSource& p) {
+  const Array& f = _fields[static_cast<int>(p._comp)];
+  const auto& sh = f.shape();
+  if (static_cast<int64_t>(p._i) >= sh[0] || static_cast<int64_t>(p._j) >= sh[1] ||
+      static_cast<int64_t>(p._k) >= sh[2])
+    throw XFDTDException("point source outside its component's array");
+  const Index nt = _time._start + _time._size;
+  p._waveform->init(_time._dt, nt);
+  p._value = Array({static_cast<int64_t>(nt)});
+  for (Index n = 0; n < nt; ++n) p._value[n] = p._amp * p._waveform->value()[n];
+  p._desc = xf_point_source{};
+  p._desc.comp = static_cast<int32_t>(p._comp);
+  p._desc.i = static_cast<int64_t>(p._i);
+  p._desc.j = static_cast<int64_t>(p._j);
+  p._desc.k = static_cast<int64_t>(p._k);
+  p._desc.value = p._value.view();
+}
+
+// -------------------------------------------------------- learned boundary
+std::shared_ptr<LearnedBoundary> LearnedBoundary::makeApply(int faces, Index thickness,
+                                                            std::vector<uint16_t> weights) {
+  if (faces <= 0 || faces > 63) throw XFDTDException("learned layer: bad face mask");
+  if (thickness < 1) throw XFDTDException("learned layer: thickness must be >= 1");
+  if (weights.size() != XF_LB_NPARAM)
+    throw XFDTDException("learned layer: expected " + std::to_string(XF_LB_NPARAM) + " weights");
+  auto l = std::make_shared<LearnedBoundary>();
+  l->_thickness = thickness;
+  l->_weights = std::move(weights);
+  l->_desc.mode = XF_LB_APPLY;
+  l->_desc.faces = faces;
+  l->_desc.inset = 0;
+  return l;
+}
+
+std::shared_ptr<LearnedBoundary> LearnedBoundary::makeRecord(int faces, Index inset, Index every,
+                                                             Index slots) {
+  if (faces <= 0 || faces > 63) throw XFDTDException("learned layer: bad face mask");
+  if (inset < 1) throw XFDTDException("learned layer: record inset must be >= 1");
+  if (every < 1 || slots < 1) throw XFDTDException("learned layer: bad record schedule");
+  auto l = std::make_shared<LearnedBoundary>();
+  l->_desc.mode = XF_LB_RECORD;
+  l->_desc.faces = faces;
+  l->_desc.inset = static_cast<int32_t>(inset);
+  l->_desc.rec_every = static_cast<int32_t>(every);
+  l->_desc.rec_slots = static_cast<int64_t>(slots);
+  return l;
+}
+
+void Simulation::buildLearnedBoundary(LearnedBoundary& l) {
+  const GridSpace& g = *_grid;
+  if (g.dimension() != GridSpace::Dimension::THREE)
+    throw XFDTDException("learned layer: 3-D grids only");
+  if (_dx != _dy || _dx != _dz) throw XFDTDException("learned layer: cells must be cubes");
+  const int64_t n[3] = {static_cast<int64_t>(g.sizeX()), static_cast<int64_t>(g.sizeY()),
+                        static_cast<int64_t>(g.sizeZ())};
+  const int64_t inset = l._desc.inset;
+  int64_t hist = 0, cols = 0;
+  for (int f = 0; f < 6; ++f) {
+    const int c = f / 2, a = (c + 1) % 3, b = (c + 2) % 3;
+    hist += 16 * (n[a] + 1) * (n[b] + 1);
+    if (!((l._desc.faces >> f) & 1)) continue;
+    if (n[c] < 2 * inset + 4) throw XFDTDException("learned layer: grid too thin for the columns");
+    for (int cc = 0; cc < 2; ++cc) {
+      const int64_t p1 = n[a] - inset - (cc == 0 ? 1 : 0);
+      const int64_t q1 = n[b] - inset - (cc == 1 ? 1 : 0);
+      if (p1 < inset || q1 < inset) throw XFDTDException("learned layer: inset too large");
+      cols += (p1 - inset + 1) * (q1 - inset + 1);
+    }
+  }
+  l._history = Array({hist});
+  l._columns = static_cast<Index>(cols);
+  l._desc.mur_k = (C_0 * _time._dt - _dx) / (C_0 * _time._dt + _dx);
+  l._desc.history = l._history.view();
+  l._desc.weights = l._weights.empty() ? nullptr : l._weights.data();
+  l._desc.rec_count = 0;
+  if (l._desc.mode == XF_LB_RECORD) {
+    l._rec.assign(static_cast<size_t>(l._desc.rec_slots) * cols * 13, 0.0f);
+    l._desc.rec = l._rec.data();
+  } else {
+    l._rec.clear();
+    l._desc.rec = nullptr;
+  }
+}
+
 void Simulation::fillScene() {
   const GridSpace& g = *_grid;
   _scene = xf_scene{};
@@ -1050,6 +1145,17 @@ void Simulation::fillScene() {
   _scene.n_nf2ff = static_cast<int32_t>(_nf_desc.size());
   _scene.nf2ff = _nf_desc.empty() ? nullptr : _nf_desc.data();
   _scene.ade = _ade ? &_ade->desc() : nullptr;
+  _pt_desc.clear();
+  for (auto& w : _sources)
+    if (auto p = dynamic_cast<PointSource*>(w.get())) _pt_desc.push_back(p->desc());
+  _scene.n_point = static_cast<int32_t>(_pt_desc.size());
+  _scene.point = _pt_desc.empty() ? nullptr : _pt_desc.data();
+  _scene.lb = nullptr;
+  for (auto& b : _boundaries)
+    if (auto l = dynamic_cast<LearnedBoundary*>(b.get())) {
+      if (_scene.lb) throw XFDTDException("only one learned layer is supported");
+      _scene.lb = &l->desc();
+    }
 }
 
 void Simulation::init(Index time_step) {
@@ -1085,6 +1191,10 @@ void Simulation::init(Index time_step) {
     if (f) buildNF2FF(*f);
   }
   buildADE();
+  for (auto& w : _sources)
+    if (auto p = dynamic_cast<PointSource*>(w.get())) buildPointSource(*p);
+  for (auto& b : _boundaries)
+    if (auto l = dynamic_cast<LearnedBoundary*>(b.get())) buildLearnedBoundary(*l);
   fillScene();
   _initialized = true;
 }

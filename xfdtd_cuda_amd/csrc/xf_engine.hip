@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -134,6 +135,11 @@ struct Engine {
   std::vector<NfDev> nf;
   // ADE
   AdeDev ade{};
+  // point sources and the learned layer (additions)
+  PointDev pts{};
+  LbDev lbd{};
+  bool has_lb{false};
+  int64_t lb_rec_count{0}, lb_rec_count_saved{0};
   bool has_ade{false};
   std::vector<int64_t> ade_cells;  // host (i,j,k) linear index in (nx+1)(ny+1)(nz+1)
   // time counter for graph replay
@@ -223,6 +229,9 @@ struct Engine {
   int upload_tfsf();
   int upload_nf2ff();
   int upload_ade();
+  int validate_additions();
+  int upload_additions();
+  int64_t lb_recorded_after(int64_t step) const;
   int download();
   int run(int64_t n);
   template <typename S>
@@ -389,6 +398,43 @@ int Engine::validate() {
     }
     has_ade = true;
   }
+  return validate_additions();
+}
+
+// Point sources and the learned layer: every index the kernels will form.
+int Engine::validate_additions() {
+  const int64_t n3[3] = {nx, ny, nz};
+  if (sc->n_point < 0 || (sc->n_point > 0 && !sc->point)) return fail(XF_ERR_INVALID, "point sources missing");
+  for (int32_t q = 0; q < sc->n_point; ++q) {
+    const xf_point_source& P = sc->point[q];
+    if (P.comp < 0 || P.comp > 2) return fail(XF_ERR_INVALID, "point source: bad component");
+    const auto sh = field_shape(0, P.comp, nx, ny, nz);
+    if (P.i < 0 || P.j < 0 || P.k < 0 || P.i >= sh[0] || P.j >= sh[1] || P.k >= sh[2])
+      return fail(XF_ERR_INVALID, "point source " + std::to_string(q) + " outside its component");
+    if (!P.value.data || P.value.ndim != 1 || P.value.shape[0] < nt)
+      return fail(XF_ERR_INVALID, "point source " + std::to_string(q) + ": value table shorter than the run");
+  }
+  if (!sc->lb || sc->lb->mode == XF_LB_OFF) return XF_OK;
+  const xf_learned_boundary& L = *sc->lb;
+  if (dim != 3) return fail(XF_ERR_UNSUPPORTED, "learned layer: 3-D grids only");
+  if (L.mode != XF_LB_APPLY && L.mode != XF_LB_RECORD) return fail(XF_ERR_INVALID, "learned layer: bad mode");
+  if (L.faces <= 0 || L.faces > 63) return fail(XF_ERR_INVALID, "learned layer: bad face mask");
+  if (L.mode == XF_LB_APPLY && (L.inset != 0 || !L.weights))
+    return fail(XF_ERR_INVALID, "learned layer: APPLY needs inset 0 and weights");
+  if (L.mode == XF_LB_RECORD && (L.inset < 1 || L.rec_every < 1 || L.rec_slots < 1 || !L.rec))
+    return fail(XF_ERR_INVALID, "learned layer: bad RECORD schedule or buffer");
+  int64_t hist = 0;
+  for (int f = 0; f < 6; ++f) {
+    const int c = f / 2, a = (c + 1) % 3, b = (c + 2) % 3;
+    hist += 16 * (n3[a] + 1) * (n3[b] + 1);
+    if (!((L.faces >> f) & 1)) continue;
+    if (n3[c] < 2 * static_cast<int64_t>(L.inset) + 4) return fail(XF_ERR_INVALID, "learned layer: grid too thin");
+    if (n3[a] - 1 - L.inset < L.inset || n3[b] - 1 - L.inset < L.inset)
+      return fail(XF_ERR_INVALID, "learned layer: inset too large");
+  }
+  if (!L.history.data || L.history.ndim != 1 || L.history.shape[0] != hist)
+    return fail(XF_ERR_INVALID, "learned layer: history must hold " + std::to_string(hist) + " values");
+  has_lb = true;
   return XF_OK;
 }
 
@@ -936,6 +982,109 @@ int Engine::upload_ade() {
   return XF_OK;
 }
 
+namespace {
+// IEEE binary16 bits -> float (exact)
+float half_bits_to_float(uint16_t h) {
+  const int ex = (h >> 10) & 0x1f, man = h & 0x3ff;
+  float v;
+  if (ex == 0)
+    v = std::ldexp(static_cast<float>(man), -24);
+  else if (ex == 31)
+    v = man ? std::numeric_limits<float>::quiet_NaN() : std::numeric_limits<float>::infinity();
+  else
+    v = std::ldexp(static_cast<float>(man | 0x400), ex - 25);
+  return (h >> 15) ? -v : v;
+}
+}  // namespace
+
+int64_t Engine::lb_recorded_after(int64_t step) const {
+  const xf_learned_boundary& L = *sc->lb;
+  if (step <= 0) return lb_rec_count;
+  const int64_t slots = std::min<int64_t>(L.rec_slots, (step + L.rec_every - 1) / L.rec_every);
+  return std::max(lb_rec_count, slots);
+}
+
+int Engine::upload_additions() {
+  pts = PointDev{};
+  if (sc->n_point > 0) {
+    const int n = sc->n_point;
+    std::vector<int> comp(n);
+    std::vector<int64_t> idx(n);
+    std::vector<double> val(static_cast<size_t>(n) * nt);
+    for (int q = 0; q < n; ++q) {
+      const xf_point_source& P = sc->point[q];
+      comp[q] = P.comp;
+      idx[q] = P.i * sx + P.j * sy + P.k;
+      std::copy(P.value.data, P.value.data + nt, val.begin() + static_cast<size_t>(q) * nt);
+    }
+    void *pc, *pi, *pv;
+    XF_TRY(upload_vec(&pc, comp));
+    XF_TRY(upload_vec(&pi, idx));
+    XF_TRY(upload_vec(&pv, val));
+    pts.n = n;
+    pts.nt = nt;
+    pts.comp = static_cast<const int*>(pc);
+    pts.idx = static_cast<const int64_t*>(pi);
+    pts.value = static_cast<const double*>(pv);
+  }
+  lbd = LbDev{};
+  if (!has_lb) return XF_OK;
+  const xf_learned_boundary& L = *sc->lb;
+  const int64_t n3[3] = {nx, ny, nz};
+  lbd.mode = L.mode;
+  lbd.faces = L.faces;
+  lbd.inset = L.inset;
+  lbd.mur_k = static_cast<float>(L.mur_k);
+  lbd.rec_every = L.rec_every > 0 ? L.rec_every : 1;
+  lbd.rec_slots = L.rec_slots;
+  for (int d = 0; d < 3; ++d) lbd.n[d] = static_cast<int>(n3[d]);
+  int64_t off = 0, col = 0;
+  for (int f = 0; f < 6; ++f) {
+    const int c = f / 2, a = (c + 1) % 3, b = (c + 2) % 3;
+    const int64_t face_off = off;
+    off += 16 * (n3[a] + 1) * (n3[b] + 1);
+    if (!((L.faces >> f) & 1)) continue;
+    LbFace& F = lbd.face[lbd.nface++];
+    F.f = f;
+    F.c = c;
+    F.a = a;
+    F.b = b;
+    F.positive = f & 1;
+    F.hist_off = face_off;
+    F.na1 = n3[a] + 1;
+    F.nb1 = n3[b] + 1;
+    for (int cc = 0; cc < 2; ++cc) {
+      F.start[cc] = col;
+      F.p1[cc] = static_cast<int>(n3[a] - L.inset - (cc == 0 ? 1 : 0));
+      F.q1[cc] = static_cast<int>(n3[b] - L.inset - (cc == 1 ? 1 : 0));
+      col += static_cast<int64_t>(F.p1[cc] - L.inset + 1) * (F.q1[cc] - L.inset + 1);
+    }
+  }
+  lbd.total = col;
+  std::vector<float> hist(off);
+  for (int64_t i = 0; i < off; ++i) hist[i] = static_cast<float>(L.history.data[i]);
+  void* ph;
+  XF_TRY(upload_vec(&ph, hist));
+  lbd.hist = static_cast<float*>(ph);
+  state.emplace_back(ph, hist.size() * sizeof(float));
+  if (L.mode == XF_LB_APPLY) {
+    std::vector<float> w(XF_LB_NPARAM);
+    for (int i = 0; i < XF_LB_NPARAM; ++i) w[i] = half_bits_to_float(L.weights[i]);
+    void* pw;
+    XF_TRY(upload_vec(&pw, w));
+    lbd.w = static_cast<const float*>(pw);
+  } else {
+    const size_t nrec = static_cast<size_t>(L.rec_slots) * col * 13;
+    std::vector<float> rec(L.rec, L.rec + nrec);
+    void* pr;
+    XF_TRY(upload_vec(&pr, rec));
+    lbd.rec = static_cast<float*>(pr);
+    state.emplace_back(pr, nrec * sizeof(float));
+  }
+  lb_rec_count = L.rec_count;
+  return XF_OK;
+}
+
 int Engine::upload() {
   release();
   XF_HIP(hipSetDevice(device));
@@ -960,6 +1109,7 @@ int Engine::upload() {
   XF_TRY(upload_tfsf());
   XF_TRY(upload_nf2ff());
   XF_TRY(upload_ade());
+  XF_TRY(upload_additions());
   void* tp;
   XF_TRY(alloc(&tp, sizeof(int)));
   t_dev = static_cast<int*>(tp);
@@ -1017,6 +1167,7 @@ int Engine::snapshot() {
   for (size_t q = 0; q < state.size(); ++q)
     XF_HIP(hipMemcpyAsync(backup[q], state[q].first, state[q].second, hipMemcpyDeviceToDevice, stream));
   XF_HIP(hipStreamSynchronize(stream));
+  lb_rec_count_saved = lb_rec_count;
   return XF_OK;
 }
 
@@ -1026,6 +1177,7 @@ int Engine::restore() {
   for (size_t q = 0; q < state.size(); ++q)
     XF_HIP(hipMemcpyAsync(state[q].first, backup[q], state[q].second, hipMemcpyDeviceToDevice, stream));
   XF_HIP(hipStreamSynchronize(stream));
+  lb_rec_count = lb_rec_count_saved;
   return XF_OK;
 }
 
@@ -1088,6 +1240,17 @@ int Engine::download() {
         }
         if (ml) EP[comp]->data[l3] = eprev[comp * n + c];
       }
+    }
+  }
+  if (has_lb) {
+    xf_learned_boundary& L = *sc->lb;
+    std::vector<float> hist(L.history.shape[0]);
+    XF_HIP(hipMemcpy(hist.data(), lbd.hist, hist.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < hist.size(); ++i) L.history.data[i] = hist[i];
+    if (L.mode == XF_LB_RECORD) {
+      XF_HIP(hipMemcpy(L.rec, lbd.rec, static_cast<size_t>(L.rec_slots) * lbd.total * 13 * sizeof(float),
+                       hipMemcpyDeviceToHost));
+      L.rec_count = lb_rec_count;
     }
   }
   sc->current_step = cur_step;
@@ -1356,6 +1519,17 @@ int Engine::launch_post_e(int t, const int* tp) {
       hipLaunchKernelGGL((k_tfsf<S>), dim3((unsigned)((P.total + 255) / 256)), dim3(256), 0, stream,
                          make_fields<S>(*this), P, t, tp, make_scales<C>(*this), make_dims(*this), dim == 2 ? 1 : 0);
   }
+  if (pts.n > 0)
+    hipLaunchKernelGGL((k_point_sources<S>), dim3(1), dim3(64), 0, stream, make_fields<S>(*this), pts,
+                       make_scales<C>(*this), t, tp);
+  if (has_lb && lbd.total > 0) {
+    const dim3 g((unsigned)((lbd.total + 255) / 256)), b(256);
+    const int phases[3] = {0, lbd.mode == XF_LB_APPLY ? 1 : -1, 2};
+    for (int ph : phases)
+      if (ph >= 0)
+        hipLaunchKernelGGL((k_lb<S>), g, b, 0, stream, make_fields<S>(*this), lbd, make_dims(*this),
+                           make_scales<C>(*this), ph, t, tp);
+  }
   for (auto& N : nf) {
     if (N.pass.total > 0)
       hipLaunchKernelGGL((k_nf2ff<S>), dim3((unsigned)((N.pass.total + 255) / 256)), dim3(256), 0, stream,
@@ -1475,6 +1649,7 @@ int Engine::run(int64_t n) {
   (void)hipEventDestroy(run1);
   timing.bytes_per_cell_step = bytes_per_cell_step();
   cur_step += n;
+  if (has_lb && lbd.mode == XF_LB_RECORD) lb_rec_count = lb_recorded_after(cur_step);
   return XF_OK;
 }
 
@@ -1634,3 +1809,4 @@ int xf_describe(const xf_handle* h, char* buf, size_t len) {
 }  // extern "C"
 
 #include "xf_batch.inl"
+#include "xf_lb_train.inl"

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/xfdtd_hip.h"
+
 namespace xf {
 
 constexpr int kMaxPml = 6;
@@ -191,6 +193,40 @@ struct NfPass {
   double2* cur;      // [freq][face cells][4] -> laid out [freq][face][comp][cell]
   const double2* te; // [freq][nt]
   const double2* th;
+};
+
+// Point sources: node index (padded layout) and component of each; the
+// values are a [source][step] table of doubles.
+struct PointDev {
+  int n;
+  int64_t nt;
+  const int* comp;
+  const int64_t* idx;
+  const double* value;
+};
+
+// The learned layer (xf_learned_boundary): the faces in the mask, in face
+// order, with their column ranges; columns are numbered face by face,
+// component E_a then E_b, then a-index, then b-index.
+struct LbFace {
+  int f, c, a, b, positive;
+  int64_t hist_off;    // first history value of the face
+  int64_t na1, nb1;    // n_a + 1, n_b + 1 (history plane)
+  int64_t start[2];    // first column of E_a, E_b
+  int p1[2], q1[2];    // last a-, b-index of E_a, E_b (first is inset)
+};
+
+struct LbDev {
+  int mode, faces, inset, nface;
+  float mur_k;
+  const float* w;      // XF_LB_NPARAM weights (fp16 values as float)
+  float* hist;
+  float* rec;          // RECORD: [slot][column][13]
+  int rec_every;
+  int64_t rec_slots;
+  int64_t total;       // columns
+  int n[3];
+  LbFace face[6];
 };
 
 // One grid's TFSF pass or NF2FF box inside a batch of grids: the batched

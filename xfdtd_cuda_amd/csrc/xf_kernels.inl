@@ -352,6 +352,172 @@ __global__ void __launch_bounds__(256)
   nf2ff_cell<S>(f, P, t_dev ? *t_dev : t_arg, d, two_d, sc, tid);
 }
 
+// --------------------------------------------------------- point sources
+// E_comp(node) += value[t], sources in registration order (one thread, so
+// sources sharing a node add in a fixed order).
+template <typename S>
+__global__ void k_point_sources(Fields<S> f, PointDev P, Scales<typename Store<S>::C> sc,
+                                int t_arg, const int* t_dev) {
+  using C = typename Store<S>::C;
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int t = t_dev ? *t_dev : t_arg;
+  for (int q = 0; q < P.n; ++q) {
+    S* fld = f.e[P.comp[q]];
+    const int64_t i = P.idx[q];
+    const C v = Store<S>::ld(fld, i, sc.inv_e) + (C)P.value[(int64_t)q * P.nt + t];
+    Store<S>::st(fld, i, v, sc.s_e);
+  }
+}
+
+// --------------------------------------------------------- learned layer
+// The column predictor of xf_learned_boundary: the Mur update plus s times
+// the MLP of x / s, s the power of two above max |x_i| (exact scaling);
+// every product and sum is one float rounding (no contraction).
+__device__ __forceinline__ float lb_predict(const float* __restrict__ w, float mur_k,
+                                            const float (&x)[XF_LB_IN]) {
+  float m = 0.0f;
+#pragma unroll
+  for (int i = 0; i < XF_LB_IN; ++i) {
+    const float ax = fabsf(x[i]);
+    if (ax > m) m = ax;
+  }
+  const float mur = x[4] + mur_k * (x[0] - x[3]);
+  if (m == 0.0f) return mur;
+  int e;
+  (void)frexpf(m, &e);
+  float xs[XF_LB_IN], h1[XF_LB_HID];
+#pragma unroll
+  for (int i = 0; i < XF_LB_IN; ++i) xs[i] = ldexpf(x[i], -e);
+  const float* W1 = w;
+  const float* b1 = W1 + XF_LB_HID * XF_LB_IN;
+  const float* W2 = b1 + XF_LB_HID;
+  const float* b2 = W2 + XF_LB_HID * XF_LB_HID;
+  const float* w3 = b2 + XF_LB_HID;
+#pragma unroll
+  for (int o = 0; o < XF_LB_HID; ++o) {
+    float acc = b1[o];
+#pragma unroll
+    for (int i = 0; i < XF_LB_IN; ++i) acc = acc + W1[o * XF_LB_IN + i] * xs[i];
+    h1[o] = acc > 0.0f ? acc : 0.0f;
+  }
+  float r = w3[XF_LB_HID];
+  for (int o = 0; o < XF_LB_HID; ++o) {
+    float acc = b2[o];
+#pragma unroll
+    for (int i = 0; i < XF_LB_HID; ++i) acc = acc + W2[o * XF_LB_HID + i] * h1[i];
+    const float h2 = acc > 0.0f ? acc : 0.0f;
+    r = r + w3[o] * h2;
+  }
+  return mur + ldexpf(r, e);
+}
+
+struct LbCol {
+  const LbFace* F;
+  int cc;
+  int p, q;
+};
+
+__device__ __forceinline__ LbCol lb_column(const LbDev& L, int64_t col) {
+  int fi = 0;
+  while (fi + 1 < L.nface && col >= L.face[fi + 1].start[0]) ++fi;
+  const LbFace& F = L.face[fi];
+  const int cc = col >= F.start[1] ? 1 : 0;
+  const int64_t local = col - F.start[cc];
+  const int64_t width = F.q1[cc] - L.inset + 1;
+  LbCol r;
+  r.F = &F;
+  r.cc = cc;
+  r.p = L.inset + (int)(local / width);
+  r.q = L.inset + (int)(local % width);
+  return r;
+}
+
+__device__ __forceinline__ int64_t lb_node(const LbDev& L, const Dims& d, const LbCol& c,
+                                           int layer) {
+  int pos[3];
+  const LbFace& F = *c.F;
+  pos[F.c] = F.positive ? L.n[F.c] - L.inset - layer : L.inset + layer;
+  pos[F.a] = c.p;
+  pos[F.b] = c.q;
+  return (int64_t)pos[0] * d.sx + (int64_t)pos[1] * d.sy + pos[2];
+}
+
+__device__ __forceinline__ float* lb_hist(const LbDev& L, const LbCol& c, int par, int layer,
+                                          int p, int q) {
+  const LbFace& F = *c.F;
+  return L.hist + F.hist_off + ((((int64_t)par * 2 + c.cc) * 4 + layer) * F.na1 + p) * F.nb1 + q;
+}
+
+// The face a node of the column also lies on (its component tangential to
+// both), or -1.
+__device__ __forceinline__ int lb_other(const LbDev& L, const LbCol& c) {
+  const LbFace& F = *c.F;
+  if (c.cc == 0) {
+    if (c.q == 0 && ((L.faces >> (2 * F.b)) & 1)) return 2 * F.b;
+    if (c.q == L.n[F.b] && ((L.faces >> (2 * F.b + 1)) & 1)) return 2 * F.b + 1;
+  } else {
+    if (c.p == 0 && ((L.faces >> (2 * F.a)) & 1)) return 2 * F.a;
+    if (c.p == L.n[F.a] && ((L.faces >> (2 * F.a + 1)) & 1)) return 2 * F.a + 1;
+  }
+  return -1;
+}
+
+// phase 0: APPLY the faces' inner nodes / RECORD samples; phase 1: APPLY
+// the edge nodes (owned by the lower face); phase 2: history roll.
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_lb(Fields<S> f, LbDev L, Dims d, Scales<typename Store<S>::C> sc, int phase, int t_arg,
+         const int* t_dev) {
+  using C = typename Store<S>::C;
+  __shared__ float w[XF_LB_NPARAM];
+  if (phase < 2 && L.mode == XF_LB_APPLY) {
+    for (int i = threadIdx.x; i < XF_LB_NPARAM; i += blockDim.x) w[i] = L.w[i];
+    __syncthreads();
+  }
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= L.total) return;
+  const int t = t_dev ? *t_dev : t_arg;
+  const int par = t & 1;
+  const LbCol c = lb_column(L, col);
+  S* fld = f.e[c.cc == 0 ? c.F->a : c.F->b];
+  if (phase == 2) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+      *lb_hist(L, c, par ^ 1, l, c.p, c.q) = (float)Store<S>::ld(fld, lb_node(L, d, c, l), sc.inv_e);
+    return;
+  }
+  bool record = false;
+  if (L.mode == XF_LB_RECORD) {
+    record = (t % L.rec_every) == 0 && t / L.rec_every < L.rec_slots;
+    if (!record || phase != 0) return;
+  } else {
+    const int other = lb_other(L, c);
+    if (phase == 0 ? other >= 0 : !(other >= 0 && c.F->f < other)) return;
+  }
+  float x[XF_LB_IN];
+#pragma unroll
+  for (int l = 1; l <= 3; ++l) x[l - 1] = (float)Store<S>::ld(fld, lb_node(L, d, c, l), sc.inv_e);
+#pragma unroll
+  for (int l = 0; l < 4; ++l) x[3 + l] = *lb_hist(L, c, par, l, c.p, c.q);
+#pragma unroll
+  for (int l = 0; l < 4; ++l) x[7 + l] = *lb_hist(L, c, par ^ 1, l, c.p, c.q);
+  const float u = x[3];
+  const float up = c.p < c.F->p1[c.cc] ? *lb_hist(L, c, par, 0, c.p + 1, c.q) : u;
+  const float um = c.p > L.inset ? *lb_hist(L, c, par, 0, c.p - 1, c.q) : u;
+  const float vp = c.q < c.F->q1[c.cc] ? *lb_hist(L, c, par, 0, c.p, c.q + 1) : u;
+  const float vm = c.q > L.inset ? *lb_hist(L, c, par, 0, c.p, c.q - 1) : u;
+  x[11] = (((up + um) + (vp + vm)) - 4.0f * u);
+  const int64_t node0 = lb_node(L, d, c, 0);
+  if (record) {
+    float* r = L.rec + ((int64_t)(t / L.rec_every) * L.total + col) * 13;
+#pragma unroll
+    for (int i = 0; i < XF_LB_IN; ++i) r[i] = x[i];
+    r[12] = (float)Store<S>::ld(fld, node0, sc.inv_e);
+    return;
+  }
+  Store<S>::st(fld, node0, (C)lb_predict(w, L.mur_k, x), sc.s_e);
+}
+
 // ----------------------------------------------------------- batched forms
 // Item g owns flat indices [start[g], start[g + 1]); start[n] is the total.
 __device__ __forceinline__ int owner_of(const int64_t* __restrict__ start, int n, int64_t x) {

@@ -1,0 +1,396 @@
+// xf_lb_train.inl -- training the learned absorbing layer's column network
+// on the GPU (no reference counterpart).  Included at the end of
+// xf_engine.hip.
+//
+// Data: raw samples (12 inputs + the next layer-0 value, recorded by the
+// engine in XF_LB_RECORD mode) are turned on the device into the network's
+// scaled inputs x^ = x / s and residual targets r^ = (y - mur) / s, with s
+// and mur exactly as the engine's predictor forms them.
+//
+// Gradient: one thread per sample runs the forward pass with the fp16-
+// rounded weights (fp32 arithmetic) and the backward pass of the mean
+// squared error; every one of the 1505 parameter gradients (and the loss) is
+// summed over the wave's 64 samples with an xor butterfly, accumulated per
+// wave in LDS, summed over the block's waves and written as a per-block
+// partial; a second kernel adds the partials in block order.  Every sum has
+// a fixed order, so a gradient is reproducible bit for bit.
+//
+// Optimiser: Adam on fp32 master weights; the forward pass sees them
+// rounded to fp16 (the layer's inference precision), the update is applied
+// to the fp32 copy (straight-through).
+#include <hip/hip_fp16.h>
+
+namespace xf {
+
+constexpr int kLbWaves = 4;  // waves per block of the gradient kernel
+constexpr int kLbAcc = XF_LB_NPARAM + 1;  // gradient + loss
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(256)
+    k_lb_prep(const float* __restrict__ raw, int64_t n, float mur_k, float* __restrict__ xs,
+              float* __restrict__ r) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float* x = raw + k * 13;
+  float m = 0.0f;
+  for (int i = 0; i < XF_LB_IN; ++i) {
+    const float a = fabsf(x[i]);
+    if (a > m) m = a;
+  }
+  const float mur = x[4] + mur_k * (x[0] - x[3]);
+  int e = 0;
+  if (m > 0.0f) (void)frexpf(m, &e);
+  for (int i = 0; i < XF_LB_IN; ++i) xs[k * XF_LB_IN + i] = m > 0.0f ? ldexpf(x[i], -e) : 0.0f;
+  r[k] = m > 0.0f ? ldexpf(x[12] - mur, -e) : 0.0f;
+}
+
+// Counter-based minibatch indices (splitmix64 of seed, step, k).
+__global__ void k_lb_sample(uint32_t* __restrict__ idx, int64_t count, int64_t n, uint64_t seed,
+                            int64_t step) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  uint64_t z = seed + 0x9e3779b97f4a7c15ull * (uint64_t)(step * 1000003 + k + 1);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  idx[k] = (uint32_t)(z % (uint64_t)n);
+}
+
+// Loss and gradient over `count` samples: sample k is idx[k] (or first + k
+// without an index list); the loss is sum d^2 * inv_count.
+__global__ void __launch_bounds__(256)
+    k_lb_grad(const float* __restrict__ xs_all, const float* __restrict__ r_all,
+              const uint32_t* __restrict__ idx, int64_t first, int64_t count, float inv_count,
+              const float* __restrict__ w16, float* __restrict__ partial) {
+  __shared__ float w[XF_LB_NPARAM];
+  __shared__ float acc[kLbWaves][kLbAcc];
+  for (int i = threadIdx.x; i < XF_LB_NPARAM; i += blockDim.x) w[i] = w16[i];
+  for (int i = threadIdx.x; i < kLbWaves * kLbAcc; i += blockDim.x) (&acc[0][0])[i] = 0.0f;
+  __syncthreads();
+  const float* W1 = w;
+  const float* b1 = W1 + XF_LB_HID * XF_LB_IN;
+  const float* W2 = b1 + XF_LB_HID;
+  const float* b2 = W2 + XF_LB_HID * XF_LB_HID;
+  const float* w3 = b2 + XF_LB_HID;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  float* a = acc[wave];
+  const int64_t stride = (int64_t)gridDim.x * kLbWaves * 64;
+  for (int64_t base = ((int64_t)blockIdx.x * kLbWaves + wave) * 64; base < count; base += stride) {
+    const int64_t k = base + lane;
+    const bool valid = k < count;
+    const int64_t sidx = valid ? (idx ? (int64_t)idx[k] : first + k) : 0;
+    float x[XF_LB_IN];
+#pragma unroll
+    for (int i = 0; i < XF_LB_IN; ++i) x[i] = valid ? xs_all[sidx * XF_LB_IN + i] : 0.0f;
+    const float target = valid ? r_all[sidx] : 0.0f;
+    float h1[XF_LB_HID], h2[XF_LB_HID];
+#pragma unroll
+    for (int o = 0; o < XF_LB_HID; ++o) {
+      float z = b1[o];
+#pragma unroll
+      for (int i = 0; i < XF_LB_IN; ++i) z = z + W1[o * XF_LB_IN + i] * x[i];
+      h1[o] = z > 0.0f ? z : 0.0f;
+    }
+    float out = w3[XF_LB_HID];
+#pragma unroll
+    for (int o = 0; o < XF_LB_HID; ++o) {
+      float z = b2[o];
+#pragma unroll
+      for (int i = 0; i < XF_LB_HID; ++i) z = z + W2[o * XF_LB_HID + i] * h1[i];
+      h2[o] = z > 0.0f ? z : 0.0f;
+      out = out + w3[o] * h2[o];
+    }
+    const float d = out - target;
+    const float g_out = valid ? 2.0f * d * inv_count : 0.0f;
+    const float l = valid ? d * d * inv_count : 0.0f;
+    // output layer
+    float s;
+#pragma unroll
+    for (int o = 0; o < XF_LB_HID; ++o) {
+      s = wave_sum(g_out * h2[o]);
+      if (lane == 0) a[XF_LB_HID * XF_LB_IN + XF_LB_HID + XF_LB_HID * XF_LB_HID + XF_LB_HID + o] += s;
+    }
+    s = wave_sum(g_out);
+    if (lane == 0) a[XF_LB_NPARAM - 1] += s;
+    s = wave_sum(l);
+    if (lane == 0) a[XF_LB_NPARAM] += s;
+    // second hidden layer
+    float gz2[XF_LB_HID];
+#pragma unroll
+    for (int o = 0; o < XF_LB_HID; ++o) gz2[o] = h2[o] > 0.0f ? g_out * w3[o] : 0.0f;
+    const int oW2 = XF_LB_HID * XF_LB_IN + XF_LB_HID;
+    const int ob2 = oW2 + XF_LB_HID * XF_LB_HID;
+#pragma unroll
+    for (int o = 0; o < XF_LB_HID; ++o) {
+#pragma unroll
+      for (int i = 0; i < XF_LB_HID; ++i) {
+        s = wave_sum(gz2[o] * h1[i]);
+        if (lane == 0) a[oW2 + o * XF_LB_HID + i] += s;
+      }
+      s = wave_sum(gz2[o]);
+      if (lane == 0) a[ob2 + o] += s;
+    }
+    // first hidden layer
+    const int ob1 = XF_LB_HID * XF_LB_IN;
+#pragma unroll
+    for (int i = 0; i < XF_LB_HID; ++i) {
+      float g = 0.0f;
+#pragma unroll
+      for (int o = 0; o < XF_LB_HID; ++o) g = g + gz2[o] * W2[o * XF_LB_HID + i];
+      const float gz1 = h1[i] > 0.0f ? g : 0.0f;
+#pragma unroll
+      for (int j = 0; j < XF_LB_IN; ++j) {
+        s = wave_sum(gz1 * x[j]);
+        if (lane == 0) a[i * XF_LB_IN + j] += s;
+      }
+      s = wave_sum(gz1);
+      if (lane == 0) a[ob1 + i] += s;
+    }
+  }
+  __syncthreads();
+  for (int p = threadIdx.x; p < kLbAcc; p += blockDim.x) {
+    float t = acc[0][p];
+#pragma unroll
+    for (int q = 1; q < kLbWaves; ++q) t += acc[q][p];
+    partial[(int64_t)blockIdx.x * kLbAcc + p] = t;
+  }
+}
+
+__global__ void k_lb_reduce(const float* __restrict__ partial, int nblocks, float* __restrict__ grad,
+                            double* __restrict__ loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kLbAcc) return;
+  double t = 0.0;
+  for (int b = 0; b < nblocks; ++b) t += (double)partial[(int64_t)b * kLbAcc + p];
+  if (p < XF_LB_NPARAM)
+    grad[p] = (float)t;
+  else
+    *loss = t;
+}
+
+__global__ void k_lb_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                          const float* __restrict__ g, float* __restrict__ w16, int64_t step, float lr,
+                          float b1, float b2, float eps) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= XF_LB_NPARAM) return;
+  const float gp = g[p];
+  const float mp = b1 * m[p] + (1.0f - b1) * gp;
+  const float vp = b2 * v[p] + (1.0f - b2) * gp * gp;
+  m[p] = mp;
+  v[p] = vp;
+  const float c1 = 1.0f - powf(b1, (float)step);
+  const float c2 = 1.0f - powf(b2, (float)step);
+  const float wp = w[p] - lr * (mp / c1) / (sqrtf(vp / c2) + eps);
+  w[p] = wp;
+  w16[p] = __half2float(__float2half_rn(wp));
+}
+
+__global__ void k_lb_round(const float* __restrict__ w, float* __restrict__ w16,
+                           uint16_t* __restrict__ bits) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= XF_LB_NPARAM) return;
+  const __half h = __float2half_rn(w[p]);
+  w16[p] = __half2float(h);
+  if (bits) bits[p] = __half_as_ushort(h);
+}
+
+struct LbTrainer {
+  int device{0};
+  hipStream_t stream{};
+  std::vector<void*> allocs;
+  int64_t n{0};
+  float mur_k{0};
+  float* xs{};
+  float* r{};
+  float* w{};     // fp32 master
+  float* w16{};   // fp16-rounded copy used by the forward pass
+  uint16_t* bits{};
+  float* m{};
+  float* v{};
+  float* g{};
+  double* loss{};
+  float* partial{};
+  uint32_t* idx{};
+  int64_t idx_cap{0};
+  int nblocks{0};
+  int64_t step{0};
+
+  ~LbTrainer() {
+    for (void* p : allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int alloc(void** p, size_t bytes) {
+    *p = nullptr;
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) return fail(XF_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(*p);
+    return XF_OK;
+  }
+
+  int create(const float* samples, int64_t count, double k, int dev) {
+    device = dev;
+    n = count;
+    mur_k = static_cast<float>(k);
+    XF_HIP(hipSetDevice(device));
+    XF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    void* p;
+    XF_TRY(alloc(&p, static_cast<size_t>(n) * 13 * sizeof(float)));
+    float* raw = static_cast<float*>(p);
+    XF_HIP(hipMemcpy(raw, samples, static_cast<size_t>(n) * 13 * sizeof(float), hipMemcpyHostToDevice));
+    XF_TRY(alloc(&p, static_cast<size_t>(n) * XF_LB_IN * sizeof(float)));
+    xs = static_cast<float*>(p);
+    XF_TRY(alloc(&p, static_cast<size_t>(n) * sizeof(float)));
+    r = static_cast<float*>(p);
+    hipLaunchKernelGGL(k_lb_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, raw, n, mur_k, xs, r);
+    XF_HIP(hipGetLastError());
+    float** vecs[5] = {&w, &w16, &m, &v, &g};
+    for (float** q : vecs) {
+      XF_TRY(alloc(&p, XF_LB_NPARAM * sizeof(float)));
+      *q = static_cast<float*>(p);
+      XF_HIP(hipMemsetAsync(*q, 0, XF_LB_NPARAM * sizeof(float), stream));
+    }
+    XF_TRY(alloc(&p, XF_LB_NPARAM * sizeof(uint16_t)));
+    bits = static_cast<uint16_t*>(p);
+    XF_TRY(alloc(&p, sizeof(double)));
+    loss = static_cast<double*>(p);
+    hipDeviceProp_t prop;
+    XF_HIP(hipGetDeviceProperties(&prop, device));
+    nblocks = 2 * prop.multiProcessorCount;
+    XF_TRY(alloc(&p, static_cast<size_t>(nblocks) * kLbAcc * sizeof(float)));
+    partial = static_cast<float*>(p);
+    XF_HIP(hipStreamSynchronize(stream));
+    return XF_OK;
+  }
+
+  int set_weights(const float* host) {
+    XF_HIP(hipMemcpy(w, host, XF_LB_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_lb_round, dim3((XF_LB_NPARAM + 255) / 256), dim3(256), 0, stream, w, w16,
+                       static_cast<uint16_t*>(nullptr));
+    XF_HIP(hipMemsetAsync(m, 0, XF_LB_NPARAM * sizeof(float), stream));
+    XF_HIP(hipMemsetAsync(v, 0, XF_LB_NPARAM * sizeof(float), stream));
+    step = 0;
+    XF_HIP(hipStreamSynchronize(stream));
+    return XF_OK;
+  }
+
+  int get_weights(float* w32_host, uint16_t* bits_host) {
+    hipLaunchKernelGGL(k_lb_round, dim3((XF_LB_NPARAM + 255) / 256), dim3(256), 0, stream, w, w16, bits);
+    XF_HIP(hipStreamSynchronize(stream));
+    if (w32_host) XF_HIP(hipMemcpy(w32_host, w, XF_LB_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+    if (bits_host) XF_HIP(hipMemcpy(bits_host, bits, XF_LB_NPARAM * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    return XF_OK;
+  }
+
+  // loss and gradient over idx[0..count) (or samples [first, first+count))
+  int launch_grad(const uint32_t* ix, int64_t first, int64_t count) {
+    const int blocks = static_cast<int>(std::min<int64_t>(nblocks, (count + kLbWaves * 64 - 1) / (kLbWaves * 64)));
+    hipLaunchKernelGGL(k_lb_grad, dim3(blocks), dim3(64 * kLbWaves), 0, stream, xs, r, ix, first, count,
+                       1.0f / static_cast<float>(count), w16, partial);
+    hipLaunchKernelGGL(k_lb_reduce, dim3((kLbAcc + 255) / 256), dim3(256), 0, stream, partial, blocks, g, loss);
+    XF_HIP(hipGetLastError());
+    return XF_OK;
+  }
+
+  int grad(int64_t first, int64_t count, double* loss_host, float* grad_host) {
+    if (first < 0 || count < 1 || first + count > n) return fail(XF_ERR_INVALID, "sample range out of bounds");
+    XF_TRY(launch_grad(nullptr, first, count));
+    XF_HIP(hipStreamSynchronize(stream));
+    if (loss_host) XF_HIP(hipMemcpy(loss_host, loss, sizeof(double), hipMemcpyDeviceToHost));
+    if (grad_host) XF_HIP(hipMemcpy(grad_host, g, XF_LB_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+    return XF_OK;
+  }
+
+  int train(int steps, int batch, float lr, uint64_t seed, double* last_loss) {
+    if (steps < 0 || batch < 1) return fail(XF_ERR_INVALID, "bad training schedule");
+    if (batch > idx_cap) {
+      void* p;
+      XF_TRY(alloc(&p, static_cast<size_t>(batch) * sizeof(uint32_t)));
+      idx = static_cast<uint32_t*>(p);
+      idx_cap = batch;
+    }
+    for (int s = 0; s < steps; ++s) {
+      ++step;
+      hipLaunchKernelGGL(k_lb_sample, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, stream, idx,
+                         static_cast<int64_t>(batch), n, seed, step);
+      XF_TRY(launch_grad(idx, 0, batch));
+      hipLaunchKernelGGL(k_lb_adam, dim3((XF_LB_NPARAM + 255) / 256), dim3(256), 0, stream, w, m, v, g, w16,
+                         step, lr, 0.9f, 0.999f, 1e-8f);
+      XF_HIP(hipGetLastError());
+    }
+    XF_HIP(hipStreamSynchronize(stream));
+    if (last_loss) XF_HIP(hipMemcpy(last_loss, loss, sizeof(double), hipMemcpyDeviceToHost));
+    return XF_OK;
+  }
+};
+
+}  // namespace xf
+
+struct xf_lb_trainer {
+  xf::LbTrainer t;
+};
+
+extern "C" {
+
+int xf_lb_trainer_create(const float* samples, int64_t n, double mur_k, int32_t device,
+                         xf_lb_trainer** out) {
+  xf::g_err.clear();
+  if (!samples || !out || n < 1) return xf::fail(XF_ERR_INVALID, "no samples");
+  if (n > (int64_t)UINT32_MAX) return xf::fail(XF_ERR_UNSUPPORTED, "too many samples (32-bit indices)");
+  *out = nullptr;
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) return xf::fail(XF_ERR_HIP, "no HIP device");
+  if (device < 0 || device >= nd) return xf::fail(XF_ERR_INVALID, "bad device ordinal");
+  auto h = new (std::nothrow) xf_lb_trainer;
+  if (!h) return xf::fail(XF_ERR_NOMEM, "out of host memory");
+  const int st = h->t.create(samples, n, mur_k, device);
+  if (st != XF_OK) {
+    delete h;
+    return st;
+  }
+  *out = h;
+  return XF_OK;
+}
+
+int xf_lb_trainer_set_weights(xf_lb_trainer* h, const float* w) {
+  xf::g_err.clear();
+  if (!h || !w) return xf::fail(XF_ERR_INVALID, "null argument");
+  return h->t.set_weights(w);
+}
+
+int xf_lb_trainer_get_weights(xf_lb_trainer* h, float* w32, uint16_t* w16) {
+  xf::g_err.clear();
+  if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
+  return h->t.get_weights(w32, w16);
+}
+
+int xf_lb_trainer_grad(xf_lb_trainer* h, int64_t first, int64_t count, double* loss, float* grad) {
+  xf::g_err.clear();
+  if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
+  return h->t.grad(first, count, loss, grad);
+}
+
+int xf_lb_trainer_train(xf_lb_trainer* h, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                        double* last_loss) {
+  xf::g_err.clear();
+  if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
+  return h->t.train(steps, batch, lr, seed, last_loss);
+}
+
+int xf_lb_trainer_samples(const xf_lb_trainer* h, int64_t* n) {
+  if (!h || !n) return xf::fail(XF_ERR_INVALID, "null argument");
+  *n = h->t.n;
+  return XF_OK;
+}
+
+int xf_lb_trainer_destroy(xf_lb_trainer* h) {
+  delete h;
+  return XF_OK;
+}
+
+}  // extern "C"

@@ -11,6 +11,9 @@ cylinder_2d         test/cylinder_scatter_2d.cu:28-183 in the 1000 x 1000,
 large_sphere        the dielectric sphere scaled to n^3 cells (+ pml), sphere
                     radius 0.35 * edge, eps_r ~ U[2, 6] from a seed.
 batch_spheres       64 sphere scatterers of uneven sizes for a batched run.
+point_source_box    vacuum cube with seeded point sources and a chosen boundary
+                    (CPML, learned layer, Mur, PEC): the learned layer's data
+                    and its judge.
 dispersive_sphere   Drude core + Lorentz shell, n^3 cells + 10-cell CPML.
 
 Insets the reference does not fix (2-D NF2FF/TFSF, scaled boxes) are chosen
@@ -126,6 +129,52 @@ def batch_spheres(count: int = 64, seed: int = 7, edge_range=(48, 160), pml: int
     return sims
 
 
+LB_NPARAM = 12 * 32 + 32 + 32 * 32 + 32 + 32 + 1  # XF_LB_NPARAM
+
+
+def point_source_box(n: int = 40, boundary: str = "cpml", size: int = 16, steps: int = 400,
+                     seed: int = 0, n_sources: int = 3, dl: float = 5e-3, weights=None,
+                     record=None, init: bool = True) -> Simulation:
+    """Vacuum cube of n^3 cells with seeded point sources, bounded by
+    boundary = "cpml" (size-cell CPML), "learned" (size-cell learned layer
+    with `weights`), "mur" (the learned layer with a zero network: first-order
+    Mur) or "pec" (nothing).  Sources (seeded by `seed`, the same physical
+    positions whatever the boundary): random E component, position in the
+    central half of the cube, cosine-modulated Gaussian pulses with carrier
+    f ~ U[2, 5] GHz (at least 12 cells per wavelength at dl = 5 mm), envelope
+    tau = 1 / f (zero mean to 5e-5), delay 3 tau + U[0, 0.3] ns, amplitude
+    +-U[0.5, 1].  record = (inset, every, slots)
+    adds a training-sample recorder on the plane `inset` nodes inside each
+    face."""
+    rng = np.random.default_rng(seed)
+    s = Simulation(dl, dl, dl, 0.9)
+    s.add_cube("domain", (0.0, 0.0, 0.0), (n * dl,) * 3, Material.air())
+    margin = 0
+    if boundary == "cpml":
+        s.add_pml_all(size)
+        margin = size
+    elif boundary in ("learned", "mur"):
+        w = np.zeros(LB_NPARAM, np.float16) if boundary == "mur" else weights
+        s.add_learned_boundary(w, thickness=size)
+        margin = size
+    elif boundary != "pec":
+        raise ValueError(f"unknown boundary {boundary!r}")
+    for _ in range(n_sources):
+        comp = int(rng.integers(0, 3))
+        pos = [margin + n // 4 + int(rng.integers(0, n // 2)) for _ in range(3)]
+        f = float(rng.uniform(2e9, 5e9))
+        tau = 1.0 / f
+        t0 = 3 * tau + float(rng.uniform(0, 0.3e-9))
+        amp = float(rng.choice([-1.0, 1.0]) * rng.uniform(0.5, 1.0))
+        s.add_point_source(comp, pos, Waveform.cosine_modulated_gaussian(tau, t0, f), amp)
+    if record is not None:
+        inset, every, slots = record
+        s.add_boundary_recorder(inset, every, slots)
+    if init:
+        s.init(steps)
+    return s
+
+
 def dispersive_sphere(n: int = 256, pml: int = 10, steps: int = 6000, dl: float = 2.5e-3,
                       freqs=(0.5e9, 1e9, 2e9), init: bool = True) -> Simulation:
     """Drude core (omega_p = 2 pi 2 GHz, gamma = 2 pi 0.2 GHz) inside a
@@ -148,5 +197,5 @@ def dispersive_sphere(n: int = 256, pml: int = 10, steps: int = 6000, dl: float 
     return s
 
 
-__all__ = ["FACES", "batch_spheres", "cylinder_2d", "dielectric_sphere", "dispersive_sphere",
-           "large_sphere", "vacuum_benchmark"]
+__all__ = ["FACES", "LB_NPARAM", "batch_spheres", "cylinder_2d", "dielectric_sphere",
+           "dispersive_sphere", "large_sphere", "point_source_box", "vacuum_benchmark"]

@@ -131,6 +131,13 @@ class Waveform:
         return np.sin(2 * np.pi * self.freq * t)
 
 
+def _face_mask(faces) -> int:
+    m = 0
+    for f in faces:
+        m |= 1 << FACES.index(f)
+    return m
+
+
 def _d3(v):
     return (ctypes.c_double * 3)(*[float(x) for x in v])
 
@@ -212,6 +219,49 @@ class Simulation:
         _native.check_host(self._lib.xfh_sim_add_nf2ff_fd(self._h, _i3(start), f, len(freqs)),
                            "addNF2FF")
         return self
+
+    def add_point_source(self, comp: int, index, waveform: Waveform, amplitude: float = 1.0):
+        """Soft source on E_comp (0 x, 1 y, 2 z) at grid index `index` of that
+        component's array (no counterpart in the reference's GPU path)."""
+        _native.check_host(self._lib.xfh_sim_add_point_source(
+            self._h, comp, _i3(index), waveform.kind, waveform.tau, waveform.t0, waveform.freq,
+            amplitude), "addPointSource")
+        return self
+
+    def add_learned_boundary(self, weights, faces=FACES, thickness: int = 2):
+        """The learned absorbing layer: `thickness` cells outside the domain on
+        each face, outer node driven by the column MLP (XF_LB_NPARAM fp16
+        weights as uint16 bits or a float array rounded to fp16)."""
+        w = np.asarray(weights)
+        if w.dtype != np.uint16:
+            w = w.astype(np.float16).view(np.uint16)
+        w = np.ascontiguousarray(w)
+        self._lb_weights = w
+        _native.check_host(self._lib.xfh_sim_add_learned_boundary(
+            self._h, 1, _face_mask(faces), thickness,
+            w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)), w.size, 0, 0),
+            "addLearnedBoundary")
+        return self
+
+    def add_boundary_recorder(self, inset: int, every: int, slots: int, faces=FACES):
+        """Records learned-layer training samples on the plane `inset` nodes
+        inside each face (no field is changed)."""
+        _native.check_host(self._lib.xfh_sim_add_learned_boundary(
+            self._h, 2, _face_mask(faces), inset, None, 0, every, slots), "addBoundaryRecorder")
+        return self
+
+    def lb_samples(self) -> np.ndarray:
+        """Recorded samples, (recorded steps, columns, 13) float32 (a copy)."""
+        p = ctypes.POINTER(ctypes.c_float)()
+        cols = ctypes.c_int64()
+        rec = ctypes.c_int64()
+        _native.check_host(self._lib.xfh_sim_lb_samples(self._h, ctypes.byref(p),
+                                                        ctypes.byref(cols), ctypes.byref(rec)),
+                           "lb_samples")
+        n = rec.value * cols.value * 13
+        if n == 0:
+            return np.zeros((0, cols.value, 13), np.float32)
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(rec.value, cols.value, 13).copy()
 
     def set_coefficient_layout(self, layout: str):
         _native.check_host(self._lib.xfh_sim_set_coefficient_layout(
