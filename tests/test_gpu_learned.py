@@ -71,6 +71,17 @@ def test_learned_layer_record_matches_oracle(gpu_available):
         assert np.array_equal(g.array(n), c.array(n)), n
 
 
+def test_point_source_table_guard(gpu_available):
+    """Running past the point sources' value table is refused at the boundary
+    (no kernel reads beyond it)."""
+    s = scenes.point_source_box(n=10, boundary="pec", steps=5, seed=0, n_sources=1)
+    e = xf.Engine(s).upload()
+    e.run(5)
+    with pytest.raises(xf.XFError, match="run past"):
+        e.run(1)
+    e.close()
+
+
 def _samples():
     s = scenes.point_source_box(n=16, boundary="cpml", size=8, steps=120, seed=9,
                                 record=(10, 3, 40))

@@ -493,7 +493,7 @@ int Batch::autotune(int trial_steps, xf_tuning* best) {
 int Engine::autotune(int trial_steps, xf_tuning* best) {
   if (!uploaded) return fail(XF_ERR_STATE, "autotune before upload");
   if (trial_steps < 1) return fail(XF_ERR_INVALID, "trial_steps must be >= 1");
-  if ((sc->tfsf || !nf.empty()) && cur_step + trial_steps + 1 > nt)
+  if ((sc->tfsf || !nf.empty() || pts.n > 0) && cur_step + trial_steps + 1 > nt)
     return fail(XF_ERR_INVALID, "not enough time-table steps left to tune");
   XF_TRY(snapshot());
   const int64_t step0 = cur_step;

@@ -1563,8 +1563,8 @@ int Engine::run(int64_t n) {
   if (!uploaded) return fail(XF_ERR_STATE, "run before upload");
   if (n < 0) return fail(XF_ERR_INVALID, "negative step count");
   if (n == 0) return XF_OK;
-  if ((sc->tfsf || !nf.empty()) && cur_step + n > nt)
-    return fail(XF_ERR_INVALID, "run past the end of the time axis the TFSF/NF2FF tables were built for");
+  if ((sc->tfsf || !nf.empty() || pts.n > 0) && cur_step + n > nt)
+    return fail(XF_ERR_INVALID, "run past the end of the time axis the TFSF/NF2FF/source tables were built for");
   XF_HIP(hipSetDevice(device));
   const bool timing_on = plan.timing != 0;
   const bool graph_on = plan.use_graph != 0;
