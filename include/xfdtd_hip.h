@@ -136,6 +136,21 @@ typedef struct xf_ade {
   xf_array jx, jy, jz, jx_prev, jy_prev, jz_prev;   /* state */
 } xf_ade;
 
+/* A movie monitor: reference MovieMonitor<F>::update / nextCount
+ * (src/monitor/movie_monitor.cu:17-45, 110-120).  In the record phase of
+ * step t (after the E corrections, before the NF2FF accumulation,
+ * domain_hd.cu:65-77), when t % interval == 0, the component's values over
+ * `task` (indices of that component's array) are copied into frame
+ * t / interval of `data`, shape (frames, task extents); steps beyond the
+ * last frame record nothing.  field: 0..2 = Ex, Ey, Ez (the reference's
+ * EX/EY/EZ agencies); 3..5 = Hx, Hy, Hz are an addition. */
+typedef struct xf_movie {
+  int32_t field;
+  xf_task task;
+  int64_t interval;
+  xf_array data; /* (frames, nx, ny, nz), state */
+} xf_movie;
+
 /* ---- Point sources (no counterpart in the reference's GPU path).  After
  * the E pass of step t and its TFSF/CPML corrections, E_comp at index
  * (i, j, k) of that component's array gains value.data[t]. */
@@ -165,7 +180,8 @@ typedef struct xf_point_source {
  * XF_LB_APPLY (inset 0): the layer-0 nodes of the outer boundary, which the
  * Yee pass never updates, take that value -- a 2-cell layer in place of a
  * CPML.  Nodes on two faces (edges) go to the lower face index and are
- * updated after the others, from their final neighbours.
+ * updated after the others, from their final neighbours, with the Mur
+ * term alone (the network is trained on face interiors).
  * XF_LB_RECORD (inset > 0): nothing is written to the fields; every
  * rec_every steps the 12 inputs and the actual layer-0 u^{t+1} of every
  * column are stored as 13 floats (training data from a thick-CPML run).
@@ -227,6 +243,8 @@ typedef struct xf_scene {
   int32_t n_point;
   xf_point_source* point;  /* additions: soft point sources ... */
   xf_learned_boundary* lb; /* ... and the learned layer (NULL: none) */
+  int32_t n_movie;
+  xf_movie* movie; /* movie monitors (MovieMonitorHD, simulation_hd.cu:162-207) */
 } xf_scene;
 
 /* Launch plan.  grid_dim/block_dim are the reference's setGridDim/
@@ -383,6 +401,10 @@ int xf_lb_trainer_grad(xf_lb_trainer* t, int64_t first, int64_t count, double* l
 int xf_lb_trainer_train(xf_lb_trainer* t, int32_t steps, int32_t batch, float lr, uint64_t seed,
                         double* last_loss);
 int xf_lb_trainer_samples(const xf_lb_trainer* t, int64_t* n);
+/* Loss weight of each sample (s / s_max)^power, s its input scale (power 0:
+ * every sample counts alike -- the default; 2: the squared error in field
+ * units, so strong wavefronts dominate). */
+int xf_lb_trainer_set_weighting(xf_lb_trainer* t, int32_t power);
 int xf_lb_trainer_destroy(xf_lb_trainer* t);
 
 #ifdef __cplusplus

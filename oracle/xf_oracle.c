@@ -522,6 +522,28 @@ static void nf2ff_face(const xf_scene* s, const xf_nf2ff* nf,
   }
 }
 
+/* ------------------------------------------------------- movie monitor */
+
+/* MovieMonitor<F>::update (movie_monitor.cu:17-45): every `interval` steps
+ * the component over the monitor's box goes to frame t / interval. */
+static void movie_record(xf_scene* s, int64_t t) {
+  for (int32_t q = 0; q < s->n_movie; ++q) {
+    xf_movie* m = &s->movie[q];
+    if (t % m->interval != 0) continue;
+    const int64_t frame = t / m->interval;
+    if (frame >= m->data.shape[0]) continue;
+    const xf_array* f = m->field < 3 ? efield(s, m->field) : hfield(s, m->field - 3);
+    const xf_task* k = &m->task;
+    const int64_t ny = k->y.end - k->y.start, nz = k->z.end - k->z.start;
+    const int64_t nx = k->x.end - k->x.start;
+    double* out = m->data.data + frame * nx * ny * nz;
+    for (int64_t i = k->x.start; i < k->x.end; ++i)
+      for (int64_t j = k->y.start; j < k->y.end; ++j)
+        for (int64_t kk = k->z.start; kk < k->z.end; ++kk)
+          out[((i - k->x.start) * ny + (j - k->y.start)) * nz + (kk - k->z.start)] = *at3(f, i, j, kk);
+  }
+}
+
 /* ------------------------------------------------------- point sources */
 
 /* Soft point sources: E_comp(i, j, k) += value[t], in registration order. */
@@ -706,7 +728,10 @@ static void lb_step(xf_scene* s, int64_t t) {
             const int other = lb_other_face(s, &g, cc, p, q);
             if (phase == 0 ? other >= 0 : !(other >= 0 && f < other)) continue;
             lb_inputs(s, &g, cc, inset, p, q, p1, q1, par, x);
-            *lb_field(s, &g, cc, inset, 0, p, q) = (double)lb_predict(w, k, x);
+            /* edge nodes (on two faces) take the first-order Mur update: the
+             * network is trained on face interiors only */
+            *lb_field(s, &g, cc, inset, 0, p, q) =
+                phase == 0 ? (double)lb_predict(w, k, x) : (double)(x[4] + k * (x[0] - x[3]));
           }
       }
     }
@@ -767,7 +792,8 @@ int xfo_run(xf_scene* s, int64_t n_steps) {
     /* additions without a reference counterpart */
     point_sources(s, t);
     if (s->lb && s->lb->mode != XF_LB_OFF) lb_step(s, t);
-    /* record (domain_hd.cu:65-77) */
+    /* record (domain_hd.cu:65-77): monitors, then NF2FF */
+    movie_record(s, t);
     for (int32_t q = 0; q < s->n_nf2ff; ++q) {
       const xf_nf2ff* nf = &s->nf2ff[q];
       for (int32_t f = 0; f < nf->n_freq; ++f)

@@ -123,8 +123,9 @@ int Batch::create(xf_scene* const* scenes, int n, const xf_plan* pl) {
     Engine& E = h->eng;
     if (E.dim != 3) return fail(XF_ERR_UNSUPPORTED, "scene " + std::to_string(g) + ": batches take 3-D grids only");
     if (E.has_ade) return fail(XF_ERR_UNSUPPORTED, "scene " + std::to_string(g) + ": ADE media are not batched");
-    if (E.has_lb || E.sc->n_point > 0)
-      return fail(XF_ERR_UNSUPPORTED, "scene " + std::to_string(g) + ": point sources and learned layers are not batched");
+    if (E.has_lb || E.sc->n_point > 0 || E.sc->n_movie > 0)
+      return fail(XF_ERR_UNSUPPORTED,
+                  "scene " + std::to_string(g) + ": point sources, learned layers and movies are not batched");
     if (g == 0) cur_step = E.cur_step;
     if (E.cur_step != cur_step) return fail(XF_ERR_INVALID, "scenes of a batch must share the current step");
     if (E.sc->tfsf || E.sc->n_nf2ff > 0) nt = std::min(nt, E.nt);

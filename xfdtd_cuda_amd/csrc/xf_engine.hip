@@ -137,6 +137,7 @@ struct Engine {
   AdeDev ade{};
   // point sources and the learned layer (additions)
   PointDev pts{};
+  std::vector<MovieDev> movies;
   LbDev lbd{};
   bool has_lb{false};
   int64_t lb_rec_count{0}, lb_rec_count_saved{0};
@@ -413,6 +414,22 @@ int Engine::validate_additions() {
       return fail(XF_ERR_INVALID, "point source " + std::to_string(q) + " outside its component");
     if (!P.value.data || P.value.ndim != 1 || P.value.shape[0] < nt)
       return fail(XF_ERR_INVALID, "point source " + std::to_string(q) + ": value table shorter than the run");
+  }
+  if (sc->n_movie < 0 || (sc->n_movie > 0 && !sc->movie)) return fail(XF_ERR_INVALID, "movie monitors missing");
+  for (int32_t q = 0; q < sc->n_movie; ++q) {
+    const xf_movie& M = sc->movie[q];
+    const std::string nm = "movie " + std::to_string(q);
+    if (M.field < 0 || M.field > 5) return fail(XF_ERR_INVALID, nm + ": bad field");
+    if (M.interval < 1) return fail(XF_ERR_INVALID, nm + ": interval must be >= 1");
+    const auto sh = field_shape(M.field < 3 ? 0 : 1, M.field % 3, nx, ny, nz);
+    const xf_range* r[3] = {&M.task.x, &M.task.y, &M.task.z};
+    for (int d = 0; d < 3; ++d)
+      if (r[d]->start < 0 || r[d]->end > sh[d] || r[d]->start >= r[d]->end)
+        return fail(XF_ERR_INVALID, nm + ": box outside the component's array");
+    if (!M.data.data || M.data.ndim != 4 || M.data.shape[0] < 1 ||
+        M.data.shape[1] != M.task.x.end - M.task.x.start || M.data.shape[2] != M.task.y.end - M.task.y.start ||
+        M.data.shape[3] != M.task.z.end - M.task.z.start)
+      return fail(XF_ERR_INVALID, nm + ": data must be (frames, box extents)");
   }
   if (!sc->lb || sc->lb->mode == XF_LB_OFF) return XF_OK;
   const xf_learned_boundary& L = *sc->lb;
@@ -1027,6 +1044,26 @@ int Engine::upload_additions() {
     pts.idx = static_cast<const int64_t*>(pi);
     pts.value = static_cast<const double*>(pv);
   }
+  movies.clear();
+  for (int32_t q = 0; q < sc->n_movie; ++q) {
+    const xf_movie& M = sc->movie[q];
+    MovieDev D{};
+    D.field = M.field;
+    const xf_range* r[3] = {&M.task.x, &M.task.y, &M.task.z};
+    for (int d = 0; d < 3; ++d) {
+      D.o[d] = static_cast<int>(r[d]->start);
+      D.n[d] = static_cast<int>(r[d]->end - r[d]->start);
+    }
+    D.interval = M.interval;
+    D.frames = M.data.shape[0];
+    D.box = static_cast<int64_t>(D.n[0]) * D.n[1] * D.n[2];
+    void* p;
+    XF_TRY(alloc(&p, static_cast<size_t>(D.frames * D.box) * sizeof(double)));
+    XF_HIP(hipMemcpy(p, M.data.data, static_cast<size_t>(D.frames * D.box) * sizeof(double), hipMemcpyHostToDevice));
+    D.data = static_cast<double*>(p);
+    state.emplace_back(p, static_cast<size_t>(D.frames * D.box) * sizeof(double));
+    movies.push_back(D);
+  }
   lbd = LbDev{};
   if (!has_lb) return XF_OK;
   const xf_learned_boundary& L = *sc->lb;
@@ -1242,6 +1279,9 @@ int Engine::download() {
       }
     }
   }
+  for (size_t q = 0; q < movies.size(); ++q)
+    XF_HIP(hipMemcpy(sc->movie[q].data.data, movies[q].data,
+                     static_cast<size_t>(movies[q].frames * movies[q].box) * sizeof(double), hipMemcpyDeviceToHost));
   if (has_lb) {
     xf_learned_boundary& L = *sc->lb;
     std::vector<float> hist(L.history.shape[0]);
@@ -1530,6 +1570,9 @@ int Engine::launch_post_e(int t, const int* tp) {
         hipLaunchKernelGGL((k_lb<S>), g, b, 0, stream, make_fields<S>(*this), lbd, make_dims(*this),
                            make_scales<C>(*this), ph, t, tp);
   }
+  for (const MovieDev& M : movies)
+    hipLaunchKernelGGL((k_movie<S>), dim3((unsigned)((M.box + 255) / 256)), dim3(256), 0, stream,
+                       make_fields<S>(*this), M, make_dims(*this), dim == 2 ? 1 : 0, make_scales<C>(*this), t, tp);
   for (auto& N : nf) {
     if (N.pass.total > 0)
       hipLaunchKernelGGL((k_nf2ff<S>), dim3((unsigned)((N.pass.total + 255) / 256)), dim3(256), 0, stream,

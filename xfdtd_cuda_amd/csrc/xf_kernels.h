@@ -70,6 +70,27 @@ struct Store<__hip_bfloat16> {
   }
 };
 
+// Loads and stores at a small unsigned element offset from a wave-uniform
+// base pointer: the offset is zero-extended from 32 bits, which lets the
+// compiler address with the SGPR base + 32-bit VGPR offset form of the
+// global instructions (no per-lane 64-bit address arithmetic).
+template <typename S>
+__device__ __forceinline__ typename Store<S>::C ld32(const S* p, uint32_t i, typename Store<S>::C inv) {
+  return Store<S>::ld(p, (int64_t)(uint64_t)i, inv);
+}
+template <typename S>
+__device__ __forceinline__ typename Store<S>::C ld32_nt(const S* p, uint32_t i, typename Store<S>::C inv) {
+  return Store<S>::ld_nt(p, (int64_t)(uint64_t)i, inv);
+}
+template <typename S>
+__device__ __forceinline__ void st32(S* p, uint32_t i, typename Store<S>::C v, typename Store<S>::C scale) {
+  Store<S>::st(p, (int64_t)(uint64_t)i, v, scale);
+}
+template <typename S>
+__device__ __forceinline__ void st32_nt(S* p, uint32_t i, typename Store<S>::C v, typename Store<S>::C scale) {
+  Store<S>::st_nt(p, (int64_t)(uint64_t)i, v, scale);
+}
+
 template <typename S>
 struct Fields {
   S* e[3];
@@ -203,6 +224,14 @@ struct PointDev {
   const int* comp;
   const int64_t* idx;
   const double* value;
+};
+
+// A movie monitor's box and its frame buffer (doubles, frames x box).
+struct MovieDev {
+  int field;           // 0..2 E, 3..5 H
+  int o[3], n[3];      // box origin and extent
+  int64_t interval, frames, box;
+  double* data;
 };
 
 // The learned layer (xf_learned_boundary): the faces in the mask, in face

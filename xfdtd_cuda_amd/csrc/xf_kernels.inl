@@ -369,6 +369,28 @@ __global__ void k_point_sources(Fields<S> f, PointDev P, Scales<typename Store<S
   }
 }
 
+// ------------------------------------------------------------- movies
+// MovieMonitor<F>::update (movie_monitor.cu:17-45): every `interval`
+// steps the component over the box goes to frame t / interval.
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_movie(Fields<S> f, MovieDev M, Dims d, int two_d, Scales<typename Store<S>::C> sc, int t_arg,
+            const int* t_dev) {
+  const int t = t_dev ? *t_dev : t_arg;
+  if (t % M.interval != 0) return;
+  const int64_t frame = t / M.interval;
+  if (frame >= M.frames) return;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= M.box) return;
+  const int k = M.o[2] + (int)(c % M.n[2]);
+  const int j = M.o[1] + (int)((c / M.n[2]) % M.n[1]);
+  const int i = M.o[0] + (int)(c / ((int64_t)M.n[2] * M.n[1]));
+  const S* p = M.field < 3 ? f.e[M.field] : f.h[M.field - 3];
+  const int64_t idx = two_d ? (int64_t)i * d.sx + j : (int64_t)i * d.sx + (int64_t)j * d.sy + k;
+  const double v = p ? (double)Store<S>::ld(p, idx, M.field < 3 ? sc.inv_e : sc.inv_h) : 0.0;
+  M.data[frame * M.box + c] = v;
+}
+
 // --------------------------------------------------------- learned layer
 // The column predictor of xf_learned_boundary: the Mur update plus s times
 // the MLP of x / s, s the power of two above max |x_i| (exact scaling);
@@ -515,7 +537,10 @@ __global__ void __launch_bounds__(256)
     r[12] = (float)Store<S>::ld(fld, node0, sc.inv_e);
     return;
   }
-  Store<S>::st(fld, node0, (C)lb_predict(w, L.mur_k, x), sc.s_e);
+  // edge nodes (phase 1) take the first-order Mur update: the network is
+  // trained on face interiors only
+  const float u1 = phase == 0 ? lb_predict(w, L.mur_k, x) : x[4] + L.mur_k * (x[0] - x[3]);
+  Store<S>::st(fld, node0, (C)u1, sc.s_e);
 }
 
 // ----------------------------------------------------------- batched forms

@@ -33,7 +33,7 @@ __device__ __forceinline__ float wave_sum(float v) {
 
 __global__ void __launch_bounds__(256)
     k_lb_prep(const float* __restrict__ raw, int64_t n, float mur_k, float* __restrict__ xs,
-              float* __restrict__ r) {
+              float* __restrict__ r, int* __restrict__ ex) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const float* x = raw + k * 13;
@@ -47,6 +47,16 @@ __global__ void __launch_bounds__(256)
   if (m > 0.0f) (void)frexpf(m, &e);
   for (int i = 0; i < XF_LB_IN; ++i) xs[k * XF_LB_IN + i] = m > 0.0f ? ldexpf(x[i], -e) : 0.0f;
   r[k] = m > 0.0f ? ldexpf(x[12] - mur, -e) : 0.0f;
+  ex[k] = m > 0.0f ? e : INT_MIN / 4;
+}
+
+// Loss weight (s / s_max)^power of each sample, s = 2^e its input scale.
+__global__ void k_lb_weight(const int* __restrict__ ex, int64_t n, int e_max, int power,
+                            float* __restrict__ lw) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int d = power * (ex[k] - e_max);
+  lw[k] = d < -126 ? 0.0f : ldexpf(1.0f, d);
 }
 
 // Counter-based minibatch indices (splitmix64 of seed, step, k).
@@ -65,8 +75,9 @@ __global__ void k_lb_sample(uint32_t* __restrict__ idx, int64_t count, int64_t n
 // without an index list); the loss is sum d^2 * inv_count.
 __global__ void __launch_bounds__(256)
     k_lb_grad(const float* __restrict__ xs_all, const float* __restrict__ r_all,
-              const uint32_t* __restrict__ idx, int64_t first, int64_t count, float inv_count,
-              const float* __restrict__ w16, float* __restrict__ partial) {
+              const float* __restrict__ lw_all, const uint32_t* __restrict__ idx, int64_t first,
+              int64_t count, float inv_count, const float* __restrict__ w16,
+              float* __restrict__ partial) {
   __shared__ float w[XF_LB_NPARAM];
   __shared__ float acc[kLbWaves][kLbAcc];
   for (int i = threadIdx.x; i < XF_LB_NPARAM; i += blockDim.x) w[i] = w16[i];
@@ -88,6 +99,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int i = 0; i < XF_LB_IN; ++i) x[i] = valid ? xs_all[sidx * XF_LB_IN + i] : 0.0f;
     const float target = valid ? r_all[sidx] : 0.0f;
+    const float lw = valid ? lw_all[sidx] : 0.0f;
     float h1[XF_LB_HID], h2[XF_LB_HID];
 #pragma unroll
     for (int o = 0; o < XF_LB_HID; ++o) {
@@ -106,8 +118,8 @@ __global__ void __launch_bounds__(256)
       out = out + w3[o] * h2[o];
     }
     const float d = out - target;
-    const float g_out = valid ? 2.0f * d * inv_count : 0.0f;
-    const float l = valid ? d * d * inv_count : 0.0f;
+    const float g_out = valid ? 2.0f * lw * d * inv_count : 0.0f;
+    const float l = valid ? lw * d * d * inv_count : 0.0f;
     // output layer
     float s;
 #pragma unroll
@@ -207,6 +219,9 @@ struct LbTrainer {
   float mur_k{0};
   float* xs{};
   float* r{};
+  float* lw{};   // per-sample loss weight
+  int* ex{};     // per-sample input-scale exponent
+  int e_max{0};
   float* w{};     // fp32 master
   float* w16{};   // fp16-rounded copy used by the forward pass
   uint16_t* bits{};
@@ -247,8 +262,17 @@ struct LbTrainer {
     xs = static_cast<float*>(p);
     XF_TRY(alloc(&p, static_cast<size_t>(n) * sizeof(float)));
     r = static_cast<float*>(p);
-    hipLaunchKernelGGL(k_lb_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, raw, n, mur_k, xs, r);
+    XF_TRY(alloc(&p, static_cast<size_t>(n) * sizeof(float)));
+    lw = static_cast<float*>(p);
+    XF_TRY(alloc(&p, static_cast<size_t>(n) * sizeof(int)));
+    ex = static_cast<int*>(p);
+    hipLaunchKernelGGL(k_lb_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, raw, n, mur_k, xs, r,
+                       ex);
     XF_HIP(hipGetLastError());
+    float smax = 0.0f;
+    for (int64_t q = 0; q < n * 13; ++q)
+      if (q % 13 != 12) smax = std::max(smax, std::fabs(samples[q]));
+    if (smax > 0.0f) (void)std::frexp(smax, &e_max);
     float** vecs[5] = {&w, &w16, &m, &v, &g};
     for (float** q : vecs) {
       XF_TRY(alloc(&p, XF_LB_NPARAM * sizeof(float)));
@@ -264,6 +288,15 @@ struct LbTrainer {
     nblocks = 2 * prop.multiProcessorCount;
     XF_TRY(alloc(&p, static_cast<size_t>(nblocks) * kLbAcc * sizeof(float)));
     partial = static_cast<float*>(p);
+    XF_TRY(set_weighting(0));
+    return XF_OK;
+  }
+
+  int set_weighting(int power) {
+    if (power < 0 || power > 4) return fail(XF_ERR_INVALID, "weighting power must be 0..4");
+    hipLaunchKernelGGL(k_lb_weight, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, ex, n, e_max, power,
+                       lw);
+    XF_HIP(hipGetLastError());
     XF_HIP(hipStreamSynchronize(stream));
     return XF_OK;
   }
@@ -290,7 +323,7 @@ struct LbTrainer {
   // loss and gradient over idx[0..count) (or samples [first, first+count))
   int launch_grad(const uint32_t* ix, int64_t first, int64_t count) {
     const int blocks = static_cast<int>(std::min<int64_t>(nblocks, (count + kLbWaves * 64 - 1) / (kLbWaves * 64)));
-    hipLaunchKernelGGL(k_lb_grad, dim3(blocks), dim3(64 * kLbWaves), 0, stream, xs, r, ix, first, count,
+    hipLaunchKernelGGL(k_lb_grad, dim3(blocks), dim3(64 * kLbWaves), 0, stream, xs, r, lw, ix, first, count,
                        1.0f / static_cast<float>(count), w16, partial);
     hipLaunchKernelGGL(k_lb_reduce, dim3((kLbAcc + 255) / 256), dim3(256), 0, stream, partial, blocks, g, loss);
     XF_HIP(hipGetLastError());
@@ -386,6 +419,12 @@ int xf_lb_trainer_samples(const xf_lb_trainer* h, int64_t* n) {
   if (!h || !n) return xf::fail(XF_ERR_INVALID, "null argument");
   *n = h->t.n;
   return XF_OK;
+}
+
+int xf_lb_trainer_set_weighting(xf_lb_trainer* h, int32_t power) {
+  xf::g_err.clear();
+  if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
+  return h->t.set_weighting(power);
 }
 
 int xf_lb_trainer_destroy(xf_lb_trainer* h) {

@@ -56,6 +56,7 @@ def _lib():
                                     ctypes.c_float, ctypes.c_uint64,
                                     ctypes.POINTER(ctypes.c_double)],
             "xf_lb_trainer_samples": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)],
+            "xf_lb_trainer_set_weighting": [ctypes.c_void_p, ctypes.c_int32],
             "xf_lb_trainer_destroy": [ctypes.c_void_p],
         }
         for name, args in sigs.items():
@@ -94,6 +95,11 @@ class Trainer:
             self._h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
             b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))), "get_weights")
         return w, b
+
+    def set_weighting(self, power: int):
+        """Loss weight (s / s_max)^power per sample (0: uniform)."""
+        _native.check_hip(self._lib.xf_lb_trainer_set_weighting(self._h, power), "set_weighting")
+        return self
 
     def grad(self, first: int = 0, count: int | None = None):
         count = self.n - first if count is None else count
