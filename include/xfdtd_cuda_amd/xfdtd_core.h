@@ -358,6 +358,57 @@ class TFSF2D : public TFSF {
              std::move(waveform), 2) {}
 };
 
+// ---------------------------------------------------------------- monitors
+struct EMF {
+  enum class Field { EX = 0, EY = 1, EZ = 2, HX = 3, HY = 4, HZ = 5 };
+};
+
+class Monitor {
+ public:
+  virtual ~Monitor() = default;
+};
+
+// A field component over the cells a shape covers (its bounding box,
+// snapped to the grid; an infinite extent takes the whole axis).
+class FieldMonitor {
+ public:
+  FieldMonitor(std::unique_ptr<Shape> shape, EMF::Field field, std::string name = "",
+               std::string output_dir = "")
+      : _shape(std::move(shape)), _field(field), _name(std::move(name)), _dir(std::move(output_dir)) {}
+  EMF::Field field() const { return _field; }
+  const Shape& shape() const { return *_shape; }
+
+ private:
+  std::unique_ptr<Shape> _shape;
+  EMF::Field _field;
+  std::string _name, _dir;
+};
+
+// Reference MovieMonitor (src/monitor/movie_monitor.cu): a frame of the
+// field monitor's box every `frame_interval` steps; output() writes frame n
+// as <output_dir>/%05d.npy (movie_monitor.cu:64-104).
+class MovieMonitor : public Monitor {
+ public:
+  MovieMonitor(std::unique_ptr<FieldMonitor> frame, Index frame_interval, std::string name = "movie",
+               std::string output_dir = "");
+  FieldMonitor& frame() { return *_frame; }
+  Index frameInterval() const { return _interval; }
+  const std::string& outputDir() const { return _dir; }
+  void setOutputDir(std::string d) { _dir = std::move(d); }
+  Array& data() { return _data; }  // (frames, box extents)
+  xf_movie& desc() { return _desc; }
+  // writes every frame as <output_dir>/%05d.npy; returns the frame count
+  Index output() const;
+
+ private:
+  friend class Simulation;
+  std::unique_ptr<FieldMonitor> _frame;
+  Index _interval;
+  std::string _name, _dir;
+  Array _data;
+  xf_movie _desc{};
+};
+
 // ---------------------------------------------------------- point source
 // Soft point source (no counterpart in the reference's GPU path): after the
 // E pass of step n, amplitude * g(n dt) is added to E_component at index
@@ -491,6 +542,7 @@ class Simulation {
   void addBoundary(std::shared_ptr<Boundary> b) { _boundaries.push_back(std::move(b)); }
   void addWaveformSource(std::shared_ptr<WaveformSource> w) { _sources.push_back(std::move(w)); }
   void addNF2FF(std::shared_ptr<NF2FF> n) { _nf2ffs.push_back(std::move(n)); }
+  void addMonitor(std::shared_ptr<Monitor> m) { _monitors.push_back(std::move(m)); }
 
   // Coefficient layout handed to the engine: dense per-field arrays (the
   // reference's layout) or a per-cell material index + material table.
@@ -508,6 +560,7 @@ class Simulation {
   const std::vector<std::shared_ptr<Boundary>>& boundaries() const { return _boundaries; }
   const std::vector<std::shared_ptr<WaveformSource>>& waveformSources() const { return _sources; }
   const std::vector<std::shared_ptr<NF2FF>>& nf2ffs() const { return _nf2ffs; }
+  const std::vector<std::shared_ptr<Monitor>>& monitors() const { return _monitors; }
   ADEMethodStorage* aDEMethodStorage() { return _ade.get(); }
   Real cfl() const { return _cfl; }
 
@@ -526,6 +579,8 @@ class Simulation {
   std::vector<std::shared_ptr<Boundary>> _boundaries;
   std::vector<std::shared_ptr<WaveformSource>> _sources;
   std::vector<std::shared_ptr<NF2FF>> _nf2ffs;
+  std::vector<std::shared_ptr<Monitor>> _monitors;
+  std::vector<xf_movie> _movie_desc;
   std::shared_ptr<GridSpace> _grid;
   TimeParam _time;
   std::array<Array, 6> _fields;
@@ -548,6 +603,7 @@ class Simulation {
   void buildADE();
   void buildPointSource(PointSource& p);
   void buildLearnedBoundary(LearnedBoundary& l);
+  void buildMovie(MovieMonitor& m);
   void fillScene();
   std::vector<xf_point_source> _pt_desc;
 };

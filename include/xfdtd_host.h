@@ -90,6 +90,16 @@ int xfh_sim_add_learned_boundary(xfh_sim* s, int32_t mode, int32_t faces, int64_
  * recorded x columns x 13 floats (12 inputs, then the target). */
 int xfh_sim_lb_samples(xfh_sim* s, const float** data, int64_t* columns, int64_t* recorded);
 
+/* A movie monitor (reference MovieMonitor + FieldMonitor over a cube):
+ * field 0..5 = Ex..Hz, the cube [lo, lo + size) in metres (an infinite
+ * extent takes the whole axis), a frame every `interval` steps. */
+int xfh_sim_add_movie_monitor(xfh_sim* s, int32_t field, const double lo[3], const double size[3],
+                              int64_t interval, const char* output_dir);
+
+/* Writes every movie monitor's frames as <output_dir>/%05d.npy
+ * (MovieMonitor::output, movie_monitor.cu:64-94); *frames = frames written. */
+int xfh_sim_output_movies(xfh_sim* s, int64_t* frames);
+
 /* 0 dense (the reference's 18 coefficient arrays), 1 compact. */
 int xfh_sim_set_coefficient_layout(xfh_sim* s, int32_t layout);
 

@@ -282,3 +282,40 @@ def test_cpp_dropin_driver(gpu_available, tmp_path):
     theta = -np.pi + 2 * np.pi * np.arange(360) / 359.0
     _, _, rcs_c = c.far_field(0, 0, theta, 0.0, c.incident_spectrum(1e9))
     assert rel_l2(rcs, rcs_c) <= 1e-10
+
+
+def test_movie_monitors_match_oracle(gpu_available):
+    """Movie frames (reference MovieMonitor, movie_monitor.cu:17-45) of an E
+    and an H component in 3-D and of Ez in 2-D, run in two xf_run calls."""
+    def build3():
+        s = scenes.dielectric_sphere(n=20, pml=4, steps=70, dl=5e-3, tfsf_inset=7, nf2ff_inset=5,
+                                     init=False)
+        h = 20 * 5e-3 / 2
+        s.add_movie_monitor("ex", (-h, 0.0, -h), (2 * h, 5e-3, 2 * h), 10)
+        s.add_movie_monitor("hy", (-h, -h, 0.0), (2 * h, 2 * h, 5e-3), 7)
+        s.init(70)
+        return s
+
+    def build2():
+        s = scenes.cylinder_2d(n=60, pml=6, steps=60, dl=2e-3, tfsf_inset=10, nf2ff_inset=8,
+                               init=False)
+        s.add_movie_monitor("ez", (-0.06, -0.06, -np.inf), (0.12, 0.12, np.inf), 15)
+        s.init(60)
+        return s
+
+    for build, steps in ((build3, 70), (build2, 60)):
+        g, c = build(), build()
+        e = xf.Engine(g).upload()
+        e.run(steps // 3).run(steps - steps // 3).download()
+        e.close()
+        oracle.run(c.scene_ptr(), steps)
+        q = 0
+        while True:
+            try:
+                a = g.movie(q)
+            except xf.XFError:
+                break
+            assert np.array_equal(a, c.movie(q)), f"movie {q}"
+            assert np.abs(a).max() > 0
+            q += 1
+        assert q >= 1

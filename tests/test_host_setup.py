@@ -184,3 +184,33 @@ def test_compact_layout_has_no_dense_arrays():
     s = scenes.dielectric_sphere(n=16, pml=4, steps=5, tfsf_inset=6, nf2ff_inset=5,
                                  layout="compact")
     assert s.array("coeff.0").size == 0
+
+
+def test_movie_monitor_box_and_frames(tmp_path):
+    """The driver's movie (test/dielectric_sphere_scatter.cu:47-52): Ex over
+    the xz plane y in [0, dl) of the 140^3 domain, a frame every 20 steps;
+    the oracle fills frame t / 20 with Ex over the box at step t, and
+    output() writes them as %05d.npy."""
+    import oracle
+    s = scenes.dielectric_sphere(n=20, pml=4, steps=45, dl=5e-3, tfsf_inset=7, nf2ff_inset=5,
+                                 init=False)
+    half = 20 * 5e-3 / 2
+    s.add_movie_monitor("ex", (-half, 0.0, -half), (2 * half, 5e-3, 2 * half), 20,
+                        str(tmp_path / "movie"))
+    s.init(45)
+    m = s.movie(0)
+    assert m.shape == (3, 20, 1, 20)  # frames at t = 0, 20, 40; x cells 4..24; y node 14
+    snaps = {}
+    for t in range(45):
+        if t % 20 == 0:
+            snaps[t // 20] = None
+        oracle.run(s.scene_ptr(), 1)
+        if t % 20 == 0:
+            snaps[t // 20] = s.array("ex")[4:24, 14:15, 4:24].copy()
+    # frame t/20 is recorded in step t after the E pass, i.e. the field after the step
+    for f in range(3):
+        assert np.array_equal(m[f], snaps[f])
+    assert np.abs(m[2]).max() > 0
+    assert s.output_movies() == 3
+    back = np.load(tmp_path / "movie" / "00002.npy")
+    assert np.array_equal(back, m[2])

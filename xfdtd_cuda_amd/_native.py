@@ -192,6 +192,8 @@ def host() -> ctypes.CDLL:
                                              c_i64],
             "xfh_sim_lb_samples": [c_vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_float)),
                                    p3i, p3i],
+            "xfh_sim_add_movie_monitor": [c_vp, c_i32, p3d, p3d, c_i64, ctypes.c_char_p],
+            "xfh_sim_output_movies": [c_vp, ctypes.POINTER(c_i64)],
         }
         for name, args in sigs.items():
             fn = getattr(lib, name)

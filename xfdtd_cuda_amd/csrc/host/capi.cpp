@@ -312,6 +312,32 @@ int xfh_sim_lb_samples(xfh_sim* s, const float** data, int64_t* columns, int64_t
   });
 }
 
+int xfh_sim_add_movie_monitor(xfh_sim* s, int32_t field, const double lo[3], const double size[3],
+                              int64_t interval, const char* output_dir) {
+  return guard([&] {
+    if (!s || !lo || !size) return fail("null argument");
+    if (field < 0 || field > 5) return fail("bad movie field");
+    if (interval < 1) return fail("movie interval must be >= 1");
+    auto cube = std::make_unique<xfdtd::Cube>(xfdtd::Vector{lo[0], lo[1], lo[2]},
+                                              xfdtd::Vector{size[0], size[1], size[2]});
+    auto fm = std::make_unique<xfdtd::FieldMonitor>(std::move(cube), static_cast<xfdtd::EMF::Field>(field));
+    s->sim->addMonitor(std::make_shared<xfdtd::MovieMonitor>(std::move(fm), static_cast<xfdtd::Index>(interval),
+                                                             "movie", output_dir ? output_dir : ""));
+    return 0;
+  });
+}
+
+int xfh_sim_output_movies(xfh_sim* s, int64_t* frames) {
+  return guard([&] {
+    if (!s) return fail("null sim");
+    int64_t n = 0;
+    for (auto& m : s->sim->monitors())
+      if (auto mm = dynamic_cast<xfdtd::MovieMonitor*>(m.get())) n += static_cast<int64_t>(mm->output());
+    if (frames) *frames = n;
+    return 0;
+  });
+}
+
 int xfh_sim_set_coefficient_layout(xfh_sim* s, int32_t layout) {
   return guard([&] {
     if (!s) return fail("null sim");
@@ -379,6 +405,9 @@ int xfh_scene_array(const xf_scene* sc, const char* name, double** data, int64_t
       if (sc->tfsf) a = tfsfMember(*sc->tfsf, parts[1]);
     } else if (h == "ade" && parts.size() == 2) {
       if (sc->ade) a = adeMember(*sc->ade, parts[1]);
+    } else if (h == "movie" && parts.size() == 3 && parts[2] == "data") {
+      const int q = std::stoi(parts[1]);
+      if (q >= 0 && q < sc->n_movie) a = &sc->movie[q].data;
     } else if (h == "lb" && parts.size() == 2 && parts[1] == "history") {
       if (sc->lb) a = &sc->lb->history;
     } else if (h == "point" && parts.size() == 3 && parts[2] == "value") {

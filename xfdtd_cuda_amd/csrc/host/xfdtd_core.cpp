@@ -5,6 +5,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <filesystem>
 #include <map>
 
 namespace xfdtd {
@@ -1008,6 +1010,80 @@ void Simulation::buildADE() {
   d.jz_prev = s._jzp.view();
 }
 
+// ------------------------------------------------------------ movie monitor
+MovieMonitor::MovieMonitor(std::unique_ptr<FieldMonitor> frame, Index frame_interval, std::string name,
+                           std::string output_dir)
+    : _frame(std::move(frame)), _interval(frame_interval), _name(std::move(name)), _dir(std::move(output_dir)) {
+  if (!_frame) throw XFDTDException("MovieMonitor: no field monitor");
+  if (_interval < 1) throw XFDTDException("MovieMonitor: frame interval must be >= 1");
+}
+
+namespace {
+// One 3-D array of doubles as a version-1.0 .npy file.
+void writeNpy3(const std::string& path, const double* data, const int64_t shape[3]) {
+  std::string header = "{'descr': '<f8', 'fortran_order': False, 'shape': (" + std::to_string(shape[0]) + ", " +
+                       std::to_string(shape[1]) + ", " + std::to_string(shape[2]) + "), }";
+  const size_t total = 10 + header.size() + 1;
+  header.append((64 - total % 64) % 64, ' ');
+  header.push_back('\n');
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw XFDTDException("cannot write " + path);
+  const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+  const uint16_t hlen = static_cast<uint16_t>(header.size());
+  const unsigned char len[2] = {static_cast<unsigned char>(hlen & 0xff), static_cast<unsigned char>(hlen >> 8)};
+  std::fwrite(magic, 1, 8, f);
+  std::fwrite(len, 1, 2, f);
+  std::fwrite(header.data(), 1, header.size(), f);
+  std::fwrite(data, sizeof(double), static_cast<size_t>(shape[0] * shape[1] * shape[2]), f);
+  std::fclose(f);
+}
+}  // namespace
+
+Index MovieMonitor::output() const {
+  if (_data.empty()) throw XFDTDException("MovieMonitor::output(): no data (simulation not initialised)");
+  std::filesystem::create_directories(_dir.empty() ? "." : _dir);
+  const auto& sh = _data.shape();
+  const int64_t box[3] = {sh[1], sh[2], sh[3]};
+  const int64_t frame = sh[1] * sh[2] * sh[3];
+  for (int64_t t = 0; t < sh[0]; ++t) {
+    char name[32];
+    std::snprintf(name, sizeof name, "%05lld.npy", static_cast<long long>(t));
+    writeNpy3((std::filesystem::path(_dir.empty() ? "." : _dir) / name).string(), _data.data() + t * frame, box);
+  }
+  return static_cast<Index>(sh[0]);
+}
+
+void Simulation::buildMovie(MovieMonitor& m) {
+  const GridSpace& g = *_grid;
+  const int comp = static_cast<int>(m.frame().field());
+  const Array& f = _fields[comp];
+  const auto& sh = f.shape();
+  const Shape& shape = m.frame().shape();
+  const Vector lo = shape.lower(), hi = shape.upper();
+  const Real o[3] = {g.origin().x, g.origin().y, g.origin().z};
+  const Real d[3] = {_dx, _dy, _dz};
+  const Real l[3] = {lo.x, lo.y, lo.z}, h[3] = {hi.x, hi.y, hi.z};
+  int64_t start[3], end[3];
+  for (int a = 0; a < 3; ++a) {
+    // a non-finite bound (an infinite cube side) takes the whole axis
+    const bool full = !std::isfinite(l[a]) || !std::isfinite(h[a]);
+    start[a] = full ? 0 : static_cast<int64_t>(std::llround((l[a] - o[a]) / d[a]));
+    end[a] = full ? sh[a] : static_cast<int64_t>(std::llround((h[a] - o[a]) / d[a]));
+    start[a] = std::max<int64_t>(0, std::min<int64_t>(start[a], sh[a] - 1));
+    end[a] = std::max<int64_t>(start[a] + 1, std::min<int64_t>(end[a], sh[a]));
+  }
+  const Index nt = _time._start + _time._size;
+  const int64_t frames = static_cast<int64_t>((nt + m._interval - 1) / m._interval);
+  m._data = Array({frames, end[0] - start[0], end[1] - start[1], end[2] - start[2]});
+  m._desc = xf_movie{};
+  m._desc.field = comp;
+  m._desc.task.x = xf_range{start[0], end[0]};
+  m._desc.task.y = xf_range{start[1], end[1]};
+  m._desc.task.z = xf_range{start[2], end[2]};
+  m._desc.interval = static_cast<int64_t>(m._interval);
+  m._desc.data = m._data.view();
+}
+
 // ----------------------------------------------------------- point source
 PointSource::PointSource(Axis::XYZ component, Index i, Index j, Index k,
                          std::unique_ptr<Waveform> waveform, Real amplitude)
@@ -1150,6 +1226,11 @@ void Simulation::fillScene() {
     if (auto p = dynamic_cast<PointSource*>(w.get())) _pt_desc.push_back(p->desc());
   _scene.n_point = static_cast<int32_t>(_pt_desc.size());
   _scene.point = _pt_desc.empty() ? nullptr : _pt_desc.data();
+  _movie_desc.clear();
+  for (auto& mo : _monitors)
+    if (auto m = dynamic_cast<MovieMonitor*>(mo.get())) _movie_desc.push_back(m->desc());
+  _scene.n_movie = static_cast<int32_t>(_movie_desc.size());
+  _scene.movie = _movie_desc.empty() ? nullptr : _movie_desc.data();
   _scene.lb = nullptr;
   for (auto& b : _boundaries)
     if (auto l = dynamic_cast<LearnedBoundary*>(b.get())) {
@@ -1195,6 +1276,8 @@ void Simulation::init(Index time_step) {
     if (auto p = dynamic_cast<PointSource*>(w.get())) buildPointSource(*p);
   for (auto& b : _boundaries)
     if (auto l = dynamic_cast<LearnedBoundary*>(b.get())) buildLearnedBoundary(*l);
+  for (auto& mo : _monitors)
+    if (auto m = dynamic_cast<MovieMonitor*>(mo.get())) buildMovie(*m);
   fillScene();
   _initialized = true;
 }

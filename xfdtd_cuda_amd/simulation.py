@@ -263,6 +263,25 @@ class Simulation:
             return np.zeros((0, cols.value, 13), np.float32)
         return np.ctypeslib.as_array(p, shape=(n,)).reshape(rec.value, cols.value, 13).copy()
 
+    def add_movie_monitor(self, field: str, lo, size, interval: int, output_dir: str = ""):
+        """MovieMonitor(FieldMonitor(Cube(lo, size), field), interval): a frame
+        of the component over the cube every `interval` steps (reference
+        src/monitor/movie_monitor.cu); field is one of ex..hz."""
+        f = ("ex", "ey", "ez", "hx", "hy", "hz").index(field.lower())
+        _native.check_host(self._lib.xfh_sim_add_movie_monitor(
+            self._h, f, _d3(lo), _d3(size), interval, output_dir.encode()), "addMonitor")
+        return self
+
+    def movie(self, index: int = 0) -> np.ndarray:
+        """Frames of movie monitor `index`: (frames, box extents), a view."""
+        return self.array(f"movie.{index}.data")
+
+    def output_movies(self) -> int:
+        """Writes every movie monitor's frames as %05d.npy; returns the count."""
+        n = c_i64()
+        _native.check_host(self._lib.xfh_sim_output_movies(self._h, ctypes.byref(n)), "output")
+        return n.value
+
     def set_coefficient_layout(self, layout: str):
         _native.check_host(self._lib.xfh_sim_set_coefficient_layout(
             self._h, {"dense": 0, "compact": 1}[layout]), "setCoefficientLayout")
