@@ -168,8 +168,10 @@ typedef struct xf_point_source {
  * step being advanced, the 12 inputs of a column are
  *   x0..x2  u^{t+1} on layers 1..3 (after the E pass)
  *   x3..x6  u^t     on layers 0..3
- *   x7..x10 u^{t-1} on layers 0..3
+ *   x7..x9  u^{t-1} on layers 0..2
+ *   x10     the in-plane 5-point Laplacian of u^t on layer 1
  *   x11     the in-plane 5-point Laplacian of u^t on layer 0
+ * (the terms of the second-order Mur / Higdon updates)
  * and the layer-0 value u^{t+1} is
  *   mur + s * MLP(x / s),  mur = x4 + mur_k * (x0 - x3),  s = max_i |x_i|
  * (the first-order Mur update plus a learned, scale-free correction;

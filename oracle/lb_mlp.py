@@ -3,7 +3,9 @@ training loss -- TEST INFRASTRUCTURE ONLY (the checker of the HIP trainer and
 of the engine's column predictor; the product never imports it).
 
 The model (xf_learned_boundary in include/xfdtd_hip.h; no reference
-counterpart): for a column x of 12 inputs
+counterpart): for a column x of 12 inputs (u^{t+1} on layers 1..3, u^t on
+layers 0..3, u^{t-1} on layers 0..2, the in-plane Laplacians of u^t on
+layers 1 and 0)
 
     s    = 2^e, the power of two with max|x_i| in [2^(e-1), 2^e)
     mur  = x4 + k (x0 - x3)

@@ -672,13 +672,17 @@ static void lb_inputs(xf_scene* s, const lb_geom* g, int cc, int64_t inset, int6
                       int64_t q, int64_t p1, int64_t q1, int par, float* x) {
   for (int l = 1; l <= 3; ++l) x[l - 1] = (float)*lb_field(s, g, cc, inset, l, p, q);
   for (int l = 0; l < 4; ++l) x[3 + l] = (float)*lb_hist(s, g, par, cc, l, p, q);
-  for (int l = 0; l < 4; ++l) x[7 + l] = (float)*lb_hist(s, g, par ^ 1, cc, l, p, q);
-  const float u = x[3];
-  const float up = p < p1 ? (float)*lb_hist(s, g, par, cc, 0, p + 1, q) : u;
-  const float um = p > inset ? (float)*lb_hist(s, g, par, cc, 0, p - 1, q) : u;
-  const float vp = q < q1 ? (float)*lb_hist(s, g, par, cc, 0, p, q + 1) : u;
-  const float vm = q > inset ? (float)*lb_hist(s, g, par, cc, 0, p, q - 1) : u;
-  x[11] = (((up + um) + (vp + vm)) - 4.0f * u);
+  for (int l = 0; l < 3; ++l) x[7 + l] = (float)*lb_hist(s, g, par ^ 1, cc, l, p, q);
+  /* in-plane 5-point Laplacians of u^t on layers 1 (x10) and 0 (x11); a
+   * neighbour outside the column range counts as the centre value */
+  for (int l = 1; l >= 0; --l) {
+    const float u = x[3 + l];
+    const float up = p < p1 ? (float)*lb_hist(s, g, par, cc, l, p + 1, q) : u;
+    const float um = p > inset ? (float)*lb_hist(s, g, par, cc, l, p - 1, q) : u;
+    const float vp = q < q1 ? (float)*lb_hist(s, g, par, cc, l, p, q + 1) : u;
+    const float vm = q > inset ? (float)*lb_hist(s, g, par, cc, l, p, q - 1) : u;
+    x[11 - l] = (((up + um) + (vp + vm)) - 4.0f * u);
+  }
 }
 
 /* One step of the learned layer (after the E pass, its corrections and the

@@ -97,15 +97,16 @@ def _gather_columns(snaps_e, t, n, inset):
                 for q in range(inset, q1 + 1):
                     x = [val(nxt, l, p, q) for l in (1, 2, 3)]
                     x += [val(cur, l, p, q) for l in range(4)]
-                    x += [val(prev, l, p, q) for l in range(4)]
-                    u = val(cur, 0, p, q)
-                    up = val(cur, 0, p + 1, q) if p < p1 else u
-                    um = val(cur, 0, p - 1, q) if p > inset else u
-                    vp = val(cur, 0, p, q + 1) if q < q1 else u
-                    vm = val(cur, 0, p, q - 1) if q > inset else u
-                    lap = np.float32(np.float32(np.float32(up + um) + np.float32(vp + vm))
-                                     - np.float32(4.0) * u)
-                    rows.append(x + [lap, val(nxt, 0, p, q)])
+                    x += [val(prev, l, p, q) for l in range(3)]
+                    for layer in (1, 0):
+                        u = val(cur, layer, p, q)
+                        up = val(cur, layer, p + 1, q) if p < p1 else u
+                        um = val(cur, layer, p - 1, q) if p > inset else u
+                        vp = val(cur, layer, p, q + 1) if q < q1 else u
+                        vm = val(cur, layer, p, q - 1) if q > inset else u
+                        x.append(np.float32(np.float32(np.float32(up + um) + np.float32(vp + vm))
+                                            - np.float32(4.0) * u))
+                    rows.append(x + [val(nxt, 0, p, q)])
     return np.array(rows, np.float32)
 
 

@@ -522,13 +522,18 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int l = 0; l < 4; ++l) x[3 + l] = *lb_hist(L, c, par, l, c.p, c.q);
 #pragma unroll
-  for (int l = 0; l < 4; ++l) x[7 + l] = *lb_hist(L, c, par ^ 1, l, c.p, c.q);
-  const float u = x[3];
-  const float up = c.p < c.F->p1[c.cc] ? *lb_hist(L, c, par, 0, c.p + 1, c.q) : u;
-  const float um = c.p > L.inset ? *lb_hist(L, c, par, 0, c.p - 1, c.q) : u;
-  const float vp = c.q < c.F->q1[c.cc] ? *lb_hist(L, c, par, 0, c.p, c.q + 1) : u;
-  const float vm = c.q > L.inset ? *lb_hist(L, c, par, 0, c.p, c.q - 1) : u;
-  x[11] = (((up + um) + (vp + vm)) - 4.0f * u);
+  for (int l = 0; l < 3; ++l) x[7 + l] = *lb_hist(L, c, par ^ 1, l, c.p, c.q);
+  // in-plane 5-point Laplacians of u^t on layers 1 (x10) and 0 (x11); a
+  // neighbour outside the column range counts as the centre value
+#pragma unroll
+  for (int l = 1; l >= 0; --l) {
+    const float u = x[3 + l];
+    const float up = c.p < c.F->p1[c.cc] ? *lb_hist(L, c, par, l, c.p + 1, c.q) : u;
+    const float um = c.p > L.inset ? *lb_hist(L, c, par, l, c.p - 1, c.q) : u;
+    const float vp = c.q < c.F->q1[c.cc] ? *lb_hist(L, c, par, l, c.p, c.q + 1) : u;
+    const float vm = c.q > L.inset ? *lb_hist(L, c, par, l, c.p, c.q - 1) : u;
+    x[11 - l] = (((up + um) + (vp + vm)) - 4.0f * u);
+  }
   const int64_t node0 = lb_node(L, d, c, 0);
   if (record) {
     float* r = L.rec + ((int64_t)(t / L.rec_every) * L.total + col) * 13;
