@@ -20,6 +20,7 @@ steps of the same scene on this host.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
        python bench.py --workload batch   (64 uneven grids, FP32, batched kernels)
        python bench.py --workload precision --steps 4000   (FP64/FP32/BF16 far-field error)
+       python bench.py --workload learned   (learned 2-cell layer: training, reflection, speed)
 """
 from __future__ import annotations
 
@@ -66,10 +67,12 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "batch", "precision"],
+    ap.add_argument("--workload", default="sphere",
+                    choices=["sphere", "batch", "precision", "learned"],
                     help="sphere: one 512^3 + CPML grid (default); batch: 64 uneven grids; "
                          "precision: the sphere run to the end in fp64, fp32, bf16 with "
-                         "far-field errors against fp64")
+                         "far-field errors against fp64; learned: the learned 2-cell layer "
+                         "trained on the GPU, its reflection and step time against CPMLs")
     ap.add_argument("--count", type=int, default=64, help="grids of the batch workload")
     args = ap.parse_args(argv)
 
@@ -81,6 +84,8 @@ def main(argv=None) -> int:
         return bench_batch(args)
     if args.workload == "precision":
         return bench_precision(args)
+    if args.workload == "learned":
+        return bench_learned(args)
     args.precision = args.precision or "fp64"
 
     import xfdtd_cuda_amd as xf
@@ -339,6 +344,65 @@ def bench_precision(args) -> int:
         "config": {"workload": f"precision study: sphere {args.n}^3 + {args.pml}-cell CPML "
                                f"({cells} cells), {S} steps per mode, RCS at 1 GHz vs FP64",
                    "modes": modes},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_learned(args) -> int:
+    """The learned absorbing layer end to end on the GPU: training data from
+    16-cell CPML point-source runs (seeds 0-5, 32^3 interior), training by
+    the HIP trainer, reflection error against the 16-cell CPML on an unseen
+    seed next to Mur and 2/4/8-cell CPMLs, and the step time of a 256^3
+    interior bounded by the 2-cell layer against the same interior with an
+    8-cell CPML (FP32 fields)."""
+    import xfdtd_cuda_amd as xf
+    from xfdtd_cuda_amd import learned, scenes
+
+    t0 = time.perf_counter()
+    smp, k = learned.record_samples(n=32, seeds=(0, 1, 2, 3, 4, 5), steps=300, every=2)
+    t_rec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    w16, loss0, loss1 = learned.train_layer(smp, k)
+    t_train = time.perf_counter() - t0
+    err = learned.reflection_errors(w16, n=32, seed=7, steps=300, every=15,
+                                    variants=[("learned", 2), ("mur", 2), ("cpml", 2), ("cpml", 4),
+                                              ("cpml", 8)])
+    print(f"# {smp.shape[0]} samples in {t_rec:.1f} s, trained in {t_train:.1f} s, "
+          f"loss {loss0:.3e} -> {loss1:.3e}, reflection {err}", file=sys.stderr, flush=True)
+
+    K, W, n = args.steps, args.warmup, 256
+    rates = {}
+    for kind, size in (("learned", 2), ("cpml", 8)):
+        sim = scenes.point_source_box(n=n, boundary=kind, size=size, steps=W + K + 4, seed=7,
+                                      weights=w16 if kind == "learned" else None)
+        cells = int(np.prod(sim.info()[0]))
+        e = xf.Engine(sim, precision="fp32", use_graph=True).upload()
+        e.run(W)
+        e.reset_timing()
+        e.run(K)
+        ms = e.timing()["run_ms"] / K
+        e.close()
+        rates[kind] = (cells, ms)
+        print(f"# {kind}{size}: {cells} cells, {ms:.3f} ms/step", file=sys.stderr, flush=True)
+    cells_l, ms_l = rates["learned"]
+    cells_c, ms_c = rates["cpml"]
+    out = {
+        "metric": "Yee cell-updates/s (Gcells/s) & % HBM roofline",
+        "value": round(cells_l / (ms_l * 1e-3) / 1e9, 4), "unit": "Gcells/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(ms_l, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 fields, fp16-weight/fp32 network",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 vacuum interior with seeded point sources bounded by the "
+                               "learned 2-cell layer (vs an 8-cell CPML)",
+                   "cells_learned": cells_l, "cells_cpml8": cells_c,
+                   "ms_per_step_cpml8": round(ms_c, 4),
+                   "interior_speedup_vs_cpml8": round(ms_c / ms_l, 3),
+                   "reflection_vs_cpml16": {k2: float(v) for k2, v in err.items()},
+                   "training": {"samples": int(smp.shape[0]), "record_s": round(t_rec, 2),
+                                "train_s": round(t_train, 2), "loss_before": loss0,
+                                "loss_after": loss1}},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

@@ -149,6 +149,22 @@ def record_samples(n: int = 40, seeds=(0, 1, 2), steps: int = 400, pml: int = 16
     return np.concatenate(out), k
 
 
+def train_layer(samples: np.ndarray, k: float, seed: int = 0, device: int = 0):
+    """The training recipe: Adam from the plain Mur start, 1500 steps at
+    lr 2e-3 on minibatches of 8192, 750 at 5e-4, 750 at 1.25e-4 on 16384
+    (longer runs fit the one-step target better but reflect more in the
+    closed loop).  Returns (fp16 weight bits, loss before, loss after)."""
+    t = Trainer(samples, k, device).set_weights(init_weights(seed))
+    l0 = t.loss()
+    t.train(1500, batch=8192, lr=2e-3, seed=seed + 1)
+    t.train(750, batch=8192, lr=5e-4, seed=seed + 2)
+    t.train(750, batch=16384, lr=1.25e-4, seed=seed + 3)
+    l1 = t.loss()
+    _, w16 = t.weights()
+    t.close()
+    return w16, l0, l1
+
+
 def _interior(sim, margin: int, n: int):
     return [sim.array(c)[margin:margin + n, margin:margin + n, margin:margin + n].copy()
             for c in ("ex", "ey", "ez")]
