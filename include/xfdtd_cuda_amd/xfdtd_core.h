@@ -26,6 +26,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../xfdtd_hip.h"
@@ -77,6 +78,12 @@ class Array {
 struct Vector {
   Real x{}, y{}, z{};
 };
+
+// .npy output (the drivers' xt::dump_npy): C-order doubles, or complex
+// doubles (interleaved re, im) written as '<c16'.
+void dumpNpy(const std::string& path, const double* data, const std::vector<int64_t>& shape,
+             bool is_complex = false);
+void dumpNpy(const std::string& path, const std::vector<Real>& v);
 
 namespace Axis {
 enum class XYZ { X = 0, Y = 1, Z = 2 };
@@ -479,28 +486,92 @@ class NF2FF {
 class NFFFTFrequencyDomain : public NF2FF {
  public:
   NFFFTFrequencyDomain(Index sx, Index sy, Index sz,
-                       std::vector<Real> frequencies);
+                       std::vector<Real> frequencies, std::string output_dir = "");
   Index freqCount() const { return _freqs.size(); }
   const std::vector<Real>& frequencies() const { return _freqs; }
   xf_nf2ff& desc() { return _desc; }
-  // Far field along directions (theta_i, phi): complex E_theta, E_phi
-  // (per unit incident spectrum when normalise is true) and bistatic RCS.
+  void setOutputDir(std::string dir) { _output_dir = std::move(dir); }
+  const std::string& outputDir() const { return _output_dir; }
+  // Far field along directions (theta_i, phi_i): complex E_theta, E_phi
+  // (divided by the incident spectrum when it is nonzero) and bistatic RCS
+  // 4 pi (|E_theta|^2 + |E_phi|^2).
   struct FarField {
     std::vector<std::complex<Real>> e_theta, e_phi;
     std::vector<Real> rcs;
   };
+  FarField farField(Index freq_index, const std::vector<Real>& theta,
+                    const std::vector<Real>& phi, std::complex<Real> incident) const;
   FarField processFarField(Index freq_index, const std::vector<Real>& theta,
                            Real phi, std::complex<Real> incident) const;
+  // The drivers' calls (dielectric_sphere_scatter.cu:86-91,
+  // dispersive_sphere_scatter.cuh:113-124): a theta cut at one phi or a phi
+  // cut at one theta, every frequency, written unnormalised to
+  // <outputDir>/<sub_dir>/: theta.npy and phi.npy (one value per direction),
+  // freq.npy, e_theta.npy and e_phi.npy (complex, frequencies x directions).
+  void processFarField(const std::vector<Real>& theta, Real phi, const std::string& sub_dir) const;
+  void processFarField(Real theta, const std::vector<Real>& phi, const std::string& sub_dir) const;
 
  private:
   friend class Simulation;
   Index _sx, _sy, _sz;
   std::vector<Real> _freqs;
+  std::string _output_dir;
+  void writeFarField(const std::vector<Real>& theta, const std::vector<Real>& phi,
+                     const std::string& sub_dir) const;
   const GridSpace* _grid{};
   std::vector<Array> _cur;           // [freq][face][4]
   std::vector<Array> _te, _th;       // [freq]
   std::vector<xf_nf2ff_freq> _fdesc;
   xf_nf2ff _desc{};
+};
+
+// Time-domain Huygens box at an inset of (sx, sy, sz) cells for one
+// observation direction (theta, phi): the host side the reference's
+// NF2FFTimeDomainHD consumes (nf2ff_time_domain_hd.cu:35-59):
+// observationDirection, distanceRange<E|H>, the per-face potentials
+// (equivalentSurfaceCurrent: wa, wb from H, ua, ub from E) and the previous
+// face fields.  The potentials hold n_pot = nt + ceil((dmax - dmin) / (c dt))
+// + 2 time bins; bin m is the observer time m dt measured from the arrival
+// of a wave leaving the surface point of largest r . rhat at time 0.
+class NFFFTTimeDomain : public NF2FF {
+ public:
+  NFFFTTimeDomain(Index sx, Index sy, Index sz, Real theta, Real phi);
+  // the driver form (dielectric_sphere_scatter.cu:72-74)
+  NFFFTTimeDomain(Index sx, Index sy, Index sz, std::string output_dir, Real theta, Real phi);
+  void setOutputDir(std::string dir) { _output_dir = std::move(dir); }
+  const std::string& outputDir() const { return _output_dir; }
+  Vector observationDirection() const { return _r_unit; }
+  Real theta() const { return _theta; }
+  Real phi() const { return _phi; }
+  // min / max of r . rhat over the surface cells (metres); E and H use the
+  // same surface points (rVector, nf2ff_time_domain_data.cu:54-68).
+  std::pair<Real, Real> distanceRange() const { return {_dmin, _dmax}; }
+  Index potentialSize() const { return _n_pot; }
+  xf_nf2ff_td& desc() { return _desc; }
+  const Array& potential(int face, int q) const { return _pot[face * 4 + q]; }
+  // Far field R E_theta(t_m), R E_phi(t_m), t_m = m dt (Luebbers et al.,
+  // IEEE TAP 39(4), 1991): W = w / (4 pi c dt), U = u / (4 pi c dt) summed
+  // over the faces as vectors, R E_theta = -(eta W_theta + U_phi),
+  // R E_phi = -eta W_phi + U_theta.
+  struct FarField {
+    std::vector<Real> time, e_theta, e_phi;
+  };
+  FarField farField() const;
+  // Writes time.npy, e_theta.npy and e_phi.npy to outputDir().
+  void processFarField() const;
+
+ private:
+  friend class Simulation;
+  Index _sx, _sy, _sz;
+  Real _theta, _phi;
+  std::string _output_dir;
+  Vector _r_unit{};
+  Real _dmin{}, _dmax{}, _dt{};
+  Index _n_pot{};
+  std::vector<Array> _pot;   // [face][wa, wb, ua, ub]
+  std::vector<Array> _prev;  // [face][ea, eb, ha, hb]
+  std::array<Array, 3> _e_node, _h_node, _e_size;
+  xf_nf2ff_td _desc{};
 };
 
 // --------------------------------------------------------------- ADE
@@ -591,6 +662,7 @@ class Simulation {
   std::unique_ptr<ADEMethodStorage> _ade;
   std::vector<xf_pml> _pml_desc;
   std::vector<xf_nf2ff> _nf_desc;
+  std::vector<xf_nf2ff_td> _nftd_desc;
   xf_scene _scene{};
   bool _initialized{false};
 
@@ -600,6 +672,7 @@ class Simulation {
   void buildPML(PML& p);
   void buildTFSF(TFSF& t);
   void buildNF2FF(NFFFTFrequencyDomain& n);
+  void buildNF2FFTD(NFFFTTimeDomain& n);
   void buildADE();
   void buildPointSource(PointSource& p);
   void buildLearnedBoundary(LearnedBoundary& l);

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define XF_ABI_VERSION 2
+#define XF_ABI_VERSION 3
 
 enum xf_status {
   XF_OK = 0,
@@ -135,6 +135,35 @@ typedef struct xf_ade {
   xf_array ex_prev, ey_prev, ez_prev;               /* state */
   xf_array jx, jy, jz, jx_prev, jy_prev, jz_prev;   /* state */
 } xf_ade;
+
+/* NF2FF time domain for one observation direction: reference
+ * NF2FFTimeDomainData<D>::update<E|H> (src/nf2ff/time_domain/
+ * nf2ff_time_domain_data.cu:103-188), run in the record phase after the
+ * frequency-domain accumulation (domain_hd.cu:65-77).  For every surface
+ * cell (i, j, k) of a face with normal axis c and tangential axes a, b:
+ *   r  = rVector (:54-68): e_node along c, h_node along a and b;
+ *   ds = surfaceArea (:70-80): e_size[a] * e_size[b];
+ *   s  = (dmax - r . r_unit) / (c0 dt) + off + t, off = 0.5 (E) / 0 (H),
+ *   nn = floor(s), f = s - nn;
+ *   A, B = interpolateSurfaceCenter of the a, b components (E or H),
+ *   dA = A - prevA, dB = B - prevB, then prevA = A, prevB = B;
+ *   pa[nn] += ca (1 - f) ds dB,  pa[nn + 1] += ca f ds dB,
+ *   pb[nn] += cb (1 - f) ds dA,  pb[nn + 1] += cb f ds dA,
+ * with (pa, pb) = (ua, ub), ca = +1, cb = -1 for E and (wa, wb), ca = -1,
+ * cb = +1 for H, both negated on a negative face.  The reference launches
+ * this on one thread per face because the scatter-add races (:213-223); the
+ * engine reduces per tile and per bin in a fixed order, so its sums are
+ * reproducible run to run.  3-D grids only. */
+typedef struct xf_nf2ff_td {
+  xf_task face_task[6];  /* nodeTaskSurfaceXN..ZP (normal extent 1); empty ranges skip a face */
+  double r_unit[3];      /* observationDirection() */
+  double dmax_e, dmax_h; /* distanceRange<E>().end(), distanceRange<H>().end() (metres) */
+  xf_array e_node[3];    /* GridSpace eNodeX/Y/Z: (n+1,) node coordinates */
+  xf_array h_node[3];    /* hNodeX/Y/Z: (n,) cell centres */
+  xf_array e_size[3];    /* eSizeX/Y/Z: (n,) cell sizes */
+  xf_array pot[6][4];    /* per face: wa, wb, ua, ub, (n_pot,) (state) */
+  xf_array prev[6][4];   /* per face: ea, eb, ha, hb of the last step, (n_a, n_b) (state) */
+} xf_nf2ff_td;
 
 /* A movie monitor: reference MovieMonitor<F>::update / nextCount
  * (src/monitor/movie_monitor.cu:17-45, 110-120).  In the record phase of
@@ -247,6 +276,8 @@ typedef struct xf_scene {
   xf_learned_boundary* lb; /* ... and the learned layer (NULL: none) */
   int32_t n_movie;
   xf_movie* movie; /* movie monitors (MovieMonitorHD, simulation_hd.cu:162-207) */
+  int32_t n_nf2ff_td;
+  xf_nf2ff_td* nf2ff_td; /* NF2FF time domain (NF2FFTimeDomainHD) */
 } xf_scene;
 
 /* Launch plan.  grid_dim/block_dim are the reference's setGridDim/

@@ -70,6 +70,11 @@ int xfh_sim_add_tfsf2d(xfh_sim* s, const int64_t start[2], double phi,
 int xfh_sim_add_nf2ff_fd(xfh_sim* s, const int64_t start[3],
                          const double* freqs, int32_t n_freq);
 
+/* NF2FF time-domain box (NFFFTTimeDomain, the host side of the reference's
+ * NF2FFTimeDomainHD, nf2ff_time_domain_hd.cu:35-59) at the same inset
+ * convention, observed along (theta, phi). */
+int xfh_sim_add_nf2ff_td(xfh_sim* s, const int64_t start[3], double theta, double phi);
+
 /* Additions without a reference counterpart.  A soft point source on
  * E_comp at index idx of that component's array (grid indices), driving
  * amplitude * waveform(n dt) after each E pass. */
@@ -112,7 +117,9 @@ int xfh_sim_info(xfh_sim* s, int64_t dims[3], double* dt, int32_t* dimension);
 
 /* Named array of any scene: "ex".."hz", "coeff.N", "pml.P.<member>",
  * "tfsf.<member>", "nf2ff.Q.F.<face>.<ja|jb|ma|mb>", "nf2ff.Q.F.transform_e",
- * "ade.<member>".  Returns a pointer into the scene's storage. */
+ * "nf2ff_td.Q.<face>.<wa|wb|ua|ub>" (potentials),
+ * "nf2ff_td.Q.<face>.prev.<ea|eb|ha|hb>", "ade.<member>", "movie.N.data",
+ * "lb.history", "point.N.value".  Returns a pointer into the scene's storage. */
 int xfh_scene_array(const xf_scene* sc, const char* name, double** data,
                     int64_t shape[5], int32_t* ndim);
 
@@ -129,6 +136,29 @@ int xfh_sim_run_hd(xfh_sim* s, int64_t time_step, const int32_t grid[3],
 int xfh_sim_far_field(xfh_sim* s, int32_t q, int32_t f, const double* theta,
                       int64_t n, double phi, double inc_re, double inc_im,
                       double* e_theta, double* e_phi, double* rcs);
+
+/* Far field of NF2FF time-domain box q: *n = potential bins; with non-NULL
+ * arrays of *n doubles, the time axis and R E_theta(t), R E_phi(t)
+ * (NFFFTTimeDomain::processFarField). */
+int xfh_sim_far_field_td(xfh_sim* s, int32_t q, int64_t* n, double* time,
+                         double* e_theta, double* e_phi);
+
+/* The drivers' on-disk outputs.  Far field of frequency-domain box q for
+ * every frequency along a theta cut at one phi (n_phi = 1) or a phi cut at
+ * one theta (n_theta = 1), unnormalised, to <output_dir>/<sub_dir>/
+ * {theta,phi,freq,e_theta,e_phi}.npy (NFFFTFrequencyDomain::processFarField
+ * as called in dielectric_sphere_scatter.cu:86-91); output_dir NULL keeps
+ * the box's own. */
+int xfh_sim_output_far_field(xfh_sim* s, int32_t q, const double* theta, int64_t n_theta,
+                             const double* phi, int64_t n_phi, const char* sub_dir,
+                             const char* output_dir);
+
+/* Time-domain box q: <output_dir>/{time,e_theta,e_phi}.npy. */
+int xfh_sim_output_far_field_td(xfh_sim* s, int32_t q, const char* output_dir);
+
+/* The TFSF waveform as the drivers dump it: <output_dir>/time.npy and
+ * incident_wave.npy (dielectric_sphere_scatter.cu:96-99). */
+int xfh_sim_output_waveform(xfh_sim* s, const char* output_dir);
 
 /* Waveform samples of the TFSF source (time(), value()), length n_steps. */
 int xfh_sim_waveform(xfh_sim* s, double* time, double* value, int64_t n);

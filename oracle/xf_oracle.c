@@ -9,14 +9,16 @@
  * reference kernels operation by operation, in the reference's step order
  * (src/domain/domain_hd.cu:35-87):
  *   updateH -> correctH (TFSF, then PML X, Y, Z) -> updateE ->
- *   correctE (TFSF, then PML) -> record (NF2FF-FD) -> nextStep.
+ *   correctE (TFSF, then PML) -> record (movies, NF2FF-FD, NF2FF-TD) ->
+ *   nextStep.
  * Floating-point expressions keep the reference's evaluation order and are
  * compiled without contraction (-ffp-contract=off), so a GPU build with the
  * same order and no FMA reproduces the result bit for bit.
  *
  * Pins (see tests/): Yee coefficients and dispersion against closed forms,
  * PEC-cavity energy, CPML reflection, TFSF leakage, the DFT of a known
- * sinusoid for NF2FF, the analytic permittivity of the ADE media, and
+ * sinusoid for NF2FF, the telescoping sums and first moments of the
+ * NF2FF-TD potentials, the analytic permittivity of the ADE media, and
  * a brute-force triple-loop form of the flat Yee update.
  */
 #include "xf_oracle.h"
@@ -522,6 +524,67 @@ static void nf2ff_face(const xf_scene* s, const xf_nf2ff* nf,
   }
 }
 
+/* NF2FFTimeDomainData<D>::update<E|H> (nf2ff_time_domain_data.cu:103-188)
+ * for one face, serially in (i, j, k) order as the reference's one-thread
+ * launch (:213-223) runs it.  r is rVector (:54-68), ds surfaceArea
+ * (:70-80), the prev arrays are indexed (a, b) (:81-186).  Returns nonzero
+ * when a time bin falls outside the potential arrays. */
+static int nf2ff_td_face(const xf_scene* s, xf_nf2ff_td* td, int face, int is_e, int64_t t) {
+  const xf_task* task = &td->face_task[face];
+  if (!(task->x.start < task->x.end && task->y.start < task->y.end &&
+        task->z.start < task->z.end))
+    return 0;
+  const int c = face / 2;
+  const int positive = face % 2;
+  const int a = (c + 1) % 3, b = (c + 2) % 3;
+  const double offset = is_e ? 0.5 : 0.0;
+  double coeff_a = is_e ? 1.0 : -1.0;
+  double coeff_b = is_e ? -1.0 : 1.0;
+  if (!positive) {
+    coeff_a *= -1.0;
+    coeff_b *= -1.0;
+  }
+  const double dmax = is_e ? td->dmax_e : td->dmax_h;
+  const double c_dt = 299792458.0 * s->dt;
+  xf_array* pa = &td->pot[face][is_e ? 2 : 0];
+  xf_array* pb = &td->pot[face][is_e ? 3 : 1];
+  xf_array* qa = &td->prev[face][is_e ? 0 : 2];
+  xf_array* qb = &td->prev[face][is_e ? 1 : 3];
+  const int64_t n_pot = pa->shape[0];
+  const int64_t o[3] = {task->x.start, task->y.start, task->z.start};
+  const int64_t nb = qa->shape[1];
+  for (int64_t i = task->x.start; i < task->x.end; ++i) {
+    for (int64_t j = task->y.start; j < task->y.end; ++j) {
+      for (int64_t k = task->z.start; k < task->z.end; ++k) {
+        const int64_t idx[3] = {i, j, k};
+        double r[3];
+        for (int ax = 0; ax < 3; ++ax)
+          r[ax] = ax == c ? td->e_node[ax].data[idx[ax]] : td->h_node[ax].data[idx[ax]];
+        const double time_delay =
+            dmax - (r[0] * td->r_unit[0] + r[1] * td->r_unit[1] + r[2] * td->r_unit[2]);
+        const double time_step_delay = time_delay / c_dt;
+        const double sn = time_step_delay + offset + (double)t;
+        const int64_t nn = (int64_t)floor(sn);
+        const double coeff_f = sn - (double)nn;
+        if (nn < 0 || nn + 1 >= n_pot) return 1;
+        const double ds = td->e_size[a].data[idx[a]] * td->e_size[b].data[idx[b]];
+        const double a_avg = interp(s, c, is_e, a, i, j, k);
+        const double b_avg = interp(s, c, is_e, b, i, j, k);
+        const int64_t lin = (idx[a] - o[a]) * nb + (idx[b] - o[b]);
+        const double delta_a = a_avg - qa->data[lin];
+        const double delta_b = b_avg - qb->data[lin];
+        pa->data[nn] += coeff_a * (1 - coeff_f) * ds * delta_b;
+        pa->data[nn + 1] += coeff_a * coeff_f * ds * delta_b;
+        pb->data[nn] += coeff_b * (1 - coeff_f) * ds * delta_a;
+        pb->data[nn + 1] += coeff_b * coeff_f * ds * delta_a;
+        qa->data[lin] = a_avg;
+        qb->data[lin] = b_avg;
+      }
+    }
+  }
+  return 0;
+}
+
 /* ------------------------------------------------------- movie monitor */
 
 /* MovieMonitor<F>::update (movie_monitor.cu:17-45): every `interval` steps
@@ -803,6 +866,20 @@ int xfo_run(xf_scene* s, int64_t n_steps) {
       for (int32_t f = 0; f < nf->n_freq; ++f)
         for (int face = 0; face < 6; ++face)
           nf2ff_face(s, nf, &nf->freq[f], face, t);
+    }
+    /* NF2FFTimeDoaminAgency::update: faces XN..ZP, E then H (:192-223) */
+    for (int32_t q = 0; q < s->n_nf2ff_td; ++q) {
+      if (s->dimension != 3) {
+        snprintf(g_err, sizeof g_err, "nf2ff_td needs a 3-D grid");
+        return 1;
+      }
+      for (int face = 0; face < 6; ++face)
+        for (int e = 1; e >= 0; --e)
+          if (nf2ff_td_face(s, &s->nf2ff_td[q], face, e, t)) {
+            snprintf(g_err, sizeof g_err, "nf2ff_td: time bin outside the potential arrays at step %lld",
+                     (long long)t);
+            return 1;
+          }
     }
     /* nextStep */
     s->current_step += 1;

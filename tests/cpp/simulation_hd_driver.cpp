@@ -2,17 +2,20 @@
 // (test/dielectric_sphere_scatter.cu:20-107) against this repository's
 // xfdtd::cuda::SimulationHD, as a user switching libraries would write it.
 //
-// usage: simulation_hd_driver <n_cells> <steps> <out.bin>
+// usage: simulation_hd_driver <n_cells> <steps> <out.bin> [data_dir]
 // Builds an n^3 vacuum cube at dl = 5e-3 with a dielectric sphere
 // (r = 0.1 m scaled to the cube, eps_r 3, mu_r 2), 6 CPML faces of 6 cells,
 // a Gaussian TFSF plane wave at a 9-cell inset and a 1 GHz NF2FF box at a
 // 7-cell inset; runs it through SimulationHD; writes the grid size, the
-// final Ex and the xz far-field RCS cut to out.bin.
+// final Ex and the xz far-field RCS cut to out.bin.  With data_dir it also
+// writes the driver's files (dielectric_sphere_scatter.cu:85-99): the xz
+// and yz far-field cuts under data_dir/fd/, time.npy and incident_wave.npy.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <iostream>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "../../include/xfdtd_cuda_amd/simulation_hd.h"
@@ -67,6 +70,17 @@ int main(int argc, char** argv) {
     std::vector<double> theta(360);
     for (int i = 0; i < 360; ++i) theta[i] = -xfdtd::constant::PI + 2 * xfdtd::constant::PI * i / 359.0;
     auto ff = nffft->processFarField(0, theta, 0.0, inc);
+
+    if (argc > 4) {
+      const std::string data_path = argv[4];
+      std::vector<double> cut(360);
+      for (int i = 0; i < 360; ++i) cut[i] = -xfdtd::constant::PI + 2 * xfdtd::constant::PI * i / 359.0;
+      nffft->setOutputDir(data_path + "/fd");
+      nffft->processFarField(cut, 0, "xz");
+      nffft->processFarField(cut, xfdtd::constant::PI * 0.5, "yz");
+      xfdtd::dumpNpy(data_path + "/time.npy", tfsf->waveform()->time());
+      xfdtd::dumpNpy(data_path + "/incident_wave.npy", tfsf->waveform()->value());
+    }
 
     std::FILE* f = std::fopen(argv[3], "wb");
     if (!f) throw std::runtime_error("cannot open output");

@@ -266,7 +266,8 @@ def test_cpp_dropin_driver(gpu_available, tmp_path):
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "simulation_hd_driver")
     out = tmp_path / "drv.bin"
-    r = subprocess.run([exe, "20", "90", str(out)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, "20", "90", str(out), str(tmp_path / "data")], capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     raw = np.fromfile(out, dtype=np.int64, count=4)
     nx, ny, nz, steps = (int(v) for v in raw)
@@ -282,6 +283,18 @@ def test_cpp_dropin_driver(gpu_available, tmp_path):
     theta = -np.pi + 2 * np.pi * np.arange(360) / 359.0
     _, _, rcs_c = c.far_field(0, 0, theta, 0.0, c.incident_spectrum(1e9))
     assert rel_l2(rcs, rcs_c) <= 1e-10
+    # the driver's files: far-field cuts, time axis and incident wave
+    d = tmp_path / "data"
+    t, v = c.waveform_samples()
+    assert np.array_equal(np.load(d / "time.npy"), t)
+    assert np.array_equal(np.load(d / "incident_wave.npy"), v)
+    for sub, phi in (("xz", 0.0), ("yz", np.pi / 2)):
+        et_c, ep_c, _ = c.far_field(0, 0, theta, phi)
+        assert np.array_equal(np.load(d / "fd" / sub / "theta.npy"), theta)
+        for name, ref in (("e_theta", et_c), ("e_phi", ep_c)):
+            got = np.load(d / "fd" / sub / f"{name}.npy")
+            assert got.dtype == np.complex128 and got.shape == (1, 360)
+            assert np.linalg.norm(got[0] - ref) <= 1e-10 * np.linalg.norm(ref), (sub, name)
 
 
 def test_movie_monitors_match_oracle(gpu_available):

@@ -194,6 +194,12 @@ def host() -> ctypes.CDLL:
                                    p3i, p3i],
             "xfh_sim_add_movie_monitor": [c_vp, c_i32, p3d, p3d, c_i64, ctypes.c_char_p],
             "xfh_sim_output_movies": [c_vp, ctypes.POINTER(c_i64)],
+            "xfh_sim_add_nf2ff_td": [c_vp, p3i, c_f64, c_f64],
+            "xfh_sim_far_field_td": [c_vp, c_i32, ctypes.POINTER(c_i64), p3d, p3d, p3d],
+            "xfh_sim_output_far_field": [c_vp, c_i32, p3d, c_i64, p3d, c_i64, ctypes.c_char_p,
+                                         ctypes.c_char_p],
+            "xfh_sim_output_far_field_td": [c_vp, c_i32, ctypes.c_char_p],
+            "xfh_sim_output_waveform": [c_vp, ctypes.c_char_p],
         }
         for name, args in sigs.items():
             fn = getattr(lib, name)

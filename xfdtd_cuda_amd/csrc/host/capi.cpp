@@ -2,6 +2,7 @@
 // (include/xfdtd_host.h).
 #include <cmath>
 #include <cstring>
+#include <filesystem>
 #include <memory>
 #include <string>
 
@@ -262,6 +263,15 @@ int xfh_sim_add_nf2ff_fd(xfh_sim* s, const int64_t st[3], const double* freqs, i
   });
 }
 
+int xfh_sim_add_nf2ff_td(xfh_sim* s, const int64_t st[3], double theta, double phi) {
+  return guard([&] {
+    if (!s || !st) return fail("null argument");
+    if (!std::isfinite(theta) || !std::isfinite(phi)) return fail("bad observation direction");
+    s->sim->addNF2FF(std::make_shared<xfdtd::NFFFTTimeDomain>(st[0], st[1], st[2], theta, phi));
+    return 0;
+  });
+}
+
 int xfh_sim_add_point_source(xfh_sim* s, int32_t comp, const int64_t idx[3],
                              int32_t wave_kind, double tau, double t0, double freq,
                              double amplitude) {
@@ -413,6 +423,18 @@ int xfh_scene_array(const xf_scene* sc, const char* name, double** data, int64_t
     } else if (h == "point" && parts.size() == 3 && parts[2] == "value") {
       const int p = std::stoi(parts[1]);
       if (p >= 0 && p < sc->n_point) a = &sc->point[p].value;
+    } else if (h == "nf2ff_td" && parts.size() >= 4) {
+      const int q = std::stoi(parts[1]);
+      const int face = faceIndex(parts[2]);
+      if (q >= 0 && q < sc->n_nf2ff_td && face >= 0) {
+        const xf_nf2ff_td& td = sc->nf2ff_td[q];
+        static const char* pn[4] = {"wa", "wb", "ua", "ub"};
+        static const char* qn[4] = {"ea", "eb", "ha", "hb"};
+        for (int c = 0; c < 4; ++c) {
+          if (parts.size() == 4 && parts[3] == pn[c]) a = &td.pot[face][c];
+          if (parts.size() == 5 && parts[3] == "prev" && parts[4] == qn[c]) a = &td.prev[face][c];
+        }
+      }
     } else if (h == "nf2ff" && parts.size() >= 4) {
       const int q = std::stoi(parts[1]);
       const int f = std::stoi(parts[2]);
@@ -475,6 +497,74 @@ int xfh_sim_far_field(xfh_sim* s, int32_t q, int32_t f, const double* theta, int
       if (rcs) rcs[i] = ff.rcs[i];
     }
     return 0;
+  });
+}
+
+int xfh_sim_far_field_td(xfh_sim* s, int32_t q, int64_t* n, double* time, double* e_theta,
+                         double* e_phi) {
+  return guard([&] {
+    if (!s || !n) return fail("null argument");
+    std::vector<xfdtd::NFFFTTimeDomain*> boxes;
+    for (auto& b : s->sim->nf2ffs())
+      if (auto p = dynamic_cast<xfdtd::NFFFTTimeDomain*>(b.get())) boxes.push_back(p);
+    if (q < 0 || q >= static_cast<int32_t>(boxes.size())) return fail("bad NF2FF-TD index");
+    *n = static_cast<int64_t>(boxes[q]->potentialSize());
+    if (!time && !e_theta && !e_phi) return 0;
+    auto ff = boxes[q]->farField();
+    for (int64_t i = 0; i < *n; ++i) {
+      if (time) time[i] = ff.time[i];
+      if (e_theta) e_theta[i] = ff.e_theta[i];
+      if (e_phi) e_phi[i] = ff.e_phi[i];
+    }
+    return 0;
+  });
+}
+
+int xfh_sim_output_far_field(xfh_sim* s, int32_t q, const double* theta, int64_t n_theta,
+                             const double* phi, int64_t n_phi, const char* sub_dir,
+                             const char* output_dir) {
+  return guard([&] {
+    if (!s || !theta || !phi || !sub_dir) return fail("null argument");
+    if (n_theta < 1 || n_phi < 1 || (n_theta > 1 && n_phi > 1))
+      return fail("give a theta cut at one phi or a phi cut at one theta");
+    std::vector<xfdtd::NFFFTFrequencyDomain*> boxes;
+    for (auto& b : s->sim->nf2ffs())
+      if (auto p = dynamic_cast<xfdtd::NFFFTFrequencyDomain*>(b.get())) boxes.push_back(p);
+    if (q < 0 || q >= static_cast<int32_t>(boxes.size())) return fail("bad NF2FF index");
+    if (output_dir) boxes[q]->setOutputDir(output_dir);
+    if (n_phi == 1)
+      boxes[q]->processFarField(std::vector<double>(theta, theta + n_theta), phi[0], sub_dir);
+    else
+      boxes[q]->processFarField(theta[0], std::vector<double>(phi, phi + n_phi), sub_dir);
+    return 0;
+  });
+}
+
+int xfh_sim_output_far_field_td(xfh_sim* s, int32_t q, const char* output_dir) {
+  return guard([&] {
+    if (!s) return fail("null sim");
+    std::vector<xfdtd::NFFFTTimeDomain*> boxes;
+    for (auto& b : s->sim->nf2ffs())
+      if (auto p = dynamic_cast<xfdtd::NFFFTTimeDomain*>(b.get())) boxes.push_back(p);
+    if (q < 0 || q >= static_cast<int32_t>(boxes.size())) return fail("bad NF2FF-TD index");
+    if (output_dir) boxes[q]->setOutputDir(output_dir);
+    boxes[q]->processFarField();
+    return 0;
+  });
+}
+
+int xfh_sim_output_waveform(xfh_sim* s, const char* output_dir) {
+  return guard([&] {
+    if (!s || !output_dir) return fail("null argument");
+    for (auto& w : s->sim->waveformSources()) {
+      auto t = dynamic_cast<xfdtd::TFSF*>(w.get());
+      if (!t) continue;
+      const auto dir = std::filesystem::path(output_dir);
+      xfdtd::dumpNpy((dir / "time.npy").string(), t->waveform()->time());
+      xfdtd::dumpNpy((dir / "incident_wave.npy").string(), t->waveform()->value());
+      return 0;
+    }
+    return fail("no TFSF source");
   });
 }
 

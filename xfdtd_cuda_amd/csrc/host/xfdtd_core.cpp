@@ -268,17 +268,55 @@ TFSF::TFSF(Index start_x, Index start_y, Index start_z, Real theta, Real phi,
 
 // ------------------------------------------------------------------ NF2FF
 NFFFTFrequencyDomain::NFFFTFrequencyDomain(Index sx, Index sy, Index sz,
-                                           std::vector<Real> frequencies)
-    : _sx(sx), _sy(sy), _sz(sz), _freqs(std::move(frequencies)) {
+                                           std::vector<Real> frequencies, std::string output_dir)
+    : _sx(sx), _sy(sy), _sz(sz), _freqs(std::move(frequencies)), _output_dir(std::move(output_dir)) {
   if (_freqs.empty()) throw XFDTDException("NFFFTFrequencyDomain: no frequency");
 }
 
 NFFFTFrequencyDomain::FarField NFFFTFrequencyDomain::processFarField(
     Index freq_index, const std::vector<Real>& theta, Real phi,
     std::complex<Real> incident) const {
+  return farField(freq_index, theta, std::vector<Real>(theta.size(), phi), incident);
+}
+
+void NFFFTFrequencyDomain::processFarField(const std::vector<Real>& theta, Real phi,
+                                           const std::string& sub_dir) const {
+  writeFarField(theta, std::vector<Real>(theta.size(), phi), sub_dir);
+}
+
+void NFFFTFrequencyDomain::processFarField(Real theta, const std::vector<Real>& phi,
+                                           const std::string& sub_dir) const {
+  writeFarField(std::vector<Real>(phi.size(), theta), phi, sub_dir);
+}
+
+void NFFFTFrequencyDomain::writeFarField(const std::vector<Real>& theta, const std::vector<Real>& phi,
+                                         const std::string& sub_dir) const {
+  const std::filesystem::path dir = std::filesystem::path(_output_dir.empty() ? "." : _output_dir) / sub_dir;
+  const int64_t nf = static_cast<int64_t>(_freqs.size()), na = static_cast<int64_t>(theta.size());
+  std::vector<Real> et(static_cast<size_t>(2 * nf * na)), ep(et.size());
+  for (int64_t f = 0; f < nf; ++f) {
+    const FarField ff = farField(static_cast<Index>(f), theta, phi, {0.0, 0.0});
+    for (int64_t n = 0; n < na; ++n) {
+      et[2 * (f * na + n)] = ff.e_theta[n].real();
+      et[2 * (f * na + n) + 1] = ff.e_theta[n].imag();
+      ep[2 * (f * na + n)] = ff.e_phi[n].real();
+      ep[2 * (f * na + n) + 1] = ff.e_phi[n].imag();
+    }
+  }
+  dumpNpy((dir / "theta.npy").string(), theta);
+  dumpNpy((dir / "phi.npy").string(), phi);
+  dumpNpy((dir / "freq.npy").string(), _freqs);
+  dumpNpy((dir / "e_theta.npy").string(), et.data(), {nf, na}, true);
+  dumpNpy((dir / "e_phi.npy").string(), ep.data(), {nf, na}, true);
+}
+
+NFFFTFrequencyDomain::FarField NFFFTFrequencyDomain::farField(
+    Index freq_index, const std::vector<Real>& theta, const std::vector<Real>& phis,
+    std::complex<Real> incident) const {
   using C = std::complex<Real>;
   if (!_grid) throw XFDTDException("processFarField: simulation not initialised");
   if (freq_index >= _freqs.size()) throw XFDTDException("processFarField: bad frequency index");
+  if (phis.size() != theta.size()) throw XFDTDException("processFarField: theta and phi lengths differ");
   const Real f = _freqs[freq_index];
   const Real k0 = 2 * PI * f / C_0;
   const GridSpace& g = *_grid;
@@ -289,6 +327,7 @@ NFFFTFrequencyDomain::FarField NFFFTFrequencyDomain::processFarField(
   const Real dx = g.basedDx(), dy = g.basedDy(), dz = g.basedDz();
   for (std::size_t n = 0; n < theta.size(); ++n) {
     const Real th = theta[n];
+    const Real phi = phis[n];
     const Real rx = std::sin(th) * std::cos(phi), ry = std::sin(th) * std::sin(phi),
                rz = std::cos(th);
     C nv[3] = {0, 0, 0}, lv[3] = {0, 0, 0};
@@ -336,6 +375,61 @@ NFFFTFrequencyDomain::FarField NFFFTFrequencyDomain::processFarField(
     out.e_theta[n] = e_th;
     out.e_phi[n] = e_ph;
     out.rcs[n] = 4 * PI * (std::norm(e_th) + std::norm(e_ph));
+  }
+  return out;
+}
+
+NFFFTTimeDomain::NFFFTTimeDomain(Index sx, Index sy, Index sz, Real theta, Real phi)
+    : _sx(sx), _sy(sy), _sz(sz), _theta(theta), _phi(phi) {
+  _r_unit = Vector{std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi),
+                   std::cos(theta)};
+}
+
+NFFFTTimeDomain::NFFFTTimeDomain(Index sx, Index sy, Index sz, std::string output_dir, Real theta,
+                                 Real phi)
+    : NFFFTTimeDomain(sx, sy, sz, theta, phi) {
+  _output_dir = std::move(output_dir);
+}
+
+void NFFFTTimeDomain::processFarField() const {
+  const FarField ff = farField();
+  const std::filesystem::path dir(_output_dir.empty() ? "." : _output_dir);
+  dumpNpy((dir / "time.npy").string(), ff.time);
+  dumpNpy((dir / "e_theta.npy").string(), ff.e_theta);
+  dumpNpy((dir / "e_phi.npy").string(), ff.e_phi);
+}
+
+NFFFTTimeDomain::FarField NFFFTTimeDomain::farField() const {
+  if (_pot.empty()) throw XFDTDException("processFarField: simulation not initialised");
+  FarField out;
+  out.time.resize(_n_pot);
+  out.e_theta.resize(_n_pot);
+  out.e_phi.resize(_n_pot);
+  const Real scale = 1.0 / (4 * PI * C_0 * _dt);
+  const Real ct = std::cos(_theta), st = std::sin(_theta), cp = std::cos(_phi),
+             sp = std::sin(_phi);
+  for (Index m = 0; m < _n_pot; ++m) {
+    Real w[3] = {0, 0, 0}, u[3] = {0, 0, 0};
+    for (int face = 0; face < 6; ++face) {
+      const xf_task& t = _desc.face_task[face];
+      if (!(t.x.start < t.x.end && t.y.start < t.y.end && t.z.start < t.z.end)) continue;
+      const int c = face / 2, a = (c + 1) % 3, b = (c + 2) % 3;
+      w[a] += _pot[face * 4 + 0][m];
+      w[b] += _pot[face * 4 + 1][m];
+      u[a] += _pot[face * 4 + 2][m];
+      u[b] += _pot[face * 4 + 3][m];
+    }
+    for (int d = 0; d < 3; ++d) {
+      w[d] *= scale;
+      u[d] *= scale;
+    }
+    const Real w_th = w[0] * ct * cp + w[1] * ct * sp - w[2] * st;
+    const Real w_ph = -w[0] * sp + w[1] * cp;
+    const Real u_th = u[0] * ct * cp + u[1] * ct * sp - u[2] * st;
+    const Real u_ph = -u[0] * sp + u[1] * cp;
+    out.time[m] = _dt * static_cast<Real>(m);
+    out.e_theta[m] = -(ETA_0 * w_th + u_ph);
+    out.e_phi[m] = -ETA_0 * w_ph + u_th;
   }
   return out;
 }
@@ -854,16 +948,15 @@ void Simulation::buildTFSF(TFSF& tf) {
   dsc.h_inc = tf._h_inc.view();
 }
 
-void Simulation::buildNF2FF(NFFFTFrequencyDomain& nf) {
-  const GridSpace& g = *_grid;
-  nf._grid = _grid.get();
+namespace {
+// The six one-cell-thick surface tasks of a Huygens box at an inset of
+// (sx, sy, sz) cells (nodeTaskSurfaceXN..ZP); z faces are empty on a 2-D grid.
+void huygensFaces(const GridSpace& g, Index sx, Index sy, Index sz, xf_task* face_task) {
   const int64_t n[3] = {static_cast<int64_t>(g.sizeX()), static_cast<int64_t>(g.sizeY()),
                         static_cast<int64_t>(g.sizeZ())};
   const bool two_d = g.dimension() == GridSpace::Dimension::TWO;
-  const int64_t s[3] = {static_cast<int64_t>(nf._sx), static_cast<int64_t>(nf._sy),
-                        static_cast<int64_t>(nf._sz)};
-  xf_nf2ff& dsc = nf._desc;
-  dsc = xf_nf2ff{};
+  const int64_t s[3] = {static_cast<int64_t>(sx), static_cast<int64_t>(sy),
+                        static_cast<int64_t>(sz)};
   for (int face = 0; face < 6; ++face) {
     const int axis = face / 2;
     xf_range r[3];
@@ -881,8 +974,17 @@ void Simulation::buildNF2FF(NFFFTFrequencyDomain& nf) {
     const int64_t c = face % 2 == 0 ? r[axis].start : r[axis].end;
     r[axis] = {c, c + 1};
     if (empty) r[0] = r[1] = r[2] = {0, 0};
-    dsc.face_task[face] = {r[0], r[1], r[2]};
+    face_task[face] = {r[0], r[1], r[2]};
   }
+}
+}  // namespace
+
+void Simulation::buildNF2FF(NFFFTFrequencyDomain& nf) {
+  const GridSpace& g = *_grid;
+  nf._grid = _grid.get();
+  xf_nf2ff& dsc = nf._desc;
+  dsc = xf_nf2ff{};
+  huygensFaces(g, nf._sx, nf._sy, nf._sz, dsc.face_task);
   const Index nt = _time._size;
   const Index nfreq = nf._freqs.size();
   nf._cur.clear();
@@ -1018,14 +1120,24 @@ MovieMonitor::MovieMonitor(std::unique_ptr<FieldMonitor> frame, Index frame_inte
   if (_interval < 1) throw XFDTDException("MovieMonitor: frame interval must be >= 1");
 }
 
-namespace {
-// One 3-D array of doubles as a version-1.0 .npy file.
-void writeNpy3(const std::string& path, const double* data, const int64_t shape[3]) {
-  std::string header = "{'descr': '<f8', 'fortran_order': False, 'shape': (" + std::to_string(shape[0]) + ", " +
-                       std::to_string(shape[1]) + ", " + std::to_string(shape[2]) + "), }";
+// An array of doubles (or of complex doubles, interleaved) as a
+// version-1.0 .npy file: magic, header length, a dict header padded to a
+// multiple of 64 bytes, then the data in C order.
+void dumpNpy(const std::string& path, const double* data, const std::vector<int64_t>& shape,
+             bool is_complex) {
+  std::string dims;
+  int64_t count = 1;
+  for (size_t d = 0; d < shape.size(); ++d) {
+    dims += std::to_string(shape[d]) + (shape.size() == 1 || d + 1 < shape.size() ? ", " : "");
+    count *= shape[d];
+  }
+  std::string header = std::string("{'descr': '") + (is_complex ? "<c16" : "<f8") +
+                       "', 'fortran_order': False, 'shape': (" + dims + "), }";
   const size_t total = 10 + header.size() + 1;
   header.append((64 - total % 64) % 64, ' ');
   header.push_back('\n');
+  const auto parent = std::filesystem::path(path).parent_path();
+  if (!parent.empty()) std::filesystem::create_directories(parent);
   FILE* f = std::fopen(path.c_str(), "wb");
   if (!f) throw XFDTDException("cannot write " + path);
   const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
@@ -1034,8 +1146,19 @@ void writeNpy3(const std::string& path, const double* data, const int64_t shape[
   std::fwrite(magic, 1, 8, f);
   std::fwrite(len, 1, 2, f);
   std::fwrite(header.data(), 1, header.size(), f);
-  std::fwrite(data, sizeof(double), static_cast<size_t>(shape[0] * shape[1] * shape[2]), f);
+  const size_t n = static_cast<size_t>(count) * (is_complex ? 2 : 1);
+  const bool ok = std::fwrite(data, sizeof(double), n, f) == n;
   std::fclose(f);
+  if (!ok) throw XFDTDException("short write to " + path);
+}
+
+void dumpNpy(const std::string& path, const std::vector<Real>& v) {
+  dumpNpy(path, v.data(), {static_cast<int64_t>(v.size())});
+}
+
+namespace {
+void writeNpy3(const std::string& path, const double* data, const int64_t shape[3]) {
+  dumpNpy(path, data, {shape[0], shape[1], shape[2]});
 }
 }  // namespace
 
@@ -1173,6 +1296,76 @@ void Simulation::buildLearnedBoundary(LearnedBoundary& l) {
   }
 }
 
+void Simulation::buildNF2FFTD(NFFFTTimeDomain& nf) {
+  const GridSpace& g = *_grid;
+  if (g.dimension() != GridSpace::Dimension::THREE)
+    throw XFDTDException("NF2FF time domain needs a 3-D grid");
+  xf_nf2ff_td& dsc = nf._desc;
+  dsc = xf_nf2ff_td{};
+  huygensFaces(g, nf._sx, nf._sy, nf._sz, dsc.face_task);
+  const int64_t n[3] = {static_cast<int64_t>(g.sizeX()), static_cast<int64_t>(g.sizeY()),
+                        static_cast<int64_t>(g.sizeZ())};
+  const Real d[3] = {g.basedDx(), g.basedDy(), g.basedDz()};
+  const Real o[3] = {g.origin().x, g.origin().y, g.origin().z};
+  for (int ax = 0; ax < 3; ++ax) {
+    nf._e_node[ax] = Array({n[ax] + 1});
+    nf._h_node[ax] = Array({n[ax]});
+    nf._e_size[ax] = Array({n[ax]});
+    for (int64_t i = 0; i <= n[ax]; ++i) nf._e_node[ax][i] = o[ax] + d[ax] * static_cast<Real>(i);
+    for (int64_t i = 0; i < n[ax]; ++i) {
+      nf._h_node[ax][i] = o[ax] + d[ax] * (static_cast<Real>(i) + 0.5);
+      nf._e_size[ax][i] = nf._e_node[ax][i + 1] - nf._e_node[ax][i];
+    }
+  }
+  const Real u[3] = {nf._r_unit.x, nf._r_unit.y, nf._r_unit.z};
+  Real dmin = std::numeric_limits<Real>::infinity(), dmax = -dmin;
+  for (int face = 0; face < 6; ++face) {
+    const xf_task& t = dsc.face_task[face];
+    const int c = face / 2;
+    for (int64_t i = t.x.start; i < t.x.end; ++i)
+      for (int64_t j = t.y.start; j < t.y.end; ++j)
+        for (int64_t k = t.z.start; k < t.z.end; ++k) {
+          const int64_t idx[3] = {i, j, k};
+          Real r[3];
+          for (int ax = 0; ax < 3; ++ax)
+            r[ax] = ax == c ? nf._e_node[ax][idx[ax]] : nf._h_node[ax][idx[ax]];
+          const Real p = r[0] * u[0] + r[1] * u[1] + r[2] * u[2];
+          dmin = std::min(dmin, p);
+          dmax = std::max(dmax, p);
+        }
+  }
+  nf._dmin = dmin;
+  nf._dmax = dmax;
+  nf._dt = _time._dt;
+  nf._n_pot = _time._size + static_cast<Index>(std::ceil((dmax - dmin) / (C_0 * _time._dt))) + 2;
+  nf._pot.clear();
+  nf._prev.clear();
+  for (int face = 0; face < 6; ++face) {
+    const xf_task& t = dsc.face_task[face];
+    const int c = face / 2, a = (c + 1) % 3, b = (c + 2) % 3;
+    const int64_t ext[3] = {t.x.end - t.x.start, t.y.end - t.y.start, t.z.end - t.z.start};
+    for (int q = 0; q < 4; ++q) {
+      nf._pot.emplace_back(std::vector<int64_t>{static_cast<int64_t>(nf._n_pot)});
+      nf._prev.emplace_back(std::vector<int64_t>{ext[a], ext[b]});
+    }
+  }
+  dsc.r_unit[0] = u[0];
+  dsc.r_unit[1] = u[1];
+  dsc.r_unit[2] = u[2];
+  dsc.dmax_e = dmax;
+  dsc.dmax_h = dmax;
+  for (int ax = 0; ax < 3; ++ax) {
+    dsc.e_node[ax] = nf._e_node[ax].view();
+    dsc.h_node[ax] = nf._h_node[ax].view();
+    dsc.e_size[ax] = nf._e_size[ax].view();
+  }
+  for (int face = 0; face < 6; ++face)
+    for (int q = 0; q < 4; ++q) {
+      dsc.pot[face][q] = nf._pot[face * 4 + q].view();
+      dsc.prev[face][q] = nf._prev[face * 4 + q].view();
+    }
+}
+
 void Simulation::fillScene() {
   const GridSpace& g = *_grid;
   _scene = xf_scene{};
@@ -1220,6 +1413,11 @@ void Simulation::fillScene() {
   }
   _scene.n_nf2ff = static_cast<int32_t>(_nf_desc.size());
   _scene.nf2ff = _nf_desc.empty() ? nullptr : _nf_desc.data();
+  _nftd_desc.clear();
+  for (auto& n : _nf2ffs)
+    if (auto f = dynamic_cast<NFFFTTimeDomain*>(n.get())) _nftd_desc.push_back(f->desc());
+  _scene.n_nf2ff_td = static_cast<int32_t>(_nftd_desc.size());
+  _scene.nf2ff_td = _nftd_desc.empty() ? nullptr : _nftd_desc.data();
   _scene.ade = _ade ? &_ade->desc() : nullptr;
   _pt_desc.clear();
   for (auto& w : _sources)
@@ -1270,6 +1468,7 @@ void Simulation::init(Index time_step) {
   for (auto& n : _nf2ffs) {
     auto f = dynamic_cast<NFFFTFrequencyDomain*>(n.get());
     if (f) buildNF2FF(*f);
+    if (auto td = dynamic_cast<NFFFTTimeDomain*>(n.get())) buildNF2FFTD(*td);
   }
   buildADE();
   for (auto& w : _sources)

@@ -123,9 +123,10 @@ int Batch::create(xf_scene* const* scenes, int n, const xf_plan* pl) {
     Engine& E = h->eng;
     if (E.dim != 3) return fail(XF_ERR_UNSUPPORTED, "scene " + std::to_string(g) + ": batches take 3-D grids only");
     if (E.has_ade) return fail(XF_ERR_UNSUPPORTED, "scene " + std::to_string(g) + ": ADE media are not batched");
-    if (E.has_lb || E.sc->n_point > 0 || E.sc->n_movie > 0)
-      return fail(XF_ERR_UNSUPPORTED,
-                  "scene " + std::to_string(g) + ": point sources, learned layers and movies are not batched");
+    if (E.has_lb || E.sc->n_point > 0 || E.sc->n_movie > 0 || E.sc->n_nf2ff_td > 0)
+      return fail(XF_ERR_UNSUPPORTED, "scene " + std::to_string(g) +
+                                          ": point sources, learned layers, movies and NF2FF time "
+                                          "domain are not batched");
     if (g == 0) cur_step = E.cur_step;
     if (E.cur_step != cur_step) return fail(XF_ERR_INVALID, "scenes of a batch must share the current step");
     if (E.sc->tfsf || E.sc->n_nf2ff > 0) nt = std::min(nt, E.nt);
@@ -494,7 +495,7 @@ int Batch::autotune(int trial_steps, xf_tuning* best) {
 int Engine::autotune(int trial_steps, xf_tuning* best) {
   if (!uploaded) return fail(XF_ERR_STATE, "autotune before upload");
   if (trial_steps < 1) return fail(XF_ERR_INVALID, "trial_steps must be >= 1");
-  if ((sc->tfsf || !nf.empty() || pts.n > 0) && cur_step + trial_steps + 1 > nt)
+  if ((sc->tfsf || !nf.empty() || !nftd.empty() || pts.n > 0) && cur_step + trial_steps + 1 > nt)
     return fail(XF_ERR_INVALID, "not enough time-table steps left to tune");
   XF_TRY(snapshot());
   const int64_t step0 = cur_step;

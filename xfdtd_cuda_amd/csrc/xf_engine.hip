@@ -133,6 +133,8 @@ struct Engine {
     NfPass pass;
   };
   std::vector<NfDev> nf;
+  // NF2FF time domain
+  std::vector<NtPass> nftd;
   // ADE
   AdeDev ade{};
   // point sources and the learned layer (additions)
@@ -229,6 +231,8 @@ struct Engine {
   int upload_pml();
   int upload_tfsf();
   int upload_nf2ff();
+  int plan_nf2ff_td(const xf_nf2ff_td& T, NtPass& P, int64_t* prev_total) const;
+  int upload_nf2ff_td();
   int upload_ade();
   int validate_additions();
   int upload_additions();
@@ -399,7 +403,123 @@ int Engine::validate() {
     }
     has_ade = true;
   }
+  if (sc->n_nf2ff_td < 0 || (sc->n_nf2ff_td > 0 && !sc->nf2ff_td))
+    return fail(XF_ERR_INVALID, "nf2ff_td boxes missing");
+  for (int32_t q = 0; q < sc->n_nf2ff_td; ++q) {
+    NtPass P;
+    int64_t prev_total = 0;
+    XF_TRY(plan_nf2ff_td(sc->nf2ff_td[q], P, &prev_total));
+  }
   return validate_additions();
+}
+
+// Geometry of an NF2FF time-domain box: faces, tiles, the bins a tile and a
+// face can touch in one step, and a check that every bin of the remaining
+// steps [cur_step, nt) lies inside the potential arrays.  The r . r_unit
+// ranges use the kernel's own expression, so the kernels' bins are bounded
+// by the ones computed here.
+int Engine::plan_nf2ff_td(const xf_nf2ff_td& T, NtPass& P, int64_t* prev_total) const {
+  P = NtPass{};
+  if (dim != 3) return fail(XF_ERR_UNSUPPORTED, "nf2ff_td: needs a 3-D grid");
+  const int64_t n3[3] = {nx, ny, nz};
+  static const char* ax_name[3] = {"x", "y", "z"};
+  for (int ax = 0; ax < 3; ++ax) {
+    XF_TRY(check_array(T.e_node[ax], {n3[ax] + 1}, std::string("nf2ff_td e_node ") + ax_name[ax]));
+    XF_TRY(check_array(T.h_node[ax], {n3[ax]}, std::string("nf2ff_td h_node ") + ax_name[ax]));
+    XF_TRY(check_array(T.e_size[ax], {n3[ax]}, std::string("nf2ff_td e_size ") + ax_name[ax]));
+  }
+  const double c_dt = 299792458.0 * sc->dt;
+  if (!(c_dt > 0)) return fail(XF_ERR_INVALID, "nf2ff_td: dt must be positive");
+  for (int d = 0; d < 3; ++d) P.u[d] = T.r_unit[d];
+  P.dmax_e = T.dmax_e;
+  P.dmax_h = T.dmax_h;
+  P.c_dt = c_dt;
+  P.n_pot = -1;
+  int blk = 0, rblk = 0, maxspan = 0;
+  int64_t prev_off = 0;
+  for (int face = 0; face < 6; ++face) {
+    NtFace& F = P.face[face];
+    F.blk0 = blk;
+    F.rblk0 = rblk;
+    F.prev_off = prev_off;
+    const xf_task& t = T.face_task[face];
+    if (empty_task(t)) continue;
+    const int c = face / 2, a = (c + 1) % 3, b = (c + 2) % 3;
+    const int uax = std::min(a, b), vax = std::max(a, b);
+    if (task_hi(t, c) - task_lo(t, c) != 1) return fail(XF_ERR_INVALID, "nf2ff_td: face task is not one cell thick");
+    for (int d = 0; d < 3; ++d)
+      if (task_lo(t, d) < 1 || task_hi(t, d) > n3[d] - 1)
+        return fail(XF_ERR_INVALID, "nf2ff_td: face task must stay one cell inside the grid");
+    const int64_t ext[3] = {t.x.end - t.x.start, t.y.end - t.y.start, t.z.end - t.z.start};
+    const std::string nm = "nf2ff_td face " + std::to_string(face);
+    for (int q = 0; q < 4; ++q) {
+      if (P.n_pot < 0) {
+        if (!T.pot[face][q].data || T.pot[face][q].ndim != 1 || T.pot[face][q].shape[0] < 2)
+          return fail(XF_ERR_INVALID, nm + ": potential arrays must be 1-D with at least 2 bins");
+        P.n_pot = T.pot[face][q].shape[0];
+      }
+      XF_TRY(check_array(T.pot[face][q], {P.n_pot}, nm + " potential"));
+      XF_TRY(check_array(T.prev[face][q], {ext[a], ext[b]}, nm + " prev"));
+    }
+    if (P.n_pot > INT_MAX / 2) return fail(XF_ERR_INVALID, nm + ": too many time bins");
+    F.o[0] = static_cast<int>(t.x.start);
+    F.o[1] = static_cast<int>(t.y.start);
+    F.o[2] = static_cast<int>(t.z.start);
+    F.nu = static_cast<int>(ext[uax]);
+    F.nv = static_cast<int>(ext[vax]);
+    F.tu = (F.nu + NT_TILE - 1) / NT_TILE;
+    F.tv = (F.nv + NT_TILE - 1) / NT_TILE;
+    F.nb = static_cast<int>(ext[b]);
+    std::vector<double> tmin(static_cast<size_t>(F.tu) * F.tv, INFINITY), tmax(tmin.size(), -INFINITY);
+    double fmin = INFINITY, fmax = -INFINITY;
+    for (int u = 0; u < F.nu; ++u)
+      for (int v = 0; v < F.nv; ++v) {
+        int64_t idx[3];
+        idx[c] = F.o[c];
+        idx[uax] = F.o[uax] + u;
+        idx[vax] = F.o[vax] + v;
+        double r[3];
+        for (int ax = 0; ax < 3; ++ax) r[ax] = ax == c ? T.e_node[ax].data[idx[ax]] : T.h_node[ax].data[idx[ax]];
+        const double rd = r[0] * P.u[0] + r[1] * P.u[1] + r[2] * P.u[2];
+        if (!std::isfinite(rd)) return fail(XF_ERR_INVALID, nm + ": non-finite surface position");
+        const size_t tl = static_cast<size_t>(u / NT_TILE) * F.tv + v / NT_TILE;
+        tmin[tl] = std::min(tmin[tl], rd);
+        tmax[tl] = std::max(tmax[tl], rd);
+        fmin = std::min(fmin, rd);
+        fmax = std::max(fmax, rd);
+      }
+    for (size_t tl = 0; tl < tmin.size(); ++tl)
+      maxspan = std::max(maxspan, static_cast<int>(std::ceil((tmax[tl] - tmin[tl]) / c_dt)) + 3);
+    F.rd_min = fmin;
+    F.rd_max = fmax;
+    F.bins = static_cast<int>(std::ceil((fmax - fmin) / c_dt)) + 4;
+    F.nrblk = (F.bins + 255) / 256;
+    if (cur_step < nt) {
+      const double offs[2] = {0.5, 0.0};
+      const double dm[2] = {T.dmax_e, T.dmax_h};
+      for (int e = 0; e < 2; ++e) {
+        const double lo = std::floor((dm[e] - fmax) / c_dt + offs[e] + static_cast<double>(cur_step));
+        const double hi = std::floor((dm[e] - fmin) / c_dt + offs[e] + static_cast<double>(nt - 1)) + 1;
+        if (lo < 0 || hi >= static_cast<double>(P.n_pot))
+          return fail(XF_ERR_INVALID, nm + ": time bins [" + std::to_string(lo) + ", " + std::to_string(hi) +
+                                          "] of the run outside the " + std::to_string(P.n_pot) +
+                                          " potential bins (dmax or the potential length)");
+      }
+    }
+    blk += F.tu * F.tv;
+    rblk += F.nrblk;
+    prev_off += 4 * ext[a] * ext[b];
+  }
+  if (maxspan > NT_MAXSPAN)
+    return fail(XF_ERR_UNSUPPORTED, "nf2ff_td: a 16x16 surface tile spans " + std::to_string(maxspan) +
+                                        " time bins (limit " + std::to_string(NT_MAXSPAN) +
+                                        "); cells are too long for c dt");
+  P.nblk = blk;
+  P.nrblk = rblk;
+  P.maxspan = std::max(maxspan, 1);
+  if (P.n_pot < 0) P.n_pot = 0;
+  *prev_total = prev_off;
+  return XF_OK;
 }
 
 // Point sources and the learned layer: every index the kernels will form.
@@ -1122,6 +1242,48 @@ int Engine::upload_additions() {
   return XF_OK;
 }
 
+int Engine::upload_nf2ff_td() {
+  nftd.clear();
+  for (int32_t q = 0; q < sc->n_nf2ff_td; ++q) {
+    const xf_nf2ff_td& T = sc->nf2ff_td[q];
+    NtPass P;
+    int64_t prev_total = 0;
+    XF_TRY(plan_nf2ff_td(T, P, &prev_total));
+    for (int ax = 0; ax < 3; ++ax) {
+      void* p;
+      XF_TRY(upload_vec(&p, std::vector<double>(T.e_node[ax].data, T.e_node[ax].data + numel(T.e_node[ax]))));
+      P.e_node[ax] = static_cast<const double*>(p);
+      XF_TRY(upload_vec(&p, std::vector<double>(T.h_node[ax].data, T.h_node[ax].data + numel(T.h_node[ax]))));
+      P.h_node[ax] = static_cast<const double*>(p);
+      XF_TRY(upload_vec(&p, std::vector<double>(T.e_size[ax].data, T.e_size[ax].data + numel(T.e_size[ax]))));
+      P.e_size[ax] = static_cast<const double*>(p);
+    }
+    std::vector<double> pot(static_cast<size_t>(24 * P.n_pot), 0.0), prev(static_cast<size_t>(prev_total));
+    for (int face = 0; face < 6; ++face) {
+      if (empty_task(T.face_task[face])) continue;
+      const NtFace& F = P.face[face];
+      const int64_t plane = numel(T.prev[face][0]);
+      for (int c = 0; c < 4; ++c) {
+        std::copy(T.pot[face][c].data, T.pot[face][c].data + P.n_pot, pot.begin() + (face * 4 + c) * P.n_pot);
+        std::copy(T.prev[face][c].data, T.prev[face][c].data + plane, prev.begin() + F.prev_off + c * plane);
+      }
+    }
+    void *pp, *pv, *pa, *ph;
+    XF_TRY(upload_vec(&pp, pot));
+    XF_TRY(upload_vec(&pv, prev));
+    state.emplace_back(pp, pot.size() * sizeof(double));
+    state.emplace_back(pv, prev.size() * sizeof(double));
+    XF_TRY(alloc(&pa, static_cast<size_t>(P.nblk) * 4 * P.maxspan * sizeof(double)));
+    XF_TRY(alloc(&ph, static_cast<size_t>(P.nblk) * 4 * sizeof(int)));
+    P.pot = static_cast<double*>(pp);
+    P.prev = static_cast<double*>(pv);
+    P.part = static_cast<double*>(pa);
+    P.hdr = static_cast<int*>(ph);
+    nftd.push_back(P);
+  }
+  return XF_OK;
+}
+
 int Engine::upload() {
   release();
   XF_HIP(hipSetDevice(device));
@@ -1145,6 +1307,7 @@ int Engine::upload() {
   XF_TRY(upload_pml());
   XF_TRY(upload_tfsf());
   XF_TRY(upload_nf2ff());
+  XF_TRY(upload_nf2ff_td());
   XF_TRY(upload_ade());
   XF_TRY(upload_additions());
   void* tp;
@@ -1259,6 +1422,20 @@ int Engine::download() {
           }
         }
       }
+  }
+  for (size_t q = 0; q < nftd.size(); ++q) {
+    const xf_nf2ff_td& T = sc->nf2ff_td[q];
+    const NtPass& P = nftd[q];
+    for (int face = 0; face < 6; ++face) {
+      if (empty_task(T.face_task[face])) continue;
+      const int64_t plane = numel(T.prev[face][0]);
+      for (int c = 0; c < 4; ++c) {
+        XF_HIP(hipMemcpy(T.pot[face][c].data, P.pot + (face * 4 + c) * P.n_pot, P.n_pot * sizeof(double),
+                         hipMemcpyDeviceToHost));
+        XF_HIP(hipMemcpy(T.prev[face][c].data, P.prev + P.face[face].prev_off + c * plane,
+                         plane * sizeof(double), hipMemcpyDeviceToHost));
+      }
+    }
   }
   if (has_ade) {
     const xf_ade& A = *sc->ade;
@@ -1582,6 +1759,13 @@ int Engine::launch_post_e(int t, const int* tp) {
                          make_fields<S>(*this), N.pass, t, tp, make_dims(*this), dim == 2 ? 1 : 0,
                          make_scales<C>(*this));
   }
+  for (const NtPass& P : nftd) {
+    if (P.nblk == 0) continue;
+    const size_t lds = 256 * sizeof(NtRec) + 16 * static_cast<size_t>(P.maxspan) * sizeof(double);
+    hipLaunchKernelGGL((k_nftd_tile<S>), dim3(P.nblk), dim3(256), lds, stream, make_fields<S>(*this), P,
+                       make_dims(*this), make_scales<C>(*this), t, tp);
+    hipLaunchKernelGGL(k_nftd_reduce, dim3(P.nrblk), dim3(256), 0, stream, P, t, tp);
+  }
   XF_HIP(hipGetLastError());
   return XF_OK;
 }
@@ -1609,7 +1793,7 @@ int Engine::run(int64_t n) {
   if (!uploaded) return fail(XF_ERR_STATE, "run before upload");
   if (n < 0) return fail(XF_ERR_INVALID, "negative step count");
   if (n == 0) return XF_OK;
-  if ((sc->tfsf || !nf.empty() || pts.n > 0) && cur_step + n > nt)
+  if ((sc->tfsf || !nf.empty() || !nftd.empty() || pts.n > 0) && cur_step + n > nt)
     return fail(XF_ERR_INVALID, "run past the end of the time axis the TFSF/NF2FF/source tables were built for");
   XF_HIP(hipSetDevice(device));
   const bool timing_on = plan.timing != 0;

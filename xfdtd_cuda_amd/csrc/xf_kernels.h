@@ -13,6 +13,7 @@
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <stdint.h>
 
 #include "../../include/xfdtd_hip.h"
@@ -214,6 +215,42 @@ struct NfPass {
   double2* cur;      // [freq][face cells][4] -> laid out [freq][face][comp][cell]
   const double2* te; // [freq][nt]
   const double2* th;
+};
+
+// NF2FF time domain (one box).  A face's surface cells are cut into tiles
+// of NT_TILE x NT_TILE cells over its two tangential axes (u < v in x, y, z
+// order); one block per tile sums the tile's contributions per time bin
+// (k_nftd_tile), one thread per bin then adds the tiles' partial sums in tile
+// order (k_nftd_reduce).
+constexpr int NT_TILE = 16;
+constexpr int NT_MAXSPAN = 256;  // time bins one tile may touch in a step (LDS: 52 KiB)
+
+struct NtFace {
+  int o[3];         // task origin
+  int nu, nv;       // extents along the tangential axes u < v
+  int tu, tv;       // tiles along u and v
+  int blk0;         // first tile of the face
+  int bins;         // bin slots of the face's reduce (>= bins touched in a step)
+  int rblk0, nrblk; // the face's reduce blocks
+  int64_t prev_off; // first prev value of the face: [ea, eb, ha, hb][n_a][n_b]
+  int nb;           // extent along b (prev row length)
+  double rd_min, rd_max;  // range of r . r_unit over the face's cells
+};
+
+struct NtPass {
+  NtFace face[6];
+  int nblk, nrblk;
+  int maxspan;      // bins of one tile's partial
+  int64_t n_pot;
+  double u[3];
+  double dmax_e, dmax_h, c_dt;
+  const double* e_node[3];
+  const double* h_node[3];
+  const double* e_size[3];
+  double* pot;      // [face][wa, wb, ua, ub][n_pot]
+  double* prev;     // per face at prev_off
+  double* part;     // [tile][wa, wb, ua, ub][maxspan]
+  int* hdr;         // [tile][bmin_e, span_e, bmin_h, span_h]
 };
 
 // Point sources: node index (padded layout) and component of each; the

@@ -352,6 +352,223 @@ __global__ void __launch_bounds__(256)
   nf2ff_cell<S>(f, P, t_dev ? *t_dev : t_arg, d, two_d, sc, tid);
 }
 
+// ------------------------------------------------------ NF2FF time domain
+// One cell's record in a tile: its contributions (wa, wb from H, ua, ub from
+// E; each as the (1 - f) part for bin nn and the f part for bin nn + 1) and
+// its E and H bins relative to the tile's first (-2: no cell).
+struct NtRec {
+  double x[8];
+  int re, rh;
+  int pad[2];
+};
+
+// NF2FFTimeDomainData<D>::update<E> and update<H> (nf2ff_time_domain_data.cu:
+// 103-188) for one tile of one face: prev fields updated in place, the
+// contributions summed per time bin (each wave over its 64 cells in the
+// reference's (i, j, k) order, then the four waves in order) into the tile's
+// partial sums.  Block: 256 threads = 16 x 16 cells; dynamic LDS: 256 records
+// and 4 waves x 4 potentials x maxspan doubles.
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_nftd_tile(Fields<S> f, NtPass P, Dims d, Scales<typename Store<S>::C> sc, int t_arg,
+                const int* t_dev) {
+  extern __shared__ double nt_lds[];
+  __shared__ int red[4][4];
+  NtRec* rec = reinterpret_cast<NtRec*>(nt_lds);
+  double* wpart = nt_lds + 256 * (sizeof(NtRec) / sizeof(double));
+  const int t = t_dev ? *t_dev : t_arg;
+  const int blk = blockIdx.x;
+  int face = 0;
+  while (face < 5 && blk >= P.face[face + 1].blk0) ++face;
+  const NtFace& F = P.face[face];
+  const int c = face >> 1, a = (c + 1) % 3, b = (c + 2) % 3;
+  const int uax = a < b ? a : b, vax = a < b ? b : a;
+  const int na = a == uax ? F.nu : F.nv, nb = b == uax ? F.nu : F.nv;
+  const int tl = blk - F.blk0;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int u = (tl / F.tv) * NT_TILE + (tid >> 4);
+  const int v = (tl % F.tv) * NT_TILE + (tid & (NT_TILE - 1));
+  const bool valid = u < F.nu && v < F.nv;
+  int nn_e = 0, nn_h = 0;
+  double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (valid) {
+    int idx[3];
+    idx[c] = F.o[c];
+    idx[uax] = F.o[uax] + u;
+    idx[vax] = F.o[vax] + v;
+    double r[3];
+    for (int ax = 0; ax < 3; ++ax) r[ax] = ax == c ? P.e_node[ax][idx[ax]] : P.h_node[ax][idx[ax]];
+    const double rd = r[0] * P.u[0] + r[1] * P.u[1] + r[2] * P.u[2];
+    const double ds = P.e_size[a][idx[a]] * P.e_size[b][idx[b]];
+    const bool positive = face & 1;
+    const int64_t plane = (int64_t)na * nb;
+    double* pv = P.prev + F.prev_off + (int64_t)(idx[a] - F.o[a]) * nb + (idx[b] - F.o[b]);
+    {  // E: (ua, ub), offset 0.5
+      double ca = 1.0, cb = -1.0;
+      if (!positive) {
+        ca *= -1.0;
+        cb *= -1.0;
+      }
+      const double tsd = (P.dmax_e - rd) / P.c_dt;
+      const double sn = tsd + 0.5 + (double)t;
+      nn_e = (int)floor(sn);
+      const double cf = sn - (double)nn_e;
+      const double A = (double)nf_interp(f, d, 0, c, true, a, idx[0], idx[1], idx[2], sc);
+      const double B = (double)nf_interp(f, d, 0, c, true, b, idx[0], idx[1], idx[2], sc);
+      const double dA = A - pv[0];
+      const double dB = B - pv[plane];
+      pv[0] = A;
+      pv[plane] = B;
+      x[4] = ca * (1 - cf) * ds * dB;
+      x[5] = ca * cf * ds * dB;
+      x[6] = cb * (1 - cf) * ds * dA;
+      x[7] = cb * cf * ds * dA;
+    }
+    {  // H: (wa, wb), offset 0
+      double ca = -1.0, cb = 1.0;
+      if (!positive) {
+        ca *= -1.0;
+        cb *= -1.0;
+      }
+      const double tsd = (P.dmax_h - rd) / P.c_dt;
+      const double sn = tsd + 0.0 + (double)t;
+      nn_h = (int)floor(sn);
+      const double cf = sn - (double)nn_h;
+      const double A = (double)nf_interp(f, d, 0, c, false, a, idx[0], idx[1], idx[2], sc);
+      const double B = (double)nf_interp(f, d, 0, c, false, b, idx[0], idx[1], idx[2], sc);
+      const double dA = A - pv[2 * plane];
+      const double dB = B - pv[3 * plane];
+      pv[2 * plane] = A;
+      pv[3 * plane] = B;
+      x[0] = ca * (1 - cf) * ds * dB;
+      x[1] = ca * cf * ds * dB;
+      x[2] = cb * (1 - cf) * ds * dA;
+      x[3] = cb * cf * ds * dA;
+    }
+  }
+  // the tile's first and last bins
+  int mn_e = valid ? nn_e : INT_MAX, mx_e = valid ? nn_e : INT_MIN;
+  int mn_h = valid ? nn_h : INT_MAX, mx_h = valid ? nn_h : INT_MIN;
+  for (int o = 32; o > 0; o >>= 1) {
+    mn_e = min(mn_e, __shfl_xor(mn_e, o));
+    mx_e = max(mx_e, __shfl_xor(mx_e, o));
+    mn_h = min(mn_h, __shfl_xor(mn_h, o));
+    mx_h = max(mx_h, __shfl_xor(mx_h, o));
+  }
+  if (lane == 0) {
+    red[wave][0] = mn_e;
+    red[wave][1] = mx_e;
+    red[wave][2] = mn_h;
+    red[wave][3] = mx_h;
+  }
+  __syncthreads();
+  int b0_e = red[0][0], b1_e = red[0][1], b0_h = red[0][2], b1_h = red[0][3];
+  for (int w = 1; w < 4; ++w) {
+    b0_e = min(b0_e, red[w][0]);
+    b1_e = max(b1_e, red[w][1]);
+    b0_h = min(b0_h, red[w][2]);
+    b1_h = max(b1_h, red[w][3]);
+  }
+  const bool any = b0_e != INT_MAX;
+  const int ms = P.maxspan;
+  const int span_e = any ? min(b1_e - b0_e + 2, ms) : 0;
+  const int span_h = any ? min(b1_h - b0_h + 2, ms) : 0;
+  NtRec& R = rec[tid];
+  for (int q = 0; q < 8; ++q) R.x[q] = x[q];
+  R.re = valid ? nn_e - b0_e : -2;
+  R.rh = valid ? nn_h - b0_h : -2;
+  __syncthreads();
+  // per-wave sums: lane m owns bins m, m + 64, ...
+  const int span = max(span_e, span_h);
+  const NtRec* W = rec + wave * 64;
+  double* wp = wpart + wave * 4 * ms;
+  for (int m = lane; m < span; m += 64) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < 64; ++q) {
+      const NtRec& r = W[q];
+      if (r.rh == m) {
+        acc[0] += r.x[0];
+        acc[1] += r.x[2];
+      } else if (r.rh + 1 == m) {
+        acc[0] += r.x[1];
+        acc[1] += r.x[3];
+      }
+      if (r.re == m) {
+        acc[2] += r.x[4];
+        acc[3] += r.x[6];
+      } else if (r.re + 1 == m) {
+        acc[2] += r.x[5];
+        acc[3] += r.x[7];
+      }
+    }
+    for (int p = 0; p < 4; ++p) wp[p * ms + m] = acc[p];
+  }
+  __syncthreads();
+  double* out = P.part + (int64_t)blk * 4 * ms;
+  for (int m = tid; m < span; m += 256)
+    for (int p = 0; p < 4; ++p)
+      out[p * ms + m] = ((wpart[p * ms + m] + wpart[(4 + p) * ms + m]) + wpart[(8 + p) * ms + m]) +
+                        wpart[(12 + p) * ms + m];
+  if (tid == 0) {
+    P.hdr[blk * 4 + 0] = b0_e;
+    P.hdr[blk * 4 + 1] = span_e;
+    P.hdr[blk * 4 + 2] = b0_h;
+    P.hdr[blk * 4 + 3] = span_h;
+  }
+}
+
+// One thread per time bin of a face: the tiles' partial sums for that bin,
+// in tile order, added to the potentials.  The face's first bin comes from
+// the largest r . r_unit (the shortest delay).
+__global__ void __launch_bounds__(256) k_nftd_reduce(NtPass P, int t_arg, const int* t_dev) {
+  __shared__ int4 sh[256];
+  const int t = t_dev ? *t_dev : t_arg;
+  const int blk = blockIdx.x;
+  int face = 0;
+  while (face < 5 && blk >= P.face[face + 1].rblk0) ++face;
+  const NtFace& F = P.face[face];
+  const int slot = (blk - F.rblk0) * 256 + (int)threadIdx.x;
+  const int n_e = (int)floor((P.dmax_e - F.rd_max) / P.c_dt + 0.5 + (double)t) + slot;
+  const int n_h = (int)floor((P.dmax_h - F.rd_max) / P.c_dt + 0.0 + (double)t) + slot;
+  const bool act = slot < F.bins;
+  const int ms = P.maxspan;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  bool any_e = false, any_h = false;
+  const int ntile = F.tu * F.tv;
+  for (int base = 0; base < ntile; base += 256) {
+    const int cnt = min(256, ntile - base);
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) sh[threadIdx.x] = reinterpret_cast<const int4*>(P.hdr)[F.blk0 + base + threadIdx.x];
+    __syncthreads();
+    if (act)
+      for (int q = 0; q < cnt; ++q) {
+        const int4 h = sh[q];
+        const double* pp = P.part + (int64_t)(F.blk0 + base + q) * 4 * ms;
+        const int rh = n_h - h.z;
+        if (rh >= 0 && rh < h.w) {
+          acc[0] += pp[rh];
+          acc[1] += pp[ms + rh];
+          any_h = true;
+        }
+        const int re = n_e - h.x;
+        if (re >= 0 && re < h.y) {
+          acc[2] += pp[2 * ms + re];
+          acc[3] += pp[3 * ms + re];
+          any_e = true;
+        }
+      }
+  }
+  double* pot = P.pot + (int64_t)face * 4 * P.n_pot;
+  if (any_h && n_h >= 0 && n_h < P.n_pot) {
+    pot[n_h] += acc[0];
+    pot[P.n_pot + n_h] += acc[1];
+  }
+  if (any_e && n_e >= 0 && n_e < P.n_pot) {
+    pot[2 * P.n_pot + n_e] += acc[2];
+    pot[3 * P.n_pot + n_e] += acc[3];
+  }
+}
+
 // --------------------------------------------------------- point sources
 // E_comp(node) += value[t], sources in registration order (one thread, so
 // sources sharing a node add in a fixed order).

@@ -29,8 +29,10 @@ from .simulation import FACES, Material, Simulation, Waveform
 def dielectric_sphere(n: int = 140, pml: int = 8, steps: int = 2400, dl: float = 2.5e-3,
                       tfsf_inset: int = 15, nf2ff_inset: int = 11, freqs=(1e9,),
                       radius: float | None = None, eps_r: float = 3.0, mu_r: float = 2.0,
-                      layout: str = "dense", init: bool = True) -> Simulation:
-    """n interior cells per edge (140 reproduces the driver's 0.35 m cube)."""
+                      layout: str = "dense", init: bool = True, td_dirs=()) -> Simulation:
+    """n interior cells per edge (140 reproduces the driver's 0.35 m cube);
+    td_dirs: (theta, phi) observation directions of NF2FF time-domain boxes on
+    the frequency-domain box's surface."""
     edge = n * dl
     half = edge / 2
     radius = (0.1 / 0.35) * edge if radius is None else radius
@@ -43,6 +45,8 @@ def dielectric_sphere(n: int = 140, pml: int = 8, steps: int = 2400, dl: float =
         s.add_pml_all(pml)
     if nf2ff_inset > 0 and freqs:
         s.add_nf2ff_fd((nf2ff_inset,) * 3, list(freqs))
+    for theta, phi in td_dirs:
+        s.add_nf2ff_td((nf2ff_inset,) * 3, theta, phi)
     s.set_coefficient_layout(layout)
     if init:
         s.init(steps)

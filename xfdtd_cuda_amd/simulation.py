@@ -220,6 +220,13 @@ class Simulation:
                            "addNF2FF")
         return self
 
+    def add_nf2ff_td(self, start, theta: float, phi: float):
+        """NF2FF time-domain box (NFFFTTimeDomain) at the inset `start` (cells,
+        as add_nf2ff_fd), observed along (theta, phi)."""
+        _native.check_host(self._lib.xfh_sim_add_nf2ff_td(self._h, _i3(start), theta, phi),
+                           "addNF2FF(time domain)")
+        return self
+
     def add_point_source(self, comp: int, index, waveform: Waveform, amplitude: float = 1.0):
         """Soft source on E_comp (0 x, 1 y, 2 z) at grid index `index` of that
         component's array (no counterpart in the reference's GPU path)."""
@@ -349,6 +356,47 @@ class Simulation:
             incident.imag, et.ctypes.data_as(c_dp), ep.ctypes.data_as(c_dp),
             rcs.ctypes.data_as(c_dp)), "processFarField")
         return et[0::2] + 1j * et[1::2], ep[0::2] + 1j * ep[1::2], rcs
+
+
+    def far_field_td(self, box: int = 0):
+        """NFFFTTimeDomain::processFarField of time-domain box `box`: the time
+        axis t_m = m dt and R E_theta(t_m), R E_phi(t_m)."""
+        n = ctypes.c_int64()
+        _native.check_host(self._lib.xfh_sim_far_field_td(self._h, box, ctypes.byref(n), None,
+                                                           None, None), "processFarField(TD)")
+        t, et, ep = np.zeros(n.value), np.zeros(n.value), np.zeros(n.value)
+        _native.check_host(self._lib.xfh_sim_far_field_td(
+            self._h, box, ctypes.byref(n), t.ctypes.data_as(c_dp), et.ctypes.data_as(c_dp),
+            ep.ctypes.data_as(c_dp)), "processFarField(TD)")
+        return t, et, ep
+
+
+    def output_far_field(self, box: int, theta, phi, sub_dir: str, output_dir: str | None = None):
+        """NFFFTFrequencyDomain::processFarField(theta, phi, sub_dir) of the
+        drivers: one of theta, phi is a scalar; writes theta.npy, phi.npy,
+        freq.npy, e_theta.npy, e_phi.npy (complex, frequencies x directions)
+        under <output_dir>/<sub_dir>/."""
+        th = np.atleast_1d(np.ascontiguousarray(theta, dtype=np.float64))
+        ph = np.atleast_1d(np.ascontiguousarray(phi, dtype=np.float64))
+        _native.check_host(self._lib.xfh_sim_output_far_field(
+            self._h, box, th.ctypes.data_as(c_dp), th.size, ph.ctypes.data_as(c_dp), ph.size,
+            sub_dir.encode(), None if output_dir is None else str(output_dir).encode()),
+            "processFarField")
+        return self
+
+    def output_far_field_td(self, box: int = 0, output_dir: str | None = None):
+        """NFFFTTimeDomain::processFarField(): time.npy, e_theta.npy, e_phi.npy."""
+        _native.check_host(self._lib.xfh_sim_output_far_field_td(
+            self._h, box, None if output_dir is None else str(output_dir).encode()),
+            "processFarField(TD)")
+        return self
+
+    def output_waveform(self, output_dir: str):
+        """time.npy and incident_wave.npy of the TFSF waveform, as the drivers
+        dump them."""
+        _native.check_host(self._lib.xfh_sim_output_waveform(self._h, str(output_dir).encode()),
+                           "output_waveform")
+        return self
 
 
 def scene_array(scene_ptr: int, name: str) -> np.ndarray:
