@@ -21,6 +21,9 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
        python bench.py --workload batch   (64 uneven grids, FP32, batched kernels)
        python bench.py --workload precision --steps 4000   (FP64/FP32/BF16 far-field error)
        python bench.py --workload learned   (learned 2-cell layer: training, reflection, speed)
+       python bench.py --workload nf2ff_td --n 256   (NF2FF time-domain accumulation cost)
+       python bench.py --workload dispersive   (Drude + Lorentz sphere, FP64 vs FP32)
+       python bench.py --workload cylinder2d   (2-D TMz PEC cylinder, FP64 vs FP32)
 """
 from __future__ import annotations
 
@@ -68,11 +71,16 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--workload", default="sphere",
-                    choices=["sphere", "batch", "precision", "learned"],
+                    choices=["sphere", "batch", "precision", "learned", "nf2ff_td",
+                             "dispersive", "cylinder2d"],
                     help="sphere: one 512^3 + CPML grid (default); batch: 64 uneven grids; "
                          "precision: the sphere run to the end in fp64, fp32, bf16 with "
                          "far-field errors against fp64; learned: the learned 2-cell layer "
-                         "trained on the GPU, its reflection and step time against CPMLs")
+                         "trained on the GPU, its reflection and step time against CPMLs; "
+                         "nf2ff_td: the sphere with and without NF2FF time-domain boxes")
+    ap.add_argument("--tile-y", type=int, default=0, help="rows per Yee tile (0: engine default)")
+    ap.add_argument("--chunks", type=int, default=0, help="marches along x (0: engine default)")
+    ap.add_argument("--td-boxes", type=int, default=3, help="NF2FF-TD boxes of nf2ff_td")
     ap.add_argument("--count", type=int, default=64, help="grids of the batch workload")
     args = ap.parse_args(argv)
 
@@ -86,6 +94,10 @@ def main(argv=None) -> int:
         return bench_precision(args)
     if args.workload == "learned":
         return bench_learned(args)
+    if args.workload == "nf2ff_td":
+        return bench_nf2ff_td(args)
+    if args.workload in ("dispersive", "cylinder2d"):
+        return bench_fp32_vs_fp64(args)
     args.precision = args.precision or "fp64"
 
     import xfdtd_cuda_amd as xf
@@ -102,6 +114,8 @@ def main(argv=None) -> int:
     (nx, ny, nz), dt, _ = sim.info()
     cells = nx * ny * nz
     eng = xf.Engine(sim, precision=args.precision, timing=False).upload()
+    if args.tile_y or args.chunks:
+        eng.set_tiling(args.tile_y or 1, args.chunks, xcd_remap=False, nontemporal=True)
     setup_s = time.perf_counter() - t_setup
     print(f"# setup {setup_s:.1f} s, grid {nx}x{ny}x{nz}, {eng.describe()}", file=sys.stderr,
           flush=True)
@@ -404,6 +418,154 @@ def bench_learned(args) -> int:
                                 "train_s": round(t_train, 2), "loss_before": loss0,
                                 "loss_after": loss1}},
         "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_fp32_vs_fp64(args) -> int:
+    """Two of the reference's scenes run to the end in FP64 and in FP32, with
+    the FP32 far field (and probe series) compared to the FP64 run's.
+      dispersive: Drude core + one-pole Lorentz shell, 256^3 + 10-cell CPML,
+        6000 steps, RCS at 0.5 / 1 / 2 GHz on the xz and yz cuts; the pole
+        recursions (J) and NF2FF sums stay FP64 in the FP32 mode.
+      cylinder2d: 2-D TMz PEC cylinder, 1000^2 + 10-cell CPML, dl = 1 mm,
+        4000 steps, far field at 1 GHz over 360 angles in the xy plane and
+        Ez probe series at three points.
+    --steps overrides the step count (the default 100 means the full run)."""
+    import tempfile
+
+    import xfdtd_cuda_amd as xf
+    from xfdtd_cuda_amd import scenes
+
+    two_d = args.workload == "cylinder2d"
+    steps = (4000 if two_d else 6000) if args.steps == 100 else args.steps
+    n = (1000 if two_d else 256) if args.n == 512 else args.n
+    angles = np.linspace(-np.pi, np.pi, 360)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        t0 = time.perf_counter()
+        if two_d:
+            sim = scenes.cylinder_2d(n=n, pml=10, steps=steps, init=False)
+            for q, (fx, fy) in enumerate(((0.3, 0.0), (-0.25, 0.2), (0.0, -0.4))):
+                half = n * 1e-3 / 2
+                sim.add_movie_monitor("ez", (fx * 2 * half, fy * 2 * half, -np.inf), (1e-3, 1e-3, np.inf), 1)
+            sim.init(steps)
+            freqs = [1e9]
+        else:
+            sim = scenes.dispersive_sphere(n=n, pml=10, steps=steps)
+            freqs = [0.5e9, 1e9, 2e9]
+        (nx, ny, nz), _, _ = sim.info()
+        eng = xf.Engine(sim, precision=prec, use_graph=True).upload()
+        eng.run(steps)
+        run_ms = eng.timing()["run_ms"]
+        eng.download()
+        eng.close()
+        out = {"gcells": nx * ny * nz * steps / (run_ms * 1e-3) / 1e9, "ms_per_step": run_ms / steps}
+        with tempfile.TemporaryDirectory() as d:
+            if two_d:
+                sim.output_far_field(0, np.pi / 2, angles, "xy", d)
+                cuts = ["xy"]
+            else:
+                sim.output_far_field(0, angles, 0.0, "xz", d)
+                sim.output_far_field(0, angles, np.pi / 2, "yz", d)
+                cuts = ["xz", "yz"]
+            rcs = {}
+            for c in cuts:
+                et = np.load(os.path.join(d, c, "e_theta.npy"))
+                ep = np.load(os.path.join(d, c, "e_phi.npy"))
+                for fi, f in enumerate(freqs):
+                    inc = sim.incident_spectrum(f)
+                    rcs[(c, f)] = 4 * np.pi * (np.abs(et[fi] / inc) ** 2 + np.abs(ep[fi] / inc) ** 2)
+        out["rcs"] = rcs
+        if two_d:
+            out["probes"] = [sim.movie(q).reshape(steps, -1)[:, 0].copy() for q in range(3)]
+        res[prec] = out
+        print(f"# {prec}: {run_ms / steps:.4f} ms/step over {steps} steps "
+              f"({time.perf_counter() - t0:.0f} s with set-up)", file=sys.stderr, flush=True)
+        del sim
+
+    errs = {f"rcs_rel_l2_{c}_{f / 1e9:g}GHz": float(np.linalg.norm(res["fp32"]["rcs"][(c, f)] - r)
+                                                   / np.linalg.norm(r))
+            for (c, f), r in res["fp64"]["rcs"].items()}
+    if two_d:
+        for q in range(3):
+            a, b = res["fp32"]["probes"][q], res["fp64"]["probes"][q]
+            errs[f"probe{q}_rel_l2"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    cells = (n + 20) ** (2 if two_d else 3)
+    out = {
+        "metric": "Yee cell-updates/s (Gcells/s) & % HBM roofline",
+        "value": round(res["fp32"]["gcells"], 4), "unit": "Gcells/s", "n_gpus": 1, "steps": steps,
+        "warmup": 0, "ms_per_step": round(res["fp32"]["ms_per_step"], 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": ("2-D TMz PEC cylinder" if two_d else "Drude + Lorentz sphere") +
+                               f" {n}^{2 if two_d else 3} + 10-cell CPML ({cells} cells), "
+                               f"{steps} steps, FP32 against FP64",
+                   "fp64_gcells": round(res["fp64"]["gcells"], 4),
+                   "speedup_vs_fp64": round(res["fp32"]["gcells"] / res["fp64"]["gcells"], 3),
+                   "errors_vs_fp64": errs,
+                   "max_error": max(errs.values())},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_nf2ff_td(args) -> int:
+    """Cost of the NF2FF time-domain accumulation: the sphere scene (default
+    here 256^3 + CPML) stepped with and without --td-boxes time-domain boxes
+    on the frequency-domain box's surface, observation directions spread
+    over the sphere.  The reference accumulates each face on one GPU thread
+    (nf2ff_time_domain_data.cu:213-223); here every surface cell is a thread.
+    The line reports the extra time per step and box and the surface cells
+    per second it corresponds to."""
+    import xfdtd_cuda_amd as xf
+    from xfdtd_cuda_amd import scenes
+
+    if xf.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        return 1
+    prec = args.precision or "fp64"
+    n = args.n if args.n != 512 else 256
+    K, W = args.steps, args.warmup
+    nb = max(1, args.td_boxes)
+    dirs = [(float(np.arccos(1 - 2 * (q + 0.5) / nb)), float(2.399963 * q)) for q in range(nb)]
+    runs = {}
+    surface = 0
+    for label, td in (("without", []), ("with", dirs)):
+        sim = scenes.large_sphere(n=n, pml=args.pml, steps=W + K + 1, layout="compact", init=False)
+        for th, ph in td:
+            sim.add_nf2ff_td((args.pml + 3,) * 3, th, ph)
+        sim.init(W + K + 1)
+        if td:
+            surface = sum(sim.array(f"nf2ff_td.0.{f}.prev.ea").size for f in xf.FACES)
+            bins = sim.array("nf2ff_td.0.XN.wa").size
+        eng = xf.Engine(sim, precision=prec).upload()
+        eng.run(W)
+        eng.reset_timing()
+        eng.run(K)
+        runs[label] = eng.timing()["run_ms"] / K
+        eng.close()
+        print(f"# {label} NF2FF-TD: {runs[label]:.4f} ms/step", file=sys.stderr, flush=True)
+    (nx, ny, nz) = (n + 2 * args.pml,) * 3
+    extra = (runs["with"] - runs["without"]) / nb
+    out = {
+        "metric": "NF2FF time-domain accumulation: extra ms per step and box",
+        "value": round(extra, 5),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(runs["with"], 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32"}[prec],
+        "data": "synthetic",
+        "config": {"workload": f"dielectric sphere {n}^3 + {args.pml}-cell CPML ({nx}^3 cells), "
+                               f"{nb} NF2FF-TD boxes of {surface} surface cells and {bins} bins",
+                   "ms_per_step_without": round(runs["without"], 4),
+                   "surface_cells_per_s": round(surface / (extra * 1e-3), 1) if extra > 0 else None},
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -238,6 +238,26 @@ def test_fp32_close_to_fp64(gpu_available):
     assert rel_l2(g.array("nf2ff.0.0.XP.jb"), c.array("nf2ff.0.0.XP.jb")) < 1e-4
 
 
+def test_ade_fp32_close_to_fp64(gpu_available):
+    """FP32 fields with FP64 pole recursions (the dispersive sphere's mixed
+    precision): Drude core + Lorentz shell against the FP64 oracle."""
+    build = _dispersive("mix")
+    g, c = _pair(build, 160, precision="fp32")
+    for n in ("ex", "ey", "ez", "hx"):
+        assert rel_l2(g.array(n), c.array(n)) < 1e-4, n
+    for n in ("ade.jx", "ade.jz"):
+        assert rel_l2(g.array(n), c.array(n)) < 1e-4, n
+    assert rel_l2(g.array("nf2ff.0.0.ZP.ma"), c.array("nf2ff.0.0.ZP.ma")) < 1e-4
+
+
+def test_cylinder_2d_fp32(gpu_available):
+    build = lambda: scenes.cylinder_2d(n=80, pml=8, steps=200, tfsf_inset=14, nf2ff_inset=11)
+    g, c = _pair(build, 200, precision="fp32")
+    for n in ("ez", "hx", "hy"):
+        assert rel_l2(g.array(n), c.array(n)) < 1e-4, n
+    assert rel_l2(g.array("nf2ff.0.0.XP.ma"), c.array("nf2ff.0.0.XP.ma")) < 1e-4
+
+
 def test_bf16_storage_runs(gpu_available):
     build = lambda: scenes.dielectric_sphere(n=24, pml=6, steps=140, tfsf_inset=9, nf2ff_inset=7)
     g, c = _pair(build, 140, precision="bf16")

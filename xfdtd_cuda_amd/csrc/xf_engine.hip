@@ -493,7 +493,7 @@ int Engine::plan_nf2ff_td(const xf_nf2ff_td& T, NtPass& P, int64_t* prev_total) 
     F.rd_min = fmin;
     F.rd_max = fmax;
     F.bins = static_cast<int>(std::ceil((fmax - fmin) / c_dt)) + 4;
-    F.nrblk = (F.bins + 255) / 256;
+    F.nrblk = (F.bins + 3) / 4;  // one wave per bin, four per block
     if (cur_step < nt) {
       const double offs[2] = {0.5, 0.0};
       const double dm[2] = {T.dmax_e, T.dmax_h};

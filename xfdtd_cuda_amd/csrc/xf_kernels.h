@@ -220,8 +220,8 @@ struct NfPass {
 // NF2FF time domain (one box).  A face's surface cells are cut into tiles
 // of NT_TILE x NT_TILE cells over its two tangential axes (u < v in x, y, z
 // order); one block per tile sums the tile's contributions per time bin
-// (k_nftd_tile), one thread per bin then adds the tiles' partial sums in tile
-// order (k_nftd_reduce).
+// (k_nftd_tile), one wave per bin then adds the tiles' partial sums in a
+// fixed order (k_nftd_reduce).
 constexpr int NT_TILE = 16;
 constexpr int NT_MAXSPAN = 256;  // time bins one tile may touch in a step (LDS: 52 KiB)
 

@@ -517,53 +517,54 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// One thread per time bin of a face: the tiles' partial sums for that bin,
-// in tile order, added to the potentials.  The face's first bin comes from
-// the largest r . r_unit (the shortest delay).
+// One wave per time bin of a face: lane l adds the partial sums of tiles
+// l, l + 64, ... that cover the bin, then a xor butterfly (a fixed pairing,
+// so every run forms the same sum) combines the lanes and lane 0 adds the
+// total to the potentials.  The face's first bin comes from the largest
+// r . r_unit (the shortest delay).
 __global__ void __launch_bounds__(256) k_nftd_reduce(NtPass P, int t_arg, const int* t_dev) {
-  __shared__ int4 sh[256];
   const int t = t_dev ? *t_dev : t_arg;
   const int blk = blockIdx.x;
   int face = 0;
   while (face < 5 && blk >= P.face[face + 1].rblk0) ++face;
   const NtFace& F = P.face[face];
-  const int slot = (blk - F.rblk0) * 256 + (int)threadIdx.x;
+  const int lane = (int)threadIdx.x & 63;
+  const int slot = (blk - F.rblk0) * 4 + ((int)threadIdx.x >> 6);
+  if (slot >= F.bins) return;
   const int n_e = (int)floor((P.dmax_e - F.rd_max) / P.c_dt + 0.5 + (double)t) + slot;
   const int n_h = (int)floor((P.dmax_h - F.rd_max) / P.c_dt + 0.0 + (double)t) + slot;
-  const bool act = slot < F.bins;
   const int ms = P.maxspan;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  bool any_e = false, any_h = false;
+  int cov_e = 0, cov_h = 0;
   const int ntile = F.tu * F.tv;
-  for (int base = 0; base < ntile; base += 256) {
-    const int cnt = min(256, ntile - base);
-    __syncthreads();
-    if ((int)threadIdx.x < cnt) sh[threadIdx.x] = reinterpret_cast<const int4*>(P.hdr)[F.blk0 + base + threadIdx.x];
-    __syncthreads();
-    if (act)
-      for (int q = 0; q < cnt; ++q) {
-        const int4 h = sh[q];
-        const double* pp = P.part + (int64_t)(F.blk0 + base + q) * 4 * ms;
-        const int rh = n_h - h.z;
-        if (rh >= 0 && rh < h.w) {
-          acc[0] += pp[rh];
-          acc[1] += pp[ms + rh];
-          any_h = true;
-        }
-        const int re = n_e - h.x;
-        if (re >= 0 && re < h.y) {
-          acc[2] += pp[2 * ms + re];
-          acc[3] += pp[3 * ms + re];
-          any_e = true;
-        }
-      }
+  for (int q = lane; q < ntile; q += 64) {
+    const int4 h = reinterpret_cast<const int4*>(P.hdr)[F.blk0 + q];
+    const double* pp = P.part + (int64_t)(F.blk0 + q) * 4 * ms;
+    const int rh = n_h - h.z;
+    if (rh >= 0 && rh < h.w) {
+      acc[0] += pp[rh];
+      acc[1] += pp[ms + rh];
+      cov_h = 1;
+    }
+    const int re = n_e - h.x;
+    if (re >= 0 && re < h.y) {
+      acc[2] += pp[2 * ms + re];
+      acc[3] += pp[3 * ms + re];
+      cov_e = 1;
+    }
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    for (int p = 0; p < 4; ++p) acc[p] += __shfl_xor(acc[p], o);
+    cov_e |= __shfl_xor(cov_e, o);
+    cov_h |= __shfl_xor(cov_h, o);
+  }
+  if (lane != 0) return;
   double* pot = P.pot + (int64_t)face * 4 * P.n_pot;
-  if (any_h && n_h >= 0 && n_h < P.n_pot) {
+  if (cov_h && n_h >= 0 && n_h < P.n_pot) {
     pot[n_h] += acc[0];
     pot[P.n_pot + n_h] += acc[1];
   }
-  if (any_e && n_e >= 0 && n_e < P.n_pot) {
+  if (cov_e && n_e >= 0 && n_e < P.n_pot) {
     pot[2 * P.n_pot + n_e] += acc[2];
     pot[3 * P.n_pot + n_e] += acc[3];
   }
