@@ -125,6 +125,21 @@ def test_tiling_variants_bit_identical(gpu_available, tile_y, chunks, xcd, nt):
     _compare(g, c, label=f"tiling ty={tile_y} ch={chunks}")
 
 
+def test_second_upload_resets_device_state(gpu_available):
+    """xf_upload twice on one handle (what a batch does when it widens the
+    coefficient ids): the second upload frees and rebuilds every device
+    buffer and the run stays exact."""
+    build = lambda: scenes.dielectric_sphere(n=20, pml=5, steps=50, tfsf_inset=8, nf2ff_inset=6)
+    g, c = build(), build()
+    eng = xf.Engine(g).upload()
+    eng.upload()
+    eng.run(50).download()
+    eng.close()
+    oracle.run(c.scene_ptr(), 50)
+    for n in FIELDS:
+        assert np.array_equal(g.array(n), c.array(n)), n
+
+
 def test_oblique_incidence_all_quadrants(gpu_available):
     """TFSF with incident directions in every octant sign pattern."""
     for theta, phi in ((2.3, 4.0), (0.4, 2.2), (1.9, 5.5)):
