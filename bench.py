@@ -19,7 +19,7 @@ steps of the same scene on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
        python bench.py --workload batch   (64 uneven grids, FP32, batched kernels)
-       python bench.py --workload precision --steps 4000   (FP64/FP32/BF16 far-field error)
+       python bench.py --workload precision --steps 4000   (FP64/FP32/BF16/BF16C far-field error)
        python bench.py --workload learned   (learned 2-cell layer: training, reflection, speed)
        python bench.py --workload nf2ff_td --n 256   (NF2FF time-domain accumulation cost)
        python bench.py --workload dispersive   (Drude + Lorentz sphere, FP64 vs FP32)
@@ -63,7 +63,7 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16"],
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16", "bf16c"],
                     help="default: fp64 for the sphere, fp32 for the batch")
     ap.add_argument("--n", type=int, default=512, help="interior cells per edge")
     ap.add_argument("--pml", type=int, default=12)
@@ -138,7 +138,7 @@ def main(argv=None) -> int:
 
     ms_per_step = tm["run_ms"] / K
     value = cells * K / (tm["run_ms"] * 1e-3) / 1e9  # Gcells/s
-    sbytes = {"fp64": 8, "fp32": 4, "bf16": 2}[args.precision]
+    sbytes = {"fp64": 8, "fp32": 4, "bf16": 2, "bf16c": 3}[args.precision]
     cbytes = 8 if args.precision == "fp64" else 4
     id_bytes = 1  # compact layout: uint8 material index (<= 256 materials)
     per_cell_pass = 9 * sbytes + id_bytes
@@ -182,7 +182,8 @@ def main(argv=None) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32"}[args.precision],
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32",
+                  "bf16c": "bf16+int8-residual-storage/f32"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"dielectric sphere {nx - 2 * args.pml}^3 + {args.pml}-cell CPML "
                                f"({nx}x{ny}x{nz} cells), TFSF + NF2FF 1 GHz, {args.precision}",
@@ -253,7 +254,7 @@ def bench_batch(args) -> int:
     value64 = cells * K / (run64 * 1e-3) / 1e9
     print(f"# batch fp64: {run64 / K:.3f} ms/step", file=sys.stderr, flush=True)
 
-    sbytes = {"fp64": 8, "fp32": 4, "bf16": 2}[prec]
+    sbytes = {"fp64": 8, "fp32": 4, "bf16": 2, "bf16c": 3}[prec]
     cbytes = 8 if prec == "fp64" else 4
     bytes_h = sum(int(np.prod(s.info()[0])) * (9 * sbytes + 1) + pml_bytes(s, "h", cbytes)
                   for s in sims)
@@ -284,7 +285,8 @@ def bench_batch(args) -> int:
         "value": round(value, 4), "unit": "Gcells/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": round(run_ms / K, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32"}[prec],
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32",
+                  "bf16c": "bf16+int8-residual-storage/f32"}[prec],
         "data": "synthetic",
         "config": {"workload": f"batch of {len(sims)} sphere grids, edges U[48,160] + 8-cell CPML "
                                f"({cells} cells), TFSF + NF2FF, {prec}, one persistent launch "
@@ -307,7 +309,8 @@ def bench_batch(args) -> int:
 
 def bench_precision(args) -> int:
     """The 512^3 + 12-cell CPML sphere stepped --steps steps (4000 for the
-    full study) in FP64, FP32 and BF16-storage/FP32-compute.  Each mode's
+    full study) in FP64, FP32, BF16-storage/FP32-compute and compensated
+    BF16 storage (bf16 + an 8-bit residual per value)/FP32-compute.  Each mode's
     bistatic RCS at 1 GHz on the xz and yz cuts (360 angles) is compared with
     the FP64 run's: relative L2 error over the cut and the largest dB
     difference.  NF2FF sums are FP64 in every mode."""
@@ -317,7 +320,7 @@ def bench_precision(args) -> int:
     S = args.steps
     theta = np.linspace(-np.pi, np.pi, 360)
     res = {}
-    for prec in ("fp64", "fp32", "bf16"):
+    for prec in ("fp64", "fp32", "bf16", "bf16c"):
         t0 = time.perf_counter()
         sim = scenes.large_sphere(n=args.n, pml=args.pml, steps=S, layout="compact")
         (nx, ny, nz), _, _ = sim.info()
@@ -342,7 +345,7 @@ def bench_precision(args) -> int:
         return rel, db
 
     modes = {}
-    for prec in ("fp64", "fp32", "bf16"):
+    for prec in ("fp64", "fp32", "bf16", "bf16c"):
         r = res[prec]
         e = [err(r["rcs"][c], res["fp64"]["rcs"][c]) for c in range(2)]
         modes[prec] = {"gcells": round(r["gcells"], 4), "ms_per_step": round(r["ms_per_step"], 4),
@@ -560,7 +563,8 @@ def bench_nf2ff_td(args) -> int:
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32"}[prec],
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32",
+                  "bf16c": "bf16+int8-residual-storage/f32"}[prec],
         "data": "synthetic",
         "config": {"workload": f"dielectric sphere {n}^3 + {args.pml}-cell CPML ({nx}^3 cells), "
                                f"{nb} NF2FF-TD boxes of {surface} surface cells and {bins} bins",

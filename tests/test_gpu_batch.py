@@ -150,19 +150,20 @@ def test_engine_autotune_leaves_state_unchanged(gpu_available):
         assert np.array_equal(g.array(n), c.array(n)), n
 
 
-def test_batch_fp32_matches_single_grid_fp32(gpu_available):
+@pytest.mark.parametrize("prec", ["fp32", "bf16c"])
+def test_batch_reduced_precision_matches_single_grid(gpu_available, prec):
     steps = 40
     build = lambda: _uneven(steps, count=4, seed=21)
     singles = build()
     for s in singles:
-        e = xf.Engine(s, precision="fp32").upload()
+        e = xf.Engine(s, precision=prec).upload()
         e.run(steps).download()
         e.close()
     sims = build()
-    b = xf.Batch(sims, precision="fp32").upload()
+    b = xf.Batch(sims, precision=prec).upload()
     b.run(steps).download()
     b.close()
-    _assert_identical(sims, singles, "fp32")
+    _assert_identical(sims, singles, prec)
 
 
 def test_batch_mixed_id_widths(gpu_available):

@@ -273,6 +273,19 @@ def test_cylinder_2d_fp32(gpu_available):
     assert rel_l2(g.array("nf2ff.0.0.XP.ma"), c.array("nf2ff.0.0.XP.ma")) < 1e-4
 
 
+def test_compensated_bf16_storage_close_to_fp64(gpu_available):
+    """bf16 + 8-bit residual storage (XF_BF16_COMP): about 16 significant
+    bits per stored value, so it lands between bf16 storage and FP32."""
+    build = lambda: scenes.dielectric_sphere(n=24, pml=6, steps=140, tfsf_inset=9, nf2ff_inset=7)
+    errs = {}
+    for prec in ("bf16c", "bf16", "fp32"):
+        g, c = _pair(build, 140, precision=prec)
+        errs[prec] = max(rel_l2(g.array(n), c.array(n)) for n in ("ex", "ey", "ez", "hy"))
+    assert errs["bf16c"] < 2e-3, errs
+    assert errs["bf16c"] < errs["bf16"] / 30, errs
+    assert errs["fp32"] < errs["bf16c"], errs
+
+
 def test_bf16_storage_runs(gpu_available):
     build = lambda: scenes.dielectric_sphere(n=24, pml=6, steps=140, tfsf_inset=9, nf2ff_inset=7)
     g, c = _pair(build, 140, precision="bf16")

@@ -341,6 +341,8 @@ int Engine::validate() {
       mn[ax] = rs[q].l0;
       for (int d = 0; d < 3; ++d)
         if (mn[d] < 0 || mx[d] >= rs[q].psi->shape[d]) return fail(XF_ERR_INVALID, nm + ": psi array too small");
+      // the kernels index psi with 32-bit arithmetic
+      if (numel(*rs[q].psi) >= (int64_t(1) << 32)) return fail(XF_ERR_UNSUPPORTED, nm + ": psi array above 2^32 values");
     }
     (void)L;
   }
@@ -721,7 +723,7 @@ double Engine::bytes_per_cell_step() const {
 int Engine::pack_fields() {
   const xf_array* src[6] = {&sc->ex, &sc->ey, &sc->ez, &sc->hx, &sc->hy, &sc->hz};
   const int prec = plan.precision;
-  const double se = 1.0, sh = (prec == XF_BF16_STORE) ? 376.730313668 : 1.0;
+  const double se = 1.0, sh = (prec == XF_BF16_STORE || prec == XF_BF16_COMP) ? 376.730313668 : 1.0;
   std::vector<unsigned char> buf(total * esize);
   for (int c = 0; c < 6; ++c) {
     const xf_array& a = *src[c];
@@ -744,7 +746,15 @@ int Engine::pack_fields() {
             reinterpret_cast<double*>(buf.data())[idx] = v;
           else if (prec == XF_FP32)
             reinterpret_cast<float*>(buf.data())[idx] = static_cast<float>(v);
-          else
+          else if (prec == XF_BF16_COMP) {
+            // the kernels' blocked layout: 64 bf16 words, then 64 residuals
+            uint16_t h;
+            int8_t r;
+            bf16c_encode(static_cast<float>(v * scale), h, r);
+            unsigned char* b = buf.data() + (idx >> 6) * 192;
+            reinterpret_cast<uint16_t*>(b)[idx & 63] = h;
+            reinterpret_cast<int8_t*>(b + 128)[idx & 63] = r;
+          } else
             reinterpret_cast<__hip_bfloat16*>(buf.data())[idx] = __float2bfloat16(static_cast<float>(v * scale));
         }
     XF_TRY(alloc(&fld[c], total * esize));
@@ -757,7 +767,7 @@ int Engine::pack_fields() {
 int Engine::unpack_fields() {
   xf_array* dst[6] = {&sc->ex, &sc->ey, &sc->ez, &sc->hx, &sc->hy, &sc->hz};
   const int prec = plan.precision;
-  const double sh = (prec == XF_BF16_STORE) ? 376.730313668 : 1.0;
+  const double sh = (prec == XF_BF16_STORE || prec == XF_BF16_COMP) ? 376.730313668 : 1.0;
   std::vector<unsigned char> buf(total * esize);
   for (int c = 0; c < 6; ++c) {
     if (!fld[c]) continue;
@@ -773,7 +783,11 @@ int Engine::unpack_fields() {
             v = reinterpret_cast<const double*>(buf.data())[idx];
           else if (prec == XF_FP32)
             v = reinterpret_cast<const float*>(buf.data())[idx];
-          else
+          else if (prec == XF_BF16_COMP) {
+            const unsigned char* b = buf.data() + (idx >> 6) * 192;
+            v = static_cast<double>(bf16c_decode(reinterpret_cast<const uint16_t*>(b)[idx & 63],
+                                                 reinterpret_cast<const int8_t*>(b + 128)[idx & 63])) * inv;
+          } else
             v = static_cast<double>(__bfloat162float(reinterpret_cast<const __hip_bfloat16*>(buf.data())[idx])) * inv;
           a.data[(i * a.shape[1] + j) * a.shape[2] + k] = v;
         }
@@ -1288,20 +1302,23 @@ int Engine::upload() {
   release();
   XF_HIP(hipSetDevice(device));
   XF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  esize = plan.precision == XF_FP64 ? 8 : plan.precision == XF_FP32 ? 4 : 2;
+  esize = plan.precision == XF_FP64 ? 8 : plan.precision == XF_FP32 ? 4 : plan.precision == XF_BF16_COMP ? 3 : 2;
+  // rows start on a 128-byte boundary (a 64-value block for the 3-byte
+  // compensated format, whose layout is blocked by 64)
+  const int64_t align = plan.precision == XF_BF16_COMP ? 64 : 128 / static_cast<int64_t>(esize);
   if (dim == 3) {
-    const int64_t align = 128 / static_cast<int64_t>(esize);
     const int64_t pz = ((nz + 1 + align - 1) / align) * align;
     sy = pz;
     sx = (ny + 1) * pz;
     total = (nx + 1) * sx;
   } else {
-    const int64_t align = 128 / static_cast<int64_t>(esize);
     const int64_t py = ((ny + 1 + align - 1) / align) * align;
     sy = 1;
     sx = py;
     total = (nx + 1) * sx;
   }
+  if (plan.precision == XF_BF16_COMP && static_cast<uint64_t>(total) * 3 >= (uint64_t(1) << 32))
+    return fail(XF_ERR_UNSUPPORTED, "compensated bf16 storage: grid above 2^32 bytes per component");
   XF_TRY(build_ids());
   XF_TRY(pack_fields());
   XF_TRY(upload_pml());
@@ -1527,7 +1544,8 @@ Fields<S> make_fields(const Engine& E) {
 template <typename C>
 Scales<C> make_scales(const Engine& E) {
   Scales<C> s;
-  const double sh = E.plan.precision == XF_BF16_STORE ? 376.730313668 : 1.0;
+  const double sh =
+      (E.plan.precision == XF_BF16_STORE || E.plan.precision == XF_BF16_COMP) ? 376.730313668 : 1.0;
   s.inv_e = 1;
   s.s_e = 1;
   s.inv_h = static_cast<C>(1.0 / sh);
@@ -1816,6 +1834,8 @@ int Engine::run(int64_t n) {
         return launch_step<double>(t, false, timing_on, e);
       case XF_FP32:
         return launch_step<float>(t, false, timing_on, e);
+      case XF_BF16_COMP:
+        return launch_step<Bf16c>(t, false, timing_on, e);
       default:
         return launch_step<__hip_bfloat16>(t, false, timing_on, e);
     }
@@ -1830,6 +1850,9 @@ int Engine::run(int64_t n) {
           break;
         case XF_FP32:
           st = launch_step<float>(0, true, false, nullptr);
+          break;
+        case XF_BF16_COMP:
+          st = launch_step<Bf16c>(0, true, false, nullptr);
           break;
         default:
           st = launch_step<__hip_bfloat16>(0, true, false, nullptr);
@@ -1909,7 +1932,7 @@ int xf_create(xf_scene* scene, const xf_plan* plan, xf_handle** out) {
   if (!h) return xf::fail(XF_ERR_NOMEM, "out of host memory");
   h->eng.sc = scene;
   if (plan) h->eng.plan = *plan;
-  if (h->eng.plan.precision < XF_FP64 || h->eng.plan.precision > XF_BF16_STORE) {
+  if (h->eng.plan.precision < XF_FP64 || h->eng.plan.precision > XF_BF16_COMP) {
     delete h;
     return xf::fail(XF_ERR_INVALID, "unknown precision");
   }

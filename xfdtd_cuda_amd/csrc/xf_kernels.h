@@ -71,6 +71,82 @@ struct Store<__hip_bfloat16> {
   }
 };
 
+// Compensated bf16 storage: a value y is kept as hi = bf16(y) (round to
+// nearest even) and the residual y - hi, which is exact in float and at most
+// half a bf16 ulp, quantised to an int8 in steps of 2^-15 of hi's power of
+// two.  hi + q 2^(e-15) has at most 16 significant bits and is rebuilt
+// exactly in float, so a stored value carries about 16 bits of mantissa in
+// 3 bytes.  Values sit in blocks of 64: 64 bf16 words (128 B), then 64
+// residual bytes (64 B), so a wave's 64 consecutive values are two
+// contiguous segments.  The tag type is never dereferenced.
+struct Bf16c {
+  unsigned char b[3];
+};
+
+__host__ __device__ __forceinline__ void bf16c_encode(float y, uint16_t& h, int8_t& l) {
+  const uint32_t yb = __builtin_bit_cast(uint32_t, y);
+  const uint32_t hb = (yb + 0x7fffu + ((yb >> 16) & 1u)) & 0xffff0000u;
+  h = static_cast<uint16_t>(hb >> 16);
+  const float hi = __builtin_bit_cast(float, hb);
+  const float p2 = __builtin_bit_cast(float, hb & 0x7f800000u);
+  float q = 0.0f;
+  if (p2 != 0.0f) q = rintf((y - hi) * 32768.0f / p2);
+  q = fminf(fmaxf(q, -127.0f), 127.0f);
+  l = static_cast<int8_t>(static_cast<int>(q));
+}
+
+__host__ __device__ __forceinline__ float bf16c_decode(uint16_t h, int8_t l) {
+  const uint32_t hb = static_cast<uint32_t>(h) << 16;
+  const float hi = __builtin_bit_cast(float, hb);
+  const float p2 = __builtin_bit_cast(float, hb & 0x7f800000u);
+  return hi + static_cast<float>(l) * (p2 * (1.0f / 32768.0f));
+}
+
+// Byte offsets are formed in 32 bits from the element index (the host keeps
+// 3 x values below 2^32), so every access is a wave-uniform base plus a
+// 32-bit lane offset: value i of block i / 64, lane l = i % 64, has its bf16
+// word at 3 (i - l) + 2 l = 3 i - l and its residual at 3 i - 2 l + 128.
+template <>
+struct Store<Bf16c> {
+  using C = float;
+  static __device__ __forceinline__ uint64_t off_hi(int64_t i) {
+    const uint32_t u = (uint32_t)i;
+    return (uint64_t)(3u * u - (u & 63u));
+  }
+  static __device__ __forceinline__ uint64_t off_lo(int64_t i) {
+    const uint32_t u = (uint32_t)i;
+    return (uint64_t)(3u * u - 2u * (u & 63u) + 128u);
+  }
+  static __device__ __forceinline__ const unsigned char* bytes(const Bf16c* p) {
+    return reinterpret_cast<const unsigned char*>(p);
+  }
+  static __device__ __forceinline__ unsigned char* bytes(Bf16c* p) { return reinterpret_cast<unsigned char*>(p); }
+  static __device__ __forceinline__ C ld(const Bf16c* p, int64_t i, C inv) {
+    return bf16c_decode(*reinterpret_cast<const uint16_t*>(bytes(p) + off_hi(i)),
+                        *reinterpret_cast<const int8_t*>(bytes(p) + off_lo(i))) *
+           inv;
+  }
+  static __device__ __forceinline__ void st(Bf16c* p, int64_t i, C v, C scale) {
+    uint16_t h;
+    int8_t r;
+    bf16c_encode(v * scale, h, r);
+    *reinterpret_cast<uint16_t*>(bytes(p) + off_hi(i)) = h;
+    *reinterpret_cast<int8_t*>(bytes(p) + off_lo(i)) = r;
+  }
+  static __device__ __forceinline__ C ld_nt(const Bf16c* p, int64_t i, C inv) {
+    return bf16c_decode(__builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(bytes(p) + off_hi(i))),
+                        __builtin_nontemporal_load(reinterpret_cast<const int8_t*>(bytes(p) + off_lo(i)))) *
+           inv;
+  }
+  static __device__ __forceinline__ void st_nt(Bf16c* p, int64_t i, C v, C scale) {
+    uint16_t h;
+    int8_t r;
+    bf16c_encode(v * scale, h, r);
+    __builtin_nontemporal_store(h, reinterpret_cast<uint16_t*>(bytes(p) + off_hi(i)));
+    __builtin_nontemporal_store(r, reinterpret_cast<int8_t*>(bytes(p) + off_lo(i)));
+  }
+};
+
 // Loads and stores at a small unsigned element offset from a wave-uniform
 // base pointer: the offset is zero-extended from 32 bits, which lets the
 // compiler address with the SGPR base + 32-bit VGPR offset form of the

@@ -42,7 +42,7 @@ __device__ __forceinline__ void pml_face(const PmlFace<C>& F, int i, int j, int 
   if (in_a && !(TWO_D && (self[A] == nullptr || dual[B] == nullptr))) {
     const C dp = Store<S>::ld(dual[B], idx, inv_dual);
     const C dq = Store<S>::ld(dual[B], ISE ? idx - stc : idx + stc, inv_dual);
-    const int64_t p = ((int64_t)px * F.sa1 + py) * F.sa2 + pz;
+    const uint64_t p = ((uint32_t)px * (uint32_t)F.sa1 + (uint32_t)py) * (uint32_t)F.sa2 + (uint32_t)pz;
     C psi = F.psi_a[p];
     if (ISE)
       psi = cb * psi + ca * (dp - dq);
@@ -59,7 +59,7 @@ __device__ __forceinline__ void pml_face(const PmlFace<C>& F, int i, int j, int 
   if (in_b && !(TWO_D && (self[B] == nullptr || dual[A] == nullptr))) {
     const C dp = Store<S>::ld(dual[A], idx, inv_dual);
     const C dq = Store<S>::ld(dual[A], ISE ? idx - stc : idx + stc, inv_dual);
-    const int64_t p = ((int64_t)px * F.sb1 + py) * F.sb2 + pz;
+    const uint64_t p = ((uint32_t)px * (uint32_t)F.sb1 + (uint32_t)py) * (uint32_t)F.sb2 + (uint32_t)pz;
     C psi = F.psi_b[p];
     if (ISE)
       psi = cb * psi + ca * (dp - dq);
@@ -75,11 +75,16 @@ __device__ __forceinline__ void pml_face(const PmlFace<C>& F, int i, int j, int 
   }
 }
 
+// Every face of the pass in registration order (X, Y, Z faces); faces whose
+// bit is clear in mask (pml_mask of the cell's column) cannot contain the
+// cell and are skipped.
 template <bool ISE, bool TWO_D, typename S, typename C>
 __device__ __forceinline__ void pml_all(const PmlPass<C>& P, int i, int j, int k, int64_t idx,
                                         const Fields<S>& f, const Dims& d, const Scales<C>& sc,
-                                        C (&v)[3], bool (&have)[3], bool (&dirty)[3]) {
+                                        C (&v)[3], bool (&have)[3], bool (&dirty)[3],
+                                        unsigned mask = ~0u) {
   for (int q = 0; q < P.n; ++q) {
+    if (!((mask >> q) & 1u)) continue;
     const PmlFace<C>& F = P.f[q];
     if (F.axis == 0)
       pml_face<0, ISE, TWO_D>(F, i, j, k, idx, f, d, sc, v, have, dirty);

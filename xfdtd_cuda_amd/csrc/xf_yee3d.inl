@@ -112,7 +112,8 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
   const int i0 = ch * D.chunk;
   const int i1 = min(i0 + D.chunk, d.nx + 1);
   const bool jk_main = j < d.ny && k < d.nz;
-  const bool any_pml = PML && pml_mask(D.pml, i0, i1, j, k) != 0u;
+  const unsigned pmask = PML ? pml_mask(D.pml, i0, i1, j, k) : 0u;
+  const bool any_pml = pmask != 0u;
   int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
   const C ie = sc.inv_e, ih = sc.inv_h;
   C ey_i = 0, ez_i = 0;
@@ -137,7 +138,7 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
       ey_i = cur.ey_n;
       ez_i = cur.ez_n;
     }
-    if (PML && any_pml) pml_all<false, false>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty);
+    if (PML && any_pml) pml_all<false, false>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask);
     st3<S, NT>(f.h, idx, v, dirty, sc.s_h);
     cur = nxt;
   }
@@ -160,7 +161,8 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
   const int i1 = min(i0 + D.chunk, d.nx + 1);
   const bool jk_main = j < d.ny && k < d.nz;
   const bool j_in = j > 0, k_in = k > 0;
-  const bool any_pml = PML && pml_mask(D.pml, i0, i1, j, k) != 0u;
+  const unsigned pmask = PML ? pml_mask(D.pml, i0, i1, j, k) : 0u;
+  const bool any_pml = pmask != 0u;
   int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
   const C ie = sc.inv_e, ih = sc.inv_h;
   C hy_m = 0, hz_m = 0;  // H at i-1
@@ -197,7 +199,7 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
       hy_m = cur.hy;
       hz_m = cur.hz;
     }
-    if (PML && any_pml) pml_all<true, false>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty);
+    if (PML && any_pml) pml_all<true, false>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask);
     st3<S, NT>(f.e, idx, v, dirty, sc.s_e);
     cur = nxt;
   }
