@@ -72,12 +72,13 @@ def main(argv=None) -> int:
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--workload", default="sphere",
                     choices=["sphere", "batch", "precision", "learned", "nf2ff_td",
-                             "dispersive", "cylinder2d"],
+                             "dispersive", "cylinder2d", "scenes"],
                     help="sphere: one 512^3 + CPML grid (default); batch: 64 uneven grids; "
                          "precision: the sphere run to the end in fp64, fp32, bf16 with "
                          "far-field errors against fp64; learned: the learned 2-cell layer "
                          "trained on the GPU, its reflection and step time against CPMLs; "
-                         "nf2ff_td: the sphere with and without NF2FF time-domain boxes")
+                         "nf2ff_td: the sphere with and without NF2FF time-domain boxes; "
+                         "scenes: every quoted scene on the GPU and the CPU oracle")
     ap.add_argument("--tile-y", type=int, default=0, help="rows per Yee tile (0: engine default)")
     ap.add_argument("--chunks", type=int, default=0, help="marches along x (0: engine default)")
     ap.add_argument("--td-boxes", type=int, default=3, help="NF2FF-TD boxes of nf2ff_td")
@@ -98,6 +99,8 @@ def main(argv=None) -> int:
         return bench_nf2ff_td(args)
     if args.workload in ("dispersive", "cylinder2d"):
         return bench_fp32_vs_fp64(args)
+    if args.workload == "scenes":
+        return bench_scenes(args)
     args.precision = args.precision or "fp64"
 
     import xfdtd_cuda_amd as xf
@@ -254,6 +257,19 @@ def bench_batch(args) -> int:
     value64 = cells * K / (run64 * 1e-3) / 1e9
     print(f"# batch fp64: {run64 / K:.3f} ms/step", file=sys.stderr, flush=True)
 
+    # the same spheres bounded by the 2-cell learned layer instead of the
+    # 8-cell CPML, weights trained here on 16-cell CPML point-source runs
+    from xfdtd_cuda_amd import learned
+    smp, kmur = learned.record_samples(n=32, seeds=(0, 1, 2, 3, 4, 5), steps=300, every=2)
+    w16, _, _ = learned.train_layer(smp, kmur)
+    sims_cpml = sims
+    sims = scenes.batch_spheres(count=args.count, steps=max(steps_needed, 2 * K + 8), pml=2,
+                                lb_weights=w16.view(np.float16))
+    cells_lb = sum(int(np.prod(s.info()[0])) for s in sims)
+    run_lb, _, best_lb, _ = timed_batch(prec)
+    print(f"# batch {prec} learned 2-cell layer: {run_lb / K:.3f} ms/step", file=sys.stderr, flush=True)
+    sims = sims_cpml
+
     sbytes = {"fp64": 8, "fp32": 4, "bf16": 2, "bf16c": 3}[prec]
     cbytes = 8 if prec == "fp64" else 4
     bytes_h = sum(int(np.prod(s.info()[0])) * (9 * sbytes + 1) + pml_bytes(s, "h", cbytes)
@@ -295,6 +311,10 @@ def bench_batch(args) -> int:
                    "sequential_single_grid_gcells": round(seq_value, 4),
                    "batch_speedup_vs_sequential": round(value / seq_value, 3),
                    "fp64_batch_gcells": round(value64, 4),
+                   "learned2_boundary": {
+                       "ms_per_step": round(run_lb / K, 4), "cells": cells_lb,
+                       "speedup_vs_cpml8_same_spheres": round(run_ms / run_lb, 3),
+                       "tuning": best_lb},
                    "speedup_vs_fp64": round(value / value64, 3), "fp64_tuning": best64},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s",
@@ -509,6 +529,126 @@ def bench_fp32_vs_fp64(args) -> int:
                    "errors_vs_fp64": errs,
                    "max_error": max(errs.values())},
         "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_scenes(args) -> int:
+    """Every scene the engine is quoted on, each timed on the GPU in FP64 and
+    in its reduced-precision modes, and on the CPU oracle with one thread and
+    with all threads:
+      sphere156: dielectric sphere 140^3 + 8-cell CPML, 2400 steps;
+      cylinder2d: 2-D TMz PEC cylinder 1000^2 + 10-cell CPML, 4000 steps;
+      sphere536: sphere 512^3 + 12-cell CPML (a 400-step window of 4000);
+      dispersive: Drude + Lorentz sphere 256^3 + 10-cell CPML, 6000 steps;
+      batch: 64 sphere grids, edges U[48, 160] + 8-cell CPML (a 400-step
+        window of 2000, tile shape from the autotuner).
+    The GPU runs whole step counts where the scene is small and windows where
+    it is large; the CPU runs windows sized to a few seconds (stated per row),
+    starting from the GPU's final FP64 state."""
+    import oracle
+    import xfdtd_cuda_amd as xf
+    from xfdtd_cuda_amd import scenes
+
+    nthr = oracle.threads()
+    specs = [
+        ("sphere156", "dielectric sphere 140^3 + 8-cell CPML",
+         lambda st: scenes.dielectric_sphere(n=140, pml=8, steps=st), 2400, ("fp64", "fp32"), 10, 200),
+        ("cylinder2d", "2-D TMz PEC cylinder 1000^2 + 10-cell CPML",
+         lambda st: scenes.cylinder_2d(n=1000, pml=10, steps=st), 4000, ("fp64", "fp32"), 40, 800),
+        ("sphere536", "dielectric sphere 512^3 + 12-cell CPML",
+         lambda st: scenes.large_sphere(n=512, pml=12, steps=st, layout="compact"), 400,
+         ("fp64", "fp32", "bf16", "bf16c"), 1, 5),
+        ("dispersive", "Drude + Lorentz sphere 256^3 + 10-cell CPML",
+         lambda st: scenes.dispersive_sphere(n=256, pml=10, steps=st), 6000, ("fp64", "fp32"), 2, 20),
+    ]
+    rows = []
+
+    def cpu_windows(ptr_list, cells, c1, cn):
+        out = {}
+        for thr, steps in ((1, c1), (nthr, cn)):
+            oracle.set_threads(thr)
+            t0 = time.perf_counter()
+            for p in ptr_list:
+                oracle.run(p, steps)
+            dt_s = time.perf_counter() - t0
+            out[f"cpu_{'1thr' if thr == 1 else 'all'}"] = {
+                "gcells": round(cells * steps / dt_s / 1e9, 5), "steps": steps, "threads": thr,
+                "seconds": round(dt_s, 2)}
+        oracle.set_threads(nthr)
+        return out
+
+    W = 3
+    for name, what, build, gsteps, modes, c1, cn in specs:
+        row = {"scene": name, "what": what}
+        for prec in modes:
+            extra = c1 + cn if prec == "fp64" else 0
+            sim = build(W + gsteps + extra + 2)
+            (nx, ny, nz), _, _ = sim.info()
+            cells = nx * ny * nz
+            eng = xf.Engine(sim, precision=prec, use_graph=True).upload()
+            eng.run(W)
+            eng.run(gsteps)
+            run_ms = eng.timing()["run_ms"]
+            row["cells"] = cells
+            row[f"gpu_{prec}"] = {"gcells": round(cells * gsteps / (run_ms * 1e-3) / 1e9, 4),
+                                  "ms_per_step": round(run_ms / gsteps, 5), "steps": gsteps}
+            if prec == "fp64":
+                eng.download()
+                eng.close()
+                row.update(cpu_windows([sim.scene_ptr()], cells, c1, cn))
+            else:
+                eng.close()
+            del sim
+        for prec in modes[1:]:
+            row[f"speedup_{prec}_vs_fp64"] = round(row[f"gpu_{prec}"]["gcells"] / row["gpu_fp64"]["gcells"], 3)
+        row["gpu_fp64_vs_cpu_all"] = round(row["gpu_fp64"]["gcells"] / row["cpu_all"]["gcells"], 1)
+        print(f"# {name}: {json.dumps(row)}", file=sys.stderr, flush=True)
+        rows.append(row)
+
+    # the batch of uneven grids
+    K = 400
+    row = {"scene": "batch", "what": "64 sphere grids, edges U[48,160] + 8-cell CPML"}
+    for prec in ("fp64", "fp32"):
+        extra = 2 if prec == "fp64" else 0
+        sims = scenes.batch_spheres(count=64, steps=W + K + extra + 20)
+        cells = sum(int(np.prod(s.info()[0])) for s in sims)
+        b = xf.Batch(sims, precision=prec, use_graph=True).upload()
+        b.autotune(trial_steps=3)
+        b.run(W)
+        b.run(K)
+        run_ms = b.timing()["run_ms"]
+        row["cells"] = cells
+        row[f"gpu_{prec}"] = {"gcells": round(cells * K / (run_ms * 1e-3) / 1e9, 4),
+                              "ms_per_step": round(run_ms / K, 5), "steps": K}
+        if prec == "fp64":
+            b.download()
+            b.close()
+            row.update(cpu_windows([s.scene_ptr() for s in sims], cells, 1, 1))
+        else:
+            b.close()
+        del sims
+    row["speedup_fp32_vs_fp64"] = round(row["gpu_fp32"]["gcells"] / row["gpu_fp64"]["gcells"], 3)
+    row["gpu_fp64_vs_cpu_all"] = round(row["gpu_fp64"]["gcells"] / row["cpu_all"]["gcells"], 1)
+    print(f"# batch: {json.dumps(row)}", file=sys.stderr, flush=True)
+    rows.append(row)
+
+    big = next(r for r in rows if r["scene"] == "sphere536")
+    out = {
+        "metric": "Yee cell-updates/s (Gcells/s) & % HBM roofline",
+        "value": big["gpu_fp64"]["gcells"], "unit": "Gcells/s", "n_gpus": 1,
+        "steps": big["gpu_fp64"]["steps"], "warmup": W,
+        "ms_per_step": big["gpu_fp64"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "every quoted scene on the GPU (FP64 and reduced precision, hipGraph) "
+                               "and on the CPU oracle (1 and all threads); value = the 536^3 sphere in FP64",
+                   "scenes": rows},
+        "roofline": None,
+        "cpu_baseline": {"value": big["cpu_all"]["gcells"], "unit": "Gcells/s", "cores": nthr,
+                         "kind": "port",
+                         "sample": f"{big['cpu_all']['steps']} FP64 steps of the 536^3 sphere, "
+                                   f"oracle/xf_oracle.c with {nthr} OpenMP threads"},
     }
     print(json.dumps(out), flush=True)
     return 0

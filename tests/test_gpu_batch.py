@@ -186,3 +186,48 @@ def test_batch_mixed_id_widths(gpu_available):
     b.run(steps).download()
     b.close()
     _assert_identical(sims, ref, "mixed ids")
+
+
+def test_batch_with_learned_layers_sources_movies_and_time_domain(gpu_available):
+    """Grids with a learned absorbing layer and point sources, one with a
+    movie monitor, one with an NF2FF time-domain box and one recording
+    training samples, stepped as one batch (point sources and learned layers
+    of all grids in one launch per phase, movies and time-domain boxes per
+    grid) and under a hipGraph: every grid bit-identical to the oracle, the
+    time-domain potentials within rel-L2 1e-12 (their tile sums are ordered
+    differently from the oracle's serial adds)."""
+    steps = 40
+    rng = np.random.default_rng(5)
+    w = (rng.standard_normal(scenes.LB_NPARAM) * 0.05).astype(np.float16)
+
+    def build():
+        sims = [scenes.point_source_box(n=14, boundary="learned", size=2, steps=steps, seed=1,
+                                        weights=w),
+                scenes.point_source_box(n=17, boundary="mur", size=2, steps=steps, seed=2)]
+        s = scenes.point_source_box(n=12, boundary="cpml", size=4, steps=steps, seed=3, init=False)
+        s.add_movie_monitor("ez", (0.0, 0.0, 0.03), (np.inf, np.inf, 5e-3), 3)
+        s.init(steps)
+        sims.append(s)
+        sims.append(scenes.dielectric_sphere(n=16, pml=4, steps=steps, tfsf_inset=7, nf2ff_inset=5,
+                                             freqs=(2e9,), td_dirs=[(0.6, 0.4)]))
+        sims.append(scenes.point_source_box(n=15, boundary="cpml", size=6, steps=steps, seed=4,
+                                            record=(3, 3, 20)))
+        return sims
+
+    ref = _oracle_copies(build, steps)
+    for graph in (False, True):
+        sims = build()
+        b = xf.Batch(sims, use_graph=graph).upload()
+        b.run(15).run(steps - 15).download()
+        b.close()
+        _assert_identical(sims, ref, f"additions graph={graph}")
+        assert np.array_equal(sims[2].movie(0), ref[2].movie(0))
+        assert np.abs(ref[2].movie(0)).max() > 0
+        assert np.array_equal(sims[0].array("lb.history"), ref[0].array("lb.history"))
+        assert np.array_equal(sims[4].lb_samples(), ref[4].lb_samples())
+        assert np.abs(ref[4].lb_samples()).max() > 0
+        for face in xf.FACES:
+            for q in ("wa", "ub"):
+                n = f"nf2ff_td.0.{face}.{q}"
+                a, c = sims[3].array(n), ref[3].array(n)
+                assert np.linalg.norm(a - c) <= 1e-12 * np.linalg.norm(c), n

@@ -44,6 +44,12 @@ struct Batch {
   int64_t* nf_start{};
   int n_nf{0};
   int64_t nf_total{0};
+  void* pt_items{};
+  int n_pt{0};
+  void* lb_items{};
+  int64_t* lb_bstart{};  // first block of each layer, one block serves one grid
+  int n_lb{0};
+  int64_t lb_blocks{0};
   int* t_dev{};
   hipGraph_t graph{};
   hipGraphExec_t graph_exec{};
@@ -67,8 +73,8 @@ struct Batch {
     allocs.clear();
     pass_h = pass_e = nullptr;
     tile_start = nullptr;
-    tf_items[0] = tf_items[1] = nf_items = nullptr;
-    tf_start[0] = tf_start[1] = nf_start = nullptr;
+    tf_items[0] = tf_items[1] = nf_items = pt_items = lb_items = nullptr;
+    tf_start[0] = tf_start[1] = nf_start = lb_bstart = nullptr;
     t_dev = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -104,6 +110,14 @@ struct Batch {
   int snapshot();
   int restore();
   int autotune(int trial_steps, xf_tuning* best);
+  // every grid's step counter (and learned-layer recording count) to cur_step
+  void sync_steps() {
+    for (auto& h : inst) {
+      Engine& E = h->eng;
+      E.cur_step = cur_step;
+      if (E.has_lb && E.lbd.mode == XF_LB_RECORD) E.lb_rec_count = E.lb_recorded_after(cur_step);
+    }
+  }
 };
 
 int Batch::create(xf_scene* const* scenes, int n, const xf_plan* pl) {
@@ -123,13 +137,11 @@ int Batch::create(xf_scene* const* scenes, int n, const xf_plan* pl) {
     Engine& E = h->eng;
     if (E.dim != 3) return fail(XF_ERR_UNSUPPORTED, "scene " + std::to_string(g) + ": batches take 3-D grids only");
     if (E.has_ade) return fail(XF_ERR_UNSUPPORTED, "scene " + std::to_string(g) + ": ADE media are not batched");
-    if (E.has_lb || E.sc->n_point > 0 || E.sc->n_movie > 0 || E.sc->n_nf2ff_td > 0)
-      return fail(XF_ERR_UNSUPPORTED, "scene " + std::to_string(g) +
-                                          ": point sources, learned layers, movies and NF2FF time "
-                                          "domain are not batched");
+    // point sources, the learned layer, movies and NF2FF time-domain boxes
+    // run per grid inside the batch's step (Engine::launch_additions / launch_td)
     if (g == 0) cur_step = E.cur_step;
     if (E.cur_step != cur_step) return fail(XF_ERR_INVALID, "scenes of a batch must share the current step");
-    if (E.sc->tfsf || E.sc->n_nf2ff > 0) nt = std::min(nt, E.nt);
+    if (E.sc->tfsf || E.sc->n_nf2ff > 0 || E.sc->n_point > 0 || E.sc->n_nf2ff_td > 0) nt = std::min(nt, E.nt);
     cells += E.nx * E.ny * E.nz;
   }
   return XF_OK;
@@ -258,6 +270,43 @@ int Batch::configure_t() {
     XF_TRY(put(&q, st.data(), st.size() * sizeof(int64_t)));
     nf_start = static_cast<int64_t*>(q);
   }
+  // point sources and learned layers
+  {
+    std::vector<PointItem<S>> items;
+    for (int g = 0; g < n; ++g) {
+      Engine& E = eng(g);
+      if (E.pts.n == 0) continue;
+      PointItem<S> it{};
+      it.f = make_fields<S>(E);
+      it.sc = make_scales<C>(E);
+      it.P = E.pts;
+      items.push_back(it);
+    }
+    n_pt = static_cast<int>(items.size());
+    XF_TRY(put(&pt_items, items.data(), items.size() * sizeof(PointItem<S>)));
+  }
+  {
+    std::vector<LbItem<S>> items;
+    std::vector<int64_t> st(1, 0);
+    for (int g = 0; g < n; ++g) {
+      Engine& E = eng(g);
+      if (!E.has_lb || E.lbd.total == 0) continue;
+      LbItem<S> it{};
+      it.f = make_fields<S>(E);
+      it.sc = make_scales<C>(E);
+      it.d = make_dims(E);
+      it.L = E.lbd;
+      items.push_back(it);
+      st.push_back(st.back() + (E.lbd.total + 255) / 256);
+    }
+    n_lb = static_cast<int>(items.size());
+    lb_blocks = st.back();
+    if (lb_blocks > INT_MAX) return fail(XF_ERR_UNSUPPORTED, "learned layers too large for one launch");
+    XF_TRY(put(&lb_items, items.data(), items.size() * sizeof(LbItem<S>)));
+    void* q = lb_bstart;
+    XF_TRY(put(&q, st.data(), st.size() * sizeof(int64_t)));
+    lb_bstart = static_cast<int64_t*>(q);
+  }
   // persistent grid: what the device holds resident, unless set
   hipDeviceProp_t prop;
   XF_HIP(hipGetDeviceProperties(&prop, device));
@@ -321,9 +370,20 @@ int Batch::launch_step(int t, bool use_tdev, bool ev_on, hipEvent_t* ev) {
   if (tf_total[1] > 0)
     hipLaunchKernelGGL((k_tfsf_batch<S>), dim3((unsigned)((tf_total[1] + 255) / 256)), dim3(256), 0, stream,
                        static_cast<const TfsfItem<S>*>(tf_items[1]), tf_start[1], n_tf[1], t, tp);
+  // per grid: point sources, then the learned layer's phases, then movies,
+  // as Engine::launch_additions orders them; the grids share no memory
+  if (n_pt > 0)
+    hipLaunchKernelGGL((k_point_sources_batch<S>), dim3((unsigned)((n_pt + 63) / 64)), dim3(64), 0, stream,
+                       static_cast<const PointItem<S>*>(pt_items), n_pt, t, tp);
+  if (n_lb > 0)
+    for (int ph = 0; ph < 3; ++ph)
+      hipLaunchKernelGGL((k_lb_batch<S>), dim3((unsigned)lb_blocks), dim3(256), 0, stream,
+                         static_cast<const LbItem<S>*>(lb_items), lb_bstart, n_lb, ph, t, tp);
+  for (auto& h : inst) XF_TRY(h->eng.launch_movies<S>(t, tp, stream));
   if (nf_total > 0)
     hipLaunchKernelGGL((k_nf2ff_batch<S>), dim3((unsigned)((nf_total + 255) / 256)), dim3(256), 0, stream,
                        static_cast<const NfItem<S>*>(nf_items), nf_start, n_nf, t, tp);
+  for (auto& h : inst) XF_TRY(h->eng.launch_td<S>(t, tp, stream));
   if (ev_on) XF_HIP(hipEventRecord(ev[4], stream));
   if (use_tdev) hipLaunchKernelGGL(k_next_step, dim3(1), dim3(1), 0, stream, t_dev);
 #undef XF_BYALL
@@ -412,7 +472,7 @@ int Batch::run(int64_t n) {
   for (auto& h : inst) bytes += h->eng.bytes_per_cell_step() * static_cast<double>(h->eng.nx * h->eng.ny * h->eng.nz);
   timing.bytes_per_cell_step = bytes / static_cast<double>(cells);
   cur_step += n;
-  for (auto& h : inst) h->eng.cur_step = cur_step;
+  sync_steps();
   return XF_OK;
 }
 
@@ -469,7 +529,7 @@ int Batch::autotune(int trial_steps, xf_tuning* best) {
             win.step_ms = ms;
           }
           cur_step = step0;
-          for (auto& h : inst) h->eng.cur_step = step0;
+          sync_steps();
           status = restore();
           if (status != XF_OK) break;
         }
@@ -482,7 +542,7 @@ int Batch::autotune(int trial_steps, xf_tuning* best) {
   plan.timing = keep_timing;
   timing = keep;
   cur_step = step0;
-  for (auto& h : inst) h->eng.cur_step = step0;
+  sync_steps();
   if (status != XF_OK) {
     (void)restore();
     return status;

@@ -249,6 +249,15 @@ struct Engine {
   int launch_post_h(int t, const int* tp);
   template <typename S>
   int launch_post_e(int t, const int* tp);
+  // after the E corrections: point sources, the learned layer, movies (in
+  // that order), and after the frequency-domain NF2FF the time-domain one;
+  // on stream st (the engine's own, or a batch's)
+  template <typename S>
+  int launch_additions(int t, const int* tp, hipStream_t st);
+  template <typename S>
+  int launch_movies(int t, const int* tp, hipStream_t st);
+  template <typename S>
+  int launch_td(int t, const int* tp, hipStream_t st);
 
   double bytes_per_cell_step() const;
 };
@@ -1757,32 +1766,54 @@ int Engine::launch_post_e(int t, const int* tp) {
       hipLaunchKernelGGL((k_tfsf<S>), dim3((unsigned)((P.total + 255) / 256)), dim3(256), 0, stream,
                          make_fields<S>(*this), P, t, tp, make_scales<C>(*this), make_dims(*this), dim == 2 ? 1 : 0);
   }
-  if (pts.n > 0)
-    hipLaunchKernelGGL((k_point_sources<S>), dim3(1), dim3(64), 0, stream, make_fields<S>(*this), pts,
-                       make_scales<C>(*this), t, tp);
-  if (has_lb && lbd.total > 0) {
-    const dim3 g((unsigned)((lbd.total + 255) / 256)), b(256);
-    const int phases[3] = {0, lbd.mode == XF_LB_APPLY ? 1 : -1, 2};
-    for (int ph : phases)
-      if (ph >= 0)
-        hipLaunchKernelGGL((k_lb<S>), g, b, 0, stream, make_fields<S>(*this), lbd, make_dims(*this),
-                           make_scales<C>(*this), ph, t, tp);
-  }
-  for (const MovieDev& M : movies)
-    hipLaunchKernelGGL((k_movie<S>), dim3((unsigned)((M.box + 255) / 256)), dim3(256), 0, stream,
-                       make_fields<S>(*this), M, make_dims(*this), dim == 2 ? 1 : 0, make_scales<C>(*this), t, tp);
+  XF_TRY(launch_additions<S>(t, tp, stream));
   for (auto& N : nf) {
     if (N.pass.total > 0)
       hipLaunchKernelGGL((k_nf2ff<S>), dim3((unsigned)((N.pass.total + 255) / 256)), dim3(256), 0, stream,
                          make_fields<S>(*this), N.pass, t, tp, make_dims(*this), dim == 2 ? 1 : 0,
                          make_scales<C>(*this));
   }
+  XF_TRY(launch_td<S>(t, tp, stream));
+  XF_HIP(hipGetLastError());
+  return XF_OK;
+}
+
+template <typename S>
+int Engine::launch_additions(int t, const int* tp, hipStream_t st) {
+  using C = typename Store<S>::C;
+  if (pts.n > 0)
+    hipLaunchKernelGGL((k_point_sources<S>), dim3(1), dim3(64), 0, st, make_fields<S>(*this), pts,
+                       make_scales<C>(*this), t, tp);
+  if (has_lb && lbd.total > 0) {
+    const dim3 g((unsigned)((lbd.total + 255) / 256)), b(256);
+    const int phases[3] = {0, lbd.mode == XF_LB_APPLY ? 1 : -1, 2};
+    for (int ph : phases)
+      if (ph >= 0)
+        hipLaunchKernelGGL((k_lb<S>), g, b, 0, st, make_fields<S>(*this), lbd, make_dims(*this),
+                           make_scales<C>(*this), ph, t, tp);
+  }
+  return launch_movies<S>(t, tp, st);
+}
+
+template <typename S>
+int Engine::launch_movies(int t, const int* tp, hipStream_t st) {
+  using C = typename Store<S>::C;
+  for (const MovieDev& M : movies)
+    hipLaunchKernelGGL((k_movie<S>), dim3((unsigned)((M.box + 255) / 256)), dim3(256), 0, st,
+                       make_fields<S>(*this), M, make_dims(*this), dim == 2 ? 1 : 0, make_scales<C>(*this), t, tp);
+  XF_HIP(hipGetLastError());
+  return XF_OK;
+}
+
+template <typename S>
+int Engine::launch_td(int t, const int* tp, hipStream_t st) {
+  using C = typename Store<S>::C;
   for (const NtPass& P : nftd) {
     if (P.nblk == 0) continue;
     const size_t lds = 256 * sizeof(NtRec) + 16 * static_cast<size_t>(P.maxspan) * sizeof(double);
-    hipLaunchKernelGGL((k_nftd_tile<S>), dim3(P.nblk), dim3(256), lds, stream, make_fields<S>(*this), P,
+    hipLaunchKernelGGL((k_nftd_tile<S>), dim3(P.nblk), dim3(256), lds, st, make_fields<S>(*this), P,
                        make_dims(*this), make_scales<C>(*this), t, tp);
-    hipLaunchKernelGGL(k_nftd_reduce, dim3(P.nrblk), dim3(256), 0, stream, P, t, tp);
+    hipLaunchKernelGGL(k_nftd_reduce, dim3(P.nrblk), dim3(256), 0, st, P, t, tp);
   }
   XF_HIP(hipGetLastError());
   return XF_OK;

@@ -371,8 +371,9 @@ struct LbDev {
   LbFace face[6];
 };
 
-// One grid's TFSF pass or NF2FF box inside a batch of grids: the batched
-// kernels read these from device memory, one per grid (or box).
+// One grid's TFSF pass, NF2FF box, point sources or learned layer inside a
+// batch of grids: the batched kernels read these from device memory, one per
+// grid (or box).
 template <typename S>
 struct TfsfItem {
   using C = typename Store<S>::C;
@@ -389,6 +390,23 @@ struct NfItem {
   Scales<C> sc;
   Dims d;
   NfPass P;
+};
+
+template <typename S>
+struct PointItem {
+  using C = typename Store<S>::C;
+  Fields<S> f;
+  Scales<C> sc;
+  PointDev P;
+};
+
+template <typename S>
+struct LbItem {
+  using C = typename Store<S>::C;
+  Fields<S> f;
+  Scales<C> sc;
+  Dims d;
+  LbDev L;
 };
 
 }  // namespace xf

@@ -709,19 +709,13 @@ __device__ __forceinline__ int lb_other(const LbDev& L, const LbCol& c) {
 
 // phase 0: APPLY the faces' inner nodes / RECORD samples; phase 1: APPLY
 // the edge nodes (owned by the lower face); phase 2: history roll.
+// One column of the layer at time t; w holds the weights (LDS) in APPLY
+// phases 0 and 1.
 template <typename S>
-__global__ void __launch_bounds__(256)
-    k_lb(Fields<S> f, LbDev L, Dims d, Scales<typename Store<S>::C> sc, int phase, int t_arg,
-         const int* t_dev) {
+__device__ __forceinline__ void lb_column_step(const Fields<S>& f, const LbDev& L, const Dims& d,
+                                               const Scales<typename Store<S>::C>& sc, int phase,
+                                               int t, const float* w, int64_t col) {
   using C = typename Store<S>::C;
-  __shared__ float w[XF_LB_NPARAM];
-  if (phase < 2 && L.mode == XF_LB_APPLY) {
-    for (int i = threadIdx.x; i < XF_LB_NPARAM; i += blockDim.x) w[i] = L.w[i];
-    __syncthreads();
-  }
-  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= L.total) return;
-  const int t = t_dev ? *t_dev : t_arg;
   const int par = t & 1;
   const LbCol c = lb_column(L, col);
   S* fld = f.e[c.cc == 0 ? c.F->a : c.F->b];
@@ -771,6 +765,20 @@ __global__ void __launch_bounds__(256)
   Store<S>::st(fld, node0, (C)u1, sc.s_e);
 }
 
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_lb(Fields<S> f, LbDev L, Dims d, Scales<typename Store<S>::C> sc, int phase, int t_arg,
+         const int* t_dev) {
+  __shared__ float w[XF_LB_NPARAM];
+  if (phase < 2 && L.mode == XF_LB_APPLY) {
+    for (int i = threadIdx.x; i < XF_LB_NPARAM; i += blockDim.x) w[i] = L.w[i];
+    __syncthreads();
+  }
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= L.total) return;
+  lb_column_step<S>(f, L, d, sc, phase, t_dev ? *t_dev : t_arg, w, col);
+}
+
 // ----------------------------------------------------------- batched forms
 // Item g owns flat indices [start[g], start[g + 1]); start[n] is the total.
 __device__ __forceinline__ int owner_of(const int64_t* __restrict__ start, int n, int64_t x) {
@@ -807,6 +815,43 @@ __global__ void __launch_bounds__(256)
   const int g = owner_of(start, n, tid);
   const NfItem<S>& I = items[g];
   nf2ff_cell<S>(I.f, I.P, t_dev ? *t_dev : t_arg, I.d, 0, I.sc, tid - start[g]);
+}
+
+// The point sources of every grid of a batch in one launch: thread g adds
+// grid g's sources in registration order, as k_point_sources does.
+template <typename S>
+__global__ void __launch_bounds__(64)
+    k_point_sources_batch(const PointItem<S>* __restrict__ items, int n, int t_arg, const int* t_dev) {
+  using C = typename Store<S>::C;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int t = t_dev ? *t_dev : t_arg;
+  const PointItem<S>& I = items[g];
+  for (int q = 0; q < I.P.n; ++q) {
+    S* fld = I.f.e[I.P.comp[q]];
+    const int64_t i = I.P.idx[q];
+    const C v = Store<S>::ld(fld, i, I.sc.inv_e) + (C)I.P.value[(int64_t)q * I.P.nt + t];
+    Store<S>::st(fld, i, v, I.sc.s_e);
+  }
+}
+
+// One phase of the learned layers of every grid of a batch in one launch.
+// Item g owns blocks [bstart[g], bstart[g + 1]), so a block serves one grid
+// and stages that grid's weights.
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_lb_batch(const LbItem<S>* __restrict__ items, const int64_t* __restrict__ bstart, int n,
+               int phase, int t_arg, const int* t_dev) {
+  __shared__ float w[XF_LB_NPARAM];
+  const int g = owner_of(bstart, n, (int64_t)blockIdx.x);
+  const LbItem<S>& I = items[g];
+  if (phase < 2 && I.L.mode == XF_LB_APPLY) {
+    for (int i = threadIdx.x; i < XF_LB_NPARAM; i += blockDim.x) w[i] = I.L.w[i];
+    __syncthreads();
+  }
+  const int64_t col = ((int64_t)blockIdx.x - bstart[g]) * blockDim.x + threadIdx.x;
+  if (col >= I.L.total) return;
+  lb_column_step<S>(I.f, I.L, I.d, I.sc, phase, t_dev ? *t_dev : t_arg, w, col);
 }
 
 // ------------------------------------------------------------------ ADE

@@ -29,10 +29,12 @@ from .simulation import FACES, Material, Simulation, Waveform
 def dielectric_sphere(n: int = 140, pml: int = 8, steps: int = 2400, dl: float = 2.5e-3,
                       tfsf_inset: int = 15, nf2ff_inset: int = 11, freqs=(1e9,),
                       radius: float | None = None, eps_r: float = 3.0, mu_r: float = 2.0,
-                      layout: str = "dense", init: bool = True, td_dirs=()) -> Simulation:
+                      layout: str = "dense", init: bool = True, td_dirs=(),
+                      lb_weights=None) -> Simulation:
     """n interior cells per edge (140 reproduces the driver's 0.35 m cube);
     td_dirs: (theta, phi) observation directions of NF2FF time-domain boxes on
-    the frequency-domain box's surface."""
+    the frequency-domain box's surface; lb_weights: a `pml`-cell learned
+    absorbing layer with these fp16 weights instead of the CPML."""
     edge = n * dl
     half = edge / 2
     radius = (0.1 / 0.35) * edge if radius is None else radius
@@ -41,7 +43,9 @@ def dielectric_sphere(n: int = 140, pml: int = 8, steps: int = 2400, dl: float =
     s.add_sphere("dielectric_sphere", (0, 0, 0), radius, Material.dielectric(eps_r, mu_r))
     tau = (dl * 20) / 6e8
     s.add_tfsf3d((tfsf_inset,) * 3, 0.0, 0.0, 0.0, Waveform.gaussian(tau, 4.5 * tau))
-    if pml > 0:
+    if pml > 0 and lb_weights is not None:
+        s.add_learned_boundary(lb_weights, thickness=pml)
+    elif pml > 0:
         s.add_pml_all(pml)
     if nf2ff_inset > 0 and freqs:
         s.add_nf2ff_fd((nf2ff_inset,) * 3, list(freqs))
@@ -109,7 +113,7 @@ def large_sphere(n: int = 512, pml: int = 12, steps: int = 4000, seed: int = 202
 
 def batch_spheres(count: int = 64, seed: int = 7, edge_range=(48, 160), pml: int = 8,
                   steps: int = 2000, dl: float = 2.5e-3, freqs=(1e9,), layout: str = "compact",
-                  init: bool = True) -> list[Simulation]:
+                  init: bool = True, lb_weights=None) -> list[Simulation]:
     """A batch of independent sphere scatterers of uneven sizes: cube edge
     (interior cells) ~ U{edge_range}, sphere radius ~ U[0.2, 0.4] x edge,
     eps_r ~ U[1.5, 8], all from numpy's default_rng(seed), each with a
@@ -117,6 +121,8 @@ def batch_spheres(count: int = 64, seed: int = 7, edge_range=(48, 160), pml: int
     inset is min(7, floor(edge/2 - r) - 1) cells inside the CPML so the box
     always encloses the sphere (a 0.4 x edge sphere in a 48-cell cube leaves
     4 cells); the NF2FF surface sits max(1, inset - 4) cells inside the CPML.
+    lb_weights: a `pml`-cell learned absorbing layer with these weights on
+    every grid instead of the CPML (the same spheres, sources and boxes).
     """
     rng = np.random.default_rng(seed)
     sims = []
@@ -129,7 +135,8 @@ def batch_spheres(count: int = 64, seed: int = 7, edge_range=(48, 160), pml: int
         n_in = max(1, t_in - 4)
         sims.append(dielectric_sphere(n=n, pml=pml, steps=steps, dl=dl, tfsf_inset=pml + t_in,
                                       nf2ff_inset=pml + n_in, freqs=freqs, radius=r_cells * dl,
-                                      eps_r=eps_r, mu_r=1.0, layout=layout, init=init))
+                                      eps_r=eps_r, mu_r=1.0, layout=layout, init=init,
+                                      lb_weights=lb_weights))
     return sims
 
 
