@@ -208,7 +208,10 @@ typedef struct xf_point_source {
  * (the first-order Mur update plus a learned, scale-free correction;
  * mur + 0 when s = 0).  MLP: 12 -> 32 ReLU -> 32 ReLU -> 1, fp16 weights,
  * fp32 accumulation; weight layout W1[32][12] b1[32] W2[32][32] b2[32]
- * w3[32] b3.
+ * w3[32] b3.  It runs on the matrix cores: x / s and the layer-1
+ * activations are rounded to fp16 (the activations saturating at 65504),
+ * layers 1 and 2 are fp16 x fp16 products with fp32 accumulation, the
+ * output unit is fp32.
  *
  * XF_LB_APPLY (inset 0): the layer-0 nodes of the outer boundary, which the
  * Yee pass never updates, take that value -- a 2-cell layer in place of a

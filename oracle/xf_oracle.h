@@ -23,6 +23,8 @@ int xfo_yee_component_triple_loop(xf_scene* s, int is_e, int xyz);
 /* The learned layer's column predictor alone (n columns of 12 inputs). */
 int xfo_lb_predict(const uint16_t* weights, double mur_k, const float* x, int64_t n,
                    float* out);
+/* The predictor's 16-term matrix-product model alone (n products). */
+int xfo_lb_mfma16(const float* c, const float* a, const float* b, int64_t n, float* out);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,7 @@ import pytest
 import oracle
 import xfdtd_cuda_amd as xf
 from xfdtd_cuda_amd import scenes
-from conftest import FIELDS
+from conftest import FIELDS, selection_weights
 
 pytestmark = pytest.mark.gpu
 
@@ -192,13 +192,13 @@ def test_batch_with_learned_layers_sources_movies_and_time_domain(gpu_available)
     """Grids with a learned absorbing layer and point sources, one with a
     movie monitor, one with an NF2FF time-domain box and one recording
     training samples, stepped as one batch (point sources and learned layers
-    of all grids in one launch per phase, movies and time-domain boxes per
-    grid) and under a hipGraph: every grid bit-identical to the oracle, the
+    of all grids in one launch per phase, the learned layers' predictor on
+    the matrix cores, movies and time-domain boxes per grid) and under a
+    hipGraph: every grid bit-identical to the oracle, the
     time-domain potentials within rel-L2 1e-12 (their tile sums are ordered
     differently from the oracle's serial adds)."""
     steps = 40
-    rng = np.random.default_rng(5)
-    w = (rng.standard_normal(scenes.LB_NPARAM) * 0.05).astype(np.float16)
+    w = selection_weights(5)  # exact matrix products: the batch stays bit-identical
 
     def build():
         sims = [scenes.point_source_box(n=14, boundary="learned", size=2, steps=steps, seed=1,

@@ -10,7 +10,7 @@ import oracle
 from oracle import lb_mlp
 import xfdtd_cuda_amd as xf
 from xfdtd_cuda_amd import learned, scenes
-from conftest import FIELDS
+from conftest import FIELDS, rel_l2, selection_weights
 
 pytestmark = pytest.mark.gpu
 
@@ -24,8 +24,12 @@ def _pair(build, steps):
     return g, c
 
 
-def test_learned_layer_apply_matches_oracle(gpu_available):
-    w = lb_mlp.init_weights(11, out_scale=0.3).astype(np.float16)
+def test_learned_layer_apply_exact_products_bit_identical(gpu_available):
+    """A network whose matrix products are exact (one non-zero term per
+    8-term group): the matrix-core predictor, its lane layout and the
+    hidden-unit order of its products agree with the oracle bit for bit over
+    80 closed-loop steps."""
+    w = selection_weights(3)
     build = lambda: scenes.point_source_box(n=18, boundary="learned", size=2, steps=80, seed=3,
                                             weights=w)
     g, c = _pair(build, 80)
@@ -35,10 +39,85 @@ def test_learned_layer_apply_matches_oracle(gpu_available):
     assert np.abs(c.array("lb.history")).max() > 0
 
 
+def test_learned_layer_apply_matches_oracle(gpu_available):
+    """A random network: the matrix cores add an 8-term group within 8 units
+    of 2^-23 of its largest term of the exact sum the oracle rounds (measured
+    on gfx950), and now and then that moves a layer-1 activation across an
+    fp16 rounding boundary (2^-11 relative).  The layer feeds the grid, so
+    after 80 closed-loop steps every node carries some of it.  Bars: rel-L2
+    1e-5 over the six fields together (measured 6.3e-6; the H components of
+    this Ez-driven box are minor and carry more relative error alone) and
+    2e-4 over the layer's own history (measured 7.9e-5)."""
+    w = lb_mlp.init_weights(11, out_scale=0.3).astype(np.float16)
+    build = lambda: scenes.point_source_box(n=18, boundary="learned", size=2, steps=80, seed=3,
+                                            weights=w)
+    g, c = _pair(build, 80)
+    num = sum(float(np.sum((g.array(n) - c.array(n)) ** 2)) for n in FIELDS)
+    den = sum(float(np.sum(c.array(n) ** 2)) for n in FIELDS)
+    err_h = rel_l2(g.array("lb.history"), c.array("lb.history"))
+    print(f"fields rel-L2 {np.sqrt(num / den):.2e}, history rel-L2 {err_h:.2e}")
+    assert np.sqrt(num / den) <= 1e-5
+    assert err_h <= 2e-4
+    assert np.abs(c.array("lb.history")).max() > 0
+
+
+def test_learned_layer_one_step_from_random_state(gpu_available):
+    """One step from random fields and history (every column of every face
+    live, with non-zero inputs of all signs): nodes off the layer's outer
+    planes are bit-identical to the oracle and most predicted outer-plane
+    nodes too (measured 85 %).  The others differ where a layer-1 activation
+    fell on the other side of an fp16 rounding boundary: within 2^-8 of the
+    column scale s, bounded here by the power of two above 8 x max|history|
+    (the Laplacian inputs) and max|field| (measured worst 2^-10 s)."""
+    w = lb_mlp.init_weights(4, out_scale=1.0).astype(np.float16)
+    build = lambda: scenes.point_source_box(n=20, boundary="learned", size=2, steps=3, seed=1,
+                                            weights=w)
+    g, c = build(), build()
+    (n, _, _), _, _ = g.info()  # cubic grid, layer included
+    rng = np.random.default_rng(9)
+    for name in FIELDS:
+        v = rng.standard_normal(g.array(name).shape) * 3.0
+        g.array(name)[...] = v
+        c.array(name)[...] = v
+    hv = (rng.standard_normal(g.array("lb.history").shape) * 2.0).astype(np.float32)
+    g.array("lb.history")[...] = hv
+    c.array("lb.history")[...] = hv
+    e = xf.Engine(g).upload()
+    e.run(1).download()
+    e.close()
+    oracle.run(c.scene_ptr(), 1)
+    # inputs: updated fields (x0..x2) and the history before the step (x3..x11)
+    s_up = 2.0 ** np.ceil(np.log2(max(max(float(np.abs(c.array(f)).max()) for f in ("ex", "ey", "ez")),
+                                      8.0 * float(np.abs(hv).max()))))
+    bound = 2.0 ** -8 * s_up
+    worst, same, cnt = 0.0, 0, 0
+    for name in ("ex", "ey", "ez"):
+        a, b = g.array(name), c.array(name)
+        for idx in np.argwhere(a != b):
+            assert min(idx) == 0 or any(idx[d] == n for d in range(3)), (name, idx)
+        outer = np.zeros(a.shape, bool)
+        for d in range(3):
+            sl = [slice(None)] * 3
+            sl[d] = 0
+            outer[tuple(sl)] = True
+            if a.shape[d] > n:
+                sl[d] = n
+                outer[tuple(sl)] = True
+        worst = max(worst, float(np.max(np.abs(a[outer] - b[outer]))))
+        same += int(np.count_nonzero(a[outer] == b[outer]))
+        cnt += int(np.count_nonzero(outer))
+    for name in ("hx", "hy", "hz"):
+        assert np.array_equal(g.array(name), c.array(name)), name
+    print(f"outer-plane nodes: worst |diff| {worst:.2e} (bound {bound:.2e}), identical {same / cnt:.3f}")
+    assert same > 0.5 * cnt
+    assert worst <= bound
+
+
 def test_learned_layer_partial_faces_and_fp32(gpu_available):
-    """Faces XN, YP, ZN only (edges between learned and PEC faces); FP32
-    stays within float rounding of the FP64 oracle."""
-    w = lb_mlp.init_weights(5, out_scale=0.2).astype(np.float16)
+    """Faces XN, YP, ZN only (edges between learned and PEC faces), with an
+    exact-product network: bit-identical to the oracle; FP32 stays within
+    float rounding of the FP64 oracle."""
+    w = selection_weights(6)
 
     def build():
         s = xf.Simulation(5e-3, 5e-3, 5e-3, 0.9)

@@ -47,9 +47,11 @@ struct Batch {
   void* pt_items{};
   int n_pt{0};
   void* lb_items{};
-  int64_t* lb_bstart{};  // first block of each layer, one block serves one grid
+  int64_t* lb_bstart{};  // first 256-column block of each layer (VALU phases)
+  int64_t* lb_mstart{};  // first kLbMfmaCols-column block of each layer (k_lb_mfma_batch)
   int n_lb{0};
-  int64_t lb_blocks{0};
+  int64_t lb_blocks{0}, lb_mblocks{0};
+  bool lb_any_record{false};
   int* t_dev{};
   hipGraph_t graph{};
   hipGraphExec_t graph_exec{};
@@ -74,7 +76,7 @@ struct Batch {
     pass_h = pass_e = nullptr;
     tile_start = nullptr;
     tf_items[0] = tf_items[1] = nf_items = pt_items = lb_items = nullptr;
-    tf_start[0] = tf_start[1] = nf_start = lb_bstart = nullptr;
+    tf_start[0] = tf_start[1] = nf_start = lb_bstart = lb_mstart = nullptr;
     t_dev = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -287,7 +289,8 @@ int Batch::configure_t() {
   }
   {
     std::vector<LbItem<S>> items;
-    std::vector<int64_t> st(1, 0);
+    std::vector<int64_t> st(1, 0), sm(1, 0);
+    lb_any_record = false;
     for (int g = 0; g < n; ++g) {
       Engine& E = eng(g);
       if (!E.has_lb || E.lbd.total == 0) continue;
@@ -298,14 +301,21 @@ int Batch::configure_t() {
       it.L = E.lbd;
       items.push_back(it);
       st.push_back(st.back() + (E.lbd.total + 255) / 256);
+      const bool apply = E.lbd.mode == XF_LB_APPLY;
+      sm.push_back(sm.back() + (apply ? (E.lbd.total + kLbMfmaCols - 1) / kLbMfmaCols : 0));
+      lb_any_record = lb_any_record || !apply;
     }
     n_lb = static_cast<int>(items.size());
     lb_blocks = st.back();
+    lb_mblocks = sm.back();
     if (lb_blocks > INT_MAX) return fail(XF_ERR_UNSUPPORTED, "learned layers too large for one launch");
     XF_TRY(put(&lb_items, items.data(), items.size() * sizeof(LbItem<S>)));
     void* q = lb_bstart;
     XF_TRY(put(&q, st.data(), st.size() * sizeof(int64_t)));
     lb_bstart = static_cast<int64_t*>(q);
+    q = lb_mstart;
+    XF_TRY(put(&q, sm.data(), sm.size() * sizeof(int64_t)));
+    lb_mstart = static_cast<int64_t*>(q);
   }
   // persistent grid: what the device holds resident, unless set
   hipDeviceProp_t prop;
@@ -375,10 +385,20 @@ int Batch::launch_step(int t, bool use_tdev, bool ev_on, hipEvent_t* ev) {
   if (n_pt > 0)
     hipLaunchKernelGGL((k_point_sources_batch<S>), dim3((unsigned)((n_pt + 63) / 64)), dim3(64), 0, stream,
                        static_cast<const PointItem<S>*>(pt_items), n_pt, t, tp);
-  if (n_lb > 0)
-    for (int ph = 0; ph < 3; ++ph)
-      hipLaunchKernelGGL((k_lb_batch<S>), dim3((unsigned)lb_blocks), dim3(256), 0, stream,
-                         static_cast<const LbItem<S>*>(lb_items), lb_bstart, n_lb, ph, t, tp);
+  if (n_lb > 0) {
+    // phase 0 (RECORD layers on the VALU, APPLY layers on the matrix
+    // cores), the edge nodes of APPLY layers, the history roll
+    const LbItem<S>* li = static_cast<const LbItem<S>*>(lb_items);
+    if (lb_any_record)
+      hipLaunchKernelGGL((k_lb_batch<S>), dim3((unsigned)lb_blocks), dim3(256), 0, stream, li, lb_bstart, n_lb, 0,
+                         t, tp);
+    if (lb_mblocks > 0)
+      hipLaunchKernelGGL((k_lb_mfma_batch<S>), dim3((unsigned)lb_mblocks), dim3(256), 0, stream, li, lb_mstart,
+                         n_lb, t, tp);
+    for (int ph = 1; ph < 3; ++ph)
+      hipLaunchKernelGGL((k_lb_batch<S>), dim3((unsigned)lb_blocks), dim3(256), 0, stream, li, lb_bstart, n_lb, ph,
+                         t, tp);
+  }
   for (auto& h : inst) XF_TRY(h->eng.launch_movies<S>(t, tp, stream));
   if (nf_total > 0)
     hipLaunchKernelGGL((k_nf2ff_batch<S>), dim3((unsigned)((nf_total + 255) / 256)), dim3(256), 0, stream,

@@ -1237,7 +1237,9 @@ int Engine::upload_additions() {
       F.start[cc] = col;
       F.p1[cc] = static_cast<int>(n3[a] - L.inset - (cc == 0 ? 1 : 0));
       F.q1[cc] = static_cast<int>(n3[b] - L.inset - (cc == 1 ? 1 : 0));
-      col += static_cast<int64_t>(F.p1[cc] - L.inset + 1) * (F.q1[cc] - L.inset + 1);
+      const int64_t cols = static_cast<int64_t>(F.p1[cc] - L.inset + 1) * (F.q1[cc] - L.inset + 1);
+      if (cols >= (int64_t(1) << 32)) return fail(XF_ERR_UNSUPPORTED, "learned layer: a face holds 2^32 columns or more");
+      col += cols;
     }
   }
   lbd.total = col;
@@ -1248,11 +1250,32 @@ int Engine::upload_additions() {
   lbd.hist = static_cast<float*>(ph);
   state.emplace_back(ph, hist.size() * sizeof(float));
   if (L.mode == XF_LB_APPLY) {
-    std::vector<float> w(XF_LB_NPARAM);
-    for (int i = 0; i < XF_LB_NPARAM; ++i) w[i] = half_bits_to_float(L.weights[i]);
+    // the weights as per-lane MFMA operands (LbPack)
+    const uint16_t* W1 = L.weights;
+    const uint16_t* b1 = W1 + XF_LB_HID * XF_LB_IN;
+    const uint16_t* W2 = b1 + XF_LB_HID;
+    const uint16_t* b2 = W2 + XF_LB_HID * XF_LB_HID;
+    const uint16_t* w3 = b2 + XF_LB_HID;
+    std::vector<LbPack> pk(1);
+    LbPack& P = pk[0];
+    for (int lane = 0; lane < 64; ++lane) {
+      const int g = lane / 32, m = lane % 32;
+      for (int j = 0; j < 8; ++j) {
+        const int k = 8 * g + j;
+        P.a1[lane][j] = k < XF_LB_IN ? W1[m * XF_LB_IN + k] : 0;
+        for (int p = 0; p < 2; ++p) P.a2[p][lane][j] = W2[m * XF_LB_HID + 16 * p + 8 * (j / 4) + 4 * g + j % 4];
+      }
+      for (int i = 0; i < 16; ++i) {
+        const int u = 8 * (i / 4) + 4 * g + i % 4;
+        P.c1[i][lane] = half_bits_to_float(b1[u]);
+        P.c2[i][lane] = half_bits_to_float(b2[u]);
+        P.w3[i][lane] = half_bits_to_float(w3[u]);
+      }
+    }
+    P.b3 = half_bits_to_float(w3[XF_LB_HID]);
     void* pw;
-    XF_TRY(upload_vec(&pw, w));
-    lbd.w = static_cast<const float*>(pw);
+    XF_TRY(upload_vec(&pw, pk));
+    lbd.pack = static_cast<const LbPack*>(pw);
   } else {
     const size_t nrec = static_cast<size_t>(L.rec_slots) * col * 13;
     std::vector<float> rec(L.rec, L.rec + nrec);
@@ -1785,12 +1808,16 @@ int Engine::launch_additions(int t, const int* tp, hipStream_t st) {
     hipLaunchKernelGGL((k_point_sources<S>), dim3(1), dim3(64), 0, st, make_fields<S>(*this), pts,
                        make_scales<C>(*this), t, tp);
   if (has_lb && lbd.total > 0) {
+    // APPLY: phase 0 on the matrix cores, then the edge nodes (phase 1);
+    // RECORD: phase 0 records; both end with the history roll (phase 2)
     const dim3 g((unsigned)((lbd.total + 255) / 256)), b(256);
-    const int phases[3] = {0, lbd.mode == XF_LB_APPLY ? 1 : -1, 2};
+    if (lbd.mode == XF_LB_APPLY)
+      hipLaunchKernelGGL((k_lb_mfma<S>), dim3((unsigned)((lbd.total + kLbMfmaCols - 1) / kLbMfmaCols)), b, 0,
+                         st, make_fields<S>(*this), lbd, make_dims(*this), make_scales<C>(*this), t, tp);
+    const int phases[2] = {lbd.mode == XF_LB_APPLY ? 1 : 0, 2};
     for (int ph : phases)
-      if (ph >= 0)
-        hipLaunchKernelGGL((k_lb<S>), g, b, 0, st, make_fields<S>(*this), lbd, make_dims(*this),
-                           make_scales<C>(*this), ph, t, tp);
+      hipLaunchKernelGGL((k_lb<S>), g, b, 0, st, make_fields<S>(*this), lbd, make_dims(*this),
+                         make_scales<C>(*this), ph, t, tp);
   }
   return launch_movies<S>(t, tp, st);
 }

@@ -358,10 +358,25 @@ struct LbFace {
   int p1[2], q1[2];    // last a-, b-index of E_a, E_b (first is inset)
 };
 
+// The MLP weights as the matrix-core predictor's per-lane operands, built
+// on the host from the fp16 weights.  For lane l of a wave, g = l / 32:
+// a1 holds W1[l % 32][8g + j] (zero for inputs 12..15); a2[p] holds
+// W2[l % 32][16p + 8(j / 4) + 4g + j % 4], the hidden unit the layer-1
+// result row of that k-slot carries; c1, c2 and w3 hold b1, b2 and w3 of
+// unit 8(i / 4) + 4g + i % 4, the row of accumulator element i.
+struct LbPack {
+  uint16_t a1[64][8];     // fp16 bits
+  uint16_t a2[2][64][8];  // fp16 bits
+  float c1[16][64];
+  float c2[16][64];
+  float w3[16][64];
+  float b3;
+};
+
 struct LbDev {
   int mode, faces, inset, nface;
   float mur_k;
-  const float* w;      // XF_LB_NPARAM weights (fp16 values as float)
+  const LbPack* pack;  // APPLY: the weights as MFMA operands
   float* hist;
   float* rec;          // RECORD: [slot][column][13]
   int rec_every;

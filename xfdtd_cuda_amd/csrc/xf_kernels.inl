@@ -615,63 +615,45 @@ __global__ void __launch_bounds__(256)
 }
 
 // --------------------------------------------------------- learned layer
-// The column predictor of xf_learned_boundary: the Mur update plus s times
-// the MLP of x / s, s the power of two above max |x_i| (exact scaling);
-// every product and sum is one float rounding (no contraction).
-__device__ __forceinline__ float lb_predict(const float* __restrict__ w, float mur_k,
-                                            const float (&x)[XF_LB_IN]) {
-  float m = 0.0f;
-#pragma unroll
-  for (int i = 0; i < XF_LB_IN; ++i) {
-    const float ax = fabsf(x[i]);
-    if (ax > m) m = ax;
-  }
-  const float mur = x[4] + mur_k * (x[0] - x[3]);
-  if (m == 0.0f) return mur;
-  int e;
-  (void)frexpf(m, &e);
-  float xs[XF_LB_IN], h1[XF_LB_HID];
-#pragma unroll
-  for (int i = 0; i < XF_LB_IN; ++i) xs[i] = ldexpf(x[i], -e);
-  const float* W1 = w;
-  const float* b1 = W1 + XF_LB_HID * XF_LB_IN;
-  const float* W2 = b1 + XF_LB_HID;
-  const float* b2 = W2 + XF_LB_HID * XF_LB_HID;
-  const float* w3 = b2 + XF_LB_HID;
-#pragma unroll
-  for (int o = 0; o < XF_LB_HID; ++o) {
-    float acc = b1[o];
-#pragma unroll
-    for (int i = 0; i < XF_LB_IN; ++i) acc = acc + W1[o * XF_LB_IN + i] * xs[i];
-    h1[o] = acc > 0.0f ? acc : 0.0f;
-  }
-  float r = w3[XF_LB_HID];
-  for (int o = 0; o < XF_LB_HID; ++o) {
-    float acc = b2[o];
-#pragma unroll
-    for (int i = 0; i < XF_LB_HID; ++i) acc = acc + W2[o * XF_LB_HID + i] * h1[i];
-    const float h2 = acc > 0.0f ? acc : 0.0f;
-    r = r + w3[o] * h2;
-  }
-  return mur + ldexpf(r, e);
-}
+// The column predictor of xf_learned_boundary runs on the matrix cores
+// (k_lb_mfma): the Mur update plus s times the MLP of x / s, s the power of
+// two above max |x_i| (exact scaling), with x / s and the layer-1
+// activations in fp16, products v_mfma_f32_32x32x16_f16 (fp16 x fp16, fp32
+// accumulation) and the output unit on the VALU.
+typedef _Float16 lb_h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 lb_h4 __attribute__((ext_vector_type(4)));
+typedef float lb_f16 __attribute__((ext_vector_type(16)));
+constexpr int kLbTilesPerWave = 1;  // 64-column tiles per wave (1 and 2 measure alike)
+constexpr int kLbMfmaCols = 4 * 64 * kLbTilesPerWave;  // columns per 256-thread block
 
+// A column and the fields of its face it needs (copied, so no pointer into
+// the kernel's argument escapes and the argument stays out of scratch).
 struct LbCol {
-  const LbFace* F;
-  int cc;
-  int p, q;
+  int f, c, a, b, positive;  // face, its normal axis and cyclic frame, side
+  int cc;                    // component: 0 E_a, 1 E_b
+  int p, q;                  // in-plane a- and b-index
+  int p1, q1;                // last a- and b-index of the component
+  int64_t hist_off, na1, nb1;
 };
 
 __device__ __forceinline__ LbCol lb_column(const LbDev& L, int64_t col) {
   int fi = 0;
   while (fi + 1 < L.nface && col >= L.face[fi + 1].start[0]) ++fi;
-  const LbFace& F = L.face[fi];
-  const int cc = col >= F.start[1] ? 1 : 0;
-  const int64_t local = col - F.start[cc];
-  const int64_t width = F.q1[cc] - L.inset + 1;
   LbCol r;
-  r.F = &F;
-  r.cc = cc;
+  r.f = L.face[fi].f;
+  r.c = L.face[fi].c;
+  r.a = L.face[fi].a;
+  r.b = L.face[fi].b;
+  r.positive = L.face[fi].positive;
+  r.hist_off = L.face[fi].hist_off;
+  r.na1 = L.face[fi].na1;
+  r.nb1 = L.face[fi].nb1;
+  r.cc = col >= L.face[fi].start[1] ? 1 : 0;
+  r.p1 = L.face[fi].p1[r.cc];
+  r.q1 = L.face[fi].q1[r.cc];
+  // a component of a face holds < 2^32 columns (checked at upload)
+  const uint32_t local = (uint32_t)(col - L.face[fi].start[r.cc]);
+  const uint32_t width = (uint32_t)(r.q1 - L.inset + 1);
   r.p = L.inset + (int)(local / width);
   r.q = L.inset + (int)(local % width);
   return r;
@@ -680,103 +662,212 @@ __device__ __forceinline__ LbCol lb_column(const LbDev& L, int64_t col) {
 __device__ __forceinline__ int64_t lb_node(const LbDev& L, const Dims& d, const LbCol& c,
                                            int layer) {
   int pos[3];
-  const LbFace& F = *c.F;
-  pos[F.c] = F.positive ? L.n[F.c] - L.inset - layer : L.inset + layer;
-  pos[F.a] = c.p;
-  pos[F.b] = c.q;
+  pos[c.c] = c.positive ? L.n[c.c] - L.inset - layer : L.inset + layer;
+  pos[c.a] = c.p;
+  pos[c.b] = c.q;
   return (int64_t)pos[0] * d.sx + (int64_t)pos[1] * d.sy + pos[2];
 }
 
 __device__ __forceinline__ float* lb_hist(const LbDev& L, const LbCol& c, int par, int layer,
                                           int p, int q) {
-  const LbFace& F = *c.F;
-  return L.hist + F.hist_off + ((((int64_t)par * 2 + c.cc) * 4 + layer) * F.na1 + p) * F.nb1 + q;
+  return L.hist + c.hist_off + ((((int64_t)par * 2 + c.cc) * 4 + layer) * c.na1 + p) * c.nb1 + q;
 }
 
 // The face a node of the column also lies on (its component tangential to
 // both), or -1.
 __device__ __forceinline__ int lb_other(const LbDev& L, const LbCol& c) {
-  const LbFace& F = *c.F;
   if (c.cc == 0) {
-    if (c.q == 0 && ((L.faces >> (2 * F.b)) & 1)) return 2 * F.b;
-    if (c.q == L.n[F.b] && ((L.faces >> (2 * F.b + 1)) & 1)) return 2 * F.b + 1;
+    if (c.q == 0 && ((L.faces >> (2 * c.b)) & 1)) return 2 * c.b;
+    if (c.q == L.n[c.b] && ((L.faces >> (2 * c.b + 1)) & 1)) return 2 * c.b + 1;
   } else {
-    if (c.p == 0 && ((L.faces >> (2 * F.a)) & 1)) return 2 * F.a;
-    if (c.p == L.n[F.a] && ((L.faces >> (2 * F.a + 1)) & 1)) return 2 * F.a + 1;
+    if (c.p == 0 && ((L.faces >> (2 * c.a)) & 1)) return 2 * c.a;
+    if (c.p == L.n[c.a] && ((L.faces >> (2 * c.a + 1)) & 1)) return 2 * c.a + 1;
   }
   return -1;
 }
 
-// phase 0: APPLY the faces' inner nodes / RECORD samples; phase 1: APPLY
-// the edge nodes (owned by the lower face); phase 2: history roll.
-// One column of the layer at time t; w holds the weights (LDS) in APPLY
-// phases 0 and 1.
+// The 12 inputs of a column at time t (x0..x2 u^{t+1} on layers 1..3, x3..x6
+// u^t on layers 0..3, x7..x9 u^{t-1} on layers 0..2, x10/x11 the in-plane
+// 5-point Laplacians of u^t on layers 1 and 0, a neighbour outside the
+// column range counting as the centre value).
 template <typename S>
-__device__ __forceinline__ void lb_column_step(const Fields<S>& f, const LbDev& L, const Dims& d,
-                                               const Scales<typename Store<S>::C>& sc, int phase,
-                                               int t, const float* w, int64_t col) {
-  using C = typename Store<S>::C;
-  const int par = t & 1;
-  const LbCol c = lb_column(L, col);
-  S* fld = f.e[c.cc == 0 ? c.F->a : c.F->b];
-  if (phase == 2) {
-#pragma unroll
-    for (int l = 0; l < 4; ++l)
-      *lb_hist(L, c, par ^ 1, l, c.p, c.q) = (float)Store<S>::ld(fld, lb_node(L, d, c, l), sc.inv_e);
-    return;
-  }
-  bool record = false;
-  if (L.mode == XF_LB_RECORD) {
-    record = (t % L.rec_every) == 0 && t / L.rec_every < L.rec_slots;
-    if (!record || phase != 0) return;
-  } else {
-    const int other = lb_other(L, c);
-    if (phase == 0 ? other >= 0 : !(other >= 0 && c.F->f < other)) return;
-  }
-  float x[XF_LB_IN];
+__device__ __forceinline__ void lb_inputs(const Fields<S>& f, const LbDev& L, const Dims& d,
+                                          const Scales<typename Store<S>::C>& sc, const LbCol& c,
+                                          int par, float (&x)[XF_LB_IN]) {
+  const S* fld = f.e[c.cc == 0 ? c.a : c.b];
 #pragma unroll
   for (int l = 1; l <= 3; ++l) x[l - 1] = (float)Store<S>::ld(fld, lb_node(L, d, c, l), sc.inv_e);
 #pragma unroll
   for (int l = 0; l < 4; ++l) x[3 + l] = *lb_hist(L, c, par, l, c.p, c.q);
 #pragma unroll
   for (int l = 0; l < 3; ++l) x[7 + l] = *lb_hist(L, c, par ^ 1, l, c.p, c.q);
-  // in-plane 5-point Laplacians of u^t on layers 1 (x10) and 0 (x11); a
-  // neighbour outside the column range counts as the centre value
 #pragma unroll
   for (int l = 1; l >= 0; --l) {
     const float u = x[3 + l];
-    const float up = c.p < c.F->p1[c.cc] ? *lb_hist(L, c, par, l, c.p + 1, c.q) : u;
+    const float up = c.p < c.p1 ? *lb_hist(L, c, par, l, c.p + 1, c.q) : u;
     const float um = c.p > L.inset ? *lb_hist(L, c, par, l, c.p - 1, c.q) : u;
-    const float vp = c.q < c.F->q1[c.cc] ? *lb_hist(L, c, par, l, c.p, c.q + 1) : u;
+    const float vp = c.q < c.q1 ? *lb_hist(L, c, par, l, c.p, c.q + 1) : u;
     const float vm = c.q > L.inset ? *lb_hist(L, c, par, l, c.p, c.q - 1) : u;
     x[11 - l] = (((up + um) + (vp + vm)) - 4.0f * u);
   }
+}
+
+// One column of the layer at time t in the VALU phases: RECORD phase 0
+// (training samples), APPLY phase 1 (edge nodes: the first-order Mur
+// update, the network being trained on face interiors), phase 2 (history
+// roll).  APPLY phase 0 is k_lb_mfma's.
+template <typename S>
+__device__ __forceinline__ void lb_column_step(const Fields<S>& f, const LbDev& L, const Dims& d,
+                                               const Scales<typename Store<S>::C>& sc, int phase,
+                                               int t, int64_t col) {
+  using C = typename Store<S>::C;
+  const int par = t & 1;
+  const LbCol c = lb_column(L, col);
+  S* fld = f.e[c.cc == 0 ? c.a : c.b];
+  if (phase == 2) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+      *lb_hist(L, c, par ^ 1, l, c.p, c.q) = (float)Store<S>::ld(fld, lb_node(L, d, c, l), sc.inv_e);
+    return;
+  }
+  if (L.mode == XF_LB_RECORD) {
+    if (phase != 0 || (t % L.rec_every) != 0 || t / L.rec_every >= L.rec_slots) return;
+  } else {
+    const int other = lb_other(L, c);
+    if (phase != 1 || !(other >= 0 && c.f < other)) return;
+  }
+  float x[XF_LB_IN];
+  lb_inputs<S>(f, L, d, sc, c, par, x);
   const int64_t node0 = lb_node(L, d, c, 0);
-  if (record) {
+  if (L.mode == XF_LB_RECORD) {
     float* r = L.rec + ((int64_t)(t / L.rec_every) * L.total + col) * 13;
 #pragma unroll
     for (int i = 0; i < XF_LB_IN; ++i) r[i] = x[i];
     r[12] = (float)Store<S>::ld(fld, node0, sc.inv_e);
     return;
   }
-  // edge nodes (phase 1) take the first-order Mur update: the network is
-  // trained on face interiors only
-  const float u1 = phase == 0 ? lb_predict(w, L.mur_k, x) : x[4] + L.mur_k * (x[0] - x[3]);
+  const float u1 = x[4] + L.mur_k * (x[0] - x[3]);
   Store<S>::st(fld, node0, (C)u1, sc.s_e);
+}
+
+// The MLP of one 32-column product tile: bx holds the tile's scaled inputs
+// in the B-operand layout (lane l: column l % 32, inputs 8g..8g+7); returns
+// the output unit's r for column l % 32 in both lanes l and l + 32.
+__device__ __forceinline__ float lb_mlp_tile(const lb_h8& bx, const lb_h8& a1, const lb_h8& a20,
+                                             const lb_h8& a21, const lb_f16& c1, const lb_f16& c2,
+                                             const float (&w3)[16], float b3) {
+  const lb_f16 z1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, bx, c1, 0, 0, 0);
+  lb_h8 h0, h1;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    float u = z1[j] > 0.0f ? z1[j] : 0.0f;
+    h0[j] = (_Float16)(u < 65504.0f ? u : 65504.0f);
+    u = z1[8 + j] > 0.0f ? z1[8 + j] : 0.0f;
+    h1[j] = (_Float16)(u < 65504.0f ? u : 65504.0f);
+  }
+  lb_f16 z2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a20, h0, c2, 0, 0, 0);
+  z2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a21, h1, z2, 0, 0, 0);
+  float s = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) s = s + w3[i] * (z2[i] > 0.0f ? z2[i] : 0.0f);
+  return (s + __shfl_xor(s, 32)) + b3;
+}
+
+// APPLY phase 0 for the tiles [col0, col0 + 64 * ntiles) of one wave.  Lane
+// l gathers and writes column col0 + l; the 64 columns are two product
+// tiles, A (lanes 0..31's columns) and B (lanes 32..63's).  Lane half g
+// feeds inputs 8g..8g+7 of its tile column, so lanes l and l ^ 32 swap the
+// half of their scaled inputs the other one feeds; lane half g then holds
+// accumulator rows 8(i / 4) + 4g + i % 4.  Every lane reaches every
+// product (no lane exits).
+template <typename S>
+__device__ __forceinline__ void lb_mfma_tiles(const Fields<S>& f, const LbDev& L, const Dims& d,
+                                              const Scales<typename Store<S>::C>& sc, int t,
+                                              int64_t col0, int ntiles) {
+  using C = typename Store<S>::C;
+  const int lane = threadIdx.x & 63, g = lane >> 5;
+  const LbPack& P = *L.pack;
+  const lb_h8 a1 = __builtin_bit_cast(lb_h8, *reinterpret_cast<const uint4*>(P.a1[lane]));
+  const lb_h8 a20 = __builtin_bit_cast(lb_h8, *reinterpret_cast<const uint4*>(P.a2[0][lane]));
+  const lb_h8 a21 = __builtin_bit_cast(lb_h8, *reinterpret_cast<const uint4*>(P.a2[1][lane]));
+  lb_f16 c1, c2;
+  float w3[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    c1[i] = P.c1[i][lane];
+    c2[i] = P.c2[i][lane];
+    w3[i] = P.w3[i][lane];
+  }
+  const float b3 = P.b3;
+  const int par = t & 1;
+  for (int q = 0; q < ntiles; ++q) {
+    const int64_t col = col0 + 64 * q + lane;
+    bool live = col < L.total;
+    LbCol c{};
+    if (live) {
+      c = lb_column(L, col);
+      live = lb_other(L, c) < 0;  // edge nodes are phase 1's
+    }
+    float x[XF_LB_IN];
+#pragma unroll
+    for (int i = 0; i < XF_LB_IN; ++i) x[i] = 0.0f;
+    if (live) lb_inputs<S>(f, L, d, sc, c, par, x);
+    float m = 0.0f;
+#pragma unroll
+    for (int i = 0; i < XF_LB_IN; ++i) {
+      const float ax = fabsf(x[i]);
+      if (ax > m) m = ax;
+    }
+    const float mur = x[4] + L.mur_k * (x[0] - x[3]);
+    int e = 0;
+    if (m > 0.0f) (void)frexpf(m, &e);
+    lb_h8 lo;
+    lb_h4 hi;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) lo[j] = (_Float16)ldexpf(x[j], -e);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) hi[j] = (_Float16)ldexpf(x[8 + j], -e);
+    const uint4 lo_u = __builtin_bit_cast(uint4, lo);
+    const uint2 hi_u = __builtin_bit_cast(uint2, hi);
+    // half 0 sends inputs 8..11 (its partner feeds them for tile A), half 1
+    // sends inputs 0..7 (its partner feeds them for tile B)
+    // (selects dword by dword: a select of whole vectors goes through scratch)
+    const bool h0 = g == 0;
+    const unsigned r0 = __shfl_xor(h0 ? hi_u.x : lo_u.x, 32);
+    const unsigned r1 = __shfl_xor(h0 ? hi_u.y : lo_u.y, 32);
+    const unsigned r2 = __shfl_xor(h0 ? 0u : lo_u.z, 32);
+    const unsigned r3 = __shfl_xor(h0 ? 0u : lo_u.w, 32);
+    const lb_h8 bA = __builtin_bit_cast(lb_h8, make_uint4(h0 ? lo_u.x : r0, h0 ? lo_u.y : r1,
+                                                          h0 ? lo_u.z : 0u, h0 ? lo_u.w : 0u));
+    const lb_h8 bB = __builtin_bit_cast(lb_h8, make_uint4(h0 ? r0 : hi_u.x, h0 ? r1 : hi_u.y,
+                                                          h0 ? r2 : 0u, h0 ? r3 : 0u));
+    const float rA = lb_mlp_tile(bA, a1, a20, a21, c1, c2, w3, b3);
+    const float rB = lb_mlp_tile(bB, a1, a20, a21, c1, c2, w3, b3);
+    if (live) {
+      const float r = g == 0 ? rA : rB;
+      const float u1 = m == 0.0f ? mur : mur + ldexpf(r, e);
+      S* fld = f.e[c.cc == 0 ? c.a : c.b];
+      Store<S>::st(fld, lb_node(L, d, c, 0), (C)u1, sc.s_e);
+    }
+  }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_lb_mfma(Fields<S> f, LbDev L, Dims d, Scales<typename Store<S>::C> sc, int t_arg,
+              const int* t_dev) {
+  const int64_t col0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 * kLbTilesPerWave);
+  if (col0 >= L.total) return;  // whole waves only
+  lb_mfma_tiles<S>(f, L, d, sc, t_dev ? *t_dev : t_arg, col0, kLbTilesPerWave);
 }
 
 template <typename S>
 __global__ void __launch_bounds__(256)
     k_lb(Fields<S> f, LbDev L, Dims d, Scales<typename Store<S>::C> sc, int phase, int t_arg,
          const int* t_dev) {
-  __shared__ float w[XF_LB_NPARAM];
-  if (phase < 2 && L.mode == XF_LB_APPLY) {
-    for (int i = threadIdx.x; i < XF_LB_NPARAM; i += blockDim.x) w[i] = L.w[i];
-    __syncthreads();
-  }
   const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= L.total) return;
-  lb_column_step<S>(f, L, d, sc, phase, t_dev ? *t_dev : t_arg, w, col);
+  lb_column_step<S>(f, L, d, sc, phase, t_dev ? *t_dev : t_arg, col);
 }
 
 // ----------------------------------------------------------- batched forms
@@ -835,23 +926,32 @@ __global__ void __launch_bounds__(64)
   }
 }
 
-// One phase of the learned layers of every grid of a batch in one launch.
-// Item g owns blocks [bstart[g], bstart[g + 1]), so a block serves one grid
-// and stages that grid's weights.
+// One VALU phase of the learned layers of every grid of a batch in one
+// launch.  Item g owns blocks [bstart[g], bstart[g + 1]) of 256 columns.
 template <typename S>
 __global__ void __launch_bounds__(256)
     k_lb_batch(const LbItem<S>* __restrict__ items, const int64_t* __restrict__ bstart, int n,
                int phase, int t_arg, const int* t_dev) {
-  __shared__ float w[XF_LB_NPARAM];
   const int g = owner_of(bstart, n, (int64_t)blockIdx.x);
   const LbItem<S>& I = items[g];
-  if (phase < 2 && I.L.mode == XF_LB_APPLY) {
-    for (int i = threadIdx.x; i < XF_LB_NPARAM; i += blockDim.x) w[i] = I.L.w[i];
-    __syncthreads();
-  }
   const int64_t col = ((int64_t)blockIdx.x - bstart[g]) * blockDim.x + threadIdx.x;
   if (col >= I.L.total) return;
-  lb_column_step<S>(I.f, I.L, I.d, I.sc, phase, t_dev ? *t_dev : t_arg, w, col);
+  lb_column_step<S>(I.f, I.L, I.d, I.sc, phase, t_dev ? *t_dev : t_arg, col);
+}
+
+// APPLY phase 0 of every learned layer of a batch in one launch.  Item g
+// owns blocks [bstart[g], bstart[g + 1]) of kLbMfmaCols columns; RECORD
+// layers own no blocks.
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_lb_mfma_batch(const LbItem<S>* __restrict__ items, const int64_t* __restrict__ bstart, int n,
+                    int t_arg, const int* t_dev) {
+  const int g = owner_of(bstart, n, (int64_t)blockIdx.x);
+  const LbItem<S>& I = items[g];
+  if (I.L.mode != XF_LB_APPLY) return;
+  const int64_t col0 = (((int64_t)blockIdx.x - bstart[g]) * 4 + (threadIdx.x >> 6)) * (64 * kLbTilesPerWave);
+  if (col0 >= I.L.total) return;  // whole waves only
+  lb_mfma_tiles<S>(I.f, I.L, I.d, I.sc, t_dev ? *t_dev : t_arg, col0, kLbTilesPerWave);
 }
 
 // ------------------------------------------------------------------ ADE
