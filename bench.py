@@ -67,7 +67,8 @@ def main(argv=None) -> int:
                     help="default: fp64 for the sphere, fp32 for the batch")
     ap.add_argument("--n", type=int, default=512, help="interior cells per edge")
     ap.add_argument("--pml", type=int, default=12)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=25,
+                    help="FP64 steps the CPU oracle is timed over (about 11 s at 536^3 with 16 threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--workload", default="sphere",
@@ -81,6 +82,9 @@ def main(argv=None) -> int:
                          "scenes: every quoted scene on the GPU and the CPU oracle")
     ap.add_argument("--tile-y", type=int, default=0, help="rows per Yee tile (0: engine default)")
     ap.add_argument("--chunks", type=int, default=0, help="marches along x (0: engine default)")
+    ap.add_argument("--pml-fused", action="store_true",
+                    help="z-face CPML corrections inside the Yee kernels instead of a launch of "
+                         "their own (sphere workload)")
     ap.add_argument("--td-boxes", type=int, default=3, help="NF2FF-TD boxes of nf2ff_td")
     ap.add_argument("--count", type=int, default=64, help="grids of the batch workload")
     args = ap.parse_args(argv)
@@ -117,8 +121,9 @@ def main(argv=None) -> int:
     (nx, ny, nz), dt, _ = sim.info()
     cells = nx * ny * nz
     eng = xf.Engine(sim, precision=args.precision, timing=False).upload()
-    if args.tile_y or args.chunks:
-        eng.set_tiling(args.tile_y or 1, args.chunks, xcd_remap=False, nontemporal=True)
+    if args.tile_y or args.chunks or args.pml_fused:
+        eng.set_tiling(args.tile_y or 1, args.chunks, xcd_remap=False, nontemporal=True,
+                       pml_split=not args.pml_fused)
     setup_s = time.perf_counter() - t_setup
     print(f"# setup {setup_s:.1f} s, grid {nx}x{ny}x{nz}, {eng.describe()}", file=sys.stderr,
           flush=True)
@@ -153,9 +158,11 @@ def main(argv=None) -> int:
         kernel, avg_ms, alg_bytes = "k_yee_e3", e_ms, bytes_e
     else:
         kernel, avg_ms, alg_bytes = "k_yee_h3", h_ms, bytes_h
+    if "pml_split=1" in eng.describe():
+        kernel += "+k_pml_z"  # the pass: Yee kernel, then its z-face CPML launch
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{args.precision}_{nx}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{kernel.replace('+', '_')}_{args.precision}_{nx}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")

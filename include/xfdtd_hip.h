@@ -304,6 +304,11 @@ typedef struct xf_plan {
 enum xf_tile_flags {
   XF_TILE_XCD_REMAP = 1, /* XCD-aware block order                        */
   XF_TILE_NT = 4,        /* non-temporal loads/stores of the updated field */
+  XF_TILE_PML_SPLIT = 8, /* z-face CPML corrections in a launch of their own
+                            after each Yee pass (3-D, FP64/FP32 only; the
+                            results are unchanged, the z faces being the
+                            last corrections of a node either way; on by
+                            default when tiling_flags is 0) */
   XF_TILE_SET = 256      /* the other bits are explicit, not auto        */
 };
 
@@ -386,6 +391,7 @@ typedef struct xf_tuning {
   int32_t blocks;      /* batch only: persistent grid size, 0 = resident   */
   int32_t nontemporal; /* stream the updated component non-temporally      */
   int32_t xcd_remap;   /* single grid only: XCD-aware block order          */
+  int32_t pml_split;   /* XF_TILE_PML_SPLIT                                */
   double step_ms;      /* autotune: measured time of one step              */
 } xf_tuning;
 

@@ -88,18 +88,20 @@ def test_batch_matches_oracle_grid_by_grid(gpu_available):
     assert np.abs(ref[0].array("ex")).max() > 1e-3
 
 
-@pytest.mark.parametrize("tile_y,planes,blocks,nt", [
-    (1, 1, 0, True), (2, 3, 7, False), (4, 100, 1, True), (3, 2, 0, False), (1, 4, 13, True)])
-def test_batch_tunings_bit_identical(gpu_available, tile_y, planes, blocks, nt):
+@pytest.mark.parametrize("tile_y,planes,blocks,nt,split", [
+    (1, 1, 0, True, False), (2, 3, 7, False, True), (4, 100, 1, True, False), (3, 2, 0, False, True),
+    (1, 4, 13, True, True)])
+def test_batch_tunings_bit_identical(gpu_available, tile_y, planes, blocks, nt, split):
     steps = 30
     build = lambda: _uneven(steps, count=4, seed=11)
     ref = _oracle_copies(build, steps)
     sims = build()
     b = xf.Batch(sims).upload()
-    b.set_tuning(tile_y=tile_y, planes=planes, blocks=blocks, nontemporal=nt)
+    b.set_tuning(tile_y=tile_y, planes=planes, blocks=blocks, nontemporal=nt, pml_split=split)
+    assert ("pml_split=0" not in b.describe()) == split
     b.run(steps).download()
     b.close()
-    _assert_identical(sims, ref, f"tuning ty={tile_y} planes={planes} blocks={blocks}")
+    _assert_identical(sims, ref, f"tuning ty={tile_y} planes={planes} blocks={blocks} split={split}")
 
 
 def test_batch_graph_replay_matches_eager(gpu_available):

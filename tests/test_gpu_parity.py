@@ -107,16 +107,20 @@ def test_sphere_uneven_sizes_and_steps_in_chunks(gpu_available):
     _compare(g, c, label="uneven")
 
 
-@pytest.mark.parametrize("tile_y,chunks,xcd,nt", [
-    (1, 0, False, True), (2, 0, False, False), (3, 7, True, True), (4, 1, False, True),
-    (1, 1000, True, False), (2, 3, False, True)])
-def test_tiling_variants_bit_identical(gpu_available, tile_y, chunks, xcd, nt):
-    """Every tile shape / march length / block order / streaming flag gives
-    the same bits: the tiling only reorders independent cell updates."""
+@pytest.mark.parametrize("tile_y,chunks,xcd,nt,split", [
+    (1, 0, False, True, False), (2, 0, False, False, False), (3, 7, True, True, False),
+    (4, 1, False, True, False), (1, 1000, True, False, False), (2, 3, False, True, False),
+    (1, 0, False, True, True), (4, 1, False, False, True), (2, 3, True, True, True)])
+def test_tiling_variants_bit_identical(gpu_available, tile_y, chunks, xcd, nt, split):
+    """Every tile shape / march length / block order / streaming flag, and
+    the z-face CPML corrections fused or in a launch of their own, gives the
+    same bits: the tiling only reorders independent cell updates, and a
+    node's z-face corrections come last either way."""
     build = lambda: scenes.dielectric_sphere(n=21, pml=5, steps=60, tfsf_inset=8, nf2ff_inset=6)
     g, c = build(), build()
     eng = xf.Engine(g).upload()
-    eng.set_tiling(tile_y=tile_y, chunks_x=chunks, xcd_remap=xcd, nontemporal=nt)
+    eng.set_tiling(tile_y=tile_y, chunks_x=chunks, xcd_remap=xcd, nontemporal=nt, pml_split=split)
+    assert ("pml_split=1" in eng.describe()) == split
     eng.run(60).download()
     eng.close()
     oracle.run(c.scene_ptr(), 60)
@@ -243,6 +247,43 @@ def test_ade(gpu_available, kind):
         # (multiplied by zero coefficients); J stays exactly zero there.
         assert rel_l2(g.array(n), c.array(n)) <= TOL, n
     assert np.abs(c.array("ade.jx")).max() > 0
+
+
+def test_pml_split_uneven_faces_fp32_and_bf16(gpu_available):
+    """The z-face split on a grid with a different CPML thickness on every
+    face and three chunked runs: FP64 bit-identical to the oracle, FP32
+    bit-identical to the fused FP32 run; BF16 storage keeps the corrections
+    fused (the split would round the intermediate field)."""
+    def build():
+        s = xf.Simulation(3e-3, 3e-3, 3e-3, 0.95)
+        s.add_cube("domain", (-0.05, -0.04, -0.062), (0.1, 0.08, 0.124), xf.Material.air())
+        s.add_sphere("s", (0.004, -0.003, 0.01), 0.021, xf.Material.dielectric(4.0, 1.5, 0.01))
+        s.add_pml(5, "XN").add_pml(7, "XP").add_pml(4, "YN").add_pml(6, "YP")
+        s.add_pml(3, "ZN").add_pml(9, "ZP")
+        tau = 20 * 3e-3 / 6e8
+        s.add_tfsf3d((11, 9, 12), 0.7, 0.4, 0.3, xf.Waveform.gaussian(tau, 4.5 * tau))
+        s.add_nf2ff_fd((9, 8, 10), [1.5e9])
+        s.init(120)
+        return s
+
+    def run(sim, precision, split):
+        e = xf.Engine(sim, precision=precision).upload()
+        e.set_tiling(tile_y=2, pml_split=split)
+        desc = e.describe()
+        e.run(30).run(50).run(40).download()
+        e.close()
+        return desc
+
+    g, c = build(), build()
+    assert "pml_split=1" in run(g, "fp64", True)
+    oracle.run(c.scene_ptr(), 120)
+    _compare(g, c, label="split fp64")
+    a, b = build(), build()
+    run(a, "fp32", True)
+    run(b, "fp32", False)
+    for n in FIELDS:
+        assert np.array_equal(a.array(n), b.array(n)), n
+    assert "pml_split=0" in run(build(), "bf16", True)
 
 
 def test_fp32_close_to_fp64(gpu_available):

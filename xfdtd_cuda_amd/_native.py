@@ -48,6 +48,7 @@ class XfPlan(ctypes.Structure):
 
 TILE_XCD_REMAP = 1
 TILE_NT = 4
+TILE_PML_SPLIT = 8
 TILE_SET = 256
 
 
@@ -78,6 +79,7 @@ class XfTuning(ctypes.Structure):
         ("blocks", c_i32),
         ("nontemporal", c_i32),
         ("xcd_remap", c_i32),
+        ("pml_split", c_i32),
         ("step_ms", c_f64),
     ]
 

@@ -27,7 +27,7 @@ struct Batch {
   int id_bytes{1};
   bool any_pml{false};
   int64_t cells{0};
-  xf_tuning tune{1, 4, 0, 1, 0, 0.0};
+  xf_tuning tune{1, 4, 0, 1, 0, 1, 0.0};
   // device descriptors (sizes fixed by the scenes, rewritten by configure)
   std::vector<void*> allocs;
   void* pass_h{};
@@ -44,6 +44,10 @@ struct Batch {
   int64_t* nf_start{};
   int n_nf{0};
   int64_t nf_total{0};
+  void* pz_items[2]{};     // z-face CPML items of the H and E passes
+  int64_t* pz_start[2]{};
+  int n_pz[2]{0, 0};
+  int64_t pz_total[2]{0, 0};
   void* pt_items{};
   int n_pt{0};
   void* lb_items{};
@@ -77,6 +81,8 @@ struct Batch {
     tile_start = nullptr;
     tf_items[0] = tf_items[1] = nf_items = pt_items = lb_items = nullptr;
     tf_start[0] = tf_start[1] = nf_start = lb_bstart = lb_mstart = nullptr;
+    pz_items[0] = pz_items[1] = nullptr;
+    pz_start[0] = pz_start[1] = nullptr;
     t_dev = nullptr;
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
@@ -207,7 +213,7 @@ int Batch::configure_t() {
   std::vector<int> ts(n + 1, 0);
   int64_t tiles = 0;
   any_pml = false;
-  const int flags = XF_TILE_SET | (tune.nontemporal ? XF_TILE_NT : 0);
+  const int flags = XF_TILE_SET | (tune.nontemporal ? XF_TILE_NT : 0) | (tune.pml_split ? XF_TILE_PML_SPLIT : 0);
   for (int g = 0; g < n; ++g) {
     Engine& E = eng(g);
     const int64_t chunks = (E.nx + 1 + tune.planes - 1) / tune.planes;
@@ -226,6 +232,35 @@ int Batch::configure_t() {
   void* p = tile_start;
   XF_TRY(put(&p, ts.data(), ts.size() * sizeof(int)));
   tile_start = static_cast<int*>(p);
+  // z-face CPML corrections of the grids that split them off
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<PmlZItem<S>> items;
+    std::vector<int64_t> st(1, 0);
+    for (int g = 0; g < n; ++g) {
+      Engine& E = eng(g);
+      if (!E.split_z()) continue;
+      PmlZItem<S> it{};
+      it.f = make_fields<S>(E);
+      it.sc = make_scales<C>(E);
+      it.d = make_dims(E);
+      fill_pml_pass<C>(E, it.P, pass == 1, 2);
+      it.B = pml_z_box<C>(E, it.P);
+      if (it.B.start[it.B.n] == 0) continue;
+      items.push_back(it);
+      st.push_back(st.back() + it.B.start[it.B.n]);
+    }
+    n_pz[pass] = static_cast<int>(items.size());
+    pz_total[pass] = st.back();
+    // sized for every grid, so a later configure with more splitting grids fits
+    std::vector<PmlZItem<S>> full(n);
+    std::copy(items.begin(), items.end(), full.begin());
+    std::vector<int64_t> stf(n + 1, st.back());
+    std::copy(st.begin(), st.end(), stf.begin());
+    XF_TRY(put(&pz_items[pass], full.data(), full.size() * sizeof(PmlZItem<S>)));
+    void* q = pz_start[pass];
+    XF_TRY(put(&q, stf.data(), stf.size() * sizeof(int64_t)));
+    pz_start[pass] = static_cast<int64_t*>(q);
+  }
   // TFSF passes and NF2FF boxes
   for (int pass = 0; pass < 2; ++pass) {
     std::vector<TfsfItem<S>> items;
@@ -345,7 +380,8 @@ int Batch::configure() {
          " tiles=" + std::to_string(total_tiles) + " blocks=" + std::to_string(grid_blocks) +
          " resident=" + std::to_string(resident) + " tile_y=" + std::to_string(tune.tile_y) +
          " planes=" + std::to_string(tune.planes) + " nt=" + std::to_string(tune.nontemporal) +
-         " id_bytes=" + std::to_string(id_bytes) + " tfsf=" + std::to_string(n_tf[0]) +
+         " id_bytes=" + std::to_string(id_bytes) + " pml_split=" + std::to_string(n_pz[1]) +
+         " tfsf=" + std::to_string(n_tf[0]) +
          " nf2ff=" + std::to_string(n_nf);
   return XF_OK;
 }
@@ -370,12 +406,18 @@ int Batch::launch_step(int t, bool use_tdev, bool ev_on, hipEvent_t* ev) {
   } while (0)
   if (ev_on) XF_HIP(hipEventRecord(ev[0], stream));
   XF_BYALL(k_yee_h3_batch, pass_h);
+  if (pz_total[0] > 0)
+    hipLaunchKernelGGL((k_pml_z_batch<S, false>), dim3((unsigned)((pz_total[0] + 255) / 256)), dim3(256), 0,
+                       stream, static_cast<const PmlZItem<S>*>(pz_items[0]), pz_start[0], n_pz[0]);
   if (ev_on) XF_HIP(hipEventRecord(ev[1], stream));
   if (tf_total[0] > 0)
     hipLaunchKernelGGL((k_tfsf_batch<S>), dim3((unsigned)((tf_total[0] + 255) / 256)), dim3(256), 0, stream,
                        static_cast<const TfsfItem<S>*>(tf_items[0]), tf_start[0], n_tf[0], t, tp);
   if (ev_on) XF_HIP(hipEventRecord(ev[2], stream));
   XF_BYALL(k_yee_e3_batch, pass_e);
+  if (pz_total[1] > 0)
+    hipLaunchKernelGGL((k_pml_z_batch<S, true>), dim3((unsigned)((pz_total[1] + 255) / 256)), dim3(256), 0,
+                       stream, static_cast<const PmlZItem<S>*>(pz_items[1]), pz_start[1], n_pz[1]);
   if (ev_on) XF_HIP(hipEventRecord(ev[3], stream));
   if (tf_total[1] > 0)
     hipLaunchKernelGGL((k_tfsf_batch<S>), dim3((unsigned)((tf_total[1] + 255) / 256)), dim3(256), 0, stream,
@@ -530,11 +572,21 @@ int Batch::autotune(int trial_steps, xf_tuning* best) {
   xf_tuning win{};
   win.step_ms = 1e30;
   int status = XF_OK;
+  // the z-face split only where some grid can take it
+  bool can_split = false;
+  for (auto& h : inst) {
+    const int keep_split = h->eng.pml_split;
+    h->eng.pml_split = 1;
+    can_split = can_split || h->eng.split_z();
+    h->eng.pml_split = keep_split;
+  }
   for (int ty : {1, 2, 4}) {
     for (int planes : {2, 4, 8}) {
       for (int nt_on : {1, 0}) {
-        for (int half : {0, 1}) {
-          tune = xf_tuning{ty, planes, 0, nt_on, 0, 0.0};
+        for (int cand = 0; cand < 4; ++cand) {
+          const int half = cand & 1, split = cand >> 1;
+          if (split && !can_split) continue;
+          tune = xf_tuning{ty, planes, 0, nt_on, 0, split, 0.0};
           status = configure();
           if (status == XF_OK && half) {
             tune.blocks = std::max(1, resident / 2);
@@ -589,18 +641,27 @@ int Engine::autotune(int trial_steps, xf_tuning* best) {
   int status = XF_OK;
   const std::vector<int> tys = dim == 3 ? std::vector<int>{1, 2, 4} : std::vector<int>{1};
   const std::vector<int> pls = dim == 3 ? std::vector<int>{2, 4, 8, 16} : std::vector<int>{1};
+  // the z-face split only where it changes the launches
+  const int keep_split = pml_split;
+  pml_split = 1;
+  const bool can_split = split_z();
+  pml_split = keep_split;
   for (int ty : tys) {
     for (int planes : pls) {
       for (int nt_on : {1, 0}) {
-        const int chunks = static_cast<int>((nx + 1 + planes - 1) / planes);
-        set_tiling(ty, chunks, XF_TILE_SET | (nt_on ? XF_TILE_NT : 0));
-        status = run(1);
-        if (status == XF_OK) status = run(trial_steps);
-        if (status != XF_OK) break;
-        const double ms = timing.run_ms / trial_steps;
-        if (ms < win.step_ms) win = xf_tuning{ty, planes, 0, nt_on, 0, ms};
-        cur_step = step0;
-        status = restore();
+        for (int split : {0, 1}) {
+          if (split && !can_split) continue;
+          const int chunks = static_cast<int>((nx + 1 + planes - 1) / planes);
+          set_tiling(ty, chunks, XF_TILE_SET | (nt_on ? XF_TILE_NT : 0) | (split ? XF_TILE_PML_SPLIT : 0));
+          status = run(1);
+          if (status == XF_OK) status = run(trial_steps);
+          if (status != XF_OK) break;
+          const double ms = timing.run_ms / trial_steps;
+          if (ms < win.step_ms) win = xf_tuning{ty, planes, 0, nt_on, 0, split, ms};
+          cur_step = step0;
+          status = restore();
+          if (status != XF_OK) break;
+        }
         if (status != XF_OK) break;
       }
       if (status != XF_OK) break;
@@ -615,7 +676,7 @@ int Engine::autotune(int trial_steps, xf_tuning* best) {
     return status;
   }
   set_tiling(win.tile_y, static_cast<int>((nx + 1 + win.planes - 1) / win.planes),
-             XF_TILE_SET | (win.nontemporal ? XF_TILE_NT : 0));
+             XF_TILE_SET | (win.nontemporal ? XF_TILE_NT : 0) | (win.pml_split ? XF_TILE_PML_SPLIT : 0));
   if (best) *best = win;
   return XF_OK;
 }

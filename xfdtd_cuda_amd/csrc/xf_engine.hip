@@ -157,6 +157,7 @@ struct Engine {
   int nkb{1}, njb{1}, nch_eff{1};
   int xcd{0};
   int nt_streams{1};
+  int pml_split{0};  // XF_TILE_PML_SPLIT requested (see split_z())
   void set_tiling(int ty, int chunks_x, int flags);
   Tiling tiling() const;
   int snapshot();  // copy every state buffer aside (device to device)
@@ -256,6 +257,10 @@ struct Engine {
   int launch_additions(int t, const int* tp, hipStream_t st);
   template <typename S>
   int launch_movies(int t, const int* tp, hipStream_t st);
+  // z-face CPML corrections in a launch of their own after each Yee pass
+  bool split_z() const;
+  template <typename S>
+  int launch_pml_z(bool is_e, hipStream_t st);
   template <typename S>
   int launch_td(int t, const int* tp, hipStream_t st);
 
@@ -1379,9 +1384,13 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
   if (flags & XF_TILE_SET) {
     xcd = (flags & XF_TILE_XCD_REMAP) ? 1 : 0;
     nt_streams = (flags & XF_TILE_NT) ? 1 : 0;
+    pml_split = (flags & XF_TILE_PML_SPLIT) ? 1 : 0;
   } else {
+    // defaults: non-temporal streams; z-face CPML corrections split off
+    // (measured at 536^3: FP32 +3 %, FP64 even; 64 small grids: +13 %)
     xcd = 0;
     nt_streams = 1;
+    pml_split = 1;
   }
   if (dim == 3) {
     nkb = static_cast<int>((nz + 1 + 63) / 64);
@@ -1404,7 +1413,7 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
                 " id_bytes=" + std::to_string(id_bytes) + " etab=" + std::to_string(ntab_e) +
                 " htab=" + std::to_string(ntab_h) + " chunk=" + std::to_string(chunk) +
                 " nchunk=" + std::to_string(nch_eff) + " tile_y=" + std::to_string(tile_y) +
-                " xcd=" + std::to_string(xcd) + " nt=" + std::to_string(nt_streams) +
+                " xcd=" + std::to_string(xcd) + " nt=" + std::to_string(nt_streams) + " pml_split=" + std::to_string(split_z() ? 1 : 0) +
                 " pml_faces=" + std::to_string(pml.size()) +
                 " ade_cells=" + std::to_string(ade_cells.size());
 }
@@ -1528,14 +1537,17 @@ int Engine::download() {
 
 // ------------------------------------------------------------- launches
 namespace {
+// faces: 0 every face, 1 all but the z faces, 2 the z faces only (faces
+// keep their registration order: X, Y, Z)
 template <typename C>
-void fill_pml_pass(const Engine& E, PmlPass<C>& P, bool is_e) {
-  P.n = static_cast<int>(E.pml_order.size());
-  for (int q = 0; q < P.n; ++q) {
+void fill_pml_pass(const Engine& E, PmlPass<C>& P, bool is_e, int faces = 0) {
+  P.n = 0;
+  for (int q = 0; q < static_cast<int>(E.pml_order.size()); ++q) {
     const xf_pml& S = E.sc->pml[E.pml_order[q]];
     const auto& D = E.pml[q];
-    PmlFace<C>& F = P.f[q];
     const int ax = S.main_axis, a = (ax + 1) % 3, b = (ax + 2) % 3;
+    if ((faces == 1 && ax == 2) || (faces == 2 && ax != 2)) continue;
+    PmlFace<C>& F = P.f[P.n++];
     F.axis = ax;
     const int64_t cs = task_lo(S.task, ax), ce = task_hi(S.task, ax);
     const int64_t off = (is_e && cs == 0) ? 1 : 0;
@@ -1611,8 +1623,68 @@ YeePass<S> Engine::yee_pass(bool is_e) const {
   D.njb = njb;
   D.chunk = chunk;
   D.nch = nch_eff;
-  fill_pml_pass<C>(*this, D.pml, is_e);
+  fill_pml_pass<C>(*this, D.pml, is_e, split_z() ? 1 : 0);
   return D;
+}
+
+// The split applies to 3-D FP64/FP32 grids with one or two z faces whose
+// corrected ranges do not overlap (each node then gets its z corrections
+// from one thread) and fewer than 2^31 nodes per face box.  BF16 storage
+// keeps them fused: the split would round the intermediate field.
+bool Engine::split_z() const {
+  if (!pml_split || dim != 3 || (plan.precision != XF_FP64 && plan.precision != XF_FP32)) return false;
+  PmlPass<double> P{};
+  for (int pass = 0; pass < 2; ++pass) {
+    fill_pml_pass<double>(*this, P, pass == 1, 2);
+    if (P.n == 0 || P.n > 2) return false;
+    if (P.n == 2 && P.f[0].c1 > P.f[1].c0 && P.f[1].c1 > P.f[0].c0) return false;
+    for (int q = 0; q < P.n; ++q) {
+      const int64_t ni = std::min<int64_t>(P.f[q].a1, nx) - P.f[q].a0 + 1;
+      const int64_t nj = std::min<int64_t>(P.f[q].b1, ny) - P.f[q].b0 + 1;
+      const int64_t nk = std::min<int64_t>(P.f[q].c1, nz + 1) - P.f[q].c0;
+      if (ni > 0 && nj > 0 && nk > 0 && ni * nj * nk >= (int64_t(1) << 31)) return false;
+    }
+  }
+  return true;
+}
+
+template <typename C>
+PmlZBox pml_z_box(const Engine& E, const PmlPass<C>& P) {
+  PmlZBox B{};
+  B.n = P.n;
+  B.start[0] = 0;
+  for (int q = 0; q < P.n; ++q) {
+    const PmlFace<C>& F = P.f[q];
+    const int64_t ni = std::max<int64_t>(0, std::min<int64_t>(F.a1, E.nx) - F.a0 + 1);
+    const int64_t nj = std::max<int64_t>(0, std::min<int64_t>(F.b1, E.ny) - F.b0 + 1);
+    const int64_t nk = std::max<int64_t>(0, std::min<int64_t>(F.c1, E.nz + 1) - F.c0);
+    B.i0[q] = F.a0;
+    B.j0[q] = F.b0;
+    B.k0[q] = F.c0;
+    B.nj[q] = static_cast<int>(std::max<int64_t>(nj, 1));
+    B.nk[q] = static_cast<int>(std::max<int64_t>(nk, 1));
+    B.start[q + 1] = B.start[q] + ni * nj * nk;
+  }
+  return B;
+}
+
+template <typename S>
+int Engine::launch_pml_z(bool is_e, hipStream_t st) {
+  using C = typename Store<S>::C;
+  PmlPass<C> P{};
+  fill_pml_pass<C>(*this, P, is_e, 2);
+  const PmlZBox B = pml_z_box<C>(*this, P);
+  const int64_t total = B.start[B.n];
+  if (total == 0) return XF_OK;
+  const dim3 grid((unsigned)((total + 255) / 256));
+  if (is_e)
+    hipLaunchKernelGGL((k_pml_z<S, true>), grid, dim3(256), 0, st, make_fields<S>(*this), P, B,
+                       make_dims(*this), make_scales<C>(*this));
+  else
+    hipLaunchKernelGGL((k_pml_z<S, false>), grid, dim3(256), 0, st, make_fields<S>(*this), P, B,
+                       make_dims(*this), make_scales<C>(*this));
+  XF_HIP(hipGetLastError());
+  return XF_OK;
 }
 
 template <typename S>
@@ -1631,6 +1703,7 @@ int Engine::launch_h(int t) {
   const C* tab = static_cast<const C*>(htab);
   if (dim == 3) {
     const YeePass<S> D = yee_pass<S>(false);
+    const bool has_pml = D.pml.n > 0;
     dim3 block(64, tile_y);
     dim3 grid((unsigned)(D.nkb * D.njb * D.nch));
 #define XF_LH3(I, L, PM)                                                                     \
@@ -1648,6 +1721,7 @@ int Engine::launch_h(int t) {
       else { if (has_pml) XF_LH3(uint16_t, false, true); else XF_LH3(uint16_t, false, false); }
     }
 #undef XF_LH3
+    if (split_z()) XF_TRY(launch_pml_z<S>(false, stream));
   } else {
     dim3 block(64, 4);
     dim3 grid((unsigned)((ny + 1 + 63) / 64), (unsigned)((nx + 1 + 3) / 4));
@@ -1683,6 +1757,7 @@ int Engine::launch_e(int t) {
   const C* tab = static_cast<const C*>(etab);
   if (dim == 3) {
     const YeePass<S> D = yee_pass<S>(true);
+    const bool has_pml = D.pml.n > 0;
     dim3 block(64, tile_y);
     dim3 grid((unsigned)(D.nkb * D.njb * D.nch));
 #define XF_LE3(I, L, PM, AD)                                                                     \
@@ -1704,6 +1779,7 @@ int Engine::launch_e(int t) {
 #undef XF_LE3A
 #undef XF_LE3
     XF_HIP(hipGetLastError());
+    if (split_z()) XF_TRY(launch_pml_z<S>(true, stream));
     if (has_ade && ade.n > 0) {
       dim3 b1(256), g1((unsigned)((ade.n + 255) / 256));
       if (id_bytes == 1)

@@ -235,6 +235,16 @@ struct PmlPass {
   PmlFace<C> f[kMaxPml];
 };
 
+// The z faces of a pass as node boxes (k fastest) for k_pml_z: face q
+// covers i in [i0, i0 + ni), j in [j0, j0 + nj), k in [k0, k0 + nk), its
+// nodes numbered from start[q]; start[n] is the total.
+struct PmlZBox {
+  int n;
+  int i0[2], j0[2], k0[2];
+  int nj[2], nk[2];
+  int64_t start[3];
+};
+
 // Everything one 3-D Yee pass (H or E) of one grid needs: the kernels take
 // it as a kernel argument (one grid) or from an array in device memory
 // (a batch of grids, one launch).
@@ -405,6 +415,16 @@ struct NfItem {
   Scales<C> sc;
   Dims d;
   NfPass P;
+};
+
+template <typename S>
+struct PmlZItem {
+  using C = typename Store<S>::C;
+  Fields<S> f;
+  Scales<C> sc;
+  Dims d;
+  PmlPass<C> P;  // the z faces
+  PmlZBox B;
 };
 
 template <typename S>

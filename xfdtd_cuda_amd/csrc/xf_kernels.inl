@@ -95,6 +95,47 @@ __device__ __forceinline__ void pml_all(const PmlPass<C>& P, int i, int j, int k
   }
 }
 
+// One node of the z-face CPML corrections of a pass, run after the Yee
+// kernel when that carries only the x and y faces (XF_TILE_PML_SPLIT).  The
+// corrections of a node read only its own components, the dual field (not
+// written in this pass) and their own psi, so per node the adds come in the
+// order pml_all gives them: x and y faces in the Yee kernel, then z faces.
+template <typename S, bool ISE>
+__device__ __forceinline__ void pml_z_node(const Fields<S>& f, const PmlPass<typename Store<S>::C>& P,
+                                           const PmlZBox& B, const Dims& d,
+                                           const Scales<typename Store<S>::C>& sc, int64_t t) {
+  using C = typename Store<S>::C;
+  if (t >= B.start[B.n]) return;
+  const int q = (B.n > 1 && t >= B.start[1]) ? 1 : 0;
+  const uint32_t local = (uint32_t)(t - B.start[q]);  // < 2^31 (checked by the engine)
+  const uint32_t nk = (uint32_t)B.nk[q], nj = (uint32_t)B.nj[q];
+  const int k = B.k0[q] + (int)(local % nk);
+  const uint32_t r = local / nk;
+  const int j = B.j0[q] + (int)(r % nj);
+  const int i = B.i0[q] + (int)(r / nj);
+  const int64_t idx = (int64_t)i * d.sx + (int64_t)j * d.sy + k;
+  C v[3] = {0, 0, 0};
+  bool have[3] = {false, false, false};
+  bool dirty[3] = {false, false, false};
+  // static face indices keep a by-value pass out of scratch
+  if (q == 0)
+    pml_face<2, ISE, false>(P.f[0], i, j, k, idx, f, d, sc, v, have, dirty);
+  else
+    pml_face<2, ISE, false>(P.f[1], i, j, k, idx, f, d, sc, v, have, dirty);
+  S* const* self = ISE ? f.e : f.h;
+  const C s = ISE ? sc.s_e : sc.s_h;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    if (dirty[c]) Store<S>::st(self[c], idx, v[c], s);
+}
+
+template <typename S, bool ISE>
+__global__ void __launch_bounds__(256)
+    k_pml_z(Fields<S> f, PmlPass<typename Store<S>::C> P, PmlZBox B, Dims d,
+            Scales<typename Store<S>::C> sc) {
+  pml_z_node<S, ISE>(f, P, B, d, sc, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Loads the pass's coefficient table into LDS (or points at global memory).
 template <typename C, bool LDS>
 __device__ __forceinline__ const C* load_table(const C* __restrict__ g, int nval, C* smem) {
@@ -906,6 +947,18 @@ __global__ void __launch_bounds__(256)
   const int g = owner_of(start, n, tid);
   const NfItem<S>& I = items[g];
   nf2ff_cell<S>(I.f, I.P, t_dev ? *t_dev : t_arg, I.d, 0, I.sc, tid - start[g]);
+}
+
+// The z-face CPML corrections of every grid of a batch that splits them
+// off, in one launch per pass.
+template <typename S, bool ISE>
+__global__ void __launch_bounds__(256)
+    k_pml_z_batch(const PmlZItem<S>* __restrict__ items, const int64_t* __restrict__ start, int n) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= start[n]) return;
+  const int g = owner_of(start, n, tid);
+  const PmlZItem<S>& I = items[g];
+  pml_z_node<S, ISE>(I.f, I.P, I.B, I.d, I.sc, tid - start[g]);
 }
 
 // The point sources of every grid of a batch in one launch: thread g adds
