@@ -397,8 +397,8 @@ def bench_precision(args) -> int:
 def bench_learned(args) -> int:
     """The learned absorbing layer end to end on the GPU: training data from
     16-cell CPML point-source runs (seeds 0-5, 32^3 interior), training by
-    the HIP trainer, reflection error against the 16-cell CPML on an unseen
-    seed next to Mur and 2/4/8-cell CPMLs, and the step time of a 256^3
+    the HIP trainer, reflection error against the 16-cell CPML averaged over
+    four unseen seeds next to Mur and 2/4/8-cell CPMLs, and the step time of a 256^3
     interior bounded by the 2-cell layer against the same interior with an
     8-cell CPML (FP32 fields)."""
     import xfdtd_cuda_amd as xf
@@ -410,11 +410,16 @@ def bench_learned(args) -> int:
     t0 = time.perf_counter()
     w16, loss0, loss1 = learned.train_layer(smp, k)
     t_train = time.perf_counter() - t0
-    err = learned.reflection_errors(w16, n=32, seed=7, steps=300, every=15,
-                                    variants=[("learned", 2), ("mur", 2), ("cpml", 2), ("cpml", 4),
-                                              ("cpml", 8)])
+    # the judge: four unseen seeds (40^3 interior, 400 steps); one seed alone
+    # varies too much from scene to scene to rank boundaries
+    judge_seeds = (7, 8, 9, 10)
+    per_seed = {s: learned.reflection_errors(w16, seed=s, variants=[("learned", 2), ("mur", 2), ("cpml", 2),
+                                                                      ("cpml", 4), ("cpml", 8)])
+                for s in judge_seeds}
+    err = {k2: float(np.mean([per_seed[s][k2] for s in judge_seeds])) for k2 in per_seed[judge_seeds[0]]}
     print(f"# {smp.shape[0]} samples in {t_rec:.1f} s, trained in {t_train:.1f} s, "
-          f"loss {loss0:.3e} -> {loss1:.3e}, reflection {err}", file=sys.stderr, flush=True)
+          f"loss {loss0:.3e} -> {loss1:.3e}, mean reflection over seeds {judge_seeds}: "
+          f"{ {k2: round(v, 6) for k2, v in err.items()} }", file=sys.stderr, flush=True)
 
     K, W, n = args.steps, args.warmup, 256
     rates = {}
@@ -443,7 +448,9 @@ def bench_learned(args) -> int:
                    "cells_learned": cells_l, "cells_cpml8": cells_c,
                    "ms_per_step_cpml8": round(ms_c, 4),
                    "interior_speedup_vs_cpml8": round(ms_c / ms_l, 3),
-                   "reflection_vs_cpml16": {k2: float(v) for k2, v in err.items()},
+                   "reflection_vs_cpml16_mean": err,
+                   "reflection_vs_cpml16_per_seed": {str(sd): {k2: float(v) for k2, v in r.items()}
+                                                     for sd, r in per_seed.items()},
                    "training": {"samples": int(smp.shape[0]), "record_s": round(t_rec, 2),
                                 "train_s": round(t_train, 2), "loss_before": loss0,
                                 "loss_after": loss1}},

@@ -204,7 +204,7 @@ def test_learned_layer_judged_against_16_cell_cpml(gpu_available):
     """Train on GPU-recorded data of seeds 0-5, judge on seed 7 (not trained
     on): against the 16-cell CPML run, the learned 2-cell layer reflects
     clearly less than the same layer without its network (first-order Mur).
-    A 2-cell CPML is printed beside it for context (about equal)."""
+    A 2-cell CPML is printed beside it for context."""
     smp, k = learned.record_samples(n=32, seeds=(0, 1, 2, 3, 4, 5), steps=300, every=2)
     w16, loss0, loss1 = learned.train_layer(smp, k)
     err = learned.reflection_errors(w16, n=32, seed=7, steps=300, every=15,

@@ -212,6 +212,23 @@ def test_point_source_adds_its_waveform():
         assert s.array(c)[tuple(w[0])] == v[0]
 
 
+def test_training_augmentation_keeps_targets_and_scales_noise_per_row():
+    """learned.augment: the original samples, then a copy whose inputs carry
+    noise proportional to each row's largest input (rows of very different
+    scale get proportionate noise), targets untouched, reproducible."""
+    from xfdtd_cuda_amd import learned
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal((4000, 13)) * 10.0 ** rng.uniform(-6, 3, (4000, 1))).astype(np.float32)
+    a = learned.augment(x, 0.01, seed=5)
+    assert a.shape == (8000, 13) and a.dtype == np.float32
+    assert np.array_equal(a[:4000], x)
+    assert np.array_equal(a[4000:, 12], x[:, 12])
+    rel = (a[4000:, :12] - x[:, :12]) / np.max(np.abs(x[:, :12]), axis=1, keepdims=True)
+    assert 0.009 < rel.std() < 0.011
+    assert abs(rel.mean()) < 1e-3
+    assert np.array_equal(learned.augment(x, 0.01, seed=5), a)
+
+
 def test_learned_layer_validation_errors():
     s = xf.Simulation(5e-3, 5e-3, 5e-3, 0.9)
     s.add_cube("domain", (0, 0, 0), (0.05,) * 3, xf.Material.air())

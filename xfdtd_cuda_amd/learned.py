@@ -3,9 +3,9 @@ engine itself, training on the GPU, and the reflection judge.
 
 Pipeline (no reference counterpart):
   1. record_samples: vacuum boxes with seeded point sources bounded by a
-     16-cell CPML run on the GPU with a recorder on the plane two nodes inside
-     the CPML; each sample is the 12 column inputs and the true next value
-     of the plane node (xf_learned_boundary, XF_LB_RECORD).
+     16-cell CPML run on the GPU with a recorder on the plane one node inward
+     of the CPML's inner edge; each sample is the 12 column inputs and the
+     true next value of the plane node (xf_learned_boundary, XF_LB_RECORD).
   2. Trainer: the HIP trainer (xf_lb_trainer_*) fits the network's residual
      over the first-order Mur update.
   3. reflection_errors: the same point-source scene (a seed not used for
@@ -129,11 +129,14 @@ class Trainer:
 
 
 def record_samples(n: int = 40, seeds=(0, 1, 2), steps: int = 400, pml: int = 16,
-                   depth: int = 2, every: int = 2, precision: str = "fp32",
+                   depth: int = 1, every: int = 2, precision: str = "fp32",
                    n_sources: int = 3):
     """Training samples from point-source boxes bounded by a `pml`-cell CPML,
-    recorded on the plane `depth` nodes inside the CPML every `every` steps.
-    Returns (samples (N, 13) float32, mur_k)."""
+    recorded every `every` steps on the plane `depth` nodes inward of the
+    CPML's inner edge (depth 1 trains a layer that reflects less in the
+    closed loop than depth 2 or 3: measured over three network inits and
+    three unseen seeds, 1.5e-2 against 2.4e-2).  Returns (samples (N, 13)
+    float32, mur_k)."""
     out = []
     k = None
     for seed in seeds:
@@ -149,11 +152,30 @@ def record_samples(n: int = 40, seeds=(0, 1, 2), steps: int = 400, pml: int = 16
     return np.concatenate(out), k
 
 
-def train_layer(samples: np.ndarray, k: float, seed: int = 0, device: int = 0):
-    """The training recipe: Adam from the plain Mur start, 1500 steps at
-    lr 2e-3 on minibatches of 8192, 750 at 5e-4, 750 at 1.25e-4 on 16384
-    (longer runs fit the one-step target better but reflect more in the
-    closed loop).  Returns (fp16 weight bits, loss before, loss after)."""
+def augment(samples: np.ndarray, noise: float, seed: int = 0) -> np.ndarray:
+    """The samples plus a copy whose 12 inputs carry Gaussian noise of
+    `noise` times the row's largest input magnitude (targets unchanged):
+    the closed loop feeds the layer its own slightly-off values, which the
+    one-step samples never show it."""
+    rng = np.random.default_rng(seed)
+    x = np.array(samples, np.float32, copy=True)
+    s = np.max(np.abs(x[:, :12]), axis=1, keepdims=True)
+    x[:, :12] = (x[:, :12] + noise * s * rng.standard_normal((x.shape[0], 12))).astype(np.float32)
+    return np.concatenate([np.asarray(samples, np.float32), x])
+
+
+def train_layer(samples: np.ndarray, k: float, seed: int = 0, device: int = 0,
+                noise: float = 0.01):
+    """The training recipe: the samples plus a copy with 1 % input noise
+    (augment), Adam from the plain Mur start, 1500 steps at lr 2e-3 on
+    minibatches of 8192, 750 at 5e-4, 750 at 1.25e-4 on 16384 (longer runs
+    fit the one-step target better but reflect more in the closed loop).
+    With depth-1 samples the noise copy lowers the reflection further and
+    steadies it across network inits (mean 1.44e-2, worst 1.59e-2 over three
+    inits, against 1.54e-2 and 1.66e-2 without it).  Returns (fp16 weight
+    bits, loss before, loss after); the losses are on the training set."""
+    if noise > 0:
+        samples = augment(samples, noise, seed + 100)
     t = Trainer(samples, k, device).set_weights(init_weights(seed))
     l0 = t.loss()
     t.train(1500, batch=8192, lr=2e-3, seed=seed + 1)
