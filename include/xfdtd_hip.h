@@ -309,6 +309,9 @@ enum xf_tile_flags {
                             results are unchanged, the z faces being the
                             last corrections of a node either way; on by
                             default when tiling_flags is 0) */
+  XF_TILE_K32 = 16,      /* 32-lane k-tiles with two j-rows per wave (fills
+                            more lanes when nz + 1 is just above a multiple
+                            of 64); the default is 64-lane k-tiles */
   XF_TILE_SET = 256      /* the other bits are explicit, not auto        */
 };
 
@@ -386,12 +389,13 @@ int xf_stream_bench(int64_t bytes, int32_t reps, int32_t device, int32_t variant
  * grid/block dims, SimulationHD::setGridDim/setBlockDim,
  * simulation_hd.cuh:51-55, and decomposes flat index ranges). */
 typedef struct xf_tuning {
-  int32_t tile_y;      /* rows of a tile, 1..4 (block = 64 x tile_y)        */
+  int32_t tile_y;      /* 1..4: a block is 64 x tile_y threads (k32: 32 k-lanes x 2 tile_y rows) */
   int32_t planes;      /* x planes one block marches through (>= 1)        */
   int32_t blocks;      /* batch only: persistent grid size, 0 = resident   */
   int32_t nontemporal; /* stream the updated component non-temporally      */
   int32_t xcd_remap;   /* single grid only: XCD-aware block order          */
   int32_t pml_split;   /* XF_TILE_PML_SPLIT                                */
+  int32_t k32;         /* XF_TILE_K32                                      */
   double step_ms;      /* autotune: measured time of one step              */
 } xf_tuning;
 

@@ -88,17 +88,19 @@ def test_batch_matches_oracle_grid_by_grid(gpu_available):
     assert np.abs(ref[0].array("ex")).max() > 1e-3
 
 
-@pytest.mark.parametrize("tile_y,planes,blocks,nt,split", [
-    (1, 1, 0, True, False), (2, 3, 7, False, True), (4, 100, 1, True, False), (3, 2, 0, False, True),
-    (1, 4, 13, True, True)])
-def test_batch_tunings_bit_identical(gpu_available, tile_y, planes, blocks, nt, split):
+@pytest.mark.parametrize("tile_y,planes,blocks,nt,split,k32", [
+    (1, 1, 0, True, False, False), (2, 3, 7, False, True, False), (4, 100, 1, True, False, False),
+    (3, 2, 0, False, True, False), (1, 4, 13, True, True, False), (1, 4, 0, False, True, True),
+    (4, 2, 5, True, False, True)])
+def test_batch_tunings_bit_identical(gpu_available, tile_y, planes, blocks, nt, split, k32):
     steps = 30
     build = lambda: _uneven(steps, count=4, seed=11)
     ref = _oracle_copies(build, steps)
     sims = build()
     b = xf.Batch(sims).upload()
-    b.set_tuning(tile_y=tile_y, planes=planes, blocks=blocks, nontemporal=nt, pml_split=split)
+    b.set_tuning(tile_y=tile_y, planes=planes, blocks=blocks, nontemporal=nt, pml_split=split, k32=k32)
     assert ("pml_split=0" not in b.describe()) == split
+    assert ("k32=1" in b.describe()) == k32
     b.run(steps).download()
     b.close()
     _assert_identical(sims, ref, f"tuning ty={tile_y} planes={planes} blocks={blocks} split={split}")

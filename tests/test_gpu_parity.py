@@ -107,20 +107,25 @@ def test_sphere_uneven_sizes_and_steps_in_chunks(gpu_available):
     _compare(g, c, label="uneven")
 
 
-@pytest.mark.parametrize("tile_y,chunks,xcd,nt,split", [
-    (1, 0, False, True, False), (2, 0, False, False, False), (3, 7, True, True, False),
-    (4, 1, False, True, False), (1, 1000, True, False, False), (2, 3, False, True, False),
-    (1, 0, False, True, True), (4, 1, False, False, True), (2, 3, True, True, True)])
-def test_tiling_variants_bit_identical(gpu_available, tile_y, chunks, xcd, nt, split):
-    """Every tile shape / march length / block order / streaming flag, and
-    the z-face CPML corrections fused or in a launch of their own, gives the
-    same bits: the tiling only reorders independent cell updates, and a
-    node's z-face corrections come last either way."""
+@pytest.mark.parametrize("tile_y,chunks,xcd,nt,split,k32", [
+    (1, 0, False, True, False, False), (2, 0, False, False, False, False),
+    (3, 7, True, True, False, False), (4, 1, False, True, False, False),
+    (1, 1000, True, False, False, False), (2, 3, False, True, False, False),
+    (1, 0, False, True, True, False), (4, 1, False, False, True, False),
+    (2, 3, True, True, True, False), (1, 0, False, True, True, True),
+    (4, 5, False, False, True, True), (3, 7, True, True, False, True)])
+def test_tiling_variants_bit_identical(gpu_available, tile_y, chunks, xcd, nt, split, k32):
+    """Every tile shape (64 or 32 k-lanes) / march length / block order /
+    streaming flag, and the z-face CPML corrections fused or in a launch of
+    their own, gives the same bits: the tiling only reorders independent
+    cell updates, and a node's z-face corrections come last either way."""
     build = lambda: scenes.dielectric_sphere(n=21, pml=5, steps=60, tfsf_inset=8, nf2ff_inset=6)
     g, c = build(), build()
     eng = xf.Engine(g).upload()
-    eng.set_tiling(tile_y=tile_y, chunks_x=chunks, xcd_remap=xcd, nontemporal=nt, pml_split=split)
+    eng.set_tiling(tile_y=tile_y, chunks_x=chunks, xcd_remap=xcd, nontemporal=nt, pml_split=split,
+                   k32=k32)
     assert ("pml_split=1" in eng.describe()) == split
+    assert ("klanes=32" in eng.describe()) == k32
     eng.run(60).download()
     eng.close()
     oracle.run(c.scene_ptr(), 60)
@@ -284,6 +289,30 @@ def test_pml_split_uneven_faces_fp32_and_bf16(gpu_available):
     for n in FIELDS:
         assert np.array_equal(a.array(n), b.array(n)), n
     assert "pml_split=0" in run(build(), "bf16", True)
+
+
+def test_pml_split_single_z_face(gpu_available):
+    """The z-face split with CPML on XN and ZP only (one z face in the
+    launch; PEC elsewhere): bit-identical to the oracle, fields and psi."""
+    def build():
+        s = xf.Simulation(3e-3, 3e-3, 3e-3, 0.95)
+        s.add_cube("domain", (-0.04, -0.04, -0.05), (0.08, 0.08, 0.1), xf.Material.air())
+        s.add_sphere("s", (0.002, 0.0, 0.004), 0.015, xf.Material.dielectric(3.0, 1.0, 0.02))
+        s.add_pml(5, "XN").add_pml(6, "ZP")
+        tau = 20 * 3e-3 / 6e8
+        s.add_tfsf3d((8, 8, 9), 0.5, 0.9, 0.2, xf.Waveform.gaussian(tau, 4.5 * tau))
+        s.init(100)
+        return s
+
+    g, c = build(), build()
+    e = xf.Engine(g).upload()
+    e.set_tiling(tile_y=1, pml_split=True)
+    assert "pml_split=1" in e.describe()
+    e.run(100).download()
+    e.close()
+    oracle.run(c.scene_ptr(), 100)
+    _compare(g, c, label="split one z face")
+    assert np.abs(c.array("pml.1.ea_psi_hb")).max() > 0
 
 
 def test_fp32_close_to_fp64(gpu_available):

@@ -49,6 +49,7 @@ class XfPlan(ctypes.Structure):
 TILE_XCD_REMAP = 1
 TILE_NT = 4
 TILE_PML_SPLIT = 8
+TILE_K32 = 16
 TILE_SET = 256
 
 
@@ -80,6 +81,7 @@ class XfTuning(ctypes.Structure):
         ("nontemporal", c_i32),
         ("xcd_remap", c_i32),
         ("pml_split", c_i32),
+        ("k32", c_i32),
         ("step_ms", c_f64),
     ]
 

@@ -27,7 +27,7 @@ struct Batch {
   int id_bytes{1};
   bool any_pml{false};
   int64_t cells{0};
-  xf_tuning tune{1, 4, 0, 1, 0, 1, 0.0};
+  xf_tuning tune{1, 4, 0, 1, 0, 1, 0, 0.0};
   // device descriptors (sizes fixed by the scenes, rewritten by configure)
   std::vector<void*> allocs;
   void* pass_h{};
@@ -213,7 +213,8 @@ int Batch::configure_t() {
   std::vector<int> ts(n + 1, 0);
   int64_t tiles = 0;
   any_pml = false;
-  const int flags = XF_TILE_SET | (tune.nontemporal ? XF_TILE_NT : 0) | (tune.pml_split ? XF_TILE_PML_SPLIT : 0);
+  const int flags = XF_TILE_SET | (tune.nontemporal ? XF_TILE_NT : 0) | (tune.pml_split ? XF_TILE_PML_SPLIT : 0) |
+                    (tune.k32 ? XF_TILE_K32 : 0);
   for (int g = 0; g < n; ++g) {
     Engine& E = eng(g);
     const int64_t chunks = (E.nx + 1 + tune.planes - 1) / tune.planes;
@@ -381,6 +382,7 @@ int Batch::configure() {
          " resident=" + std::to_string(resident) + " tile_y=" + std::to_string(tune.tile_y) +
          " planes=" + std::to_string(tune.planes) + " nt=" + std::to_string(tune.nontemporal) +
          " id_bytes=" + std::to_string(id_bytes) + " pml_split=" + std::to_string(n_pz[1]) +
+         " k32=" + std::to_string(tune.k32) +
          " tfsf=" + std::to_string(n_tf[0]) +
          " nf2ff=" + std::to_string(n_nf);
   return XF_OK;
@@ -390,7 +392,7 @@ template <typename S>
 int Batch::launch_step(int t, bool use_tdev, bool ev_on, hipEvent_t* ev) {
   const int* tp = use_tdev ? t_dev : nullptr;
   const int n = static_cast<int>(inst.size());
-  const dim3 block(64, tune.tile_y), grid(grid_blocks);
+  const dim3 block(tune.k32 ? 32 : 64, tune.k32 ? 2 * tune.tile_y : tune.tile_y), grid(grid_blocks);
   const bool nt_on = tune.nontemporal != 0;
 #define XF_BY(KER, I, PM, NT, PASS) \
   hipLaunchKernelGGL((KER<S, I, PM, NT>), grid, block, 0, stream, static_cast<const YeePass<S>*>(PASS), tile_start, n)
@@ -583,10 +585,10 @@ int Batch::autotune(int trial_steps, xf_tuning* best) {
   for (int ty : {1, 2, 4}) {
     for (int planes : {2, 4, 8}) {
       for (int nt_on : {1, 0}) {
-        for (int cand = 0; cand < 4; ++cand) {
-          const int half = cand & 1, split = cand >> 1;
+        for (int cand = 0; cand < 8; ++cand) {
+          const int half = cand & 1, split = (cand >> 1) & 1, k32 = cand >> 2;
           if (split && !can_split) continue;
-          tune = xf_tuning{ty, planes, 0, nt_on, 0, split, 0.0};
+          tune = xf_tuning{ty, planes, 0, nt_on, 0, split, k32, 0.0};
           status = configure();
           if (status == XF_OK && half) {
             tune.blocks = std::max(1, resident / 2);
@@ -649,15 +651,17 @@ int Engine::autotune(int trial_steps, xf_tuning* best) {
   for (int ty : tys) {
     for (int planes : pls) {
       for (int nt_on : {1, 0}) {
-        for (int split : {0, 1}) {
-          if (split && !can_split) continue;
+        for (int cand = 0; cand < 4; ++cand) {
+          const int split = cand & 1, k32 = dim == 3 ? cand >> 1 : 0;
+          if ((split && !can_split) || (dim != 3 && cand >> 1)) continue;
           const int chunks = static_cast<int>((nx + 1 + planes - 1) / planes);
-          set_tiling(ty, chunks, XF_TILE_SET | (nt_on ? XF_TILE_NT : 0) | (split ? XF_TILE_PML_SPLIT : 0));
+          set_tiling(ty, chunks, XF_TILE_SET | (nt_on ? XF_TILE_NT : 0) | (split ? XF_TILE_PML_SPLIT : 0) |
+                                     (k32 ? XF_TILE_K32 : 0));
           status = run(1);
           if (status == XF_OK) status = run(trial_steps);
           if (status != XF_OK) break;
           const double ms = timing.run_ms / trial_steps;
-          if (ms < win.step_ms) win = xf_tuning{ty, planes, 0, nt_on, 0, split, ms};
+          if (ms < win.step_ms) win = xf_tuning{ty, planes, 0, nt_on, 0, split, k32, ms};
           cur_step = step0;
           status = restore();
           if (status != XF_OK) break;
@@ -676,7 +680,8 @@ int Engine::autotune(int trial_steps, xf_tuning* best) {
     return status;
   }
   set_tiling(win.tile_y, static_cast<int>((nx + 1 + win.planes - 1) / win.planes),
-             XF_TILE_SET | (win.nontemporal ? XF_TILE_NT : 0) | (win.pml_split ? XF_TILE_PML_SPLIT : 0));
+             XF_TILE_SET | (win.nontemporal ? XF_TILE_NT : 0) | (win.pml_split ? XF_TILE_PML_SPLIT : 0) |
+                 (win.k32 ? XF_TILE_K32 : 0));
   if (best) *best = win;
   return XF_OK;
 }

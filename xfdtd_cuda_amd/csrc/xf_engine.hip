@@ -158,6 +158,8 @@ struct Engine {
   int xcd{0};
   int nt_streams{1};
   int pml_split{0};  // XF_TILE_PML_SPLIT requested (see split_z())
+  int klanes{64};    // k-lanes of a tile row: 64, or 32 (XF_TILE_K32)
+  int rows() const { return tile_y * (64 / klanes); }  // j-rows of a block
   void set_tiling(int ty, int chunks_x, int flags);
   Tiling tiling() const;
   int snapshot();  // copy every state buffer aside (device to device)
@@ -1385,16 +1387,18 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
     xcd = (flags & XF_TILE_XCD_REMAP) ? 1 : 0;
     nt_streams = (flags & XF_TILE_NT) ? 1 : 0;
     pml_split = (flags & XF_TILE_PML_SPLIT) ? 1 : 0;
+    klanes = (flags & XF_TILE_K32) ? 32 : 64;
   } else {
     // defaults: non-temporal streams; z-face CPML corrections split off
     // (measured at 536^3: FP32 +3 %, FP64 even; 64 small grids: +13 %)
     xcd = 0;
     nt_streams = 1;
     pml_split = 1;
+    klanes = 64;
   }
   if (dim == 3) {
-    nkb = static_cast<int>((nz + 1 + 63) / 64);
-    njb = static_cast<int>((ny + 1 + tile_y - 1) / tile_y);
+    nkb = static_cast<int>((nz + 1 + klanes - 1) / klanes);
+    njb = static_cast<int>((ny + 1 + rows() - 1) / rows());
     // default march: 4 planes for 8-byte fields, 8 for narrower ones (the
     // measured optimum on MI355X at 536^3)
     const int64_t planes = esize == 8 ? 4 : 8;
@@ -1414,6 +1418,7 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
                 " htab=" + std::to_string(ntab_h) + " chunk=" + std::to_string(chunk) +
                 " nchunk=" + std::to_string(nch_eff) + " tile_y=" + std::to_string(tile_y) +
                 " xcd=" + std::to_string(xcd) + " nt=" + std::to_string(nt_streams) + " pml_split=" + std::to_string(split_z() ? 1 : 0) +
+                " klanes=" + std::to_string(klanes) +
                 " pml_faces=" + std::to_string(pml.size()) +
                 " ade_cells=" + std::to_string(ade_cells.size());
 }
@@ -1704,7 +1709,7 @@ int Engine::launch_h(int t) {
   if (dim == 3) {
     const YeePass<S> D = yee_pass<S>(false);
     const bool has_pml = D.pml.n > 0;
-    dim3 block(64, tile_y);
+    dim3 block(klanes, rows());
     dim3 grid((unsigned)(D.nkb * D.njb * D.nch));
 #define XF_LH3(I, L, PM)                                                                     \
   do {                                                                                       \
@@ -1758,7 +1763,7 @@ int Engine::launch_e(int t) {
   if (dim == 3) {
     const YeePass<S> D = yee_pass<S>(true);
     const bool has_pml = D.pml.n > 0;
-    dim3 block(64, tile_y);
+    dim3 block(klanes, rows());
     dim3 grid((unsigned)(D.nkb * D.njb * D.nch));
 #define XF_LE3(I, L, PM, AD)                                                                     \
   do {                                                                                           \

@@ -7,7 +7,8 @@
 // (update_scheme.cuh:27-32, skipping a == 0 || b == 0) for Ex, Ey, Ez
 // fused, plus the E corrections of every PML face (pml_corrector.cu:50-178).
 //
-// Work decomposition: a tile is 64 lanes along k x blockDim.y rows along j;
+// Work decomposition: a tile is blockDim.x (64, or 32) lanes along k x
+// blockDim.y rows along j;
 // each thread marches i through the tile's x-chunk, carrying the plane it
 // has loaded as the next cell's i-1 (E pass: Hy, Hz) or i (H pass: Ey, Ez)
 // neighbour, and issuing the loads of plane i+1 before computing plane i
@@ -103,7 +104,7 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
                                             int kb, int jb, int ch) {
   using C = typename Store<S>::C;
   const Dims& d = D.d;
-  const int k = kb * 64 + threadIdx.x;
+  const int k = kb * blockDim.x + threadIdx.x;  // 64 (or 32) k-lanes per row
   const int j = jb * blockDim.y + threadIdx.y;
   if (k > d.nz || j > d.ny) return;
   const I* __restrict__ hid = static_cast<const I*>(D.ids);
@@ -151,7 +152,7 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
                                             int kb, int jb, int ch) {
   using C = typename Store<S>::C;
   const Dims& d = D.d;
-  const int k = kb * 64 + threadIdx.x;
+  const int k = kb * blockDim.x + threadIdx.x;  // 64 (or 32) k-lanes per row
   const int j = jb * blockDim.y + threadIdx.y;
   if (k > d.nz || j > d.ny) return;
   const I* __restrict__ eid = static_cast<const I*>(D.ids);
