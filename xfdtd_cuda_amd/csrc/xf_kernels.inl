@@ -246,6 +246,24 @@ __device__ __forceinline__ C tfsf_inc(const TfsfPass<C>& P, bool inc_is_e, int c
 // written cell of one job; cells shared by several faces of the same
 // component are applied by the first of them, in the reference's launch
 // order (TFSFCorrector3DAgency/2DAgency, tfsf_corrector.cu:155-271).
+// (i, j, k) offsets of flat index r (k fastest) in a box of n1 x n2 rows;
+// 32-bit division when r fits (every box here but the very largest), which
+// is several times cheaper than the 64-bit routine.
+__device__ __forceinline__ void box_coords(int64_t r, int n1, int n2, int& li, int& lj, int& lk) {
+  if (r < (int64_t(1) << 32)) {
+    uint32_t u = (uint32_t)r;
+    lk = (int)(u % (uint32_t)n2);
+    u /= (uint32_t)n2;
+    lj = (int)(u % (uint32_t)n1);
+    li = (int)(u / (uint32_t)n1);
+  } else {
+    lk = (int)(r % n2);
+    r /= n2;
+    lj = (int)(r % n1);
+    li = (int)(r / n1);
+  }
+}
+
 template <typename S>
 __device__ __forceinline__ void tfsf_cell(const Fields<S>& f, const TfsfPass<typename Store<S>::C>& P,
                                           int t, const Scales<typename Store<S>::C>& sc, const Dims& d,
@@ -254,11 +272,8 @@ __device__ __forceinline__ void tfsf_cell(const Fields<S>& f, const TfsfPass<typ
   int q = 0;
   while (q + 1 < P.njob && tid >= P.job[q + 1].start) ++q;
   const TfsfJob<C>& J = P.job[q];
-  int64_t r = tid - J.start;
-  const int lk = (int)(r % J.n[2]);
-  r /= J.n[2];
-  const int lj = (int)(r % J.n[1]);
-  const int li = (int)(r / J.n[1]);
+  int li, lj, lk;
+  box_coords(tid - J.start, J.n[1], J.n[2], li, lj, lk);
   const int cell[3] = {J.o[0] + li, J.o[1] + lj, J.o[2] + lk};
   auto inside = [&](const TfsfJob<C>& X) {
     return cell[0] >= X.o[0] && cell[0] < X.o[0] + X.n[0] && cell[1] >= X.o[1] &&
@@ -346,12 +361,9 @@ __device__ __forceinline__ void nf2ff_cell(const Fields<S>& f, const NfPass& P, 
   int face = 0;
   while (face < 5 && tid >= P.face[face + 1].start) ++face;
   const NfFace& F = P.face[face];
-  int64_t r = tid - F.start;
-  const int64_t cell = r;
-  const int lk = (int)(r % F.n[2]);
-  r /= F.n[2];
-  const int lj = (int)(r % F.n[1]);
-  const int li = (int)(r / F.n[1]);
+  const int64_t cell = tid - F.start;
+  int li, lj, lk;
+  box_coords(cell, F.n[1], F.n[2], li, lj, lk);
   const int i = F.o[0] + li, j = F.o[1] + lj, k = F.o[2] + lk;
   const int axis = face / 2;
   const bool positive = face % 2;
@@ -646,9 +658,9 @@ __global__ void __launch_bounds__(256)
   if (frame >= M.frames) return;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= M.box) return;
-  const int k = M.o[2] + (int)(c % M.n[2]);
-  const int j = M.o[1] + (int)((c / M.n[2]) % M.n[1]);
-  const int i = M.o[0] + (int)(c / ((int64_t)M.n[2] * M.n[1]));
+  int li, lj, lk;
+  box_coords(c, M.n[1], M.n[2], li, lj, lk);
+  const int i = M.o[0] + li, j = M.o[1] + lj, k = M.o[2] + lk;
   const S* p = M.field < 3 ? f.e[M.field] : f.h[M.field - 3];
   const int64_t idx = two_d ? (int64_t)i * d.sx + j : (int64_t)i * d.sx + (int64_t)j * d.sy + k;
   const double v = p ? (double)Store<S>::ld(p, idx, M.field < 3 ? sc.inv_e : sc.inv_h) : 0.0;
