@@ -62,13 +62,13 @@ def test_learned_layer_apply_matches_oracle(gpu_available):
 
 
 def test_learned_layer_one_step_from_random_state(gpu_available):
-    """One step from random fields and history (every column of every face
+    """One step from random E fields and history (every column of every face
     live, with non-zero inputs of all signs): nodes off the layer's outer
     planes are bit-identical to the oracle and most predicted outer-plane
     nodes too (measured 85 %).  The others differ where a layer-1 activation
-    fell on the other side of an fp16 rounding boundary: within 2^-8 of the
+    fell on the other side of an fp16 rounding boundary: within 2^-14 of the
     column scale s, bounded here by the power of two above 8 x max|history|
-    (the Laplacian inputs) and max|field| (measured worst 2^-10 s)."""
+    (the Laplacian inputs) and max|field| (measured worst about 2^-20 s)."""
     w = lb_mlp.init_weights(4, out_scale=1.0).astype(np.float16)
     build = lambda: scenes.point_source_box(n=20, boundary="learned", size=2, steps=3, seed=1,
                                             weights=w)
@@ -76,7 +76,9 @@ def test_learned_layer_one_step_from_random_state(gpu_available):
     (n, _, _), _, _ = g.info()  # cubic grid, layer included
     rng = np.random.default_rng(9)
     for name in FIELDS:
-        v = rng.standard_normal(g.array(name).shape) * 3.0
+        # random E and zero H: the updated E stays of the order of the
+        # random one, which keeps the column scales (and the bound) tight
+        v = rng.standard_normal(g.array(name).shape) * 3.0 if name[0] == "e" else 0.0
         g.array(name)[...] = v
         c.array(name)[...] = v
     hv = (rng.standard_normal(g.array("lb.history").shape) * 2.0).astype(np.float32)
@@ -89,7 +91,7 @@ def test_learned_layer_one_step_from_random_state(gpu_available):
     # inputs: updated fields (x0..x2) and the history before the step (x3..x11)
     s_up = 2.0 ** np.ceil(np.log2(max(max(float(np.abs(c.array(f)).max()) for f in ("ex", "ey", "ez")),
                                       8.0 * float(np.abs(hv).max()))))
-    bound = 2.0 ** -8 * s_up
+    bound = 2.0 ** -14 * s_up
     worst, same, cnt = 0.0, 0, 0
     for name in ("ex", "ey", "ez"):
         a, b = g.array(name), c.array(name)

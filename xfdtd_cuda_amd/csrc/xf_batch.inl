@@ -53,6 +53,8 @@ struct Batch {
   void* lb_items{};
   int64_t* lb_bstart{};  // first 256-column block of each layer (VALU phases)
   int64_t* lb_mstart{};  // first kLbMfmaCols-column block of each layer (k_lb_mfma_batch)
+  int64_t* lb_estart{};  // first edge-list entry of each layer (k_lb_edges_batch)
+  int64_t lb_edges{0};
   int n_lb{0};
   int64_t lb_blocks{0}, lb_mblocks{0};
   bool lb_any_record{false};
@@ -80,7 +82,7 @@ struct Batch {
     pass_h = pass_e = nullptr;
     tile_start = nullptr;
     tf_items[0] = tf_items[1] = nf_items = pt_items = lb_items = nullptr;
-    tf_start[0] = tf_start[1] = nf_start = lb_bstart = lb_mstart = nullptr;
+    tf_start[0] = tf_start[1] = nf_start = lb_bstart = lb_mstart = lb_estart = nullptr;
     pz_items[0] = pz_items[1] = nullptr;
     pz_start[0] = pz_start[1] = nullptr;
     t_dev = nullptr;
@@ -325,7 +327,7 @@ int Batch::configure_t() {
   }
   {
     std::vector<LbItem<S>> items;
-    std::vector<int64_t> st(1, 0), sm(1, 0);
+    std::vector<int64_t> st(1, 0), sm(1, 0), se(1, 0);
     lb_any_record = false;
     for (int g = 0; g < n; ++g) {
       Engine& E = eng(g);
@@ -339,8 +341,10 @@ int Batch::configure_t() {
       st.push_back(st.back() + (E.lbd.total + 255) / 256);
       const bool apply = E.lbd.mode == XF_LB_APPLY;
       sm.push_back(sm.back() + (apply ? (E.lbd.total + kLbMfmaCols - 1) / kLbMfmaCols : 0));
+      se.push_back(se.back() + (apply ? E.lbd.n_edge : 0));
       lb_any_record = lb_any_record || !apply;
     }
+    lb_edges = se.back();
     n_lb = static_cast<int>(items.size());
     lb_blocks = st.back();
     lb_mblocks = sm.back();
@@ -352,6 +356,9 @@ int Batch::configure_t() {
     q = lb_mstart;
     XF_TRY(put(&q, sm.data(), sm.size() * sizeof(int64_t)));
     lb_mstart = static_cast<int64_t*>(q);
+    q = lb_estart;
+    XF_TRY(put(&q, se.data(), se.size() * sizeof(int64_t)));
+    lb_estart = static_cast<int64_t*>(q);
   }
   // persistent grid: what the device holds resident, unless set
   hipDeviceProp_t prop;
@@ -433,14 +440,21 @@ int Batch::launch_step(int t, bool use_tdev, bool ev_on, hipEvent_t* ev) {
     // phase 0 (RECORD layers on the VALU, APPLY layers on the matrix
     // cores), the edge nodes of APPLY layers, the history roll
     const LbItem<S>* li = static_cast<const LbItem<S>*>(lb_items);
+    // recording layers: phase 0 (record) and 2 (roll) over all columns;
+    // apply layers: the network with the roll of its columns, then the edge
+    // columns' Mur update and roll
     if (lb_any_record)
       hipLaunchKernelGGL((k_lb_batch<S>), dim3((unsigned)lb_blocks), dim3(256), 0, stream, li, lb_bstart, n_lb, 0,
                          t, tp);
     if (lb_mblocks > 0)
       hipLaunchKernelGGL((k_lb_mfma_batch<S>), dim3((unsigned)lb_mblocks), dim3(256), 0, stream, li, lb_mstart,
                          n_lb, t, tp);
-    for (int ph = 1; ph < 3; ++ph)
-      hipLaunchKernelGGL((k_lb_batch<S>), dim3((unsigned)lb_blocks), dim3(256), 0, stream, li, lb_bstart, n_lb, ph,
+    if (lb_edges > 0)
+      for (int ph = 1; ph < 3; ++ph)
+        hipLaunchKernelGGL((k_lb_edges_batch<S>), dim3((unsigned)((lb_edges + 255) / 256)), dim3(256), 0, stream,
+                           li, lb_estart, n_lb, ph, t, tp);
+    if (lb_any_record)
+      hipLaunchKernelGGL((k_lb_batch<S>), dim3((unsigned)lb_blocks), dim3(256), 0, stream, li, lb_bstart, n_lb, 2,
                          t, tp);
   }
   for (auto& h : inst) XF_TRY(h->eng.launch_movies<S>(t, tp, stream));

@@ -1250,6 +1250,36 @@ int Engine::upload_additions() {
     }
   }
   lbd.total = col;
+  if (L.mode == XF_LB_APPLY && col >= (int64_t(1) << 32))
+    return fail(XF_ERR_UNSUPPORTED, "learned layer: 2^32 columns or more");
+  if (L.mode == XF_LB_APPLY) {
+    // edge columns (their node also lies on another face of the layer),
+    // found as lb_column / lb_other find them on the device
+    std::vector<uint32_t> edges;
+    for (int fi = 0; fi < lbd.nface; ++fi) {
+      const LbFace& F = lbd.face[fi];
+      for (int cc = 0; cc < 2; ++cc) {
+        const int other_axis = cc == 0 ? F.b : F.a;
+        const int lim = static_cast<int>(n3[other_axis]);
+        for (int p = lbd.inset; p <= F.p1[cc]; ++p)
+          for (int q = lbd.inset; q <= F.q1[cc]; ++q) {
+            const int x = cc == 0 ? q : p;  // the in-plane index along other_axis
+            const bool edge = (x == 0 && ((L.faces >> (2 * other_axis)) & 1)) ||
+                              (x == lim && ((L.faces >> (2 * other_axis + 1)) & 1));
+            if (!edge) continue;
+            const int64_t c = F.start[cc] + static_cast<int64_t>(p - lbd.inset) * (F.q1[cc] - lbd.inset + 1) +
+                              (q - lbd.inset);
+            edges.push_back(static_cast<uint32_t>(c));
+          }
+      }
+    }
+    lbd.n_edge = static_cast<int64_t>(edges.size());
+    if (!edges.empty()) {
+      void* pe;
+      XF_TRY(upload_vec(&pe, edges));
+      lbd.edge = static_cast<const uint32_t*>(pe);
+    }
+  }
   std::vector<float> hist(off);
   for (int64_t i = 0; i < off; ++i) hist[i] = static_cast<float>(L.history.data[i]);
   void* ph;
@@ -1889,16 +1919,23 @@ int Engine::launch_additions(int t, const int* tp, hipStream_t st) {
     hipLaunchKernelGGL((k_point_sources<S>), dim3(1), dim3(64), 0, st, make_fields<S>(*this), pts,
                        make_scales<C>(*this), t, tp);
   if (has_lb && lbd.total > 0) {
-    // APPLY: phase 0 on the matrix cores, then the edge nodes (phase 1);
-    // RECORD: phase 0 records; both end with the history roll (phase 2)
-    const dim3 g((unsigned)((lbd.total + 255) / 256)), b(256);
-    if (lbd.mode == XF_LB_APPLY)
+    const dim3 b(256);
+    if (lbd.mode == XF_LB_APPLY) {
+      // the network on the matrix cores (with the history roll of its
+      // columns), then the edge columns: Mur update, then their roll
       hipLaunchKernelGGL((k_lb_mfma<S>), dim3((unsigned)((lbd.total + kLbMfmaCols - 1) / kLbMfmaCols)), b, 0,
                          st, make_fields<S>(*this), lbd, make_dims(*this), make_scales<C>(*this), t, tp);
-    const int phases[2] = {lbd.mode == XF_LB_APPLY ? 1 : 0, 2};
-    for (int ph : phases)
-      hipLaunchKernelGGL((k_lb<S>), g, b, 0, st, make_fields<S>(*this), lbd, make_dims(*this),
-                         make_scales<C>(*this), ph, t, tp);
+      if (lbd.n_edge > 0)
+        for (int ph = 1; ph < 3; ++ph)
+          hipLaunchKernelGGL((k_lb_edges<S>), dim3((unsigned)((lbd.n_edge + 255) / 256)), b, 0, st,
+                             make_fields<S>(*this), lbd, make_dims(*this), make_scales<C>(*this), ph, t, tp);
+    } else {
+      // RECORD: phase 0 records, phase 2 rolls the history
+      const dim3 g((unsigned)((lbd.total + 255) / 256));
+      for (int ph : {0, 2})
+        hipLaunchKernelGGL((k_lb<S>), g, b, 0, st, make_fields<S>(*this), lbd, make_dims(*this),
+                           make_scales<C>(*this), ph, t, tp);
+    }
   }
   return launch_movies<S>(t, tp, st);
 }

@@ -394,6 +394,11 @@ struct LbDev {
   int64_t total;       // columns
   int n[3];
   LbFace face[6];
+  // APPLY: the columns whose layer-0 node lies on two faces (edges), for the
+  // edge phase and their history roll; the other columns roll their history
+  // in the matrix-core phase
+  const uint32_t* edge;
+  int64_t n_edge;
 };
 
 // One grid's TFSF pass, NF2FF box, point sources or learned layer inside a

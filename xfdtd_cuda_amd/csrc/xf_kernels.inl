@@ -900,9 +900,29 @@ __device__ __forceinline__ void lb_mfma_tiles(const Fields<S>& f, const LbDev& L
       const float r = g == 0 ? rA : rB;
       const float u1 = m == 0.0f ? mur : mur + ldexpf(r, e);
       S* fld = f.e[c.cc == 0 ? c.a : c.b];
-      Store<S>::st(fld, lb_node(L, d, c, 0), (C)u1, sc.s_e);
+      const int64_t node0 = lb_node(L, d, c, 0);
+      Store<S>::st(fld, node0, (C)u1, sc.s_e);
+      // the history roll of this column (phase 2's work): u^{t+1} of layers
+      // 0..3 into the parity slot of t + 1.  Layer 0 is the value just
+      // stored (read back, so a narrow storage format rounds it as it
+      // would), layers 1..3 are x0..x2, interior nodes nothing else of the
+      // step writes; the slot is u^{t-1}'s, which only this column reads.
+      *lb_hist(L, c, par ^ 1, 0, c.p, c.q) = (float)Store<S>::ld(fld, node0, sc.inv_e);
+#pragma unroll
+      for (int l = 1; l < 4; ++l) *lb_hist(L, c, par ^ 1, l, c.p, c.q) = x[l - 1];
     }
   }
+}
+
+// APPLY phases 1 (edge nodes) and 2 (their history roll) over the edge
+// columns only; the other columns roll their history in k_lb_mfma.
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_lb_edges(Fields<S> f, LbDev L, Dims d, Scales<typename Store<S>::C> sc, int phase, int t_arg,
+               const int* t_dev) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.n_edge) return;
+  lb_column_step<S>(f, L, d, sc, phase, t_dev ? *t_dev : t_arg, (int64_t)L.edge[i]);
 }
 
 template <typename S>
@@ -991,17 +1011,31 @@ __global__ void __launch_bounds__(64)
   }
 }
 
-// One VALU phase of the learned layers of every grid of a batch in one
-// launch.  Item g owns blocks [bstart[g], bstart[g + 1]) of 256 columns.
+// One VALU phase of the recording learned layers of a batch in one launch.
+// Item g owns blocks [bstart[g], bstart[g + 1]) of 256 columns.
 template <typename S>
 __global__ void __launch_bounds__(256)
     k_lb_batch(const LbItem<S>* __restrict__ items, const int64_t* __restrict__ bstart, int n,
                int phase, int t_arg, const int* t_dev) {
   const int g = owner_of(bstart, n, (int64_t)blockIdx.x);
   const LbItem<S>& I = items[g];
+  if (I.L.mode != XF_LB_RECORD) return;  // apply layers: k_lb_mfma_batch, k_lb_edges_batch
   const int64_t col = ((int64_t)blockIdx.x - bstart[g]) * blockDim.x + threadIdx.x;
   if (col >= I.L.total) return;
   lb_column_step<S>(I.f, I.L, I.d, I.sc, phase, t_dev ? *t_dev : t_arg, col);
+}
+
+// APPLY phases 1 and 2 of every learned layer of a batch over their edge
+// columns; item g owns edge-list entries [estart[g], estart[g + 1]).
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_lb_edges_batch(const LbItem<S>* __restrict__ items, const int64_t* __restrict__ estart, int n,
+                     int phase, int t_arg, const int* t_dev) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= estart[n]) return;
+  const int g = owner_of(estart, n, tid);
+  const LbItem<S>& I = items[g];
+  lb_column_step<S>(I.f, I.L, I.d, I.sc, phase, t_dev ? *t_dev : t_arg, (int64_t)I.L.edge[tid - estart[g]]);
 }
 
 // APPLY phase 0 of every learned layer of a batch in one launch.  Item g
