@@ -261,6 +261,9 @@ struct Engine {
   int launch_movies(int t, const int* tp, hipStream_t st);
   // z-face CPML corrections in a launch of their own after each Yee pass
   bool split_z() const;
+  // device index of history entry i (xf_learned_boundary layout); y faces
+  // are stored [b][a] on the device (LbFace::tr)
+  int64_t lb_dev_hist_index(int64_t i) const;
   template <typename S>
   int launch_pml_z(bool is_e, hipStream_t st);
   template <typename S>
@@ -1237,6 +1240,7 @@ int Engine::upload_additions() {
     F.a = a;
     F.b = b;
     F.positive = f & 1;
+    F.tr = c == 1 ? 1 : 0;
     F.hist_off = face_off;
     F.na1 = n3[a] + 1;
     F.nb1 = n3[b] + 1;
@@ -1267,8 +1271,11 @@ int Engine::upload_additions() {
             const bool edge = (x == 0 && ((L.faces >> (2 * other_axis)) & 1)) ||
                               (x == lim && ((L.faces >> (2 * other_axis + 1)) & 1));
             if (!edge) continue;
-            const int64_t c = F.start[cc] + static_cast<int64_t>(p - lbd.inset) * (F.q1[cc] - lbd.inset + 1) +
-                              (q - lbd.inset);
+            // thread-order index (lb_column's decode)
+            const int64_t c = F.tr ? F.start[cc] + static_cast<int64_t>(q - lbd.inset) * (F.p1[cc] - lbd.inset + 1) +
+                                         (p - lbd.inset)
+                                   : F.start[cc] + static_cast<int64_t>(p - lbd.inset) * (F.q1[cc] - lbd.inset + 1) +
+                                         (q - lbd.inset);
             edges.push_back(static_cast<uint32_t>(c));
           }
       }
@@ -1281,7 +1288,7 @@ int Engine::upload_additions() {
     }
   }
   std::vector<float> hist(off);
-  for (int64_t i = 0; i < off; ++i) hist[i] = static_cast<float>(L.history.data[i]);
+  for (int64_t i = 0; i < off; ++i) hist[lb_dev_hist_index(i)] = static_cast<float>(L.history.data[i]);
   void* ph;
   XF_TRY(upload_vec(&ph, hist));
   lbd.hist = static_cast<float*>(ph);
@@ -1559,7 +1566,7 @@ int Engine::download() {
     xf_learned_boundary& L = *sc->lb;
     std::vector<float> hist(L.history.shape[0]);
     XF_HIP(hipMemcpy(hist.data(), lbd.hist, hist.size() * sizeof(float), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < hist.size(); ++i) L.history.data[i] = hist[i];
+    for (size_t i = 0; i < hist.size(); ++i) L.history.data[i] = hist[lb_dev_hist_index(static_cast<int64_t>(i))];
     if (L.mode == XF_LB_RECORD) {
       XF_HIP(hipMemcpy(L.rec, lbd.rec, static_cast<size_t>(L.rec_slots) * lbd.total * 13 * sizeof(float),
                        hipMemcpyDeviceToHost));
@@ -1660,6 +1667,24 @@ YeePass<S> Engine::yee_pass(bool is_e) const {
   D.nch = nch_eff;
   fill_pml_pass<C>(*this, D.pml, is_e, split_z() ? 1 : 0);
   return D;
+}
+
+int64_t Engine::lb_dev_hist_index(int64_t i) const {
+  // face blocks sit at 16 (n_a + 1)(n_b + 1) intervals in face order, with
+  // faces outside the mask keeping their block (unused)
+  const int64_t n3[3] = {nx, ny, nz};
+  int64_t off = 0;
+  for (int f = 0; f < 6; ++f) {
+    const int c = f / 2, a = (c + 1) % 3, b = (c + 2) % 3;
+    const int64_t na1 = n3[a] + 1, nb1 = n3[b] + 1, size = 16 * na1 * nb1;
+    if (i < off + size) {
+      if (c != 1) return i;
+      const int64_t r = i - off, plane = r / (na1 * nb1), p = (r / nb1) % na1, q = r % nb1;
+      return off + (plane * nb1 + q) * na1 + p;
+    }
+    off += size;
+  }
+  return i;
 }
 
 // The split applies to 3-D FP64/FP32 grids with one or two z faces whose

@@ -362,6 +362,8 @@ struct MovieDev {
 // component E_a then E_b, then a-index, then b-index.
 struct LbFace {
   int f, c, a, b, positive;
+  int tr;              // y faces: threads walk a (= z) fastest and the device
+                       // history is stored [b][a], so both stream along z
   int64_t hist_off;    // first history value of the face
   int64_t na1, nb1;    // n_a + 1, n_b + 1 (history plane)
   int64_t start[2];    // first column of E_a, E_b

@@ -683,10 +683,12 @@ constexpr int kLbMfmaCols = 4 * 64 * kLbTilesPerWave;  // columns per 256-thread
 // the kernel's argument escapes and the argument stays out of scratch).
 struct LbCol {
   int f, c, a, b, positive;  // face, its normal axis and cyclic frame, side
+  int tr;                    // LbFace::tr
   int cc;                    // component: 0 E_a, 1 E_b
   int p, q;                  // in-plane a- and b-index
   int p1, q1;                // last a- and b-index of the component
   int64_t hist_off, na1, nb1;
+  int64_t abi_col;           // the column's index in xf_learned_boundary order
 };
 
 __device__ __forceinline__ LbCol lb_column(const LbDev& L, int64_t col) {
@@ -698,17 +700,27 @@ __device__ __forceinline__ LbCol lb_column(const LbDev& L, int64_t col) {
   r.a = L.face[fi].a;
   r.b = L.face[fi].b;
   r.positive = L.face[fi].positive;
+  r.tr = L.face[fi].tr;
   r.hist_off = L.face[fi].hist_off;
   r.na1 = L.face[fi].na1;
   r.nb1 = L.face[fi].nb1;
   r.cc = col >= L.face[fi].start[1] ? 1 : 0;
   r.p1 = L.face[fi].p1[r.cc];
   r.q1 = L.face[fi].q1[r.cc];
-  // a component of a face holds < 2^32 columns (checked at upload)
-  const uint32_t local = (uint32_t)(col - L.face[fi].start[r.cc]);
+  // a component of a face holds < 2^32 columns (checked at upload); thread
+  // order is a-major (xf_learned_boundary order) or, on tr faces, b-major
+  const int64_t start = L.face[fi].start[r.cc];
+  const uint32_t local = (uint32_t)(col - start);
   const uint32_t width = (uint32_t)(r.q1 - L.inset + 1);
-  r.p = L.inset + (int)(local / width);
-  r.q = L.inset + (int)(local % width);
+  if (r.tr) {
+    const uint32_t height = (uint32_t)(r.p1 - L.inset + 1);
+    r.q = L.inset + (int)(local / height);
+    r.p = L.inset + (int)(local % height);
+  } else {
+    r.p = L.inset + (int)(local / width);
+    r.q = L.inset + (int)(local % width);
+  }
+  r.abi_col = start + (int64_t)(r.p - L.inset) * width + (r.q - L.inset);
   return r;
 }
 
@@ -723,7 +735,8 @@ __device__ __forceinline__ int64_t lb_node(const LbDev& L, const Dims& d, const 
 
 __device__ __forceinline__ float* lb_hist(const LbDev& L, const LbCol& c, int par, int layer,
                                           int p, int q) {
-  return L.hist + c.hist_off + ((((int64_t)par * 2 + c.cc) * 4 + layer) * c.na1 + p) * c.nb1 + q;
+  const int64_t plane = ((int64_t)par * 2 + c.cc) * 4 + layer;
+  return L.hist + c.hist_off + (c.tr ? (plane * c.nb1 + q) * c.na1 + p : (plane * c.na1 + p) * c.nb1 + q);
 }
 
 // The face a node of the column also lies on (its component tangential to
@@ -793,7 +806,7 @@ __device__ __forceinline__ void lb_column_step(const Fields<S>& f, const LbDev& 
   lb_inputs<S>(f, L, d, sc, c, par, x);
   const int64_t node0 = lb_node(L, d, c, 0);
   if (L.mode == XF_LB_RECORD) {
-    float* r = L.rec + ((int64_t)(t / L.rec_every) * L.total + col) * 13;
+    float* r = L.rec + ((int64_t)(t / L.rec_every) * L.total + c.abi_col) * 13;
 #pragma unroll
     for (int i = 0; i < XF_LB_IN; ++i) r[i] = x[i];
     r[12] = (float)Store<S>::ld(fld, node0, sc.inv_e);
