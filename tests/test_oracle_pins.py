@@ -16,6 +16,8 @@ an independent closed form or invariant chosen so that a plausible mistake
                              Lorentz and Debye media, closed forms
   oracle                     bitwise determinism with OpenMP
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -24,7 +26,8 @@ import xfdtd_cuda_amd as xf
 from xfdtd_cuda_amd import scenes
 from conftest import FIELDS
 
-GOLDEN = "tests/golden/mie_sphere_eps3_mu2_r0p1_1ghz.npz"
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                      "mie_sphere_eps3_mu2_r0p1_1ghz.npz")
 
 
 # ------------------------------------------------------------------ Mie
