@@ -63,7 +63,7 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16", "bf16c"],
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16", "bf16c", "fp16"],
                     help="default: fp64 for the sphere, fp32 for the batch")
     ap.add_argument("--n", type=int, default=512, help="interior cells per edge")
     ap.add_argument("--pml", type=int, default=12)
@@ -148,7 +148,7 @@ def main(argv=None) -> int:
 
     ms_per_step = tm["run_ms"] / K
     value = cells * K / (tm["run_ms"] * 1e-3) / 1e9  # Gcells/s
-    sbytes = {"fp64": 8, "fp32": 4, "bf16": 2, "bf16c": 3}[args.precision]
+    sbytes = {"fp64": 8, "fp32": 4, "bf16": 2, "bf16c": 3, "fp16": 2}[args.precision]
     cbytes = 8 if args.precision == "fp64" else 4
     id_bytes = 1  # compact layout: uint8 material index (<= 256 materials)
     per_cell_pass = 9 * sbytes + id_bytes
@@ -195,7 +195,7 @@ def main(argv=None) -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32",
-                  "bf16c": "bf16+int8-residual-storage/f32"}[args.precision],
+                  "bf16c": "bf16+int8-residual-storage/f32", "fp16": "fp16-storage/f32"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"dielectric sphere {nx - 2 * args.pml}^3 + {args.pml}-cell CPML "
                                f"({nx}x{ny}x{nz} cells), TFSF + NF2FF 1 GHz, {args.precision}",
@@ -279,7 +279,7 @@ def bench_batch(args) -> int:
     print(f"# batch {prec} learned 2-cell layer: {run_lb / K:.3f} ms/step", file=sys.stderr, flush=True)
     sims = sims_cpml
 
-    sbytes = {"fp64": 8, "fp32": 4, "bf16": 2, "bf16c": 3}[prec]
+    sbytes = {"fp64": 8, "fp32": 4, "bf16": 2, "bf16c": 3, "fp16": 2}[prec]
     cbytes = 8 if prec == "fp64" else 4
     bytes_h = sum(int(np.prod(s.info()[0])) * (9 * sbytes + 1) + pml_bytes(s, "h", cbytes)
                   for s in sims)
@@ -311,7 +311,7 @@ def bench_batch(args) -> int:
         "ms_per_step": round(run_ms / K, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32",
-                  "bf16c": "bf16+int8-residual-storage/f32"}[prec],
+                  "bf16c": "bf16+int8-residual-storage/f32", "fp16": "fp16-storage/f32"}[prec],
         "data": "synthetic",
         "config": {"workload": f"batch of {len(sims)} sphere grids, edges U[48,160] + 8-cell CPML "
                                f"({cells} cells), TFSF + NF2FF, {prec}, one persistent launch "
@@ -338,8 +338,9 @@ def bench_batch(args) -> int:
 
 def bench_precision(args) -> int:
     """The 512^3 + 12-cell CPML sphere stepped --steps steps (4000 for the
-    full study) in FP64, FP32, BF16-storage/FP32-compute and compensated
-    BF16 storage (bf16 + an 8-bit residual per value)/FP32-compute.  Each mode's
+    full study) in FP64, FP32, BF16-storage/FP32-compute, compensated BF16
+    storage (bf16 + an 8-bit residual per value)/FP32-compute and
+    FP16-storage/FP32-compute.  Each mode's
     bistatic RCS at 1 GHz on the xz and yz cuts (360 angles) is compared with
     the FP64 run's: relative L2 error over the cut and the largest dB
     difference.  NF2FF sums are FP64 in every mode."""
@@ -349,7 +350,7 @@ def bench_precision(args) -> int:
     S = args.steps
     theta = np.linspace(-np.pi, np.pi, 360)
     res = {}
-    for prec in ("fp64", "fp32", "bf16", "bf16c"):
+    for prec in ("fp64", "fp32", "bf16", "bf16c", "fp16"):
         t0 = time.perf_counter()
         sim = scenes.large_sphere(n=args.n, pml=args.pml, steps=S, layout="compact")
         (nx, ny, nz), _, _ = sim.info()
@@ -374,7 +375,7 @@ def bench_precision(args) -> int:
         return rel, db
 
     modes = {}
-    for prec in ("fp64", "fp32", "bf16", "bf16c"):
+    for prec in ("fp64", "fp32", "bf16", "bf16c", "fp16"):
         r = res[prec]
         e = [err(r["rcs"][c], res["fp64"]["rcs"][c]) for c in range(2)]
         modes[prec] = {"gcells": round(r["gcells"], 4), "ms_per_step": round(r["ms_per_step"], 4),
@@ -575,7 +576,7 @@ def bench_scenes(args) -> int:
          lambda st: scenes.cylinder_2d(n=1000, pml=10, steps=st), 4000, ("fp64", "fp32"), 40, 800),
         ("sphere536", "dielectric sphere 512^3 + 12-cell CPML",
          lambda st: scenes.large_sphere(n=512, pml=12, steps=st, layout="compact"), 400,
-         ("fp64", "fp32", "bf16", "bf16c"), 1, 5),
+         ("fp64", "fp32", "bf16", "bf16c", "fp16"), 1, 5),
         ("dispersive", "Drude + Lorentz sphere 256^3 + 10-cell CPML",
          lambda st: scenes.dispersive_sphere(n=256, pml=10, steps=st), 6000, ("fp64", "fp32"), 2, 20),
     ]
@@ -720,7 +721,7 @@ def bench_nf2ff_td(args) -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32", "bf16": "bf16-storage/f32",
-                  "bf16c": "bf16+int8-residual-storage/f32"}[prec],
+                  "bf16c": "bf16+int8-residual-storage/f32", "fp16": "fp16-storage/f32"}[prec],
         "data": "synthetic",
         "config": {"workload": f"dielectric sphere {n}^3 + {args.pml}-cell CPML ({nx}^3 cells), "
                                f"{nb} NF2FF-TD boxes of {surface} surface cells and {bins} bins",

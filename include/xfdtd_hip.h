@@ -64,8 +64,10 @@ enum xf_precision {
   XF_FP64 = 0,      /* fields, coefficients and arithmetic in double       */
   XF_FP32 = 1,      /* fields and Yee/CPML/TFSF arithmetic in float        */
   XF_BF16_STORE = 2, /* fields stored bf16 (scaled), arithmetic in float   */
-  XF_BF16_COMP = 3   /* fields stored bf16 + an 8-bit residual (3 bytes,
+  XF_BF16_COMP = 3,  /* fields stored bf16 + an 8-bit residual (3 bytes,
                         16 significant bits, scaled), arithmetic in float  */
+  XF_FP16_STORE = 4  /* fields stored IEEE fp16 (scaled: H by the free-space
+                        impedance), arithmetic in float                     */
 };
 
 /* Half-open index range, reference IndexRange (index_task.cuh:16-40). */

@@ -154,7 +154,7 @@ def test_engine_autotune_leaves_state_unchanged(gpu_available):
         assert np.array_equal(g.array(n), c.array(n)), n
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16c"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16c", "fp16"])
 def test_batch_reduced_precision_matches_single_grid(gpu_available, prec):
     steps = 40
     build = lambda: _uneven(steps, count=4, seed=21)

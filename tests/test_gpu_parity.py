@@ -356,6 +356,21 @@ def test_compensated_bf16_storage_close_to_fp64(gpu_available):
     assert errs["fp32"] < errs["bf16c"], errs
 
 
+def test_fp16_storage_between_bf16_and_compensated(gpu_available):
+    """IEEE fp16 storage (XF_FP16_STORE, H scaled by the free-space
+    impedance): 11 significant bits against bf16's 8, so well inside bf16's
+    error and outside the compensated format's 16 bits."""
+    build = lambda: scenes.dielectric_sphere(n=24, pml=6, steps=140, tfsf_inset=9, nf2ff_inset=7)
+    errs = {}
+    for prec in ("fp16", "bf16", "bf16c"):
+        g, c = _pair(build, 140, precision=prec)
+        errs[prec] = max(rel_l2(g.array(n), c.array(n)) for n in ("ex", "ey", "ez", "hy"))
+    print(errs)
+    assert np.isfinite(errs["fp16"])
+    assert errs["fp16"] < errs["bf16"] / 4, errs
+    assert errs["bf16c"] < errs["fp16"], errs
+
+
 def test_bf16_storage_runs(gpu_available):
     build = lambda: scenes.dielectric_sphere(n=24, pml=6, steps=140, tfsf_inset=9, nf2ff_inset=7)
     g, c = _pair(build, 140, precision="bf16")

@@ -133,7 +133,7 @@ struct Batch {
 int Batch::create(xf_scene* const* scenes, int n, const xf_plan* pl) {
   if (n < 1) return fail(XF_ERR_INVALID, "a batch needs at least one scene");
   if (pl) plan = *pl;
-  if (plan.precision < XF_FP64 || plan.precision > XF_BF16_COMP) return fail(XF_ERR_INVALID, "unknown precision");
+  if (plan.precision < XF_FP64 || plan.precision > XF_FP16_STORE) return fail(XF_ERR_INVALID, "unknown precision");
   device = plan.device;
   xf_plan p = plan;
   p.use_graph = 0;
@@ -381,6 +381,7 @@ int Batch::configure() {
     case XF_FP64: st = configure_t<double>(); break;
     case XF_FP32: st = configure_t<float>(); break;
     case XF_BF16_COMP: st = configure_t<Bf16c>(); break;
+    case XF_FP16_STORE: st = configure_t<__half>(); break;
     default: st = configure_t<__hip_bfloat16>(); break;
   }
   if (st != XF_OK) return st;
@@ -484,6 +485,7 @@ int Batch::run(int64_t n) {
       case XF_FP64: return launch_step<double>(t, tdev, ev_on, e);
       case XF_FP32: return launch_step<float>(t, tdev, ev_on, e);
       case XF_BF16_COMP: return launch_step<Bf16c>(t, tdev, ev_on, e);
+      case XF_FP16_STORE: return launch_step<__half>(t, tdev, ev_on, e);
       default: return launch_step<__hip_bfloat16>(t, tdev, ev_on, e);
     }
   };

@@ -742,7 +742,7 @@ double Engine::bytes_per_cell_step() const {
 int Engine::pack_fields() {
   const xf_array* src[6] = {&sc->ex, &sc->ey, &sc->ez, &sc->hx, &sc->hy, &sc->hz};
   const int prec = plan.precision;
-  const double se = 1.0, sh = (prec == XF_BF16_STORE || prec == XF_BF16_COMP) ? 376.730313668 : 1.0;
+  const double se = 1.0, sh = (prec == XF_BF16_STORE || prec == XF_BF16_COMP || prec == XF_FP16_STORE) ? 376.730313668 : 1.0;
   std::vector<unsigned char> buf(total * esize);
   for (int c = 0; c < 6; ++c) {
     const xf_array& a = *src[c];
@@ -773,7 +773,9 @@ int Engine::pack_fields() {
             unsigned char* b = buf.data() + (idx >> 6) * 192;
             reinterpret_cast<uint16_t*>(b)[idx & 63] = h;
             reinterpret_cast<int8_t*>(b + 128)[idx & 63] = r;
-          } else
+          } else if (prec == XF_FP16_STORE)
+            reinterpret_cast<__half*>(buf.data())[idx] = __float2half(static_cast<float>(v * scale));
+          else
             reinterpret_cast<__hip_bfloat16*>(buf.data())[idx] = __float2bfloat16(static_cast<float>(v * scale));
         }
     XF_TRY(alloc(&fld[c], total * esize));
@@ -786,7 +788,7 @@ int Engine::pack_fields() {
 int Engine::unpack_fields() {
   xf_array* dst[6] = {&sc->ex, &sc->ey, &sc->ez, &sc->hx, &sc->hy, &sc->hz};
   const int prec = plan.precision;
-  const double sh = (prec == XF_BF16_STORE || prec == XF_BF16_COMP) ? 376.730313668 : 1.0;
+  const double sh = (prec == XF_BF16_STORE || prec == XF_BF16_COMP || prec == XF_FP16_STORE) ? 376.730313668 : 1.0;
   std::vector<unsigned char> buf(total * esize);
   for (int c = 0; c < 6; ++c) {
     if (!fld[c]) continue;
@@ -806,7 +808,9 @@ int Engine::unpack_fields() {
             const unsigned char* b = buf.data() + (idx >> 6) * 192;
             v = static_cast<double>(bf16c_decode(reinterpret_cast<const uint16_t*>(b)[idx & 63],
                                                  reinterpret_cast<const int8_t*>(b + 128)[idx & 63])) * inv;
-          } else
+          } else if (prec == XF_FP16_STORE)
+            v = static_cast<double>(__half2float(reinterpret_cast<const __half*>(buf.data())[idx])) * inv;
+          else
             v = static_cast<double>(__bfloat162float(reinterpret_cast<const __hip_bfloat16*>(buf.data())[idx])) * inv;
           a.data[(i * a.shape[1] + j) * a.shape[2] + k] = v;
         }
@@ -1631,7 +1635,8 @@ template <typename C>
 Scales<C> make_scales(const Engine& E) {
   Scales<C> s;
   const double sh =
-      (E.plan.precision == XF_BF16_STORE || E.plan.precision == XF_BF16_COMP) ? 376.730313668 : 1.0;
+      (E.plan.precision == XF_BF16_STORE || E.plan.precision == XF_BF16_COMP ||
+       E.plan.precision == XF_FP16_STORE) ? 376.730313668 : 1.0;
   s.inv_e = 1;
   s.s_e = 1;
   s.inv_h = static_cast<C>(1.0 / sh);
@@ -2037,6 +2042,8 @@ int Engine::run(int64_t n) {
         return launch_step<float>(t, false, timing_on, e);
       case XF_BF16_COMP:
         return launch_step<Bf16c>(t, false, timing_on, e);
+      case XF_FP16_STORE:
+        return launch_step<__half>(t, false, timing_on, e);
       default:
         return launch_step<__hip_bfloat16>(t, false, timing_on, e);
     }
@@ -2054,6 +2061,9 @@ int Engine::run(int64_t n) {
           break;
         case XF_BF16_COMP:
           st = launch_step<Bf16c>(0, true, false, nullptr);
+          break;
+        case XF_FP16_STORE:
+          st = launch_step<__half>(0, true, false, nullptr);
           break;
         default:
           st = launch_step<__hip_bfloat16>(0, true, false, nullptr);
@@ -2133,7 +2143,7 @@ int xf_create(xf_scene* scene, const xf_plan* plan, xf_handle** out) {
   if (!h) return xf::fail(XF_ERR_NOMEM, "out of host memory");
   h->eng.sc = scene;
   if (plan) h->eng.plan = *plan;
-  if (h->eng.plan.precision < XF_FP64 || h->eng.plan.precision > XF_BF16_COMP) {
+  if (h->eng.plan.precision < XF_FP64 || h->eng.plan.precision > XF_FP16_STORE) {
     delete h;
     return xf::fail(XF_ERR_INVALID, "unknown precision");
   }

@@ -12,6 +12,7 @@
 #define XF_KERNELS_H_
 
 #include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <limits.h>
 #include <stdint.h>
@@ -69,6 +70,19 @@ struct Store<__hip_bfloat16> {
   static __device__ __forceinline__ void st_nt(__hip_bfloat16* p, int64_t i, C v, C scale) {
     st(p, i, v, scale);
   }
+};
+
+template <>
+struct Store<__half> {
+  using C = float;
+  static __device__ __forceinline__ C ld(const __half* p, int64_t i, C inv) {
+    return __half2float(p[i]) * inv;
+  }
+  static __device__ __forceinline__ void st(__half* p, int64_t i, C v, C scale) {
+    p[i] = __float2half(v * scale);
+  }
+  static __device__ __forceinline__ C ld_nt(const __half* p, int64_t i, C inv) { return ld(p, i, inv); }
+  static __device__ __forceinline__ void st_nt(__half* p, int64_t i, C v, C scale) { st(p, i, v, scale); }
 };
 
 // Compensated bf16 storage: a value y is kept as hi = bf16(y) (round to

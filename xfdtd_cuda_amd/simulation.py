@@ -20,7 +20,7 @@ from ._native import XfhMaterial, XfPlan, XfTiming, XfTuning, c_dp, c_i32, c_i64
 
 DIRECTIONS = {"XN": 0, "XP": 1, "YN": 2, "YP": 3, "ZN": 4, "ZP": 5}
 FACES = ("XN", "XP", "YN", "YP", "ZN", "ZP")
-PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2, "bf16c": 3}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16": 2, "bf16c": 3, "fp16": 4}
 
 EPSILON_0 = 8.854187817e-12
 MU_0 = 4.0 * np.pi * 1e-7
