@@ -82,6 +82,8 @@ def main(argv=None) -> int:
                          "scenes: every quoted scene on the GPU and the CPU oracle")
     ap.add_argument("--tile-y", type=int, default=0, help="rows per Yee tile (0: engine default)")
     ap.add_argument("--chunks", type=int, default=0, help="marches along x (0: engine default)")
+    ap.add_argument("--no-nt", action="store_true",
+                    help="plain loads/stores for the updated field (sphere workload)")
     ap.add_argument("--k32", action="store_true",
                     help="32-lane k-tiles with two j-rows per wave (sphere workload)")
     ap.add_argument("--pml-fused", action="store_true",
@@ -123,8 +125,8 @@ def main(argv=None) -> int:
     (nx, ny, nz), dt, _ = sim.info()
     cells = nx * ny * nz
     eng = xf.Engine(sim, precision=args.precision, timing=False).upload()
-    if args.tile_y or args.chunks or args.pml_fused or args.k32:
-        eng.set_tiling(args.tile_y or 1, args.chunks, xcd_remap=False, nontemporal=True,
+    if args.tile_y or args.chunks or args.pml_fused or args.k32 or args.no_nt:
+        eng.set_tiling(args.tile_y or 1, args.chunks, xcd_remap=False, nontemporal=not args.no_nt,
                        pml_split=not args.pml_fused, k32=args.k32)
     setup_s = time.perf_counter() - t_setup
     print(f"# setup {setup_s:.1f} s, grid {nx}x{ny}x{nz}, {eng.describe()}", file=sys.stderr,

@@ -64,6 +64,8 @@ struct Store<__hip_bfloat16> {
   static __device__ __forceinline__ void st(__hip_bfloat16* p, int64_t i, C v, C scale) {
     p[i] = __float2bfloat16(v * scale);
   }
+  // the streaming slots use plain accesses: non-temporal 16-bit loads and
+  // stores measured slower at 536^3 (59.9 against 61.9 Gcells/s)
   static __device__ __forceinline__ C ld_nt(const __hip_bfloat16* p, int64_t i, C inv) {
     return ld(p, i, inv);
   }
@@ -81,6 +83,8 @@ struct Store<__half> {
   static __device__ __forceinline__ void st(__half* p, int64_t i, C v, C scale) {
     p[i] = __float2half(v * scale);
   }
+  // plain accesses in the streaming slots, as for bf16 (non-temporal 16-bit
+  // accesses measured slower: 57.9 against 59.9 Gcells/s at 536^3)
   static __device__ __forceinline__ C ld_nt(const __half* p, int64_t i, C inv) { return ld(p, i, inv); }
   static __device__ __forceinline__ void st_nt(__half* p, int64_t i, C v, C scale) { st(p, i, v, scale); }
 };
