@@ -124,7 +124,10 @@ def main(argv=None) -> int:
     sim = scenes.large_sphere(n=args.n, pml=args.pml, steps=total_steps, layout="compact")
     (nx, ny, nz), dt, _ = sim.info()
     cells = nx * ny * nz
+    t_up = time.perf_counter()
+    build_s = t_up - t_setup
     eng = xf.Engine(sim, precision=args.precision, timing=False).upload()
+    upload_s = time.perf_counter() - t_up
     if args.tile_y or args.chunks or args.pml_fused or args.k32 or args.no_nt:
         eng.set_tiling(args.tile_y or 1, args.chunks, xcd_remap=False, nontemporal=not args.no_nt,
                        pml_split=not args.pml_fused, k32=args.k32)
@@ -171,10 +174,12 @@ def main(argv=None) -> int:
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
+    t_dn = time.perf_counter()
+    eng.download()  # fields, psi and NF2FF currents back into the host arrays
+    download_s = time.perf_counter() - t_dn
     cpu = None
     if not args.no_cpu:
         import oracle
-        eng.download()
         threads = oracle.threads()
         tc = time.perf_counter()
         oracle.run(sim.scene_ptr(), args.cpu_steps)
@@ -202,7 +207,9 @@ def main(argv=None) -> int:
         "config": {"workload": f"dielectric sphere {nx - 2 * args.pml}^3 + {args.pml}-cell CPML "
                                f"({nx}x{ny}x{nz} cells), TFSF + NF2FF 1 GHz, {args.precision}",
                    "cells": cells, "layout": "compact (uint8 material index)",
-                   "wall_s": round(wall, 4)},
+                   "wall_s": round(wall, 4),
+                   "end_to_end_s": {"scene_build": round(build_s, 2), "upload": round(upload_s, 2),
+                                    "steps_timed": round(wall, 3), "download": round(download_s, 2)}},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": traffic, "alg_bytes_per_launch": alg_bytes,
