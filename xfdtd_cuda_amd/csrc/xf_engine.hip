@@ -9,6 +9,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -739,6 +740,25 @@ double Engine::bytes_per_cell_step() const {
 }
 
 // ------------------------------------------------------------- upload
+namespace {
+// fn(lo, hi) over [0, n) split across the host's cores (the packing loops
+// write disjoint elements)
+template <typename F>
+void host_parallel_for(int64_t n, F fn) {
+  const int64_t hw = std::max<int64_t>(1, static_cast<int64_t>(std::thread::hardware_concurrency()));
+  const int64_t nth = std::min<int64_t>(std::min<int64_t>(hw, 32), std::max<int64_t>(1, n / 4));
+  if (nth <= 1) {
+    fn(int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(nth);
+  for (int64_t t = 0; t < nth; ++t)
+    pool.emplace_back([&, t] { fn(n * t / nth, n * (t + 1) / nth); });
+  for (auto& th : pool) th.join();
+}
+}  // namespace
+
 int Engine::pack_fields() {
   const xf_array* src[6] = {&sc->ex, &sc->ey, &sc->ez, &sc->hx, &sc->hy, &sc->hz};
   const int prec = plan.precision;
@@ -756,7 +776,9 @@ int Engine::pack_fields() {
     }
     std::fill(buf.begin(), buf.end(), 0);
     const double scale = c < 3 ? se : sh;
-    for (int64_t i = 0; i < a.shape[0]; ++i)
+    // compensated bf16 packs 64-value blocks: split on block-aligned planes
+    host_parallel_for(a.shape[0], [&](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i)
       for (int64_t j = 0; j < a.shape[1]; ++j)
         for (int64_t k = 0; k < a.shape[2]; ++k) {
           const double v = a.data[(i * a.shape[1] + j) * a.shape[2] + k];
@@ -778,6 +800,7 @@ int Engine::pack_fields() {
           else
             reinterpret_cast<__hip_bfloat16*>(buf.data())[idx] = __float2bfloat16(static_cast<float>(v * scale));
         }
+    });
     XF_TRY(alloc(&fld[c], total * esize));
     XF_HIP(hipMemcpy(fld[c], buf.data(), total * esize, hipMemcpyHostToDevice));
     state.emplace_back(fld[c], total * esize);
@@ -795,7 +818,8 @@ int Engine::unpack_fields() {
     XF_HIP(hipMemcpy(buf.data(), fld[c], total * esize, hipMemcpyDeviceToHost));
     xf_array& a = *dst[c];
     const double inv = c < 3 ? 1.0 : 1.0 / sh;
-    for (int64_t i = 0; i < a.shape[0]; ++i)
+    host_parallel_for(a.shape[0], [&](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i)
       for (int64_t j = 0; j < a.shape[1]; ++j)
         for (int64_t k = 0; k < a.shape[2]; ++k) {
           const int64_t idx = dim == 3 ? i * sx + j * sy + k : i * sx + j;
@@ -814,6 +838,7 @@ int Engine::unpack_fields() {
             v = static_cast<double>(__bfloat162float(reinterpret_cast<const __hip_bfloat16*>(buf.data())[idx])) * inv;
           a.data[(i * a.shape[1] + j) * a.shape[2] + k] = v;
         }
+    });
   }
   return XF_OK;
 }
