@@ -2,14 +2,17 @@
 //
 // H pass: BasicUpdator::updateH (basic_updator.cuh:16-32) with hNext
 // (update_scheme.cuh:58-63) for Hx, Hy, Hz fused, plus the H corrections of
-// every PML face (pml_corrector.cu:180-270).
+// the PML faces the pass carries (pml_corrector.cu:180-270).
 // E pass: BasicUpdator3D::updateE (basic_updator_3d.cu:10-34) with eNext
 // (update_scheme.cuh:27-32, skipping a == 0 || b == 0) for Ex, Ey, Ez
-// fused, plus the E corrections of every PML face (pml_corrector.cu:50-178).
+// fused, plus the E corrections of the PML faces the pass carries
+// (pml_corrector.cu:50-178).  The pass carries every face, or every face
+// but the z faces, which k_pml_z then applies after it (they are the last
+// corrections of a node either way).
 //
 // Work decomposition: a tile is blockDim.x (64, or 32) lanes along k x
-// blockDim.y rows along j;
-// each thread marches i through the tile's x-chunk, carrying the plane it
+// blockDim.y rows along j; each thread marches i through the tile's
+// x-chunk, carrying the plane it
 // has loaded as the next cell's i-1 (E pass: Hy, Hz) or i (H pass: Ey, Ez)
 // neighbour, and issuing the loads of plane i+1 before computing plane i
 // (two planes of loads in flight per wave).  Arithmetic and its order are
