@@ -89,6 +89,9 @@ def main(argv=None) -> int:
     ap.add_argument("--pml-fused", action="store_true",
                     help="z-face CPML corrections inside the Yee kernels instead of a launch of "
                          "their own (sphere workload)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each step as a hipGraph (batch, precision, learned, scenes, "
+                         "dispersive, cylinder2d); eager launches measured faster on MI355X")
     ap.add_argument("--td-boxes", type=int, default=3, help="NF2FF-TD boxes of nf2ff_td")
     ap.add_argument("--count", type=int, default=64, help="grids of the batch workload")
     args = ap.parse_args(argv)
@@ -225,10 +228,11 @@ def main(argv=None) -> int:
 def bench_batch(args) -> int:
     """64 sphere grids of uneven sizes (edges U[48, 160] + 8-cell CPML,
     radius U[0.2, 0.4] x edge, eps_r U[1.5, 8], seed 7) stepped together in
-    one persistent launch per Yee pass under a hipGraph, tile shape from the
-    autotuner.  Besides the batch rate the line carries the same grids run
-    one after another through the single-grid engine, and the batch in FP64,
-    for the speed-ups batching and FP32 give."""
+    one persistent launch per Yee pass (eager launches, or a hipGraph with
+    --graph), tile shape from the autotuner.  Besides the batch rate the
+    line carries the same grids run one after another through the
+    single-grid engine, and the batch in FP64, for the speed-ups batching and
+    FP32 give."""
     import xfdtd_cuda_amd as xf
     from xfdtd_cuda_amd import scenes
 
@@ -242,7 +246,7 @@ def bench_batch(args) -> int:
           file=sys.stderr, flush=True)
 
     def timed_batch(precision):
-        b = xf.Batch(sims, precision=precision, use_graph=True).upload()
+        b = xf.Batch(sims, precision=precision, use_graph=args.graph).upload()
         best = b.autotune(trial_steps=3)
         b.run(W)
         b.reset_timing()
@@ -263,7 +267,7 @@ def bench_batch(args) -> int:
     # the same grids one after another through the single-grid engine
     seq_ms = 0.0
     for s in sims:
-        e = xf.Engine(s, precision=prec, use_graph=True).upload()
+        e = xf.Engine(s, precision=prec, use_graph=args.graph).upload()
         e.run(2)
         e.run(K)
         seq_ms += e.timing()["run_ms"]
@@ -324,7 +328,7 @@ def bench_batch(args) -> int:
         "data": "synthetic",
         "config": {"workload": f"batch of {len(sims)} sphere grids, edges U[48,160] + 8-cell CPML "
                                f"({cells} cells), TFSF + NF2FF, {prec}, one persistent launch "
-                               "per pass, hipGraph step",
+                               "per pass, " + ("hipGraph step" if args.graph else "eager launches"),
                    "cells": cells, "tuning": best, "describe": desc,
                    "sequential_single_grid_gcells": round(seq_value, 4),
                    "batch_speedup_vs_sequential": round(value / seq_value, 3),
@@ -363,7 +367,7 @@ def bench_precision(args) -> int:
         t0 = time.perf_counter()
         sim = scenes.large_sphere(n=args.n, pml=args.pml, steps=S, layout="compact")
         (nx, ny, nz), _, _ = sim.info()
-        eng = xf.Engine(sim, precision=prec, use_graph=True).upload()
+        eng = xf.Engine(sim, precision=prec, use_graph=args.graph).upload()
         eng.run(S)
         run_ms = eng.timing()["run_ms"]
         eng.download()
@@ -439,7 +443,7 @@ def bench_learned(args) -> int:
         sim = scenes.point_source_box(n=n, boundary=kind, size=size, steps=W + K + 4, seed=7,
                                       weights=w16 if kind == "learned" else None)
         cells = int(np.prod(sim.info()[0]))
-        e = xf.Engine(sim, precision="fp32", use_graph=True).upload()
+        e = xf.Engine(sim, precision="fp32", use_graph=args.graph).upload()
         e.run(W)
         e.reset_timing()
         e.run(K)
@@ -505,7 +509,7 @@ def bench_fp32_vs_fp64(args) -> int:
             sim = scenes.dispersive_sphere(n=n, pml=10, steps=steps)
             freqs = [0.5e9, 1e9, 2e9]
         (nx, ny, nz), _, _ = sim.info()
-        eng = xf.Engine(sim, precision=prec, use_graph=True).upload()
+        eng = xf.Engine(sim, precision=prec, use_graph=args.graph).upload()
         eng.run(steps)
         run_ms = eng.timing()["run_ms"]
         eng.download()
@@ -613,7 +617,7 @@ def bench_scenes(args) -> int:
             sim = build(W + gsteps + extra + 2)
             (nx, ny, nz), _, _ = sim.info()
             cells = nx * ny * nz
-            eng = xf.Engine(sim, precision=prec, use_graph=True).upload()
+            eng = xf.Engine(sim, precision=prec, use_graph=args.graph).upload()
             eng.run(W)
             eng.run(gsteps)
             run_ms = eng.timing()["run_ms"]
@@ -640,7 +644,7 @@ def bench_scenes(args) -> int:
         extra = 2 if prec == "fp64" else 0
         sims = scenes.batch_spheres(count=64, steps=W + K + extra + 20)
         cells = sum(int(np.prod(s.info()[0])) for s in sims)
-        b = xf.Batch(sims, precision=prec, use_graph=True).upload()
+        b = xf.Batch(sims, precision=prec, use_graph=args.graph).upload()
         b.autotune(trial_steps=3)
         b.run(W)
         b.run(K)
@@ -667,8 +671,9 @@ def bench_scenes(args) -> int:
         "steps": big["gpu_fp64"]["steps"], "warmup": W,
         "ms_per_step": big["gpu_fp64"]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "every quoted scene on the GPU (FP64 and reduced precision, hipGraph) "
-                               "and on the CPU oracle (1 and all threads); value = the 536^3 sphere in FP64",
+        "config": {"workload": "every quoted scene on the GPU (FP64 and reduced precision, "
+                               + ("hipGraph" if args.graph else "eager launches")
+                               + ") and on the CPU oracle (1 and all threads); value = the 536^3 sphere in FP64",
                    "scenes": rows},
         "roofline": None,
         "cpu_baseline": {"value": big["cpu_all"]["gcells"], "unit": "Gcells/s", "cores": nthr,
