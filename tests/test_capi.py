@@ -170,3 +170,84 @@ def test_batch_and_tuning_null_arguments():
     assert lib.xf_batch_set_tuning(None, ctypes.byref(t)) == 1
     assert lib.xf_batch_autotune(None, 1, ctypes.byref(t)) == 1
     assert lib.xf_autotune(None, 1, ctypes.byref(t)) == 1
+
+
+def test_set_tiling_rejects_more_than_four_rows():
+    """The Yee kernels are built for at most 256 threads per block (64 k-lanes
+    x 4 rows); 5..8 rows would launch 320..512 threads and fail at launch.
+    A tiling set before the upload is kept in the plan (xf_describe shows it;
+    upload re-applies the plan)."""
+    s = scenes.dielectric_sphere(n=16, pml=4, steps=20, tfsf_inset=6, nf2ff_inset=5)
+    st, h = _create(s)
+    assert st == 0, _msg()
+    lib = _native.hip()
+    try:
+        for rows in (5, 6, 7, 8, -1):
+            assert lib.xf_set_tiling(h, rows, 0, 0) == 1, rows
+            assert "tile_y" in _msg()
+        assert lib.xf_set_tiling(h, 3, 0, 0) == 0, _msg()
+        buf = ctypes.create_string_buffer(512)
+        assert lib.xf_describe(h, buf, 512) == 0
+        assert "tile_y=3" in buf.value.decode()
+    finally:
+        lib.xf_destroy(h)
+
+
+def test_packed_size_overflow_returns_a_status():
+    """A grid whose padded field arrays would not fit any device (here
+    (2^20 + 1)^3 values of 8 bytes, which also overflows int64 bytes) comes
+    back as XF_ERR_NOMEM from xf_create, not as an exception or a crash."""
+    s = scenes.dielectric_sphere(n=16, pml=4, steps=20, tfsf_inset=6, nf2ff_inset=5)
+    ptr = s.scene_ptr()
+    dims = [ctypes.c_int64.from_address(ptr + off) for off in (8, 16, 24)]  # xf_scene nx, ny, nz
+    keep = [d.value for d in dims]
+    assert keep == [24, 24, 24]
+    try:
+        for d in dims:
+            d.value = 1 << 20
+        st, _ = _create(s)
+        assert st == 5 and "packed grid too large" in _msg(), (st, _msg())
+    finally:
+        for d, v in zip(dims, keep):
+            d.value = v
+    st, h = _create(s)
+    assert st == 0, _msg()
+    _native.hip().xf_destroy(h)
+
+
+def test_dispersive_cell_inside_cpml_rejected():
+    """The engine applies the CPML E corrections inside the Yee kernels,
+    before the ADE kernel, while the reference adds them after the ADE update
+    (domain_hd.cu:35-87): a dispersive medium reaching into a CPML slab is
+    refused with XF_ERR_UNSUPPORTED instead of being stepped in the wrong
+    order.  The same medium kept clear of the slabs is accepted."""
+    dl = 3e-3
+    for reach, expect in ((0.03, 4), (0.006, 0)):
+        s = xf.Simulation(dl, dl, dl, 0.9)
+        s.add_cube("domain", (0, 0, 0), (0.03,) * 3, xf.Material.air())
+        s.add_pml_all(3)
+        c = 0.015
+        s.add_cube("drude", (c - reach / 2,) * 3, (reach,) * 3, xf.Material.drude(1.0, [1e10], [1e9]))
+        s.init(10)
+        st, h = _create(s)
+        assert st == expect, (reach, st, _msg())
+        if st == 0:
+            _native.hip().xf_destroy(h)
+        else:
+            assert "inside a CPML slab" in _msg()
+
+
+def test_dispersive_material_without_parameters_rejected():
+    """xfh_material kinds 2-5 read eps_inf and the poles from p: a null p or a
+    pole count below 1 is an XF_ERR_INVALID, not a null dereference."""
+    lib = _native.host()
+    s = xf.Simulation(3e-3, 3e-3, 3e-3, 0.9)
+    s.add_cube("domain", (0, 0, 0), (0.03,) * 3, xf.Material.air())
+    centre = (ctypes.c_double * 3)(0.015, 0.015, 0.015)
+    for kind, n_pole, p in ((2, 1, None), (3, 1, None), (4, 0, (ctypes.c_double * 4)(1, 1, 1, 1)),
+                            (5, -1, (ctypes.c_double * 6)(1, 1, 1, 1, 1, 1))):
+        m = _native.XfhMaterial(kind, 1.0, 1.0, 0.0, 0.0, n_pole, None if p is None else
+                                ctypes.cast(p, _native.c_dp))
+        st = lib.xfh_sim_add_sphere(s._h, b"bad", centre, ctypes.c_double(0.006), ctypes.byref(m))
+        assert st != 0, kind
+        assert b"dispersive material" in lib.xfh_last_error(), lib.xfh_last_error()

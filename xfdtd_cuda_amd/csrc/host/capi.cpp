@@ -42,6 +42,11 @@ std::unique_ptr<xfdtd::Material> makeMaterial(const xfh_material* m) {
   if (m == nullptr) return Material::createAir();
   std::vector<double> p0, p1, p2;
   const double* p = m->p;
+  if (m->kind >= 2 && m->kind <= 5) {
+    // eps_inf and the poles are read from p: it must hold them
+    if (p == nullptr) throw xfdtd::XFDTDException("dispersive material: null parameter array");
+    if (m->n_pole < 1 || m->n_pole > 64) throw xfdtd::XFDTDException("dispersive material: n_pole must be 1..64");
+  }
   switch (m->kind) {
     case 0:
       return std::make_unique<Material>(

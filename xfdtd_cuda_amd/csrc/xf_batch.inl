@@ -489,24 +489,16 @@ int Batch::run(int64_t n) {
       default: return launch_step<__hip_bfloat16>(t, tdev, ev_on, e);
     }
   };
-  std::vector<hipEvent_t> ev;
-  if (timing_on) {
-    ev.resize(5 * n);
-    for (auto& e : ev) XF_HIP(hipEventCreate(&e));
-  }
-  hipEvent_t run0, run1;
-  XF_HIP(hipEventCreate(&run0));
-  XF_HIP(hipEventCreate(&run1));
+  HipEvents ev, runs;
+  if (timing_on) XF_TRY(ev.create(5 * n));
+  XF_TRY(runs.create(2));
+  hipEvent_t run0 = runs[0], run1 = runs[1];
   int status = XF_OK;
   if (graph_on) {
     if (!graph_ready) {
       XF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       const int st = step(0, true, false, nullptr);
-      hipGraph_t g;
-      const hipError_t ce = hipStreamEndCapture(stream, &g);
-      if (st != XF_OK) return st;
-      if (ce != hipSuccess) return fail(XF_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-      graph = g;
+      XF_TRY(end_capture(stream, st, &graph));
       XF_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
       graph_ready = true;
     }
@@ -543,10 +535,7 @@ int Batch::run(int64_t n) {
     }
     timing.yee_h_launches += n;
     timing.yee_e_launches += n;
-    for (auto& e : ev) (void)hipEventDestroy(e);
   }
-  (void)hipEventDestroy(run0);
-  (void)hipEventDestroy(run1);
   // algorithmic bytes per cell-step, cell-weighted over the grids
   double bytes = 0;
   for (auto& h : inst) bytes += h->eng.bytes_per_cell_step() * static_cast<double>(h->eng.nx * h->eng.ny * h->eng.nz);
@@ -711,104 +700,127 @@ struct xf_batch {
 extern "C" {
 
 int xf_autotune(xf_handle* h, int32_t trial_steps, xf_tuning* best) {
-  xf::g_err.clear();
-  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
-  return h->eng.autotune(trial_steps, best);
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+    return h->eng.autotune(trial_steps, best);
+  });
 }
 
 int xf_batch_create(xf_scene* const* scenes, int32_t n, const xf_plan* plan, xf_batch** out) {
-  xf::g_err.clear();
-  if (!scenes || !out) return xf::fail(XF_ERR_INVALID, "null scenes or out");
-  *out = nullptr;
-  auto b = new (std::nothrow) xf_batch;
-  if (!b) return xf::fail(XF_ERR_NOMEM, "out of host memory");
-  const int st = b->b.create(scenes, n, plan);
-  if (st != XF_OK) {
-    delete b;
-    return st;
-  }
-  *out = b;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!scenes || !out) return xf::fail(XF_ERR_INVALID, "null scenes or out");
+    *out = nullptr;
+    std::unique_ptr<xf_batch> b(new (std::nothrow) xf_batch);
+    if (!b) return xf::fail(XF_ERR_NOMEM, "out of host memory");
+    const int st = b->b.create(scenes, n, plan);
+    if (st != XF_OK) return st;
+    *out = b.release();
+    return XF_OK;
+  });
 }
 
 int xf_batch_upload(xf_batch* b) {
-  xf::g_err.clear();
-  if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
-  const int st = b->b.upload();
-  if (st != XF_OK) b->b.release();
-  return st;
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
+    const int st = xf::guarded([&] { return b->b.upload(); });
+    if (st != XF_OK) b->b.release();
+    return st;
+  });
 }
 
 int xf_batch_run(xf_batch* b, int64_t n_steps) {
-  xf::g_err.clear();
-  if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
-  return b->b.run(n_steps);
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
+    return b->b.run(n_steps);
+  });
 }
 
 int xf_batch_download(xf_batch* b) {
-  xf::g_err.clear();
-  if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
-  return b->b.download();
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
+    return b->b.download();
+  });
 }
 
 int xf_batch_destroy(xf_batch* b) {
-  delete b;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    delete b;
+    return XF_OK;
+  });
 }
 
 int xf_batch_set_tuning(xf_batch* b, const xf_tuning* t) {
-  xf::g_err.clear();
-  if (!b || !t) return xf::fail(XF_ERR_INVALID, "null argument");
-  const xf_tuning keep = b->b.tune;
-  b->b.tune = *t;
-  b->b.tune.step_ms = 0;
-  if (t->tile_y < 1 || t->tile_y > 4 || t->planes < 1 || t->blocks < 0) {
-    b->b.tune = keep;
-    return xf::fail(XF_ERR_INVALID, "bad tuning (tile_y 1..4, planes >= 1, blocks >= 0)");
-  }
-  if (!b->b.uploaded) return XF_OK;
-  const int st = b->b.configure();
-  if (st != XF_OK) {
-    b->b.tune = keep;
-    (void)b->b.configure();
-  }
-  return st;
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!b || !t) return xf::fail(XF_ERR_INVALID, "null argument");
+    const xf_tuning keep = b->b.tune;
+    b->b.tune = *t;
+    b->b.tune.step_ms = 0;
+    if (t->tile_y < 1 || t->tile_y > 4 || t->planes < 1 || t->blocks < 0) {
+      b->b.tune = keep;
+      return xf::fail(XF_ERR_INVALID, "bad tuning (tile_y 1..4, planes >= 1, blocks >= 0)");
+    }
+    if (!b->b.uploaded) return XF_OK;
+    const int st = b->b.configure();
+    if (st != XF_OK) {
+      b->b.tune = keep;
+      (void)b->b.configure();
+    }
+    return st;
+  });
 }
 
 int xf_batch_get_tuning(const xf_batch* b, xf_tuning* t) {
-  if (!b || !t) return xf::fail(XF_ERR_INVALID, "null argument");
-  *t = b->b.tune;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!b || !t) return xf::fail(XF_ERR_INVALID, "null argument");
+    *t = b->b.tune;
+    return XF_OK;
+  });
 }
 
 int xf_batch_autotune(xf_batch* b, int32_t trial_steps, xf_tuning* best) {
-  xf::g_err.clear();
-  if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
-  return b->b.autotune(trial_steps, best);
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
+    return b->b.autotune(trial_steps, best);
+  });
 }
 
 int xf_batch_get_timing(const xf_batch* b, xf_timing* out) {
-  if (!b || !out) return xf::fail(XF_ERR_INVALID, "null argument");
-  *out = b->b.timing;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!b || !out) return xf::fail(XF_ERR_INVALID, "null argument");
+    *out = b->b.timing;
+    return XF_OK;
+  });
 }
 
 int xf_batch_reset_timing(xf_batch* b) {
-  if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
-  b->b.timing = xf_timing{};
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
+    b->b.timing = xf_timing{};
+    return XF_OK;
+  });
 }
 
 int xf_batch_set_timing(xf_batch* b, int32_t on) {
-  if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
-  b->b.plan.timing = on ? 1 : 0;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!b) return xf::fail(XF_ERR_INVALID, "null batch");
+    b->b.plan.timing = on ? 1 : 0;
+    return XF_OK;
+  });
 }
 
 int xf_batch_describe(const xf_batch* b, char* buf, size_t len) {
-  if (!b || !buf || len == 0) return xf::fail(XF_ERR_INVALID, "null argument");
-  std::snprintf(buf, len, "%s", b->b.desc.c_str());
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!b || !buf || len == 0) return xf::fail(XF_ERR_INVALID, "null argument");
+    std::snprintf(buf, len, "%s", b->b.desc.c_str());
+    return XF_OK;
+  });
 }
 
 }  // extern "C"

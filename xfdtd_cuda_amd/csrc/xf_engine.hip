@@ -8,7 +8,11 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <memory>
+#include <new>
+#include <stdexcept>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -38,6 +42,60 @@ int fail(int code, const std::string& m) {
     int s_ = (call);              \
     if (s_ != XF_OK) return s_;   \
   } while (0)
+
+// Every extern "C" entry point runs its body through guarded(): no C++
+// exception crosses the C ABI.  Host allocations (std::vector staging of a
+// field, the hash tables of the coefficient compression) and thread start-up
+// can throw; they come back as status codes.
+template <typename F>
+int guarded(F&& body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return fail(XF_ERR_NOMEM, "host allocation failed");
+  } catch (const std::length_error& e) {
+    return fail(XF_ERR_NOMEM, std::string("host allocation too large: ") + e.what());
+  } catch (const std::system_error& e) {
+    return fail(XF_ERR_NOMEM, std::string("host resources exhausted: ") + e.what());
+  } catch (const std::exception& e) {
+    return fail(XF_ERR_INVALID, std::string("internal error: ") + e.what());
+  } catch (...) {
+    return fail(XF_ERR_INVALID, "internal error: unknown exception");
+  }
+}
+
+// HIP events owned for the length of one run: destroyed on every return.
+struct HipEvents {
+  std::vector<hipEvent_t> ev;
+  HipEvents() = default;
+  HipEvents(const HipEvents&) = delete;
+  HipEvents& operator=(const HipEvents&) = delete;
+  ~HipEvents() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  int create(size_t n) {
+    ev.assign(n, nullptr);
+    for (auto& e : ev) XF_HIP(hipEventCreate(&e));
+    return XF_OK;
+  }
+  hipEvent_t& operator[](size_t q) { return ev[q]; }
+};
+
+// Ends a stream capture started by the caller; on a failed capture (status
+// st of the captured launches, or the end itself) the partial graph is
+// destroyed and the failure returned, else *out holds the graph.
+int end_capture(hipStream_t stream, int st, hipGraph_t* out) {
+  hipGraph_t g = nullptr;
+  const hipError_t ce = hipStreamEndCapture(stream, &g);
+  if (st != XF_OK || ce != hipSuccess) {
+    if (ce == hipSuccess && g) (void)hipGraphDestroy(g);
+    if (st != XF_OK) return st;
+    return fail(XF_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+  }
+  *out = g;
+  return XF_OK;
+}
 
 bool empty_task(const xf_task& t) {
   return !(t.x.start < t.x.end && t.y.start < t.y.end && t.z.start < t.z.end);
@@ -228,6 +286,8 @@ struct Engine {
   int validate();
   int build_tfsf_jobs();
   int check_overlaps();
+  int check_ade_pml() const;
+  bool dispersive(int64_t i, int64_t j, int64_t k) const;
   int upload();
   int pack_fields();
   int unpack_fields();
@@ -284,6 +344,16 @@ int Engine::validate() {
   if (nx < 1 || ny < 1 || nz < 1) return fail(XF_ERR_INVALID, "empty grid");
   if (dim == 2 && nz != 1) return fail(XF_ERR_INVALID, "a 2-D scene has nz = 1");
   if (nx > (1 << 20) || ny > (1 << 20) || nz > (1 << 20)) return fail(XF_ERR_INVALID, "grid too large");
+  {
+    // one field component on the padded node grid at 8 bytes per value
+    // (rows padded to 128 bytes); (2^20 + 1)^3 * 8 does not fit in int64
+    const unsigned __int128 per_field =
+        static_cast<unsigned __int128>(nx + 1) * static_cast<unsigned __int128>(ny + 1) *
+        static_cast<unsigned __int128>(nz + 1 + 16) * 8u;
+    if (per_field > (static_cast<unsigned __int128>(1) << 50))
+      return fail(XF_ERR_NOMEM, "packed grid too large: " + std::to_string(static_cast<double>(per_field)) +
+                                    " bytes per field component (limit 2^50)");
+  }
   if (!(sc->dt > 0)) return fail(XF_ERR_INVALID, "dt must be positive");
   nt = sc->start_step + sc->n_steps;
   cur_step = sc->current_step;
@@ -703,6 +773,7 @@ int Engine::build_tfsf_jobs() {
 // 124-160).  Both only add to a field, so the result is the same unless one
 // cell receives both, which is rejected here.
 int Engine::check_overlaps() {
+  XF_TRY(check_ade_pml());
   if (!sc->tfsf) return XF_OK;
   for (int pass = 0; pass < 2; ++pass) {
     const bool is_e = pass == 1;
@@ -732,6 +803,39 @@ int Engine::check_overlaps() {
   return XF_OK;
 }
 
+// A dispersive cell's E components take the ADE update (after the E pass)
+// and then, in the reference, the CPML correction (domain_hd.cu:35-87:
+// updateE, then correctE).  The engine applies the CPML E corrections inside
+// the Yee kernels, before the ADE kernel runs, so a dispersive cell inside a
+// CPML slab's corrected E box would be updated in the wrong order: rejected.
+int Engine::check_ade_pml() const {
+  if (!has_ade) return XF_OK;
+  for (int p = 0; p < sc->n_pml; ++p) {
+    const xf_pml& P = sc->pml[p];
+    if (empty_task(P.task)) continue;
+    const int ax = P.main_axis, a = (ax + 1) % 3, b = (ax + 2) % 3;
+    const int64_t cs = task_lo(P.task, ax), ce = task_hi(P.task, ax);
+    const int64_t off = cs == 0 ? 1 : 0;
+    int64_t lo[3], hi[3];  // E nodes the face corrects, half-open
+    lo[ax] = cs + off;
+    hi[ax] = ce + off;
+    lo[a] = task_lo(P.task, a);
+    hi[a] = task_hi(P.task, a) + 1;
+    lo[b] = task_lo(P.task, b);
+    hi[b] = task_hi(P.task, b) + 1;
+    const int64_t n3[3] = {nx, ny, nz};
+    for (int d = 0; d < 3; ++d) hi[d] = std::min(hi[d], n3[d]);  // dispersive cells have i < nx, ...
+    for (int64_t i = lo[0]; i < hi[0]; ++i)
+      for (int64_t j = lo[1]; j < hi[1]; ++j)
+        for (int64_t k = lo[2]; k < hi[2]; ++k)
+          if (dispersive(i, j, k))
+            return fail(XF_ERR_UNSUPPORTED, "dispersive (ADE) cell (" + std::to_string(i) + ", " + std::to_string(j) +
+                                                ", " + std::to_string(k) +
+                                                ") inside a CPML slab; keep dispersive media out of the PML");
+  }
+  return XF_OK;
+}
+
 double Engine::bytes_per_cell_step() const {
   // H pass: read H3 + E3, write H3; E pass: read E3 + H3, write E3, plus one
   // coefficient index per pass.
@@ -753,8 +857,16 @@ void host_parallel_for(int64_t n, F fn) {
   }
   std::vector<std::thread> pool;
   pool.reserve(nth);
-  for (int64_t t = 0; t < nth; ++t)
-    pool.emplace_back([&, t] { fn(n * t / nth, n * (t + 1) / nth); });
+  int64_t started = 0;
+  try {
+    for (; started < nth; ++started) {
+      const int64_t t = started;
+      pool.emplace_back([&, t] { fn(n * t / nth, n * (t + 1) / nth); });
+    }
+  } catch (const std::system_error&) {
+    // a thread could not start: its range and the rest run on this thread
+  }
+  for (int64_t t = started; t < nth; ++t) fn(n * t / nth, n * (t + 1) / nth);
   for (auto& th : pool) th.join();
 }
 }  // namespace
@@ -860,6 +972,25 @@ uint64_t bits(double v) {
 }
 }  // namespace
 
+// A cell is dispersive when any of its ADE coefficients is nonzero; the
+// E pass leaves such cells to the ADE kernel.
+bool Engine::dispersive(int64_t i, int64_t j, int64_t k) const {
+  const xf_ade& A = *sc->ade;
+  const int64_t np = A.num_pole;
+  const int64_t l3 = (i * (ny + 1) + j) * (nz + 1) + k;
+  if (A.coeff_e_j_sum.data[l3] != 0.0) return true;
+  if (A.kind == XF_ADE_MLORENTZ && A.coeff_e_e_p.data[l3] != 0.0) return true;
+  for (int64_t p = 0; p < np; ++p) {
+    const int64_t l4 = l3 * np + p;
+    if (A.coeff_j_j.data[l4] != 0.0 || A.coeff_j_e.data[l4] != 0.0 || A.coeff_j_sum_j.data[l4] != 0.0) return true;
+    if (A.kind == XF_ADE_MLORENTZ &&
+        (A.coeff_j_j_p.data[l4] != 0.0 || A.coeff_j_e_n.data[l4] != 0.0 || A.coeff_j_e_p.data[l4] != 0.0 ||
+         A.coeff_j_sum_j_p.data[l4] != 0.0))
+      return true;
+  }
+  return false;
+}
+
 // Per-cell coefficient indices.  Dense input: the 9 coefficients of each
 // pass at (i, j, k) (each from its own field-shaped array) are deduplicated
 // bit-exactly into a table.  Compact input: the material index is the id.
@@ -871,25 +1002,10 @@ int Engine::build_ids() {
   // dispersive cells
   std::vector<unsigned char> disp;
   if (has_ade) {
-    const xf_ade& A = *sc->ade;
     disp.assign(nx * ny * nz, 0);
-    const int64_t np = A.num_pole;
     for (int64_t i = 0; i < nx; ++i)
       for (int64_t j = 0; j < ny; ++j)
-        for (int64_t k = 0; k < nz; ++k) {
-          const int64_t l3 = (i * (ny + 1) + j) * (nz + 1) + k;
-          bool d = A.coeff_e_j_sum.data[l3] != 0.0;
-          if (A.kind == XF_ADE_MLORENTZ && A.coeff_e_e_p.data[l3] != 0.0) d = true;
-          for (int64_t p = 0; p < np && !d; ++p) {
-            const int64_t l4 = l3 * np + p;
-            if (A.coeff_j_j.data[l4] != 0.0 || A.coeff_j_e.data[l4] != 0.0 || A.coeff_j_sum_j.data[l4] != 0.0) d = true;
-            if (A.kind == XF_ADE_MLORENTZ &&
-                (A.coeff_j_j_p.data[l4] != 0.0 || A.coeff_j_e_n.data[l4] != 0.0 ||
-                 A.coeff_j_e_p.data[l4] != 0.0 || A.coeff_j_sum_j_p.data[l4] != 0.0))
-              d = true;
-          }
-          disp[(i * ny + j) * nz + k] = d ? 1 : 0;
-        }
+        for (int64_t k = 0; k < nz; ++k) disp[(i * ny + j) * nz + k] = dispersive(i, j, k) ? 1 : 0;
   }
   if (sc->coeff[0].data == nullptr) {
     const int nm = sc->n_material;
@@ -1447,6 +1563,10 @@ int Engine::upload() {
 // blocks and keeps far-apart planes open at once), while the carried plane
 // still saves a neighbour re-read every step of the march.
 void Engine::set_tiling(int ty, int chunks_x, int flags) {
+  // kept in the plan, so a later upload (which re-applies the plan) keeps it
+  plan.tile_y = ty;
+  plan.chunks_x = chunks_x;
+  plan.tiling_flags = flags;
   // 64 x tile_y threads must fit the kernels' __launch_bounds__(256)
   tile_y = ty > 0 ? std::min(ty, 4) : 1;
   if (flags & XF_TILE_SET) {
@@ -2047,14 +2167,10 @@ int Engine::run(int64_t n) {
   XF_HIP(hipSetDevice(device));
   const bool timing_on = plan.timing != 0;
   const bool graph_on = plan.use_graph != 0;
-  std::vector<hipEvent_t> ev;
-  hipEvent_t run0, run1;
-  XF_HIP(hipEventCreate(&run0));
-  XF_HIP(hipEventCreate(&run1));
-  if (timing_on) {
-    ev.resize(5 * n);
-    for (auto& e : ev) XF_HIP(hipEventCreate(&e));
-  }
+  HipEvents ev, runs;
+  XF_TRY(runs.create(2));
+  hipEvent_t run0 = runs[0], run1 = runs[1];
+  if (timing_on) XF_TRY(ev.create(5 * n));
   XF_HIP(hipEventRecord(run0, stream));
   int status = XF_OK;
   auto step = [&](int64_t s) -> int {
@@ -2093,11 +2209,7 @@ int Engine::run(int64_t n) {
         default:
           st = launch_step<__hip_bfloat16>(0, true, false, nullptr);
       }
-      hipGraph_t g;
-      hipError_t ce = hipStreamEndCapture(stream, &g);
-      if (st != XF_OK) return st;
-      if (ce != hipSuccess) return fail(XF_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-      graph = g;
+      XF_TRY(end_capture(stream, st, &graph));
       XF_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
       graph_ready = true;
     }
@@ -2132,10 +2244,7 @@ int Engine::run(int64_t n) {
     }
     timing.yee_h_launches += n;
     timing.yee_e_launches += n;
-    for (auto& e : ev) (void)hipEventDestroy(e);
   }
-  (void)hipEventDestroy(run0);
-  (void)hipEventDestroy(run1);
   timing.bytes_per_cell_step = bytes_per_cell_step();
   cur_step += n;
   if (has_lb && lbd.mode == XF_LB_RECORD) lb_rec_count = lb_recorded_after(cur_step);
@@ -2153,146 +2262,168 @@ extern "C" {
 const char* xf_last_error(void) { return xf::g_err.c_str(); }
 
 int xf_device_count(int* count) {
-  if (!count) return xf::fail(XF_ERR_INVALID, "null count");
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
-  *count = n;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!count) return xf::fail(XF_ERR_INVALID, "null count");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return XF_OK;
+  });
 }
 
 int xf_create(xf_scene* scene, const xf_plan* plan, xf_handle** out) {
-  xf::g_err.clear();
-  if (!scene || !out) return xf::fail(XF_ERR_INVALID, "null scene or out");
-  *out = nullptr;
-  auto h = new (std::nothrow) xf_handle;
-  if (!h) return xf::fail(XF_ERR_NOMEM, "out of host memory");
-  h->eng.sc = scene;
-  if (plan) h->eng.plan = *plan;
-  if (h->eng.plan.precision < XF_FP64 || h->eng.plan.precision > XF_FP16_STORE) {
-    delete h;
-    return xf::fail(XF_ERR_INVALID, "unknown precision");
-  }
-  h->eng.device = h->eng.plan.device;
-  int st = h->eng.validate();
-  if (st == XF_OK) st = h->eng.check_overlaps();
-  if (st != XF_OK) {
-    delete h;
-    return st;
-  }
-  *out = h;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!scene || !out) return xf::fail(XF_ERR_INVALID, "null scene or out");
+    *out = nullptr;
+    std::unique_ptr<xf_handle> h(new (std::nothrow) xf_handle);
+    if (!h) return xf::fail(XF_ERR_NOMEM, "out of host memory");
+    h->eng.sc = scene;
+    if (plan) h->eng.plan = *plan;
+    if (h->eng.plan.precision < XF_FP64 || h->eng.plan.precision > XF_FP16_STORE)
+      return xf::fail(XF_ERR_INVALID, "unknown precision");
+    h->eng.device = h->eng.plan.device;
+    int st = h->eng.validate();
+    if (st == XF_OK) st = h->eng.check_overlaps();
+    if (st != XF_OK) return st;
+    *out = h.release();
+    return XF_OK;
+  });
 }
 
 int xf_upload(xf_handle* h) {
-  xf::g_err.clear();
-  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return xf::fail(XF_ERR_HIP, "no HIP device");
-  if (h->eng.device < 0 || h->eng.device >= n) return xf::fail(XF_ERR_INVALID, "bad device ordinal");
-  const int st = h->eng.upload();
-  if (st != XF_OK) h->eng.release();
-  return st;
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return xf::fail(XF_ERR_HIP, "no HIP device");
+    if (h->eng.device < 0 || h->eng.device >= n) return xf::fail(XF_ERR_INVALID, "bad device ordinal");
+    // a failed upload (status or exception) leaves nothing on the device
+    const int st = xf::guarded([&] { return h->eng.upload(); });
+    if (st != XF_OK) h->eng.release();
+    return st;
+  });
 }
 
 int xf_run(xf_handle* h, int64_t n_steps) {
-  xf::g_err.clear();
-  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
-  return h->eng.run(n_steps);
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+    return h->eng.run(n_steps);
+  });
 }
 
 int xf_download(xf_handle* h) {
-  xf::g_err.clear();
-  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
-  return h->eng.download();
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+    return h->eng.download();
+  });
 }
 
 int xf_destroy(xf_handle* h) {
-  delete h;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    delete h;
+    return XF_OK;
+  });
 }
 
 int xf_current_step(const xf_handle* h, int64_t* step) {
-  if (!h || !step) return xf::fail(XF_ERR_INVALID, "null argument");
-  *step = h->eng.cur_step;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!h || !step) return xf::fail(XF_ERR_INVALID, "null argument");
+    *step = h->eng.cur_step;
+    return XF_OK;
+  });
 }
 
 int xf_get_timing(const xf_handle* h, xf_timing* out) {
-  if (!h || !out) return xf::fail(XF_ERR_INVALID, "null argument");
-  *out = h->eng.timing;
-  out->bytes_per_cell_step = h->eng.bytes_per_cell_step();
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!h || !out) return xf::fail(XF_ERR_INVALID, "null argument");
+    *out = h->eng.timing;
+    out->bytes_per_cell_step = h->eng.bytes_per_cell_step();
+    return XF_OK;
+  });
 }
 
 int xf_reset_timing(xf_handle* h) {
-  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
-  h->eng.timing = xf_timing{};
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+    h->eng.timing = xf_timing{};
+    return XF_OK;
+  });
 }
 
 int xf_stream_bench(int64_t bytes, int32_t reps, int32_t device, int32_t variant, int32_t blocks,
                     double* gbps) {
-  xf::g_err.clear();
-  if (!gbps || bytes < 16 || reps < 1) return xf::fail(XF_ERR_INVALID, "bad stream bench arguments");
-  if (variant < 0 || variant > 6) return xf::fail(XF_ERR_INVALID, "bad stream bench variant");
-  if (hipSetDevice(device) != hipSuccess) return xf::fail(XF_ERR_HIP, "hipSetDevice failed");
-  const int64_t n = bytes / 16;
-  xf::xf_d2 *a = nullptr, *b = nullptr;
-  if (hipMalloc(&a, n * 16) != hipSuccess) return xf::fail(XF_ERR_NOMEM, "stream bench allocation failed");
-  if (hipMalloc(&b, n * 16) != hipSuccess) {
-    (void)hipFree(a);
-    return xf::fail(XF_ERR_NOMEM, "stream bench allocation failed");
-  }
-  hipEvent_t e0, e1;
-  (void)hipMemset(a, 0, n * 16);
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
-  const dim3 block(256), grid(blocks > 0 ? blocks : 256 * 8 * 4);
-  auto launch = [&]() {
-    switch (variant) {
-      case 0: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<1, false>), grid, block, 0, 0, a, b, n); break;
-      case 1: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<4, false>), grid, block, 0, 0, a, b, n); break;
-      case 2: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<4, true>), grid, block, 0, 0, a, b, n); break;
-      case 3: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<8, false>), grid, block, 0, 0, a, b, n); break;
-      case 4: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<8, true>), grid, block, 0, 0, a, b, n); break;
-      case 5: hipLaunchKernelGGL((xf::xf_stream_copy1_kernel<false>), dim3((unsigned)((n + 255) / 256)), block, 0, 0, a, b, n); break;
-      default: hipLaunchKernelGGL((xf::xf_stream_copy1_kernel<true>), dim3((unsigned)((n + 255) / 256)), block, 0, 0, a, b, n); break;
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!gbps || bytes < 16 || reps < 1) return xf::fail(XF_ERR_INVALID, "bad stream bench arguments");
+    if (variant < 0 || variant > 6) return xf::fail(XF_ERR_INVALID, "bad stream bench variant");
+    if (hipSetDevice(device) != hipSuccess) return xf::fail(XF_ERR_HIP, "hipSetDevice failed");
+    const int64_t n = bytes / 16;
+    xf::xf_d2 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * 16) != hipSuccess) return xf::fail(XF_ERR_NOMEM, "stream bench allocation failed");
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+      (void)hipFree(a);
+      return xf::fail(XF_ERR_NOMEM, "stream bench allocation failed");
     }
-  };
-  launch();  // warm-up
-  (void)hipEventRecord(e0, 0);
-  for (int r = 0; r < reps; ++r) launch();
-  (void)hipEventRecord(e1, 0);
-  hipError_t err = hipEventSynchronize(e1);
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(a);
-  (void)hipFree(b);
-  if (err != hipSuccess) return xf::fail(XF_ERR_HIP, hipGetErrorString(err));
-  *gbps = 2.0 * static_cast<double>(n) * 16.0 * reps / (ms * 1e-3) / 1e9;
-  return XF_OK;
+    hipEvent_t e0, e1;
+    (void)hipMemset(a, 0, n * 16);
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const dim3 block(256), grid(blocks > 0 ? blocks : 256 * 8 * 4);
+    auto launch = [&]() {
+      switch (variant) {
+        case 0: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<1, false>), grid, block, 0, 0, a, b, n); break;
+        case 1: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<4, false>), grid, block, 0, 0, a, b, n); break;
+        case 2: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<4, true>), grid, block, 0, 0, a, b, n); break;
+        case 3: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<8, false>), grid, block, 0, 0, a, b, n); break;
+        case 4: hipLaunchKernelGGL((xf::xf_stream_copy_kernel<8, true>), grid, block, 0, 0, a, b, n); break;
+        case 5: hipLaunchKernelGGL((xf::xf_stream_copy1_kernel<false>), dim3((unsigned)((n + 255) / 256)), block, 0, 0, a, b, n); break;
+        default: hipLaunchKernelGGL((xf::xf_stream_copy1_kernel<true>), dim3((unsigned)((n + 255) / 256)), block, 0, 0, a, b, n); break;
+      }
+    };
+    launch();  // warm-up
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r) launch();
+    (void)hipEventRecord(e1, 0);
+    hipError_t err = hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (err != hipSuccess) return xf::fail(XF_ERR_HIP, hipGetErrorString(err));
+    *gbps = 2.0 * static_cast<double>(n) * 16.0 * reps / (ms * 1e-3) / 1e9;
+    return XF_OK;
+  });
 }
 
 int xf_set_tiling(xf_handle* h, int32_t tile_y, int32_t chunks_x, int32_t flags) {
-  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
-  // the Yee kernels are compiled for at most 256 threads per block (64 x 4)
-  if (tile_y < 0 || tile_y > 4 || chunks_x < 0) return xf::fail(XF_ERR_INVALID, "bad tiling (tile_y 0..4)");
-  h->eng.set_tiling(tile_y, chunks_x, flags);
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+    // the Yee kernels are compiled for at most 256 threads per block (64 x 4)
+    if (tile_y < 0 || tile_y > 4 || chunks_x < 0) return xf::fail(XF_ERR_INVALID, "bad tiling (tile_y 0..4)");
+    h->eng.set_tiling(tile_y, chunks_x, flags);
+    return XF_OK;
+  });
 }
 
 int xf_set_timing(xf_handle* h, int32_t on) {
-  if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
-  h->eng.plan.timing = on ? 1 : 0;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!h) return xf::fail(XF_ERR_INVALID, "null handle");
+    h->eng.plan.timing = on ? 1 : 0;
+    return XF_OK;
+  });
 }
 
 int xf_describe(const xf_handle* h, char* buf, size_t len) {
-  if (!h || !buf || len == 0) return xf::fail(XF_ERR_INVALID, "null argument");
-  std::snprintf(buf, len, "%s", h->eng.layout_desc.c_str());
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!h || !buf || len == 0) return xf::fail(XF_ERR_INVALID, "null argument");
+    std::snprintf(buf, len, "%s", h->eng.layout_desc.c_str());
+    return XF_OK;
+  });
 }
 
 }  // extern "C"

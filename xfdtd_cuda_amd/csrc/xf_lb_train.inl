@@ -372,64 +372,77 @@ extern "C" {
 
 int xf_lb_trainer_create(const float* samples, int64_t n, double mur_k, int32_t device,
                          xf_lb_trainer** out) {
-  xf::g_err.clear();
-  if (!samples || !out || n < 1) return xf::fail(XF_ERR_INVALID, "no samples");
-  if (n > (int64_t)UINT32_MAX) return xf::fail(XF_ERR_UNSUPPORTED, "too many samples (32-bit indices)");
-  *out = nullptr;
-  int nd = 0;
-  if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) return xf::fail(XF_ERR_HIP, "no HIP device");
-  if (device < 0 || device >= nd) return xf::fail(XF_ERR_INVALID, "bad device ordinal");
-  auto h = new (std::nothrow) xf_lb_trainer;
-  if (!h) return xf::fail(XF_ERR_NOMEM, "out of host memory");
-  const int st = h->t.create(samples, n, mur_k, device);
-  if (st != XF_OK) {
-    delete h;
-    return st;
-  }
-  *out = h;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!samples || !out || n < 1) return xf::fail(XF_ERR_INVALID, "no samples");
+    if (n > (int64_t)UINT32_MAX) return xf::fail(XF_ERR_UNSUPPORTED, "too many samples (32-bit indices)");
+    *out = nullptr;
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) return xf::fail(XF_ERR_HIP, "no HIP device");
+    if (device < 0 || device >= nd) return xf::fail(XF_ERR_INVALID, "bad device ordinal");
+    std::unique_ptr<xf_lb_trainer> h(new (std::nothrow) xf_lb_trainer);
+    if (!h) return xf::fail(XF_ERR_NOMEM, "out of host memory");
+    const int st = h->t.create(samples, n, mur_k, device);
+    if (st != XF_OK) return st;
+    *out = h.release();
+    return XF_OK;
+  });
 }
 
 int xf_lb_trainer_set_weights(xf_lb_trainer* h, const float* w) {
-  xf::g_err.clear();
-  if (!h || !w) return xf::fail(XF_ERR_INVALID, "null argument");
-  return h->t.set_weights(w);
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!h || !w) return xf::fail(XF_ERR_INVALID, "null argument");
+    return h->t.set_weights(w);
+  });
 }
 
 int xf_lb_trainer_get_weights(xf_lb_trainer* h, float* w32, uint16_t* w16) {
-  xf::g_err.clear();
-  if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
-  return h->t.get_weights(w32, w16);
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
+    return h->t.get_weights(w32, w16);
+  });
 }
 
 int xf_lb_trainer_grad(xf_lb_trainer* h, int64_t first, int64_t count, double* loss, float* grad) {
-  xf::g_err.clear();
-  if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
-  return h->t.grad(first, count, loss, grad);
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
+    return h->t.grad(first, count, loss, grad);
+  });
 }
 
 int xf_lb_trainer_train(xf_lb_trainer* h, int32_t steps, int32_t batch, float lr, uint64_t seed,
                         double* last_loss) {
-  xf::g_err.clear();
-  if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
-  return h->t.train(steps, batch, lr, seed, last_loss);
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
+    return h->t.train(steps, batch, lr, seed, last_loss);
+  });
 }
 
 int xf_lb_trainer_samples(const xf_lb_trainer* h, int64_t* n) {
-  if (!h || !n) return xf::fail(XF_ERR_INVALID, "null argument");
-  *n = h->t.n;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    if (!h || !n) return xf::fail(XF_ERR_INVALID, "null argument");
+    *n = h->t.n;
+    return XF_OK;
+  });
 }
 
 int xf_lb_trainer_set_weighting(xf_lb_trainer* h, int32_t power) {
-  xf::g_err.clear();
-  if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
-  return h->t.set_weighting(power);
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
+    return h->t.set_weighting(power);
+  });
 }
 
 int xf_lb_trainer_destroy(xf_lb_trainer* h) {
-  delete h;
-  return XF_OK;
+  return xf::guarded([&]() -> int {
+    delete h;
+    return XF_OK;
+  });
 }
 
 }  // extern "C"
