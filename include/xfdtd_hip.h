@@ -457,6 +457,24 @@ int xf_lb_trainer_samples(const xf_lb_trainer* t, int64_t* n);
 int xf_lb_trainer_set_weighting(xf_lb_trainer* t, int32_t power);
 int xf_lb_trainer_destroy(xf_lb_trainer* t);
 
+/* The gradient of xf_lb_trainer_grad without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).
+ * raw: n samples of 13 floats, device memory; w32: XF_LB_NPARAM fp32 master
+ * weights, device memory (the forward pass uses their fp16 rounding);
+ * loss: one double and grad: XF_LB_NPARAM floats, device memory.  Equal bit
+ * for bit to a trainer created from the same samples, with
+ * set_weighting(power), set_weights(w32) and grad(first, count).  The work
+ * is enqueued on `stream` (a hipStream_t of `device`; NULL = the null
+ * stream) and the call returns without waiting.  Its working memory is a
+ * static array of about 6 MB per device (up to 1024 blocks of partial
+ * sums); a device that would need more blocks gets XF_ERR_UNSUPPORTED.
+ * Calls that overlap on one device share that array and race: serialise
+ * them (one stream, or events). */
+int xf_lb_grad_noalloc(const float* raw, int64_t n, double mur_k, int32_t power, const float* w32,
+                       int64_t first, int64_t count, double* loss, float* grad, int32_t device,
+                       void* stream);
+
 #ifdef __cplusplus
 }
 #endif

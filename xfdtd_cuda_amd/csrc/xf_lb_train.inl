@@ -31,12 +31,12 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
-__global__ void __launch_bounds__(256)
-    k_lb_prep(const float* __restrict__ raw, int64_t n, float mur_k, float* __restrict__ xs,
-              float* __restrict__ r, int* __restrict__ ex) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const float* x = raw + k * 13;
+// The network's inputs and target of one raw sample (13 floats): x^ = x / s,
+// r^ = (y - mur) / s with s = 2^e the power of two above the largest |x|
+// (frexp), exactly as the predictor scales them; e = INT_MIN / 4 marks an
+// all-zero sample.
+__device__ __forceinline__ void lb_prep_one(const float* __restrict__ x, float mur_k, float* xs, float& r,
+                                            int& ex) {
   float m = 0.0f;
   for (int i = 0; i < XF_LB_IN; ++i) {
     const float a = fabsf(x[i]);
@@ -45,18 +45,38 @@ __global__ void __launch_bounds__(256)
   const float mur = x[4] + mur_k * (x[0] - x[3]);
   int e = 0;
   if (m > 0.0f) (void)frexpf(m, &e);
-  for (int i = 0; i < XF_LB_IN; ++i) xs[k * XF_LB_IN + i] = m > 0.0f ? ldexpf(x[i], -e) : 0.0f;
-  r[k] = m > 0.0f ? ldexpf(x[12] - mur, -e) : 0.0f;
-  ex[k] = m > 0.0f ? e : INT_MIN / 4;
+  for (int i = 0; i < XF_LB_IN; ++i) xs[i] = m > 0.0f ? ldexpf(x[i], -e) : 0.0f;
+  r = m > 0.0f ? ldexpf(x[12] - mur, -e) : 0.0f;
+  ex = m > 0.0f ? e : INT_MIN / 4;
 }
 
-// Loss weight (s / s_max)^power of each sample, s = 2^e its input scale.
+// Loss weight (s / s_max)^power of a sample whose input scale is s = 2^ex
+// (0 for an all-zero sample; the exponent difference is formed in 64 bits).
+__device__ __forceinline__ float lb_weight(int ex, int e_max, int power) {
+  const int64_t d = (int64_t)power * ((int64_t)ex - e_max);
+  return d < -126 ? 0.0f : ldexpf(1.0f, (int)d);
+}
+
+__global__ void __launch_bounds__(256)
+    k_lb_prep(const float* __restrict__ raw, int64_t n, float mur_k, float* __restrict__ xs,
+              float* __restrict__ r, int* __restrict__ ex) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  float x[XF_LB_IN];
+  float rk;
+  int e;
+  lb_prep_one(raw + k * 13, mur_k, x, rk, e);
+  for (int i = 0; i < XF_LB_IN; ++i) xs[k * XF_LB_IN + i] = x[i];
+  r[k] = rk;
+  ex[k] = e;
+}
+
+// Loss weight of each sample.
 __global__ void k_lb_weight(const int* __restrict__ ex, int64_t n, int e_max, int power,
                             float* __restrict__ lw) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const int d = power * (ex[k] - e_max);
-  lw[k] = d < -126 ? 0.0f : ldexpf(1.0f, d);
+  lw[k] = lb_weight(ex[k], e_max, power);
 }
 
 // Counter-based minibatch indices (splitmix64 of seed, step, k).
@@ -71,16 +91,50 @@ __global__ void k_lb_sample(uint32_t* __restrict__ idx, int64_t count, int64_t n
   idx[k] = (uint32_t)(z % (uint64_t)n);
 }
 
+// Where the gradient kernels find a sample's scaled inputs, target and loss
+// weight: the trainer's prepared arrays ...
+struct LbPrepared {
+  const float* __restrict__ xs;
+  const float* __restrict__ r;
+  const float* __restrict__ lw;
+  __device__ __forceinline__ void get(int64_t s, float* x, float& target, float& w) const {
+#pragma unroll
+    for (int i = 0; i < XF_LB_IN; ++i) x[i] = xs[s * XF_LB_IN + i];
+    target = r[s];
+    w = lw[s];
+  }
+};
+
+// ... or the raw samples, prepared on the fly by the same arithmetic
+// (lb_prep_one, lb_weight), with the largest |input| over all samples as
+// float bits at *absmax_bits.
+struct LbRaw {
+  const float* __restrict__ raw;
+  float mur_k;
+  int power;
+  const unsigned* __restrict__ absmax_bits;
+  __device__ __forceinline__ void get(int64_t s, float* x, float& target, float& w) const {
+    int e;
+    lb_prep_one(raw + s * 13, mur_k, x, target, e);
+    const float smax = __uint_as_float(*absmax_bits);
+    int e_max = 0;
+    if (smax > 0.0f) (void)frexpf(smax, &e_max);
+    w = lb_weight(e, e_max, power);
+  }
+};
+
 // Loss and gradient over `count` samples: sample k is idx[k] (or first + k
-// without an index list); the loss is sum d^2 * inv_count.
-__global__ void __launch_bounds__(256)
-    k_lb_grad(const float* __restrict__ xs_all, const float* __restrict__ r_all,
-              const float* __restrict__ lw_all, const uint32_t* __restrict__ idx, int64_t first,
-              int64_t count, float inv_count, const float* __restrict__ w16,
-              float* __restrict__ partial) {
+// without an index list); the loss is sum d^2 * inv_count.  The weights are
+// the fp16 rounding of w (RoundW: w is fp32 master weights; otherwise w
+// already holds the rounded values).
+template <typename Src, bool RoundW>
+__device__ __forceinline__ void lb_grad_body(const Src& src, const uint32_t* __restrict__ idx, int64_t first,
+                                             int64_t count, float inv_count, const float* __restrict__ w_in,
+                                             float* __restrict__ partial) {
   __shared__ float w[XF_LB_NPARAM];
   __shared__ float acc[kLbWaves][kLbAcc];
-  for (int i = threadIdx.x; i < XF_LB_NPARAM; i += blockDim.x) w[i] = w16[i];
+  for (int i = threadIdx.x; i < XF_LB_NPARAM; i += blockDim.x)
+    w[i] = RoundW ? __half2float(__float2half_rn(w_in[i])) : w_in[i];
   for (int i = threadIdx.x; i < kLbWaves * kLbAcc; i += blockDim.x) (&acc[0][0])[i] = 0.0f;
   __syncthreads();
   const float* W1 = w;
@@ -94,12 +148,14 @@ __global__ void __launch_bounds__(256)
   for (int64_t base = ((int64_t)blockIdx.x * kLbWaves + wave) * 64; base < count; base += stride) {
     const int64_t k = base + lane;
     const bool valid = k < count;
-    const int64_t sidx = valid ? (idx ? (int64_t)idx[k] : first + k) : 0;
     float x[XF_LB_IN];
+    float target = 0.0f, lw = 0.0f;
+    if (valid) {
+      src.get(idx ? (int64_t)idx[k] : first + k, x, target, lw);
+    } else {
 #pragma unroll
-    for (int i = 0; i < XF_LB_IN; ++i) x[i] = valid ? xs_all[sidx * XF_LB_IN + i] : 0.0f;
-    const float target = valid ? r_all[sidx] : 0.0f;
-    const float lw = valid ? lw_all[sidx] : 0.0f;
+      for (int i = 0; i < XF_LB_IN; ++i) x[i] = 0.0f;
+    }
     float h1[XF_LB_HID], h2[XF_LB_HID];
 #pragma unroll
     for (int o = 0; o < XF_LB_HID; ++o) {
@@ -173,6 +229,49 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+__global__ void __launch_bounds__(256)
+    k_lb_grad(const float* __restrict__ xs_all, const float* __restrict__ r_all,
+              const float* __restrict__ lw_all, const uint32_t* __restrict__ idx, int64_t first,
+              int64_t count, float inv_count, const float* __restrict__ w16,
+              float* __restrict__ partial) {
+  lb_grad_body<LbPrepared, false>(LbPrepared{xs_all, r_all, lw_all}, idx, first, count, inv_count, w16,
+                                  partial);
+}
+
+__global__ void __launch_bounds__(256)
+    k_lb_grad_raw(const float* __restrict__ raw, float mur_k, int power,
+                  const unsigned* __restrict__ absmax_bits, int64_t first, int64_t count, float inv_count,
+                  const float* __restrict__ w32, float* __restrict__ partial) {
+  lb_grad_body<LbRaw, true>(LbRaw{raw, mur_k, power, absmax_bits}, nullptr, first, count, inv_count, w32,
+                            partial);
+}
+
+// Largest |input| over the 12 inputs of all n raw samples, as float bits
+// (non-negative floats order like their bit patterns), folded into *bits,
+// which the caller cleared.
+__global__ void __launch_bounds__(256)
+    k_lb_absmax(const float* __restrict__ raw, int64_t n, unsigned* __restrict__ bits) {
+  __shared__ unsigned wmax[4];
+  unsigned m = 0u;
+  const int64_t total = n * 13;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * blockDim.x)
+    if (q % 13 != 12) m = max(m, __float_as_uint(fabsf(raw[q])));
+#pragma unroll
+  for (int s = 32; s >= 1; s >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, s, 64));
+  if (threadIdx.x % 64 == 0) wmax[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned b = wmax[0];
+    for (int w = 1; w < (int)(blockDim.x / 64); ++w) b = max(b, wmax[w]);
+    atomicMax(bits, b);
+  }
+}
+
+__global__ void k_lb_clear_u32(unsigned* __restrict__ p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
+}
+
 __global__ void k_lb_reduce(const float* __restrict__ partial, int nblocks, float* __restrict__ grad,
                             double* __restrict__ loss) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -210,6 +309,12 @@ __global__ void k_lb_round(const float* __restrict__ w, float* __restrict__ w16,
   w16[p] = __half2float(h);
   if (bits) bits[p] = __half_as_ushort(h);
 }
+
+// Blocks of the gradient kernel on a device with `cus` compute units (the
+// trainer and the no-allocation entry point use the same count, so their
+// per-block partials, and hence their sums, are the same).
+constexpr int kLbMaxBlocks = 1024;
+inline int lb_grad_blocks(int cus) { return std::min(2 * cus, kLbMaxBlocks); }
 
 struct LbTrainer {
   int device{0};
@@ -285,7 +390,7 @@ struct LbTrainer {
     loss = static_cast<double*>(p);
     hipDeviceProp_t prop;
     XF_HIP(hipGetDeviceProperties(&prop, device));
-    nblocks = 2 * prop.multiProcessorCount;
+    nblocks = lb_grad_blocks(prop.multiProcessorCount);
     XF_TRY(alloc(&p, static_cast<size_t>(nblocks) * kLbAcc * sizeof(float)));
     partial = static_cast<float*>(p);
     XF_TRY(set_weighting(0));
@@ -362,6 +467,86 @@ struct LbTrainer {
   }
 };
 
+// ---- The no-allocation gradient entry point.  Its working memory is this
+// file-scope array, one copy per device: word 0 holds the largest |input|
+// (float bits); the per-block partials start at float 64.  Its device
+// address is looked up once per device with hipGetSymbolAddress and kept;
+// host code never passes the array's own name to a kernel.
+constexpr int64_t kLbScratchHead = 64;
+constexpr int64_t kLbScratchFloats = kLbScratchHead + (int64_t)kLbMaxBlocks * kLbAcc;
+static __device__ __align__(256) float scratch[kLbScratchFloats];
+
+constexpr int kLbMaxDevices = 64;
+struct LbScratchSlot {
+  std::once_flag once;
+  int status{XF_ERR_HIP};
+  std::string err;
+  float* ptr{nullptr};
+  int cus{0};
+};
+LbScratchSlot g_lb_scratch[kLbMaxDevices];
+
+// Set-up of one device's scratch (with that device current): its address
+// and the device's compute-unit count.
+int lb_scratch_setup(int device, float** ptr, int* cus) {
+  LbScratchSlot& S = g_lb_scratch[device];
+  std::call_once(S.once, [&]() {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+      S.err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return;
+    }
+    int n = 0;
+    e = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) {
+      S.err = std::string("hipDeviceGetAttribute: ") + hipGetErrorString(e);
+      return;
+    }
+    S.ptr = static_cast<float*>(p);
+    S.cus = n;
+    S.status = XF_OK;
+  });
+  if (S.status != XF_OK) return fail(S.status, S.err);
+  *ptr = S.ptr;
+  *cus = S.cus;
+  return XF_OK;
+}
+
+int lb_grad_noalloc(const float* raw, int64_t n, double mur_k, int power, const float* w32, int64_t first,
+                    int64_t count, double* loss, float* grad, int device, hipStream_t stream) {
+  if (!raw || !w32 || !loss || !grad) return fail(XF_ERR_INVALID, "null argument");
+  if (n < 1) return fail(XF_ERR_INVALID, "no samples");
+  if (n > (int64_t)UINT32_MAX) return fail(XF_ERR_UNSUPPORTED, "too many samples (32-bit indices)");
+  if (first < 0 || count < 1 || first + count > n) return fail(XF_ERR_INVALID, "sample range out of bounds");
+  if (power < 0 || power > 4) return fail(XF_ERR_INVALID, "weighting power must be 0..4");
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) return fail(XF_ERR_HIP, "no HIP device");
+  if (device < 0 || device >= nd) return fail(XF_ERR_INVALID, "bad device ordinal");
+  if (device >= kLbMaxDevices) return fail(XF_ERR_UNSUPPORTED, "device ordinal beyond the scratch table");
+  XF_HIP(hipSetDevice(device));
+  float* base = nullptr;
+  int cus = 0;
+  XF_TRY(lb_scratch_setup(device, &base, &cus));
+  const int nblocks = lb_grad_blocks(cus);
+  const int blocks = static_cast<int>(std::min<int64_t>(nblocks, (count + kLbWaves * 64 - 1) / (kLbWaves * 64)));
+  if ((int64_t)kLbScratchHead + (int64_t)blocks * kLbAcc > kLbScratchFloats)
+    return fail(XF_ERR_UNSUPPORTED, "gradient partials exceed the static scratch");
+  unsigned* absmax = reinterpret_cast<unsigned*>(base);
+  float* partial = base + kLbScratchHead;
+  hipLaunchKernelGGL(k_lb_clear_u32, dim3(1), dim3(64), 0, stream, absmax);
+  const int64_t total = n * 13;
+  const unsigned mblocks = static_cast<unsigned>(std::min<int64_t>(4 * cus, (total + 255) / 256));
+  hipLaunchKernelGGL(k_lb_absmax, dim3(mblocks), dim3(256), 0, stream, raw, n, absmax);
+  hipLaunchKernelGGL(k_lb_grad_raw, dim3(blocks), dim3(64 * kLbWaves), 0, stream, raw,
+                     static_cast<float>(mur_k), power, static_cast<const unsigned*>(absmax), first, count,
+                     1.0f / static_cast<float>(count), w32, partial);
+  hipLaunchKernelGGL(k_lb_reduce, dim3((kLbAcc + 255) / 256), dim3(256), 0, stream,
+                     static_cast<const float*>(partial), blocks, grad, loss);
+  XF_HIP(hipGetLastError());
+  return XF_OK;
+}
+
 }  // namespace xf
 
 struct xf_lb_trainer {
@@ -435,6 +620,16 @@ int xf_lb_trainer_set_weighting(xf_lb_trainer* h, int32_t power) {
     xf::g_err.clear();
     if (!h) return xf::fail(XF_ERR_INVALID, "null trainer");
     return h->t.set_weighting(power);
+  });
+}
+
+int xf_lb_grad_noalloc(const float* raw, int64_t n, double mur_k, int32_t power, const float* w32,
+                       int64_t first, int64_t count, double* loss, float* grad, int32_t device,
+                       void* stream) {
+  return xf::guarded([&]() -> int {
+    xf::g_err.clear();
+    return xf::lb_grad_noalloc(raw, n, mur_k, power, w32, first, count, loss, grad, device,
+                               static_cast<hipStream_t>(stream));
   });
 }
 
