@@ -93,6 +93,10 @@ def main(argv=None) -> int:
                     help="replay each step as a hipGraph (batch, precision, learned, scenes, "
                          "dispersive, cylinder2d); eager launches measured faster on MI355X "
                          "in all cases tried but the 536^3 sphere in FP32")
+    ap.add_argument("--coeff", default="table", choices=["table", "dense"],
+                    help="sphere workload: per-cell coefficient ids into a table (the compact "
+                         "layout, default) or the reference's 18 per-cell coefficient arrays "
+                         "streamed by the kernels (XF_COEFF_DENSE)")
     ap.add_argument("--td-boxes", type=int, default=3, help="NF2FF-TD boxes of nf2ff_td")
     ap.add_argument("--count", type=int, default=64, help="grids of the batch workload")
     args = ap.parse_args(argv)
@@ -125,12 +129,15 @@ def main(argv=None) -> int:
     K, W = args.steps, args.warmup
     total_steps = W + K + (0 if args.no_cpu else args.cpu_steps) + 1
     t_setup = time.perf_counter()
-    sim = scenes.large_sphere(n=args.n, pml=args.pml, steps=total_steps, layout="compact")
+    dense = args.coeff == "dense"
+    sim = scenes.large_sphere(n=args.n, pml=args.pml, steps=total_steps,
+                              layout="dense" if dense else "compact")
     (nx, ny, nz), dt, _ = sim.info()
     cells = nx * ny * nz
     t_up = time.perf_counter()
     build_s = t_up - t_setup
-    eng = xf.Engine(sim, precision=args.precision, timing=False).upload()
+    eng = xf.Engine(sim, precision=args.precision, timing=False,
+                    coeff="dense" if dense else "auto").upload()
     upload_s = time.perf_counter() - t_up
     if args.tile_y or args.chunks or args.pml_fused or args.k32 or args.no_nt:
         eng.set_tiling(args.tile_y or 1, args.chunks, xcd_remap=False, nontemporal=not args.no_nt,
@@ -160,7 +167,8 @@ def main(argv=None) -> int:
     sbytes = {"fp64": 8, "fp32": 4, "bf16": 2, "bf16c": 3, "fp16": 2}[args.precision]
     cbytes = 8 if args.precision == "fp64" else 4
     id_bytes = 1  # compact layout: uint8 material index (<= 256 materials)
-    per_cell_pass = 9 * sbytes + id_bytes
+    # dense: the uint8 id (dispersive flag) plus 9 coefficients per cell and pass
+    per_cell_pass = 9 * sbytes + id_bytes + (9 * cbytes if dense else 0)
     bytes_h = cells * per_cell_pass + pml_bytes(sim, "h", cbytes)
     bytes_e = cells * per_cell_pass + pml_bytes(sim, "e", cbytes)
     h_ms = tm["yee_h_ms"] / tm["yee_h_launches"]
@@ -173,7 +181,8 @@ def main(argv=None) -> int:
         kernel += "+k_pml_z"  # the pass: Yee kernel, then its z-face CPML launch
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{kernel.replace('+', '_')}_{args.precision}_{nx}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{kernel.replace('+', '_')}_{args.precision}_{nx}"
+                                           f"{'_dense' if dense else ''}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -210,7 +219,10 @@ def main(argv=None) -> int:
         "data": "synthetic",
         "config": {"workload": f"dielectric sphere {nx - 2 * args.pml}^3 + {args.pml}-cell CPML "
                                f"({nx}x{ny}x{nz} cells), TFSF + NF2FF 1 GHz, {args.precision}",
-                   "cells": cells, "layout": "compact (uint8 material index)",
+                   "cells": cells,
+                   "layout": ("dense (18 per-cell coefficient arrays, "
+                              f"{9 * cbytes} B per cell and pass)") if dense
+                   else "compact (uint8 material index)",
                    "wall_s": round(wall, 4),
                    "end_to_end_s": {"scene_build": round(build_s, 2), "upload": round(upload_s, 2),
                                     "steps_timed": round(wall, 3), "download": round(download_s, 2)}},

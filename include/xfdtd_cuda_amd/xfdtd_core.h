@@ -238,6 +238,11 @@ class Waveform {
 class Simulation;
 
 // ------------------------------------------------------------------- grid
+// The grid (grid_space.cuh:18-120): per axis the E nodes (cell corners,
+// n + 1), the H nodes (cell centres, n), the E sizes (cell edges, n) and
+// the H sizes (dual edges around each E node, n + 1; the two end values
+// are the end cells' sizes).  An axis is uniform (based size) or graded
+// (Simulation::setCellSizes).
 class GridSpace {
  public:
   enum class Dimension { TWO = 2, THREE = 3 };
@@ -248,18 +253,60 @@ class GridSpace {
   Real basedDx() const { return _dx; }
   Real basedDy() const { return _dy; }
   Real basedDz() const { return _dz; }
+  Real minDx() const { return minSize(0); }
+  Real minDy() const { return minSize(1); }
+  Real minDz() const { return minSize(2); }
   Vector origin() const { return _origin; }
+  bool graded(int axis) const { return !_e_node[axis].empty(); }
+  bool graded() const { return graded(0) || graded(1) || graded(2); }
   // node coordinate
-  Real nodeX(Index i) const { return _origin.x + _dx * static_cast<Real>(i); }
-  Real nodeY(Index j) const { return _origin.y + _dy * static_cast<Real>(j); }
-  Real nodeZ(Index k) const { return _origin.z + _dz * static_cast<Real>(k); }
+  Real nodeX(Index i) const { return node(0, i); }
+  Real nodeY(Index j) const { return node(1, j); }
+  Real nodeZ(Index k) const { return node(2, k); }
+  Real node(int axis, Index i) const {
+    if (graded(axis)) return _e_node[axis][i];
+    const Real o = axis == 0 ? _origin.x : axis == 1 ? _origin.y : _origin.z;
+    return o + based(axis) * static_cast<Real>(i);
+  }
+  Real based(int axis) const { return axis == 0 ? _dx : axis == 1 ? _dy : _dz; }
+  // size of cell i (E edge) and of the dual edge around E node i
+  Real cellSize(int axis, Index i) const { return graded(axis) ? _e_size[axis][i] : based(axis); }
+  Real dualSize(int axis, Index i) const {
+    if (!graded(axis)) return based(axis);
+    const Index n = size(axis);
+    if (i == 0) return cellSize(axis, 0);
+    if (i >= n) return cellSize(axis, n - 1);
+    return (_e_size[axis][i - 1] + _e_size[axis][i]) / 2;
+  }
+  Index size(int axis) const { return axis == 0 ? _nx : axis == 1 ? _ny : _nz; }
+  std::vector<Real> eNode(int axis) const;
+  std::vector<Real> hNode(int axis) const;
+  std::vector<Real> eSize(int axis) const;
+  std::vector<Real> hSize(int axis) const;
+  std::vector<Real> eNodeX() const { return eNode(0); }
+  std::vector<Real> eNodeY() const { return eNode(1); }
+  std::vector<Real> eNodeZ() const { return eNode(2); }
+  std::vector<Real> hNodeX() const { return hNode(0); }
+  std::vector<Real> hNodeY() const { return hNode(1); }
+  std::vector<Real> hNodeZ() const { return hNode(2); }
+  std::vector<Real> eSizeX() const { return eSize(0); }
+  std::vector<Real> eSizeY() const { return eSize(1); }
+  std::vector<Real> eSizeZ() const { return eSize(2); }
+  std::vector<Real> hSizeX() const { return hSize(0); }
+  std::vector<Real> hSizeY() const { return hSize(1); }
+  std::vector<Real> hSizeZ() const { return hSize(2); }
+  // the cell whose [node(i), node(i + 1)) holds x, clamped to [0, n - 1]
+  Index cellOf(int axis, Real x) const;
 
  private:
   friend class Simulation;
+  Real minSize(int axis) const;
   Dimension _dim{Dimension::THREE};
   Index _nx{}, _ny{}, _nz{};
   Real _dx{}, _dy{}, _dz{};
   Vector _origin{};
+  std::vector<Real> _e_node[3];  // graded axes only: node coordinates
+  std::vector<Real> _e_size[3];  // and cell sizes (as given)
 };
 
 class TimeParam {
@@ -620,6 +667,14 @@ class Simulation {
   enum class CoefficientLayout { Dense = 0, Compact = 1 };
   void setCoefficientLayout(CoefficientLayout l) { _layout = l; }
 
+  // A graded axis: the sizes of the domain's cells along it, from its lower
+  // end (they must add up to the domain's extent); PML cells outside the
+  // domain keep the based size.  The Yee coefficients then use each node's
+  // own cell and dual-edge sizes (dense layout only), dt the smallest cells;
+  // TF/SF boxes and NF2FF time-domain boxes need uniform cells where they
+  // lie, learned layers a uniform grid.
+  void setCellSizes(Axis::XYZ axis, std::vector<Real> sizes);
+
   // Builds grid, materials, coefficients, PML, TFSF tables, NF2FF phasors,
   // ADE storage and zeroed fields for time_step steps.
   void init(Index time_step);
@@ -679,6 +734,14 @@ class Simulation {
   void buildMovie(MovieMonitor& m);
   void fillScene();
   std::vector<xf_point_source> _pt_desc;
+  std::vector<Real> _graded_sizes[3];
+  // per material: the E and H denominators, and for dispersive media the
+  // ADE curl numerator (the curl coefficients divide these by a size)
+  std::vector<Real> _mat_den_e, _mat_den_h, _mat_ccurl;
+  std::vector<unsigned char> _mat_disp;
+  // Yee coefficient n (0..17) of the cell (i, j, k) from its material and
+  // the local sizes (graded grids)
+  Real gradedCoeff(int n, uint16_t m, int64_t i, int64_t j, int64_t k) const;
 };
 
 }  // namespace xfdtd

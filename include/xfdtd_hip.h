@@ -300,7 +300,19 @@ typedef struct xf_plan {
   int32_t device;       /* HIP device ordinal */
   int32_t chunks_x;     /* 0 = auto; x-chunks of the 3-D marching kernels */
   int32_t tiling_flags; /* 0 = auto, else XF_TILE_SET | XF_TILE_* bits    */
+  int32_t coeff_mode;   /* enum xf_coeff_mode                              */
 } xf_plan;
+
+/* How the Yee coefficients reach the kernels (dense coefficient input only;
+ * the compact material table is always a table). */
+enum xf_coeff_mode {
+  XF_COEFF_AUTO = 0,  /* a table of distinct 9-coefficient rows per pass when
+                         there are at most 65536 of them, else dense */
+  XF_COEFF_TABLE = 1, /* the table only: more rows is XF_ERR_UNSUPPORTED  */
+  XF_COEFF_DENSE = 2  /* the per-cell arrays streamed by the kernels
+                         (fdtd_coefficient.cuh:149-167): 9 values per cell
+                         per pass, in the compute type */
+};
 
 /* Tiling flags of the 3-D Yee kernels. */
 enum xf_tile_flags {

@@ -108,6 +108,11 @@ int xfh_sim_output_movies(xfh_sim* s, int64_t* frames);
 /* 0 dense (the reference's 18 coefficient arrays), 1 compact. */
 int xfh_sim_set_coefficient_layout(xfh_sim* s, int32_t layout);
 
+/* Simulation::setCellSizes: a graded axis (0 x, 1 y, 2 z) -- the sizes of
+ * the domain's n cells along it from its lower end (the reference's
+ * GridSpace e/h node and size arrays, grid_space.cuh:36-90). */
+int xfh_sim_set_cell_sizes(xfh_sim* s, int32_t axis, const double* sizes, int64_t n);
+
 /* Simulation::init(n): builds everything for n steps. */
 int xfh_sim_init(xfh_sim* s, int64_t time_step);
 

@@ -43,7 +43,11 @@ class XfPlan(ctypes.Structure):
         ("device", c_i32),
         ("chunks_x", c_i32),
         ("tiling_flags", c_i32),
+        ("coeff_mode", c_i32),
     ]
+
+
+COEFF_AUTO, COEFF_TABLE, COEFF_DENSE = 0, 1, 2
 
 
 TILE_XCD_REMAP = 1
@@ -179,6 +183,7 @@ def host() -> ctypes.CDLL:
             "xfh_sim_add_tfsf2d": [c_vp, p3i, c_f64, c_i32, c_f64, c_f64, c_f64],
             "xfh_sim_add_nf2ff_fd": [c_vp, p3i, p3d, c_i32],
             "xfh_sim_set_coefficient_layout": [c_vp, c_i32],
+            "xfh_sim_set_cell_sizes": [c_vp, c_i32, c_dp, c_i64],
             "xfh_sim_init": [c_vp, c_i64],
             "xfh_sim_scene": [c_vp, ctypes.POINTER(c_vp)],
             "xfh_sim_info": [c_vp, p3i, ctypes.POINTER(c_f64), ctypes.POINTER(c_i32)],

@@ -362,6 +362,16 @@ int xfh_sim_set_coefficient_layout(xfh_sim* s, int32_t layout) {
   });
 }
 
+int xfh_sim_set_cell_sizes(xfh_sim* s, int32_t axis, const double* sizes, int64_t n) {
+  return guard([&] {
+    if (!s) return fail("null sim");
+    if (axis < 0 || axis > 2) return fail("bad axis");
+    if (!sizes || n < 1) return fail("no cell sizes");
+    s->sim->setCellSizes(static_cast<xfdtd::Axis::XYZ>(axis), std::vector<double>(sizes, sizes + n));
+    return 0;
+  });
+}
+
 int xfh_sim_init(xfh_sim* s, int64_t n) {
   return guard([&] {
     if (!s) return fail("null sim");

@@ -336,7 +336,8 @@ NFFFTFrequencyDomain::FarField NFFFTFrequencyDomain::farField(
       if (!(t.x.start < t.x.end && t.y.start < t.y.end && t.z.start < t.z.end)) continue;
       const int axis = face / 2;
       const int a = (axis + 1) % 3, b = (axis + 2) % 3;
-      const Real ds = axis == 0 ? dy * dz : axis == 1 ? dz * dx : dx * dy;
+      const Real ds_u = axis == 0 ? dy * dz : axis == 1 ? dz * dx : dx * dy;
+      const bool gr = g.graded(a) || g.graded(b);
       const Array* cur = &_cur[(freq_index * 6 + face) * 4];
       const int64_t sy = t.y.end - t.y.start, sz = t.z.end - t.z.start;
       for (int64_t i = t.x.start; i < t.x.end; ++i)
@@ -345,8 +346,18 @@ NFFFTFrequencyDomain::FarField NFFFTFrequencyDomain::farField(
             // face centre: on the face plane, half a cell along a and b
             Real p[3] = {g.nodeX(i), g.nodeY(j), g.nodeZ(k)};
             const Real d[3] = {dx, dy, dz};
-            p[a] += 0.5 * d[a];
-            p[b] += 0.5 * d[b];
+            Real ds = ds_u;
+            if (gr) {
+              // graded: the face cell's own centre and area
+              const int64_t at[3] = {i, j, k};
+              const Real sa = g.cellSize(a, at[a]), sb = g.cellSize(b, at[b]);
+              p[a] += 0.5 * sa;
+              p[b] += 0.5 * sb;
+              ds = sa * sb;
+            } else {
+              p[a] += 0.5 * d[a];
+              p[b] += 0.5 * d[b];
+            }
             if (g.dimension() == GridSpace::Dimension::TWO) p[2] = 0.0;
             const C phase = std::exp(C(0.0, k0 * (rx * p[0] + ry * p[1] + rz * p[2]))) * ds;
             const int64_t lin = (((i - t.x.start) * sy + (j - t.y.start)) * sz + (k - t.z.start)) * 2;
@@ -441,6 +452,59 @@ Simulation::Simulation(Real dx, Real dy, Real dz, Real cfl)
   if (!(cfl > 0 && cfl <= 1)) throw XFDTDException("Simulation: cfl not in (0, 1]");
 }
 
+std::vector<Real> GridSpace::eNode(int axis) const {
+  std::vector<Real> v(size(axis) + 1);
+  for (Index i = 0; i < v.size(); ++i) v[i] = node(axis, i);
+  return v;
+}
+
+std::vector<Real> GridSpace::hNode(int axis) const {
+  std::vector<Real> v(size(axis));
+  for (Index i = 0; i < v.size(); ++i)
+    v[i] = graded(axis) ? (node(axis, i) + node(axis, i + 1)) / 2 : node(axis, i) + 0.5 * based(axis);
+  return v;
+}
+
+std::vector<Real> GridSpace::eSize(int axis) const {
+  std::vector<Real> v(size(axis));
+  for (Index i = 0; i < v.size(); ++i) v[i] = cellSize(axis, i);
+  return v;
+}
+
+std::vector<Real> GridSpace::hSize(int axis) const {
+  std::vector<Real> v(size(axis) + 1);
+  for (Index i = 0; i < v.size(); ++i) v[i] = dualSize(axis, i);
+  return v;
+}
+
+Real GridSpace::minSize(int axis) const {
+  if (!graded(axis)) return based(axis);
+  Real m = std::numeric_limits<Real>::infinity();
+  for (Index i = 0; i < size(axis); ++i) m = std::min(m, cellSize(axis, i));
+  return m;
+}
+
+Index GridSpace::cellOf(int axis, Real x) const {
+  const Index n = size(axis);
+  if (!graded(axis)) {
+    const Real o = axis == 0 ? _origin.x : axis == 1 ? _origin.y : _origin.z;
+    const Real f = std::floor((x - o) / based(axis));
+    return static_cast<Index>(std::min<Real>(std::max<Real>(f, 0), static_cast<Real>(n - 1)));
+  }
+  const std::vector<Real>& e = _e_node[axis];
+  const auto it = std::upper_bound(e.begin(), e.end(), x);
+  const int64_t c = static_cast<int64_t>(it - e.begin()) - 1;
+  return static_cast<Index>(std::min<int64_t>(std::max<int64_t>(c, 0), static_cast<int64_t>(n) - 1));
+}
+
+void Simulation::setCellSizes(Axis::XYZ axis, std::vector<Real> sizes) {
+  if (sizes.empty()) throw XFDTDException("setCellSizes: no cells");
+  for (Real d : sizes)
+    if (!(d > 0) || !std::isfinite(d)) throw XFDTDException("setCellSizes: cell size <= 0");
+  _graded_sizes[static_cast<int>(axis)] = std::move(sizes);
+  _initialized = false;
+}
+
 namespace {
 // Cells a face's boundary adds outside the domain: the thickest PML, or
 // the learned layer's thickness (the two do not share a face).
@@ -487,14 +551,48 @@ void Simulation::buildGrid() {
   g._origin.y = lo.y - _dy * static_cast<Real>(lyn);
   if (two_d) {
     if (lzn || lzp) throw XFDTDException("Simulation::init: z PML on a 2-D grid");
+    if (!_graded_sizes[2].empty()) throw XFDTDException("Simulation::init: graded z on a 2-D grid");
     g._nz = 1;
     g._origin.z = -0.5 * _dz;
   } else {
     g._nz = cells(lo.z, hi.z, _dz) + lzn + lzp;
     g._origin.z = lo.z - _dz * static_cast<Real>(lzn);
   }
-  const Real inv2 = two_d ? 1 / (_dx * _dx) + 1 / (_dy * _dy)
-                          : 1 / (_dx * _dx) + 1 / (_dy * _dy) + 1 / (_dz * _dz);
+  // graded axes: PML cells of the based size, then the given domain cells
+  const Real lo3[3] = {lo.x, lo.y, lo.z}, hi3[3] = {hi.x, hi.y, hi.z};
+  const Index pn[3] = {lxn, lyn, lzn}, pp[3] = {lxp, lyp, lzp};
+  for (int ax = 0; ax < 3; ++ax) {
+    const std::vector<Real>& sz = _graded_sizes[ax];
+    g._e_node[ax].clear();
+    g._e_size[ax].clear();
+    if (sz.empty()) continue;
+    Real sum = 0;
+    for (Real d : sz) sum += d;
+    const Real len = hi3[ax] - lo3[ax];
+    if (!(std::abs(sum - len) <= 1e-9 * len))
+      throw XFDTDException("setCellSizes: the cells do not add up to the domain's extent");
+    const Real d = g.based(ax);
+    std::vector<Real>& e = g._e_node[ax];
+    e.reserve(pn[ax] + sz.size() + pp[ax] + 1);
+    for (Index l = 0; l <= pn[ax]; ++l) e.push_back(lo3[ax] - d * static_cast<Real>(pn[ax] - l));
+    Real x = lo3[ax];
+    for (std::size_t c = 0; c < sz.size(); ++c) {
+      x += sz[c];
+      e.push_back(c + 1 == sz.size() ? hi3[ax] : x);
+    }
+    for (Index l = 1; l <= pp[ax]; ++l) e.push_back(hi3[ax] + d * static_cast<Real>(l));
+    std::vector<Real>& es = g._e_size[ax];
+    es.assign(pn[ax], d);
+    es.insert(es.end(), sz.begin(), sz.end());
+    es.insert(es.end(), pp[ax], d);
+    const Index n = static_cast<Index>(e.size() - 1);
+    if (ax == 0) g._nx = n;
+    if (ax == 1) g._ny = n;
+    if (ax == 2) g._nz = n;
+  }
+  const Real mx = g.minDx(), my = g.minDy(), mz = g.minDz();
+  const Real inv2 = two_d ? 1 / (mx * mx) + 1 / (my * my)
+                          : 1 / (mx * mx) + 1 / (my * my) + 1 / (mz * mz);
   _time._dt = _cfl / (C_0 * std::sqrt(inv2));
 }
 
@@ -531,22 +629,32 @@ void Simulation::voxelize() {
       b = static_cast<Index>(std::max(fa, fb));
     };
     Index i0, i1, j0, j1, k0, k1;
-    range(lo.x, hi.x, g.origin().x, _dx, nx, i0, i1);
-    range(lo.y, hi.y, g.origin().y, _dy, ny, j0, j1);
+    // graded axes: the cells around the bounds, one cell of margin
+    auto grange = [&](int ax, Real lo, Real hi, Index n, Index& a, Index& b) {
+      a = std::isinf(lo) ? 0 : (g.cellOf(ax, lo) > 0 ? g.cellOf(ax, lo) - 1 : 0);
+      b = std::isinf(hi) ? n : std::min<Index>(n, g.cellOf(ax, hi) + 2);
+      b = std::max(a, b);
+    };
+    if (g.graded(0)) grange(0, lo.x, hi.x, nx, i0, i1); else range(lo.x, hi.x, g.origin().x, _dx, nx, i0, i1);
+    if (g.graded(1)) grange(1, lo.y, hi.y, ny, j0, j1); else range(lo.y, hi.y, g.origin().y, _dy, ny, j0, j1);
     if (g.dimension() == GridSpace::Dimension::TWO) {
       k0 = 0;
       k1 = 1;
+    } else if (g.graded(2)) {
+      grange(2, lo.z, hi.z, nz, k0, k1);
     } else {
       range(lo.z, hi.z, g.origin().z, _dz, nz, k0, k1);
     }
+    auto centre = [&](int ax, Index c) {
+      return g.graded(ax) ? (g.node(ax, c) + g.node(ax, c + 1)) / 2 : g.node(ax, c) + 0.5 * g.based(ax);
+    };
     const uint16_t m = obj_mat[o];
 #pragma omp parallel for schedule(static)
     for (int64_t i = static_cast<int64_t>(i0); i < static_cast<int64_t>(i1); ++i)
       for (Index j = j0; j < j1; ++j)
         for (Index k = k0; k < k1; ++k) {
-          Vector p{g.nodeX(i) + 0.5 * _dx, g.nodeY(j) + 0.5 * _dy,
-                   g.dimension() == GridSpace::Dimension::TWO ? 0.0
-                                                              : g.nodeZ(k) + 0.5 * _dz};
+          Vector p{centre(0, i), centre(1, j),
+                   g.dimension() == GridSpace::Dimension::TWO ? 0.0 : centre(2, k)};
           if (s.isInside(p)) _cell_mat[(i * ny + j) * nz + k] = m;
         }
   }
@@ -660,16 +768,24 @@ void Simulation::buildCoefficients() {
   for (auto m : _mat_table) np = std::max(np, polesOf(*m, kind));
   const std::size_t nm = _mat_table.size();
   _mat_coeff.assign(nm * 18, 0.0);
+  _mat_den_e.assign(nm, 0.0);
+  _mat_den_h.assign(nm, 0.0);
+  _mat_ccurl.assign(nm, 0.0);
+  _mat_disp.assign(nm, 0);
   for (std::size_t m = 0; m < nm; ++m) {
     const Material& mat = *_mat_table[m];
     const Real eps = EPSILON_0 * mat.emp().epsilon_r;
     const Real mu = MU_0 * mat.emp().mu_r;
     const Real se = mat.emp().sigma_e, sm = mat.emp().sigma_m;
     Real* row = &_mat_coeff[m * 18];
+    _mat_den_e[m] = 2 * eps + se * dt;
+    _mat_den_h[m] = 2 * mu + sm * dt;
     for (int c = 0; c < 3; ++c) {
       const int a = (c + 1) % 3, b = (c + 2) % 3;
       if (mat.dispersion() != Material::Dispersion::None) {
         const AdeCoeff ac = adeCoefficients(mat, kind, np, dt);
+        _mat_disp[m] = 1;
+        _mat_ccurl[m] = ac.ccurl;
         row[3 * c] = ac.ce;
         row[3 * c + 1] = -ac.ccurl / d[b];
         row[3 * c + 2] = ac.ccurl / d[a];
@@ -688,9 +804,11 @@ void Simulation::buildCoefficients() {
   const GridSpace& g = *_grid;
   const int64_t nx = g.sizeX(), ny = g.sizeY(), nz = g.sizeZ();
   if (_layout == CoefficientLayout::Compact) {
+    if (g.graded()) throw XFDTDException("a graded grid needs the dense coefficient layout");
     for (auto& c : _coeff) c = Array();
     return;
   }
+  const bool graded = g.graded();
   for (int n = 0; n < 18; ++n) {
     const int attr = n / 9, comp = (n % 9) / 3;
     std::vector<int64_t> shape = {nx + 1, ny + 1, nz + 1};
@@ -710,11 +828,32 @@ void Simulation::buildCoefficients() {
         for (int64_t k = 0; k < s2; ++k) {
           const int64_t ck = std::min(k, nz - 1);
           const uint16_t m = _cell_mat[(ci * ny + cj) * nz + ck];
-          arr[(i * s1 + j) * s2 + k] = _mat_coeff[m * 18 + n];
+          arr[(i * s1 + j) * s2 + k] = graded ? gradedCoeff(n, m, i, j, k) : _mat_coeff[m * 18 + n];
         }
       }
     }
   }
+}
+
+// Coefficient n of node (i, j, k) on a graded grid: the uniform formulas
+// with the based size replaced by the node's own -- the dual edge (H node
+// to H node) around an E node for E's curl terms, the cell edge (E node to
+// E node) for H's (the sizes the curl differences span).
+Real Simulation::gradedCoeff(int n, uint16_t m, int64_t i, int64_t j, int64_t k) const {
+  const GridSpace& g = *_grid;
+  const int attr = n / 9, c = (n % 9) / 3, which = n % 3;
+  if (which == 0) return _mat_coeff[m * 18 + n];
+  const int64_t at[3] = {i, j, k};
+  const int ax = which == 1 ? (c + 2) % 3 : (c + 1) % 3;
+  const Real dt = _time._dt;
+  if (attr == 0) {
+    const Real s = g.dualSize(ax, static_cast<Index>(at[ax]));
+    if (_mat_disp[m]) return which == 1 ? -_mat_ccurl[m] / s : _mat_ccurl[m] / s;
+    return which == 1 ? -(2 * dt / s) / _mat_den_e[m] : (2 * dt / s) / _mat_den_e[m];
+  }
+  const int64_t nn = static_cast<int64_t>(g.size(ax));
+  const Real s = g.cellSize(ax, static_cast<Index>(std::min<int64_t>(at[ax], nn - 1)));
+  return which == 1 ? (2 * dt / s) / _mat_den_h[m] : -(2 * dt / s) / _mat_den_h[m];
 }
 
 // PML set-up, with the index conventions PMLCorrector reads
@@ -802,7 +941,8 @@ void Simulation::buildPML(PML& p) {
           const int64_t ci = std::min(pos[0], nxx - 1), cj = std::min(pos[1], nyy - 1),
                         ck = std::min(pos[2], nzz - 1);
           const uint16_t mm = _cell_mat[(ci * nyy + cj) * nzz + ck];
-          cp.at(i, j, k) = _mat_coeff[mm * 18 + coeff_n];
+          cp.at(i, j, k) = g.graded() ? gradedCoeff(coeff_n, mm, pos[0], pos[1], pos[2])
+                                      : _mat_coeff[mm * 18 + coeff_n];
         }
   };
   fill_cpsi(p._c_ea, 0, a, 1, dsc.node_e_start);
@@ -861,6 +1001,14 @@ void Simulation::buildTFSF(TFSF& tf) {
     if (2 * st[ax] + 1 > n[ax] || st[ax] < 1)
       throw XFDTDException("TFSF box does not fit in the grid");
     r[ax] = {st[ax], n[ax] - st[ax]};
+  }
+  // the incident field is placed by node index: the box must sit in cells
+  // of the based size along every graded axis
+  for (int ax = 0; ax < 3; ++ax) {
+    if (!g.graded(ax)) continue;
+    for (int64_t c = std::max<int64_t>(r[ax].start - 1, 0); c < std::min<int64_t>(r[ax].end + 1, n[ax]); ++c)
+      if (std::abs(g.cellSize(ax, c) - g.based(ax)) > 1e-12 * g.based(ax))
+        throw XFDTDException("TFSF box over graded cells: the box needs uniform cells of the based size");
   }
   xf_tfsf& dsc = tf._desc;
   dsc = xf_tfsf{};
@@ -1190,8 +1338,13 @@ void Simulation::buildMovie(MovieMonitor& m) {
   for (int a = 0; a < 3; ++a) {
     // a non-finite bound (an infinite cube side) takes the whole axis
     const bool full = !std::isfinite(l[a]) || !std::isfinite(h[a]);
-    start[a] = full ? 0 : static_cast<int64_t>(std::llround((l[a] - o[a]) / d[a]));
-    end[a] = full ? sh[a] : static_cast<int64_t>(std::llround((h[a] - o[a]) / d[a]));
+    auto nearest = [&](Real x) -> int64_t {
+      if (!g.graded(a)) return static_cast<int64_t>(std::llround((x - o[a]) / d[a]));
+      const int64_t c = static_cast<int64_t>(g.cellOf(a, x));
+      return (x - g.node(a, c) <= g.node(a, c + 1) - x) ? c : c + 1;
+    };
+    start[a] = full ? 0 : nearest(l[a]);
+    end[a] = full ? sh[a] : nearest(h[a]);
     start[a] = std::max<int64_t>(0, std::min<int64_t>(start[a], sh[a] - 1));
     end[a] = std::max<int64_t>(start[a] + 1, std::min<int64_t>(end[a], sh[a]));
   }
@@ -1264,7 +1417,7 @@ void Simulation::buildLearnedBoundary(LearnedBoundary& l) {
   const GridSpace& g = *_grid;
   if (g.dimension() != GridSpace::Dimension::THREE)
     throw XFDTDException("learned layer: 3-D grids only");
-  if (_dx != _dy || _dx != _dz) throw XFDTDException("learned layer: cells must be cubes");
+  if (_dx != _dy || _dx != _dz || _grid->graded()) throw XFDTDException("learned layer: cells must be cubes");
   const int64_t n[3] = {static_cast<int64_t>(g.sizeX()), static_cast<int64_t>(g.sizeY()),
                         static_cast<int64_t>(g.sizeZ())};
   const int64_t inset = l._desc.inset;
@@ -1300,6 +1453,7 @@ void Simulation::buildNF2FFTD(NFFFTTimeDomain& nf) {
   const GridSpace& g = *_grid;
   if (g.dimension() != GridSpace::Dimension::THREE)
     throw XFDTDException("NF2FF time domain needs a 3-D grid");
+  if (g.graded()) throw XFDTDException("NF2FF time domain needs a uniform grid");
   xf_nf2ff_td& dsc = nf._desc;
   dsc = xf_nf2ff_td{};
   huygensFaces(g, nf._sx, nf._sy, nf._sz, dsc.face_task);

@@ -138,6 +138,10 @@ int Batch::create(xf_scene* const* scenes, int n, const xf_plan* pl) {
   xf_plan p = plan;
   p.use_graph = 0;
   p.timing = 0;
+  // the batched Yee kernels read coefficient tables only
+  if (p.coeff_mode == XF_COEFF_DENSE)
+    return fail(XF_ERR_UNSUPPORTED, "batches take the coefficient table mode only");
+  p.coeff_mode = XF_COEFF_TABLE;
   for (int g = 0; g < n; ++g) {
     if (!scenes[g]) return fail(XF_ERR_INVALID, "null scene " + std::to_string(g));
     xf_handle* h = nullptr;

@@ -171,6 +171,12 @@ struct Engine {
   unsigned char* eflag{};
   std::vector<double> etab_h, htab_h;  // host copies (double)
   std::vector<unsigned char> eflag_h;
+  // dense mode (XF_COEFF_DENSE, or too many rows for a table): the 18
+  // per-cell coefficient arrays in the padded layout (E pass 0..8, H pass
+  // 9..17), compute type; the E ids then only flag dispersive cells
+  bool dense{false};
+  void* dcoef[18]{};
+  int upload_dense_coeff();
   // PML
   std::vector<int> pml_order;  // scene index sorted by axis
   struct PmlDev {
@@ -227,6 +233,14 @@ struct Engine {
   int autotune(int trial_steps, xf_tuning* best);
   template <typename S>
   YeePass<S> yee_pass(bool is_e) const;
+  // the dense coefficient arrays of a pass (all null in table mode)
+  template <typename C>
+  Dense9<C> dense9(bool is_e) const {
+    Dense9<C> d{};
+    if (dense)
+      for (int n = 0; n < 9; ++n) d.a[n] = static_cast<const C*>(dcoef[(is_e ? 0 : 9) + n]);
+    return d;
+  }
   std::string layout_desc;
 
   ~Engine() { release(); }
@@ -1008,6 +1022,11 @@ int Engine::build_ids() {
       for (int64_t j = 0; j < ny; ++j)
         for (int64_t k = 0; k < nz; ++k) disp[(i * ny + j) * nz + k] = dispersive(i, j, k) ? 1 : 0;
   }
+  dense = false;
+  if (sc->coeff[0].data == nullptr && plan.coeff_mode == XF_COEFF_DENSE)
+    return fail(XF_ERR_UNSUPPORTED, "dense coefficient mode needs the dense coefficient input");
+  if (plan.coeff_mode < XF_COEFF_AUTO || plan.coeff_mode > XF_COEFF_DENSE)
+    return fail(XF_ERR_INVALID, "unknown coefficient mode");
   if (sc->coeff[0].data == nullptr) {
     const int nm = sc->n_material;
     if (has_ade) return fail(XF_ERR_UNSUPPORTED, "ADE media need the dense coefficient layout");
@@ -1024,13 +1043,19 @@ int Engine::build_ids() {
       ide[c] = idh[c] = m;
     }
     ntab_e = ntab_h = nm;
-  } else {
+  } else if (plan.coeff_mode != XF_COEFF_DENSE) {
     std::unordered_map<std::vector<uint64_t>, uint32_t, KeyHash> me, mh;
     std::vector<uint64_t> ke(10), kh(9), last_e, last_h;
     uint32_t last_ide = 0, last_idh = 0;
-    for (int64_t i = 0; i < nx; ++i)
-      for (int64_t j = 0; j < ny; ++j)
+    for (int64_t i = 0; i < nx && !dense; ++i)
+      for (int64_t j = 0; j < ny && !dense; ++j)
         for (int64_t k = 0; k < nz; ++k) {
+          // past the table's limit the per-cell arrays are streamed instead
+          if (me.size() > 65536 || mh.size() > 65536) {
+            if (plan.coeff_mode == XF_COEFF_TABLE) break;
+            dense = true;
+            break;
+          }
           const int64_t c = (i * ny + j) * nz + k;
           for (int n = 0; n < 18; ++n) {
             const xf_array& a = sc->coeff[n];
@@ -1077,9 +1102,24 @@ int Engine::build_ids() {
         }
     ntab_e = static_cast<int>(me.size());
     ntab_h = static_cast<int>(mh.size());
+  } else {
+    dense = true;
+  }
+  if (dense) {
+    // the ids flag dispersive cells (E pass row 1), the tables are unused
+    for (int64_t c = 0; c < nx * ny * nz; ++c) {
+      ide[c] = has_ade ? disp[c] : 0;
+      idh[c] = 0;
+    }
+    etab_h.assign(18, 0.0);
+    htab_h.assign(9, 0.0);
+    eflag_h = {0, 1};
+    ntab_e = 2;
+    ntab_h = 1;
+    XF_TRY(upload_dense_coeff());
   }
   if (ntab_e > 65536 || ntab_h > 65536)
-    return fail(XF_ERR_UNSUPPORTED, "more than 65536 distinct coefficient rows; the engine needs a smaller coefficient variety");
+    return fail(XF_ERR_UNSUPPORTED, "more than 65536 distinct coefficient rows and the table-only coefficient mode");
   id_bytes = (ntab_e <= 256 && ntab_h <= 256 && !wide_ids) ? 1 : 2;
   // scatter ids into the padded layout
   auto put = [&](void** dptr, const std::vector<uint32_t>& id) -> int {
@@ -1111,6 +1151,32 @@ int Engine::build_ids() {
       for (int64_t j = 0; j < ny; ++j)
         for (int64_t k = 0; k < nz; ++k)
           if (disp[(i * ny + j) * nz + k]) ade_cells.push_back((i * (ny + 1) + j) * (nz + 1) + k);
+  }
+  return XF_OK;
+}
+
+// The 18 per-cell coefficient arrays in the padded layout, in the compute
+// type (converted exactly as the tables are).
+int Engine::upload_dense_coeff() {
+  const bool f64 = plan.precision == XF_FP64;
+  const size_t cb = f64 ? sizeof(double) : sizeof(float);
+  std::vector<unsigned char> buf(static_cast<size_t>(total) * cb);
+  for (int n = 0; n < 18; ++n) {
+    const xf_array& a = sc->coeff[n];
+    std::fill(buf.begin(), buf.end(), 0);
+    host_parallel_for(nx, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i)
+        for (int64_t j = 0; j < ny; ++j)
+          for (int64_t k = 0; k < nz; ++k) {
+            const double v = a.data[(i * a.shape[1] + j) * a.shape[2] + k];
+            const int64_t idx = dim == 3 ? i * sx + j * sy + k : i * sx + j;
+            if (f64)
+              reinterpret_cast<double*>(buf.data())[idx] = v;
+            else
+              reinterpret_cast<float*>(buf.data())[idx] = static_cast<float>(v);
+          }
+    });
+    XF_TRY(upload_vec(&dcoef[n], buf));
   }
   return XF_OK;
 }
@@ -1601,7 +1667,8 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
   graph = nullptr;
   graph_ready = false;
   layout_desc = std::string("dim=") + std::to_string(dim) + " pad_stride=" + std::to_string(dim == 3 ? sy : sx) +
-                " id_bytes=" + std::to_string(id_bytes) + " etab=" + std::to_string(ntab_e) +
+                " id_bytes=" + std::to_string(id_bytes) + " coeff=" + (dense ? "dense" : "table") +
+                " etab=" + std::to_string(ntab_e) +
                 " htab=" + std::to_string(ntab_h) + " chunk=" + std::to_string(chunk) +
                 " nchunk=" + std::to_string(nch_eff) + " tile_y=" + std::to_string(tile_y) +
                 " xcd=" + std::to_string(xcd) + " nt=" + std::to_string(nt_streams) + " pml_split=" + std::to_string(split_z() ? 1 : 0) +
@@ -1810,6 +1877,8 @@ YeePass<S> Engine::yee_pass(bool is_e) const {
   D.tab = static_cast<const C*>(is_e ? etab : htab);
   D.ntab = is_e ? ntab_e : ntab_h;
   D.eflag = is_e ? eflag : nullptr;
+  const Dense9<C> dn = dense9<C>(is_e);
+  for (int n = 0; n < 9; ++n) D.dense[n] = dn.a[n];
   D.d = make_dims(*this);
   D.sc = make_scales<C>(*this);
   D.nkb = nkb;
@@ -1924,7 +1993,16 @@ int Engine::launch_h(int t) {
     else                                                                                     \
       hipLaunchKernelGGL((k_yee_h3<S, I, L, PM, false>), grid, block, shm, stream, D, xcd);  \
   } while (0)
-    if (id_bytes == 1) {
+#define XF_LH3D(PM)                                                                                   \
+  do {                                                                                                \
+    if (nt_streams)                                                                                   \
+      hipLaunchKernelGGL((k_yee_h3<S, uint8_t, false, PM, true, true>), grid, block, 0, stream, D, xcd);  \
+    else                                                                                              \
+      hipLaunchKernelGGL((k_yee_h3<S, uint8_t, false, PM, false, true>), grid, block, 0, stream, D, xcd); \
+  } while (0)
+    if (dense) {
+      if (has_pml) XF_LH3D(true); else XF_LH3D(false);
+    } else if (id_bytes == 1) {
       if (lds) { if (has_pml) XF_LH3(uint8_t, true, true); else XF_LH3(uint8_t, true, false); }
       else { if (has_pml) XF_LH3(uint8_t, false, true); else XF_LH3(uint8_t, false, false); }
     } else {
@@ -1932,13 +2010,14 @@ int Engine::launch_h(int t) {
       else { if (has_pml) XF_LH3(uint16_t, false, true); else XF_LH3(uint16_t, false, false); }
     }
 #undef XF_LH3
+#undef XF_LH3D
     if (split_z()) XF_TRY(launch_pml_z<S>(false, stream));
   } else {
     dim3 block(64, 4);
     dim3 grid((unsigned)((ny + 1 + 63) / 64), (unsigned)((nx + 1 + 3) / 4));
 #define XF_LH2(I, L, PM)                                                                              \
   hipLaunchKernelGGL((k_yee_h2<S, I, L, PM>), grid, block, shm, stream, f, static_cast<const I*>(hid), \
-                     tab, ntab_h, P, d, scl)
+                     tab, ntab_h, P, d, scl, dense9<C>(false))
     if (id_bytes == 1) {
       if (lds) { if (has_pml) XF_LH2(uint8_t, true, true); else XF_LH2(uint8_t, true, false); }
       else { if (has_pml) XF_LH2(uint8_t, false, true); else XF_LH2(uint8_t, false, false); }
@@ -1980,7 +2059,17 @@ int Engine::launch_e(int t) {
   } while (0)
 #define XF_LE3A(I, L, PM) \
   do { if (has_ade) XF_LE3(I, L, PM, true); else XF_LE3(I, L, PM, false); } while (0)
-    if (id_bytes == 1) {
+#define XF_LE3D(PM, AD)                                                                                   \
+  do {                                                                                                    \
+    if (nt_streams)                                                                                       \
+      hipLaunchKernelGGL((k_yee_e3<S, uint8_t, false, PM, AD, true, true>), grid, block, 0, stream, D, xcd);  \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_yee_e3<S, uint8_t, false, PM, AD, false, true>), grid, block, 0, stream, D, xcd); \
+  } while (0)
+    if (dense) {
+      if (has_pml) { if (has_ade) XF_LE3D(true, true); else XF_LE3D(true, false); }
+      else { if (has_ade) XF_LE3D(false, true); else XF_LE3D(false, false); }
+    } else if (id_bytes == 1) {
       if (lds) { if (has_pml) XF_LE3A(uint8_t, true, true); else XF_LE3A(uint8_t, true, false); }
       else { if (has_pml) XF_LE3A(uint8_t, false, true); else XF_LE3A(uint8_t, false, false); }
     } else {
@@ -1989,21 +2078,24 @@ int Engine::launch_e(int t) {
     }
 #undef XF_LE3A
 #undef XF_LE3
+#undef XF_LE3D
     XF_HIP(hipGetLastError());
     if (split_z()) XF_TRY(launch_pml_z<S>(true, stream));
     if (has_ade && ade.n > 0) {
       dim3 b1(256), g1((unsigned)((ade.n + 255) / 256));
       if (id_bytes == 1)
-        hipLaunchKernelGGL((k_ade_e3<S, uint8_t>), g1, b1, 0, stream, f, static_cast<const uint8_t*>(eid), tab, ade, d, scl);
+        hipLaunchKernelGGL((k_ade_e3<S, uint8_t>), g1, b1, 0, stream, f, static_cast<const uint8_t*>(eid), tab, ade, d, scl,
+                           dense9<C>(true));
       else
-        hipLaunchKernelGGL((k_ade_e3<S, uint16_t>), g1, b1, 0, stream, f, static_cast<const uint16_t*>(eid), tab, ade, d, scl);
+        hipLaunchKernelGGL((k_ade_e3<S, uint16_t>), g1, b1, 0, stream, f, static_cast<const uint16_t*>(eid), tab, ade, d, scl,
+                           dense9<C>(true));
     }
   } else {
     dim3 block(64, 4);
     dim3 grid((unsigned)((ny + 1 + 63) / 64), (unsigned)((nx + 1 + 3) / 4));
 #define XF_LE2(I, L, PM)                                                                              \
   hipLaunchKernelGGL((k_yee_e2<S, I, L, PM>), grid, block, shm, stream, f, static_cast<const I*>(eid), \
-                     tab, ntab_e, P, d, scl)
+                     tab, ntab_e, P, d, scl, dense9<C>(true))
     if (id_bytes == 1) {
       if (lds) { if (has_pml) XF_LE2(uint8_t, true, true); else XF_LE2(uint8_t, true, false); }
       else { if (has_pml) XF_LE2(uint8_t, false, true); else XF_LE2(uint8_t, false, false); }

@@ -263,6 +263,16 @@ struct PmlZBox {
   int64_t start[3];
 };
 
+// Dense per-cell coefficients of one pass: array 3 c + m holds coefficient
+// m (self, curl a, curl b) of component c at every cell, in the padded
+// layout of the fields (the reference's per-field coefficient arrays,
+// fdtd_coefficient.cuh:149-167).  Used when a scene's coefficient rows are
+// too many to deduplicate into a table; a[0] == nullptr means table mode.
+template <typename C>
+struct Dense9 {
+  const C* a[9];
+};
+
 // Everything one 3-D Yee pass (H or E) of one grid needs: the kernels take
 // it as a kernel argument (one grid) or from an array in device memory
 // (a batch of grids, one launch).
@@ -274,6 +284,7 @@ struct YeePass {
   const C* tab;                 // 9 coefficients per id
   int ntab;
   const unsigned char* eflag;   // E pass: table rows left to the ADE kernel
+  const C* dense[9];            // dense coefficients (padded layout), or null
   Dims d;
   Scales<C> sc;
   int nkb, njb, chunk, nch;     // tiling of this grid

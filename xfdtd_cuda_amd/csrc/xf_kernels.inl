@@ -158,7 +158,8 @@ template <typename S, typename I, bool LDS, bool PML>
 __global__ void __launch_bounds__(256)
     k_yee_h2(Fields<S> f, const I* __restrict__ hid,
              const typename Store<S>::C* __restrict__ htab, int ntab,
-             PmlPass<typename Store<S>::C> pml, Dims d, Scales<typename Store<S>::C> sc) {
+             PmlPass<typename Store<S>::C> pml, Dims d, Scales<typename Store<S>::C> sc,
+             Dense9<typename Store<S>::C> dn) {
   using C = typename Store<S>::C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const C* tab = load_table<C, LDS>(htab, ntab * 9, reinterpret_cast<C*>(smem_raw));
@@ -170,7 +171,11 @@ __global__ void __launch_bounds__(256)
   bool have[3] = {false, false, false};
   bool dirty[3] = {false, false, false};
   if (i < d.nx && j < d.ny) {
-    const C* t = tab + 9 * (int)hid[idx];
+    C td[9];
+    if (dn.a[0]) {
+      for (int n = 0; n < 6; ++n) td[n] = dn.a[n][idx];
+    }
+    const C* t = dn.a[0] ? td : tab + 9 * (int)hid[idx];
     const C ez = Store<S>::ld(f.e[2], idx, sc.inv_e);
     const C ez_j = Store<S>::ld(f.e[2], idx + 1, sc.inv_e);
     const C ez_n = Store<S>::ld(f.e[2], idx + d.sx, sc.inv_e);
@@ -190,7 +195,8 @@ template <typename S, typename I, bool LDS, bool PML>
 __global__ void __launch_bounds__(256)
     k_yee_e2(Fields<S> f, const I* __restrict__ eid,
              const typename Store<S>::C* __restrict__ etab, int ntab,
-             PmlPass<typename Store<S>::C> pml, Dims d, Scales<typename Store<S>::C> sc) {
+             PmlPass<typename Store<S>::C> pml, Dims d, Scales<typename Store<S>::C> sc,
+             Dense9<typename Store<S>::C> dn) {
   using C = typename Store<S>::C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const C* tab = load_table<C, LDS>(etab, ntab * 9, reinterpret_cast<C*>(smem_raw));
@@ -202,7 +208,11 @@ __global__ void __launch_bounds__(256)
   bool have[3] = {false, false, false};
   bool dirty[3] = {false, false, false};
   if (i < d.nx && j < d.ny && i > 0 && j > 0) {
-    const C* t = tab + 9 * (int)eid[idx];
+    C td[9];
+    if (dn.a[0]) {
+      for (int n = 6; n < 9; ++n) td[n] = dn.a[n][idx];
+    }
+    const C* t = dn.a[0] ? td : tab + 9 * (int)eid[idx];
     const C hx = Store<S>::ld(f.h[0], idx, sc.inv_h);
     const C hx_j = Store<S>::ld(f.h[0], idx - 1, sc.inv_h);
     const C hy = Store<S>::ld(f.h[1], idx, sc.inv_h);
@@ -1097,13 +1107,17 @@ template <typename S, typename I>
 __global__ void __launch_bounds__(256)
     k_ade_e3(Fields<S> f, const I* __restrict__ eid,
              const typename Store<S>::C* __restrict__ etab, AdeDev A, Dims d,
-             Scales<typename Store<S>::C> sc) {
+             Scales<typename Store<S>::C> sc, Dense9<typename Store<S>::C> dn) {
   using C = typename Store<S>::C;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= A.n) return;
   const int64_t idx = A.idx[c];
   const int i = A.ijk[3 * c], j = A.ijk[3 * c + 1], k = A.ijk[3 * c + 2];
-  const C* t = etab + 9 * (int)eid[idx];
+  C td[9];
+  if (dn.a[0]) {
+    for (int n = 0; n < 9; ++n) td[n] = dn.a[n][idx];
+  }
+  const C* t = dn.a[0] ? td : etab + 9 * (int)eid[idx];
   const int cell[3] = {i, j, k};
   const int64_t st[3] = {d.sx, d.sy, 1};
   for (int comp = 0; comp < 3; ++comp) {

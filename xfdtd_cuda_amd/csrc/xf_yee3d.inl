@@ -102,7 +102,7 @@ __device__ __forceinline__ unsigned pml_mask(const PmlPass<C>& P, int i0, int i1
 // One tile of the H pass.  Lanes beyond the grid return at once; the tile
 // bodies hold no barrier, so the batched kernels may loop over tiles with
 // some lanes idle.
-template <typename S, typename I, bool PML, bool NT>
+template <typename S, typename I, bool PML, bool NT, bool DN = false>
 __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename Store<S>::C* tab,
                                             int kb, int jb, int ch) {
   using C = typename Store<S>::C;
@@ -133,10 +133,19 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
     bool have[3] = {false, false, false};
     bool dirty[3] = {false, false, false};
     if (jk_main && i < d.nx) {
-      const C* t = tab + 9 * cur.id;
-      v[0] = t[0] * cur.hx + t[1] * (cur.ey_k - ey_i) + t[2] * (cur.ez_j - ez_i);
-      v[1] = t[3] * cur.hy + t[4] * (cur.ez_n - ez_i) + t[5] * (cur.ex_k - cur.ex_i);
-      v[2] = t[6] * cur.hz + t[7] * (cur.ex_j - cur.ex_i) + t[8] * (cur.ey_n - ey_i);
+      if constexpr (DN) {
+        C t[9];
+#pragma unroll
+        for (int n = 0; n < 9; ++n) t[n] = D.dense[n][idx];
+        v[0] = t[0] * cur.hx + t[1] * (cur.ey_k - ey_i) + t[2] * (cur.ez_j - ez_i);
+        v[1] = t[3] * cur.hy + t[4] * (cur.ez_n - ez_i) + t[5] * (cur.ex_k - cur.ex_i);
+        v[2] = t[6] * cur.hz + t[7] * (cur.ex_j - cur.ex_i) + t[8] * (cur.ey_n - ey_i);
+      } else {
+        const C* t = tab + 9 * cur.id;
+        v[0] = t[0] * cur.hx + t[1] * (cur.ey_k - ey_i) + t[2] * (cur.ez_j - ez_i);
+        v[1] = t[3] * cur.hy + t[4] * (cur.ez_n - ez_i) + t[5] * (cur.ex_k - cur.ex_i);
+        v[2] = t[6] * cur.hz + t[7] * (cur.ex_j - cur.ex_i) + t[8] * (cur.ey_n - ey_i);
+      }
       have[0] = have[1] = have[2] = true;
       dirty[0] = dirty[1] = dirty[2] = true;
       ey_i = cur.ey_n;
@@ -150,7 +159,7 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
 
 // One tile of the E pass; table rows flagged dispersive are left to the
 // ADE kernel.
-template <typename S, typename I, bool PML, bool ADE, bool NT>
+template <typename S, typename I, bool PML, bool ADE, bool NT, bool DN = false>
 __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename Store<S>::C* tab,
                                             int kb, int jb, int ch) {
   using C = typename Store<S>::C;
@@ -186,7 +195,12 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
     bool dirty[3] = {false, false, false};
     if (jk_main && i < d.nx) {
       if (!(ADE && D.eflag[cur.id])) {
-        const C* t = tab + 9 * cur.id;
+        C td[9];
+        if constexpr (DN) {
+#pragma unroll
+          for (int n = 0; n < 9; ++n) td[n] = D.dense[n][idx];
+        }
+        const C* t = DN ? td : tab + 9 * cur.id;
         if (j_in && k_in) {
           v[0] = t[0] * cur.ex + t[1] * (cur.hy - cur.hy_k) + t[2] * (cur.hz - cur.hz_j);
           have[0] = dirty[0] = true;
@@ -211,26 +225,28 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
 
 // One grid: the descriptor is a kernel argument, block -> tile by
 // tile_coords (optionally XCD-aware).
-template <typename S, typename I, bool LDS, bool PML, bool NT>
+// DN: coefficients from the dense per-cell arrays (the ids then only carry
+// the E pass's dispersive flag) instead of the id table.
+template <typename S, typename I, bool LDS, bool PML, bool NT, bool DN = false>
 __global__ void __launch_bounds__(256) k_yee_h3(YeePass<S> D, int xcd) {
   using C = typename Store<S>::C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const C* tab = load_table<C, LDS>(D.tab, D.ntab * 9, reinterpret_cast<C*>(smem_raw));
+  const C* tab = DN ? nullptr : load_table<C, LDS>(D.tab, D.ntab * 9, reinterpret_cast<C*>(smem_raw));
   Tiling T{D.nkb, D.njb, D.nch, D.chunk, xcd};
   int kb, jb, ch;
   tile_coords(T, kb, jb, ch);
-  yee_h3_tile<S, I, PML, NT>(D, tab, kb, jb, ch);
+  yee_h3_tile<S, I, PML, NT, DN>(D, tab, kb, jb, ch);
 }
 
-template <typename S, typename I, bool LDS, bool PML, bool ADE, bool NT>
+template <typename S, typename I, bool LDS, bool PML, bool ADE, bool NT, bool DN = false>
 __global__ void __launch_bounds__(256) k_yee_e3(YeePass<S> D, int xcd) {
   using C = typename Store<S>::C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const C* tab = load_table<C, LDS>(D.tab, D.ntab * 9, reinterpret_cast<C*>(smem_raw));
+  const C* tab = DN ? nullptr : load_table<C, LDS>(D.tab, D.ntab * 9, reinterpret_cast<C*>(smem_raw));
   Tiling T{D.nkb, D.njb, D.nch, D.chunk, xcd};
   int kb, jb, ch;
   tile_coords(T, kb, jb, ch);
-  yee_e3_tile<S, I, PML, ADE, NT>(D, tab, kb, jb, ch);
+  yee_e3_tile<S, I, PML, ADE, NT, DN>(D, tab, kb, jb, ch);
 }
 
 // A batch of grids of different sizes in one persistent launch: the grid is

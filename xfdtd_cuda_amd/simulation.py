@@ -294,6 +294,14 @@ class Simulation:
             self._h, {"dense": 0, "compact": 1}[layout]), "setCoefficientLayout")
         return self
 
+    def set_cell_sizes(self, axis: int, sizes):
+        """A graded axis (0 x, 1 y, 2 z): the sizes of the domain's cells
+        along it from its lower end; PML cells keep the based size."""
+        v = np.ascontiguousarray(sizes, dtype=np.float64)
+        _native.check_host(self._lib.xfh_sim_set_cell_sizes(
+            self._h, axis, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), v.size), "setCellSizes")
+        return self
+
     # ---- set-up and state
     def init(self, time_step: int):
         _native.check_host(self._lib.xfh_sim_init(self._h, time_step), "init")
@@ -450,7 +458,10 @@ class Engine:
     xf_download), for callers that want the plan knobs and timings."""
 
     def __init__(self, sim: Simulation, precision: str = "fp64", use_graph: bool = False,
-                 timing: bool = False, tile_y: int = 0, device: int = 0):
+                 timing: bool = False, tile_y: int = 0, device: int = 0, coeff: str = "auto"):
+        """coeff: "auto" (a table of distinct coefficient rows when there
+        are at most 65536 per pass, else the dense per-cell arrays),
+        "table" or "dense" (the scene's 18 coefficient arrays streamed)."""
         self._lib = _native.hip()
         self._sim = sim
         plan = XfPlan()
@@ -459,6 +470,8 @@ class Engine:
         plan.timing = int(timing)
         plan.tile_y = tile_y
         plan.device = device
+        plan.coeff_mode = {"auto": _native.COEFF_AUTO, "table": _native.COEFF_TABLE,
+                           "dense": _native.COEFF_DENSE}[coeff]
         self._plan = plan
         h = c_vp()
         _native.check_hip(self._lib.xf_create(ctypes.c_void_p(sim.scene_ptr()),
