@@ -67,10 +67,14 @@ def main(argv=None) -> int:
                     help="default: fp64 for the sphere, fp32 for the batch")
     ap.add_argument("--n", type=int, default=512, help="interior cells per edge")
     ap.add_argument("--pml", type=int, default=12)
-    ap.add_argument("--cpu-steps", type=int, default=25,
-                    help="FP64 steps the CPU oracle is timed over (about 11 s at 536^3 with 16 threads)")
+    ap.add_argument("--cpu-steps", type=int, default=5,
+                    help="FP64 steps of each all-core CPU window (three windows, median; plus "
+                         "three one-step single-thread windows), native build of the oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run the plain-load copy kernel (2 GiB) before the steps, so a "
+                         "counter run holds both copy kernels for the FETCH_SIZE scale")
     ap.add_argument("--workload", default="sphere",
                     choices=["sphere", "batch", "precision", "learned", "nf2ff_td",
                              "dispersive", "cylinder2d", "scenes"],
@@ -127,7 +131,7 @@ def main(argv=None) -> int:
         return 1
 
     K, W = args.steps, args.warmup
-    total_steps = W + K + (0 if args.no_cpu else args.cpu_steps) + 1
+    total_steps = W + K + (0 if args.no_cpu else 3 * args.cpu_steps + 3) + 1
     t_setup = time.perf_counter()
     dense = args.coeff == "dense"
     sim = scenes.large_sphere(n=args.n, pml=args.pml, steps=total_steps,
@@ -152,6 +156,9 @@ def main(argv=None) -> int:
         # measured on MI355X (grid-stride copy loops reach ~4.8 TB/s only)
         copy_gbps = _native.stream_bench(2 << 30, 10, 0, 6)
         print(f"# stream copy {copy_gbps:.0f} GB/s", file=sys.stderr, flush=True)
+        if args.calib:
+            plain = _native.stream_bench(2 << 30, 10, 0, 5)
+            print(f"# plain-load copy {plain:.0f} GB/s", file=sys.stderr, flush=True)
 
     eng.run(W)
     eng.set_timing(True)
@@ -192,15 +199,7 @@ def main(argv=None) -> int:
     download_s = time.perf_counter() - t_dn
     cpu = None
     if not args.no_cpu:
-        import oracle
-        threads = oracle.threads()
-        tc = time.perf_counter()
-        oracle.run(sim.scene_ptr(), args.cpu_steps)
-        cpu_s = time.perf_counter() - tc
-        cpu = {"value": cells * args.cpu_steps / cpu_s / 1e9, "unit": "Gcells/s",
-               "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} FP64 steps of the same {nx}x{ny}x{nz} scene "
-                         f"({cpu_s:.1f} s), oracle/xf_oracle.c with {threads} OpenMP threads"}
+        cpu = cpu_baseline(sim, cells, args.cpu_steps, f"{nx}x{ny}x{nz}")
     eng.close()
 
     out = {
@@ -236,6 +235,34 @@ def main(argv=None) -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def cpu_baseline(sim, cells: int, steps: int, shape: str) -> dict:
+    """The oracle (a C restatement of the reference kernels) built on this
+    host with -march=native (no FP contraction), timed on the same scene:
+    three all-core windows of `steps` steps and three one-step single-thread
+    windows, medians."""
+    import os as _os
+    import oracle
+    nthr = len(_os.sched_getaffinity(0)) if hasattr(_os, "sched_getaffinity") else _os.cpu_count()
+    nthr = max(1, min(nthr, int(_os.environ.get("OMP_NUM_THREADS", nthr))))
+    all_s = oracle.time_steps(sim.scene_ptr(), steps, nthr, 3)
+    one_s = oracle.time_steps(sim.scene_ptr(), 1, 1, 3)
+    med_all, med_one = float(np.median(all_s)), float(np.median(one_s))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return {"value": round(cells * steps / med_all / 1e9, 4), "unit": "Gcells/s", "cores": nthr,
+            "kind": "port",
+            "sample": f"{steps} FP64 steps of the same {shape} scene per window, median of 3 "
+                      f"windows ({', '.join(f'{t:.2f}' for t in all_s)} s), oracle/xf_oracle.c "
+                      f"built -O3 -march=native -ffp-contract=off, {nthr} OpenMP threads",
+            "single_thread": round(cells / med_one / 1e9, 4),
+            "single_thread_sample": f"1 step per window, median of 3 ({', '.join(f'{t:.2f}' for t in one_s)} s)",
+            "cpu_model": model}
 
 
 def bench_batch(args) -> int:
@@ -523,11 +550,38 @@ def bench_fp32_vs_fp64(args) -> int:
             freqs = [0.5e9, 1e9, 2e9]
         (nx, ny, nz), _, _ = sim.info()
         eng = xf.Engine(sim, precision=prec, use_graph=args.graph).upload()
-        eng.run(steps)
+        # all but the last 10 steps untimed per kernel, the last 10 with
+        # per-pass HIP events (the roofline's average pass time)
+        eng.run(steps - 10)
         run_ms = eng.timing()["run_ms"]
+        eng.set_timing(True)
+        eng.reset_timing()
+        eng.run(10)
+        tm = eng.timing()
+        run_ms += tm["run_ms"]
+        desc = eng.describe()
         eng.download()
         eng.close()
         out = {"gcells": nx * ny * nz * steps / (run_ms * 1e-3) / 1e9, "ms_per_step": run_ms / steps}
+        sb = 8 if prec == "fp64" else 4
+        cb = 8 if prec == "fp64" else 4
+        ncell = nx * ny * nz
+        # E pass: 3-D reads E3, H3 and writes E3 (9 values) + id; 2-D reads
+        # Ez, Hx, Hy and writes Ez (4 values) + id; plus the CPML psi traffic
+        e_bytes = ncell * ((4 if two_d else 9) * sb + 1) + pml_bytes(sim, "e", cb)
+        e_ms = tm["yee_e_ms"] / tm["yee_e_launches"]
+        out["roofline"] = {"bound": "hbm", "kernel": ("k_yee_e2" if two_d else
+                                                      "k_yee_e3+k_pml_z+k_ade_e3"),
+                           "achieved": round(e_bytes / (e_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS,
+                           "unit": "GB/s",
+                           "frac": round(e_bytes / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                           "traffic": None, "alg_bytes_per_launch": e_bytes,
+                           "avg_launch_ms": round(e_ms, 5), "h_ms": round(tm["yee_h_ms"] / tm["yee_h_launches"], 5),
+                           "tfsf_ms": round(tm["tfsf_ms"] / 10, 5), "nf2ff_ms": round(tm["nf2ff_ms"] / 10, 5),
+                           "note": ("E pass events (the Yee kernel with its CPML, plus the ADE kernel "
+                                    "in 3-D); algorithmic bytes count the fields, ids and psi, not the "
+                                    "ADE J arrays") }
+        out["desc"] = desc
         with tempfile.TemporaryDirectory() as d:
             if two_d:
                 sim.output_far_field(0, np.pi / 2, angles, "xy", d)
@@ -571,7 +625,8 @@ def bench_fp32_vs_fp64(args) -> int:
                    "speedup_vs_fp64": round(res["fp32"]["gcells"] / res["fp64"]["gcells"], 3),
                    "errors_vs_fp64": errs,
                    "max_error": max(errs.values())},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": res["fp32"]["roofline"], "roofline_fp64": res["fp64"]["roofline"],
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
     return 0

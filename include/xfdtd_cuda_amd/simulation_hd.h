@@ -33,6 +33,9 @@ class SimulationHD {
   // Additions (no reference counterpart): arithmetic mode and step capture.
   auto setPrecision(xf_precision p) -> void { _precision = p; }
   auto setUseGraph(bool g) -> void { _use_graph = g; }
+  // check every stored field value every n steps (0: off); run() then throws
+  // std::runtime_error when one is a NaN or an infinity (xf_plan.watchdog)
+  auto setFieldCheckInterval(Index n) -> void { _watchdog = n; }
   auto host() -> xfdtd::Simulation* { return _host; }
 
  private:
@@ -41,6 +44,7 @@ class SimulationHD {
   dim3 _block_dim{1, 1, 1};
   xf_precision _precision{XF_FP64};
   bool _use_graph{false};
+  Index _watchdog{0};
   xf_handle* _handle{nullptr};
   auto check(int status, const char* what) -> void;
 };

@@ -43,7 +43,8 @@ enum xf_status {
   XF_ERR_HIP = 2,         /* HIP runtime error                           */
   XF_ERR_STATE = 3,       /* call out of order (e.g. run before upload)  */
   XF_ERR_UNSUPPORTED = 4, /* configuration the engine does not implement */
-  XF_ERR_NOMEM = 5        /* device or host allocation failed            */
+  XF_ERR_NOMEM = 5,       /* device or host allocation failed            */
+  XF_ERR_NUMERIC = 6      /* a field value became non-finite (watchdog)  */
 };
 
 enum xf_axis { XF_AXIS_X = 0, XF_AXIS_Y = 1, XF_AXIS_Z = 2 };
@@ -301,6 +302,14 @@ typedef struct xf_plan {
   int32_t chunks_x;     /* 0 = auto; x-chunks of the 3-D marching kernels */
   int32_t tiling_flags; /* 0 = auto, else XF_TILE_SET | XF_TILE_* bits    */
   int32_t coeff_mode;   /* enum xf_coeff_mode                              */
+  int32_t watchdog;     /* 0 = off; else every `watchdog` steps (and after
+                           the last step of each xf_run) every stored field
+                           value is checked on the device, and xf_run returns
+                           XF_ERR_NUMERIC, naming the first step checked
+                           with a NaN or an infinity (a diverging run, or a
+                           16-bit storage mode past its range).  The
+                           reference only checks for launch errors at the
+                           end of a run (simulation_hd.cu:251-256). */
 } xf_plan;
 
 /* How the Yee coefficients reach the kernels (dense coefficient input only;

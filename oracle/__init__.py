@@ -53,6 +53,47 @@ def run(scene_ptr: int, n_steps: int) -> None:
         raise RuntimeError("oracle: " + lib().xfo_last_error().decode())
 
 
+_native_lib = None
+
+
+def native_lib() -> ctypes.CDLL:
+    """The oracle built with -march=native for this host (make -C oracle
+    native, run here): the CPU baseline's timing build.  Same C source and
+    evaluation order as lib()."""
+    global _native_lib
+    if _native_lib is None:
+        subprocess.run(["make", "-C", _HERE, "-s", "-B", "native"], check=True)
+        l = ctypes.CDLL(os.path.join(_HERE, "libxf_oracle_native.so"))
+        l.xfo_run.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        l.xfo_run.restype = ctypes.c_int
+        l.xfo_last_error.restype = ctypes.c_char_p
+        l.xfo_num_threads.restype = ctypes.c_int
+        l.xfo_set_num_threads.argtypes = [ctypes.c_int]
+        _native_lib = l
+    return _native_lib
+
+
+def time_steps(scene_ptr: int, n_steps: int, threads: int, repeats: int = 3):
+    """Wall time of n_steps oracle steps on the native build with `threads`
+    OpenMP threads, `repeats` windows one after another (the scene advances);
+    returns the list of seconds per window."""
+    import time
+    l = native_lib()
+    keep = l.xfo_num_threads()
+    l.xfo_set_num_threads(threads)
+    out = []
+    try:
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            st = l.xfo_run(ctypes.c_void_p(scene_ptr), n_steps)
+            out.append(time.perf_counter() - t0)
+            if st != 0:
+                raise RuntimeError("oracle: " + l.xfo_last_error().decode())
+    finally:
+        l.xfo_set_num_threads(keep)
+    return out
+
+
 def set_threads(n: int) -> None:
     lib().xfo_set_num_threads(n)
 

@@ -44,6 +44,7 @@ class XfPlan(ctypes.Structure):
         ("chunks_x", c_i32),
         ("tiling_flags", c_i32),
         ("coeff_mode", c_i32),
+        ("watchdog", c_i32),
     ]
 
 

@@ -4,6 +4,7 @@
 // std::runtime_error.
 #include "../../../include/xfdtd_cuda_amd/simulation_hd.h"
 
+#include <algorithm>
 #include <string>
 
 namespace xfdtd::cuda {
@@ -39,6 +40,7 @@ auto SimulationHD::copyHostToDevice() -> void {
   plan.block_dim[1] = static_cast<int32_t>(_block_dim.y);
   plan.block_dim[2] = static_cast<int32_t>(_block_dim.z);
   plan.use_graph = _use_graph ? 1 : 0;
+  plan.watchdog = static_cast<int32_t>(std::min<Index>(_watchdog, 0x7fffffff));
   check(xf_create(_host->scene(), &plan, &_handle), "copyHostToDevice");
   check(xf_upload(_handle), "copyHostToDevice");
 }

@@ -498,6 +498,26 @@ int Batch::run(int64_t n) {
   XF_TRY(runs.create(2));
   hipEvent_t run0 = runs[0], run1 = runs[1];
   int status = XF_OK;
+  // the watchdog checks each grid's fields (Engine::launch_watch) on the
+  // batch's stream
+  const int64_t wd = plan.watchdog > 0 ? plan.watchdog : 0;
+  auto watch = [&](int64_t done) -> int {
+    for (auto& h : inst) {
+      Engine& E = h->eng;
+      int st;
+      switch (plan.precision) {
+        case XF_FP64: st = E.launch_watch<double>(cur_step + done, stream); break;
+        case XF_FP32: st = E.launch_watch<float>(cur_step + done, stream); break;
+        case XF_BF16_COMP: st = E.launch_watch<Bf16c>(cur_step + done, stream); break;
+        case XF_FP16_STORE: st = E.launch_watch<__half>(cur_step + done, stream); break;
+        default: st = E.launch_watch<__hip_bfloat16>(cur_step + done, stream);
+      }
+      if (st != XF_OK) return st;
+    }
+    return XF_OK;
+  };
+  if (wd)
+    for (auto& h : inst) hipLaunchKernelGGL(k_watch_clear, dim3(1), dim3(64), 0, stream, h->eng.wd_dev);
   if (graph_on) {
     if (!graph_ready) {
       XF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -510,16 +530,32 @@ int Batch::run(int64_t n) {
     XF_HIP(hipMemcpyAsync(t_dev, &t0, sizeof(int), hipMemcpyHostToDevice, stream));
     XF_HIP(hipStreamSynchronize(stream));
     XF_HIP(hipEventRecord(run0, stream));
-    for (int64_t s = 0; s < n; ++s) XF_HIP(hipGraphLaunch(graph_exec, stream));
+    for (int64_t s = 0; s < n; ++s) {
+      XF_HIP(hipGraphLaunch(graph_exec, stream));
+      if (wd && ((cur_step + s + 1) % wd == 0 || s + 1 == n)) XF_TRY(watch(s + 1));
+    }
   } else {
     XF_HIP(hipEventRecord(run0, stream));
-    for (int64_t s = 0; s < n && status == XF_OK; ++s)
+    for (int64_t s = 0; s < n && status == XF_OK; ++s) {
       status = step(static_cast<int>(cur_step + s), false, timing_on, timing_on ? &ev[5 * s] : nullptr);
+      if (status == XF_OK && wd && ((cur_step + s + 1) % wd == 0 || s + 1 == n)) status = watch(s + 1);
+    }
   }
   XF_HIP(hipEventRecord(run1, stream));
   XF_HIP(hipEventSynchronize(run1));
   if (status != XF_OK) return status;
   XF_HIP(hipGetLastError());
+  int bad_grid = -1;
+  unsigned first_bad = 0xffffffffu;
+  if (wd)
+    for (size_t g = 0; g < inst.size(); ++g) {
+      unsigned f = 0xffffffffu;
+      XF_HIP(hipMemcpy(&f, inst[g]->eng.wd_dev, sizeof(unsigned), hipMemcpyDeviceToHost));
+      if (f < first_bad) {
+        first_bad = f;
+        bad_grid = static_cast<int>(g);
+      }
+    }
   float ms = 0;
   XF_HIP(hipEventElapsedTime(&ms, run0, run1));
   timing.run_ms = ms;
@@ -546,6 +582,9 @@ int Batch::run(int64_t n) {
   timing.bytes_per_cell_step = bytes / static_cast<double>(cells);
   cur_step += n;
   sync_steps();
+  if (bad_grid >= 0)
+    return fail(XF_ERR_NUMERIC, "watchdog: non-finite field value in grid " + std::to_string(bad_grid) +
+                                    " found after step " + std::to_string(first_bad));
   return XF_OK;
 }
 

@@ -213,6 +213,10 @@ struct Engine {
   std::vector<int64_t> ade_cells;  // host (i,j,k) linear index in (nx+1)(ny+1)(nz+1)
   // time counter for graph replay
   int* t_dev{};
+  // watchdog: first step found with a non-finite field value (UINT_MAX: none)
+  unsigned* wd_dev{};
+  template <typename S>
+  int launch_watch(int64_t step, hipStream_t st);
   hipGraph_t graph{};
   hipGraphExec_t graph_exec{};
   bool graph_ready{false};
@@ -1618,6 +1622,8 @@ int Engine::upload() {
   void* tp;
   XF_TRY(alloc(&tp, sizeof(int)));
   t_dev = static_cast<int*>(tp);
+  XF_TRY(alloc(&tp, 64));
+  wd_dev = static_cast<unsigned*>(tp);
   set_tiling(plan.tile_y, plan.chunks_x, plan.tiling_flags);
   XF_HIP(hipDeviceSynchronize());
   uploaded = true;
@@ -1867,6 +1873,19 @@ Dims make_dims(const Engine& E) {
   return d;
 }
 }  // namespace
+
+// The watchdog's check of every stored field value after step `step`.
+template <typename S>
+int Engine::launch_watch(int64_t step, hipStream_t st) {
+  int cus = 0;
+  XF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  const int64_t want = (total + 255) / 256;
+  const unsigned blocks = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(want, 8 * cus)));
+  hipLaunchKernelGGL((k_watch<S>), dim3(blocks), dim3(256), 0, st, make_fields<S>(*this), total,
+                     static_cast<unsigned>(std::min<int64_t>(step, 0xfffffffe)), wd_dev);
+  XF_HIP(hipGetLastError());
+  return XF_OK;
+}
 
 template <typename S>
 YeePass<S> Engine::yee_pass(bool is_e) const {
@@ -2264,6 +2283,17 @@ int Engine::run(int64_t n) {
   XF_TRY(runs.create(2));
   hipEvent_t run0 = runs[0], run1 = runs[1];
   if (timing_on) XF_TRY(ev.create(5 * n));
+  const int64_t wd = plan.watchdog > 0 ? plan.watchdog : 0;
+  auto watch = [&](int64_t done) -> int {  // after `done` steps of this run
+    switch (plan.precision) {
+      case XF_FP64: return launch_watch<double>(cur_step + done, stream);
+      case XF_FP32: return launch_watch<float>(cur_step + done, stream);
+      case XF_BF16_COMP: return launch_watch<Bf16c>(cur_step + done, stream);
+      case XF_FP16_STORE: return launch_watch<__half>(cur_step + done, stream);
+      default: return launch_watch<__hip_bfloat16>(cur_step + done, stream);
+    }
+  };
+  if (wd) hipLaunchKernelGGL(k_watch_clear, dim3(1), dim3(64), 0, stream, wd_dev);
   XF_HIP(hipEventRecord(run0, stream));
   int status = XF_OK;
   auto step = [&](int64_t s) -> int {
@@ -2310,14 +2340,22 @@ int Engine::run(int64_t n) {
     XF_HIP(hipMemcpyAsync(t_dev, &t0, sizeof(int), hipMemcpyHostToDevice, stream));
     XF_HIP(hipStreamSynchronize(stream));
     XF_HIP(hipEventRecord(run0, stream));
-    for (int64_t s = 0; s < n; ++s) XF_HIP(hipGraphLaunch(graph_exec, stream));
+    for (int64_t s = 0; s < n; ++s) {
+      XF_HIP(hipGraphLaunch(graph_exec, stream));
+      if (wd && ((cur_step + s + 1) % wd == 0 || s + 1 == n)) XF_TRY(watch(s + 1));
+    }
   } else {
-    for (int64_t s = 0; s < n && status == XF_OK; ++s) status = step(s);
+    for (int64_t s = 0; s < n && status == XF_OK; ++s) {
+      status = step(s);
+      if (status == XF_OK && wd && ((cur_step + s + 1) % wd == 0 || s + 1 == n)) status = watch(s + 1);
+    }
   }
   XF_HIP(hipEventRecord(run1, stream));
   XF_HIP(hipEventSynchronize(run1));
   if (status != XF_OK) return status;
   XF_HIP(hipGetLastError());
+  unsigned first_bad = 0xffffffffu;
+  if (wd) XF_HIP(hipMemcpy(&first_bad, wd_dev, sizeof(unsigned), hipMemcpyDeviceToHost));
   float ms = 0;
   XF_HIP(hipEventElapsedTime(&ms, run0, run1));
   timing.run_ms = ms;
@@ -2341,6 +2379,9 @@ int Engine::run(int64_t n) {
   timing.bytes_per_cell_step = bytes_per_cell_step();
   cur_step += n;
   if (has_lb && lbd.mode == XF_LB_RECORD) lb_rec_count = lb_recorded_after(cur_step);
+  if (first_bad != 0xffffffffu)
+    return fail(XF_ERR_NUMERIC, "watchdog: non-finite field value found after step " + std::to_string(first_bad) +
+                                    " (the run went on to step " + std::to_string(cur_step) + ")");
   return XF_OK;
 }
 

@@ -1169,6 +1169,32 @@ __global__ void __launch_bounds__(256)
 
 __global__ void k_next_step(int* t) { *t += 1; }
 
+// The watchdog: every stored field value is read back and any NaN or
+// infinity lowers *first to the step t being checked (*first starts at
+// UINT_MAX, set by k_watch_clear; a fill kernel, not a memset, so a caller
+// may capture it).
+__global__ void k_watch_clear(unsigned* __restrict__ first) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *first = 0xffffffffu;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_watch(Fields<S> f, int64_t total, unsigned t, unsigned* __restrict__ first) {
+  using C = typename Store<S>::C;
+  const S* arr[6] = {f.e[0], f.e[1], f.e[2], f.h[0], f.h[1], f.h[2]};
+  bool bad = false;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int c = 0; c < 6; ++c) {
+    const S* p = arr[c];
+    if (!p) continue;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+      const C v = Store<S>::ld(p, i, (C)1);
+      bad |= !isfinite(v);
+    }
+  }
+  if (bad) atomicMin(first, t);
+}
+
 // Plain streaming copy, 16 bytes per lane, grid-stride, U independent loads
 // in flight per lane, optionally non-temporal.
 typedef double xf_d2 __attribute__((ext_vector_type(2)));

@@ -458,7 +458,8 @@ class Engine:
     xf_download), for callers that want the plan knobs and timings."""
 
     def __init__(self, sim: Simulation, precision: str = "fp64", use_graph: bool = False,
-                 timing: bool = False, tile_y: int = 0, device: int = 0, coeff: str = "auto"):
+                 timing: bool = False, tile_y: int = 0, device: int = 0, coeff: str = "auto",
+                 watchdog: int = 0):
         """coeff: "auto" (a table of distinct coefficient rows when there
         are at most 65536 per pass, else the dense per-cell arrays),
         "table" or "dense" (the scene's 18 coefficient arrays streamed)."""
@@ -472,6 +473,9 @@ class Engine:
         plan.device = device
         plan.coeff_mode = {"auto": _native.COEFF_AUTO, "table": _native.COEFF_TABLE,
                            "dense": _native.COEFF_DENSE}[coeff]
+        # every `watchdog` steps the fields are checked for NaN/Inf on the
+        # device; run() then raises XFError (status XF_ERR_NUMERIC)
+        plan.watchdog = watchdog
         self._plan = plan
         h = c_vp()
         _native.check_hip(self._lib.xf_create(ctypes.c_void_p(sim.scene_ptr()),
@@ -553,7 +557,7 @@ class Batch:
     one for all NF2FF boxes per step, optionally replayed as a hipGraph."""
 
     def __init__(self, sims, precision: str = "fp64", use_graph: bool = False,
-                 timing: bool = False, device: int = 0):
+                 timing: bool = False, device: int = 0, watchdog: int = 0):
         self._lib = _native.hip()
         self._sims = list(sims)
         plan = XfPlan()
@@ -561,6 +565,7 @@ class Batch:
         plan.use_graph = int(use_graph)
         plan.timing = int(timing)
         plan.device = device
+        plan.watchdog = watchdog
         ptrs = (c_vp * len(self._sims))(*[s.scene_ptr() for s in self._sims])
         h = c_vp()
         self._h = None
