@@ -267,30 +267,62 @@ __device__ __forceinline__ int batch_instance(const int* __restrict__ tile_start
   return lo;
 }
 
+// Each wave keeps the table of the grid it is working on in its own slice
+// of LDS (kBatchTab values, written by the wave when its grid changes; no
+// block barrier), read like the single-grid kernels' staged table; grids
+// with larger tables read theirs through L1.
+constexpr int kBatchTab = 9 * 16;
+
+template <typename C>
+__device__ __forceinline__ const C* batch_table(const C* __restrict__ g_tab, int ntab, C* mine, int g,
+                                                int& cur_g) {
+  if (ntab * 9 > kBatchTab) return g_tab;
+  if (g != cur_g) {
+    const int lane = (threadIdx.x + blockDim.x * threadIdx.y) & 63;
+    __builtin_amdgcn_wave_barrier();  // the wave's reads of the previous table are issued
+    for (int n = lane; n < ntab * 9; n += 64) mine[n] = g_tab[n];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    cur_g = g;
+  }
+  return mine;
+}
+
 template <typename S, typename I, bool PML, bool NT>
 __global__ void __launch_bounds__(256)
     k_yee_h3_batch(const YeePass<S>* __restrict__ passes, const int* __restrict__ tile_start, int n) {
+  using C = typename Store<S>::C;
+  __shared__ C wtab[4][kBatchTab];
+  C* mine = wtab[(threadIdx.x + blockDim.x * threadIdx.y) >> 6];
+  int cur_g = -1;
   const int total = tile_start[n];
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
     const int g = batch_instance(tile_start, n, t);
     const YeePass<S>& D = passes[g];
+    const C* tab = batch_table<C>(D.tab, D.ntab, mine, g, cur_g);
     const int local = t - tile_start[g];
     const int kb = local % D.nkb;
     const int rest = local / D.nkb;
-    yee_h3_tile<S, I, PML, NT>(D, D.tab, kb, rest % D.njb, rest / D.njb);
+    yee_h3_tile<S, I, PML, NT>(D, tab, kb, rest % D.njb, rest / D.njb);
   }
 }
 
 template <typename S, typename I, bool PML, bool NT>
 __global__ void __launch_bounds__(256)
     k_yee_e3_batch(const YeePass<S>* __restrict__ passes, const int* __restrict__ tile_start, int n) {
+  using C = typename Store<S>::C;
+  __shared__ C wtab[4][kBatchTab];
+  C* mine = wtab[(threadIdx.x + blockDim.x * threadIdx.y) >> 6];
+  int cur_g = -1;
   const int total = tile_start[n];
   for (int t = blockIdx.x; t < total; t += gridDim.x) {
     const int g = batch_instance(tile_start, n, t);
     const YeePass<S>& D = passes[g];
+    const C* tab = batch_table<C>(D.tab, D.ntab, mine, g, cur_g);
     const int local = t - tile_start[g];
     const int kb = local % D.nkb;
     const int rest = local / D.nkb;
-    yee_e3_tile<S, I, PML, false, NT>(D, D.tab, kb, rest % D.njb, rest / D.njb);
+    yee_e3_tile<S, I, PML, false, NT>(D, tab, kb, rest % D.njb, rest / D.njb);
   }
 }
