@@ -103,6 +103,9 @@ def main(argv=None) -> int:
                          "streamed by the kernels (XF_COEFF_DENSE)")
     ap.add_argument("--td-boxes", type=int, default=3, help="NF2FF-TD boxes of nf2ff_td")
     ap.add_argument("--count", type=int, default=64, help="grids of the batch workload")
+    ap.add_argument("--full-steps", type=int, default=2000,
+                    help="batch workload: steps of the full runs whose far fields are compared "
+                         "(FP32 batch and the learned-layer batch against the FP64 batch; 0 skips)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -346,6 +349,8 @@ def bench_batch(args) -> int:
         kernel, avg_ms, alg_bytes = "k_yee_h3_batch", h_ms, bytes_h
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
 
+    accuracy = batch_far_field(args, prec, w16) if args.full_steps > 0 else None
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -377,7 +382,8 @@ def bench_batch(args) -> int:
                        "ms_per_step": round(run_lb / K, 4), "cells": cells_lb,
                        "speedup_vs_cpml8_same_spheres": round(run_ms / run_lb, 3),
                        "tuning": best_lb},
-                   "speedup_vs_fp64": round(value / value64, 3), "fp64_tuning": best64},
+                   "speedup_vs_fp64": round(value / value64, 3), "fp64_tuning": best64,
+                   "far_field_accuracy": accuracy},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
@@ -448,6 +454,80 @@ def bench_precision(args) -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _rcs_cut(sim, phi: float, n: int = 360) -> np.ndarray:
+    """RCS over n angles of theta in the phi plane at 1 GHz (the box's
+    first frequency), normalised by the incident spectrum."""
+    angles = np.linspace(-np.pi, np.pi, n)
+    inc = sim.incident_spectrum(1e9)
+    return np.asarray(sim.far_field(0, 0, angles, phi, inc)[2])
+
+
+def batch_far_field(args, prec: str, w16) -> dict:
+    """The north star's accuracy condition for the batch: every grid of the
+    batch run to the end (`--full-steps`) in `prec` and in FP64 (bit-identical
+    to the oracle), and the same spheres bounded by the learned 2-cell layer
+    instead of the 8-cell CPML; RCS (xz and yz cuts at 1 GHz) relative L2
+    error of each grid against its FP64 CPML run, max and mean over grids."""
+    import xfdtd_cuda_amd as xf
+    from xfdtd_cuda_amd import scenes
+
+    steps = args.full_steps
+    t0 = time.perf_counter()
+
+    def run(sims, precision, label):
+        b = xf.Batch(sims, precision=precision).upload()
+        b.autotune(trial_steps=2)
+        ms = 0.0
+        for _ in range(steps // 250):  # in pieces, so the run reports progress
+            b.run(250)
+            ms += b.timing()["run_ms"]
+            print(f"#   {label}: {ms / 1e3:.1f} s of steps", file=sys.stderr, flush=True)
+        if steps % 250:
+            b.run(steps % 250)
+            ms += b.timing()["run_ms"]
+        b.download()
+        b.close()
+        return ms / steps
+
+    def cuts(sims, label):
+        # 72 angles per cut (every 5 degrees): the host far-field sum is the
+        # slow part of this check
+        out = []
+        for q, s in enumerate(sims):
+            out.append((_rcs_cut(s, 0.0, 72), _rcs_cut(s, np.pi / 2, 72)))
+            if q % 16 == 15:
+                print(f"#   {label}: far fields of {q + 1} grids", file=sys.stderr, flush=True)
+        return out
+
+    ref = scenes.batch_spheres(count=args.count, steps=steps)
+    ms64 = run(ref, "fp64", "fp64")
+    r64 = cuts(ref, "fp64")
+    del ref
+    red = scenes.batch_spheres(count=args.count, steps=steps)
+    msr = run(red, prec, prec)
+    rr = cuts(red, prec)
+    del red
+    lb = scenes.batch_spheres(count=args.count, steps=steps, pml=2, lb_weights=w16.view(np.float16))
+    msl = run(lb, prec, "learned2")
+    rl = cuts(lb, "learned2")
+    del lb
+
+    def errs(test):
+        e = [max(float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(t, r))
+             for t, r in zip(test, r64)]
+        return {"max": max(e), "mean": float(np.mean(e)), "median": float(np.median(e))}
+
+    out = {"steps": steps, "reference": "the same grids in FP64 with the 8-cell CPML (the batch's FP64 "
+                                        "results equal the oracle's bit for bit)",
+           f"{prec}_rcs_rel_l2": errs(rr), "learned2_rcs_rel_l2": errs(rl),
+           "ms_per_step": {"fp64": round(ms64, 4), prec: round(msr, 4), "learned2": round(msl, 4)},
+           "speedup_vs_fp64": {prec: round(ms64 / msr, 3), "learned2": round(ms64 / msl, 3)},
+           "bar": 1e-3}
+    print(f"# far fields after {steps} steps ({time.perf_counter() - t0:.0f} s): {out}", file=sys.stderr,
+          flush=True)
+    return out
 
 
 def bench_learned(args) -> int:
