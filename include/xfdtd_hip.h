@@ -335,6 +335,10 @@ enum xf_tile_flags {
   XF_TILE_K32 = 16,      /* 32-lane k-tiles with two j-rows per wave (fills
                             more lanes when nz + 1 is just above a multiple
                             of 64); the default is 64-lane k-tiles */
+  XF_TILE_PML_STREAMS = 32, /* a pass's CPML-free tiles and its CPML tiles
+                            in two launches on two streams, running
+                            together (the CPML-free launch without the
+                            CPML code and its registers); 3-D, table mode */
   XF_TILE_SET = 256      /* the other bits are explicit, not auto        */
 };
 

@@ -134,6 +134,38 @@ def test_tiling_variants_bit_identical(gpu_available, tile_y, chunks, xcd, nt, s
     _compare(g, c, label=f"tiling ty={tile_y} ch={chunks}")
 
 
+@pytest.mark.parametrize("tile_y,chunks,split,k32,graph", [
+    (1, 0, True, False, False), (2, 3, False, False, False), (1, 1000, True, True, False),
+    (4, 5, True, False, True)])
+def test_pml_streams_bit_identical(gpu_available, tile_y, chunks, split, k32, graph):
+    """CPML-free tiles and CPML tiles of each pass in two launches on two
+    streams: the same bits as the oracle, eagerly and as a replayed graph."""
+    build = lambda: scenes.dielectric_sphere(n=21, pml=5, steps=60, tfsf_inset=8, nf2ff_inset=6)
+    g, c = build(), build()
+    eng = xf.Engine(g, use_graph=graph).upload()
+    eng.set_tiling(tile_y=tile_y, chunks_x=chunks, nontemporal=True, pml_split=split, k32=k32,
+                   pml_streams=True)
+    assert "pml_streams=1" in eng.describe()
+    eng.run(20).run(40).download()
+    eng.close()
+    oracle.run(c.scene_ptr(), 60)
+    for n in _state_names(c):
+        assert np.array_equal(g.array(n), c.array(n)), f"{n} differs"
+
+
+def test_pml_streams_dispersive(gpu_available):
+    """The two-stream E pass with ADE cells (their Yee update skipped, the
+    ADE kernel after the join)."""
+    build = _dispersive("mix")
+    g, c = build(), build()
+    eng = xf.Engine(g).upload()
+    eng.set_tiling(tile_y=1, nontemporal=True, pml_split=True, pml_streams=True)
+    eng.run(160).download()
+    eng.close()
+    oracle.run(c.scene_ptr(), 160)
+    _compare(g, c, label="pml streams ade")
+
+
 def test_second_upload_resets_device_state(gpu_available):
     """xf_upload twice on one handle (what a batch does when it widens the
     coefficient ids): the second upload frees and rebuilds every device

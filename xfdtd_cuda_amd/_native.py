@@ -55,6 +55,7 @@ TILE_XCD_REMAP = 1
 TILE_NT = 4
 TILE_PML_SPLIT = 8
 TILE_K32 = 16
+TILE_PML_STREAMS = 32
 TILE_SET = 256
 
 

@@ -229,6 +229,17 @@ struct Engine {
   int nt_streams{1};
   int pml_split{0};  // XF_TILE_PML_SPLIT requested (see split_z())
   int klanes{64};    // k-lanes of a tile row: 64, or 32 (XF_TILE_K32)
+  // XF_TILE_PML_STREAMS: per pass a tile list, CPML-free tiles first, then
+  // the tiles that meet a face the Yee kernel carries; the CPML part runs on
+  // stream2 beside the rest
+  int pml_streams{0};
+  hipStream_t stream2{};
+  hipEvent_t ev_fork{}, ev_join{};
+  void* tlist[2]{};
+  int64_t tlist_cap[2]{0, 0};
+  int n_int[2]{0, 0}, n_pml[2]{0, 0};
+  int build_tile_lists();
+  bool use_lists() const { return pml_streams && dim == 3 && !dense && tlist[0] && tlist[1]; }
   int rows() const { return tile_y * (64 / klanes); }  // j-rows of a block
   void set_tiling(int ty, int chunks_x, int flags);
   Tiling tiling() const;
@@ -261,6 +272,13 @@ struct Engine {
     backup.clear();
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
+    if (stream2) (void)hipStreamDestroy(stream2);
+    stream2 = nullptr;
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    ev_fork = ev_join = nullptr;
+    tlist[0] = tlist[1] = nullptr;
+    tlist_cap[0] = tlist_cap[1] = 0;
     uploaded = false;
   }
 
@@ -1594,6 +1612,9 @@ int Engine::upload() {
   release();
   XF_HIP(hipSetDevice(device));
   XF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  XF_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+  XF_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  XF_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   esize = plan.precision == XF_FP64 ? 8 : plan.precision == XF_FP32 ? 4 : plan.precision == XF_BF16_COMP ? 3 : 2;
   // rows start on a 128-byte boundary (a 64-value block for the 3-byte
   // compensated format, whose layout is blocked by 64)
@@ -1647,12 +1668,14 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
     nt_streams = (flags & XF_TILE_NT) ? 1 : 0;
     pml_split = (flags & XF_TILE_PML_SPLIT) ? 1 : 0;
     klanes = (flags & XF_TILE_K32) ? 32 : 64;
+    pml_streams = (flags & XF_TILE_PML_STREAMS) ? 1 : 0;
   } else {
     // defaults: non-temporal streams; z-face CPML corrections split off
     // (measured at 536^3: FP32 +3 %, FP64 even; 64 small grids: +13 %)
     xcd = 0;
     nt_streams = 1;
     pml_split = 1;
+    pml_streams = 0;
     klanes = 64;
   }
   if (dim == 3) {
@@ -1672,13 +1695,14 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
   graph_exec = nullptr;
   graph = nullptr;
   graph_ready = false;
+  if (pml_streams && dim == 3 && stream && build_tile_lists() != XF_OK) pml_streams = 0;
   layout_desc = std::string("dim=") + std::to_string(dim) + " pad_stride=" + std::to_string(dim == 3 ? sy : sx) +
                 " id_bytes=" + std::to_string(id_bytes) + " coeff=" + (dense ? "dense" : "table") +
                 " etab=" + std::to_string(ntab_e) +
                 " htab=" + std::to_string(ntab_h) + " chunk=" + std::to_string(chunk) +
                 " nchunk=" + std::to_string(nch_eff) + " tile_y=" + std::to_string(tile_y) +
                 " xcd=" + std::to_string(xcd) + " nt=" + std::to_string(nt_streams) + " pml_split=" + std::to_string(split_z() ? 1 : 0) +
-                " klanes=" + std::to_string(klanes) +
+                " klanes=" + std::to_string(klanes) + " pml_streams=" + std::to_string(pml_streams) +
                 " pml_faces=" + std::to_string(pml.size()) +
                 " ade_cells=" + std::to_string(ade_cells.size());
 }
@@ -1887,6 +1911,58 @@ int Engine::launch_watch(int64_t step, hipStream_t st) {
   return XF_OK;
 }
 
+// The tile lists of XF_TILE_PML_STREAMS: a tile is a CPML tile when its
+// (k, j) lanes and its x-chunk meet the corrected box of a face the pass's
+// Yee kernel carries (the same test as pml_mask, per tile).
+int Engine::build_tile_lists() {
+  const int64_t ntile = static_cast<int64_t>(nkb) * njb * nch_eff;
+  if (ntile > INT_MAX) return fail(XF_ERR_UNSUPPORTED, "too many tiles for the tile lists");
+  const int64_t rws = rows();
+  for (int pass = 0; pass < 2; ++pass) {
+    PmlPass<double> P{};
+    fill_pml_pass<double>(*this, P, pass == 1, split_z() ? 1 : 0);
+    std::vector<unsigned char> mark(ntile, 0);
+    auto lo_hi = [](int64_t a, int64_t b, int64_t w, int64_t n_tiles, int64_t& t0, int64_t& t1) {
+      // tiles of width w meeting the closed range [a, b]
+      t0 = std::max<int64_t>(0, a / w);
+      t1 = std::min<int64_t>(n_tiles - 1, b / w);
+    };
+    for (int q = 0; q < P.n; ++q) {
+      const PmlFace<double>& F = P.f[q];
+      int64_t i_a, i_b, j_a, j_b, k_a, k_b;  // closed node ranges the face can touch
+      if (F.axis == 0) {
+        i_a = F.c0; i_b = F.c1 - 1; j_a = F.a0; j_b = F.a1; k_a = F.b0; k_b = F.b1;
+      } else if (F.axis == 1) {
+        j_a = F.c0; j_b = F.c1 - 1; k_a = F.a0; k_b = F.a1; i_a = F.b0; i_b = F.b1;
+      } else {
+        k_a = F.c0; k_b = F.c1 - 1; i_a = F.a0; i_b = F.a1; j_a = F.b0; j_b = F.b1;
+      }
+      if (i_b < i_a || j_b < j_a || k_b < k_a) continue;
+      int64_t c0, c1, jb0, jb1, kb0, kb1;
+      lo_hi(i_a, i_b, chunk, nch_eff, c0, c1);
+      lo_hi(j_a, j_b, rws, njb, jb0, jb1);
+      lo_hi(k_a, k_b, klanes, nkb, kb0, kb1);
+      for (int64_t c = c0; c <= c1; ++c)
+        for (int64_t jb = jb0; jb <= jb1; ++jb)
+          for (int64_t kb = kb0; kb <= kb1; ++kb) mark[kb + nkb * (jb + njb * c)] = 1;
+    }
+    std::vector<int> list;
+    list.reserve(ntile);
+    for (int64_t t = 0; t < ntile; ++t)
+      if (!mark[t]) list.push_back(static_cast<int>(t));
+    n_int[pass] = static_cast<int>(list.size());
+    for (int64_t t = 0; t < ntile; ++t)
+      if (mark[t]) list.push_back(static_cast<int>(t));
+    n_pml[pass] = static_cast<int>(list.size()) - n_int[pass];
+    if (tlist_cap[pass] < ntile) {
+      XF_TRY(alloc(&tlist[pass], static_cast<size_t>(ntile) * sizeof(int)));
+      tlist_cap[pass] = ntile;
+    }
+    XF_HIP(hipMemcpy(tlist[pass], list.data(), list.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  return XF_OK;
+}
+
 template <typename S>
 YeePass<S> Engine::yee_pass(bool is_e) const {
   using C = typename Store<S>::C;
@@ -2021,6 +2097,32 @@ int Engine::launch_h(int t) {
   } while (0)
     if (dense) {
       if (has_pml) XF_LH3D(true); else XF_LH3D(false);
+    } else if (use_lists()) {
+      // CPML tiles on stream2, the rest (no CPML code) on the engine's stream
+      const int* li = static_cast<const int*>(tlist[0]);
+      XF_HIP(hipEventRecord(ev_fork, stream));
+      XF_HIP(hipStreamWaitEvent(stream2, ev_fork, 0));
+#define XF_LHL(I, L, PM, ST, N, LIST)                                                                    \
+  do {                                                                                                 \
+    if (nt_streams)                                                                                    \
+      hipLaunchKernelGGL((k_yee_h3_list<S, I, L, PM, true>), dim3(N), block, shm, ST, D, LIST);         \
+    else                                                                                               \
+      hipLaunchKernelGGL((k_yee_h3_list<S, I, L, PM, false>), dim3(N), block, shm, ST, D, LIST);        \
+  } while (0)
+#define XF_LHL2(I, L)                                                                                  \
+  do {                                                                                                 \
+    if (n_pml[0] > 0) XF_LHL(I, L, true, stream2, n_pml[0], li + n_int[0]);                            \
+    if (n_int[0] > 0) XF_LHL(I, L, false, stream, n_int[0], li);                                       \
+  } while (0)
+      if (id_bytes == 1) {
+        if (lds) XF_LHL2(uint8_t, true); else XF_LHL2(uint8_t, false);
+      } else {
+        if (lds) XF_LHL2(uint16_t, true); else XF_LHL2(uint16_t, false);
+      }
+#undef XF_LHL2
+#undef XF_LHL
+      XF_HIP(hipEventRecord(ev_join, stream2));
+      XF_HIP(hipStreamWaitEvent(stream, ev_join, 0));
     } else if (id_bytes == 1) {
       if (lds) { if (has_pml) XF_LH3(uint8_t, true, true); else XF_LH3(uint8_t, true, false); }
       else { if (has_pml) XF_LH3(uint8_t, false, true); else XF_LH3(uint8_t, false, false); }
@@ -2088,6 +2190,33 @@ int Engine::launch_e(int t) {
     if (dense) {
       if (has_pml) { if (has_ade) XF_LE3D(true, true); else XF_LE3D(true, false); }
       else { if (has_ade) XF_LE3D(false, true); else XF_LE3D(false, false); }
+    } else if (use_lists()) {
+      const int* li = static_cast<const int*>(tlist[1]);
+      XF_HIP(hipEventRecord(ev_fork, stream));
+      XF_HIP(hipStreamWaitEvent(stream2, ev_fork, 0));
+#define XF_LEL(I, L, PM, AD, ST, N, LIST)                                                                   \
+  do {                                                                                                    \
+    if (nt_streams)                                                                                       \
+      hipLaunchKernelGGL((k_yee_e3_list<S, I, L, PM, AD, true>), dim3(N), block, shm, ST, D, LIST);        \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_yee_e3_list<S, I, L, PM, AD, false>), dim3(N), block, shm, ST, D, LIST);       \
+  } while (0)
+#define XF_LEL2(I, L, AD)                                                                                 \
+  do {                                                                                                    \
+    if (n_pml[1] > 0) XF_LEL(I, L, true, AD, stream2, n_pml[1], li + n_int[1]);                           \
+    if (n_int[1] > 0) XF_LEL(I, L, false, AD, stream, n_int[1], li);                                      \
+  } while (0)
+#define XF_LEL3(I, L) do { if (has_ade) XF_LEL2(I, L, true); else XF_LEL2(I, L, false); } while (0)
+      if (id_bytes == 1) {
+        if (lds) XF_LEL3(uint8_t, true); else XF_LEL3(uint8_t, false);
+      } else {
+        if (lds) XF_LEL3(uint16_t, true); else XF_LEL3(uint16_t, false);
+      }
+#undef XF_LEL3
+#undef XF_LEL2
+#undef XF_LEL
+      XF_HIP(hipEventRecord(ev_join, stream2));
+      XF_HIP(hipStreamWaitEvent(stream, ev_join, 0));
     } else if (id_bytes == 1) {
       if (lds) { if (has_pml) XF_LE3A(uint8_t, true, true); else XF_LE3A(uint8_t, true, false); }
       else { if (has_pml) XF_LE3A(uint8_t, false, true); else XF_LE3A(uint8_t, false, false); }

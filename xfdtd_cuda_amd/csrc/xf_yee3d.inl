@@ -249,6 +249,29 @@ __global__ void __launch_bounds__(256) k_yee_e3(YeePass<S> D, int xcd) {
   yee_e3_tile<S, I, PML, ADE, NT, DN>(D, tab, kb, jb, ch);
 }
 
+// Tiles taken from a list (tile id kb + nkb (jb + njb ch)): the engine
+// launches a pass's CPML-free tiles and its CPML tiles apart, on two streams,
+// so the CPML-free ones run in the leaner PML = false instantiation.
+template <typename S, typename I, bool LDS, bool PML, bool NT>
+__global__ void __launch_bounds__(256) k_yee_h3_list(YeePass<S> D, const int* __restrict__ list) {
+  using C = typename Store<S>::C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* tab = load_table<C, LDS>(D.tab, D.ntab * 9, reinterpret_cast<C*>(smem_raw));
+  const int t = list[blockIdx.x];
+  const int kb = t % D.nkb, rest = t / D.nkb;
+  yee_h3_tile<S, I, PML, NT>(D, tab, kb, rest % D.njb, rest / D.njb);
+}
+
+template <typename S, typename I, bool LDS, bool PML, bool ADE, bool NT>
+__global__ void __launch_bounds__(256) k_yee_e3_list(YeePass<S> D, const int* __restrict__ list) {
+  using C = typename Store<S>::C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* tab = load_table<C, LDS>(D.tab, D.ntab * 9, reinterpret_cast<C*>(smem_raw));
+  const int t = list[blockIdx.x];
+  const int kb = t % D.nkb, rest = t / D.nkb;
+  yee_e3_tile<S, I, PML, ADE, NT>(D, tab, kb, rest % D.njb, rest / D.njb);
+}
+
 // A batch of grids of different sizes in one persistent launch: the grid is
 // sized to the chip's resident capacity and each block walks the
 // concatenated tile list of all grids (tile t, t + gridDim.x, ...).

@@ -509,7 +509,7 @@ class Engine:
 
     def set_tiling(self, tile_y: int = 0, chunks_x: int = 0, xcd_remap: bool | None = None,
                    nontemporal: bool | None = None, pml_split: bool | None = None,
-                   k32: bool | None = None):
+                   k32: bool | None = None, pml_streams: bool | None = None):
         """Tiling of the 3-D Yee kernels; None leaves a flag to the engine's
         default (no XCD remap, non-temporal streams on, z-face CPML split
         off).  pml_split moves the z-face CPML corrections into a launch of
@@ -517,9 +517,10 @@ class Engine:
         lanes with two j-rows per wave."""
         flags = 0
         if xcd_remap is not None or nontemporal is not None or pml_split is not None or \
-                k32 is not None:
+                k32 is not None or pml_streams is not None:
             flags = _native.TILE_SET
             flags |= _native.TILE_XCD_REMAP if xcd_remap else 0
+            flags |= _native.TILE_PML_STREAMS if pml_streams else 0
             flags |= _native.TILE_NT if (nontemporal is None or nontemporal) else 0
             flags |= _native.TILE_PML_SPLIT if (pml_split is None or pml_split) else 0
             flags |= _native.TILE_K32 if k32 else 0
