@@ -660,10 +660,12 @@ def bench_fp32_vs_fp64(args) -> int:
                            "frac": round(e_bytes / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
                            "traffic": None, "alg_bytes_per_launch": e_bytes,
                            "avg_launch_ms": round(e_ms, 5), "h_ms": round(tm["yee_h_ms"] / tm["yee_h_launches"], 5),
-                           "tfsf_ms": round(tm["tfsf_ms"] / 10, 5), "nf2ff_ms": round(tm["nf2ff_ms"] / 10, 5),
+                           "post_h_ms": round(tm["tfsf_ms"] / 10, 5), "post_e_ms": round(tm["nf2ff_ms"] / 10, 5),
                            "note": ("E pass events (the Yee kernel with its CPML, plus the ADE kernel "
-                                    "in 3-D); algorithmic bytes count the fields, ids and psi, not the "
-                                    "ADE J arrays") }
+                                    "in 3-D; on 2-D FP64/FP32 grids the TF/SF corrections run inside "
+                                    "k_yee_h2/k_yee_e2); algorithmic bytes count the fields, ids and "
+                                    "psi, not the ADE J arrays. post_h_ms / post_e_ms: what follows "
+                                    "each pass (TF/SF launches where not fused, sources, NF2FF, movies)") }
         out["desc"] = desc
         with tempfile.TemporaryDirectory() as d:
             if two_d:

@@ -211,6 +211,15 @@ struct Engine {
   int64_t lb_rec_count{0}, lb_rec_count_saved{0};
   bool has_ade{false};
   std::vector<int64_t> ade_cells;  // host (i,j,k) linear index in (nx+1)(ny+1)(nz+1)
+  // 2-D FP64/FP32: the TF/SF corrections run inside the Yee kernels
+  bool tfsf_fused() const {
+    return dim == 2 && sc && sc->tfsf && (plan.precision == XF_FP64 || plan.precision == XF_FP32) &&
+           tfsf_jobs[0].size() <= static_cast<size_t>(kTfsf2dJobs) &&
+           tfsf_jobs[1].size() <= static_cast<size_t>(kTfsf2dJobs);
+  }
+  // all movie monitors in one launch: their descriptors and first threads
+  void* movie_items{};
+  int64_t* movie_start{};
   // time counter for graph replay
   int* t_dev{};
   // watchdog: first step found with a non-finite field value (UINT_MAX: none)
@@ -343,9 +352,9 @@ struct Engine {
   template <typename S>
   int launch_step(int t, bool use_tdev, bool timing_events, hipEvent_t* ev);
   template <typename S>
-  int launch_h(int t);
+  int launch_h(int t, const int* tp);
   template <typename S>
-  int launch_e(int t);
+  int launch_e(int t, const int* tp);
   template <typename S>
   int launch_post_h(int t, const int* tp);
   template <typename S>
@@ -1451,6 +1460,16 @@ int Engine::upload_additions() {
     state.emplace_back(p, static_cast<size_t>(D.frames * D.box) * sizeof(double));
     movies.push_back(D);
   }
+  movie_items = nullptr;
+  movie_start = nullptr;
+  if (!movies.empty()) {
+    std::vector<int64_t> st(1, 0);
+    for (const MovieDev& D : movies) st.push_back(st.back() + D.box);
+    XF_TRY(upload_vec(&movie_items, movies));
+    void* sp = nullptr;
+    XF_TRY(upload_vec(&sp, st));
+    movie_start = static_cast<int64_t*>(sp);
+  }
   lbd = LbDev{};
   if (!has_lb) return XF_OK;
   const xf_learned_boundary& L = *sc->lb;
@@ -2062,9 +2081,43 @@ int Engine::launch_pml_z(bool is_e, hipStream_t st) {
   return XF_OK;
 }
 
+namespace {
+template <typename C>
+void fill_tfsf_pass(const Engine& E, TfsfPass<C>& P, int pass) {
+  const auto& jobs = E.tfsf_jobs[pass];
+  P.njob = static_cast<int>(jobs.size());
+  P.is_e = pass == 1;
+  int64_t start = 0;
+  for (int q = 0; q < P.njob; ++q) {
+    const TfsfJobHost& J = jobs[q];
+    TfsfJob<C>& D = P.job[q];
+    D.comp = J.comp;
+    D.face = J.face;
+    for (int d = 0; d < 3; ++d) {
+      D.o[d] = static_cast<int>(J.o[d]);
+      D.n[d] = static_cast<int>(J.n[d]);
+    }
+    D.start = start;
+    D.coef = static_cast<C>(J.coef);
+    start += J.n[0] * J.n[1] * J.n[2];
+  }
+  P.total = start;
+  for (int d = 0; d < 3; ++d) {
+    P.proj_int[d] = static_cast<const C*>(E.proj_int[d]);
+    P.proj_half[d] = static_cast<const C*>(E.proj_half[d]);
+  }
+  const xf_tfsf& T = *E.sc->tfsf;
+  P.inc = static_cast<const C*>(pass == 1 ? E.h_inc : E.e_inc);
+  P.n_aux = pass == 1 ? T.h_inc.shape[1] : T.e_inc.shape[1];
+  P.g[0] = static_cast<int>(T.total_task.x.start);
+  P.g[1] = static_cast<int>(T.total_task.y.start);
+  P.g[2] = static_cast<int>(T.total_task.z.start);
+  for (int d = 0; d < 3; ++d) P.tr[d] = static_cast<C>(pass == 1 ? T.transform_h[d] : T.transform_e[d]);
+}
+}  // namespace
+
 template <typename S>
-int Engine::launch_h(int t) {
-  (void)t;
+int Engine::launch_h(int t, const int* tp) {
   using C = typename Store<S>::C;
   PmlPass<C> P{};
   fill_pml_pass<C>(*this, P, false);
@@ -2136,9 +2189,11 @@ int Engine::launch_h(int t) {
   } else {
     dim3 block(64, 4);
     dim3 grid((unsigned)((ny + 1 + 63) / 64), (unsigned)((nx + 1 + 3) / 4));
+    TfsfPass<C> tf{};
+    if (tfsf_fused()) fill_tfsf_pass<C>(*this, tf, 0);
 #define XF_LH2(I, L, PM)                                                                              \
   hipLaunchKernelGGL((k_yee_h2<S, I, L, PM>), grid, block, shm, stream, f, static_cast<const I*>(hid), \
-                     tab, ntab_h, P, d, scl, dense9<C>(false))
+                     tab, ntab_h, P, d, scl, dense9<C>(false), tf, t, tp)
     if (id_bytes == 1) {
       if (lds) { if (has_pml) XF_LH2(uint8_t, true, true); else XF_LH2(uint8_t, true, false); }
       else { if (has_pml) XF_LH2(uint8_t, false, true); else XF_LH2(uint8_t, false, false); }
@@ -2153,8 +2208,7 @@ int Engine::launch_h(int t) {
 }
 
 template <typename S>
-int Engine::launch_e(int t) {
-  (void)t;
+int Engine::launch_e(int t, const int* tp) {
   using C = typename Store<S>::C;
   PmlPass<C> P{};
   fill_pml_pass<C>(*this, P, true);
@@ -2241,9 +2295,11 @@ int Engine::launch_e(int t) {
   } else {
     dim3 block(64, 4);
     dim3 grid((unsigned)((ny + 1 + 63) / 64), (unsigned)((nx + 1 + 3) / 4));
+    TfsfPass<C> tf{};
+    if (tfsf_fused()) fill_tfsf_pass<C>(*this, tf, 1);
 #define XF_LE2(I, L, PM)                                                                              \
   hipLaunchKernelGGL((k_yee_e2<S, I, L, PM>), grid, block, shm, stream, f, static_cast<const I*>(eid), \
-                     tab, ntab_e, P, d, scl, dense9<C>(true))
+                     tab, ntab_e, P, d, scl, dense9<C>(true), tf, t, tp)
     if (id_bytes == 1) {
       if (lds) { if (has_pml) XF_LE2(uint8_t, true, true); else XF_LE2(uint8_t, true, false); }
       else { if (has_pml) XF_LE2(uint8_t, false, true); else XF_LE2(uint8_t, false, false); }
@@ -2257,45 +2313,11 @@ int Engine::launch_e(int t) {
   return XF_OK;
 }
 
-namespace {
-template <typename C>
-void fill_tfsf_pass(const Engine& E, TfsfPass<C>& P, int pass) {
-  const auto& jobs = E.tfsf_jobs[pass];
-  P.njob = static_cast<int>(jobs.size());
-  P.is_e = pass == 1;
-  int64_t start = 0;
-  for (int q = 0; q < P.njob; ++q) {
-    const TfsfJobHost& J = jobs[q];
-    TfsfJob<C>& D = P.job[q];
-    D.comp = J.comp;
-    D.face = J.face;
-    for (int d = 0; d < 3; ++d) {
-      D.o[d] = static_cast<int>(J.o[d]);
-      D.n[d] = static_cast<int>(J.n[d]);
-    }
-    D.start = start;
-    D.coef = static_cast<C>(J.coef);
-    start += J.n[0] * J.n[1] * J.n[2];
-  }
-  P.total = start;
-  for (int d = 0; d < 3; ++d) {
-    P.proj_int[d] = static_cast<const C*>(E.proj_int[d]);
-    P.proj_half[d] = static_cast<const C*>(E.proj_half[d]);
-  }
-  const xf_tfsf& T = *E.sc->tfsf;
-  P.inc = static_cast<const C*>(pass == 1 ? E.h_inc : E.e_inc);
-  P.n_aux = pass == 1 ? T.h_inc.shape[1] : T.e_inc.shape[1];
-  P.g[0] = static_cast<int>(T.total_task.x.start);
-  P.g[1] = static_cast<int>(T.total_task.y.start);
-  P.g[2] = static_cast<int>(T.total_task.z.start);
-  for (int d = 0; d < 3; ++d) P.tr[d] = static_cast<C>(pass == 1 ? T.transform_h[d] : T.transform_e[d]);
-}
-}  // namespace
 
 template <typename S>
 int Engine::launch_post_h(int t, const int* tp) {
   using C = typename Store<S>::C;
-  if (sc->tfsf) {
+  if (sc->tfsf && !tfsf_fused()) {
     TfsfPass<C> P{};
     fill_tfsf_pass<C>(*this, P, 0);
     if (P.total > 0)
@@ -2309,7 +2331,7 @@ int Engine::launch_post_h(int t, const int* tp) {
 template <typename S>
 int Engine::launch_post_e(int t, const int* tp) {
   using C = typename Store<S>::C;
-  if (sc->tfsf) {
+  if (sc->tfsf && !tfsf_fused()) {
     TfsfPass<C> P{};
     fill_tfsf_pass<C>(*this, P, 1);
     if (P.total > 0)
@@ -2359,9 +2381,13 @@ int Engine::launch_additions(int t, const int* tp, hipStream_t st) {
 template <typename S>
 int Engine::launch_movies(int t, const int* tp, hipStream_t st) {
   using C = typename Store<S>::C;
-  for (const MovieDev& M : movies)
-    hipLaunchKernelGGL((k_movie<S>), dim3((unsigned)((M.box + 255) / 256)), dim3(256), 0, st,
-                       make_fields<S>(*this), M, make_dims(*this), dim == 2 ? 1 : 0, make_scales<C>(*this), t, tp);
+  if (movies.empty()) return XF_OK;
+  int64_t total_box = 0;
+  for (const MovieDev& M : movies) total_box += M.box;
+  hipLaunchKernelGGL((k_movies<S>), dim3((unsigned)((total_box + 255) / 256)), dim3(256), 0, st,
+                     make_fields<S>(*this), static_cast<const MovieDev*>(movie_items), movie_start,
+                     static_cast<int>(movies.size()), make_dims(*this), dim == 2 ? 1 : 0, make_scales<C>(*this), t,
+                     tp);
   XF_HIP(hipGetLastError());
   return XF_OK;
 }
@@ -2384,11 +2410,11 @@ template <typename S>
 int Engine::launch_step(int t, bool use_tdev, bool timing_events, hipEvent_t* ev) {
   const int* tp = use_tdev ? t_dev : nullptr;
   if (timing_events) XF_HIP(hipEventRecord(ev[0], stream));
-  XF_TRY(launch_h<S>(t));
+  XF_TRY(launch_h<S>(t, tp));
   if (timing_events) XF_HIP(hipEventRecord(ev[1], stream));
   XF_TRY(launch_post_h<S>(t, tp));
   if (timing_events) XF_HIP(hipEventRecord(ev[2], stream));
-  XF_TRY(launch_e<S>(t));
+  XF_TRY(launch_e<S>(t, tp));
   if (timing_events) XF_HIP(hipEventRecord(ev[3], stream));
   XF_TRY(launch_post_e<S>(t, tp));
   if (timing_events) XF_HIP(hipEventRecord(ev[4], stream));

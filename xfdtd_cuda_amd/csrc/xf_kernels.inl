@@ -149,81 +149,6 @@ __device__ __forceinline__ const C* load_table(const C* __restrict__ g, int nval
 
 #include "xf_yee3d.inl"
 
-// ------------------------------------------------------- 2-D TMz, H and E
-// The TE updator (basic_updator_te.cu:9-20, basic_updator.cuh:16-32) on a
-// 2-D grid (nz = 1): Ex, Ey and Hz stay exactly zero there, so only Hx, Hy
-// and Ez are stored and updated; the dropped terms are products with zero
-// differences.  One thread per (i, j) cell, 64 lanes along j.
-template <typename S, typename I, bool LDS, bool PML>
-__global__ void __launch_bounds__(256)
-    k_yee_h2(Fields<S> f, const I* __restrict__ hid,
-             const typename Store<S>::C* __restrict__ htab, int ntab,
-             PmlPass<typename Store<S>::C> pml, Dims d, Scales<typename Store<S>::C> sc,
-             Dense9<typename Store<S>::C> dn) {
-  using C = typename Store<S>::C;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const C* tab = load_table<C, LDS>(htab, ntab * 9, reinterpret_cast<C*>(smem_raw));
-  const int j = blockIdx.x * 64 + threadIdx.x;
-  const int i = blockIdx.y * blockDim.y + threadIdx.y;
-  if (j > d.ny || i > d.nx) return;
-  const int64_t idx = (int64_t)i * d.sx + j;
-  C v[3] = {0, 0, 0};
-  bool have[3] = {false, false, false};
-  bool dirty[3] = {false, false, false};
-  if (i < d.nx && j < d.ny) {
-    C td[9];
-    if (dn.a[0]) {
-      for (int n = 0; n < 6; ++n) td[n] = dn.a[n][idx];
-    }
-    const C* t = dn.a[0] ? td : tab + 9 * (int)hid[idx];
-    const C ez = Store<S>::ld(f.e[2], idx, sc.inv_e);
-    const C ez_j = Store<S>::ld(f.e[2], idx + 1, sc.inv_e);
-    const C ez_n = Store<S>::ld(f.e[2], idx + d.sx, sc.inv_e);
-    const C zero = 0;
-    // Hx: chxh*Hx + chxey*(Ey(k+1)-Ey(k)) + chxez*(Ez(j+1)-Ez(j)), Ey = 0
-    v[0] = t[0] * Store<S>::ld(f.h[0], idx, sc.inv_h) + t[1] * zero + t[2] * (ez_j - ez);
-    // Hy: chyh*Hy + chyez*(Ez(i+1)-Ez(i)) + chyex*(Ex(k+1)-Ex(k)), Ex = 0
-    v[1] = t[3] * Store<S>::ld(f.h[1], idx, sc.inv_h) + t[4] * (ez_n - ez) + t[5] * zero;
-    have[0] = have[1] = dirty[0] = dirty[1] = true;
-  }
-  if (PML) pml_all<false, true>(pml, i, j, 0, idx, f, d, sc, v, have, dirty);
-  if (dirty[0]) Store<S>::st(f.h[0], idx, v[0], sc.s_h);
-  if (dirty[1]) Store<S>::st(f.h[1], idx, v[1], sc.s_h);
-}
-
-template <typename S, typename I, bool LDS, bool PML>
-__global__ void __launch_bounds__(256)
-    k_yee_e2(Fields<S> f, const I* __restrict__ eid,
-             const typename Store<S>::C* __restrict__ etab, int ntab,
-             PmlPass<typename Store<S>::C> pml, Dims d, Scales<typename Store<S>::C> sc,
-             Dense9<typename Store<S>::C> dn) {
-  using C = typename Store<S>::C;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const C* tab = load_table<C, LDS>(etab, ntab * 9, reinterpret_cast<C*>(smem_raw));
-  const int j = blockIdx.x * 64 + threadIdx.x;
-  const int i = blockIdx.y * blockDim.y + threadIdx.y;
-  if (j > d.ny || i > d.nx) return;
-  const int64_t idx = (int64_t)i * d.sx + j;
-  C v[3] = {0, 0, 0};
-  bool have[3] = {false, false, false};
-  bool dirty[3] = {false, false, false};
-  if (i < d.nx && j < d.ny && i > 0 && j > 0) {
-    C td[9];
-    if (dn.a[0]) {
-      for (int n = 6; n < 9; ++n) td[n] = dn.a[n][idx];
-    }
-    const C* t = dn.a[0] ? td : tab + 9 * (int)eid[idx];
-    const C hx = Store<S>::ld(f.h[0], idx, sc.inv_h);
-    const C hx_j = Store<S>::ld(f.h[0], idx - 1, sc.inv_h);
-    const C hy = Store<S>::ld(f.h[1], idx, sc.inv_h);
-    const C hy_i = Store<S>::ld(f.h[1], idx - d.sx, sc.inv_h);
-    v[2] = t[6] * Store<S>::ld(f.e[2], idx, sc.inv_e) + t[7] * (hx - hx_j) + t[8] * (hy - hy_i);
-    have[2] = dirty[2] = true;
-  }
-  if (PML) pml_all<true, true>(pml, i, j, 0, idx, f, d, sc, v, have, dirty);
-  if (dirty[2]) Store<S>::st(f.e[2], idx, v[2], sc.s_e);
-}
-
 // ------------------------------------------------------------------ TFSF
 // exInc..hzInc (tfsf_corrector.cu:18-130): interpolated 1-D incident value.
 template <typename C>
@@ -314,6 +239,49 @@ __device__ __forceinline__ void tfsf_cell(const Fields<S>& f, const TfsfPass<typ
   Store<S>::st(field, idx, v, scl);
 }
 
+// The TF/SF corrections of a 2-D pass applied in the Yee thread that has
+// just formed (i, j)'s values (FP64/FP32 storage, where a store and a load
+// give back the same value, so the result equals k_tfsf's): every job whose
+// written box holds the node, in job order, adds inc * coef to its
+// component.  TF/SF faces lie outside the CPML slabs, so the node's CPML
+// correction, applied before, touches other nodes.
+// A 2-D pass has four jobs (one per face), so the loop is unrolled over
+// that bound: the job descriptors are wave-uniform and their loads issue
+// together at the start of the kernel, not one after another.  The time
+// step is read only by a thread that applies a correction.
+constexpr int kTfsf2dJobs = 4;
+template <typename S>
+__device__ __forceinline__ void tfsf_fused_2d(const Fields<S>& f, const TfsfPass<typename Store<S>::C>& P,
+                                              int t_arg, const int* t_dev,
+                                              const Scales<typename Store<S>::C>& sc, int i, int j,
+                                              int64_t idx, typename Store<S>::C (&v)[3], bool (&have)[3],
+                                              bool (&dirty)[3]) {
+  using C = typename Store<S>::C;
+#pragma unroll
+  for (int p = 0; p < kTfsf2dJobs; ++p) {
+    if (p >= P.njob) break;
+    const TfsfJob<C>& X = P.job[p];
+    if (i < X.o[0] || i >= X.o[0] + X.n[0] || j < X.o[1] || j >= X.o[1] + X.n[1] || 0 < X.o[2] ||
+        0 >= X.o[2] + X.n[2])
+      continue;
+    const int c = X.comp;
+    const int axis = X.face / 2;
+    const bool negative = (X.face % 2) == 0;
+    const int a_axis = (axis + 1) % 3, b_axis = (axis + 2) % 3;
+    const int dual = (c == a_axis) ? b_axis : a_axis;
+    int pos[3] = {i, j, 0};
+    if (negative) pos[axis] += P.is_e ? -1 : 1;
+    const int t = t_dev ? *t_dev : t_arg;
+    const C inc = tfsf_inc(P, !P.is_e, dual, t, pos[0], pos[1], pos[2]);
+    if (!have[c]) {
+      v[c] = Store<S>::ld(P.is_e ? f.e[c] : f.h[c], idx, P.is_e ? sc.inv_e : sc.inv_h);
+      have[c] = true;
+    }
+    v[c] += inc * X.coef;
+    dirty[c] = true;
+  }
+}
+
 template <typename S>
 __global__ void __launch_bounds__(256)
     k_tfsf(Fields<S> f, TfsfPass<typename Store<S>::C> P, int t_arg, const int* t_dev,
@@ -321,6 +289,97 @@ __global__ void __launch_bounds__(256)
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= P.total) return;
   tfsf_cell<S>(f, P, t_dev ? *t_dev : t_arg, sc, d, two_d, tid);
+}
+
+// Coefficients LO..HI-1 of a node's row: the dense per-cell arrays when
+// they are set, else the table row of the node's id.  Filled element by
+// element, with no pointer chosen between the two sources, so the row stays
+// in registers (a pointer that may address a private array puts the array in
+// scratch memory).
+template <int LO, int HI, typename C, typename I>
+__device__ __forceinline__ void coef_row(const Dense9<C>& dn, const C* __restrict__ tab,
+                                         const I* __restrict__ ids, int64_t idx, C (&c)[9]) {
+  if (dn.a[0]) {
+#pragma unroll
+    for (int n = LO; n < HI; ++n) c[n] = dn.a[n][idx];
+  } else {
+    const C* t = tab + 9 * (int)ids[idx];
+#pragma unroll
+    for (int n = LO; n < HI; ++n) c[n] = t[n];
+  }
+}
+
+// ------------------------------------------------------- 2-D TMz, H and E
+// The TE updator (basic_updator_te.cu:9-20, basic_updator.cuh:16-32) on a
+// 2-D grid (nz = 1): Ex, Ey and Hz stay exactly zero there, so only Hx, Hy
+// and Ez are stored and updated; the dropped terms are products with zero
+// differences.  One thread per (i, j) cell, 64 lanes along j.
+template <typename S, typename I, bool LDS, bool PML>
+__global__ void __launch_bounds__(256)
+    k_yee_h2(Fields<S> f, const I* __restrict__ hid,
+             const typename Store<S>::C* __restrict__ htab, int ntab,
+             PmlPass<typename Store<S>::C> pml, Dims d, Scales<typename Store<S>::C> sc,
+             Dense9<typename Store<S>::C> dn, TfsfPass<typename Store<S>::C> tf, int t_arg,
+             const int* t_dev) {
+  using C = typename Store<S>::C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* tab = load_table<C, LDS>(htab, ntab * 9, reinterpret_cast<C*>(smem_raw));
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (j > d.ny || i > d.nx) return;
+  const int64_t idx = (int64_t)i * d.sx + j;
+  C v[3] = {0, 0, 0};
+  bool have[3] = {false, false, false};
+  bool dirty[3] = {false, false, false};
+  if (i < d.nx && j < d.ny) {
+    C t[9];
+    coef_row<0, 6>(dn, tab, hid, idx, t);
+    const C ez = Store<S>::ld(f.e[2], idx, sc.inv_e);
+    const C ez_j = Store<S>::ld(f.e[2], idx + 1, sc.inv_e);
+    const C ez_n = Store<S>::ld(f.e[2], idx + d.sx, sc.inv_e);
+    const C zero = 0;
+    // Hx: chxh*Hx + chxey*(Ey(k+1)-Ey(k)) + chxez*(Ez(j+1)-Ez(j)), Ey = 0
+    v[0] = t[0] * Store<S>::ld(f.h[0], idx, sc.inv_h) + t[1] * zero + t[2] * (ez_j - ez);
+    // Hy: chyh*Hy + chyez*(Ez(i+1)-Ez(i)) + chyex*(Ex(k+1)-Ex(k)), Ex = 0
+    v[1] = t[3] * Store<S>::ld(f.h[1], idx, sc.inv_h) + t[4] * (ez_n - ez) + t[5] * zero;
+    have[0] = have[1] = dirty[0] = dirty[1] = true;
+  }
+  if (PML) pml_all<false, true>(pml, i, j, 0, idx, f, d, sc, v, have, dirty);
+  if (tf.njob > 0) tfsf_fused_2d<S>(f, tf, t_arg, t_dev, sc, i, j, idx, v, have, dirty);
+  if (dirty[0]) Store<S>::st(f.h[0], idx, v[0], sc.s_h);
+  if (dirty[1]) Store<S>::st(f.h[1], idx, v[1], sc.s_h);
+}
+
+template <typename S, typename I, bool LDS, bool PML>
+__global__ void __launch_bounds__(256)
+    k_yee_e2(Fields<S> f, const I* __restrict__ eid,
+             const typename Store<S>::C* __restrict__ etab, int ntab,
+             PmlPass<typename Store<S>::C> pml, Dims d, Scales<typename Store<S>::C> sc,
+             Dense9<typename Store<S>::C> dn, TfsfPass<typename Store<S>::C> tf, int t_arg,
+             const int* t_dev) {
+  using C = typename Store<S>::C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const C* tab = load_table<C, LDS>(etab, ntab * 9, reinterpret_cast<C*>(smem_raw));
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (j > d.ny || i > d.nx) return;
+  const int64_t idx = (int64_t)i * d.sx + j;
+  C v[3] = {0, 0, 0};
+  bool have[3] = {false, false, false};
+  bool dirty[3] = {false, false, false};
+  if (i < d.nx && j < d.ny && i > 0 && j > 0) {
+    C t[9];
+    coef_row<6, 9>(dn, tab, eid, idx, t);
+    const C hx = Store<S>::ld(f.h[0], idx, sc.inv_h);
+    const C hx_j = Store<S>::ld(f.h[0], idx - 1, sc.inv_h);
+    const C hy = Store<S>::ld(f.h[1], idx, sc.inv_h);
+    const C hy_i = Store<S>::ld(f.h[1], idx - d.sx, sc.inv_h);
+    v[2] = t[6] * Store<S>::ld(f.e[2], idx, sc.inv_e) + t[7] * (hx - hx_j) + t[8] * (hy - hy_i);
+    have[2] = dirty[2] = true;
+  }
+  if (PML) pml_all<true, true>(pml, i, j, 0, idx, f, d, sc, v, have, dirty);
+  if (tf.njob > 0) tfsf_fused_2d<S>(f, tf, t_arg, t_dev, sc, i, j, idx, v, have, dirty);
+  if (dirty[2]) Store<S>::st(f.e[2], idx, v[2], sc.s_e);
 }
 
 // ----------------------------------------------------------------- NF2FF
@@ -657,24 +716,31 @@ __global__ void k_point_sources(Fields<S> f, PointDev P, Scales<typename Store<S
 
 // ------------------------------------------------------------- movies
 // MovieMonitor<F>::update (movie_monitor.cu:17-45): every `interval`
-// steps the component over the box goes to frame t / interval.
+// steps the component over the box goes to frame t / interval.  All movie
+// monitors of a grid run in one launch: thread c of the concatenated boxes
+// (start[q] = first thread of monitor q) copies its node when its monitor
+// records this step.
 template <typename S>
 __global__ void __launch_bounds__(256)
-    k_movie(Fields<S> f, MovieDev M, Dims d, int two_d, Scales<typename Store<S>::C> sc, int t_arg,
-            const int* t_dev) {
+    k_movies(Fields<S> f, const MovieDev* __restrict__ M, const int64_t* __restrict__ start, int n, Dims d,
+             int two_d, Scales<typename Store<S>::C> sc, int t_arg, const int* t_dev) {
   const int t = t_dev ? *t_dev : t_arg;
-  if (t % M.interval != 0) return;
-  const int64_t frame = t / M.interval;
-  if (frame >= M.frames) return;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= M.box) return;
+  if (c >= start[n]) return;
+  int q = 0;
+  while (q + 1 < n && c >= start[q + 1]) ++q;
+  const MovieDev& m = M[q];
+  if (t % m.interval != 0) return;
+  const int64_t frame = t / m.interval;
+  if (frame >= m.frames) return;
+  const int64_t r = c - start[q];
   int li, lj, lk;
-  box_coords(c, M.n[1], M.n[2], li, lj, lk);
-  const int i = M.o[0] + li, j = M.o[1] + lj, k = M.o[2] + lk;
-  const S* p = M.field < 3 ? f.e[M.field] : f.h[M.field - 3];
+  box_coords(r, m.n[1], m.n[2], li, lj, lk);
+  const int i = m.o[0] + li, j = m.o[1] + lj, k = m.o[2] + lk;
+  const S* p = m.field < 3 ? f.e[m.field] : f.h[m.field - 3];
   const int64_t idx = two_d ? (int64_t)i * d.sx + j : (int64_t)i * d.sx + (int64_t)j * d.sy + k;
-  const double v = p ? (double)Store<S>::ld(p, idx, M.field < 3 ? sc.inv_e : sc.inv_h) : 0.0;
-  M.data[frame * M.box + c] = v;
+  const double v = p ? (double)Store<S>::ld(p, idx, m.field < 3 ? sc.inv_e : sc.inv_h) : 0.0;
+  m.data[frame * m.box + r] = v;
 }
 
 // --------------------------------------------------------- learned layer
@@ -1113,13 +1179,11 @@ __global__ void __launch_bounds__(256)
   if (c >= A.n) return;
   const int64_t idx = A.idx[c];
   const int i = A.ijk[3 * c], j = A.ijk[3 * c + 1], k = A.ijk[3 * c + 2];
-  C td[9];
-  if (dn.a[0]) {
-    for (int n = 0; n < 9; ++n) td[n] = dn.a[n][idx];
-  }
-  const C* t = dn.a[0] ? td : etab + 9 * (int)eid[idx];
+  C t[9];
+  coef_row<0, 9>(dn, etab, eid, idx, t);
   const int cell[3] = {i, j, k};
   const int64_t st[3] = {d.sx, d.sy, 1};
+#pragma unroll
   for (int comp = 0; comp < 3; ++comp) {
     const int a_axis = (comp + 1) % 3, b_axis = (comp + 2) % 3;
     const int a = cell[a_axis], b = cell[b_axis];
