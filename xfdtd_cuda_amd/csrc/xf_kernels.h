@@ -24,6 +24,17 @@ namespace xf {
 constexpr int kMaxPml = 6;
 constexpr int kMaxTfsfJobs = 12;
 
+// Every device buffer of the engine is global memory.  A pointer read from
+// memory (a batch's per-grid descriptors) is not known to the compiler to
+// be global, so it would be accessed with flat instructions; a flat load
+// counts on the LDS/scalar counter as well, and every wait for a scalar load
+// then also waits for the field loads in flight.  gp() states the address
+// space, so such accesses become global loads and stores.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gp(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 // Storage <-> compute conversion.  FP64 and FP32 store what they compute;
 // bf16 stores value * scale (scale per E/H) and computes in float.
 template <typename S>
@@ -34,35 +45,35 @@ struct Store;
 template <>
 struct Store<double> {
   using C = double;
-  static __device__ __forceinline__ C ld(const double* p, int64_t i, C) { return p[i]; }
-  static __device__ __forceinline__ void st(double* p, int64_t i, C v, C) { p[i] = v; }
+  static __device__ __forceinline__ C ld(const double* p, int64_t i, C) { return gp(p)[i]; }
+  static __device__ __forceinline__ void st(double* p, int64_t i, C v, C) { gp(p)[i] = v; }
   static __device__ __forceinline__ C ld_nt(const double* p, int64_t i, C) {
-    return __builtin_nontemporal_load(p + i);
+    return __builtin_nontemporal_load(gp(p) + i);
   }
   static __device__ __forceinline__ void st_nt(double* p, int64_t i, C v, C) {
-    __builtin_nontemporal_store(v, p + i);
+    __builtin_nontemporal_store(v, gp(p) + i);
   }
 };
 template <>
 struct Store<float> {
   using C = float;
-  static __device__ __forceinline__ C ld(const float* p, int64_t i, C) { return p[i]; }
-  static __device__ __forceinline__ void st(float* p, int64_t i, C v, C) { p[i] = v; }
+  static __device__ __forceinline__ C ld(const float* p, int64_t i, C) { return gp(p)[i]; }
+  static __device__ __forceinline__ void st(float* p, int64_t i, C v, C) { gp(p)[i] = v; }
   static __device__ __forceinline__ C ld_nt(const float* p, int64_t i, C) {
-    return __builtin_nontemporal_load(p + i);
+    return __builtin_nontemporal_load(gp(p) + i);
   }
   static __device__ __forceinline__ void st_nt(float* p, int64_t i, C v, C) {
-    __builtin_nontemporal_store(v, p + i);
+    __builtin_nontemporal_store(v, gp(p) + i);
   }
 };
 template <>
 struct Store<__hip_bfloat16> {
   using C = float;
   static __device__ __forceinline__ C ld(const __hip_bfloat16* p, int64_t i, C inv) {
-    return __bfloat162float(p[i]) * inv;
+    return __bfloat162float(__ushort_as_bfloat16(gp(reinterpret_cast<const unsigned short*>(p))[i])) * inv;
   }
   static __device__ __forceinline__ void st(__hip_bfloat16* p, int64_t i, C v, C scale) {
-    p[i] = __float2bfloat16(v * scale);
+    gp(reinterpret_cast<unsigned short*>(p))[i] = __bfloat16_as_ushort(__float2bfloat16(v * scale));
   }
   // the streaming slots use plain accesses: non-temporal 16-bit loads and
   // stores measured slower at 536^3 (59.9 against 61.9 Gcells/s)
@@ -78,10 +89,10 @@ template <>
 struct Store<__half> {
   using C = float;
   static __device__ __forceinline__ C ld(const __half* p, int64_t i, C inv) {
-    return __half2float(p[i]) * inv;
+    return __half2float(__ushort_as_half(gp(reinterpret_cast<const unsigned short*>(p))[i])) * inv;
   }
   static __device__ __forceinline__ void st(__half* p, int64_t i, C v, C scale) {
-    p[i] = __float2half(v * scale);
+    gp(reinterpret_cast<unsigned short*>(p))[i] = __half_as_ushort(__float2half(v * scale));
   }
   // plain accesses in the streaming slots, as for bf16 (non-temporal 16-bit
   // accesses measured slower: 57.9 against 59.9 Gcells/s at 536^3)
@@ -140,28 +151,28 @@ struct Store<Bf16c> {
   }
   static __device__ __forceinline__ unsigned char* bytes(Bf16c* p) { return reinterpret_cast<unsigned char*>(p); }
   static __device__ __forceinline__ C ld(const Bf16c* p, int64_t i, C inv) {
-    return bf16c_decode(*reinterpret_cast<const uint16_t*>(bytes(p) + off_hi(i)),
-                        *reinterpret_cast<const int8_t*>(bytes(p) + off_lo(i))) *
+    return bf16c_decode(*gp(reinterpret_cast<const uint16_t*>(bytes(p) + off_hi(i))),
+                        *gp(reinterpret_cast<const int8_t*>(bytes(p) + off_lo(i)))) *
            inv;
   }
   static __device__ __forceinline__ void st(Bf16c* p, int64_t i, C v, C scale) {
     uint16_t h;
     int8_t r;
     bf16c_encode(v * scale, h, r);
-    *reinterpret_cast<uint16_t*>(bytes(p) + off_hi(i)) = h;
-    *reinterpret_cast<int8_t*>(bytes(p) + off_lo(i)) = r;
+    *gp(reinterpret_cast<uint16_t*>(bytes(p) + off_hi(i))) = h;
+    *gp(reinterpret_cast<int8_t*>(bytes(p) + off_lo(i))) = r;
   }
   static __device__ __forceinline__ C ld_nt(const Bf16c* p, int64_t i, C inv) {
-    return bf16c_decode(__builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(bytes(p) + off_hi(i))),
-                        __builtin_nontemporal_load(reinterpret_cast<const int8_t*>(bytes(p) + off_lo(i)))) *
+    return bf16c_decode(__builtin_nontemporal_load(gp(reinterpret_cast<const uint16_t*>(bytes(p) + off_hi(i)))),
+                        __builtin_nontemporal_load(gp(reinterpret_cast<const int8_t*>(bytes(p) + off_lo(i))))) *
            inv;
   }
   static __device__ __forceinline__ void st_nt(Bf16c* p, int64_t i, C v, C scale) {
     uint16_t h;
     int8_t r;
     bf16c_encode(v * scale, h, r);
-    __builtin_nontemporal_store(h, reinterpret_cast<uint16_t*>(bytes(p) + off_hi(i)));
-    __builtin_nontemporal_store(r, reinterpret_cast<int8_t*>(bytes(p) + off_lo(i)));
+    __builtin_nontemporal_store(h, gp(reinterpret_cast<uint16_t*>(bytes(p) + off_hi(i))));
+    __builtin_nontemporal_store(r, gp(reinterpret_cast<int8_t*>(bytes(p) + off_lo(i))));
   }
 };
 

@@ -30,7 +30,7 @@ __device__ __forceinline__ void pml_face(const PmlFace<C>& F, int i, int j, int 
   const int64_t stc = AX == 0 ? d.sx : AX == 1 ? d.sy : 1;
   const int ci = c + F.off_c - F.gstart;
   const int layer = c - F.nstart;
-  const C ca = F.ca[ci], cb = F.cb[ci];
+  const C ca = gp(F.ca)[ci], cb = gp(F.cb)[ci];
   // psi index aBCToXYZ(a, b, layer)
   const int px = AX == 0 ? layer : AX == 1 ? b : a;
   const int py = AX == 0 ? a : AX == 1 ? layer : b;
@@ -43,34 +43,34 @@ __device__ __forceinline__ void pml_face(const PmlFace<C>& F, int i, int j, int 
     const C dp = Store<S>::ld(dual[B], idx, inv_dual);
     const C dq = Store<S>::ld(dual[B], ISE ? idx - stc : idx + stc, inv_dual);
     const uint64_t p = ((uint32_t)px * (uint32_t)F.sa1 + (uint32_t)py) * (uint32_t)F.sa2 + (uint32_t)pz;
-    C psi = F.psi_a[p];
+    C psi = gp(F.psi_a)[p];
     if (ISE)
       psi = cb * psi + ca * (dp - dq);
     else
       psi = cb * psi + ca * (dq - dp);
-    F.psi_a[p] = psi;
+    gp(F.psi_a)[p] = psi;
     if (!have[A]) {
       v[A] = Store<S>::ld(self[A], idx, inv_self);
       have[A] = true;
     }
-    v[A] += F.cpsi_a[p] * psi;
+    v[A] += gp(F.cpsi_a)[p] * psi;
     dirty[A] = true;
   }
   if (in_b && !(TWO_D && (self[B] == nullptr || dual[A] == nullptr))) {
     const C dp = Store<S>::ld(dual[A], idx, inv_dual);
     const C dq = Store<S>::ld(dual[A], ISE ? idx - stc : idx + stc, inv_dual);
     const uint64_t p = ((uint32_t)px * (uint32_t)F.sb1 + (uint32_t)py) * (uint32_t)F.sb2 + (uint32_t)pz;
-    C psi = F.psi_b[p];
+    C psi = gp(F.psi_b)[p];
     if (ISE)
       psi = cb * psi + ca * (dp - dq);
     else
       psi = cb * psi + ca * (dq - dp);
-    F.psi_b[p] = psi;
+    gp(F.psi_b)[p] = psi;
     if (!have[B]) {
       v[B] = Store<S>::ld(self[B], idx, inv_self);
       have[B] = true;
     }
-    v[B] += F.cpsi_b[p] * psi;
+    v[B] += gp(F.cpsi_b)[p] * psi;
     dirty[B] = true;
   }
 }
@@ -158,22 +158,22 @@ __device__ __forceinline__ C tfsf_inc(const TfsfPass<C>& P, bool inc_is_e, int c
   C proj;
   if (inc_is_e) {
     if (comp == 0)
-      proj = P.proj_half[0][li + 1] + P.proj_int[1][lj] + P.proj_int[2][lk];
+      proj = gp(P.proj_half[0])[li + 1] + gp(P.proj_int[1])[lj] + gp(P.proj_int[2])[lk];
     else if (comp == 1)
-      proj = P.proj_int[0][li] + P.proj_half[1][lj + 1] + P.proj_int[2][lk];
+      proj = gp(P.proj_int[0])[li] + gp(P.proj_half[1])[lj + 1] + gp(P.proj_int[2])[lk];
     else
-      proj = P.proj_int[0][li] + P.proj_int[1][lj] + P.proj_half[2][lk + 1];
+      proj = gp(P.proj_int[0])[li] + gp(P.proj_int[1])[lj] + gp(P.proj_half[2])[lk + 1];
   } else {
     if (comp == 0)
-      proj = P.proj_int[0][li] + P.proj_half[1][lj + 1] + P.proj_half[2][lk + 1] - (C)0.5;
+      proj = gp(P.proj_int[0])[li] + gp(P.proj_half[1])[lj + 1] + gp(P.proj_half[2])[lk + 1] - (C)0.5;
     else if (comp == 1)
-      proj = P.proj_half[0][li + 1] + P.proj_int[1][lj] + P.proj_half[2][lk + 1] - (C)0.5;
+      proj = gp(P.proj_half[0])[li + 1] + gp(P.proj_int[1])[lj] + gp(P.proj_half[2])[lk + 1] - (C)0.5;
     else
-      proj = P.proj_half[0][li + 1] + P.proj_half[1][lj + 1] + P.proj_int[2][lk] - (C)0.5;
+      proj = gp(P.proj_half[0])[li + 1] + gp(P.proj_half[1])[lj + 1] + gp(P.proj_int[2])[lk] - (C)0.5;
   }
   const uint64_t index = (uint64_t)proj;
   const C weight = proj - (C)index;
-  const C* row = P.inc + (int64_t)t * P.n_aux;
+  const __attribute__((address_space(1))) C* row = gp(P.inc) + (int64_t)t * P.n_aux;
   return P.tr[comp] * ((1 - weight) * row[index] + weight * row[index + 1]);
 }
 
@@ -419,6 +419,11 @@ __device__ __forceinline__ typename Store<S>::C nf_interp(const Fields<S>& f, co
   return (C)0.5 * (F(1, 0, 0) + F(0, 0, 0));
 }
 
+// a complex FP64 sum as a plain two-double vector (the accumulators and
+// time tables are double2 arrays; this type is read and written through
+// global-space pointers, which HIP's double2 class cannot bind to)
+typedef double nf_d2 __attribute__((ext_vector_type(2)));
+
 // calculateJ / calculateM of all six faces and all frequencies in one launch
 // (nf2ff_frequency_domain.data.cu:80-232).  The four interpolated tangential
 // fields are formed once per surface cell and accumulated into every
@@ -447,10 +452,11 @@ __device__ __forceinline__ void nf2ff_cell(const Fields<S>& f, const NfPass& P, 
   const double r_mb = (positive ? -1.0 : 1.0) * ea;
   const int64_t nface = (int64_t)F.n[0] * F.n[1] * F.n[2];
   for (int q = 0; q < P.nfreq; ++q) {
-    const double2 th = P.th[(int64_t)q * P.nt + t];
-    const double2 te = P.te[(int64_t)q * P.nt + t];
-    double2* base = P.cur + (int64_t)q * P.block * 4 + F.off * 4 + cell;
-    double2 v;
+    const nf_d2 th = gp(reinterpret_cast<const nf_d2*>(P.th))[(int64_t)q * P.nt + t];
+    const nf_d2 te = gp(reinterpret_cast<const nf_d2*>(P.te))[(int64_t)q * P.nt + t];
+    __attribute__((address_space(1))) nf_d2* base =
+        gp(reinterpret_cast<nf_d2*>(P.cur)) + (int64_t)q * P.block * 4 + F.off * 4 + cell;
+    nf_d2 v;
     v = base[0];
     v.x += r_ja * th.x;
     v.y += r_ja * th.y;
@@ -1093,9 +1099,9 @@ __global__ void __launch_bounds__(64)
   const int t = t_dev ? *t_dev : t_arg;
   const PointItem<S>& I = items[g];
   for (int q = 0; q < I.P.n; ++q) {
-    S* fld = I.f.e[I.P.comp[q]];
-    const int64_t i = I.P.idx[q];
-    const C v = Store<S>::ld(fld, i, I.sc.inv_e) + (C)I.P.value[(int64_t)q * I.P.nt + t];
+    S* fld = I.f.e[gp(I.P.comp)[q]];
+    const int64_t i = gp(I.P.idx)[q];
+    const C v = Store<S>::ld(fld, i, I.sc.inv_e) + (C)gp(I.P.value)[(int64_t)q * I.P.nt + t];
     Store<S>::st(fld, i, v, I.sc.s_e);
   }
 }

@@ -18,6 +18,40 @@
 // (two planes of loads in flight per wave).  Arithmetic and its order are
 // the reference's; the file is built without FMA contraction.
 
+// Where a tile reads its coefficient table: TAB_ANY lets the compiler work
+// it out (the single-grid kernels, whose table pointer it can trace to LDS or
+// to a kernel argument); TAB_LDS and TAB_GLOBAL state it for the batched
+// kernels, whose table is a wave's LDS slice or a global array chosen at run
+// time (a pointer that may be either would be read with flat loads).
+enum { TAB_ANY = 0, TAB_LDS = 1, TAB_GLOBAL = 2 };
+
+template <int TAB, typename C>
+__device__ __forceinline__ C tab_at(const C* tab, int off) {
+  if constexpr (TAB == TAB_LDS)
+    return ((const __attribute__((address_space(3))) C*)tab)[off];
+  else if constexpr (TAB == TAB_GLOBAL)
+    return gp(tab)[off];
+  else
+    return tab[off];
+}
+
+template <int TAB, typename C>
+__device__ __forceinline__ void tab_row(const C* tab, int id, C (&t)[9]) {
+  if constexpr (TAB == TAB_LDS) {
+    const __attribute__((address_space(3))) C* q = (const __attribute__((address_space(3))) C*)tab + 9 * id;
+#pragma unroll
+    for (int n = 0; n < 9; ++n) t[n] = q[n];
+  } else if constexpr (TAB == TAB_GLOBAL) {
+    const __attribute__((address_space(1))) C* q = gp(tab) + 9 * id;
+#pragma unroll
+    for (int n = 0; n < 9; ++n) t[n] = q[n];
+  } else {
+    const C* q = tab + 9 * id;
+#pragma unroll
+    for (int n = 0; n < 9; ++n) t[n] = q[n];
+  }
+}
+
 template <typename S>
 struct HLoads {
   typename Store<S>::C ex_i, ex_k, ex_j, ey_k, ey_n, ez_j, ez_n, hx, hy, hz;
@@ -28,7 +62,7 @@ template <typename S, typename I, bool NT>
 __device__ __forceinline__ void h_load(HLoads<S>& L, const Fields<S>& f, const I* __restrict__ hid,
                                        int64_t idx, const Dims& d, typename Store<S>::C ie,
                                        typename Store<S>::C ih) {
-  L.id = (int)hid[idx];
+  L.id = (int)gp(hid)[idx];
   L.ex_i = Store<S>::ld(f.e[0], idx, ie);
   L.ex_k = Store<S>::ld(f.e[0], idx + 1, ie);
   L.ex_j = Store<S>::ld(f.e[0], idx + d.sy, ie);
@@ -52,7 +86,7 @@ __device__ __forceinline__ void e_load(ELoads<S>& L, const Fields<S>& f, const I
                                        int64_t idx, const Dims& d, bool j_in, bool k_in,
                                        typename Store<S>::C ie, typename Store<S>::C ih) {
   using C = typename Store<S>::C;
-  L.id = (int)eid[idx];
+  L.id = (int)gp(eid)[idx];
   L.ex = NT ? Store<S>::ld_nt(f.e[0], idx, ie) : Store<S>::ld(f.e[0], idx, ie);
   L.ey = NT ? Store<S>::ld_nt(f.e[1], idx, ie) : Store<S>::ld(f.e[1], idx, ie);
   L.ez = NT ? Store<S>::ld_nt(f.e[2], idx, ie) : Store<S>::ld(f.e[2], idx, ie);
@@ -102,7 +136,7 @@ __device__ __forceinline__ unsigned pml_mask(const PmlPass<C>& P, int i0, int i1
 // One tile of the H pass.  Lanes beyond the grid return at once; the tile
 // bodies hold no barrier, so the batched kernels may loop over tiles with
 // some lanes idle.
-template <typename S, typename I, bool PML, bool NT, bool DN = false>
+template <typename S, typename I, bool PML, bool NT, bool DN = false, int TAB = TAB_ANY>
 __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename Store<S>::C* tab,
                                             int kb, int jb, int ch) {
   using C = typename Store<S>::C;
@@ -136,12 +170,13 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
       if constexpr (DN) {
         C t[9];
 #pragma unroll
-        for (int n = 0; n < 9; ++n) t[n] = D.dense[n][idx];
+        for (int n = 0; n < 9; ++n) t[n] = gp(D.dense[n])[idx];
         v[0] = t[0] * cur.hx + t[1] * (cur.ey_k - ey_i) + t[2] * (cur.ez_j - ez_i);
         v[1] = t[3] * cur.hy + t[4] * (cur.ez_n - ez_i) + t[5] * (cur.ex_k - cur.ex_i);
         v[2] = t[6] * cur.hz + t[7] * (cur.ex_j - cur.ex_i) + t[8] * (cur.ey_n - ey_i);
       } else {
-        const C* t = tab + 9 * cur.id;
+        C t[9];
+        tab_row<TAB>(tab, cur.id, t);
         v[0] = t[0] * cur.hx + t[1] * (cur.ey_k - ey_i) + t[2] * (cur.ez_j - ez_i);
         v[1] = t[3] * cur.hy + t[4] * (cur.ez_n - ez_i) + t[5] * (cur.ex_k - cur.ex_i);
         v[2] = t[6] * cur.hz + t[7] * (cur.ex_j - cur.ex_i) + t[8] * (cur.ey_n - ey_i);
@@ -159,7 +194,7 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
 
 // One tile of the E pass; table rows flagged dispersive are left to the
 // ADE kernel.
-template <typename S, typename I, bool PML, bool ADE, bool NT, bool DN = false>
+template <typename S, typename I, bool PML, bool ADE, bool NT, bool DN = false, int TAB = TAB_ANY>
 __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename Store<S>::C* tab,
                                             int kb, int jb, int ch) {
   using C = typename Store<S>::C;
@@ -195,22 +230,30 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
     bool dirty[3] = {false, false, false};
     if (jk_main && i < d.nx) {
       if (!(ADE && D.eflag[cur.id])) {
+        // coefficient n of the cell, read where its component is updated
+        // (the E components have different boundary conditions)
         C td[9];
         if constexpr (DN) {
 #pragma unroll
-          for (int n = 0; n < 9; ++n) td[n] = D.dense[n][idx];
+          for (int n = 0; n < 9; ++n) td[n] = gp(D.dense[n])[idx];
         }
-        const C* t = DN ? td : tab + 9 * cur.id;
+        const int row = 9 * cur.id;
+        auto t = [&](int n) -> C {
+          if constexpr (DN)
+            return td[n];
+          else
+            return tab_at<TAB>(tab, row + n);
+        };
         if (j_in && k_in) {
-          v[0] = t[0] * cur.ex + t[1] * (cur.hy - cur.hy_k) + t[2] * (cur.hz - cur.hz_j);
+          v[0] = t(0) * cur.ex + t(1) * (cur.hy - cur.hy_k) + t(2) * (cur.hz - cur.hz_j);
           have[0] = dirty[0] = true;
         }
         if (k_in && i > 0) {
-          v[1] = t[3] * cur.ey + t[4] * (cur.hz - hz_m) + t[5] * (cur.hx - cur.hx_k);
+          v[1] = t(3) * cur.ey + t(4) * (cur.hz - hz_m) + t(5) * (cur.hx - cur.hx_k);
           have[1] = dirty[1] = true;
         }
         if (i > 0 && j_in) {
-          v[2] = t[6] * cur.ez + t[7] * (cur.hx - cur.hx_j) + t[8] * (cur.hy - hy_m);
+          v[2] = t(6) * cur.ez + t(7) * (cur.hx - cur.hx_j) + t(8) * (cur.hy - hy_m);
           have[2] = dirty[2] = true;
         }
       }
@@ -303,7 +346,7 @@ __device__ __forceinline__ const C* batch_table(const C* __restrict__ g_tab, int
   if (g != cur_g) {
     const int lane = (threadIdx.x + blockDim.x * threadIdx.y) & 63;
     __builtin_amdgcn_wave_barrier();  // the wave's reads of the previous table are issued
-    for (int n = lane; n < ntab * 9; n += 64) mine[n] = g_tab[n];
+    for (int n = lane; n < ntab * 9; n += 64) mine[n] = gp(g_tab)[n];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -327,7 +370,10 @@ __global__ void __launch_bounds__(256)
     const int local = t - tile_start[g];
     const int kb = local % D.nkb;
     const int rest = local / D.nkb;
-    yee_h3_tile<S, I, PML, NT>(D, tab, kb, rest % D.njb, rest / D.njb);
+    if (tab == mine)
+      yee_h3_tile<S, I, PML, NT, false, TAB_LDS>(D, tab, kb, rest % D.njb, rest / D.njb);
+    else
+      yee_h3_tile<S, I, PML, NT, false, TAB_GLOBAL>(D, tab, kb, rest % D.njb, rest / D.njb);
   }
 }
 
@@ -346,6 +392,9 @@ __global__ void __launch_bounds__(256)
     const int local = t - tile_start[g];
     const int kb = local % D.nkb;
     const int rest = local / D.nkb;
-    yee_e3_tile<S, I, PML, false, NT>(D, tab, kb, rest % D.njb, rest / D.njb);
+    if (tab == mine)
+      yee_e3_tile<S, I, PML, false, NT, false, TAB_LDS>(D, tab, kb, rest % D.njb, rest / D.njb);
+    else
+      yee_e3_tile<S, I, PML, false, NT, false, TAB_GLOBAL>(D, tab, kb, rest % D.njb, rest / D.njb);
   }
 }
