@@ -106,6 +106,12 @@ def main(argv=None) -> int:
                          "streamed by the kernels (XF_COEFF_DENSE)")
     ap.add_argument("--td-boxes", type=int, default=3, help="NF2FF-TD boxes of nf2ff_td")
     ap.add_argument("--count", type=int, default=64, help="grids of the batch workload")
+    ap.add_argument("--batch-tuning", default=None,
+                    help="batch workload: fixed tiling 'tile_y,planes,k32,xcd' instead of the "
+                         "autotuner (for counter runs)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batch workload: time only the batch in the chosen precision (no "
+                         "sequential, FP64 or learned-layer runs; for counter runs)")
     ap.add_argument("--full-steps", type=int, default=2000,
                     help="batch workload: steps of the full runs whose far fields are compared "
                          "(FP32 batch and the learned-layer batch against the FP64 batch; 0 skips)")
@@ -293,7 +299,12 @@ def bench_batch(args) -> int:
 
     def timed_batch(precision):
         b = xf.Batch(sims, precision=precision, use_graph=args.graph).upload()
-        best = b.autotune(trial_steps=3)
+        if args.batch_tuning:
+            ty, planes, k32, xcd = (int(v) for v in args.batch_tuning.split(","))
+            b.set_tuning(tile_y=ty, planes=planes, k32=bool(k32), xcd_remap=bool(xcd))
+            best = b.tuning()
+        else:
+            best = b.autotune(trial_steps=3)
         b.run(W)
         b.reset_timing()
         b.run(K)
@@ -309,6 +320,14 @@ def bench_batch(args) -> int:
     run_ms, tm, best, desc = timed_batch(prec)
     print(f"# batch {prec}: {run_ms / K:.3f} ms/step, {desc}", file=sys.stderr, flush=True)
     value = cells * K / (run_ms * 1e-3) / 1e9
+    if args.batch_only:
+        h_ms = tm["yee_h_ms"] / tm["yee_h_launches"]
+        e_ms = tm["yee_e_ms"] / tm["yee_e_launches"]
+        print(json.dumps({"metric": "Yee cell-updates/s (Gcells/s) & % HBM roofline",
+                          "value": round(value, 4), "unit": "Gcells/s", "steps": K, "warmup": W,
+                          "ms_per_step": round(run_ms / K, 4), "batch_only": True, "describe": desc,
+                          "h_ms": round(h_ms, 4), "e_ms": round(e_ms, 4)}), flush=True)
+        return 0
 
     # the same grids one after another through the single-grid engine
     seq_ms = 0.0

@@ -420,7 +420,7 @@ typedef struct xf_tuning {
   int32_t planes;      /* x planes one block marches through (>= 1)        */
   int32_t blocks;      /* batch only: persistent grid size, 0 = resident   */
   int32_t nontemporal; /* stream the updated component non-temporally      */
-  int32_t xcd_remap;   /* single grid only: XCD-aware block order          */
+  int32_t xcd_remap;   /* XCD-aware block order (batch: each XCD takes a contiguous eighth of the tiles) */
   int32_t pml_split;   /* XF_TILE_PML_SPLIT                                */
   int32_t k32;         /* XF_TILE_K32                                      */
   double step_ms;      /* autotune: measured time of one step              */

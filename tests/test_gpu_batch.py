@@ -88,19 +88,26 @@ def test_batch_matches_oracle_grid_by_grid(gpu_available):
     assert np.abs(ref[0].array("ex")).max() > 1e-3
 
 
-@pytest.mark.parametrize("tile_y,planes,blocks,nt,split,k32", [
-    (1, 1, 0, True, False, False), (2, 3, 7, False, True, False), (4, 100, 1, True, False, False),
-    (3, 2, 0, False, True, False), (1, 4, 13, True, True, False), (1, 4, 0, False, True, True),
-    (4, 2, 5, True, False, True)])
-def test_batch_tunings_bit_identical(gpu_available, tile_y, planes, blocks, nt, split, k32):
+@pytest.mark.parametrize("tile_y,planes,blocks,nt,split,k32,xcd", [
+    (1, 1, 0, True, False, False, False), (2, 3, 7, False, True, False, False),
+    (4, 100, 1, True, False, False, False), (3, 2, 0, False, True, False, False),
+    (1, 4, 13, True, True, False, False), (1, 4, 0, False, True, True, False),
+    (4, 2, 5, True, False, True, False),
+    # XCD-contiguous tile walk: resident grid, block counts not a multiple of
+    # 8, fewer than 8 blocks (plain walk), and with 32-lane tiles
+    (1, 4, 0, True, True, False, True), (2, 2, 13, True, True, False, True),
+    (4, 8, 5, False, False, False, True), (1, 3, 0, True, True, True, True)])
+def test_batch_tunings_bit_identical(gpu_available, tile_y, planes, blocks, nt, split, k32, xcd):
     steps = 30
     build = lambda: _uneven(steps, count=4, seed=11)
     ref = _oracle_copies(build, steps)
     sims = build()
     b = xf.Batch(sims).upload()
-    b.set_tuning(tile_y=tile_y, planes=planes, blocks=blocks, nontemporal=nt, pml_split=split, k32=k32)
+    b.set_tuning(tile_y=tile_y, planes=planes, blocks=blocks, nontemporal=nt, pml_split=split, k32=k32,
+                 xcd_remap=xcd)
     assert ("pml_split=0" not in b.describe()) == split
     assert ("k32=1" in b.describe()) == k32
+    assert ("xcd=1" in b.describe()) == xcd
     b.run(steps).download()
     b.close()
     _assert_identical(sims, ref, f"tuning ty={tile_y} planes={planes} blocks={blocks} split={split}")

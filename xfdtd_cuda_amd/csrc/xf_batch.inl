@@ -239,6 +239,11 @@ int Batch::configure_t() {
   void* p = tile_start;
   XF_TRY(put(&p, ts.data(), ts.size() * sizeof(int)));
   tile_start = static_cast<int*>(p);
+  // Per-item thread ranges of the one-launch passes below are padded to
+  // whole 256-thread blocks, so each block serves one item and finds it
+  // from its first thread (a block-uniform index: the item's descriptor is
+  // then read with scalar loads).
+  auto whole_blocks = [](int64_t count) { return (count + 255) / 256 * 256; };
   // z-face CPML corrections of the grids that split them off
   for (int pass = 0; pass < 2; ++pass) {
     std::vector<PmlZItem<S>> items;
@@ -254,7 +259,7 @@ int Batch::configure_t() {
       it.B = pml_z_box<C>(E, it.P);
       if (it.B.start[it.B.n] == 0) continue;
       items.push_back(it);
-      st.push_back(st.back() + it.B.start[it.B.n]);
+      st.push_back(st.back() + whole_blocks(it.B.start[it.B.n]));
     }
     n_pz[pass] = static_cast<int>(items.size());
     pz_total[pass] = st.back();
@@ -282,7 +287,7 @@ int Batch::configure_t() {
       fill_tfsf_pass<C>(E, it.P, pass);
       if (it.P.total == 0) continue;
       items.push_back(it);
-      st.push_back(st.back() + it.P.total);
+      st.push_back(st.back() + whole_blocks(it.P.total));
     }
     n_tf[pass] = static_cast<int>(items.size());
     tf_total[pass] = st.back();
@@ -304,7 +309,7 @@ int Batch::configure_t() {
         it.d = make_dims(E);
         it.P = N.pass;
         items.push_back(it);
-        st.push_back(st.back() + N.pass.total);
+        st.push_back(st.back() + whole_blocks(N.pass.total));
       }
     }
     n_nf = static_cast<int>(items.size());
@@ -394,7 +399,7 @@ int Batch::configure() {
          " resident=" + std::to_string(resident) + " tile_y=" + std::to_string(tune.tile_y) +
          " planes=" + std::to_string(tune.planes) + " nt=" + std::to_string(tune.nontemporal) +
          " id_bytes=" + std::to_string(id_bytes) + " pml_split=" + std::to_string(n_pz[1]) +
-         " k32=" + std::to_string(tune.k32) +
+         " k32=" + std::to_string(tune.k32) + " xcd=" + std::to_string(tune.xcd_remap) +
          " tfsf=" + std::to_string(n_tf[0]) +
          " nf2ff=" + std::to_string(n_nf);
   return XF_OK;
@@ -407,7 +412,8 @@ int Batch::launch_step(int t, bool use_tdev, bool ev_on, hipEvent_t* ev) {
   const dim3 block(tune.k32 ? 32 : 64, tune.k32 ? 2 * tune.tile_y : tune.tile_y), grid(grid_blocks);
   const bool nt_on = tune.nontemporal != 0;
 #define XF_BY(KER, I, PM, NT, PASS) \
-  hipLaunchKernelGGL((KER<S, I, PM, NT>), grid, block, 0, stream, static_cast<const YeePass<S>*>(PASS), tile_start, n)
+  hipLaunchKernelGGL((KER<S, I, PM, NT>), grid, block, 0, stream, static_cast<const YeePass<S>*>(PASS), tile_start, n, \
+                     tune.xcd_remap)
 #define XF_BYALL(KER, PASS)                                                                    \
   do {                                                                                         \
     if (id_bytes == 1) {                                                                       \
@@ -631,12 +637,13 @@ int Batch::autotune(int trial_steps, xf_tuning* best) {
     h->eng.pml_split = keep_split;
   }
   for (int ty : {1, 2, 4}) {
-    for (int planes : {2, 4, 8}) {
+    for (int planes : {2, 4, 8, 16}) {
       for (int nt_on : {1, 0}) {
-        for (int cand = 0; cand < 8; ++cand) {
-          const int half = cand & 1, split = (cand >> 1) & 1, k32 = cand >> 2;
+        for (int cand = 0; cand < 16; ++cand) {
+          const int half = cand & 1, split = (cand >> 1) & 1, k32 = (cand >> 2) & 1, xcd = cand >> 3;
           if (split && !can_split) continue;
-          tune = xf_tuning{ty, planes, 0, nt_on, 0, split, k32, 0.0};
+          if (xcd && half) continue;
+          tune = xf_tuning{ty, planes, 0, nt_on, xcd, split, k32, 0.0};
           status = configure();
           if (status == XF_OK && half) {
             tune.blocks = std::max(1, resident / 2);

@@ -815,10 +815,10 @@ __device__ __forceinline__ int64_t lb_node(const LbDev& L, const Dims& d, const 
   return (int64_t)pos[0] * d.sx + (int64_t)pos[1] * d.sy + pos[2];
 }
 
-__device__ __forceinline__ float* lb_hist(const LbDev& L, const LbCol& c, int par, int layer,
-                                          int p, int q) {
+__device__ __forceinline__ __attribute__((address_space(1))) float* lb_hist(const LbDev& L, const LbCol& c,
+                                                                           int par, int layer, int p, int q) {
   const int64_t plane = ((int64_t)par * 2 + c.cc) * 4 + layer;
-  return L.hist + c.hist_off + (c.tr ? (plane * c.nb1 + q) * c.na1 + p : (plane * c.na1 + p) * c.nb1 + q);
+  return gp(L.hist) + c.hist_off + (c.tr ? (plane * c.nb1 + q) * c.na1 + p : (plane * c.na1 + p) * c.nb1 + q);
 }
 
 // The face a node of the column also lies on (its component tangential to
@@ -888,7 +888,7 @@ __device__ __forceinline__ void lb_column_step(const Fields<S>& f, const LbDev& 
   lb_inputs<S>(f, L, d, sc, c, par, x);
   const int64_t node0 = lb_node(L, d, c, 0);
   if (L.mode == XF_LB_RECORD) {
-    float* r = L.rec + ((int64_t)(t / L.rec_every) * L.total + c.abi_col) * 13;
+    __attribute__((address_space(1))) float* r = gp(L.rec) + ((int64_t)(t / L.rec_every) * L.total + c.abi_col) * 13;
 #pragma unroll
     for (int i = 0; i < XF_LB_IN; ++i) r[i] = x[i];
     r[12] = (float)Store<S>::ld(fld, node0, sc.inv_e);
@@ -934,19 +934,22 @@ __device__ __forceinline__ void lb_mfma_tiles(const Fields<S>& f, const LbDev& L
                                               int64_t col0, int ntiles) {
   using C = typename Store<S>::C;
   const int lane = threadIdx.x & 63, g = lane >> 5;
-  const LbPack& P = *L.pack;
-  const lb_h8 a1 = __builtin_bit_cast(lb_h8, *reinterpret_cast<const uint4*>(P.a1[lane]));
-  const lb_h8 a20 = __builtin_bit_cast(lb_h8, *reinterpret_cast<const uint4*>(P.a2[0][lane]));
-  const lb_h8 a21 = __builtin_bit_cast(lb_h8, *reinterpret_cast<const uint4*>(P.a2[1][lane]));
+  // the pack is read through a global-space pointer (it is found in a
+  // descriptor, so the compiler cannot tell its address space)
+  typedef unsigned lb_u4 __attribute__((ext_vector_type(4)));
+  const __attribute__((address_space(1))) LbPack* P = gp(L.pack);
+  const lb_h8 a1 = __builtin_bit_cast(lb_h8, *gp(reinterpret_cast<const lb_u4*>(L.pack->a1[lane])));
+  const lb_h8 a20 = __builtin_bit_cast(lb_h8, *gp(reinterpret_cast<const lb_u4*>(L.pack->a2[0][lane])));
+  const lb_h8 a21 = __builtin_bit_cast(lb_h8, *gp(reinterpret_cast<const lb_u4*>(L.pack->a2[1][lane])));
   lb_f16 c1, c2;
   float w3[16];
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
-    c1[i] = P.c1[i][lane];
-    c2[i] = P.c2[i][lane];
-    w3[i] = P.w3[i][lane];
+    c1[i] = P->c1[i][lane];
+    c2[i] = P->c2[i][lane];
+    w3[i] = P->w3[i][lane];
   }
-  const float b3 = P.b3;
+  const float b3 = P->b3;
   const int par = t & 1;
   for (int q = 0; q < ntiles; ++q) {
     const int64_t col = col0 + 64 * q + lane;
@@ -1052,16 +1055,22 @@ __device__ __forceinline__ int owner_of(const int64_t* __restrict__ start, int n
   return lo;
 }
 
+// In the one-launch batch passes below, item q owns the 256-thread blocks
+// covering [start[q], start[q + 1]) (ranges padded to whole blocks), so the
+// item is found from the block's first thread and is the same for the whole
+// block.
 // The TFSF pass of every grid of a batch in one launch.
 template <typename S>
 __global__ void __launch_bounds__(256)
     k_tfsf_batch(const TfsfItem<S>* __restrict__ items, const int64_t* __restrict__ start, int n,
                  int t_arg, const int* t_dev) {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= start[n]) return;
-  const int g = owner_of(start, n, tid);
+  const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
+  if (b0 >= start[n]) return;
+  const int g = owner_of(start, n, b0);
   const TfsfItem<S>& I = items[g];
-  tfsf_cell<S>(I.f, I.P, t_dev ? *t_dev : t_arg, I.sc, I.d, 0, tid - start[g]);
+  const int64_t local = b0 - start[g] + threadIdx.x;
+  if (local >= I.P.total) return;
+  tfsf_cell<S>(I.f, I.P, t_dev ? *t_dev : t_arg, I.sc, I.d, 0, local);
 }
 
 // The NF2FF accumulation of every box of a batch in one launch.
@@ -1069,11 +1078,13 @@ template <typename S>
 __global__ void __launch_bounds__(256)
     k_nf2ff_batch(const NfItem<S>* __restrict__ items, const int64_t* __restrict__ start, int n,
                   int t_arg, const int* t_dev) {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= start[n]) return;
-  const int g = owner_of(start, n, tid);
+  const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
+  if (b0 >= start[n]) return;
+  const int g = owner_of(start, n, b0);
   const NfItem<S>& I = items[g];
-  nf2ff_cell<S>(I.f, I.P, t_dev ? *t_dev : t_arg, I.d, 0, I.sc, tid - start[g]);
+  const int64_t local = b0 - start[g] + threadIdx.x;
+  if (local >= I.P.total) return;
+  nf2ff_cell<S>(I.f, I.P, t_dev ? *t_dev : t_arg, I.d, 0, I.sc, local);
 }
 
 // The z-face CPML corrections of every grid of a batch that splits them
@@ -1081,11 +1092,13 @@ __global__ void __launch_bounds__(256)
 template <typename S, bool ISE>
 __global__ void __launch_bounds__(256)
     k_pml_z_batch(const PmlZItem<S>* __restrict__ items, const int64_t* __restrict__ start, int n) {
-  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= start[n]) return;
-  const int g = owner_of(start, n, tid);
+  const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
+  if (b0 >= start[n]) return;
+  const int g = owner_of(start, n, b0);
   const PmlZItem<S>& I = items[g];
-  pml_z_node<S, ISE>(I.f, I.P, I.B, I.d, I.sc, tid - start[g]);
+  const int64_t local = b0 - start[g] + threadIdx.x;
+  if (local >= I.B.start[I.B.n]) return;
+  pml_z_node<S, ISE>(I.f, I.P, I.B, I.d, I.sc, local);
 }
 
 // The point sources of every grid of a batch in one launch: thread g adds

@@ -333,6 +333,25 @@ __device__ __forceinline__ int batch_instance(const int* __restrict__ tile_start
   return lo;
 }
 
+// The tiles a persistent batch block takes.  xcd = 0: block b takes tiles
+// b, b + G, b + 2G, ... (G blocks).  xcd = 1: blocks are dealt to the 8 XCDs
+// round-robin by index, so the blocks b = x (mod 8) of XCD x take the
+// contiguous eighth [x S, x S + S) of the tile list (S = ceil(T / 8)), with
+// their count as stride: tiles next to each other, which read each other's
+// halo rows, then go through one XCD's L2.  Every tile is taken once, and
+// every block ends when its range does.
+struct BatchWalk {
+  int first, end, stride;
+};
+__device__ __forceinline__ BatchWalk batch_walk(int total, int xcd) {
+  if (!xcd || gridDim.x < 8) return BatchWalk{(int)blockIdx.x, total, (int)gridDim.x};
+  const int x = blockIdx.x & 7, r = blockIdx.x >> 3;
+  const int per = ((int)gridDim.x - x + 7) >> 3;
+  const int span = (total + 7) >> 3;
+  const int lo = min(x * span, total), hi = min(lo + span, total);
+  return BatchWalk{lo + r, hi, per};
+}
+
 // Each wave keeps the table of the grid it is working on in its own slice
 // of LDS (kBatchTab values, written by the wave when its grid changes; no
 // block barrier), read like the single-grid kernels' staged table; grids
@@ -357,13 +376,14 @@ __device__ __forceinline__ const C* batch_table(const C* __restrict__ g_tab, int
 
 template <typename S, typename I, bool PML, bool NT>
 __global__ void __launch_bounds__(256)
-    k_yee_h3_batch(const YeePass<S>* __restrict__ passes, const int* __restrict__ tile_start, int n) {
+    k_yee_h3_batch(const YeePass<S>* __restrict__ passes, const int* __restrict__ tile_start, int n,
+                     int xcd) {
   using C = typename Store<S>::C;
   __shared__ C wtab[4][kBatchTab];
   C* mine = wtab[(threadIdx.x + blockDim.x * threadIdx.y) >> 6];
   int cur_g = -1;
-  const int total = tile_start[n];
-  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+  const BatchWalk w = batch_walk(tile_start[n], xcd);
+  for (int t = w.first; t < w.end; t += w.stride) {
     const int g = batch_instance(tile_start, n, t);
     const YeePass<S>& D = passes[g];
     const C* tab = batch_table<C>(D.tab, D.ntab, mine, g, cur_g);
@@ -379,13 +399,14 @@ __global__ void __launch_bounds__(256)
 
 template <typename S, typename I, bool PML, bool NT>
 __global__ void __launch_bounds__(256)
-    k_yee_e3_batch(const YeePass<S>* __restrict__ passes, const int* __restrict__ tile_start, int n) {
+    k_yee_e3_batch(const YeePass<S>* __restrict__ passes, const int* __restrict__ tile_start, int n,
+                     int xcd) {
   using C = typename Store<S>::C;
   __shared__ C wtab[4][kBatchTab];
   C* mine = wtab[(threadIdx.x + blockDim.x * threadIdx.y) >> 6];
   int cur_g = -1;
-  const int total = tile_start[n];
-  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+  const BatchWalk w = batch_walk(tile_start[n], xcd);
+  for (int t = w.first; t < w.end; t += w.stride) {
     const int g = batch_instance(tile_start, n, t);
     const YeePass<S>& D = passes[g];
     const C* tab = batch_table<C>(D.tab, D.ntab, mine, g, cur_g);

@@ -587,8 +587,9 @@ class Batch:
         return self
 
     def set_tuning(self, tile_y: int = 1, planes: int = 4, blocks: int = 0,
-                   nontemporal: bool = True, pml_split: bool = True, k32: bool = False):
-        t = XfTuning(tile_y, planes, blocks, int(nontemporal), 0, int(pml_split), int(k32), 0.0)
+                   nontemporal: bool = True, pml_split: bool = True, k32: bool = False,
+                   xcd_remap: bool = False):
+        t = XfTuning(tile_y, planes, blocks, int(nontemporal), int(xcd_remap), int(pml_split), int(k32), 0.0)
         _native.check_hip(self._lib.xf_batch_set_tuning(self._h, ctypes.byref(t)),
                           "xf_batch_set_tuning")
         return self
