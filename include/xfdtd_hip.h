@@ -339,6 +339,11 @@ enum xf_tile_flags {
                             in two launches on two streams, running
                             together (the CPML-free launch without the
                             CPML code and its registers); 3-D, table mode */
+  XF_TILE_PML_ZTAIL = 64, /* z-face CPML corrections applied by each Yee
+                            tile after its march, by the lanes of the slab
+                            columns, while the tile's lines are still in
+                            cache (3-D, FP64/FP32 only; takes precedence
+                            over XF_TILE_PML_SPLIT; results unchanged) */
   XF_TILE_SET = 256      /* the other bits are explicit, not auto        */
 };
 
@@ -421,7 +426,7 @@ typedef struct xf_tuning {
   int32_t blocks;      /* batch only: persistent grid size, 0 = resident   */
   int32_t nontemporal; /* stream the updated component non-temporally      */
   int32_t xcd_remap;   /* XCD-aware block order (batch: each XCD takes a contiguous eighth of the tiles) */
-  int32_t pml_split;   /* XF_TILE_PML_SPLIT                                */
+  int32_t pml_split;   /* 0 z faces in the march, 1 XF_TILE_PML_SPLIT, 2 XF_TILE_PML_ZTAIL */
   int32_t k32;         /* XF_TILE_K32                                      */
   double step_ms;      /* autotune: measured time of one step              */
 } xf_tuning;

@@ -96,16 +96,19 @@ def test_batch_matches_oracle_grid_by_grid(gpu_available):
     # XCD-contiguous tile walk: resident grid, block counts not a multiple of
     # 8, fewer than 8 blocks (plain walk), and with 32-lane tiles
     (1, 4, 0, True, True, False, True), (2, 2, 13, True, True, False, True),
-    (4, 8, 5, False, False, False, True), (1, 3, 0, True, True, True, True)])
+    (4, 8, 5, False, False, False, True), (1, 3, 0, True, True, True, True),
+    # z-face CPML at the end of each tile (split = 2)
+    (1, 4, 0, True, 2, False, False), (2, 3, 13, False, 2, True, True), (4, 16, 0, True, 2, True, False)])
 def test_batch_tunings_bit_identical(gpu_available, tile_y, planes, blocks, nt, split, k32, xcd):
     steps = 30
     build = lambda: _uneven(steps, count=4, seed=11)
     ref = _oracle_copies(build, steps)
     sims = build()
     b = xf.Batch(sims).upload()
-    b.set_tuning(tile_y=tile_y, planes=planes, blocks=blocks, nontemporal=nt, pml_split=split, k32=k32,
-                 xcd_remap=xcd)
-    assert ("pml_split=0" not in b.describe()) == split
+    b.set_tuning(tile_y=tile_y, planes=planes, blocks=blocks, nontemporal=nt, pml_split=bool(split), k32=k32,
+                 xcd_remap=xcd, pml_ztail=split == 2)
+    assert ("pml_split=0" not in b.describe()) == (split is True)
+    assert ("pml_ztail=1" in b.describe()) == (split == 2)
     assert ("k32=1" in b.describe()) == k32
     assert ("xcd=1" in b.describe()) == xcd
     b.run(steps).download()

@@ -58,19 +58,28 @@ def _assert_same(g, c, label):
     return len(names)
 
 
-def test_full_size_sphere_fp64_bit_identical(gpu_available):
+@pytest.mark.parametrize("tiling", ["default", "split"])
+def test_full_size_sphere_fp64_bit_identical(gpu_available, tiling):
+    """The bench's 536^3 scene from a random state: the default FP64 tiling
+    (z-face CPML at the end of each 3-plane, 32-lane tile) and the split
+    z-face launch."""
     steps = 3
-    print("case: 536^3 FP64 sphere, random state, 3 steps", flush=True)
+    print(f"case: 536^3 FP64 sphere, random state, 3 steps, {tiling} tiling", flush=True)
     g = scenes.large_sphere(n=512, pml=12, steps=steps + 1, layout="compact")
     c = scenes.large_sphere(n=512, pml=12, steps=steps + 1, layout="compact")
     _randomise([g, c], 21)
     e = xf.Engine(g).upload()
+    if tiling == "split":
+        e.set_tiling(tile_y=1, pml_split=True)
     desc = e.describe()
     e.run(1).run(steps - 1).download()
     e.close()
     oracle.run(c.scene_ptr(), steps)
     n = _assert_same(g, c, "536^3")
-    assert "pml_split=1" in desc
+    if tiling == "split":
+        assert "pml_split=1" in desc and "pml_ztail=0" in desc, desc
+    else:
+        assert "pml_ztail=1" in desc and "klanes=32" in desc and "chunk=3" in desc, desc
     assert n > 6 + 24
     assert np.abs(c.array("pml.5.ea_psi_hb")).max() > 0
 

@@ -347,6 +347,76 @@ def test_pml_split_single_z_face(gpu_available):
     assert np.abs(c.array("pml.1.ea_psi_hb")).max() > 0
 
 
+def _uneven_faces(steps):
+    s = xf.Simulation(3e-3, 3e-3, 3e-3, 0.95)
+    s.add_cube("domain", (-0.05, -0.04, -0.062), (0.1, 0.08, 0.124), xf.Material.air())
+    s.add_sphere("s", (0.004, -0.003, 0.01), 0.021, xf.Material.dielectric(4.0, 1.5, 0.01))
+    s.add_pml(5, "XN").add_pml(7, "XP").add_pml(4, "YN").add_pml(6, "YP")
+    s.add_pml(3, "ZN").add_pml(9, "ZP")
+    tau = 20 * 3e-3 / 6e8
+    s.add_tfsf3d((11, 9, 12), 0.7, 0.4, 0.3, xf.Waveform.gaussian(tau, 4.5 * tau))
+    s.add_nf2ff_fd((9, 8, 10), [1.5e9])
+    s.init(steps)
+    return s
+
+
+@pytest.mark.parametrize("scene,tile_y,chunks,k32,streams", [
+    ("sphere", 1, 0, False, False), ("sphere", 4, 1, False, False), ("sphere", 2, 7, True, False),
+    ("sphere", 3, 1000, False, True), ("uneven", 2, 5, False, False), ("uneven", 1, 0, True, True),
+    ("one_z", 1, 3, False, False)])
+def test_pml_ztail_bit_identical(gpu_available, scene, tile_y, chunks, k32, streams):
+    """The z-face CPML corrections applied at the end of each Yee tile by
+    the slab columns' own lanes (XF_TILE_PML_ZTAIL): bit-identical to the
+    oracle, fields and every psi array, for tile shapes and march lengths
+    that put the z slabs at tile starts, inside tiles and across several
+    k-tiles, with one or two z faces of different thicknesses."""
+    steps = 90
+    if scene == "sphere":
+        build = lambda: scenes.dielectric_sphere(n=21, pml=5, steps=steps, tfsf_inset=8, nf2ff_inset=6)
+    elif scene == "uneven":
+        build = lambda: _uneven_faces(steps)
+    else:
+        def build():
+            s = xf.Simulation(3e-3, 3e-3, 3e-3, 0.95)
+            s.add_cube("domain", (-0.04, -0.04, -0.05), (0.08, 0.08, 0.1), xf.Material.air())
+            s.add_sphere("s", (0.002, 0.0, 0.004), 0.015, xf.Material.dielectric(3.0, 1.0, 0.02))
+            s.add_pml(5, "XN").add_pml(6, "ZP")
+            tau = 20 * 3e-3 / 6e8
+            s.add_tfsf3d((8, 8, 9), 0.5, 0.9, 0.2, xf.Waveform.gaussian(tau, 4.5 * tau))
+            s.init(steps)
+            return s
+    print(f"case: ztail {scene} ty={tile_y} ch={chunks} k32={k32} streams={streams}", flush=True)
+    g, c = build(), build()
+    e = xf.Engine(g).upload()
+    e.set_tiling(tile_y=tile_y, chunks_x=chunks, pml_ztail=True, k32=k32, pml_streams=streams)
+    desc = e.describe()
+    assert "pml_ztail=1" in desc and "pml_split=0" in desc, desc
+    e.run(35).run(steps - 35).download()
+    e.close()
+    oracle.run(c.scene_ptr(), steps)
+    for n in _state_names(c):
+        assert np.array_equal(g.array(n), c.array(n)), f"{n} differs ({scene} ty={tile_y} ch={chunks})"
+
+
+def test_pml_ztail_fp32_and_bf16(gpu_available):
+    """FP32 with the tail equals FP32 with the z faces in the march, bit for
+    bit; BF16 storage keeps the z faces in the march."""
+    def run(sim, precision, tail):
+        e = xf.Engine(sim, precision=precision).upload()
+        e.set_tiling(tile_y=2, pml_split=False, pml_ztail=tail)
+        desc = e.describe()
+        e.run(40).run(40).download()
+        e.close()
+        return desc
+
+    a, b = _uneven_faces(80), _uneven_faces(80)
+    assert "pml_ztail=1" in run(a, "fp32", True)
+    assert "pml_ztail=0" in run(b, "fp32", False)
+    for n in FIELDS:
+        assert np.array_equal(a.array(n), b.array(n)), n
+    assert "pml_ztail=0" in run(_uneven_faces(80), "bf16", True)
+
+
 def test_fp32_close_to_fp64(gpu_available):
     build = lambda: scenes.dielectric_sphere(n=24, pml=6, steps=140, tfsf_inset=9, nf2ff_inset=7)
     g, c = _pair(build, 140, precision="fp32")

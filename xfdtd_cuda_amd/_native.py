@@ -56,6 +56,7 @@ TILE_NT = 4
 TILE_PML_SPLIT = 8
 TILE_K32 = 16
 TILE_PML_STREAMS = 32
+TILE_PML_ZTAIL = 64
 TILE_SET = 256
 
 

@@ -25,7 +25,7 @@ struct Batch {
   int64_t cur_step{0};
   int64_t nt{INT64_MAX};  // end of the shortest time table
   int id_bytes{1};
-  bool any_pml{false};
+  int pml_level{0};  // 0 no CPML, 1 CPML faces in the march, 2 and z faces at the tile end
   int64_t cells{0};
   xf_tuning tune{1, 4, 0, 1, 0, 1, 0, 0.0};
   // device descriptors (sizes fixed by the scenes, rewritten by configure)
@@ -189,7 +189,7 @@ int Batch::upload() {
 }
 
 namespace {
-template <typename S, typename I, bool PM, bool NT>
+template <typename S, typename I, int PM, bool NT>
 int batch_occupancy(int threads) {
   int a = 0, b = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_yee_h3_batch<S, I, PM, NT>, threads, 0) != hipSuccess) a = 1;
@@ -197,17 +197,22 @@ int batch_occupancy(int threads) {
   return std::max(1, std::min(a, b));
 }
 
+template <typename S, typename I, int PM>
+int batch_occupancy_nt(bool nt, int threads) {
+  return nt ? batch_occupancy<S, I, PM, true>(threads) : batch_occupancy<S, I, PM, false>(threads);
+}
+
+template <typename S, typename I>
+int batch_occupancy_lv(int level, bool nt, int threads) {
+  if (level == 2) return batch_occupancy_nt<S, I, 2>(nt, threads);
+  if (level == 1) return batch_occupancy_nt<S, I, 1>(nt, threads);
+  return batch_occupancy_nt<S, I, 0>(nt, threads);
+}
+
 template <typename S>
-int batch_occupancy_any(int id_bytes, bool pml, bool nt, int threads) {
-#define XF_OCC(I, PM, NT) return batch_occupancy<S, I, PM, NT>(threads)
-  if (id_bytes == 1) {
-    if (pml) { if (nt) XF_OCC(uint8_t, true, true); else XF_OCC(uint8_t, true, false); }
-    else { if (nt) XF_OCC(uint8_t, false, true); else XF_OCC(uint8_t, false, false); }
-  } else {
-    if (pml) { if (nt) XF_OCC(uint16_t, true, true); else XF_OCC(uint16_t, true, false); }
-    else { if (nt) XF_OCC(uint16_t, false, true); else XF_OCC(uint16_t, false, false); }
-  }
-#undef XF_OCC
+int batch_occupancy_any(int id_bytes, int level, bool nt, int threads) {
+  return id_bytes == 1 ? batch_occupancy_lv<S, uint8_t>(level, nt, threads)
+                       : batch_occupancy_lv<S, uint16_t>(level, nt, threads);
 }
 }  // namespace
 
@@ -218,16 +223,17 @@ int Batch::configure_t() {
   std::vector<YeePass<S>> ph(n), pe(n);
   std::vector<int> ts(n + 1, 0);
   int64_t tiles = 0;
-  any_pml = false;
-  const int flags = XF_TILE_SET | (tune.nontemporal ? XF_TILE_NT : 0) | (tune.pml_split ? XF_TILE_PML_SPLIT : 0) |
-                    (tune.k32 ? XF_TILE_K32 : 0);
+  pml_level = 0;
+  const int flags = XF_TILE_SET | (tune.nontemporal ? XF_TILE_NT : 0) | (tune.pml_split == 1 ? XF_TILE_PML_SPLIT : 0) |
+                    (tune.pml_split == 2 ? XF_TILE_PML_ZTAIL : 0) | (tune.k32 ? XF_TILE_K32 : 0);
   for (int g = 0; g < n; ++g) {
     Engine& E = eng(g);
     const int64_t chunks = (E.nx + 1 + tune.planes - 1) / tune.planes;
     E.set_tiling(tune.tile_y, static_cast<int>(chunks), flags);
     ph[g] = E.yee_pass<S>(false);
     pe[g] = E.yee_pass<S>(true);
-    any_pml = any_pml || ph[g].pml.n > 0;
+    if (ph[g].zt.n > 0 || pe[g].zt.n > 0) pml_level = 2;
+    else if ((ph[g].pml.n > 0 || pe[g].pml.n > 0) && pml_level == 0) pml_level = 1;
     ts[g] = static_cast<int>(tiles);
     tiles += static_cast<int64_t>(ph[g].nkb) * ph[g].njb * ph[g].nch;
     if (tiles > INT_MAX / 2) return fail(XF_ERR_UNSUPPORTED, "batch too large for 32-bit tile ids");
@@ -372,7 +378,7 @@ int Batch::configure_t() {
   // persistent grid: what the device holds resident, unless set
   hipDeviceProp_t prop;
   XF_HIP(hipGetDeviceProperties(&prop, device));
-  resident = batch_occupancy_any<S>(id_bytes, any_pml, tune.nontemporal != 0, 64 * tune.tile_y) *
+  resident = batch_occupancy_any<S>(id_bytes, pml_level, tune.nontemporal != 0, 64 * tune.tile_y) *
              prop.multiProcessorCount;
   grid_blocks = tune.blocks > 0 ? tune.blocks : resident;
   grid_blocks = std::max(1, std::min(grid_blocks, total_tiles));
@@ -399,6 +405,7 @@ int Batch::configure() {
          " resident=" + std::to_string(resident) + " tile_y=" + std::to_string(tune.tile_y) +
          " planes=" + std::to_string(tune.planes) + " nt=" + std::to_string(tune.nontemporal) +
          " id_bytes=" + std::to_string(id_bytes) + " pml_split=" + std::to_string(n_pz[1]) +
+         " pml_ztail=" + std::to_string(tune.pml_split == 2 ? 1 : 0) +
          " k32=" + std::to_string(tune.k32) + " xcd=" + std::to_string(tune.xcd_remap) +
          " tfsf=" + std::to_string(n_tf[0]) +
          " nf2ff=" + std::to_string(n_nf);
@@ -414,15 +421,19 @@ int Batch::launch_step(int t, bool use_tdev, bool ev_on, hipEvent_t* ev) {
 #define XF_BY(KER, I, PM, NT, PASS) \
   hipLaunchKernelGGL((KER<S, I, PM, NT>), grid, block, 0, stream, static_cast<const YeePass<S>*>(PASS), tile_start, n, \
                      tune.xcd_remap)
+#define XF_BYNT(KER, I, PM, PASS)                                                              \
+  do {                                                                                         \
+    if (nt_on) XF_BY(KER, I, PM, true, PASS); else XF_BY(KER, I, PM, false, PASS);             \
+  } while (0)
+#define XF_BYLV(KER, I, PASS)                                                                  \
+  do {                                                                                         \
+    if (pml_level == 2) XF_BYNT(KER, I, 2, PASS);                                              \
+    else if (pml_level == 1) XF_BYNT(KER, I, 1, PASS);                                         \
+    else XF_BYNT(KER, I, 0, PASS);                                                             \
+  } while (0)
 #define XF_BYALL(KER, PASS)                                                                    \
   do {                                                                                         \
-    if (id_bytes == 1) {                                                                       \
-      if (any_pml) { if (nt_on) XF_BY(KER, uint8_t, true, true, PASS); else XF_BY(KER, uint8_t, true, false, PASS); } \
-      else { if (nt_on) XF_BY(KER, uint8_t, false, true, PASS); else XF_BY(KER, uint8_t, false, false, PASS); }       \
-    } else {                                                                                   \
-      if (any_pml) { if (nt_on) XF_BY(KER, uint16_t, true, true, PASS); else XF_BY(KER, uint16_t, true, false, PASS); } \
-      else { if (nt_on) XF_BY(KER, uint16_t, false, true, PASS); else XF_BY(KER, uint16_t, false, false, PASS); }       \
-    }                                                                                          \
+    if (id_bytes == 1) XF_BYLV(KER, uint8_t, PASS); else XF_BYLV(KER, uint16_t, PASS);         \
   } while (0)
   if (ev_on) XF_HIP(hipEventRecord(ev[0], stream));
   XF_BYALL(k_yee_h3_batch, pass_h);
@@ -476,6 +487,8 @@ int Batch::launch_step(int t, bool use_tdev, bool ev_on, hipEvent_t* ev) {
   if (ev_on) XF_HIP(hipEventRecord(ev[4], stream));
   if (use_tdev) hipLaunchKernelGGL(k_next_step, dim3(1), dim3(1), 0, stream, t_dev);
 #undef XF_BYALL
+#undef XF_BYLV
+#undef XF_BYNT
 #undef XF_BY
   XF_HIP(hipGetLastError());
   return XF_OK;
@@ -628,20 +641,25 @@ int Batch::autotune(int trial_steps, xf_tuning* best) {
   xf_tuning win{};
   win.step_ms = 1e30;
   int status = XF_OK;
-  // the z-face split only where some grid can take it
-  bool can_split = false;
+  // the z-face split and tail only where some grid can take them
+  bool can_split = false, can_tail = false;
   for (auto& h : inst) {
-    const int keep_split = h->eng.pml_split;
+    const int keep_split = h->eng.pml_split, keep_tail = h->eng.pml_ztail;
     h->eng.pml_split = 1;
+    h->eng.pml_ztail = 0;
     can_split = can_split || h->eng.split_z();
+    h->eng.pml_ztail = 1;
+    can_tail = can_tail || h->eng.ztail();
     h->eng.pml_split = keep_split;
+    h->eng.pml_ztail = keep_tail;
   }
   for (int ty : {1, 2, 4}) {
     for (int planes : {2, 4, 8, 16}) {
       for (int nt_on : {1, 0}) {
-        for (int cand = 0; cand < 16; ++cand) {
-          const int half = cand & 1, split = (cand >> 1) & 1, k32 = (cand >> 2) & 1, xcd = cand >> 3;
-          if (split && !can_split) continue;
+        for (int cand = 0; cand < 32; ++cand) {
+          // split: 0 z faces in the march, 1 split launch, 2 tile tail
+          const int half = cand & 1, split = (cand >> 1) & 3, k32 = (cand >> 3) & 1, xcd = cand >> 4;
+          if (split == 3 || (split == 1 && !can_split) || (split == 2 && !can_tail)) continue;
           if (xcd && half) continue;
           tune = xf_tuning{ty, planes, 0, nt_on, xcd, split, k32, 0.0};
           status = configure();
@@ -698,20 +716,25 @@ int Engine::autotune(int trial_steps, xf_tuning* best) {
   int status = XF_OK;
   const std::vector<int> tys = dim == 3 ? std::vector<int>{1, 2, 4} : std::vector<int>{1};
   const std::vector<int> pls = dim == 3 ? std::vector<int>{2, 4, 8, 16} : std::vector<int>{1};
-  // the z-face split only where it changes the launches
-  const int keep_split = pml_split;
+  // the z-face split and tail only where they change the launches
+  const int keep_split = pml_split, keep_tail = pml_ztail;
   pml_split = 1;
+  pml_ztail = 0;
   const bool can_split = split_z();
+  pml_ztail = 1;
+  const bool can_tail = ztail();
   pml_split = keep_split;
+  pml_ztail = keep_tail;
   for (int ty : tys) {
     for (int planes : pls) {
       for (int nt_on : {1, 0}) {
-        for (int cand = 0; cand < 4; ++cand) {
-          const int split = cand & 1, k32 = dim == 3 ? cand >> 1 : 0;
-          if ((split && !can_split) || (dim != 3 && cand >> 1)) continue;
+        for (int cand = 0; cand < 6; ++cand) {
+          // split: 0 z faces in the march, 1 split launch, 2 tile tail
+          const int split = cand % 3, k32 = dim == 3 ? cand / 3 : 0;
+          if ((split == 1 && !can_split) || (split == 2 && !can_tail) || (dim != 3 && cand >= 3)) continue;
           const int chunks = static_cast<int>((nx + 1 + planes - 1) / planes);
-          set_tiling(ty, chunks, XF_TILE_SET | (nt_on ? XF_TILE_NT : 0) | (split ? XF_TILE_PML_SPLIT : 0) |
-                                     (k32 ? XF_TILE_K32 : 0));
+          set_tiling(ty, chunks, XF_TILE_SET | (nt_on ? XF_TILE_NT : 0) | (split == 1 ? XF_TILE_PML_SPLIT : 0) |
+                                     (split == 2 ? XF_TILE_PML_ZTAIL : 0) | (k32 ? XF_TILE_K32 : 0));
           status = run(1);
           if (status == XF_OK) status = run(trial_steps);
           if (status != XF_OK) break;
@@ -735,8 +758,8 @@ int Engine::autotune(int trial_steps, xf_tuning* best) {
     return status;
   }
   set_tiling(win.tile_y, static_cast<int>((nx + 1 + win.planes - 1) / win.planes),
-             XF_TILE_SET | (win.nontemporal ? XF_TILE_NT : 0) | (win.pml_split ? XF_TILE_PML_SPLIT : 0) |
-                 (win.k32 ? XF_TILE_K32 : 0));
+             XF_TILE_SET | (win.nontemporal ? XF_TILE_NT : 0) | (win.pml_split == 1 ? XF_TILE_PML_SPLIT : 0) |
+                 (win.pml_split == 2 ? XF_TILE_PML_ZTAIL : 0) | (win.k32 ? XF_TILE_K32 : 0));
   if (best) *best = win;
   return XF_OK;
 }
@@ -811,9 +834,9 @@ int xf_batch_set_tuning(xf_batch* b, const xf_tuning* t) {
     const xf_tuning keep = b->b.tune;
     b->b.tune = *t;
     b->b.tune.step_ms = 0;
-    if (t->tile_y < 1 || t->tile_y > 4 || t->planes < 1 || t->blocks < 0) {
+    if (t->tile_y < 1 || t->tile_y > 4 || t->planes < 1 || t->blocks < 0 || t->pml_split < 0 || t->pml_split > 2) {
       b->b.tune = keep;
-      return xf::fail(XF_ERR_INVALID, "bad tuning (tile_y 1..4, planes >= 1, blocks >= 0)");
+      return xf::fail(XF_ERR_INVALID, "bad tuning (tile_y 1..4, planes >= 1, blocks >= 0, pml_split 0..2)");
     }
     if (!b->b.uploaded) return XF_OK;
     const int st = b->b.configure();

@@ -237,6 +237,7 @@ struct Engine {
   int xcd{0};
   int nt_streams{1};
   int pml_split{0};  // XF_TILE_PML_SPLIT requested (see split_z())
+  int pml_ztail{0};  // XF_TILE_PML_ZTAIL requested (see ztail())
   int klanes{64};    // k-lanes of a tile row: 64, or 32 (XF_TILE_K32)
   // XF_TILE_PML_STREAMS: per pass a tile list, CPML-free tiles first, then
   // the tiles that meet a face the Yee kernel carries; the CPML part runs on
@@ -368,6 +369,7 @@ struct Engine {
   int launch_movies(int t, const int* tp, hipStream_t st);
   // z-face CPML corrections in a launch of their own after each Yee pass
   bool split_z() const;
+  bool ztail() const;
   // device index of history entry i (xf_learned_boundary layout); y faces
   // are stored [b][a] on the device (LbFace::tr)
   int64_t lb_dev_hist_index(int64_t i) const;
@@ -1671,7 +1673,7 @@ int Engine::upload() {
 }
 
 // Tiling of the 3-D marching kernels.  chunks_x = 0 gives short marches
-// (4 planes FP64, 8 FP32/BF16): measured at 536^3 on MI355X, many short
+// (3 or 4 planes FP64, 8 FP32/BF16): measured at 536^3 on MI355X, many short
 // marches beat long ones (a long march leaves a partly empty tail round of
 // blocks and keeps far-apart planes open at once), while the carried plane
 // still saves a neighbour re-read every step of the march.
@@ -1686,23 +1688,28 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
     xcd = (flags & XF_TILE_XCD_REMAP) ? 1 : 0;
     nt_streams = (flags & XF_TILE_NT) ? 1 : 0;
     pml_split = (flags & XF_TILE_PML_SPLIT) ? 1 : 0;
+    pml_ztail = (flags & XF_TILE_PML_ZTAIL) ? 1 : 0;
     klanes = (flags & XF_TILE_K32) ? 32 : 64;
     pml_streams = (flags & XF_TILE_PML_STREAMS) ? 1 : 0;
   } else {
     // defaults: non-temporal streams; z-face CPML corrections split off
-    // (measured at 536^3: FP32 +3 %, FP64 even; 64 small grids: +13 %)
+    // (measured at 536^3: FP32 +3 %, FP64 even; 64 small grids: +13 %),
+    // except FP64, where 32-lane tiles that apply them at the end of each
+    // 3-plane march measured fastest at 536^3 (32.5 against 31.8 Gcells/s on
+    // one box; FP32 keeps the split, 56.7 against 53.8)
     xcd = 0;
     nt_streams = 1;
     pml_split = 1;
+    pml_ztail = esize == 8 ? 1 : 0;
     pml_streams = 0;
-    klanes = 64;
+    klanes = esize == 8 ? 32 : 64;
   }
   if (dim == 3) {
     nkb = static_cast<int>((nz + 1 + klanes - 1) / klanes);
     njb = static_cast<int>((ny + 1 + rows() - 1) / rows());
-    // default march: 4 planes for 8-byte fields, 8 for narrower ones (the
-    // measured optimum on MI355X at 536^3)
-    const int64_t planes = esize == 8 ? 4 : 8;
+    // default march: 4 planes for 8-byte fields (3 with the z faces at the
+    // tile end), 8 for narrower ones (the measured optima on MI355X at 536^3)
+    const int64_t planes = esize == 8 ? (ztail() ? 3 : 4) : 8;
     int64_t nch = chunks_x;
     if (nch <= 0) nch = std::max<int64_t>(1, (nx + 1 + planes - 1) / planes);
     nch = std::min<int64_t>(nch, nx + 1);
@@ -1720,7 +1727,7 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
                 " etab=" + std::to_string(ntab_e) +
                 " htab=" + std::to_string(ntab_h) + " chunk=" + std::to_string(chunk) +
                 " nchunk=" + std::to_string(nch_eff) + " tile_y=" + std::to_string(tile_y) +
-                " xcd=" + std::to_string(xcd) + " nt=" + std::to_string(nt_streams) + " pml_split=" + std::to_string(split_z() ? 1 : 0) +
+                " xcd=" + std::to_string(xcd) + " nt=" + std::to_string(nt_streams) + " pml_split=" + std::to_string(split_z() ? 1 : 0) + " pml_ztail=" + std::to_string(ztail() ? 1 : 0) +
                 " klanes=" + std::to_string(klanes) + " pml_streams=" + std::to_string(pml_streams) +
                 " pml_faces=" + std::to_string(pml.size()) +
                 " ade_cells=" + std::to_string(ade_cells.size());
@@ -1999,7 +2006,15 @@ YeePass<S> Engine::yee_pass(bool is_e) const {
   D.njb = njb;
   D.chunk = chunk;
   D.nch = nch_eff;
-  fill_pml_pass<C>(*this, D.pml, is_e, split_z() ? 1 : 0);
+  const bool tail = ztail();
+  fill_pml_pass<C>(*this, D.pml, is_e, (split_z() || tail) ? 1 : 0);
+  D.zt = PmlZTail<C>{};
+  if (tail) {
+    PmlPass<C> Z{};
+    fill_pml_pass<C>(*this, Z, is_e, 2);
+    D.zt.n = Z.n;
+    for (int q = 0; q < Z.n; ++q) D.zt.f[q] = Z.f[q];
+  }
   return D;
 }
 
@@ -2025,8 +2040,22 @@ int64_t Engine::lb_dev_hist_index(int64_t i) const {
 // corrected ranges do not overlap (each node then gets its z corrections
 // from one thread) and fewer than 2^31 nodes per face box.  BF16 storage
 // keeps them fused: the split would round the intermediate field.
+// The tail applies to 3-D FP64/FP32 grids in table mode with one or two z
+// faces in each pass.  BF16 storage keeps the z faces in the march: read back from storage,
+// the intermediate field would be rounded once more.
+bool Engine::ztail() const {
+  if (!pml_ztail || dense || dim != 3 || (plan.precision != XF_FP64 && plan.precision != XF_FP32)) return false;
+  PmlPass<double> P{};
+  for (int pass = 0; pass < 2; ++pass) {
+    fill_pml_pass<double>(*this, P, pass == 1, 2);
+    if (P.n == 0 || P.n > 2) return false;
+  }
+  return true;
+}
+
 bool Engine::split_z() const {
-  if (!pml_split || dim != 3 || (plan.precision != XF_FP64 && plan.precision != XF_FP32)) return false;
+  if (!pml_split || ztail() || dim != 3 || (plan.precision != XF_FP64 && plan.precision != XF_FP32))
+    return false;
   PmlPass<double> P{};
   for (int pass = 0; pass < 2; ++pass) {
     fill_pml_pass<double>(*this, P, pass == 1, 2);
@@ -2131,7 +2160,7 @@ int Engine::launch_h(int t, const int* tp) {
   const C* tab = static_cast<const C*>(htab);
   if (dim == 3) {
     const YeePass<S> D = yee_pass<S>(false);
-    const bool has_pml = D.pml.n > 0;
+    const bool has_pml = D.pml.n > 0 || D.zt.n > 0;
     dim3 block(klanes, rows());
     dim3 grid((unsigned)(D.nkb * D.njb * D.nch));
 #define XF_LH3(I, L, PM)                                                                     \
@@ -2164,7 +2193,8 @@ int Engine::launch_h(int t, const int* tp) {
   } while (0)
 #define XF_LHL2(I, L)                                                                                  \
   do {                                                                                                 \
-    if (n_pml[0] > 0) XF_LHL(I, L, true, stream2, n_pml[0], li + n_int[0]);                            \
+    if (n_pml[0] > 0 && D.zt.n > 0) XF_LHL(I, L, 2, stream2, n_pml[0], li + n_int[0]);                 \
+    else if (n_pml[0] > 0) XF_LHL(I, L, 1, stream2, n_pml[0], li + n_int[0]);                          \
     if (n_int[0] > 0) XF_LHL(I, L, false, stream, n_int[0], li);                                       \
   } while (0)
       if (id_bytes == 1) {
@@ -2176,12 +2206,20 @@ int Engine::launch_h(int t, const int* tp) {
 #undef XF_LHL
       XF_HIP(hipEventRecord(ev_join, stream2));
       XF_HIP(hipStreamWaitEvent(stream, ev_join, 0));
-    } else if (id_bytes == 1) {
-      if (lds) { if (has_pml) XF_LH3(uint8_t, true, true); else XF_LH3(uint8_t, true, false); }
-      else { if (has_pml) XF_LH3(uint8_t, false, true); else XF_LH3(uint8_t, false, false); }
     } else {
-      if (lds) { if (has_pml) XF_LH3(uint16_t, true, true); else XF_LH3(uint16_t, true, false); }
-      else { if (has_pml) XF_LH3(uint16_t, false, true); else XF_LH3(uint16_t, false, false); }
+      // CPML level: 0 none, 1 faces in the march, 2 and z faces at the tile end
+#define XF_LH3P(I, L)                       \
+  do {                                      \
+    if (!has_pml) XF_LH3(I, L, 0);          \
+    else if (D.zt.n > 0) XF_LH3(I, L, 2);   \
+    else XF_LH3(I, L, 1);                   \
+  } while (0)
+      if (id_bytes == 1) {
+        if (lds) XF_LH3P(uint8_t, true); else XF_LH3P(uint8_t, false);
+      } else {
+        if (lds) XF_LH3P(uint16_t, true); else XF_LH3P(uint16_t, false);
+      }
+#undef XF_LH3P
     }
 #undef XF_LH3
 #undef XF_LH3D
@@ -2222,7 +2260,7 @@ int Engine::launch_e(int t, const int* tp) {
   const C* tab = static_cast<const C*>(etab);
   if (dim == 3) {
     const YeePass<S> D = yee_pass<S>(true);
-    const bool has_pml = D.pml.n > 0;
+    const bool has_pml = D.pml.n > 0 || D.zt.n > 0;
     dim3 block(klanes, rows());
     dim3 grid((unsigned)(D.nkb * D.njb * D.nch));
 #define XF_LE3(I, L, PM, AD)                                                                     \
@@ -2257,7 +2295,8 @@ int Engine::launch_e(int t, const int* tp) {
   } while (0)
 #define XF_LEL2(I, L, AD)                                                                                 \
   do {                                                                                                    \
-    if (n_pml[1] > 0) XF_LEL(I, L, true, AD, stream2, n_pml[1], li + n_int[1]);                           \
+    if (n_pml[1] > 0 && D.zt.n > 0) XF_LEL(I, L, 2, AD, stream2, n_pml[1], li + n_int[1]);                \
+    else if (n_pml[1] > 0) XF_LEL(I, L, 1, AD, stream2, n_pml[1], li + n_int[1]);                         \
     if (n_int[1] > 0) XF_LEL(I, L, false, AD, stream, n_int[1], li);                                      \
   } while (0)
 #define XF_LEL3(I, L) do { if (has_ade) XF_LEL2(I, L, true); else XF_LEL2(I, L, false); } while (0)
@@ -2271,12 +2310,19 @@ int Engine::launch_e(int t, const int* tp) {
 #undef XF_LEL
       XF_HIP(hipEventRecord(ev_join, stream2));
       XF_HIP(hipStreamWaitEvent(stream, ev_join, 0));
-    } else if (id_bytes == 1) {
-      if (lds) { if (has_pml) XF_LE3A(uint8_t, true, true); else XF_LE3A(uint8_t, true, false); }
-      else { if (has_pml) XF_LE3A(uint8_t, false, true); else XF_LE3A(uint8_t, false, false); }
     } else {
-      if (lds) { if (has_pml) XF_LE3A(uint16_t, true, true); else XF_LE3A(uint16_t, true, false); }
-      else { if (has_pml) XF_LE3A(uint16_t, false, true); else XF_LE3A(uint16_t, false, false); }
+#define XF_LE3P(I, L)                        \
+  do {                                       \
+    if (!has_pml) XF_LE3A(I, L, 0);          \
+    else if (D.zt.n > 0) XF_LE3A(I, L, 2);   \
+    else XF_LE3A(I, L, 1);                   \
+  } while (0)
+      if (id_bytes == 1) {
+        if (lds) XF_LE3P(uint8_t, true); else XF_LE3P(uint8_t, false);
+      } else {
+        if (lds) XF_LE3P(uint16_t, true); else XF_LE3P(uint16_t, false);
+      }
+#undef XF_LE3P
     }
 #undef XF_LE3A
 #undef XF_LE3

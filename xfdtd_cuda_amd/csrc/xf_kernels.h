@@ -70,10 +70,10 @@ template <>
 struct Store<__hip_bfloat16> {
   using C = float;
   static __device__ __forceinline__ C ld(const __hip_bfloat16* p, int64_t i, C inv) {
-    return __bfloat162float(__ushort_as_bfloat16(gp(reinterpret_cast<const unsigned short*>(p))[i])) * inv;
+    return (float)gp(reinterpret_cast<const __bf16*>(p))[i] * inv;
   }
   static __device__ __forceinline__ void st(__hip_bfloat16* p, int64_t i, C v, C scale) {
-    gp(reinterpret_cast<unsigned short*>(p))[i] = __bfloat16_as_ushort(__float2bfloat16(v * scale));
+    p[i] = __float2bfloat16(v * scale);  // generic store: a global-space one costs a register
   }
   // the streaming slots use plain accesses: non-temporal 16-bit loads and
   // stores measured slower at 536^3 (59.9 against 61.9 Gcells/s)
@@ -89,10 +89,10 @@ template <>
 struct Store<__half> {
   using C = float;
   static __device__ __forceinline__ C ld(const __half* p, int64_t i, C inv) {
-    return __half2float(__ushort_as_half(gp(reinterpret_cast<const unsigned short*>(p))[i])) * inv;
+    return (float)gp(reinterpret_cast<const _Float16*>(p))[i] * inv;
   }
   static __device__ __forceinline__ void st(__half* p, int64_t i, C v, C scale) {
-    gp(reinterpret_cast<unsigned short*>(p))[i] = __half_as_ushort(__float2half(v * scale));
+    p[i] = __float2half(v * scale);  // generic store: a global-space one costs a register
   }
   // plain accesses in the streaming slots, as for bf16 (non-temporal 16-bit
   // accesses measured slower: 57.9 against 59.9 Gcells/s at 536^3)
@@ -264,6 +264,14 @@ struct PmlPass {
   PmlFace<C> f[kMaxPml];
 };
 
+// The z faces of a pass applied at the end of each Yee tile (at most two,
+// in registration order), or n = 0.
+template <typename C>
+struct PmlZTail {
+  int n;
+  PmlFace<C> f[2];
+};
+
 // The z faces of a pass as node boxes (k fastest) for k_pml_z: face q
 // covers i in [i0, i0 + ni), j in [j0, j0 + nj), k in [k0, k0 + nk), its
 // nodes numbered from start[q]; start[n] is the total.
@@ -300,6 +308,7 @@ struct YeePass {
   Scales<C> sc;
   int nkb, njb, chunk, nch;     // tiling of this grid
   PmlPass<C> pml;
+  PmlZTail<C> zt;               // z faces applied after each tile's march
 };
 
 // A TFSF face correction of one component (one reference launch of

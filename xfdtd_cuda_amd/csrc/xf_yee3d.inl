@@ -37,19 +37,8 @@ __device__ __forceinline__ C tab_at(const C* tab, int off) {
 
 template <int TAB, typename C>
 __device__ __forceinline__ void tab_row(const C* tab, int id, C (&t)[9]) {
-  if constexpr (TAB == TAB_LDS) {
-    const __attribute__((address_space(3))) C* q = (const __attribute__((address_space(3))) C*)tab + 9 * id;
 #pragma unroll
-    for (int n = 0; n < 9; ++n) t[n] = q[n];
-  } else if constexpr (TAB == TAB_GLOBAL) {
-    const __attribute__((address_space(1))) C* q = gp(tab) + 9 * id;
-#pragma unroll
-    for (int n = 0; n < 9; ++n) t[n] = q[n];
-  } else {
-    const C* q = tab + 9 * id;
-#pragma unroll
-    for (int n = 0; n < 9; ++n) t[n] = q[n];
-  }
+  for (int n = 0; n < 9; ++n) t[n] = tab_at<TAB>(tab, 9 * id + n);
 }
 
 template <typename S>
@@ -133,10 +122,44 @@ __device__ __forceinline__ unsigned pml_mask(const PmlPass<C>& P, int i0, int i1
   return bits;
 }
 
+// The z-face CPML corrections of the tile's column (j, k) over [i0, i1),
+// applied after its march (XF_TILE_PML_ZTAIL): the lanes whose column lies
+// in a z slab revisit their nodes, which this lane stored itself, so the
+// lines are still in cache, instead of a separate launch fetching them
+// again.  Per node the work and its order are k_pml_z's (pml_z_node): the
+// x and y corrections came in the march, the z faces follow in
+// registration order; the dual field is not written in this pass.
+template <bool ISE, typename S>
+__device__ __forceinline__ void pml_z_tail(const PmlZTail<typename Store<S>::C>& Z, int i0, int i1, int j,
+                                           int k, int64_t idx0, const Fields<S>& f, const Dims& d,
+                                           const Scales<typename Store<S>::C>& sc) {
+  using C = typename Store<S>::C;
+  auto meets = [&](const PmlFace<C>& F) {
+    return k >= F.c0 && k < F.c1 && F.a0 < i1 && F.a1 >= i0 && j >= F.b0 && j <= F.b1;
+  };
+  const bool m0 = Z.n > 0 && meets(Z.f[0]);
+  const bool m1 = Z.n > 1 && meets(Z.f[1]);
+  if (!m0 && !m1) return;
+  S* const* self = ISE ? f.e : f.h;
+  const C s = ISE ? sc.s_e : sc.s_h;
+  int64_t idx = idx0;
+  for (int i = i0; i < i1; ++i, idx += d.sx) {
+    C v[3] = {0, 0, 0};
+    bool have[3] = {false, false, false};
+    bool dirty[3] = {false, false, false};
+    // static face indices keep the descriptor out of scratch
+    if (m0) pml_face<2, ISE, false>(Z.f[0], i, j, k, idx, f, d, sc, v, have, dirty);
+    if (m1) pml_face<2, ISE, false>(Z.f[1], i, j, k, idx, f, d, sc, v, have, dirty);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (dirty[c]) Store<S>::st(self[c], idx, v[c], s);
+  }
+}
+
 // One tile of the H pass.  Lanes beyond the grid return at once; the tile
 // bodies hold no barrier, so the batched kernels may loop over tiles with
 // some lanes idle.
-template <typename S, typename I, bool PML, bool NT, bool DN = false, int TAB = TAB_ANY>
+template <typename S, typename I, int PML, bool NT, bool DN = false, int TAB = TAB_ANY>
 __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename Store<S>::C* tab,
                                             int kb, int jb, int ch) {
   using C = typename Store<S>::C;
@@ -153,6 +176,7 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
   const unsigned pmask = PML ? pml_mask(D.pml, i0, i1, j, k) : 0u;
   const bool any_pml = pmask != 0u;
   int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
+  const int64_t idx0 = idx;
   const C ie = sc.inv_e, ih = sc.inv_h;
   C ey_i = 0, ez_i = 0;
   HLoads<S> cur{}, nxt{};
@@ -175,6 +199,7 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
         v[1] = t[3] * cur.hy + t[4] * (cur.ez_n - ez_i) + t[5] * (cur.ex_k - cur.ex_i);
         v[2] = t[6] * cur.hz + t[7] * (cur.ex_j - cur.ex_i) + t[8] * (cur.ey_n - ey_i);
       } else {
+        // the whole row at once (the three components all update here)
         C t[9];
         tab_row<TAB>(tab, cur.id, t);
         v[0] = t[0] * cur.hx + t[1] * (cur.ey_k - ey_i) + t[2] * (cur.ez_j - ez_i);
@@ -190,11 +215,14 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
     st3<S, NT>(f.h, idx, v, dirty, sc.s_h);
     cur = nxt;
   }
+  if constexpr (PML == 2) {
+    if (D.zt.n > 0) pml_z_tail<false, S>(D.zt, i0, i1, j, k, idx0, f, d, sc);
+  }
 }
 
 // One tile of the E pass; table rows flagged dispersive are left to the
 // ADE kernel.
-template <typename S, typename I, bool PML, bool ADE, bool NT, bool DN = false, int TAB = TAB_ANY>
+template <typename S, typename I, int PML, bool ADE, bool NT, bool DN = false, int TAB = TAB_ANY>
 __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename Store<S>::C* tab,
                                             int kb, int jb, int ch) {
   using C = typename Store<S>::C;
@@ -212,6 +240,7 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
   const unsigned pmask = PML ? pml_mask(D.pml, i0, i1, j, k) : 0u;
   const bool any_pml = pmask != 0u;
   int64_t idx = (int64_t)i0 * d.sx + (int64_t)j * d.sy + k;
+  const int64_t idx0 = idx;
   const C ie = sc.inv_e, ih = sc.inv_h;
   C hy_m = 0, hz_m = 0;  // H at i-1
   ELoads<S> cur{}, nxt{};
@@ -232,15 +261,10 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
       if (!(ADE && D.eflag[cur.id])) {
         // coefficient n of the cell, read where its component is updated
         // (the E components have different boundary conditions)
-        C td[9];
-        if constexpr (DN) {
-#pragma unroll
-          for (int n = 0; n < 9; ++n) td[n] = gp(D.dense[n])[idx];
-        }
         const int row = 9 * cur.id;
         auto t = [&](int n) -> C {
           if constexpr (DN)
-            return td[n];
+            return gp(D.dense[n])[idx];
           else
             return tab_at<TAB>(tab, row + n);
         };
@@ -264,13 +288,16 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
     st3<S, NT>(f.e, idx, v, dirty, sc.s_e);
     cur = nxt;
   }
+  if constexpr (PML == 2) {
+    if (D.zt.n > 0) pml_z_tail<true, S>(D.zt, i0, i1, j, k, idx0, f, d, sc);
+  }
 }
 
 // One grid: the descriptor is a kernel argument, block -> tile by
 // tile_coords (optionally XCD-aware).
 // DN: coefficients from the dense per-cell arrays (the ids then only carry
 // the E pass's dispersive flag) instead of the id table.
-template <typename S, typename I, bool LDS, bool PML, bool NT, bool DN = false>
+template <typename S, typename I, bool LDS, int PML, bool NT, bool DN = false>
 __global__ void __launch_bounds__(256) k_yee_h3(YeePass<S> D, int xcd) {
   using C = typename Store<S>::C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -281,7 +308,7 @@ __global__ void __launch_bounds__(256) k_yee_h3(YeePass<S> D, int xcd) {
   yee_h3_tile<S, I, PML, NT, DN>(D, tab, kb, jb, ch);
 }
 
-template <typename S, typename I, bool LDS, bool PML, bool ADE, bool NT, bool DN = false>
+template <typename S, typename I, bool LDS, int PML, bool ADE, bool NT, bool DN = false>
 __global__ void __launch_bounds__(256) k_yee_e3(YeePass<S> D, int xcd) {
   using C = typename Store<S>::C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -295,7 +322,7 @@ __global__ void __launch_bounds__(256) k_yee_e3(YeePass<S> D, int xcd) {
 // Tiles taken from a list (tile id kb + nkb (jb + njb ch)): the engine
 // launches a pass's CPML-free tiles and its CPML tiles apart, on two streams,
 // so the CPML-free ones run in the leaner PML = false instantiation.
-template <typename S, typename I, bool LDS, bool PML, bool NT>
+template <typename S, typename I, bool LDS, int PML, bool NT>
 __global__ void __launch_bounds__(256) k_yee_h3_list(YeePass<S> D, const int* __restrict__ list) {
   using C = typename Store<S>::C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -305,7 +332,7 @@ __global__ void __launch_bounds__(256) k_yee_h3_list(YeePass<S> D, const int* __
   yee_h3_tile<S, I, PML, NT>(D, tab, kb, rest % D.njb, rest / D.njb);
 }
 
-template <typename S, typename I, bool LDS, bool PML, bool ADE, bool NT>
+template <typename S, typename I, bool LDS, int PML, bool ADE, bool NT>
 __global__ void __launch_bounds__(256) k_yee_e3_list(YeePass<S> D, const int* __restrict__ list) {
   using C = typename Store<S>::C;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -374,7 +401,7 @@ __device__ __forceinline__ const C* batch_table(const C* __restrict__ g_tab, int
   return mine;
 }
 
-template <typename S, typename I, bool PML, bool NT>
+template <typename S, typename I, int PML, bool NT>
 __global__ void __launch_bounds__(256)
     k_yee_h3_batch(const YeePass<S>* __restrict__ passes, const int* __restrict__ tile_start, int n,
                      int xcd) {
@@ -397,7 +424,7 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-template <typename S, typename I, bool PML, bool NT>
+template <typename S, typename I, int PML, bool NT>
 __global__ void __launch_bounds__(256)
     k_yee_e3_batch(const YeePass<S>* __restrict__ passes, const int* __restrict__ tile_start, int n,
                      int xcd) {

@@ -509,20 +509,23 @@ class Engine:
 
     def set_tiling(self, tile_y: int = 0, chunks_x: int = 0, xcd_remap: bool | None = None,
                    nontemporal: bool | None = None, pml_split: bool | None = None,
-                   k32: bool | None = None, pml_streams: bool | None = None):
+                   k32: bool | None = None, pml_streams: bool | None = None,
+                   pml_ztail: bool | None = None):
         """Tiling of the 3-D Yee kernels; None leaves a flag to the engine's
         default (no XCD remap, non-temporal streams on, z-face CPML split
         off).  pml_split moves the z-face CPML corrections into a launch of
-        their own (FP64/FP32; the results are the same); k32 tiles k by 32
-        lanes with two j-rows per wave."""
+        their own, pml_ztail into the end of each Yee tile (FP64/FP32; the
+        results are the same either way; the tail takes precedence); k32
+        tiles k by 32 lanes with two j-rows per wave."""
         flags = 0
         if xcd_remap is not None or nontemporal is not None or pml_split is not None or \
-                k32 is not None or pml_streams is not None:
+                k32 is not None or pml_streams is not None or pml_ztail is not None:
             flags = _native.TILE_SET
             flags |= _native.TILE_XCD_REMAP if xcd_remap else 0
             flags |= _native.TILE_PML_STREAMS if pml_streams else 0
             flags |= _native.TILE_NT if (nontemporal is None or nontemporal) else 0
             flags |= _native.TILE_PML_SPLIT if (pml_split is None or pml_split) else 0
+            flags |= _native.TILE_PML_ZTAIL if pml_ztail else 0
             flags |= _native.TILE_K32 if k32 else 0
         _native.check_hip(self._lib.xf_set_tiling(self._h, tile_y, chunks_x, flags),
                           "xf_set_tiling")
@@ -588,8 +591,11 @@ class Batch:
 
     def set_tuning(self, tile_y: int = 1, planes: int = 4, blocks: int = 0,
                    nontemporal: bool = True, pml_split: bool = True, k32: bool = False,
-                   xcd_remap: bool = False):
-        t = XfTuning(tile_y, planes, blocks, int(nontemporal), int(xcd_remap), int(pml_split), int(k32), 0.0)
+                   xcd_remap: bool = False, pml_ztail: bool = False):
+        """pml_split: z-face CPML in a launch of its own; pml_ztail (takes
+        precedence): at the end of each Yee tile."""
+        split = 2 if pml_ztail else int(bool(pml_split))
+        t = XfTuning(tile_y, planes, blocks, int(nontemporal), int(xcd_remap), split, int(k32), 0.0)
         _native.check_hip(self._lib.xf_batch_set_tuning(self._h, ctypes.byref(t)),
                           "xf_batch_set_tuning")
         return self
