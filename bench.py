@@ -678,7 +678,8 @@ def bench_fp32_vs_fp64(args) -> int:
         e_bytes = ncell * ((4 if two_d else 9) * sb + 1) + pml_bytes(sim, "e", cb)
         e_ms = tm["yee_e_ms"] / tm["yee_e_launches"]
         out["roofline"] = {"bound": "hbm", "kernel": ("k_yee_e2" if two_d else
-                                                      "k_yee_e3+k_pml_z+k_ade_e3"),
+                                                      ("k_yee_e3+k_pml_z+k_ade_e3" if "pml_split=1" in desc
+                                                       else "k_yee_e3+k_ade_e3")),
                            "achieved": round(e_bytes / (e_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS,
                            "unit": "GB/s",
                            "frac": round(e_bytes / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
