@@ -9,6 +9,8 @@ rel-L2 <= 1e-12 (the engine and oracle share evaluation order and neither
 contracts to FMA, so in practice the arrays are bit-identical; the test also
 reports max |diff|).
 """
+import re
+
 import numpy as np
 import pytest
 
@@ -284,6 +286,28 @@ def test_ade(gpu_available, kind):
         # (multiplied by zero coefficients); J stays exactly zero there.
         assert rel_l2(g.array(n), c.array(n)) <= TOL, n
     assert np.abs(c.array("ade.jx")).max() > 0
+
+
+def test_ade_coefficient_table(gpu_available):
+    """The ADE kernel reads each dispersive cell's coefficients from a table
+    of the distinct rows (a row id per cell); the dense coefficient mode
+    keeps the per-cell arrays.  Both equal the oracle."""
+    build = _dispersive("mix")
+    steps = 120
+    c = build()
+    oracle.run(c.scene_ptr(), steps)
+    for mode in ("auto", "dense"):
+        print(f"case: ADE coefficients, coeff={mode}", flush=True)
+        g = build()
+        e = xf.Engine(g, coeff=mode).upload()
+        desc = e.describe()
+        e.run(steps).download()
+        e.close()
+        rows = int(re.search(r"ade_rows=(-?\d+)", desc).group(1))
+        assert (rows > 0) if mode == "auto" else (rows == -1), desc
+        _compare(g, c, label=f"ade {mode}")
+        for n in ("ade.jx", "ade.jy", "ade.jz", "ade.jx_prev"):
+            assert rel_l2(g.array(n), c.array(n)) <= TOL, (mode, n)
 
 
 def test_pml_split_uneven_faces_fp32_and_bf16(gpu_available):

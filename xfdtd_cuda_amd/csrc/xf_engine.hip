@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <limits>
 #include <memory>
 #include <mutex>
@@ -203,6 +204,7 @@ struct Engine {
   std::vector<NtPass> nftd;
   // ADE
   AdeDev ade{};
+  int64_t ade_rows{-1};  // distinct ADE coefficient rows in the table, -1 per-cell arrays
   // point sources and the learned layer (additions)
   PointDev pts{};
   std::vector<MovieDev> movies;
@@ -1327,7 +1329,6 @@ int Engine::upload_ade() {
   ade.np = static_cast<int>(np);
   ade.n = n;
   std::vector<int64_t> idx(n);
-  std::vector<int> ijk(3 * n);
   std::vector<double> cejs(n), ceep(n), cjj(np * n), cjjp(np * n), cjen(np * n), cje(np * n),
       cjep(np * n), cjsj(np * n), cjsjp(np * n), eprev(3 * n), jc(3 * np * n), jp(3 * np * n);
   const xf_array* J[3] = {&A.jx, &A.jy, &A.jz};
@@ -1338,9 +1339,6 @@ int Engine::upload_ade() {
     const int64_t l3 = ade_cells[c];
     const int64_t k = l3 % (nz + 1), j = (l3 / (nz + 1)) % (ny + 1), i = l3 / ((nz + 1) * (ny + 1));
     idx[c] = i * sx + j * sy + k;
-    ijk[3 * c] = static_cast<int>(i);
-    ijk[3 * c + 1] = static_cast<int>(j);
-    ijk[3 * c + 2] = static_cast<int>(k);
     cejs[c] = A.coeff_e_j_sum.data[l3];
     ceep[c] = ml ? A.coeff_e_e_p.data[l3] : 0.0;
     for (int64_t p = 0; p < np; ++p) {
@@ -1365,26 +1363,71 @@ int Engine::upload_ade() {
   void* p;
   XF_TRY(upload_vec(&p, idx));
   ade.idx = static_cast<const int64_t*>(p);
-  XF_TRY(upload_vec(&p, ijk));
-  ade.ijk = static_cast<const int*>(p);
-  XF_TRY(upload_vec(&p, cejs));
-  ade.cejs = static_cast<const double*>(p);
-  XF_TRY(upload_vec(&p, ceep));
-  ade.ceep = static_cast<const double*>(p);
-  XF_TRY(upload_vec(&p, cjj));
-  ade.cjj = static_cast<const double*>(p);
-  XF_TRY(upload_vec(&p, cjjp));
-  ade.cjjp = static_cast<const double*>(p);
-  XF_TRY(upload_vec(&p, cjen));
-  ade.cjen = static_cast<const double*>(p);
-  XF_TRY(upload_vec(&p, cje));
-  ade.cje = static_cast<const double*>(p);
-  XF_TRY(upload_vec(&p, cjep));
-  ade.cjep = static_cast<const double*>(p);
-  XF_TRY(upload_vec(&p, cjsj));
-  ade.cjsj = static_cast<const double*>(p);
-  XF_TRY(upload_vec(&p, cjsjp));
-  ade.cjsjp = static_cast<const double*>(p);
+  // the distinct coefficient rows (compared by bit pattern, so the table
+  // gives back exactly each cell's values) and each cell's row id; the
+  // dense coefficient mode keeps the reference's per-cell arrays here too
+  const int tw = static_cast<int>(2 + 7 * np);
+  std::map<std::vector<uint64_t>, int> rows;
+  std::vector<double> tab;
+  std::vector<uint16_t> row_of(n);
+  bool use_tab = !dense;
+  std::vector<uint64_t> key(tw);
+  for (int64_t c = 0; c < n && use_tab; ++c) {
+    double r[2 + 7 * 64];
+    r[0] = cejs[c];
+    r[1] = ceep[c];
+    for (int64_t q = 0; q < np; ++q) {
+      const int64_t pc = q * n + c;
+      double* o = r + 2 + 7 * q;
+      o[0] = cjj[pc];
+      o[1] = cje[pc];
+      o[2] = cjsj[pc];
+      o[3] = ml ? cjjp[pc] : 0.0;
+      o[4] = ml ? cjen[pc] : 0.0;
+      o[5] = ml ? cjep[pc] : 0.0;
+      o[6] = ml ? cjsjp[pc] : 0.0;
+    }
+    for (int w = 0; w < tw; ++w) std::memcpy(&key[w], &r[w], sizeof(double));
+    auto it = rows.find(key);
+    if (it == rows.end()) {
+      if (rows.size() >= 65535) {
+        use_tab = false;
+        break;
+      }
+      it = rows.emplace(key, static_cast<int>(rows.size())).first;
+      tab.insert(tab.end(), r, r + tw);
+    }
+    row_of[c] = static_cast<uint16_t>(it->second);
+  }
+  ade.tw = tw;
+  if (use_tab && n > 0) {
+    XF_TRY(upload_vec(&p, row_of));
+    ade.row = static_cast<const uint16_t*>(p);
+    XF_TRY(upload_vec(&p, tab));
+    ade.tab = static_cast<const double*>(p);
+  } else {
+    ade.row = nullptr;
+    ade.tab = nullptr;
+    XF_TRY(upload_vec(&p, cejs));
+    ade.cejs = static_cast<const double*>(p);
+    XF_TRY(upload_vec(&p, ceep));
+    ade.ceep = static_cast<const double*>(p);
+    XF_TRY(upload_vec(&p, cjj));
+    ade.cjj = static_cast<const double*>(p);
+    XF_TRY(upload_vec(&p, cjjp));
+    ade.cjjp = static_cast<const double*>(p);
+    XF_TRY(upload_vec(&p, cjen));
+    ade.cjen = static_cast<const double*>(p);
+    XF_TRY(upload_vec(&p, cje));
+    ade.cje = static_cast<const double*>(p);
+    XF_TRY(upload_vec(&p, cjep));
+    ade.cjep = static_cast<const double*>(p);
+    XF_TRY(upload_vec(&p, cjsj));
+    ade.cjsj = static_cast<const double*>(p);
+    XF_TRY(upload_vec(&p, cjsjp));
+    ade.cjsjp = static_cast<const double*>(p);
+  }
+  ade_rows = use_tab ? static_cast<int64_t>(rows.size()) : -1;
   XF_TRY(upload_vec(&p, eprev));
   ade.eprev = static_cast<double*>(p);
   state.emplace_back(p, eprev.size() * sizeof(double));
@@ -1730,7 +1773,8 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
                 " xcd=" + std::to_string(xcd) + " nt=" + std::to_string(nt_streams) + " pml_split=" + std::to_string(split_z() ? 1 : 0) + " pml_ztail=" + std::to_string(ztail() ? 1 : 0) +
                 " klanes=" + std::to_string(klanes) + " pml_streams=" + std::to_string(pml_streams) +
                 " pml_faces=" + std::to_string(pml.size()) +
-                " ade_cells=" + std::to_string(ade_cells.size());
+                " ade_cells=" + std::to_string(ade_cells.size()) +
+                (has_ade ? " ade_rows=" + std::to_string(ade_rows) : std::string());
 }
 
 int Engine::snapshot() {

@@ -1170,7 +1170,14 @@ struct AdeDev {
   int np;
   int64_t n;                // dispersive cells
   const int64_t* idx;       // padded-grid index of each cell
-  const int* ijk;           // (i, j, k) of each cell
+  // Coefficients: the cells of one material share them, so by default each
+  // cell holds a row id into a table of the distinct rows (tw doubles each:
+  // cejs, ceep, then per pole cjj, cje, cjsj, cjjp, cjen, cjep, cjsjp);
+  // with more than 65535 distinct rows, or row == nullptr, the per-cell
+  // arrays below are read instead.
+  const uint16_t* row;      // [n]
+  const double* tab;        // [rows][tw]
+  int tw;
   const double* cejs;       // [n]
   const double* ceep;       // [n]
   const double* cjj;        // [np][n]
@@ -1197,7 +1204,25 @@ __global__ void __launch_bounds__(256)
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= A.n) return;
   const int64_t idx = A.idx[c];
-  const int i = A.ijk[3 * c], j = A.ijk[3 * c + 1], k = A.ijk[3 * c + 2];
+  // (i, j, k) from the padded index (32-bit division when the grid allows)
+  int i, j, k;
+  if ((uint64_t)d.sx * (uint64_t)(d.nx + 1) < (uint64_t(1) << 32)) {
+    const uint32_t u = (uint32_t)idx, sx = (uint32_t)d.sx, sy = (uint32_t)d.sy;
+    i = (int)(u / sx);
+    const uint32_t r = u - (uint32_t)i * sx;
+    j = (int)(r / sy);
+    k = (int)(r - (uint32_t)j * sy);
+  } else {
+    i = (int)(idx / d.sx);
+    const int64_t r = idx - (int64_t)i * d.sx;
+    j = (int)(r / d.sy);
+    k = (int)(r - (int64_t)j * d.sy);
+  }
+  // the cell's coefficient row, or the per-cell arrays
+  const double* crow = A.row ? A.tab + (int64_t)A.row[c] * A.tw : nullptr;
+  auto coef = [&](const double* per_cell, int64_t at, int off) -> double {
+    return crow ? crow[off] : per_cell[at];
+  };
   C t[9];
   coef_row<0, 9>(dn, etab, eid, idx, t);
   const int cell[3] = {i, j, k};
@@ -1218,30 +1243,31 @@ __global__ void __launch_bounds__(256)
       const int64_t pj = ((int64_t)comp * A.np + p) * A.n + c;
       const int64_t pc = (int64_t)p * A.n + c;
       if (A.kind == 3)
-        j_sum += A.cjsj[pc] * A.jc[pj] + A.cjsjp[pc] * A.jp[pj];
+        j_sum += coef(A.cjsj, pc, 2 + 7 * p + 2) * A.jc[pj] + coef(A.cjsjp, pc, 2 + 7 * p + 6) * A.jp[pj];
       else
-        j_sum += A.cjsj[pc] * A.jc[pj];
+        j_sum += coef(A.cjsj, pc, 2 + 7 * p + 2) * A.jc[pj];
     }
     double e_prev = 0.0, ceep = 0.0;
     if (A.kind == 3) {
       e_prev = A.eprev[(int64_t)comp * A.n + c];
-      ceep = A.ceep[c];
+      ceep = coef(A.ceep, c, 1);
     }
     const double tc = (double)t[3 * comp], ta = (double)t[3 * comp + 1],
                  tb = (double)t[3 * comp + 2];
     const double e_new = ceep * e_prev + (tc * e_cur + ta * (ha_p - ha_q) + tb * (hb_p - hb_q)) +
-                         A.cejs[c] * j_sum;
+                         coef(A.cejs, c, 0) * j_sum;
     Store<S>::st(f.e[comp], idx, (C)e_new, sc.s_e);
     for (int p = 0; p < A.np; ++p) {
       const int64_t pj = ((int64_t)comp * A.np + p) * A.n + c;
       const int64_t pc = (int64_t)p * A.n + c;
       if (A.kind == 1) {
-        A.jc[pj] = A.cjj[pc] * A.jc[pj] + A.cje[pc] * (e_new + e_cur);
+        A.jc[pj] = coef(A.cjj, pc, 2 + 7 * p) * A.jc[pj] + coef(A.cje, pc, 2 + 7 * p + 1) * (e_new + e_cur);
       } else if (A.kind == 2) {
-        A.jc[pj] = A.cjj[pc] * A.jc[pj] + A.cje[pc] * (e_new - e_cur);
+        A.jc[pj] = coef(A.cjj, pc, 2 + 7 * p) * A.jc[pj] + coef(A.cje, pc, 2 + 7 * p + 1) * (e_new - e_cur);
       } else {
-        const double jn = A.cjen[pc] * e_new + A.cje[pc] * e_cur + A.cjep[pc] * e_prev +
-                          A.cjj[pc] * A.jc[pj] + A.cjjp[pc] * A.jp[pj];
+        const double jn = coef(A.cjen, pc, 2 + 7 * p + 4) * e_new + coef(A.cje, pc, 2 + 7 * p + 1) * e_cur +
+                          coef(A.cjep, pc, 2 + 7 * p + 5) * e_prev + coef(A.cjj, pc, 2 + 7 * p) * A.jc[pj] +
+                          coef(A.cjjp, pc, 2 + 7 * p + 3) * A.jp[pj];
         A.jp[pj] = A.jc[pj];
         A.jc[pj] = jn;
       }
