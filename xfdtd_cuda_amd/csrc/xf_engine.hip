@@ -366,7 +366,7 @@ struct Engine {
   // that order), and after the frequency-domain NF2FF the time-domain one;
   // on stream st (the engine's own, or a batch's)
   template <typename S>
-  int launch_additions(int t, const int* tp, hipStream_t st);
+  int launch_additions(int t, const int* tp, hipStream_t st, bool with_movies = true);
   template <typename S>
   int launch_movies(int t, const int* tp, hipStream_t st);
   // z-face CPML corrections in a launch of their own after each Yee pass
@@ -2428,12 +2428,25 @@ int Engine::launch_post_e(int t, const int* tp) {
       hipLaunchKernelGGL((k_tfsf<S>), dim3((unsigned)((P.total + 255) / 256)), dim3(256), 0, stream,
                          make_fields<S>(*this), P, t, tp, make_scales<C>(*this), make_dims(*this), dim == 2 ? 1 : 0);
   }
-  XF_TRY(launch_additions<S>(t, tp, stream));
-  for (auto& N : nf) {
-    if (N.pass.total > 0)
-      hipLaunchKernelGGL((k_nf2ff<S>), dim3((unsigned)((N.pass.total + 255) / 256)), dim3(256), 0, stream,
-                         make_fields<S>(*this), N.pass, t, tp, make_dims(*this), dim == 2 ? 1 : 0,
+  // the movies share the first NF2FF box's launch (both only read the
+  // fields as the step leaves them)
+  const bool nf_movies = !movies.empty() && !nf.empty() && nf[0].pass.total > 0;
+  XF_TRY(launch_additions<S>(t, tp, stream, !nf_movies));
+  for (size_t q = 0; q < nf.size(); ++q) {
+    const NfPass& P = nf[q].pass;
+    if (P.total == 0) continue;
+    if (q == 0 && nf_movies) {
+      int64_t total_box = 0;
+      for (const MovieDev& M : movies) total_box += M.box;
+      hipLaunchKernelGGL((k_nf2ff_movies<S>), dim3((unsigned)((P.total + total_box + 255) / 256)), dim3(256), 0,
+                         stream, make_fields<S>(*this), P, static_cast<const MovieDev*>(movie_items), movie_start,
+                         static_cast<int>(movies.size()), make_dims(*this), dim == 2 ? 1 : 0,
+                         make_scales<C>(*this), t, tp);
+    } else {
+      hipLaunchKernelGGL((k_nf2ff<S>), dim3((unsigned)((P.total + 255) / 256)), dim3(256), 0, stream,
+                         make_fields<S>(*this), P, t, tp, make_dims(*this), dim == 2 ? 1 : 0,
                          make_scales<C>(*this));
+    }
   }
   XF_TRY(launch_td<S>(t, tp, stream));
   XF_HIP(hipGetLastError());
@@ -2441,7 +2454,7 @@ int Engine::launch_post_e(int t, const int* tp) {
 }
 
 template <typename S>
-int Engine::launch_additions(int t, const int* tp, hipStream_t st) {
+int Engine::launch_additions(int t, const int* tp, hipStream_t st, bool with_movies) {
   using C = typename Store<S>::C;
   if (pts.n > 0)
     hipLaunchKernelGGL((k_point_sources<S>), dim3(1), dim3(64), 0, st, make_fields<S>(*this), pts,
@@ -2465,7 +2478,7 @@ int Engine::launch_additions(int t, const int* tp, hipStream_t st) {
                            make_scales<C>(*this), ph, t, tp);
     }
   }
-  return launch_movies<S>(t, tp, st);
+  return with_movies ? launch_movies<S>(t, tp, st) : XF_OK;
 }
 
 template <typename S>

@@ -727,11 +727,9 @@ __global__ void k_point_sources(Fields<S> f, PointDev P, Scales<typename Store<S
 // (start[q] = first thread of monitor q) copies its node when its monitor
 // records this step.
 template <typename S>
-__global__ void __launch_bounds__(256)
-    k_movies(Fields<S> f, const MovieDev* __restrict__ M, const int64_t* __restrict__ start, int n, Dims d,
-             int two_d, Scales<typename Store<S>::C> sc, int t_arg, const int* t_dev) {
-  const int t = t_dev ? *t_dev : t_arg;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void movie_node(const Fields<S>& f, const MovieDev* __restrict__ M,
+                                           const int64_t* __restrict__ start, int n, const Dims& d, int two_d,
+                                           const Scales<typename Store<S>::C>& sc, int t, int64_t c) {
   if (c >= start[n]) return;
   int q = 0;
   while (q + 1 < n && c >= start[q + 1]) ++q;
@@ -747,6 +745,29 @@ __global__ void __launch_bounds__(256)
   const int64_t idx = two_d ? (int64_t)i * d.sx + j : (int64_t)i * d.sx + (int64_t)j * d.sy + k;
   const double v = p ? (double)Store<S>::ld(p, idx, m.field < 3 ? sc.inv_e : sc.inv_h) : 0.0;
   m.data[frame * m.box + r] = v;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_movies(Fields<S> f, const MovieDev* __restrict__ M, const int64_t* __restrict__ start, int n, Dims d,
+             int two_d, Scales<typename Store<S>::C> sc, int t_arg, const int* t_dev) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  movie_node<S>(f, M, start, n, d, two_d, sc, t_dev ? *t_dev : t_arg, c);
+}
+
+// An NF2FF box and the movie monitors in one launch: both only read the
+// fields at the end of the step, so they are independent.  Threads
+// [0, P.total) accumulate the box, the rest record the movies.
+template <typename S>
+__global__ void __launch_bounds__(256)
+    k_nf2ff_movies(Fields<S> f, NfPass P, const MovieDev* __restrict__ M, const int64_t* __restrict__ start,
+                   int n, Dims d, int two_d, Scales<typename Store<S>::C> sc, int t_arg, const int* t_dev) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = t_dev ? *t_dev : t_arg;
+  if (tid < P.total)
+    nf2ff_cell<S>(f, P, t, d, two_d, sc, tid);
+  else
+    movie_node<S>(f, M, start, n, d, two_d, sc, t, tid - P.total);
 }
 
 // --------------------------------------------------------- learned layer
