@@ -93,6 +93,9 @@ def main(argv=None) -> int:
     ap.add_argument("--pml-streams", action="store_true",
                     help="CPML-free and CPML tiles of a pass in two launches on two streams "
                          "(sphere workload)")
+    ap.add_argument("--xcd", action="store_true",
+                    help="XCD-aware block order: each XCD takes a contiguous eighth of the tiles "
+                         "(sphere workload)")
     ap.add_argument("--pml-ztail", action="store_true",
                     help="z-face CPML corrections at the end of each Yee tile instead of a launch "
                          "of their own (sphere workload)")
@@ -159,8 +162,8 @@ def main(argv=None) -> int:
                     coeff="dense" if dense else "auto").upload()
     upload_s = time.perf_counter() - t_up
     if args.tile_y or args.chunks or args.pml_fused or args.k32 or args.no_nt or args.pml_streams or \
-            args.pml_ztail:
-        eng.set_tiling(args.tile_y or 1, args.chunks, xcd_remap=False, nontemporal=not args.no_nt,
+            args.pml_ztail or args.xcd:
+        eng.set_tiling(args.tile_y or 1, args.chunks, xcd_remap=args.xcd, nontemporal=not args.no_nt,
                        pml_split=not args.pml_fused, k32=args.k32, pml_streams=args.pml_streams,
                        pml_ztail=args.pml_ztail)
     setup_s = time.perf_counter() - t_setup
