@@ -330,8 +330,9 @@ enum xf_tile_flags {
   XF_TILE_PML_SPLIT = 8, /* z-face CPML corrections in a launch of their own
                             after each Yee pass (3-D, FP64/FP32 only; the
                             results are unchanged, the z faces being the
-                            last corrections of a node either way; on by
-                            default when tiling_flags is 0) */
+                            last corrections of a node either way; the FP32
+                            default when tiling_flags is 0, and the FP64
+                            fallback where the tail below does not apply) */
   XF_TILE_K32 = 16,      /* 32-lane k-tiles with two j-rows per wave (fills
                             more lanes when nz + 1 is just above a multiple
                             of 64); the default is 64-lane k-tiles */
@@ -342,8 +343,10 @@ enum xf_tile_flags {
   XF_TILE_PML_ZTAIL = 64, /* z-face CPML corrections applied by each Yee
                             tile after its march, by the lanes of the slab
                             columns, while the tile's lines are still in
-                            cache (3-D, FP64/FP32 only; takes precedence
-                            over XF_TILE_PML_SPLIT; results unchanged) */
+                            cache (3-D, FP64/FP32, table coefficient mode;
+                            takes precedence over XF_TILE_PML_SPLIT; results
+                            unchanged; with XF_TILE_K32 and 3-plane marches
+                            the FP64 default when tiling_flags is 0) */
   XF_TILE_SET = 256      /* the other bits are explicit, not auto        */
 };
 
