@@ -203,14 +203,20 @@ def test_trainer_reduces_loss(gpu_available):
 
 
 def test_learned_layer_judged_against_16_cell_cpml(gpu_available):
-    """Train on GPU-recorded data of seeds 0-5, judge on seed 7 (not trained
-    on): against the 16-cell CPML run, the learned 2-cell layer reflects
-    clearly less than the same layer without its network (first-order Mur).
-    A 2-cell CPML is printed beside it for context."""
+    """Train on GPU-recorded data of seeds 0-5, six candidate layers (three
+    inits, two input-noise levels) chosen on validation seeds 11-13, judged
+    on seeds 7-10 (neither trained on nor used for the choice): against the
+    16-cell CPML run, the learned 2-cell layer reflects less than a 2-cell
+    CPML and clearly less than the same layer without its network
+    (first-order Mur), averaged over the four judge seeds.  Training and the
+    FP64 judge runs are deterministic, so the outcome is too."""
+    print("case: learned layer, 6 candidates, validation 11-13, judge 7-10", flush=True)
     smp, k = learned.record_samples(n=32, seeds=(0, 1, 2, 3, 4, 5), steps=300, every=2)
-    w16, loss0, loss1 = learned.train_layer(smp, k)
-    err = learned.reflection_errors(w16, n=32, seed=7, steps=300, every=15,
-                                    variants=[("learned", 2), ("mur", 2), ("cpml", 2)])
-    print("loss", loss0, "->", loss1, "reflection", err)
-    assert loss1 < 0.1 * loss0
-    assert err["learned2"] < 0.8 * err["mur2"]
+    w16, table = learned.train_selected(smp, k)
+    print("candidates", [(r["noise"], r["init"], round(r["val_mean"], 5)) for r in table])
+    errs = [learned.reflection_errors(w16, seed=s, variants=[("learned", 2), ("mur", 2), ("cpml", 2)])
+            for s in (7, 8, 9, 10)]
+    mean = {key: float(np.mean([e[key] for e in errs])) for key in errs[0]}
+    print("judge seeds 7-10:", errs, "mean", mean)
+    assert mean["learned2"] < mean["cpml2"], mean
+    assert mean["learned2"] < 0.8 * mean["mur2"], mean

@@ -210,10 +210,13 @@ def run_snapshots(boundary: str, n: int, seed: int, steps: int, every: int, size
 
 
 def reflection_errors(weights, n: int = 40, seed: int = 7, steps: int = 400, every: int = 20,
-                      precision: str = "fp64", variants=None):
+                      precision: str = "fp64", variants=None, ref=None):
     """Relative L2 error of the interior E field over time of each boundary
-    against the 16-cell CPML run (the judge)."""
-    ref, _, _ = run_snapshots("cpml", n, seed, steps, every, 16, precision=precision)
+    against the 16-cell CPML run (the judge).  `ref` may pass that run's
+    snapshots (run_snapshots("cpml", ..., 16)) when several boundaries are
+    judged on one seed."""
+    if ref is None:
+        ref, _, _ = run_snapshots("cpml", n, seed, steps, every, 16, precision=precision)
     den = np.sqrt(sum(float((c ** 2).sum()) for snap in ref for c in snap))
     variants = variants or [("learned", 2), ("mur", 2), ("cpml", 2), ("cpml", 4), ("cpml", 8)]
     out = {}
@@ -225,3 +228,32 @@ def reflection_errors(weights, n: int = 40, seed: int = 7, steps: int = 400, eve
                           for a, b in zip(sa, sb)))
         out[f"{kind}{size}"] = num / den
     return out
+
+
+def train_selected(samples: np.ndarray, k: float, inits=(0, 1, 2), noises=(0.01, 0.02),
+                   val_seeds=(11, 12, 13), n: int = 40, steps: int = 400, every: int = 20,
+                   device: int = 0):
+    """One layer per (input-noise level, network init) by train_layer, and
+    the one whose closed-loop reflection against the 16-cell CPML, averaged
+    over the validation seeds, is lowest.  The validation seeds are scenes
+    neither the training data (seeds 0-5) nor the judge (seeds 7-10) use:
+    one-step training loss does not rank layers by their closed-loop
+    reflection (over six inits and three noise levels the validation means
+    ranged from 1.3e-2 to 2.5e-2 at nearly equal loss), so the choice is
+    made on held-out closed-loop runs.  Returns (fp16 weight bits, table of
+    the candidates: noise, init, loss, validation errors and mean)."""
+    refs = {sd: run_snapshots("cpml", n, sd, steps, every, 16)[0] for sd in val_seeds}
+    table = []
+    best = None
+    for noise in noises:
+        for init in inits:
+            w16, _, l1 = train_layer(samples, k, seed=init, device=device, noise=noise)
+            val = [reflection_errors(w16, n=n, seed=sd, steps=steps, every=every,
+                                     variants=[("learned", 2)], ref=refs[sd])["learned2"]
+                   for sd in val_seeds]
+            row = {"noise": noise, "init": init, "loss": float(l1),
+                   "val": [float(v) for v in val], "val_mean": float(np.mean(val))}
+            table.append(row)
+            if best is None or row["val_mean"] < best[1]["val_mean"]:
+                best = (w16, row)
+    return best[0], table
