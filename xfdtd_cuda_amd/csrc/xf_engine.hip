@@ -1716,7 +1716,8 @@ int Engine::upload() {
 }
 
 // Tiling of the 3-D marching kernels.  chunks_x = 0 gives short marches
-// (3 or 4 planes FP64, 6 FP32, 8 16-bit): measured at 536^3 on MI355X, many short
+// (3 or 4 planes FP64, 6 FP32, 16 for 16-bit and compensated storage):
+// measured at 536^3 on MI355X, many short
 // marches beat long ones (a long march leaves a partly empty tail round of
 // blocks and keeps far-apart planes open at once), while the carried plane
 // still saves a neighbour re-read every step of the march.
@@ -1751,9 +1752,11 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
     nkb = static_cast<int>((nz + 1 + klanes - 1) / klanes);
     njb = static_cast<int>((ny + 1 + rows() - 1) / rows());
     // default march: 4 planes for 8-byte fields (3 with the z faces at the
-    // tile end), 6 for 4-byte ones, 8 for narrower ones (the measured optima
-    // on MI355X at 536^3; FP32 6 against 8 planes: 56.9 against 56.2 Gcells/s)
-    const int64_t planes = esize == 8 ? (ztail() ? 3 : 4) : esize == 4 ? 6 : 8;
+    // tile end), 6 for 4-byte ones, 16 for narrower ones (the measured optima
+    // on MI355X at 536^3; FP32 6 against 8 planes: 56.9 against 56.2 Gcells/s;
+    // bf16 16 against 8: 63.5 against 61.4, fp16 61.6 against 59.8,
+    // compensated bf16 37.0 against 35.8)
+    const int64_t planes = esize == 8 ? (ztail() ? 3 : 4) : esize == 4 ? 6 : 16;
     int64_t nch = chunks_x;
     if (nch <= 0) nch = std::max<int64_t>(1, (nx + 1 + planes - 1) / planes);
     nch = std::min<int64_t>(nch, nx + 1);
