@@ -42,6 +42,21 @@ if ROOT not in sys.path:
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
 
 
+def traffic_file(name: str, tuning: dict | None = None):
+    """HBM bytes per launch from profiles/traffic_<name>.json (two rocprofv3
+    counter runs, FETCH_SIZE and WRITE_SIZE, of the same workload), or None
+    when there is no such file or it was taken at another tiling."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    want = rec.get("tuning")
+    if want and tuning is not None and any(tuning.get(k) != v for k, v in want.items()):
+        return None
+    return rec.get("hbm_bytes_per_launch")
+
+
 def pml_bytes(sim, pass_name: str, cbytes: int) -> int:
     """Algorithmic CPML bytes of one pass: for every psi element updated,
     read psi, write psi, read c_psi (the four psi arrays of a face hold
@@ -416,7 +431,9 @@ def bench_batch(args) -> int:
                    "far_field_accuracy": accuracy},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": (traffic_file(f"{kernel}_{prec}_64grids", best)
+                                 if kernel == "k_yee_e3_batch" and len(sims) == 64 else None),
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_ms, 4),
                      "h_ms": round(h_ms, 4), "e_ms": round(e_ms, 4)},
         "cpu_baseline": cpu,
@@ -688,13 +705,15 @@ def bench_fp32_vs_fp64(args) -> int:
         # Ez, Hx, Hy and writes Ez (4 values) + id; plus the CPML psi traffic
         e_bytes = ncell * ((4 if two_d else 9) * sb + 1) + pml_bytes(sim, "e", cb)
         e_ms = tm["yee_e_ms"] / tm["yee_e_launches"]
-        out["roofline"] = {"bound": "hbm", "kernel": ("k_yee_e2" if two_d else
-                                                      ("k_yee_e3+k_pml_z+k_ade_e3" if "pml_split=1" in desc
-                                                       else "k_yee_e3+k_ade_e3")),
+        kname = ("k_yee_e2" if two_d else
+                 ("k_yee_e3+k_pml_z+k_ade_e3" if "pml_split=1" in desc else "k_yee_e3+k_ade_e3"))
+        out["roofline"] = {"bound": "hbm", "kernel": kname,
                            "achieved": round(e_bytes / (e_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS,
                            "unit": "GB/s",
                            "frac": round(e_bytes / (e_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-                           "traffic": None, "alg_bytes_per_launch": e_bytes,
+                           "traffic": (None if two_d else
+                                       traffic_file(f"{kname.replace('+', '_')}_{prec}_{nx}")),
+                           "alg_bytes_per_launch": e_bytes,
                            "avg_launch_ms": round(e_ms, 5), "h_ms": round(tm["yee_h_ms"] / tm["yee_h_launches"], 5),
                            "post_h_ms": round(tm["tfsf_ms"] / 10, 5), "post_e_ms": round(tm["nf2ff_ms"] / 10, 5),
                            "note": ("E pass events (the Yee kernel with its CPML, plus the ADE kernel "
