@@ -114,6 +114,9 @@ def main(argv=None) -> int:
     ap.add_argument("--pml-ztail", action="store_true",
                     help="z-face CPML corrections at the end of each Yee tile instead of a launch "
                          "of their own (sphere workload)")
+    ap.add_argument("--tfsf-ztail", action="store_true",
+                    help="with --pml-ztail: the TF/SF corrections of the z = const faces applied "
+                         "by the Yee tiles too (sphere workload)")
     ap.add_argument("--pml-fused", action="store_true",
                     help="z-face CPML corrections inside the Yee kernels instead of a launch of "
                          "their own (sphere workload)")
@@ -177,10 +180,10 @@ def main(argv=None) -> int:
                     coeff="dense" if dense else "auto").upload()
     upload_s = time.perf_counter() - t_up
     if args.tile_y or args.chunks or args.pml_fused or args.k32 or args.no_nt or args.pml_streams or \
-            args.pml_ztail or args.xcd:
+            args.pml_ztail or args.tfsf_ztail or args.xcd:
         eng.set_tiling(args.tile_y or 1, args.chunks, xcd_remap=args.xcd, nontemporal=not args.no_nt,
                        pml_split=not args.pml_fused, k32=args.k32, pml_streams=args.pml_streams,
-                       pml_ztail=args.pml_ztail)
+                       pml_ztail=args.pml_ztail, tfsf_ztail=args.tfsf_ztail)
     setup_s = time.perf_counter() - t_setup
     print(f"# setup {setup_s:.1f} s, grid {nx}x{ny}x{nz}, {eng.describe()}", file=sys.stderr,
           flush=True)

@@ -347,6 +347,13 @@ enum xf_tile_flags {
                             takes precedence over XF_TILE_PML_SPLIT; results
                             unchanged; with XF_TILE_K32 and 3-plane marches
                             the FP64 default when tiling_flags is 0) */
+  XF_TILE_TFSF_ZTAIL = 128, /* with the z-face CPML tail above: the TF/SF
+                            corrections of the z = const faces applied by
+                            the Yee tiles after their march as well, except
+                            at the nodes other faces also correct, which
+                            stay with the TF/SF launch (3-D, no ADE media,
+                            not with XF_TILE_PML_STREAMS; results
+                            unchanged) */
   XF_TILE_SET = 256      /* the other bits are explicit, not auto        */
 };
 

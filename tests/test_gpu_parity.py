@@ -441,6 +441,72 @@ def test_pml_ztail_fp32_and_bf16(gpu_available):
     assert "pml_ztail=0" in run(_uneven_faces(80), "bf16", True)
 
 
+@pytest.mark.parametrize("scene,tile_y,chunks,k32,streams", [
+    ("sphere", 1, 0, True, False), ("sphere", 4, 1, False, False), ("sphere", 2, 7, True, False),
+    ("sphere", 3, 1000, False, False), ("uneven", 2, 5, False, False), ("uneven", 1, 0, True, False),
+    ("one_z", 1, 3, True, False), ("uneven", 1, 0, True, True)])
+def test_tfsf_ztail_bit_identical(gpu_available, scene, tile_y, chunks, k32, streams):
+    """The TF/SF corrections of the z = const faces applied by the Yee tiles
+    after their march (XF_TILE_TFSF_ZTAIL), the face edges that other faces
+    also correct left to k_tfsf: bit-identical to the oracle, fields, psi
+    and NF2FF currents, for tile shapes and march lengths that put the TF/SF
+    z faces inside and at the ends of k-tiles and marches.  With the two-
+    stream CPML split the tail stays off."""
+    steps = 90
+    if scene == "sphere":
+        build = lambda: scenes.dielectric_sphere(n=21, pml=5, steps=steps, tfsf_inset=8, nf2ff_inset=6)
+    elif scene == "uneven":
+        build = lambda: _uneven_faces(steps)
+    else:
+        def build():
+            s = xf.Simulation(3e-3, 3e-3, 3e-3, 0.95)
+            s.add_cube("domain", (-0.04, -0.04, -0.05), (0.08, 0.08, 0.1), xf.Material.air())
+            s.add_sphere("s", (0.002, 0.0, 0.004), 0.015, xf.Material.dielectric(3.0, 1.0, 0.02))
+            s.add_pml(5, "XN").add_pml(6, "ZP")
+            tau = 20 * 3e-3 / 6e8
+            s.add_tfsf3d((8, 8, 9), 0.5, 0.9, 0.2, xf.Waveform.gaussian(tau, 4.5 * tau))
+            s.init(steps)
+            return s
+    print(f"case: tfsf ztail {scene} ty={tile_y} ch={chunks} k32={k32} streams={streams}", flush=True)
+    g, c = build(), build()
+    e = xf.Engine(g).upload()
+    e.set_tiling(tile_y=tile_y, chunks_x=chunks, pml_ztail=True, tfsf_ztail=True, k32=k32,
+                 pml_streams=streams)
+    desc = e.describe()
+    assert f"tfsf_ztail={0 if streams else 1}" in desc, desc
+    e.run(35).run(steps - 35).download()
+    e.close()
+    oracle.run(c.scene_ptr(), steps)
+    for n in _state_names(c):
+        assert np.array_equal(g.array(n), c.array(n)), f"{n} differs ({scene} ty={tile_y} ch={chunks})"
+
+
+def test_tfsf_ztail_fp32_graph_and_ade(gpu_available):
+    """FP32 with the TF/SF z-face tail equals FP32 without it, bit for bit,
+    stepped eagerly and as hipGraph replays (the tail reads the step from the
+    device counter there); a scene with ADE media keeps the tail off."""
+    def run(sim, tail, graph=False):
+        e = xf.Engine(sim, precision="fp32", use_graph=graph).upload()
+        e.set_tiling(tile_y=1, k32=True, pml_ztail=True, tfsf_ztail=tail)
+        desc = e.describe()
+        e.run(40).run(40).download()
+        e.close()
+        return desc
+
+    a, b, c = _uneven_faces(80), _uneven_faces(80), _uneven_faces(80)
+    assert "tfsf_ztail=1" in run(a, True)
+    assert "tfsf_ztail=0" in run(b, False)
+    assert "tfsf_ztail=1" in run(c, True, graph=True)
+    for n in FIELDS:
+        assert np.array_equal(a.array(n), b.array(n)), n
+        assert np.array_equal(c.array(n), b.array(n)), n
+    d = _dispersive("mix")()
+    e = xf.Engine(d).upload()
+    e.set_tiling(tile_y=1, k32=True, pml_ztail=True, tfsf_ztail=True)
+    assert "tfsf_ztail=0" in e.describe()
+    e.close()
+
+
 def test_fp32_close_to_fp64(gpu_available):
     build = lambda: scenes.dielectric_sphere(n=24, pml=6, steps=140, tfsf_inset=9, nf2ff_inset=7)
     g, c = _pair(build, 140, precision="fp32")

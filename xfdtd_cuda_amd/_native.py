@@ -57,6 +57,7 @@ TILE_PML_SPLIT = 8
 TILE_K32 = 16
 TILE_PML_STREAMS = 32
 TILE_PML_ZTAIL = 64
+TILE_TFSF_ZTAIL = 128
 TILE_SET = 256
 
 

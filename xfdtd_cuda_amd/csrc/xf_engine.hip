@@ -137,6 +137,9 @@ struct TfsfJobHost {
   int comp, face;
   int64_t o[3], n[3];
   double coef;
+  // z = const job whose nodes other jobs of its component meet only at the
+  // two ends of this axis (0 or 1): the inner ones may go to the Yee tiles
+  int tail_axis{-1};
 };
 
 }  // namespace
@@ -240,6 +243,7 @@ struct Engine {
   int nt_streams{1};
   int pml_split{0};  // XF_TILE_PML_SPLIT requested (see split_z())
   int pml_ztail{0};  // XF_TILE_PML_ZTAIL requested (see ztail())
+  int tfsf_zt{0};    // XF_TILE_TFSF_ZTAIL requested (see tfsf_ztail())
   int klanes{64};    // k-lanes of a tile row: 64, or 32 (XF_TILE_K32)
   // XF_TILE_PML_STREAMS: per pass a tile list, CPML-free tiles first, then
   // the tiles that meet a face the Yee kernel carries; the CPML part runs on
@@ -372,6 +376,10 @@ struct Engine {
   // z-face CPML corrections in a launch of their own after each Yee pass
   bool split_z() const;
   bool ztail() const;
+  // the z = const TF/SF corrections applied by the Yee tiles: with the
+  // z-face CPML tail (whose instantiation carries the code), one launch per
+  // pass, and no ADE (whose kernel rewrites E after the Yee pass)
+  bool tfsf_ztail() const { return tfsf_zt && dim == 3 && sc && sc->tfsf && !has_ade && !use_lists() && ztail(); }
   // device index of history entry i (xf_learned_boundary layout); y faces
   // are stored [b][a] on the device (LbFace::tr)
   int64_t lb_dev_hist_index(int64_t i) const;
@@ -813,6 +821,34 @@ int Engine::build_tfsf_jobs() {
             if (index + 1 >= tab.shape[1]) return fail(XF_ERR_INVALID, "tfsf: incident table too short");
           }
       tfsf_jobs[pass].push_back(J);
+    }
+    // z = const jobs whose inner nodes no other job of the same component
+    // writes: XF_TILE_TFSF_ZTAIL hands those to the Yee tiles
+    if (dim != 3) continue;
+    auto& jobs = tfsf_jobs[pass];
+    int tails = 0;
+    for (size_t q = 0; q < jobs.size() && tails < 4; ++q) {
+      TfsfJobHost& J = jobs[q];
+      if (J.face / 2 != 2 || J.n[2] != 1) continue;
+      for (int e = 0; e < 2 && J.tail_axis < 0; ++e) {
+        if (J.n[e] < 3) continue;
+        bool ends_only = true;
+        for (size_t p = 0; p < jobs.size() && ends_only; ++p) {
+          const TfsfJobHost& X = jobs[p];
+          if (p == q || X.comp != J.comp) continue;
+          int64_t lo[3], hi[3];
+          bool empty = false;
+          for (int d = 0; d < 3; ++d) {
+            lo[d] = std::max(J.o[d], X.o[d]);
+            hi[d] = std::min(J.o[d] + J.n[d], X.o[d] + X.n[d]);
+            empty = empty || lo[d] >= hi[d];
+          }
+          if (empty) continue;
+          ends_only = hi[e] == J.o[e] + 1 || lo[e] == J.o[e] + J.n[e] - 1;
+        }
+        if (ends_only) J.tail_axis = e;
+      }
+      if (J.tail_axis >= 0) ++tails;
     }
   }
   return XF_OK;
@@ -1733,6 +1769,7 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
     nt_streams = (flags & XF_TILE_NT) ? 1 : 0;
     pml_split = (flags & XF_TILE_PML_SPLIT) ? 1 : 0;
     pml_ztail = (flags & XF_TILE_PML_ZTAIL) ? 1 : 0;
+    tfsf_zt = (flags & XF_TILE_TFSF_ZTAIL) ? 1 : 0;
     klanes = (flags & XF_TILE_K32) ? 32 : 64;
     pml_streams = (flags & XF_TILE_PML_STREAMS) ? 1 : 0;
   } else {
@@ -1745,6 +1782,7 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
     nt_streams = 1;
     pml_split = 1;
     pml_ztail = esize == 8 ? 1 : 0;
+    tfsf_zt = 0;
     pml_streams = 0;
     klanes = esize == 8 ? 32 : 64;
   }
@@ -1775,6 +1813,7 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
                 " htab=" + std::to_string(ntab_h) + " chunk=" + std::to_string(chunk) +
                 " nchunk=" + std::to_string(nch_eff) + " tile_y=" + std::to_string(tile_y) +
                 " xcd=" + std::to_string(xcd) + " nt=" + std::to_string(nt_streams) + " pml_split=" + std::to_string(split_z() ? 1 : 0) + " pml_ztail=" + std::to_string(ztail() ? 1 : 0) +
+                " tfsf_ztail=" + std::to_string(tfsf_ztail() ? 1 : 0) +
                 " klanes=" + std::to_string(klanes) + " pml_streams=" + std::to_string(pml_streams) +
                 " pml_faces=" + std::to_string(pml.size()) +
                 " ade_cells=" + std::to_string(ade_cells.size()) +
@@ -2159,8 +2198,26 @@ int Engine::launch_pml_z(bool is_e, hipStream_t st) {
 }
 
 namespace {
+// What tfsf_inc reads, for a TfsfPass or a TfsfZTail.
+template <template <typename> class Pass, typename C>
+void fill_tfsf_inc(const Engine& E, Pass<C>& P, int pass) {
+  for (int d = 0; d < 3; ++d) {
+    P.proj_int[d] = static_cast<const C*>(E.proj_int[d]);
+    P.proj_half[d] = static_cast<const C*>(E.proj_half[d]);
+  }
+  const xf_tfsf& T = *E.sc->tfsf;
+  P.inc = static_cast<const C*>(pass == 1 ? E.h_inc : E.e_inc);
+  P.n_aux = pass == 1 ? T.h_inc.shape[1] : T.e_inc.shape[1];
+  P.g[0] = static_cast<int>(T.total_task.x.start);
+  P.g[1] = static_cast<int>(T.total_task.y.start);
+  P.g[2] = static_cast<int>(T.total_task.z.start);
+  for (int d = 0; d < 3; ++d) P.tr[d] = static_cast<C>(pass == 1 ? T.transform_h[d] : T.transform_e[d]);
+}
+
+// tail: the inner nodes of the jobs with a tail axis go to the Yee tiles
+// (fill_tfsf_ztail); their jobs then enumerate only the two end rows.
 template <typename C>
-void fill_tfsf_pass(const Engine& E, TfsfPass<C>& P, int pass) {
+void fill_tfsf_pass(const Engine& E, TfsfPass<C>& P, int pass, bool tail = false) {
   const auto& jobs = E.tfsf_jobs[pass];
   P.njob = static_cast<int>(jobs.size());
   P.is_e = pass == 1;
@@ -2176,20 +2233,37 @@ void fill_tfsf_pass(const Engine& E, TfsfPass<C>& P, int pass) {
     }
     D.start = start;
     D.coef = static_cast<C>(J.coef);
-    start += J.n[0] * J.n[1] * J.n[2];
+    D.edge_axis = tail ? J.tail_axis : -1;
+    int64_t count = 1;
+    for (int d = 0; d < 3; ++d) count *= d == D.edge_axis ? 2 : J.n[d];
+    start += count;
   }
   P.total = start;
-  for (int d = 0; d < 3; ++d) {
-    P.proj_int[d] = static_cast<const C*>(E.proj_int[d]);
-    P.proj_half[d] = static_cast<const C*>(E.proj_half[d]);
+  fill_tfsf_inc(E, P, pass);
+}
+
+// The jobs fill_tfsf_pass(.., tail = true) leaves to the Yee tiles.
+template <typename C>
+void fill_tfsf_ztail(const Engine& E, TfsfZTail<C>& Z, int pass, int t, const int* tp) {
+  Z = TfsfZTail<C>{};
+  const bool is_e = pass == 1;
+  for (const TfsfJobHost& J : E.tfsf_jobs[pass]) {
+    if (J.tail_axis < 0) continue;
+    TfsfZJob<C>& D = Z.z[Z.n++];
+    const int e = J.tail_axis;
+    D.comp = J.comp;
+    D.dual = J.comp == 0 ? 1 : 0;  // face axis z: a = x, b = y
+    D.k = static_cast<int>(J.o[2]);
+    D.kinc = D.k + (J.face % 2 == 0 ? (is_e ? -1 : 1) : 0);
+    D.i0 = static_cast<int>(J.o[0] + (e == 0 ? 1 : 0));
+    D.i1 = static_cast<int>(J.o[0] + J.n[0] - (e == 0 ? 1 : 0));
+    D.j0 = static_cast<int>(J.o[1] + (e == 1 ? 1 : 0));
+    D.j1 = static_cast<int>(J.o[1] + J.n[1] - (e == 1 ? 1 : 0));
+    D.coef = static_cast<C>(J.coef);
   }
-  const xf_tfsf& T = *E.sc->tfsf;
-  P.inc = static_cast<const C*>(pass == 1 ? E.h_inc : E.e_inc);
-  P.n_aux = pass == 1 ? T.h_inc.shape[1] : T.e_inc.shape[1];
-  P.g[0] = static_cast<int>(T.total_task.x.start);
-  P.g[1] = static_cast<int>(T.total_task.y.start);
-  P.g[2] = static_cast<int>(T.total_task.z.start);
-  for (int d = 0; d < 3; ++d) P.tr[d] = static_cast<C>(pass == 1 ? T.transform_h[d] : T.transform_e[d]);
+  Z.t_arg = t;
+  Z.t_dev = tp;
+  fill_tfsf_inc(E, Z, pass);
 }
 }  // namespace
 
@@ -2207,7 +2281,8 @@ int Engine::launch_h(int t, const int* tp) {
   const bool has_pml = P.n > 0;
   const C* tab = static_cast<const C*>(htab);
   if (dim == 3) {
-    const YeePass<S> D = yee_pass<S>(false);
+    YeePass<S> D = yee_pass<S>(false);
+    if (tfsf_ztail()) fill_tfsf_ztail<C>(*this, D.tz, 0, t, tp);
     const bool has_pml = D.pml.n > 0 || D.zt.n > 0;
     dim3 block(klanes, rows());
     dim3 grid((unsigned)(D.nkb * D.njb * D.nch));
@@ -2307,7 +2382,8 @@ int Engine::launch_e(int t, const int* tp) {
   const bool has_pml = P.n > 0;
   const C* tab = static_cast<const C*>(etab);
   if (dim == 3) {
-    const YeePass<S> D = yee_pass<S>(true);
+    YeePass<S> D = yee_pass<S>(true);
+    if (tfsf_ztail()) fill_tfsf_ztail<C>(*this, D.tz, 1, t, tp);
     const bool has_pml = D.pml.n > 0 || D.zt.n > 0;
     dim3 block(klanes, rows());
     dim3 grid((unsigned)(D.nkb * D.njb * D.nch));
@@ -2413,7 +2489,7 @@ int Engine::launch_post_h(int t, const int* tp) {
   using C = typename Store<S>::C;
   if (sc->tfsf && !tfsf_fused()) {
     TfsfPass<C> P{};
-    fill_tfsf_pass<C>(*this, P, 0);
+    fill_tfsf_pass<C>(*this, P, 0, tfsf_ztail());
     if (P.total > 0)
       hipLaunchKernelGGL((k_tfsf<S>), dim3((unsigned)((P.total + 255) / 256)), dim3(256), 0, stream,
                          make_fields<S>(*this), P, t, tp, make_scales<C>(*this), make_dims(*this), dim == 2 ? 1 : 0);
@@ -2427,7 +2503,7 @@ int Engine::launch_post_e(int t, const int* tp) {
   using C = typename Store<S>::C;
   if (sc->tfsf && !tfsf_fused()) {
     TfsfPass<C> P{};
-    fill_tfsf_pass<C>(*this, P, 1);
+    fill_tfsf_pass<C>(*this, P, 1, tfsf_ztail());
     if (P.total > 0)
       hipLaunchKernelGGL((k_tfsf<S>), dim3((unsigned)((P.total + 255) / 256)), dim3(256), 0, stream,
                          make_fields<S>(*this), P, t, tp, make_scales<C>(*this), make_dims(*this), dim == 2 ? 1 : 0);

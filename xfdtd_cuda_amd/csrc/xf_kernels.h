@@ -292,6 +292,35 @@ struct Dense9 {
   const C* a[9];
 };
 
+// A z = const TF/SF correction left to the Yee tiles (XF_TILE_TFSF_ZTAIL):
+// the written nodes at k that no other face of the same component also
+// writes, [i0, i1) x [j0, j1); the incident value is taken at kinc (k, or
+// the node beyond the face on its negative side).
+template <typename C>
+struct TfsfZJob {
+  int comp, dual;
+  int k, kinc;
+  int i0, i1, j0, j1;
+  C coef;
+};
+
+// The z = const TF/SF corrections of a pass applied after each tile's
+// march, with what the incident interpolation needs (field names as in
+// TfsfPass, so tfsf_inc reads either).
+template <typename C>
+struct TfsfZTail {
+  int n;
+  int t_arg;
+  const int* t_dev;
+  TfsfZJob<C> z[4];
+  const C* proj_int[3];
+  const C* proj_half[3];
+  const C* inc;
+  int64_t n_aux;
+  int g[3];
+  C tr[3];
+};
+
 // Everything one 3-D Yee pass (H or E) of one grid needs: the kernels take
 // it as a kernel argument (one grid) or from an array in device memory
 // (a batch of grids, one launch).
@@ -309,6 +338,7 @@ struct YeePass {
   int nkb, njb, chunk, nch;     // tiling of this grid
   PmlPass<C> pml;
   PmlZTail<C> zt;               // z faces applied after each tile's march
+  TfsfZTail<C> tz;              // z = const TF/SF faces, likewise (n = 0: none)
 };
 
 // A TFSF face correction of one component (one reference launch of
@@ -319,6 +349,8 @@ struct TfsfJob {
   int o[3], n[3];  // written box origin and extent
   int64_t start;   // first flat thread index of the job
   C coef;          // c{a|b}{axis} * coefficient_flag
+  int edge_axis;   // -1: a thread per written node; else only the nodes at
+                   // the two ends of this axis (the rest is a TfsfZJob's)
 };
 
 template <typename C>

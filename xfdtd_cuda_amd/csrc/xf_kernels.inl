@@ -147,12 +147,11 @@ __device__ __forceinline__ const C* load_table(const C* __restrict__ g, int nval
   return smem;
 }
 
-#include "xf_yee3d.inl"
-
 // ------------------------------------------------------------------ TFSF
 // exInc..hzInc (tfsf_corrector.cu:18-130): interpolated 1-D incident value.
-template <typename C>
-__device__ __forceinline__ C tfsf_inc(const TfsfPass<C>& P, bool inc_is_e, int comp, int t,
+// P is a TfsfPass or a TfsfZTail (the same field names).
+template <template <typename> class Pass, typename C>
+__device__ __forceinline__ C tfsf_inc(const Pass<C>& P, bool inc_is_e, int comp, int t,
                                       int i, int j, int k) {
   const int li = i - P.g[0], lj = j - P.g[1], lk = k - P.g[2];
   C proj;
@@ -176,6 +175,8 @@ __device__ __forceinline__ C tfsf_inc(const TfsfPass<C>& P, bool inc_is_e, int c
   const __attribute__((address_space(1))) C* row = gp(P.inc) + (int64_t)t * P.n_aux;
   return P.tr[comp] * ((1 - weight) * row[index] + weight * row[index + 1]);
 }
+
+#include "xf_yee3d.inl"
 
 // All TFSF face corrections of one pass in one launch.  Each thread owns a
 // written cell of one job; cells shared by several faces of the same
@@ -207,8 +208,14 @@ __device__ __forceinline__ void tfsf_cell(const Fields<S>& f, const TfsfPass<typ
   int q = 0;
   while (q + 1 < P.njob && tid >= P.job[q + 1].start) ++q;
   const TfsfJob<C>& J = P.job[q];
+  // a job whose inner nodes the Yee tiles correct enumerates only the two
+  // end rows of its edge axis (offset 0 or n - 1 along it)
+  const int ea = J.edge_axis;
   int li, lj, lk;
-  box_coords(tid - J.start, J.n[1], J.n[2], li, lj, lk);
+  box_coords(tid - J.start, ea == 1 ? 2 : J.n[1], ea == 2 ? 2 : J.n[2], li, lj, lk);
+  if (ea == 0 && li) li = J.n[0] - 1;
+  if (ea == 1 && lj) lj = J.n[1] - 1;
+  if (ea == 2 && lk) lk = J.n[2] - 1;
   const int cell[3] = {J.o[0] + li, J.o[1] + lj, J.o[2] + lk};
   auto inside = [&](const TfsfJob<C>& X) {
     return cell[0] >= X.o[0] && cell[0] < X.o[0] + X.n[0] && cell[1] >= X.o[1] &&

@@ -156,6 +156,66 @@ __device__ __forceinline__ void pml_z_tail(const PmlZTail<typename Store<S>::C>&
   }
 }
 
+// The z = const TF/SF corrections of the tile's column (j, k) over
+// [i0, i1), applied after its march (XF_TILE_TFSF_ZTAIL) instead of by
+// k_tfsf, whose threads would each fetch a line for one node of such a
+// face.  The host hands over only nodes that no other face of the same
+// component writes, so each gets exactly the one correction k_tfsf would
+// add (tfsf_cell), after the node's CPML corrections as there.
+template <bool ISE, typename S>
+__device__ __forceinline__ void tfsf_z_tail(const TfsfZTail<typename Store<S>::C>& Z, int i0, int i1, int j,
+                                            int k, int64_t idx0, const Fields<S>& f, const Dims& d,
+                                            const Scales<typename Store<S>::C>& sc) {
+  using C = typename Store<S>::C;
+  const C inv = ISE ? sc.inv_e : sc.inv_h;
+  const C s = ISE ? sc.s_e : sc.s_h;
+  const bool inc_e = !ISE;  // the H pass adds incident E, the E pass incident H
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (q >= Z.n) break;
+    const TfsfZJob<C>& J = Z.z[q];
+    if (k != J.k || j < J.j0 || j >= J.j1) continue;
+    const int ia = max(i0, J.i0), ib = min(i1, J.i1);
+    if (ia >= ib) continue;
+    const int t = Z.t_dev ? *Z.t_dev : Z.t_arg;
+    S* field = ISE ? f.e[J.comp] : f.h[J.comp];
+    // tfsf_inc for this job with its j and k terms read once: the x term is
+    // on the half grid where tfsf_inc takes it so (incident E along x, or
+    // incident H across x), the sums in tfsf_inc's order
+    const bool xh = inc_e ? J.dual == 0 : J.dual != 0;
+    const bool yh = inc_e ? J.dual == 1 : J.dual != 1;
+    const __attribute__((address_space(1))) C* px = gp(xh ? Z.proj_half[0] + 1 : Z.proj_int[0]) - Z.g[0];
+    const int lj = j - Z.g[1], lk = J.kinc - Z.g[2];
+    const C py = yh ? gp(Z.proj_half[1])[lj + 1] : gp(Z.proj_int[1])[lj];
+    const C pz = inc_e ? gp(Z.proj_int[2])[lk] : gp(Z.proj_half[2])[lk + 1];
+    const __attribute__((address_space(1))) C* row = gp(Z.inc) + (int64_t)t * Z.n_aux;
+    const C tr = Z.tr[J.dual];
+    // four planes at a time, every load issued before the first use (the
+    // last plane repeated where fewer are left; only real ones are stored)
+    for (int i = ia; i < ib; i += 4) {
+      C v[4], w0[4], w1[4], wt[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int iu = min(i + u, ib - 1);
+        v[u] = Store<S>::ld(field, idx0 + (int64_t)(iu - i0) * d.sx, inv);
+        C proj = px[iu] + py + pz;
+        if (!inc_e) proj = proj - (C)0.5;
+        const uint64_t index = (uint64_t)proj;
+        wt[u] = proj - (C)index;
+        w0[u] = row[index];
+        w1[u] = row[index + 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (i + u >= ib) break;
+        const C inc = tr * ((1 - wt[u]) * w0[u] + wt[u] * w1[u]);
+        const C val = v[u] + inc * J.coef;
+        Store<S>::st(field, idx0 + (int64_t)(i + u - i0) * d.sx, val, s);
+      }
+    }
+  }
+}
+
 // One tile of the H pass.  Lanes beyond the grid return at once; the tile
 // bodies hold no barrier, so the batched kernels may loop over tiles with
 // some lanes idle.
@@ -217,6 +277,7 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
   }
   if constexpr (PML == 2) {
     if (D.zt.n > 0) pml_z_tail<false, S>(D.zt, i0, i1, j, k, idx0, f, d, sc);
+    if (D.tz.n > 0) tfsf_z_tail<false, S>(D.tz, i0, i1, j, k, idx0, f, d, sc);
   }
 }
 
@@ -290,6 +351,7 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
   }
   if constexpr (PML == 2) {
     if (D.zt.n > 0) pml_z_tail<true, S>(D.zt, i0, i1, j, k, idx0, f, d, sc);
+    if (D.tz.n > 0) tfsf_z_tail<true, S>(D.tz, i0, i1, j, k, idx0, f, d, sc);
   }
 }
 

@@ -510,22 +510,26 @@ class Engine:
     def set_tiling(self, tile_y: int = 0, chunks_x: int = 0, xcd_remap: bool | None = None,
                    nontemporal: bool | None = None, pml_split: bool | None = None,
                    k32: bool | None = None, pml_streams: bool | None = None,
-                   pml_ztail: bool | None = None):
+                   pml_ztail: bool | None = None, tfsf_ztail: bool | None = None):
         """Tiling of the 3-D Yee kernels; None leaves a flag to the engine's
         default (no XCD remap, non-temporal streams on, z-face CPML split
         off).  pml_split moves the z-face CPML corrections into a launch of
         their own, pml_ztail into the end of each Yee tile (FP64/FP32; the
-        results are the same either way; the tail takes precedence); k32
-        tiles k by 32 lanes with two j-rows per wave."""
+        results are the same either way; the tail takes precedence);
+        tfsf_ztail, with pml_ztail, has the tiles apply the TF/SF
+        corrections of the z = const faces too; k32 tiles k by 32 lanes with
+        two j-rows per wave."""
         flags = 0
         if xcd_remap is not None or nontemporal is not None or pml_split is not None or \
-                k32 is not None or pml_streams is not None or pml_ztail is not None:
+                k32 is not None or pml_streams is not None or pml_ztail is not None or \
+                tfsf_ztail is not None:
             flags = _native.TILE_SET
             flags |= _native.TILE_XCD_REMAP if xcd_remap else 0
             flags |= _native.TILE_PML_STREAMS if pml_streams else 0
             flags |= _native.TILE_NT if (nontemporal is None or nontemporal) else 0
             flags |= _native.TILE_PML_SPLIT if (pml_split is None or pml_split) else 0
             flags |= _native.TILE_PML_ZTAIL if pml_ztail else 0
+            flags |= _native.TILE_TFSF_ZTAIL if tfsf_ztail else 0
             flags |= _native.TILE_K32 if k32 else 0
         _native.check_hip(self._lib.xf_set_tiling(self._h, tile_y, chunks_x, flags),
                           "xf_set_tiling")
