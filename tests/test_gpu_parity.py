@@ -155,6 +155,42 @@ def test_pml_streams_bit_identical(gpu_available, tile_y, chunks, split, k32, gr
         assert np.array_equal(g.array(n), c.array(n)), f"{n} differs"
 
 
+@pytest.mark.parametrize("kappa", [False, True])
+def test_cpsi_from_table(gpu_available, kappa):
+    """The CPML correction's c_psi read from the node's coefficient row
+    where the engine found every value equal (the usual air layer: all 24
+    psi arrays of six faces), and from the per-node array on faces whose
+    kappa stretching makes the Yee coefficient differ: bit-identical to the
+    oracle either way, in the default tiling and with the z faces split."""
+    def build():
+        s = xf.Simulation(3e-3, 3e-3, 3e-3, 0.95)
+        s.set_coefficient_layout("dense")
+        s.add_cube("domain", (-0.04, -0.04, -0.05), (0.08, 0.08, 0.1), xf.Material.air())
+        s.add_sphere("s", (0.002, 0.0, 0.004), 0.015, xf.Material.dielectric(3.0, 1.0, 0.02))
+        for f in xf.FACES:
+            s.add_pml(5, f, kappa_max=3.0 if kappa and f in ("XN", "ZP") else 1.0)
+        tau = 20 * 3e-3 / 6e8
+        s.add_tfsf3d((8, 8, 9), 0.5, 0.9, 0.2, xf.Waveform.gaussian(tau, 4.5 * tau))
+        s.add_nf2ff_fd((6, 6, 7), [1.5e9])
+        s.init(80)
+        return s
+
+    for split in (None, True):
+        print(f"case: cpsi kappa={kappa} split={split}", flush=True)
+        g, c = build(), build()
+        e = xf.Engine(g, coeff="table").upload()
+        if split:
+            e.set_tiling(tile_y=1, pml_split=True)
+        desc = e.describe()
+        n = int(re.search(r"cpsi_tab=(\d+)", desc).group(1))
+        assert (n < 24 and n > 0) if kappa else n == 24, desc
+        e.run(80).download()
+        e.close()
+        oracle.run(c.scene_ptr(), 80)
+        for name in _state_names(c):
+            assert np.array_equal(g.array(name), c.array(name)), f"{name} differs (kappa={kappa})"
+
+
 def test_pml_streams_dispersive(gpu_available):
     """The two-stream E pass with ADE cells (their Yee update skipped, the
     ADE kernel after the join)."""

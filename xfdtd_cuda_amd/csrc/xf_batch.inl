@@ -400,6 +400,10 @@ int Batch::configure() {
     default: st = configure_t<__hip_bfloat16>(); break;
   }
   if (st != XF_OK) return st;
+  // grids whose every c_psi value the tiles read from the coefficient table
+  int cpsi_tab = 0;
+  for (size_t g = 0; g < inst.size(); ++g)
+    cpsi_tab += eng(static_cast<int>(g)).cpsi_matched() == 4 * static_cast<int>(eng(static_cast<int>(g)).pml.size()) ? 1 : 0;
   desc = "grids=" + std::to_string(inst.size()) + " cells=" + std::to_string(cells) +
          " tiles=" + std::to_string(total_tiles) + " blocks=" + std::to_string(grid_blocks) +
          " resident=" + std::to_string(resident) + " tile_y=" + std::to_string(tune.tile_y) +
@@ -408,7 +412,7 @@ int Batch::configure() {
          " pml_ztail=" + std::to_string(tune.pml_split == 2 ? 1 : 0) +
          " k32=" + std::to_string(tune.k32) + " xcd=" + std::to_string(tune.xcd_remap) +
          " tfsf=" + std::to_string(n_tf[0]) +
-         " nf2ff=" + std::to_string(n_nf);
+         " nf2ff=" + std::to_string(n_nf) + " cpsi_tab=" + std::to_string(cpsi_tab);
   return XF_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -174,6 +175,9 @@ struct Engine {
   void* htab{};
   unsigned char* eflag{};
   std::vector<double> etab_h, htab_h;  // host copies (double)
+  // per scene CPML face, per psi array (ea, eb, ha, hb): the coefficient of
+  // the node's table row that equals each of its c_psi values, or -1
+  std::vector<std::array<int, 4>> cpsi_row;
   std::vector<unsigned char> eflag_h;
   // dense mode (XF_COEFF_DENSE, or too many rows for a table): the 18
   // per-cell coefficient arrays in the padded layout (E pass 0..8, H pass
@@ -376,6 +380,13 @@ struct Engine {
   // z-face CPML corrections in a launch of their own after each Yee pass
   bool split_z() const;
   bool ztail() const;
+  void match_cpsi(const std::vector<uint32_t>& ide, const std::vector<uint32_t>& idh);
+  int cpsi_matched() const {  // psi arrays whose c_psi the tiles take from the table
+    int m = 0;
+    for (const auto& r : cpsi_row)
+      for (int v : r) m += (v >= 0 && !dense) ? 1 : 0;
+    return m;
+  }
   // the z = const TF/SF corrections applied by the Yee tiles: with the
   // z-face CPML tail (whose instantiation carries the code), one launch per
   // pass, and no ADE (whose kernel rewrites E after the Yee pass)
@@ -1223,7 +1234,67 @@ int Engine::build_ids() {
         for (int64_t k = 0; k < nz; ++k)
           if (disp[(i * ny + j) * nz + k]) ade_cells.push_back((i * (ny + 1) + j) * (nz + 1) + k);
   }
+  match_cpsi(ide, idh);
   return XF_OK;
+}
+
+// The c_psi value of a corrected node is the curl coefficient, along the
+// CPML axis, of the component it corrects, so for the usual scene it equals
+// coefficient a 3 + 1 (psi_a) or b 3 + 2 (psi_b) of the node's own table
+// row.  Where that holds bit for bit at every node the 3-D march corrects
+// (pml_face's ranges), the tiles read it from the table they hold instead of
+// the per-node array; anywhere else (kappa-stretched or graded layers, dense
+// coefficients) the array stays in use.
+void Engine::match_cpsi(const std::vector<uint32_t>& ide, const std::vector<uint32_t>& idh) {
+  cpsi_row.assign(sc->n_pml, {-1, -1, -1, -1});
+  if (dense || dim != 3) return;
+  const int64_t n3[3] = {nx, ny, nz};
+  for (int p = 0; p < sc->n_pml; ++p) {
+    const xf_pml& S = sc->pml[p];
+    if (empty_task(S.task)) continue;
+    const int ax = S.main_axis, a = (ax + 1) % 3, b = (ax + 2) % 3;
+    const xf_array* cp[4] = {&S.c_ea_psi_hb, &S.c_eb_psi_ha, &S.c_ha_psi_eb, &S.c_hb_psi_ea};
+    for (int q = 0; q < 4; ++q) {
+      const bool is_e = q < 2, comp_a = q % 2 == 0;
+      const int n = comp_a ? a * 3 + 1 : b * 3 + 2;
+      const std::vector<uint32_t>& id = is_e ? ide : idh;
+      const std::vector<double>& tab = is_e ? etab_h : htab_h;
+      const xf_array& C = *cp[q];
+      const int64_t cs = task_lo(S.task, ax), ce = task_hi(S.task, ax);
+      const int64_t off = (is_e && cs == 0) ? 1 : 0;
+      const int64_t nstart = is_e ? S.node_e_start : S.node_h_start;
+      // pml_face: a half-open and b closed for E psi_a and H psi_b, the
+      // other way round for E psi_b and H psi_a; nodes up to n on each axis
+      const bool a_open = is_e == comp_a;
+      const int64_t c_lo = std::max<int64_t>(cs + off, 0), c_hi = std::min(ce + off, n3[ax] + 1);
+      const int64_t a_lo = std::max<int64_t>(task_lo(S.task, a), 0);
+      const int64_t a_hi = std::min(task_hi(S.task, a) + (a_open ? 0 : 1), n3[a] + 1);
+      const int64_t b_lo = std::max<int64_t>(task_lo(S.task, b), 0);
+      const int64_t b_hi = std::min(task_hi(S.task, b) + (a_open ? 1 : 0), n3[b] + 1);
+      bool same = C.ndim == 3;
+      for (int64_t c = c_lo; c < c_hi && same; ++c)
+        for (int64_t u = a_lo; u < a_hi && same; ++u)
+          for (int64_t w = b_lo; w < b_hi && same; ++w) {
+            const int64_t layer = c - nstart;
+            int64_t pxyz[3], node[3];
+            pxyz[ax] = layer;
+            pxyz[a] = u;
+            pxyz[b] = w;
+            node[ax] = c;
+            node[a] = u;
+            node[b] = w;
+            if (layer < 0 || pxyz[0] >= C.shape[0] || pxyz[1] >= C.shape[1] || pxyz[2] >= C.shape[2]) {
+              same = false;
+              break;
+            }
+            const double cv = C.data[(pxyz[0] * C.shape[1] + pxyz[1]) * C.shape[2] + pxyz[2]];
+            const bool cell = node[0] < nx && node[1] < ny && node[2] < nz;
+            const uint32_t row = cell ? id[(node[0] * ny + node[1]) * nz + node[2]] : 0;
+            same = bits(cv) == bits(tab[static_cast<size_t>(row) * 9 + n]);
+          }
+      if (same) cpsi_row[p][q] = n;
+    }
+  }
 }
 
 // The 18 per-cell coefficient arrays in the padded layout, in the compute
@@ -1815,7 +1886,7 @@ void Engine::set_tiling(int ty, int chunks_x, int flags) {
                 " xcd=" + std::to_string(xcd) + " nt=" + std::to_string(nt_streams) + " pml_split=" + std::to_string(split_z() ? 1 : 0) + " pml_ztail=" + std::to_string(ztail() ? 1 : 0) +
                 " tfsf_ztail=" + std::to_string(tfsf_ztail() ? 1 : 0) +
                 " klanes=" + std::to_string(klanes) + " pml_streams=" + std::to_string(pml_streams) +
-                " pml_faces=" + std::to_string(pml.size()) +
+                " pml_faces=" + std::to_string(pml.size()) + " cpsi_tab=" + std::to_string(cpsi_matched()) +
                 " ade_cells=" + std::to_string(ade_cells.size()) +
                 (has_ade ? " ade_rows=" + std::to_string(ade_rows) : std::string());
 }
@@ -1974,6 +2045,9 @@ void fill_pml_pass(const Engine& E, PmlPass<C>& P, bool is_e, int faces = 0) {
     F.sa2 = ps[ia]->shape[2];
     F.sb1 = ps[ib]->shape[1];
     F.sb2 = ps[ib]->shape[2];
+    const bool rows = !E.dense && static_cast<size_t>(E.pml_order[q]) < E.cpsi_row.size();
+    F.ta = rows ? E.cpsi_row[E.pml_order[q]][ia] : -1;
+    F.tb = rows ? E.cpsi_row[E.pml_order[q]][ib] : -1;
   }
 }
 

@@ -256,6 +256,8 @@ struct PmlFace {
   C* psi_b;
   const C* cpsi_b;
   int64_t sa1, sa2, sb1, sb2;  // shape[1], shape[2] of the two psi arrays
+  int ta, tb;  // the coefficient (0..8) of a node's table row equal to every
+               // c_psi value of psi_a / psi_b, or -1 (read the c_psi array)
 };
 
 template <typename C>

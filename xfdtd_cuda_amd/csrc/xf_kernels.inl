@@ -8,15 +8,42 @@
 
 namespace xf {
 
+// Where a tile reads its coefficient table: TAB_ANY lets the compiler work
+// it out (the single-grid kernels, whose table pointer it can trace to LDS or
+// to a kernel argument); TAB_LDS and TAB_GLOBAL state it for the batched
+// kernels, whose table is a wave's LDS slice or a global array chosen at run
+// time (a pointer that may be either would be read with flat loads).
+enum { TAB_ANY = 0, TAB_LDS = 1, TAB_GLOBAL = 2 };
+
+template <int TAB, typename C>
+__device__ __forceinline__ C tab_at(const C* tab, int off) {
+  if constexpr (TAB == TAB_LDS)
+    return ((const __attribute__((address_space(3))) C*)tab)[off];
+  else if constexpr (TAB == TAB_GLOBAL)
+    return gp(tab)[off];
+  else
+    return tab[off];
+}
+
+template <int TAB, typename C>
+__device__ __forceinline__ void tab_row(const C* tab, int id, C (&t)[9]) {
+#pragma unroll
+  for (int n = 0; n < 9; ++n) t[n] = tab_at<TAB>(tab, 9 * id + n);
+}
+
 // ------------------------------------------------------------------ PML
 // One face of correctPML (pml_corrector.cu:9-48) applied to the cell
 // (i, j, k) of the current pass; v/have/dirty hold the three components of
 // the pass being written (loaded lazily when not main-updated).
-template <int AX, bool ISE, bool TWO_D, typename S, typename C>
+// USE_TAB: the caller holds the node's coefficient id and table, and a face
+// whose c_psi values all equal a table coefficient (F.ta / F.tb >= 0,
+// checked bit for bit at upload) reads them there instead of from its
+// per-node array.
+template <int AX, bool ISE, bool TWO_D, bool USE_TAB = false, int TAB = TAB_ANY, typename S, typename C>
 __device__ __forceinline__ void pml_face(const PmlFace<C>& F, int i, int j, int k,
                                          int64_t idx, const Fields<S>& f, const Dims& d,
                                          const Scales<C>& sc, C (&v)[3], bool (&have)[3],
-                                         bool (&dirty)[3]) {
+                                         bool (&dirty)[3], const C* tab = nullptr, int id = 0) {
   constexpr int A = (AX + 1) % 3, B = (AX + 2) % 3;
   const int a = AX == 0 ? j : AX == 1 ? k : i;
   const int b = AX == 0 ? k : AX == 1 ? i : j;
@@ -53,7 +80,8 @@ __device__ __forceinline__ void pml_face(const PmlFace<C>& F, int i, int j, int 
       v[A] = Store<S>::ld(self[A], idx, inv_self);
       have[A] = true;
     }
-    v[A] += gp(F.cpsi_a)[p] * psi;
+    const C cp = (USE_TAB && F.ta >= 0) ? tab_at<TAB>(tab, 9 * id + F.ta) : gp(F.cpsi_a)[p];
+    v[A] += cp * psi;
     dirty[A] = true;
   }
   if (in_b && !(TWO_D && (self[B] == nullptr || dual[A] == nullptr))) {
@@ -70,7 +98,8 @@ __device__ __forceinline__ void pml_face(const PmlFace<C>& F, int i, int j, int 
       v[B] = Store<S>::ld(self[B], idx, inv_self);
       have[B] = true;
     }
-    v[B] += gp(F.cpsi_b)[p] * psi;
+    const C cp = (USE_TAB && F.tb >= 0) ? tab_at<TAB>(tab, 9 * id + F.tb) : gp(F.cpsi_b)[p];
+    v[B] += cp * psi;
     dirty[B] = true;
   }
 }
@@ -78,20 +107,20 @@ __device__ __forceinline__ void pml_face(const PmlFace<C>& F, int i, int j, int 
 // Every face of the pass in registration order (X, Y, Z faces); faces whose
 // bit is clear in mask (pml_mask of the cell's column) cannot contain the
 // cell and are skipped.
-template <bool ISE, bool TWO_D, typename S, typename C>
+template <bool ISE, bool TWO_D, bool USE_TAB = false, int TAB = TAB_ANY, typename S, typename C>
 __device__ __forceinline__ void pml_all(const PmlPass<C>& P, int i, int j, int k, int64_t idx,
                                         const Fields<S>& f, const Dims& d, const Scales<C>& sc,
                                         C (&v)[3], bool (&have)[3], bool (&dirty)[3],
-                                        unsigned mask = ~0u) {
+                                        unsigned mask = ~0u, const C* tab = nullptr, int id = 0) {
   for (int q = 0; q < P.n; ++q) {
     if (!((mask >> q) & 1u)) continue;
     const PmlFace<C>& F = P.f[q];
     if (F.axis == 0)
-      pml_face<0, ISE, TWO_D>(F, i, j, k, idx, f, d, sc, v, have, dirty);
+      pml_face<0, ISE, TWO_D, USE_TAB, TAB>(F, i, j, k, idx, f, d, sc, v, have, dirty, tab, id);
     else if (F.axis == 1)
-      pml_face<1, ISE, TWO_D>(F, i, j, k, idx, f, d, sc, v, have, dirty);
+      pml_face<1, ISE, TWO_D, USE_TAB, TAB>(F, i, j, k, idx, f, d, sc, v, have, dirty, tab, id);
     else
-      pml_face<2, ISE, TWO_D>(F, i, j, k, idx, f, d, sc, v, have, dirty);
+      pml_face<2, ISE, TWO_D, USE_TAB, TAB>(F, i, j, k, idx, f, d, sc, v, have, dirty, tab, id);
   }
 }
 

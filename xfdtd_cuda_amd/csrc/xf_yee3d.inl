@@ -18,29 +18,6 @@
 // (two planes of loads in flight per wave).  Arithmetic and its order are
 // the reference's; the file is built without FMA contraction.
 
-// Where a tile reads its coefficient table: TAB_ANY lets the compiler work
-// it out (the single-grid kernels, whose table pointer it can trace to LDS or
-// to a kernel argument); TAB_LDS and TAB_GLOBAL state it for the batched
-// kernels, whose table is a wave's LDS slice or a global array chosen at run
-// time (a pointer that may be either would be read with flat loads).
-enum { TAB_ANY = 0, TAB_LDS = 1, TAB_GLOBAL = 2 };
-
-template <int TAB, typename C>
-__device__ __forceinline__ C tab_at(const C* tab, int off) {
-  if constexpr (TAB == TAB_LDS)
-    return ((const __attribute__((address_space(3))) C*)tab)[off];
-  else if constexpr (TAB == TAB_GLOBAL)
-    return gp(tab)[off];
-  else
-    return tab[off];
-}
-
-template <int TAB, typename C>
-__device__ __forceinline__ void tab_row(const C* tab, int id, C (&t)[9]) {
-#pragma unroll
-  for (int n = 0; n < 9; ++n) t[n] = tab_at<TAB>(tab, 9 * id + n);
-}
-
 template <typename S>
 struct HLoads {
   typename Store<S>::C ex_i, ex_k, ex_j, ey_k, ey_n, ez_j, ez_n, hx, hy, hz;
@@ -271,7 +248,11 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
       ey_i = cur.ey_n;
       ez_i = cur.ez_n;
     }
-    if (PML && any_pml) pml_all<false, false>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask);
+    // the node's row where it is a cell (id 0 elsewhere, as in the padded
+    // id array): the row the c_psi match was made against
+    if (PML && any_pml)
+      pml_all<false, false, !DN, TAB>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask, tab,
+                                   (jk_main && i < d.nx) ? cur.id : 0);
     st3<S, NT>(f.h, idx, v, dirty, sc.s_h);
     cur = nxt;
   }
@@ -345,7 +326,11 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
       hy_m = cur.hy;
       hz_m = cur.hz;
     }
-    if (PML && any_pml) pml_all<true, false>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask);
+    // the node's row where it is a cell (id 0 elsewhere, as in the padded
+    // id array): the row the c_psi match was made against
+    if (PML && any_pml)
+      pml_all<true, false, !DN, TAB>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask, tab,
+                                   (jk_main && i < d.nx) ? cur.id : 0);
     st3<S, NT>(f.e, idx, v, dirty, sc.s_e);
     cur = nxt;
   }
