@@ -263,6 +263,7 @@ int Batch::configure_t() {
       it.d = make_dims(E);
       fill_pml_pass<C>(E, it.P, pass == 1, 2);
       it.B = pml_z_box<C>(E, it.P);
+      it.R = row_ref<C>(E, pass == 1);
       if (it.B.start[it.B.n] == 0) continue;
       items.push_back(it);
       st.push_back(st.back() + whole_blocks(it.B.start[it.B.n]));

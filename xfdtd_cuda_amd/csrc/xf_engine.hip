@@ -2252,6 +2252,12 @@ PmlZBox pml_z_box(const Engine& E, const PmlPass<C>& P) {
   return B;
 }
 
+// The table and ids a pass's CPML corrections read c_psi rows through.
+template <typename C>
+RowRef<C> row_ref(const Engine& E, bool is_e) {
+  return RowRef<C>{static_cast<const C*>(is_e ? E.etab : E.htab), is_e ? E.eid : E.hid, E.id_bytes == 2 ? 1 : 0};
+}
+
 template <typename S>
 int Engine::launch_pml_z(bool is_e, hipStream_t st) {
   using C = typename Store<S>::C;
@@ -2263,10 +2269,10 @@ int Engine::launch_pml_z(bool is_e, hipStream_t st) {
   const dim3 grid((unsigned)((total + 255) / 256));
   if (is_e)
     hipLaunchKernelGGL((k_pml_z<S, true>), grid, dim3(256), 0, st, make_fields<S>(*this), P, B,
-                       make_dims(*this), make_scales<C>(*this));
+                       make_dims(*this), make_scales<C>(*this), row_ref<C>(*this, true));
   else
     hipLaunchKernelGGL((k_pml_z<S, false>), grid, dim3(256), 0, st, make_fields<S>(*this), P, B,
-                       make_dims(*this), make_scales<C>(*this));
+                       make_dims(*this), make_scales<C>(*this), row_ref<C>(*this, false));
   XF_HIP(hipGetLastError());
   return XF_OK;
 }

@@ -274,6 +274,16 @@ struct PmlZTail {
   PmlFace<C> f[2];
 };
 
+// A pass's coefficient table and per-node ids (uint8, or uint16 when wide),
+// for the CPML corrections that take c_psi from the node's row
+// (PmlFace::ta / tb).
+template <typename C>
+struct RowRef {
+  const C* tab;
+  const void* ids;
+  int wide;
+};
+
 // The z faces of a pass as node boxes (k fastest) for k_pml_z: face q
 // covers i in [i0, i0 + ni), j in [j0, j0 + nj), k in [k0, k0 + nk), its
 // nodes numbered from start[q]; start[n] is the total.
@@ -515,6 +525,7 @@ struct PmlZItem {
   Dims d;
   PmlPass<C> P;  // the z faces
   PmlZBox B;
+  RowRef<C> R;
 };
 
 template <typename S>

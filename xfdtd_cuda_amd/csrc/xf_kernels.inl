@@ -132,7 +132,8 @@ __device__ __forceinline__ void pml_all(const PmlPass<C>& P, int i, int j, int k
 template <typename S, bool ISE>
 __device__ __forceinline__ void pml_z_node(const Fields<S>& f, const PmlPass<typename Store<S>::C>& P,
                                            const PmlZBox& B, const Dims& d,
-                                           const Scales<typename Store<S>::C>& sc, int64_t t) {
+                                           const Scales<typename Store<S>::C>& sc,
+                                           const RowRef<typename Store<S>::C>& R, int64_t t) {
   using C = typename Store<S>::C;
   if (t >= B.start[B.n]) return;
   const int q = (B.n > 1 && t >= B.start[1]) ? 1 : 0;
@@ -146,11 +147,17 @@ __device__ __forceinline__ void pml_z_node(const Fields<S>& f, const PmlPass<typ
   C v[3] = {0, 0, 0};
   bool have[3] = {false, false, false};
   bool dirty[3] = {false, false, false};
+  // the node's row id where a face takes c_psi from the table (0 where the
+  // node is not a cell, as stored)
+  const bool rows = q == 0 ? (P.f[0].ta >= 0 || P.f[0].tb >= 0) : (P.f[1].ta >= 0 || P.f[1].tb >= 0);
+  int id = 0;
+  if (rows)
+    id = R.wide ? (int)gp(static_cast<const uint16_t*>(R.ids))[idx] : (int)gp(static_cast<const uint8_t*>(R.ids))[idx];
   // static face indices keep a by-value pass out of scratch
   if (q == 0)
-    pml_face<2, ISE, false>(P.f[0], i, j, k, idx, f, d, sc, v, have, dirty);
+    pml_face<2, ISE, false, true, TAB_GLOBAL>(P.f[0], i, j, k, idx, f, d, sc, v, have, dirty, R.tab, id);
   else
-    pml_face<2, ISE, false>(P.f[1], i, j, k, idx, f, d, sc, v, have, dirty);
+    pml_face<2, ISE, false, true, TAB_GLOBAL>(P.f[1], i, j, k, idx, f, d, sc, v, have, dirty, R.tab, id);
   S* const* self = ISE ? f.e : f.h;
   const C s = ISE ? sc.s_e : sc.s_h;
 #pragma unroll
@@ -161,8 +168,8 @@ __device__ __forceinline__ void pml_z_node(const Fields<S>& f, const PmlPass<typ
 template <typename S, bool ISE>
 __global__ void __launch_bounds__(256)
     k_pml_z(Fields<S> f, PmlPass<typename Store<S>::C> P, PmlZBox B, Dims d,
-            Scales<typename Store<S>::C> sc) {
-  pml_z_node<S, ISE>(f, P, B, d, sc, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+            Scales<typename Store<S>::C> sc, RowRef<typename Store<S>::C> R) {
+  pml_z_node<S, ISE>(f, P, B, d, sc, R, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Loads the pass's coefficient table into LDS (or points at global memory).
@@ -1155,7 +1162,7 @@ __global__ void __launch_bounds__(256)
   const PmlZItem<S>& I = items[g];
   const int64_t local = b0 - start[g] + threadIdx.x;
   if (local >= I.B.start[I.B.n]) return;
-  pml_z_node<S, ISE>(I.f, I.P, I.B, I.d, I.sc, local);
+  pml_z_node<S, ISE>(I.f, I.P, I.B, I.d, I.sc, I.R, local);
 }
 
 // The point sources of every grid of a batch in one launch: thread g adds

@@ -106,10 +106,14 @@ __device__ __forceinline__ unsigned pml_mask(const PmlPass<C>& P, int i0, int i1
 // again.  Per node the work and its order are k_pml_z's (pml_z_node): the
 // x and y corrections came in the march, the z faces follow in
 // registration order; the dual field is not written in this pass.
-template <bool ISE, typename S>
+// USE_TAB: c_psi from the node's table row where the face allows it (the
+// node's id read again here; 0 where the node is not a cell, as stored).
+template <bool ISE, typename S, bool USE_TAB = false, int TAB = TAB_ANY, typename I = uint8_t>
 __device__ __forceinline__ void pml_z_tail(const PmlZTail<typename Store<S>::C>& Z, int i0, int i1, int j,
                                            int k, int64_t idx0, const Fields<S>& f, const Dims& d,
-                                           const Scales<typename Store<S>::C>& sc) {
+                                           const Scales<typename Store<S>::C>& sc,
+                                           const typename Store<S>::C* tab = nullptr,
+                                           const I* __restrict__ ids = nullptr) {
   using C = typename Store<S>::C;
   auto meets = [&](const PmlFace<C>& F) {
     return k >= F.c0 && k < F.c1 && F.a0 < i1 && F.a1 >= i0 && j >= F.b0 && j <= F.b1;
@@ -124,9 +128,10 @@ __device__ __forceinline__ void pml_z_tail(const PmlZTail<typename Store<S>::C>&
     C v[3] = {0, 0, 0};
     bool have[3] = {false, false, false};
     bool dirty[3] = {false, false, false};
+    const int id = USE_TAB ? (int)gp(ids)[idx] : 0;
     // static face indices keep the descriptor out of scratch
-    if (m0) pml_face<2, ISE, false>(Z.f[0], i, j, k, idx, f, d, sc, v, have, dirty);
-    if (m1) pml_face<2, ISE, false>(Z.f[1], i, j, k, idx, f, d, sc, v, have, dirty);
+    if (m0) pml_face<2, ISE, false, USE_TAB, TAB>(Z.f[0], i, j, k, idx, f, d, sc, v, have, dirty, tab, id);
+    if (m1) pml_face<2, ISE, false, USE_TAB, TAB>(Z.f[1], i, j, k, idx, f, d, sc, v, have, dirty, tab, id);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
       if (dirty[c]) Store<S>::st(self[c], idx, v[c], s);
@@ -257,7 +262,7 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
     cur = nxt;
   }
   if constexpr (PML == 2) {
-    if (D.zt.n > 0) pml_z_tail<false, S>(D.zt, i0, i1, j, k, idx0, f, d, sc);
+    if (D.zt.n > 0) pml_z_tail<false, S, !DN, TAB>(D.zt, i0, i1, j, k, idx0, f, d, sc, tab, hid);
     if (D.tz.n > 0) tfsf_z_tail<false, S>(D.tz, i0, i1, j, k, idx0, f, d, sc);
   }
 }
@@ -335,7 +340,7 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
     cur = nxt;
   }
   if constexpr (PML == 2) {
-    if (D.zt.n > 0) pml_z_tail<true, S>(D.zt, i0, i1, j, k, idx0, f, d, sc);
+    if (D.zt.n > 0) pml_z_tail<true, S, !DN, TAB>(D.zt, i0, i1, j, k, idx0, f, d, sc, tab, eid);
     if (D.tz.n > 0) tfsf_z_tail<true, S>(D.tz, i0, i1, j, k, idx0, f, d, sc);
   }
 }
