@@ -18,6 +18,13 @@
 // (two planes of loads in flight per wave).  Arithmetic and its order are
 // the reference's; the file is built without FMA contraction.
 
+// Whether a tile's in-march CPML corrections take c_psi from the table row
+// (PmlFace::ta / tb): not with dense coefficients (no table), nor with
+// compensated bf16 storage, whose H pass it slowed from 1.70 to 2.21 ms at
+// 536^3 (the step from 37.1 to 33.4 Gcells/s).
+template <typename S, bool DN>
+constexpr bool kCpsiRows = !DN && !__is_same(S, Bf16c);
+
 template <typename S>
 struct HLoads {
   typename Store<S>::C ex_i, ex_k, ex_j, ey_k, ey_n, ez_j, ez_n, hx, hy, hz;
@@ -256,7 +263,7 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
     // the node's row where it is a cell (id 0 elsewhere, as in the padded
     // id array): the row the c_psi match was made against
     if (PML && any_pml)
-      pml_all<false, false, !DN, TAB>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask, tab,
+      pml_all<false, false, kCpsiRows<S, DN>, TAB>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask, tab,
                                    (jk_main && i < d.nx) ? cur.id : 0);
     st3<S, NT>(f.h, idx, v, dirty, sc.s_h);
     cur = nxt;
@@ -334,7 +341,7 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
     // the node's row where it is a cell (id 0 elsewhere, as in the padded
     // id array): the row the c_psi match was made against
     if (PML && any_pml)
-      pml_all<true, false, !DN, TAB>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask, tab,
+      pml_all<true, false, kCpsiRows<S, DN>, TAB>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask, tab,
                                    (jk_main && i < d.nx) ? cur.id : 0);
     st3<S, NT>(f.e, idx, v, dirty, sc.s_e);
     cur = nxt;
