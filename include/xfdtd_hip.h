@@ -406,7 +406,10 @@ int xf_set_timing(xf_handle* h, int32_t on);
  * xf_plan.tiling_flags. */
 int xf_set_tiling(xf_handle* h, int32_t tile_y, int32_t chunks_x, int32_t flags);
 
-/* Description of the chosen layout (coefficient compression, tile). */
+/* Description of the chosen layout (coefficient compression, tile), as
+ * key=value words; among them cpsi_tab, the number of CPML psi arrays (four
+ * per face) whose c_psi values all equal a coefficient of the node's table
+ * row, which the kernels then read there instead of the c_psi array. */
 int xf_describe(const xf_handle* h, char* buf, size_t len);
 
 /* Last error message of the calling thread ("" if none). */
