@@ -19,11 +19,11 @@
 // the reference's; the file is built without FMA contraction.
 
 // Whether a tile's in-march CPML corrections take c_psi from the table row
-// (PmlFace::ta / tb): not with dense coefficients (no table), nor with
-// compensated bf16 storage, whose H pass it slowed from 1.70 to 2.21 ms at
-// 536^3 (the step from 37.1 to 33.4 Gcells/s).
-template <typename S, bool DN>
-constexpr bool kCpsiRows = !DN && !__is_same(S, Bf16c);
+// (PmlFace::ta / tb): not with dense coefficients (no table), nor in the H
+// pass of compensated bf16 storage, which it slowed from 1.70 to 2.21 ms at
+// 536^3 (its E pass gained: 2.05 against 1.99 ms).
+template <typename S, bool DN, bool ISE>
+constexpr bool kCpsiRows = !DN && (ISE || !__is_same(S, Bf16c));
 
 template <typename S>
 struct HLoads {
@@ -263,7 +263,7 @@ __device__ __forceinline__ void yee_h3_tile(const YeePass<S>& D, const typename 
     // the node's row where it is a cell (id 0 elsewhere, as in the padded
     // id array): the row the c_psi match was made against
     if (PML && any_pml)
-      pml_all<false, false, kCpsiRows<S, DN>, TAB>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask, tab,
+      pml_all<false, false, kCpsiRows<S, DN, false>, TAB>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask, tab,
                                    (jk_main && i < d.nx) ? cur.id : 0);
     st3<S, NT>(f.h, idx, v, dirty, sc.s_h);
     cur = nxt;
@@ -341,7 +341,7 @@ __device__ __forceinline__ void yee_e3_tile(const YeePass<S>& D, const typename 
     // the node's row where it is a cell (id 0 elsewhere, as in the padded
     // id array): the row the c_psi match was made against
     if (PML && any_pml)
-      pml_all<true, false, kCpsiRows<S, DN>, TAB>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask, tab,
+      pml_all<true, false, kCpsiRows<S, DN, true>, TAB>(D.pml, i, j, k, idx, f, d, sc, v, have, dirty, pmask, tab,
                                    (jk_main && i < d.nx) ? cur.id : 0);
     st3<S, NT>(f.e, idx, v, dirty, sc.s_e);
     cur = nxt;
