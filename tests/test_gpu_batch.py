@@ -116,6 +116,43 @@ def test_batch_tunings_bit_identical(gpu_available, tile_y, planes, blocks, nt, 
     _assert_identical(sims, ref, f"tuning ty={tile_y} planes={planes} blocks={blocks} split={split}")
 
 
+@pytest.mark.parametrize("split", [1, 2])
+def test_batch_cpsi_rows_mixed(gpu_available, split):
+    """Grids whose CPML c_psi values all come from the coefficient table and
+    grids with kappa-stretched faces that keep the c_psi arrays, in one
+    batch, with the z faces split off or at the tile end: each grid
+    bit-identical to the oracle."""
+    steps = 40
+
+    def build():
+        sims = []
+        for g in range(3):
+            s = xf.Simulation(3e-3, 3e-3, 3e-3, 0.95)
+            s.set_coefficient_layout("dense")
+            e = 0.03 + 0.006 * g
+            s.add_cube("domain", (-e, -e, -e - 0.003 * g), (2 * e, 2 * e, 2 * e + 0.006 * g),
+                       xf.Material.air())
+            s.add_sphere("s", (0.0, 0.001, 0.0), 0.6 * e, xf.Material.dielectric(2.0 + g, 1.0, 0.01))
+            for f in xf.FACES:
+                s.add_pml(4, f, kappa_max=2.5 if (g == 1 and f in ("YP", "ZN")) else 1.0)
+            tau = 20 * 3e-3 / 6e8
+            s.add_tfsf3d((6, 6, 6), 0.4, 0.7, 0.1, xf.Waveform.gaussian(tau, 4.5 * tau))
+            s.add_nf2ff_fd((5, 5, 5), [1.5e9])
+            s.init(steps)
+            sims.append(s)
+        return sims
+
+    ref = _oracle_copies(build, steps)
+    sims = build()
+    b = xf.Batch(sims).upload()
+    b.set_tuning(tile_y=1, planes=3, pml_split=True, k32=True, pml_ztail=split == 2)
+    desc = b.describe()
+    assert "cpsi_tab=2" in desc, desc  # the kappa grid keeps its arrays
+    b.run(15).run(steps - 15).download()
+    b.close()
+    _assert_identical(sims, ref, f"c_psi rows split={split}")
+
+
 def test_batch_graph_replay_matches_eager(gpu_available):
     steps = 40
     build = lambda: _uneven(steps, count=4, seed=5)
